@@ -1,0 +1,146 @@
+// Batch ray queries on a triangle mesh: the C-ABI owe_hip_mesh_* and owe_hip_trace_batch.
+// Stands in for MeshShape::intersect over many rays (src/owe/scene/mesh.cpp:73-97) and the float
+// kernels' any-hit occludedMesh (src/owe/backends/gpu/shaders/geometry.slang:577-615).
+//
+// One lane per ray, 128 lanes per block; the traversal stack lives in LDS with the capacity the
+// build proved sufficient (bvh4.hpp), chosen from a few compiled sizes.
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "bvh4.hpp"
+#include "common.hpp"
+#include "traverse.hpp"
+
+using namespace owehip;
+
+struct owe_hip_mesh {
+    int device = 0;
+    WideNode* nodes = nullptr;
+    TriRecord* tris = nullptr;
+    owe_hip_mesh_stats stats{};
+};
+
+namespace owehip {
+namespace {
+
+constexpr int kQueryBlock = 128;
+
+template <int kCap, bool kAny>
+__global__ __launch_bounds__(kQueryBlock) void traceBatchKernel(const WideNode* __restrict__ nodes,
+                                                                const TriRecord* __restrict__ tris,
+                                                                const float4* __restrict__ rays, uint64_t n,
+                                                                uint32_t* __restrict__ primOut,
+                                                                float* __restrict__ tOut) {
+    __shared__ uint32_t stack[kCap * kQueryBlock];
+    const uint64_t i = uint64_t(blockIdx.x) * kQueryBlock + threadIdx.x;
+    if (i >= n) return;  // no barrier follows: lanes past the end simply leave
+    const float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
+    const RayHit h = traverseBvh4<kAny>(nodes, tris, mk3(r0.x, r0.y, r0.z), mk3(r1.x, r1.y, r1.z), r0.w, r1.w, ~0u,
+                                        stack + threadIdx.x, kQueryBlock);
+    if (h.slot == kEmptyChild) {
+        primOut[i] = 0xFFFFFFFFu;
+        tOut[i] = __builtin_inff();
+    } else {
+        primOut[i] = tris[h.slot].prim;
+        tOut[i] = h.t;
+    }
+}
+
+template <int kCap>
+void launchTrace(int mode, const owe_hip_mesh* m, const float* rays, uint64_t n, uint32_t* prim, float* t,
+                 hipStream_t s) {
+    const uint64_t blocks = (n + kQueryBlock - 1) / kQueryBlock;
+    if (blocks > 0x7FFFFFFFull) fail(OWE_HIP_ERR_INVALID, "batch too large for one launch");
+    const dim3 grid{unsigned(blocks)}, block{unsigned(kQueryBlock)};
+    if (mode == OWE_HIP_QUERY_ANY)
+        hipLaunchKernelGGL((traceBatchKernel<kCap, true>), grid, block, 0, s, m->nodes, m->tris,
+                           reinterpret_cast<const float4*>(rays), n, prim, t);
+    else
+        hipLaunchKernelGGL((traceBatchKernel<kCap, false>), grid, block, 0, s, m->nodes, m->tris,
+                           reinterpret_cast<const float4*>(rays), n, prim, t);
+    OWE_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+}  // namespace owehip
+
+extern "C" {
+
+int owe_hip_mesh_create(int device, const float* positions, uint32_t nVertices, const uint32_t* triangles,
+                        uint32_t nTriangles, owe_hip_mesh** out) {
+    return guard([&] {
+        require(out != nullptr, "out is null");
+        require(positions != nullptr || nVertices == 0, "positions is null");
+        require(triangles != nullptr || nTriangles == 0, "triangles is null");
+        *out = nullptr;
+        selectDevice(device);
+        Bvh4 b = buildBvh4(positions, nVertices, triangles, nTriangles);
+        if (b.stackBound > 64)
+            fail(OWE_HIP_ERR_UNSUPPORTED, "mesh hierarchy needs a traversal stack of " + std::to_string(b.stackBound) +
+                                              " entries; the kernels hold at most 64");
+        auto* m = new owe_hip_mesh;
+        m->device = device;
+        const size_t nb = b.nodes.size() * sizeof(WideNode), tb = std::max<size_t>(1, b.tris.size()) * sizeof(TriRecord);
+        hipError_t e = hipMalloc(&m->nodes, nb);
+        if (e == hipSuccess) e = hipMalloc(&m->tris, tb);
+        if (e == hipSuccess) e = hipMemcpy(m->nodes, b.nodes.data(), nb, hipMemcpyHostToDevice);
+        if (e == hipSuccess && !b.tris.empty())
+            e = hipMemcpy(m->tris, b.tris.data(), b.tris.size() * sizeof(TriRecord), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            if (m->nodes) (void)hipFree(m->nodes);
+            if (m->tris) (void)hipFree(m->tris);
+            delete m;
+            hipCheck(e, "mesh upload");
+        }
+        m->stats.triangles = nTriangles;
+        m->stats.nodes = b.nodes.size();
+        m->stats.depth = b.depth;
+        m->stats.stack_bound = b.stackBound;
+        m->stats.build_seconds = b.seconds;
+        m->stats.device_bytes = nb + tb;
+        *out = m;
+    });
+}
+
+int owe_hip_mesh_destroy(owe_hip_mesh* m) {
+    return guard([&] {
+        if (!m) return;
+        (void)hipSetDevice(m->device);
+        if (m->nodes) OWE_HIP_CHECK(hipFree(m->nodes));
+        if (m->tris) OWE_HIP_CHECK(hipFree(m->tris));
+        delete m;
+    });
+}
+
+int owe_hip_mesh_stats_get(const owe_hip_mesh* m, owe_hip_mesh_stats* out) {
+    return guard([&] {
+        require(m != nullptr && out != nullptr, "null argument");
+        *out = m->stats;
+    });
+}
+
+int owe_hip_trace_batch(owe_hip_mesh* m, int mode, const float* rays, uint64_t n, uint32_t* prim, float* t,
+                        void* stream) {
+    return guard([&] {
+        require(m != nullptr, "mesh is null");
+        require(mode == OWE_HIP_QUERY_CLOSEST || mode == OWE_HIP_QUERY_ANY, "mode must be closest (0) or any (1)");
+        if (n == 0) return;
+        require(rays != nullptr && prim != nullptr && t != nullptr, "null buffer");
+        require((reinterpret_cast<uintptr_t>(rays) & 15) == 0, "rays must be 16-byte aligned");
+        for (const void* p : {static_cast<const void*>(rays), static_cast<const void*>(prim), static_cast<const void*>(t)}) {
+            hipPointerAttribute_t a;
+            if (hipPointerGetAttributes(&a, p) != hipSuccess || a.type != hipMemoryTypeDevice || a.device != m->device)
+                fail(OWE_HIP_ERR_INVALID, "rays, prim and t must be device memory on the mesh's device");
+        }
+        OWE_HIP_CHECK(hipSetDevice(m->device));
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        const uint32_t cap = m->stats.stack_bound;
+        if (cap <= 16) launchTrace<16>(mode, m, rays, n, prim, t, s);
+        else if (cap <= 32) launchTrace<32>(mode, m, rays, n, prim, t, s);
+        else if (cap <= 48) launchTrace<48>(mode, m, rays, n, prim, t, s);
+        else launchTrace<64>(mode, m, rays, n, prim, t, s);
+    });
+}
+
+}  // extern "C"

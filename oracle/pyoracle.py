@@ -1,0 +1,185 @@
+"""TEST INFRASTRUCTURE ONLY — ctypes loaders of the double-precision CPU oracle.
+
+``Oracle`` wraps build/liboracle.so (the repository's restatement of the reference transport, see
+oracle_core.hpp for the citations); ``Reference`` wraps _ref/libowe_ref.so, the harness over the
+reference engine's own sources, which exists only where the reference was mounted at build time.
+Only tests, __graft_entry__.smoke() and bench.py's CPU-baseline leg import this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ORACLE_LIB = HERE / "build" / "liboracle.so"
+REF_LIB = HERE / "_ref" / "libowe_ref.so"
+
+_d, _i, _u32, _u64, _vp = ctypes.c_double, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_void_p
+_pd = ctypes.POINTER(ctypes.c_double)
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class _Lib:
+    prefix = ""
+
+    def __init__(self, path: Path):
+        if not path.exists():
+            raise FileNotFoundError(f"{path} is not built (make -C oracle{' ref' if 'ref' in path.name else ''})")
+        self.lib = ctypes.CDLL(os.fspath(path))
+        p = self.prefix
+        sigs = {
+            "pcg_u32": (None, [_u64, _u64, _i, _vp]),
+            "pcg_uniform": (None, [_u64, _u64, _i, _vp]),
+            "mix64": (_u64, [_u64]),
+            "hash_pair": (_u64, [_u64, _u64]),
+            "fresnel_dielectric": (_d, [_d, _d]),
+            "fresnel_conductor": (_d, [_d, _d, _d]),
+            "sample_visible": (_d, [_d]),
+            "visible_pdf": (_d, [_d]),
+            "cie": (None, [_d, _vp]),
+            "cie_y_integral": (_d, []),
+            "planck": (_d, [_d, _d]),
+            "hg_phase": (_d, [_d, _d]),
+            "sample_hg": (_d, [_vp, _d, _d, _d, _vp]),
+            "ggx_d": (_d, [_d, _vp]),
+            "ggx_pdf": (_d, [_d, _vp, _vp]),
+            "ggx_sample_wm": (None, [_d, _vp, _d, _d, _vp]),
+            "sample_rough_dielectric": (_i, [_d, _vp, _d, _d, _d, _d, _i, _vp]),
+            "eval_rough_dielectric": (_d, [_d, _vp, _vp, _d, _i, _vp]),
+            "sample_rough_conductor": (_i, [_d, _vp, _d, _d, _d, _d, _d, _vp]),
+            "eval_rough_conductor": (_d, [_d, _vp, _vp, _d, _d, _d, _vp]),
+            "line_crosses_triangle": (_i, [_vp, _vp, _vp]),
+            "mesh_create": (_vp, [_vp, _u32, _vp, _u32]),
+            "mesh_destroy": (None, [_vp]),
+            "mesh_node_count": (_u64, [_vp]),
+            "mesh_order": (None, [_vp, _vp]),
+            "mesh_closest": (None, [_vp, _vp, _u64, _vp, _vp, _i]),
+        }
+        sigs.update(self.extra_sigs())
+        for name, (res, args) in sigs.items():
+            fn = getattr(self.lib, p + name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+            setattr(self, name, fn)
+
+    def extra_sigs(self):
+        return {}
+
+    # ---- convenience wrappers shared by the oracle and the reference harness
+
+    def pcg(self, seed: int, stream: int, n: int) -> np.ndarray:
+        out = np.empty(n, dtype=np.uint32)
+        self.pcg_u32(seed, stream, n, _ptr(out))
+        return out
+
+    def uniforms(self, seed: int, stream: int, n: int) -> np.ndarray:
+        out = np.empty(n, dtype=np.float64)
+        self.pcg_uniform(seed, stream, n, _ptr(out))
+        return out
+
+    def cie_xyz(self, lam: float) -> np.ndarray:
+        out = np.empty(3)
+        self.cie(lam, _ptr(out))
+        return out
+
+    def mesh(self, positions: np.ndarray, triangles: np.ndarray):
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+        tri = np.ascontiguousarray(triangles, dtype=np.uint32).reshape(-1, 3)
+        h = self.mesh_create(_ptr(pos), len(pos), _ptr(tri), len(tri))
+        return _Mesh(self, h, len(tri))
+
+
+class _Mesh:
+    def __init__(self, owner: _Lib, handle, n_tris: int):
+        self.o, self.h, self.n_tris = owner, handle, n_tris
+
+    def __del__(self):
+        try:
+            self.o.mesh_destroy(self.h)
+        except Exception:
+            pass
+
+    def _rays(self, rays):
+        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        return r, len(r)
+
+    def closest(self, rays, threads: int = 0):
+        r, n = self._rays(rays)
+        prim = np.empty(n, dtype=np.uint32)
+        t = np.empty(n, dtype=np.float64)
+        self.o.mesh_closest(self.h, _ptr(r), n, _ptr(prim), _ptr(t), threads)
+        return prim, t
+
+    def any(self, rays, threads: int = 0):
+        r, n = self._rays(rays)
+        hit = np.empty(n, dtype=np.uint8)
+        self.o.mesh_any(self.h, _ptr(r), n, _ptr(hit), threads)
+        return hit.astype(bool)
+
+    def brute(self, rays, threads: int = 0):
+        r, n = self._rays(rays)
+        prim = np.empty(n, dtype=np.uint32)
+        t = np.empty(n, dtype=np.float64)
+        self.o.mesh_brute(self.h, _ptr(r), n, _ptr(prim), _ptr(t), threads)
+        return prim, t
+
+    def traversal_stats(self, rays, threads: int = 0) -> dict:
+        r, n = self._rays(rays)
+        out = np.zeros(4)
+        self.o.mesh_traversal_stats(self.h, _ptr(r), n, _ptr(out), threads)
+        return {"slab_tests": out[0], "triangle_tests": out[1], "inner_nodes": out[3]}
+
+    def order(self) -> np.ndarray:
+        out = np.empty(self.n_tris, dtype=np.uint32)
+        self.o.mesh_order(self.h, _ptr(out))
+        return out
+
+    def node_count(self) -> int:
+        return int(self.o.mesh_node_count(self.h))
+
+
+class Oracle(_Lib):
+    """The repository's fp64 restatement (liboracle.so)."""
+
+    prefix = "oracle_"
+
+    def __init__(self, path: Path = ORACLE_LIB):
+        super().__init__(path)
+
+    def extra_sigs(self):
+        return {
+            "ciddor_air": (_d, [_d]),
+            "mesh_any": (None, [_vp, _vp, _u64, _vp, _i]),
+            "mesh_brute": (None, [_vp, _vp, _u64, _vp, _vp, _i]),
+            "mesh_traversal_stats": (None, [_vp, _vp, _u64, _vp, _i]),
+        }
+
+
+class Reference(_Lib):
+    """The reference engine's own code (libowe_ref.so), built from /root/reference in this container."""
+
+    prefix = "ref_"
+
+    def __init__(self, path: Path = REF_LIB):
+        super().__init__(path)
+
+    def extra_sigs(self):
+        return {
+            "value_noise3": (_d, [_vp]),
+            "catalog_index": (_d, [ctypes.c_char_p, _d]),
+            "sellmeier_index": (_d, [_vp, _vp, _i, _i, _d]),
+            "blackbody": (_d, [_d, _d, _d]),
+            "wavelengths": (None, [_d, _vp]),
+        }
+
+
+def reference_available() -> bool:
+    return REF_LIB.exists()
