@@ -10,12 +10,15 @@
  *   owe::Renderer           src/owe/render/renderer.hpp:19-79
  *   owe::MeshShape::intersect, Bvh::traverse   src/owe/scene/mesh.cpp:73-97, src/owe/scene/bvh.hpp:31-78
  *   occludedMesh (any hit)  src/owe/backends/gpu/shaders/geometry.slang:577-615
+ *   Tracer::radiance        src/owe/transport/transport.cpp:473-529 (camera paths, per pass)
  */
 #ifndef OWE_HIP_H
 #define OWE_HIP_H
 
 #include <stddef.h>
 #include <stdint.h>
+
+#include "owe_hip_scene.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -87,6 +90,41 @@ int owe_hip_mesh_stats_get(const owe_hip_mesh* mesh, owe_hip_mesh_stats* out);
  * (NULL: the default stream). The call is asynchronous on that stream. */
 int owe_hip_trace_batch(owe_hip_mesh* mesh, int mode, const float* rays, uint64_t n, uint32_t* prim, float* t,
                         void* stream);
+
+/* ------------------------------------------------------------------ camera-path renderer
+ * owe::Backend::createRenderer (backend.hpp:45-46) and owe::Renderer (renderer.hpp:19-79) over the
+ * flattened scene of owe_hip_scene.h. Each pass adds spp samples per pixel of the scene's observer
+ * (Renderer::runPass → pass, renderer.hpp:40-46); resolve divides the film by the samples so far.
+ * Streams are one PCG32 per (seed, pixel, sample, pass); the pixel's own stream (seed, pixel, pass)
+ * gives its hero-wavelength rotation, as CpuRenderer::pass (cpu_backend.cpp:52-94) does per pixel.
+ * The scene is copied at creation: the caller's arrays need not outlive the call. */
+typedef struct owe_hip_renderer owe_hip_renderer;
+
+/* Fails with OWE_HIP_ERR_UNSUPPORTED for an integrator other than path (renderer.hpp:22-36). */
+int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render_settings* settings,
+                            owe_hip_renderer** out);
+int owe_hip_renderer_destroy(owe_hip_renderer* renderer);
+/* Renderer::runPass(spp): synchronous; spp ≤ 0 does nothing. */
+int owe_hip_renderer_pass(owe_hip_renderer* renderer, int spp);
+/* Renderer::resolve: width × height × 3 doubles of XYZ (row-major), film ÷ samples per pixel. */
+int owe_hip_renderer_resolve(owe_hip_renderer* renderer, double* xyz);
+/* Renderer::stats: paths, segments, inconsistencies, leaks (transport_stats.hpp:12-31). */
+int owe_hip_renderer_stats(const owe_hip_renderer* renderer, owe_transport_stats* out);
+/* Renderer::passes() and samplesPerPixel() (renderer.hpp:48-49). */
+int owe_hip_renderer_progress(const owe_hip_renderer* renderer, int* passes, long long* samples_per_pixel);
+/* Renderer::resetProgress (renderer.hpp:63-69): clears the film, the pass count and the statistics. */
+int owe_hip_renderer_reset(owe_hip_renderer* renderer);
+/* The XYZ estimate of single camera paths (pixel[i], sample[i]) of a pass of spp samples, without
+ * touching the film: the same kernels, for per-path comparison with the oracle. Host arrays; xyz
+ * receives n × 3 doubles. */
+int owe_hip_renderer_camera_paths(owe_hip_renderer* renderer, int spp, int pass, const uint32_t* pixel,
+                                  const uint32_t* sample, uint64_t n, double* xyz);
+/* Per-kernel timing with HIP events on the renderer's stream (the reference's OWE_GPU_PROFILE,
+ * src/owe/backends/gpu/vulkan_context.cpp:350-422). Enabling clears the totals. */
+int owe_hip_renderer_set_timing(owe_hip_renderer* renderer, int enable);
+/* Milliseconds and launches per kernel since timing was enabled, in the order generate, intersect,
+ * shade, shadow, splat (5 entries each). */
+int owe_hip_renderer_kernel_times(const owe_hip_renderer* renderer, double* ms, uint64_t* launches);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,178 @@
+/* owe_hip_scene.h — the flattened scene the hip backend renders: plain structs, plain pointers.
+ *
+ * It carries what the reference's World, Scene and RenderSettings hold for transport
+ * (src/owe/scene/world.hpp:23-344, src/owe/scene/scene.hpp:15-49, src/owe/scene/detector.hpp:44-116):
+ * media with n(λ), absorption, scattering and HG g; regions; surface optics; emission; boundaries
+ * (a shape in its own frame with a rigid frame→world transform and the regions on its two sides);
+ * the environment (sun, uniform sky); one ideal observer. A reference-side adapter fills it from a
+ * built owe::Scene; the Python tooling fills it directly. All lengths in metres,
+ * wavelengths in nanometres, all values IEEE-754 double (the backend rebases and rounds to float).
+ */
+#ifndef OWE_HIP_SCENE_H
+#define OWE_HIP_SCENE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Spectrum kinds (owe::Spectrum::Kind, core/spectrum.hpp:28). Tabulated: `count` (λ, value) pairs
+ * interleaved at table[offset ..], clamped outside, linear inside (spectrum.cpp:135-143). */
+#define OWE_SPEC_CONSTANT 0
+#define OWE_SPEC_BLACKBODY 1
+#define OWE_SPEC_TABULATED 2
+
+typedef struct owe_spectrum {
+    int32_t kind;
+    int32_t count;
+    int32_t offset;
+    int32_t reserved;
+    double value;  /* constant: the value */
+    double kelvin; /* blackbody temperature */
+    double scale;  /* multiplies every kind (blackbody: luminance normalisation) */
+    double reserved2;
+} owe_spectrum;
+
+/* Refractive-index models (owe::IndexModel::Kind, core/medium.hpp:12-38). */
+#define OWE_INDEX_CONSTANT 0
+#define OWE_INDEX_CAUCHY 1    /* n = A + B/λ² + C/λ⁴, λ in µm: coeff_a[0..2] */
+#define OWE_INDEX_SELLMEIER 2 /* n² = 1 + Σ B λ²/(λ² − C): B in coeff_a, C (µm²) in coeff_b */
+#define OWE_INDEX_CIDDOR 3    /* standard dry air */
+
+typedef struct owe_medium {
+    int32_t index_kind;
+    int32_t terms;   /* Sellmeier terms (≤ 4) */
+    int32_t rel_air; /* coefficients are relative to air: n = n_rel · n_air(λ) */
+    int32_t opaque;
+    double coeff_a[4];
+    double coeff_b[4];
+    owe_spectrum absorption; /* σa(λ) in 1/m */
+    owe_spectrum scattering; /* σs(λ) in 1/m */
+    double g;                /* Henyey–Greenstein anisotropy */
+    double reserved;
+} owe_medium;
+
+/* Surface types (owe::SurfaceType, scene/world.hpp:61-70; stained glass is not supported). */
+#define OWE_SURF_NULL 0
+#define OWE_SURF_DIELECTRIC 1
+#define OWE_SURF_DIFFUSE 2
+#define OWE_SURF_CONDUCTOR 3
+#define OWE_SURF_MIRROR 4
+#define OWE_SURF_ABSORBER 5
+#define OWE_SURF_DETECTOR 6
+
+typedef struct owe_optics {
+    int32_t type;
+    int32_t back_absorbs;
+    double roughness;          /* GGX α; below 1e-3 the surface is smooth (a delta) */
+    owe_spectrum reflectance;  /* diffuse albedo, mirror reflectance, conductor tint */
+    owe_spectrum conductor_n;
+    owe_spectrum conductor_k;
+} owe_optics;
+
+typedef struct owe_emission {
+    owe_spectrum radiance; /* Lambertian spectral radiance */
+    int32_t front;
+    int32_t back;
+    int32_t nee;           /* sampled by next-event estimation */
+    int32_t reserved;
+} owe_emission;
+
+/* Shapes, each in its own frame (scene/geometry.hpp). */
+#define OWE_SHAPE_MESH 0   /* triangles [first_triangle, first_triangle + triangle_count) */
+#define OWE_SHAPE_PLANE 1  /* z = 0 inside an aperture, normal +z */
+#define OWE_SHAPE_SPHERE 2 /* radius a about the origin, normal outward */
+
+#define OWE_APERTURE_DISK 0    /* a = outer radius, b = inner radius */
+#define OWE_APERTURE_RECT 1    /* a, b = half sizes along x, y */
+#define OWE_APERTURE_ELLIPSE 2 /* a, b = semi-axes */
+
+#define OWE_REGION_NONE 0xFFFFFFFEu
+
+typedef struct owe_boundary {
+    int32_t shape;
+    int32_t aperture;
+    uint32_t front;   /* region on the +normal side */
+    uint32_t back;
+    int32_t optics;
+    int32_t emission; /* -1: none */
+    uint32_t first_triangle;
+    uint32_t triangle_count;
+    double a, b;
+    double to_world[12]; /* row-major [R | t]: x_world = R x_local + t (rigid) */
+} owe_boundary;
+
+#define OWE_SKY_NONE 0
+#define OWE_SKY_UNIFORM 1
+
+typedef struct owe_environment {
+    int32_t has_sun;
+    int32_t sky;
+    int32_t reserved[2];
+    double sun_dir[3];         /* unit vector toward the sun */
+    double sun_angular_radius; /* radians */
+    owe_spectrum sun_radiance;
+    owe_spectrum sky_radiance;
+    double sun_nee_share;      /* fraction of light selections given to the sun (0: by power) */
+    double reserved2;
+} owe_environment;
+
+/* An ideal observer (owe::IdealObserver, scene/detector.hpp:84-116): pinhole when pupil_radius is 0. */
+typedef struct owe_observer {
+    int32_t width;
+    int32_t height;
+    uint32_t region;  /* region containing the eye */
+    int32_t reserved;
+    double position[3];
+    double look_at[3];
+    double up[3];
+    double fov_y;          /* radians */
+    double pupil_radius;
+    double focus_distance; /* +inf: focus at infinity */
+    double pixel_sigma;    /* Gaussian pixel filter in pixels (0.4 by default) */
+} owe_observer;
+
+typedef struct owe_scene {
+    uint32_t n_media, n_regions, n_optics, n_emissions, n_boundaries, n_vertices, n_triangles, n_table;
+    const owe_medium* media;
+    const uint32_t* region_medium; /* region → medium; region 0 is the ambient region */
+    const owe_optics* optics;
+    const owe_emission* emissions;
+    const owe_boundary* boundaries;
+    const double* vertices;     /* n_vertices × 3, each in its boundary's own frame */
+    const uint32_t* triangles;  /* n_triangles × 3 vertex indices */
+    const double* table;        /* tabulated spectra */
+    owe_environment env;
+    owe_observer observer;
+} owe_scene;
+
+#define OWE_INTEGRATOR_PATH 0
+#define OWE_INTEGRATOR_LIGHT 1
+
+/* Wavelength sampling: the reference's four stratified hero wavelengths importance-sampled over
+ * 360–830 nm (core/wavelengths.hpp:36-45), or one wavelength per path uniform over
+ * [lambda_min, lambda_max] whose estimate stands for all four strata (pdf/4, as terminateSecondary). */
+#define OWE_SPECTRAL_HERO4 0
+#define OWE_SPECTRAL_UNIFORM1 1
+
+typedef struct owe_render_settings {
+    uint64_t seed;
+    int32_t max_depth;
+    int32_t rr_depth;
+    int32_t integrator;
+    int32_t spectral;
+    double lambda_min, lambda_max;
+    double fresnel_floor;
+} owe_render_settings;
+
+/* TransportStats (scene/transport_stats.hpp:12-31). */
+typedef struct owe_transport_stats {
+    uint64_t paths, segments, inconsistencies, leaks;
+} owe_transport_stats;
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OWE_HIP_SCENE_H */

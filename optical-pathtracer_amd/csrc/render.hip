@@ -1,0 +1,1179 @@
+// The wavefront camera-path renderer: owe_hip_renderer_* of the C ABI.
+//
+// One pass of spp samples per pixel is a batch of W·H·spp independent paths (one PCG32 stream per
+// (pixel, sample, pass); the pixel's own stream gives its hero-wavelength rotation, as
+// CpuRenderer::pass, src/owe/backends/cpu/cpu_backend.cpp:52-94). Paths advance one segment per
+// round of four kernels over structure-of-arrays state in HBM:
+//
+//   generate  camera rays, wavelengths, streams                 (detector.cpp:61-91, cpu_backend.cpp:69-81)
+//   intersect nearest crossing per live path                    (World::intersect, world.cpp:553-582)
+//   shade     media, emission met, sky/sun met, NEE set-up,     (Tracer::radiance body, transport.cpp:481-526)
+//             BSDF sample, Russian roulette; survivors are
+//             compacted into the next queue by a wave ballot
+//   shadow    transmittance of each pending NEE ray             (transport.cpp:237-348)
+//
+// and a final splat converts each path's spectral estimate to XYZ with its final wavelengths
+// (Wavelengths::toXYZ, core/wavelengths.hpp:71-83) and adds it to the film with 64-bit integer
+// atomics in 32.32 fixed point, so the film is independent of the order paths finish.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "transport_dev.hpp"
+
+using namespace owehip;
+
+namespace owehip {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr double kFilmScale = 4294967296.0;  // 2^32 units per unit of XYZ
+
+enum : uint32_t { kFlagSpecular = 1u, kFlagTerminated = 2u };
+
+struct Settings {
+    uint64_t seed;
+    int maxDepth, rrDepth, spectral, pad;
+    float lambdaMin, lambdaMax, fresnelFloor, pad2;
+};
+
+// Structure-of-arrays path state, one slot per path of the current chunk.
+struct Paths {
+    float4* ray0;     // o.xyz, prevPdf
+    float4* ray1;     // d.xyz, camera weight
+    float4* beta;     // throughput per wavelength
+    float4* L;        // radiance estimate per wavelength
+    float4* lam;      // wavelengths
+    float4* prevP;    // last scattering point (MIS of emitters met)
+    uint4* info;      // pixel, depth | flags << 16, region, departed boundary
+    uint32_t* face;   // departed mesh face (~0: none)
+    uint2* rng;       // PCG32 state (the increment is the pass's)
+    uint4* hit;       // t bits, boundary, face, triangle slot
+    float4* shPend;   // pending NEE contribution (β·f·Le·cos·w/pdf), zero when none
+    float4* shO;      // shadow ray origin, distance (inf: the sun)
+    float4* shD;      // direction, unused
+    uint4* shInfo;    // start region, light's region, excluded boundary, departed boundary
+    uint32_t* shFace; // departed face
+    uint32_t* queue[2];
+    uint32_t* count;  // [2] queue lengths; [2] inconsistencies; [3] leaks (as 32-bit halves of 64-bit)
+};
+
+__device__ inline WL loadWL(const float4 l, uint32_t flags, const Settings& s) {
+    WL w;
+    w.lambda[0] = l.x;
+    w.lambda[1] = l.y;
+    w.lambda[2] = l.z;
+    w.lambda[3] = l.w;
+    if (s.spectral == OWE_SPECTRAL_UNIFORM1) {
+        w.pdf[0] = 1.0f / (s.lambdaMax - s.lambdaMin) / NW;
+        w.pdf[1] = w.pdf[2] = w.pdf[3] = 0.0f;
+    } else if (flags & kFlagTerminated) {
+        w.pdf[0] = visiblePdfF(l.x) / NW;
+        w.pdf[1] = w.pdf[2] = w.pdf[3] = 0.0f;
+    } else {
+        for (int i = 0; i < NW; ++i) w.pdf[i] = visiblePdfF(w.lambda[i]);
+    }
+    return w;
+}
+
+__device__ inline float get(const float4& v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w)); }
+__device__ inline void setc(float4& v, int i, float x) {
+    if (i == 0) v.x = x;
+    else if (i == 1) v.y = x;
+    else if (i == 2) v.z = x;
+    else v.w = x;
+}
+
+// ---------------------------------------------------------------- generate
+
+__global__ __launch_bounds__(kBlock) void generateKernel(DScene sc, Settings set, Paths P, uint64_t base, uint32_t count,
+                                                          uint32_t spp, uint32_t pass, const uint32_t* pixelList,
+                                                          const uint32_t* sampleList) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= count) return;
+    const DObserver& ob = sc.obs;
+    uint64_t pixel, s;
+    if (pixelList) {
+        pixel = pixelList[i];
+        s = sampleList[i];
+    } else {
+        const uint64_t g = base + i;
+        pixel = g / spp;
+        s = g % spp;
+    }
+    const uint32_t x = uint32_t(pixel % uint64_t(ob.width)), y = uint32_t(pixel / uint64_t(ob.width));
+    Rng pr;
+    pr.seed(hashPair(set.seed, pixel), pass);
+    const float rot = pr.uniform();
+    Rng rng;
+    rng.seed(hashPair(hashPair(set.seed, pixel), s), pass);
+    float u = (float(s) + rng.uniform()) / float(spp) + rot;
+    u -= floorf(u);
+    float4 lam;
+    float4 beta = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    if (set.spectral == OWE_SPECTRAL_UNIFORM1) {
+        const float l = set.lambdaMin + (set.lambdaMax - set.lambdaMin) * u;
+        lam = make_float4(l, l, l, l);
+        beta = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+    } else {
+        float ls[NW];
+        for (int k = 0; k < NW; ++k) {
+            float up = u + float(k) / NW;
+            if (up >= 1.0f) up -= 1.0f;
+            ls[k] = sampleVisibleF(up);
+        }
+        lam = make_float4(ls[0], ls[1], ls[2], ls[3]);
+    }
+    const float px = float(x) + rng.uniform(), py = float(y) + rng.uniform();
+    float fx = 0.0f, fy = 0.0f;
+    if (ob.sigma > 0.0f) {
+        const float u2 = rng.uniform(), u1 = rng.uniform();
+        const float r = sqrtf(-2.0f * logf(1.0f - u1)), phi = 2.0f * kPiF * u2;
+        fx = r * cosf(phi) * ob.sigma;
+        fy = r * sinf(phi) * ob.sigma;
+    }
+    const float sx = (2.0f * (px + fx) / float(ob.width) - 1.0f) * ob.tanX;
+    const float sy = (1.0f - 2.0f * (py + fy) / float(ob.height)) * ob.tanY;
+    const f3 right = ld3(ob.right), up = ld3(ob.up), fwd = ld3(ob.fwd);
+    const f3 aim = right * sx + up * sy + fwd;
+    f3 q = mk3(0.0f, 0.0f, 0.0f), dir = norm3(aim);
+    if (ob.pupil > 0.0f) {
+        const float u2 = rng.uniform(), u1 = rng.uniform();
+        float dx, dy;
+        concentricDisk(u1, u2, dx, dy);
+        q = (right * dx + up * dy) * ob.pupil;
+        if (isfinite(ob.focus)) dir = norm3(aim * ob.focus - q);
+    }
+    P.ray0[i] = make_float4(q.x, q.y, q.z, 0.0f);
+    P.ray1[i] = make_float4(dir.x, dir.y, dir.z, 1.0f);
+    P.beta[i] = beta;
+    P.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    P.lam[i] = lam;
+    P.prevP[i] = make_float4(q.x, q.y, q.z, 0.0f);
+    P.info[i] = make_uint4(uint32_t(pixel), kFlagSpecular << 16, ob.region, kNoBoundary);
+    P.face[i] = ~0u;
+    P.rng[i] = make_uint2(uint32_t(rng.state), uint32_t(rng.state >> 32));
+    P.queue[0][i] = i;
+}
+
+// ---------------------------------------------------------------- intersect
+
+template <int kCap>
+__global__ __launch_bounds__(kBlock) void intersectKernel(DScene sc, Paths P, int q, uint32_t count) {
+    __shared__ uint32_t stack[kCap * kBlock];
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= count) return;
+    const uint32_t slot = P.queue[q][k];
+    const float4 r0 = P.ray0[slot], r1 = P.ray1[slot];
+    Hit h;
+    const bool found = intersectWorld(sc, mk3(r0.x, r0.y, r0.z), mk3(r1.x, r1.y, r1.z), __builtin_inff(), P.info[slot].w,
+                                      P.face[slot], kNoBoundary, stack + threadIdx.x, kBlock, h);
+    P.hit[slot] = found ? make_uint4(__float_as_uint(h.t), h.boundary, h.face, h.slot)
+                        : make_uint4(__float_as_uint(__builtin_inff()), kNoBoundary, 0u, kEmptyChild);
+}
+
+// ---------------------------------------------------------------- shade
+
+struct ShadeCtx {
+    const DScene& sc;
+    const Settings& set;
+    Rng rng;
+    WL wl;
+    float4 beta, L;
+    f3 o, d, prevP;
+    float prevPdf, weight;
+    uint32_t pixel, depth, flags, region, departed, face;
+    float4 shPend;
+    f3 shO, shD;
+    float shDist;
+    uint32_t shStart, shLightRegion, shExclude, shDeparted, shFace;
+};
+
+__device__ inline float regionIndex(const DScene& sc, uint32_t region, float l) {
+    return mediumIndex(sc.media[sc.regionMedium[region]], l);
+}
+
+struct Iface {
+    const DOptics* o;
+    f3 n;
+    float nFront, nBack, lambda, floor;
+};
+
+struct SSample {
+    f3 wi;
+    float weight, pdf;
+    bool delta, transmitted;
+};
+
+__device__ inline Cx conductorEta(const DScene& sc, const DOptics& o, float lambda, float nMed) {
+    const float inv = 1.0f / nMed;
+    return {evalSpec(o.condN, sc.table, lambda) * inv, evalSpec(o.condK, sc.table, lambda) * inv};
+}
+
+__device__ inline bool specularReflection(const Frame& f, f3 wo, float weight, SSample& s) {
+    s.wi = toWorld(f, mk3(-wo.x, -wo.y, wo.z));
+    s.weight = weight;
+    s.delta = true;
+    return weight > 0.0f;
+}
+
+// sampleScatter (src/owe/transport/scattering.cpp:65-157), radiance transport.
+__device__ inline bool sampleScatter(const DScene& sc, const Iface& it, f3 d, float uc, float u1, float u2, SSample& s) {
+    const DOptics& o = *it.o;
+    const Frame frame = makeFrame(it.n);
+    const f3 wo = toLocal(frame, -d);
+    const bool fromFront = wo.z > 0.0f;
+    s = SSample{d, 0.0f, 0.0f, false, false};
+    switch (o.type) {
+        case OWE_SURF_DIELECTRIC: {
+            const float eta = it.nBack / it.nFront;
+            if (o.roughness >= 1e-3f && eta != 1.0f) {
+                Lobe l;
+                if (!sampleRoughDielectric(ggxAlpha(o.roughness), wo, eta, uc, u1, u2, true, l)) return false;
+                s.wi = toWorld(frame, l.wi);
+                s.weight = l.weight;
+                s.pdf = l.pdf;
+                s.transmitted = l.transmitted;
+                return true;
+            }
+            const float ni = fromFront ? it.nFront : it.nBack, nt = fromFront ? it.nBack : it.nFront;
+            const float R = fresnelDielectric(wo.z, eta);
+            const float pR = it.floor > 0.0f && R < 1.0f ? clampf(R, it.floor, 1.0f - it.floor) : R;
+            s.delta = true;
+            if (uc < pR) {
+                specularReflection(frame, wo, R / pR, s);
+                return true;
+            }
+            if (!refractDir(d, it.n, ni, nt, s.wi)) return false;
+            s.weight = (1.0f - R) / (1.0f - pR) * sqf(ni / nt);
+            s.transmitted = true;
+            return true;
+        }
+        case OWE_SURF_DIFFUSE: {
+            if (!fromFront && o.backAbsorbs) return false;
+            const float albedo = evalSpec(o.reflectance, sc.table, it.lambda);
+            if (albedo <= 0.0f) return false;
+            f3 w = cosineHemisphere(u1, u2);
+            if (!fromFront) w.z = -w.z;
+            s.wi = toWorld(frame, w);
+            s.pdf = fabsf(w.z) * kInvPiF;
+            s.weight = albedo;
+            return s.pdf > 0.0f;
+        }
+        case OWE_SURF_MIRROR:
+            if (!fromFront && o.backAbsorbs) return false;
+            return specularReflection(frame, wo, evalSpec(o.reflectance, sc.table, it.lambda), s);
+        case OWE_SURF_CONDUCTOR: {
+            if (!fromFront && o.backAbsorbs) return false;
+            const Cx eta = conductorEta(sc, o, it.lambda, fromFront ? it.nFront : it.nBack);
+            const float tint = evalSpec(o.reflectance, sc.table, it.lambda);
+            const f3 w = fromFront ? wo : mk3(wo.x, wo.y, -wo.z);
+            if (o.roughness < 1e-3f) return specularReflection(frame, wo, fresnelConductor(w.z, eta) * tint, s);
+            Lobe l;
+            if (!sampleRoughConductor(ggxAlpha(o.roughness), w, eta, tint, u1, u2, l)) return false;
+            if (!fromFront) l.wi.z = -l.wi.z;
+            s.wi = toWorld(frame, l.wi);
+            s.weight = l.weight;
+            s.pdf = l.pdf;
+            return true;
+        }
+        default: return false;
+    }
+}
+
+// evalScatter (scattering.cpp:159-188).
+__device__ inline float evalScatter(const DScene& sc, const Iface& it, f3 d, f3 wiW, float& pdf) {
+    pdf = 0.0f;
+    const DOptics& o = *it.o;
+    if (o.delta) return 0.0f;
+    const Frame frame = makeFrame(it.n);
+    f3 wo = toLocal(frame, -d), wi = toLocal(frame, wiW);
+    const bool fromFront = wo.z > 0.0f;
+    switch (o.type) {
+        case OWE_SURF_DIFFUSE:
+            if (!fromFront && o.backAbsorbs) return 0.0f;
+            if (!sameHemi(wo, wi)) return 0.0f;
+            pdf = fabsf(wi.z) * kInvPiF;
+            return evalSpec(o.reflectance, sc.table, it.lambda) * kInvPiF;
+        case OWE_SURF_CONDUCTOR: {
+            if (!fromFront && o.backAbsorbs) return 0.0f;
+            if (!sameHemi(wo, wi)) return 0.0f;
+            if (!fromFront) {
+                wo.z = -wo.z;
+                wi.z = -wi.z;
+            }
+            const Cx eta = conductorEta(sc, o, it.lambda, fromFront ? it.nFront : it.nBack);
+            return evalRoughConductor(ggxAlpha(o.roughness), wo, wi, eta, evalSpec(o.reflectance, sc.table, it.lambda), pdf);
+        }
+        case OWE_SURF_DIELECTRIC:
+            return evalRoughDielectric(ggxAlpha(o.roughness), wo, wi, it.nBack / it.nFront, true, pdf);
+        default: return 0.0f;
+    }
+}
+
+__device__ inline float secondaryWeight(const DScene& sc, const Iface& it, f3 d, const SSample& s) {
+    const DOptics& o = *it.o;
+    if (!s.delta) {
+        float pdf;
+        const float f = evalScatter(sc, it, d, s.wi, pdf);
+        return s.pdf > 0.0f ? f * fabsf(dot3(s.wi, it.n)) / s.pdf : 0.0f;
+    }
+    if (o.type == OWE_SURF_DIELECTRIC) return s.weight;
+    if (o.type == OWE_SURF_MIRROR) return evalSpec(o.reflectance, sc.table, it.lambda);
+    if (o.type == OWE_SURF_CONDUCTOR) {
+        const float nMed = dot3(d, it.n) < 0.0f ? it.nFront : it.nBack;
+        return fresnelConductor(fabsf(dot3(d, it.n)), conductorEta(sc, o, it.lambda, nMed)) *
+               evalSpec(o.reflectance, sc.table, it.lambda);
+    }
+    return 0.0f;
+}
+
+// Solid-angle pdf of a light point on analytic boundary b seen from ref (Shape::pdfFrom).
+__device__ inline float sphereConeSize(f3 ref, float R) {
+    const float dc2 = dot3(ref, ref);
+    if (dc2 <= R * R * (1.0f + 1e-6f)) return 0.0f;
+    const float s2 = R * R / dc2;
+    return s2 / (1.0f + sqrt0f(1.0f - s2));
+}
+__device__ inline bool areaToSolidAngle(f3 ref, f3 p, f3 n, float area, float& pdfW) {
+    pdfW = 0.0f;
+    const f3 w = p - ref;
+    const float d2 = dot3(w, w);
+    if (!(d2 > 0.0f)) return false;
+    const float cosL = fabsf(dot3(n, w)) / sqrtf(d2);
+    if (cosL <= 1e-7f) return false;
+    pdfW = d2 / (cosL * area);
+    return true;
+}
+__device__ inline float pdfFrom(const DBoundary& b, f3 refLocal, f3 pLocal, f3 nLocal) {
+    if (b.shape == OWE_SHAPE_SPHERE) {
+        const float omc = sphereConeSize(refLocal, b.a);
+        if (omc > 0.0f) return 1.0f / (2.0f * kPiF * omc);
+    }
+    float pdfW;
+    return b.area > 0.0f && areaToSolidAngle(refLocal, pLocal, nLocal, b.area, pdfW) ? pdfW : 0.0f;
+}
+__device__ inline bool sampleFrom(const DBoundary& b, f3 ref, float u1, float u2, float u3, f3& p, f3& n, float& pdfW) {
+    if (b.shape == OWE_SHAPE_SPHERE) {
+        const float omc = sphereConeSize(ref, b.a);
+        if (omc > 0.0f) {
+            const f3 axis = -ref / sqrtf(dot3(ref, ref));
+            const Frame f = makeFrame(axis);
+            const f3 wl = uniformConeOmc(u1, u2, omc);
+            const f3 w = f.s * wl.x + f.t * wl.y + axis * wl.z;
+            float t0, t1;
+            if (sphereRoots(ref, w, b.a, t0, t1) && (t0 > 0.0f || t1 > 0.0f)) p = ref + w * (t0 > 0.0f ? t0 : t1);
+            else p = norm3(ref + w * dot3(-ref, w)) * b.a;
+            n = norm3(p);
+            pdfW = 1.0f / (2.0f * kPiF * omc);
+            return true;
+        }
+        n = uniformSphere(u1, u2);
+        p = n * b.a;
+        return areaToSolidAngle(ref, p, n, b.area, pdfW);
+    }
+    n = mk3(0, 0, 1);
+    if (b.aperture == OWE_APERTURE_DISK) {
+        const float r = sqrtf(b.b * b.b + u1 * (b.a * b.a - b.b * b.b)), phi = 2.0f * kPiF * u2;
+        p = mk3(r * cosf(phi), r * sinf(phi), 0.0f);
+    } else if (b.aperture == OWE_APERTURE_RECT) {
+        p = mk3((2.0f * u1 - 1.0f) * b.a, (2.0f * u2 - 1.0f) * b.b, 0.0f);
+    } else {
+        const float r = sqrtf(u1), phi = 2.0f * kPiF * u2;
+        p = mk3(b.a * r * cosf(phi), b.b * r * sinf(phi), 0.0f);
+    }
+    return areaToSolidAngle(ref, p, n, b.area, pdfW);
+}
+
+// Next-event estimation at point p (sampleLightFrom + directLighting, transport.cpp:35-70, 323-348):
+// draws the light, evaluates everything but the transmittance, and leaves a shadow ray.
+__device__ inline void prepareDirect(ShadeCtx& c, f3 p, bool surface, const Iface* it, uint32_t hitBoundary,
+                                     uint32_t hitFace, uint32_t hitSlot, f3 n, float g) {
+    const DScene& sc = c.sc;
+    if (sc.nLights == 0) return;
+    const float ul = c.rng.uniform();
+    uint32_t li = 0;
+    while (li + 1 < sc.nLights && sc.lights[li].cdf <= ul) ++li;
+    const DLight L = sc.lights[li];
+    const float u1 = c.rng.uniform(), u2 = c.rng.uniform(), u3 = c.rng.uniform();
+    f3 wi;
+    float dist = __builtin_inff(), lpdf;
+    float Le[NW];
+    uint32_t lightRegion = kRegionNone, exclude = kNoBoundary;
+    if (L.kind == 1) {
+        const Frame f = makeFrame(ld3(sc.env.sunDir));
+        wi = toWorld(f, uniformConeOmc(u1, u2, sc.env.sunOneMinusCos));
+        for (int i = 0; i < NW; ++i) Le[i] = evalSpec(sc.env.sunRadiance, sc.table, c.wl.lambda[i]);
+        lpdf = L.pdf / sc.env.sunSolidAngle;
+    } else {
+        const DBoundary& b = sc.boundaries[L.boundary];
+        f3 pl, nl;
+        float pdfW;
+        if (!sampleFrom(b, xfPoint(b.toLocal, p), u1, u2, u3, pl, nl, pdfW)) return;
+        const f3 ep = xfPoint(b.toWorld, pl), en = norm3(xfVector(b.toWorld, nl));
+        const f3 dv = ep - p;
+        dist = len3(dv);
+        if (!(dist > 0.0f)) return;
+        wi = dv / dist;
+        const DEmission& e = sc.emissions[b.emission];
+        const float cosL = dot3(en, -wi);
+        const bool lit = (cosL > 0.0f && e.front) || (cosL < 0.0f && e.back);
+        for (int i = 0; i < NW; ++i) Le[i] = lit ? evalSpec(e.radiance, sc.table, c.wl.lambda[i]) : 0.0f;
+        lpdf = L.pdf * pdfW;
+        lightRegion = cosL > 0.0f ? b.front : b.back;
+        exclude = L.boundary;
+    }
+    if (!(lpdf > 0.0f)) return;
+    bool any = false;
+    for (int i = 0; i < NW; ++i) any = any || Le[i] != 0.0f;
+    if (!any) return;
+    float f[NW] = {0, 0, 0, 0};
+    float pdfScatter = 0.0f, cosFactor = 1.0f;
+    uint32_t start = c.region;
+    if (!surface) {
+        pdfScatter = hgPhase(dot3(-c.d, wi), g);
+        for (int i = 0; i < NW; ++i) f[i] = pdfScatter;
+    } else {
+        const DBoundary& b = sc.boundaries[hitBoundary];
+        bool nz = false;
+        for (int i = 0; i < NW; ++i) {
+            if (c.wl.pdf[i] == 0.0f) continue;
+            Iface at = *it;
+            if (i > 0) {
+                at.nFront = regionIndex(sc, b.front, c.wl.lambda[i]);
+                at.nBack = regionIndex(sc, b.back, c.wl.lambda[i]);
+                at.lambda = c.wl.lambda[i];
+            }
+            float pdf;
+            f[i] = evalScatter(sc, at, c.d, wi, pdf);
+            if (i == 0) pdfScatter = pdf;
+            nz = nz || f[i] != 0.0f;
+        }
+        if (!nz) return;
+        cosFactor = fabsf(dot3(wi, n));
+        start = dot3(wi, n) > 0.0f ? b.front : b.back;
+    }
+    const float w = powerHeuristic(lpdf, pdfScatter) * cosFactor / lpdf;
+    c.shPend = make_float4(get(c.beta, 0) * f[0] * Le[0] * w, get(c.beta, 1) * f[1] * Le[1] * w,
+                           get(c.beta, 2) * f[2] * Le[2] * w, get(c.beta, 3) * f[3] * Le[3] * w);
+    c.shO = p;
+    c.shD = wi;
+    c.shDist = dist;
+    c.shStart = start;
+    c.shLightRegion = lightRegion;
+    c.shExclude = exclude;
+    c.shDeparted = surface ? hitBoundary : kNoBoundary;
+    c.shFace = surface && hitSlot != kEmptyChild ? hitFace : ~0u;
+}
+
+__device__ inline bool survives(ShadeCtx& c) {
+    const float q = fminf(1.0f, fmaxf(fmaxf(c.beta.x, c.beta.y), fmaxf(c.beta.z, c.beta.w)));
+    if (c.rng.uniform() >= q) return false;
+    const float s = 1.0f / q;
+    c.beta = make_float4(c.beta.x * s, c.beta.y * s, c.beta.z * s, c.beta.w * s);
+    return true;
+}
+
+// One iteration of Tracer::radiance's loop after the intersection; false when the path ends.
+__device__ inline bool shadeOne(ShadeCtx& c, const uint4 hraw, uint32_t* counters) {
+    const DScene& sc = c.sc;
+    const bool found = hraw.y != kNoBoundary;
+    const float t = __uint_as_float(hraw.x);
+    const DMedium& med = sc.media[sc.regionMedium[c.region]];
+    // sampleMedium (transport.cpp:95-113).
+    if (med.flags & 2) {
+        float sT[NW];
+        for (int i = 0; i < NW; ++i)
+            sT[i] = evalSpec(med.absorption, sc.table, c.wl.lambda[i]) + evalSpec(med.scattering, sc.table, c.wl.lambda[i]);
+        const float tm = -logf(1.0f - c.rng.uniform()) / sT[0];
+        const float seg = found ? t : __builtin_inff();
+        if (tm < seg) {
+            const float pdf0 = sT[0] * expf(-sT[0] * tm);
+            for (int i = 0; i < NW; ++i)
+                setc(c.beta, i, get(c.beta, i) * evalSpec(med.scattering, sc.table, c.wl.lambda[i]) * expf(-sT[i] * tm) / pdf0);
+            // scatterInMedium (transport.cpp:379-397).
+            const f3 p = c.o + c.d * tm;
+            if (int(c.depth) >= c.set.maxDepth) return false;
+            prepareDirect(c, p, false, nullptr, kNoBoundary, 0, kEmptyChild, mk3(0, 0, 0), med.g);
+            const float u2 = c.rng.uniform(), u1 = c.rng.uniform();
+            f3 wi;
+            c.prevPdf = sampleHg(-c.d, med.g, u1, u2, wi);
+            c.flags &= ~kFlagSpecular;
+            c.prevP = p;
+            c.o = p;
+            c.d = wi;
+            c.departed = kNoBoundary;
+            c.face = ~0u;
+            return int(c.depth) < c.set.rrDepth || survives(c);
+        }
+        if (!found) return false;
+        const float pass0 = expf(-sT[0] * seg);
+        for (int i = 0; i < NW; ++i) setc(c.beta, i, get(c.beta, i) * expf(-sT[i] * seg) / pass0);
+    } else if (med.flags & 1) {
+        if (!found) return false;
+        for (int i = 0; i < NW; ++i) setc(c.beta, i, get(c.beta, i) * expf(-evalSpec(med.absorption, sc.table, c.wl.lambda[i]) * t));
+    }
+    if (!found) {
+        if (c.region == 0) {
+            // skyMet (transport.cpp:401-417); the sun's disc test in the cancellation-free form
+            // 1 − cos θ = |d − s|² / 2 for unit vectors.
+            const f3 s = ld3(sc.env.sunDir), dd = c.d - s;
+            const bool inSun = sc.env.hasSun && 0.5f * dot3(dd, dd) <= sc.env.sunOneMinusCos;
+            float w = 1.0f;
+            if (inSun && !(c.flags & kFlagSpecular)) w = powerHeuristic(c.prevPdf, sc.env.sunPdf);
+            for (int i = 0; i < NW; ++i) {
+                const float sky = sc.env.sky == OWE_SKY_UNIFORM ? evalSpec(sc.env.skyRadiance, sc.table, c.wl.lambda[i]) : 0.0f;
+                const float sun = inSun ? evalSpec(sc.env.sunRadiance, sc.table, c.wl.lambda[i]) : 0.0f;
+                setc(c.L, i, get(c.L, i) + get(c.beta, i) * (sky + sun * w));
+            }
+        } else {
+            atomicAdd(&counters[3], 1u);
+        }
+        return false;
+    }
+    // A boundary met: the point, the normal and the sides (meetBoundary, transport.cpp:121-133).
+    const uint32_t bi = hraw.y;
+    const DBoundary& b = sc.boundaries[bi];
+    f3 p, n, pl = mk3(0, 0, 0), nl = mk3(0, 0, 1);
+    if (hraw.w != kEmptyChild) {
+        const float4* tr = reinterpret_cast<const float4*>(sc.tris + hraw.w);
+        const float4 a = tr[0], bb = tr[1], cc = tr[2];
+        n = norm3(triNormal(mk3(a.x, a.y, a.z), mk3(bb.x, bb.y, bb.z), mk3(cc.x, cc.y, cc.z)));
+        p = c.o + c.d * t;
+    } else {
+        const f3 lo = xfPoint(b.toLocal, c.o), ldv = xfVector(b.toLocal, c.d);
+        pl = lo + ldv * t;
+        if (b.shape == OWE_SHAPE_PLANE) {
+            pl.z = 0.0f;
+            nl = mk3(0, 0, 1);
+        } else {
+            nl = norm3(pl / b.a);
+        }
+        p = xfPoint(b.toWorld, pl);
+        n = norm3(xfVector(b.toWorld, nl));
+    }
+    const bool fromFront = dot3(c.d, n) < 0.0f;
+    const uint32_t inc = fromFront ? b.front : b.back, other = fromFront ? b.back : b.front;
+    if (inc != c.region) {
+        atomicAdd(&counters[2], 1u);
+        c.region = inc;
+    }
+    const DOptics& o = sc.optics[b.optics];
+    // emissionMet (transport.cpp:420-433).
+    if (b.emission >= 0) {
+        const DEmission& e = sc.emissions[b.emission];
+        if (fromFront ? e.front : e.back) {
+            float w = 1.0f;
+            if (!(c.flags & kFlagSpecular) && b.light >= 0) {
+                const float pl2 = sc.lights[b.light].pdf * pdfFrom(b, xfPoint(b.toLocal, c.prevP), pl, nl);
+                if (pl2 > 0.0f) w = powerHeuristic(c.prevPdf, pl2);
+            }
+            for (int i = 0; i < NW; ++i)
+                setc(c.L, i, get(c.L, i) + get(c.beta, i) * evalSpec(e.radiance, sc.table, c.wl.lambda[i]) * w);
+        }
+    }
+    const uint32_t hitFace = hraw.w != kEmptyChild ? hraw.z : ~0u;
+    if (o.type == OWE_SURF_NULL) {
+        c.region = other;
+        c.o = p;
+        c.departed = bi;
+        c.face = hitFace;
+        return int(c.depth) < 4 * c.set.maxDepth;
+    }
+    if (o.type == OWE_SURF_ABSORBER || o.type == OWE_SURF_DETECTOR) return false;
+    if (int(c.depth) >= c.set.maxDepth) return false;
+    // scatterAtSurface (transport.cpp:437-471).
+    const float hero = c.wl.lambda[0];
+    Iface it{&o, n, regionIndex(sc, b.front, hero), regionIndex(sc, b.back, hero), hero, c.set.fresnelFloor};
+    auto at = [&](int i) {
+        Iface r = it;
+        r.nFront = regionIndex(sc, b.front, c.wl.lambda[i]);
+        r.nBack = regionIndex(sc, b.back, c.wl.lambda[i]);
+        r.lambda = c.wl.lambda[i];
+        return r;
+    };
+    if (o.type == OWE_SURF_DIELECTRIC && !secondaryTerminated(c.wl)) {
+        const float a0 = it.nBack / it.nFront;
+        for (int i = 1; i < NW; ++i) {
+            if (c.wl.pdf[i] == 0.0f) continue;
+            const Iface ai = at(i);
+            if (fabsf(a0 - ai.nBack / ai.nFront) > 1e-12f * fabsf(a0)) {
+                c.flags |= kFlagTerminated;
+                c.wl.pdf[0] /= NW;
+                c.wl.pdf[1] = c.wl.pdf[2] = c.wl.pdf[3] = 0.0f;
+                c.beta.y = c.beta.z = c.beta.w = 0.0f;
+                break;
+            }
+        }
+    }
+    if (!o.delta) prepareDirect(c, p, true, &it, bi, hraw.z, hraw.w, n, 0.0f);
+    const float u3 = c.rng.uniform(), u2 = c.rng.uniform(), u1 = c.rng.uniform();
+    SSample s;
+    if (!sampleScatter(sc, it, c.d, u1, u2, u3, s) || s.weight <= 0.0f) return false;
+    c.beta.x *= s.weight;
+    for (int i = 1; i < NW; ++i)
+        if (c.wl.pdf[i] != 0.0f) setc(c.beta, i, get(c.beta, i) * secondaryWeight(sc, at(i), c.d, s));
+    if (s.transmitted) c.region = other;
+    if (s.delta) c.flags |= kFlagSpecular;
+    else c.flags &= ~kFlagSpecular;
+    c.prevPdf = s.pdf;
+    c.prevP = p;
+    c.o = p;
+    c.d = s.wi;
+    c.departed = bi;
+    c.face = hitFace;
+    if (int(c.depth) >= c.set.rrDepth && !survives(c)) return false;
+    return true;
+}
+
+__global__ __launch_bounds__(kBlock) void shadeKernel(DScene sc, Settings set, Paths P, int q, uint32_t count,
+                                                      uint64_t passInc) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    bool alive = false;
+    uint32_t slot = 0;
+    if (k < count) {
+        slot = P.queue[q][k];
+        const float4 r0 = P.ray0[slot], r1 = P.ray1[slot];
+        const uint4 info = P.info[slot];
+        const uint2 rs = P.rng[slot];
+        ShadeCtx c{sc, set};
+        c.rng.state = uint64_t(rs.x) | (uint64_t(rs.y) << 32);
+        c.rng.inc = passInc;
+        c.flags = info.y >> 16;
+        c.depth = info.y & 0xFFFFu;
+        c.wl = loadWL(P.lam[slot], c.flags, set);
+        c.beta = P.beta[slot];
+        c.L = P.L[slot];
+        c.o = mk3(r0.x, r0.y, r0.z);
+        c.prevPdf = r0.w;
+        c.d = mk3(r1.x, r1.y, r1.z);
+        const float4 pp = P.prevP[slot];
+        c.prevP = mk3(pp.x, pp.y, pp.z);
+        c.pixel = info.x;
+        c.region = info.z;
+        c.departed = info.w;
+        c.face = P.face[slot];
+        c.shPend = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        alive = shadeOne(c, P.hit[slot], P.count);
+        P.L[slot] = c.L;
+        P.shPend[slot] = c.shPend;
+        if (c.shPend.x != 0.0f || c.shPend.y != 0.0f || c.shPend.z != 0.0f || c.shPend.w != 0.0f) {
+            P.shO[slot] = make_float4(c.shO.x, c.shO.y, c.shO.z, c.shDist);
+            P.shD[slot] = make_float4(c.shD.x, c.shD.y, c.shD.z, 0.0f);
+            P.shInfo[slot] = make_uint4(c.shStart, c.shLightRegion, c.shExclude, c.shDeparted);
+            P.shFace[slot] = c.shFace;
+        }
+        // The wavelengths' termination changes the splat; the rest only matters to a live path.
+        P.info[slot] = make_uint4(c.pixel, (c.depth + 1) | (c.flags << 16), c.region, c.departed);
+        if (alive) {
+            P.ray0[slot] = make_float4(c.o.x, c.o.y, c.o.z, c.prevPdf);
+            P.ray1[slot] = make_float4(c.d.x, c.d.y, c.d.z, r1.w);
+            P.beta[slot] = c.beta;
+            P.prevP[slot] = make_float4(c.prevP.x, c.prevP.y, c.prevP.z, 0.0f);
+            P.face[slot] = c.face;
+            P.rng[slot] = make_uint2(uint32_t(c.rng.state), uint32_t(c.rng.state >> 32));
+        }
+    }
+    // Wave-level compaction of the survivors: one atomic per wave, a prefix count within it.
+    const uint64_t mask = __ballot(alive);
+    if (mask == 0) return;
+    const uint32_t lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if (int(lane) == leader) base = atomicAdd(&P.count[1 - q], uint32_t(__popcll(mask)));
+    base = __shfl(base, leader);
+    if (alive) {
+        const uint32_t below = __popcll(mask & ((1ull << lane) - 1ull));
+        P.queue[1 - q][base + below] = slot;
+    }
+}
+
+// ---------------------------------------------------------------- shadow rays
+
+// Tracer::transmittance (transport.cpp:237-285): through null boundaries with Beer–Lambert.
+template <int kCap>
+__global__ __launch_bounds__(kBlock) void shadowKernel(DScene sc, Settings set, Paths P, int q, uint32_t count,
+                                                       int simple) {
+    __shared__ uint32_t stack[kCap * kBlock];
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= count) return;
+    const uint32_t slot = P.queue[q][k];
+    const float4 pend = P.shPend[slot];
+    if (pend.x == 0.0f && pend.y == 0.0f && pend.z == 0.0f && pend.w == 0.0f) return;
+    const float4 so = P.shO[slot], sd = P.shD[slot];
+    const uint4 si = P.shInfo[slot];
+    f3 o = mk3(so.x, so.y, so.z);
+    const f3 d = mk3(sd.x, sd.y, sd.z);
+    float remaining = so.w;
+    const bool infinite = isinf(remaining);
+    uint32_t departed = si.w, face = P.shFace[slot], region = si.x;
+    float Tr[NW] = {1.0f, 1.0f, 1.0f, 1.0f};
+    const float4 lam = P.lam[slot];
+    for (int guard = 0; guard < 64; ++guard) {
+        Hit h;
+        const bool found = intersectWorld(sc, o, d, remaining, departed, face, si.z, stack + threadIdx.x, kBlock, h);
+        if (simple) {
+            // No null boundaries or participating media in the scene: any crossing blocks.
+            if (found) return;
+            if (infinite && region != 0) return;
+            break;
+        }
+        const float seg = found ? fminf(h.t, remaining) : remaining;
+        const DMedium& m = sc.media[sc.regionMedium[region]];
+        if (m.flags) {
+            if (isinf(seg)) return;
+            for (int i = 0; i < NW; ++i) {
+                const float l = get(lam, i);
+                Tr[i] *= expf(-(evalSpec(m.absorption, sc.table, l) + evalSpec(m.scattering, sc.table, l)) * seg);
+            }
+        }
+        if (!found) {
+            if (infinite && region != 0) return;
+            if (!infinite && region != si.y) return;
+            break;
+        }
+        const DBoundary& b = sc.boundaries[h.boundary];
+        const bool front = dot3(d, h.n) < 0.0f;
+        if ((front ? b.front : b.back) != region) return;
+        if (sc.optics[b.optics].type != OWE_SURF_NULL) return;
+        region = front ? b.back : b.front;
+        o = h.p;
+        departed = h.boundary;
+        face = h.slot != kEmptyChild ? h.face : ~0u;
+        if (!infinite) remaining -= h.t;
+        if (guard == 63) return;  // exhausting the guard is not evidence the segment is clear
+    }
+    if (simple && !infinite && region != si.y) return;
+    float4 L = P.L[slot];
+    L.x += pend.x * Tr[0];
+    L.y += pend.y * Tr[1];
+    L.z += pend.z * Tr[2];
+    L.w += pend.w * Tr[3];
+    P.L[slot] = L;
+}
+
+// ---------------------------------------------------------------- splat
+
+__global__ __launch_bounds__(kBlock) void splatKernel(Settings set, Paths P, uint32_t count, float invNWY,
+                                                      unsigned long long* film, double* perPath) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= count) return;
+    const float4 L = P.L[i];
+    const uint4 info = P.info[i];
+    const WL wl = loadWL(P.lam[i], info.y >> 16, set);
+    const float weight = P.ray1[i].w;
+    float X = 0.0f, Y = 0.0f, Z = 0.0f;
+    for (int k = 0; k < NW; ++k) {
+        const float v = get(L, k) * weight;
+        if (wl.pdf[k] == 0.0f || v == 0.0f) continue;
+        const float w = v / wl.pdf[k] * invNWY;
+        const f3 c = cieXYZ(wl.lambda[k]);
+        X += w * c.x;
+        Y += w * c.y;
+        Z += w * c.z;
+    }
+    if (perPath) {
+        const bool ok = isfinite(X + Y + Z);
+        perPath[3 * size_t(i)] = ok ? X : 0.0;
+        perPath[3 * size_t(i) + 1] = ok ? Y : 0.0;
+        perPath[3 * size_t(i) + 2] = ok ? Z : 0.0;
+        return;
+    }
+    if (!isfinite(X + Y + Z) || (X == 0.0f && Y == 0.0f && Z == 0.0f)) return;
+    const float c[3] = {X, Y, Z};
+    for (int k = 0; k < 3; ++k) {
+        // |value| stays below 2^30 units of XYZ: far above any sample here, and no overflow of the
+        // 64-bit two's-complement sum.
+        const double scaled = fmin(fmax(double(c[k]) * kFilmScale, -4.6e18), 4.6e18);
+        const long long v = llrint(scaled);
+        atomicAdd(&film[3 * size_t(info.x) + k], (unsigned long long)v);
+    }
+}
+
+__global__ void clearCounts(uint32_t* c, int n) {
+    if (threadIdx.x < unsigned(n)) c[threadIdx.x] = 0;
+}
+
+}  // namespace
+}  // namespace owehip
+
+// ---------------------------------------------------------------- the renderer object
+
+struct owe_hip_renderer {
+    int device = 0;
+    HostScene host;
+    DScene dev{};
+    Settings set{};
+    std::vector<void*> allocations;
+    Paths P{};
+    uint32_t capacity = 0;  // path slots
+    unsigned long long* film = nullptr;
+    double* perPath = nullptr;
+    uint32_t* pathPixels = nullptr;
+    uint32_t* pathSamples = nullptr;
+    uint32_t listCapacity = 0;
+    int passes = 0;
+    long long totalSpp = 0;
+    owe_transport_stats stats{};
+    hipStream_t stream = nullptr;
+    uint32_t stackCap = 16;
+    bool simpleShadows = true;
+    float invNWY = 0;
+    // Optional per-kernel timing with HIP events on the renderer's stream: generate, intersect,
+    // shade, shadow, splat.
+    bool timing = false;
+    hipEvent_t ev[2][5] = {};
+    double kernelMs[5] = {0, 0, 0, 0, 0};
+    uint64_t kernelLaunches[5] = {0, 0, 0, 0, 0};
+};
+
+namespace owehip {
+namespace {
+
+template <class T>
+T* devAlloc(owe_hip_renderer* r, size_t count) {
+    void* p = nullptr;
+    OWE_HIP_CHECK(hipMalloc(&p, std::max<size_t>(1, count) * sizeof(T)));
+    r->allocations.push_back(p);
+    return static_cast<T*>(p);
+}
+template <class T>
+const T* upload(owe_hip_renderer* r, const std::vector<T>& v) {
+    T* p = devAlloc<T>(r, v.size());
+    if (!v.empty()) OWE_HIP_CHECK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return p;
+}
+
+void allocPaths(owe_hip_renderer* r, uint32_t n) {
+    Paths& P = r->P;
+    P.ray0 = devAlloc<float4>(r, n);
+    P.ray1 = devAlloc<float4>(r, n);
+    P.beta = devAlloc<float4>(r, n);
+    P.L = devAlloc<float4>(r, n);
+    P.lam = devAlloc<float4>(r, n);
+    P.prevP = devAlloc<float4>(r, n);
+    P.info = devAlloc<uint4>(r, n);
+    P.face = devAlloc<uint32_t>(r, n);
+    P.rng = devAlloc<uint2>(r, n);
+    P.hit = devAlloc<uint4>(r, n);
+    P.shPend = devAlloc<float4>(r, n);
+    P.shO = devAlloc<float4>(r, n);
+    P.shD = devAlloc<float4>(r, n);
+    P.shInfo = devAlloc<uint4>(r, n);
+    P.shFace = devAlloc<uint32_t>(r, n);
+    P.queue[0] = devAlloc<uint32_t>(r, n);
+    P.queue[1] = devAlloc<uint32_t>(r, n);
+    P.count = devAlloc<uint32_t>(r, 4);
+    r->capacity = n;
+}
+
+dim3 gridFor(uint64_t n) { return dim3{unsigned((n + kBlock - 1) / kBlock)}; }
+
+enum { kGenerate = 0, kIntersect = 1, kShade = 2, kShadow = 3, kSplat = 4 };
+
+// Brackets a launch with events when timing is on; the elapsed time is collected after the next
+// stream synchronisation (collectTimes).
+template <class F>
+void timed(owe_hip_renderer* r, int which, F&& launch) {
+    if (r->timing) OWE_HIP_CHECK(hipEventRecord(r->ev[0][which], r->stream));
+    launch();
+    if (r->timing) {
+        OWE_HIP_CHECK(hipEventRecord(r->ev[1][which], r->stream));
+        r->kernelLaunches[which]++;
+    }
+}
+void collectTimes(owe_hip_renderer* r, std::initializer_list<int> which) {
+    if (!r->timing) return;
+    for (int k : which) {
+        float ms = 0;
+        OWE_HIP_CHECK(hipEventElapsedTime(&ms, r->ev[0][k], r->ev[1][k]));
+        r->kernelMs[k] += ms;
+    }
+}
+
+// One segment of every live path; the shade kernel's PCG increment is the pass's ((pass << 1) | 1).
+template <int kCap>
+void launchBounce(owe_hip_renderer* r, int q, uint32_t count, uint64_t passInc) {
+    timed(r, kIntersect, [&] {
+        hipLaunchKernelGGL((intersectKernel<kCap>), gridFor(count), dim3{kBlock}, 0, r->stream, r->dev, r->P, q, count);
+    });
+    timed(r, kShade, [&] {
+        hipLaunchKernelGGL(shadeKernel, gridFor(count), dim3{kBlock}, 0, r->stream, r->dev, r->set, r->P, q, count,
+                           passInc);
+    });
+    timed(r, kShadow, [&] {
+        hipLaunchKernelGGL((shadowKernel<kCap>), gridFor(count), dim3{kBlock}, 0, r->stream, r->dev, r->set, r->P, q,
+                           count, int(r->simpleShadows));
+    });
+}
+
+// Runs count paths: slots [0, count) with global path ids base + slot, or the (pixel, sample) lists.
+void runPaths(owe_hip_renderer* r, uint64_t base, uint32_t count, uint32_t spp, uint32_t pass, bool lists,
+              double* perPath) {
+    const uint64_t passInc = (uint64_t(pass) << 1) | 1u;
+    hipLaunchKernelGGL(clearCounts, dim3{1}, dim3{64}, 0, r->stream, r->P.count, 4);
+    timed(r, kGenerate, [&] {
+        hipLaunchKernelGGL(generateKernel, gridFor(count), dim3{kBlock}, 0, r->stream, r->dev, r->set, r->P, base, count,
+                           spp, pass, lists ? r->pathPixels : nullptr, lists ? r->pathSamples : nullptr);
+    });
+    OWE_HIP_CHECK(hipGetLastError());
+    bool generated = true;
+    r->stats.paths += count;
+    uint32_t live = count;
+    int q = 0;
+    uint32_t hostCounts[4] = {0, 0, 0, 0};
+    // Every round raises each live path's depth by one and no path outlives depth 4·maxDepth (null
+    // crossings); the guard only bounds the loop should that ever fail.
+    const int maxRounds = 4 * r->set.maxDepth + 4;
+    for (int round = 0; live > 0 && round <= maxRounds; ++round) {
+        r->stats.segments += live;
+        if (r->stackCap <= 16) launchBounce<16>(r, q, live, passInc);
+        else if (r->stackCap <= 32) launchBounce<32>(r, q, live, passInc);
+        else launchBounce<48>(r, q, live, passInc);
+        OWE_HIP_CHECK(hipGetLastError());
+        OWE_HIP_CHECK(hipMemcpyAsync(hostCounts, r->P.count, sizeof hostCounts, hipMemcpyDeviceToHost, r->stream));
+        OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
+        if (generated) collectTimes(r, {kGenerate});
+        generated = false;
+        collectTimes(r, {kIntersect, kShade, kShadow});
+        live = hostCounts[1 - q];
+        // Reset the consumed queue's counter for the next bounce (a kernel, not a memset).
+        hipLaunchKernelGGL(clearCounts, dim3{1}, dim3{64}, 0, r->stream, r->P.count + q, 1);
+        q = 1 - q;
+    }
+    r->stats.inconsistencies += hostCounts[2];
+    r->stats.leaks += hostCounts[3];
+    timed(r, kSplat, [&] {
+        hipLaunchKernelGGL(splatKernel, gridFor(count), dim3{kBlock}, 0, r->stream, r->set, r->P, count, r->invNWY, r->film,
+                           perPath);
+    });
+    OWE_HIP_CHECK(hipGetLastError());
+    if (r->timing) {
+        OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
+        if (generated) collectTimes(r, {kGenerate});
+        collectTimes(r, {kSplat});
+    }
+}
+
+}  // namespace
+}  // namespace owehip
+
+using namespace owehip;
+
+extern "C" {
+
+int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render_settings* settings,
+                            owe_hip_renderer** out) {
+    owe_hip_renderer* r = nullptr;
+    const int st = guard([&] {
+        require(out != nullptr && scene != nullptr && settings != nullptr, "null argument");
+        *out = nullptr;
+        const owe_render_settings& s = *settings;
+        require(s.max_depth >= 0 && s.rr_depth >= 0, "depths must be non-negative");
+        require(s.max_depth <= 4096, "max depth above 4096 is not supported (depth is kept in 16 bits)");
+        if (s.integrator != OWE_INTEGRATOR_PATH)
+            fail(OWE_HIP_ERR_UNSUPPORTED, "unknown integrator (this backend implements path)");
+        require(s.spectral == OWE_SPECTRAL_HERO4 || s.spectral == OWE_SPECTRAL_UNIFORM1, "unknown spectral mode");
+        if (s.spectral == OWE_SPECTRAL_UNIFORM1)
+            require(s.lambda_min >= 360.0 && s.lambda_max <= 830.0 && s.lambda_min < s.lambda_max,
+                    "the uniform wavelength range must lie inside [360, 830] nm");
+        require(s.fresnel_floor >= 0 && s.fresnel_floor < 0.5, "fresnel floor must lie in [0, 0.5)");
+        selectDevice(device);
+        r = new owe_hip_renderer;
+        r->device = device;
+        r->host = flattenScene(*scene);
+        const HostScene& h = r->host;
+        if (h.bvh.stackBound > 48)
+            fail(OWE_HIP_ERR_UNSUPPORTED, "scene hierarchy needs a traversal stack of " + std::to_string(h.bvh.stackBound) +
+                                              " entries; the kernels hold at most 48");
+        r->stackCap = std::max<uint32_t>(1, h.bvh.stackBound);
+        OWE_HIP_CHECK(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
+        DScene& d = r->dev;
+        d.media = upload(r, h.media);
+        d.regionMedium = upload(r, h.regionMedium);
+        d.optics = upload(r, h.optics);
+        d.emissions = upload(r, h.emissions);
+        d.boundaries = upload(r, h.boundaries);
+        d.lights = upload(r, h.lights);
+        d.table = upload(r, h.table);
+        if (h.hasTriangles) {
+            d.nodes = upload(r, h.bvh.nodes);
+            d.tris = upload(r, h.bvh.tris);
+        }
+        d.nBoundaries = uint32_t(h.boundaries.size());
+        d.nLights = uint32_t(h.lights.size());
+        d.nRegions = uint32_t(h.regionMedium.size());
+        d.stackCap = r->stackCap;
+        d.env = h.env;
+        d.obs = h.obs;
+        bool anyNull = false, anyMedium = false;
+        for (const DBoundary& b : h.boundaries) anyNull = anyNull || h.optics[size_t(b.optics)].type == OWE_SURF_NULL;
+        for (const DMedium& m : h.media) anyMedium = anyMedium || m.flags != 0;
+        r->simpleShadows = !anyNull && !anyMedium;
+        r->set.seed = s.seed;
+        r->set.maxDepth = s.max_depth;
+        r->set.rrDepth = s.rr_depth;
+        r->set.spectral = s.spectral;
+        r->set.lambdaMin = float(s.lambda_min);
+        r->set.lambdaMax = float(s.lambda_max);
+        r->set.fresnelFloor = float(s.fresnel_floor);
+        // 1 / (NW · ∫ȳ) with ∫ȳ by composite Simpson over [360, 830] nm at 0.5 nm (core/spectrum.cpp:11-23).
+        {
+            auto cy = [](double l) {
+                auto lobe = [](double x, double mu, double s1, double s2) {
+                    const double t = (x - mu) / (x < mu ? s1 : s2);
+                    return std::exp(-0.5 * t * t);
+                };
+                return 0.821 * lobe(l, 568.8, 46.9, 40.5) + 0.286 * lobe(l, 530.9, 16.3, 31.1);
+            };
+            const int n = 940;
+            const double hh = 470.0 / n;
+            double sum = cy(360.0) + cy(830.0);
+            for (int i = 1; i < n; ++i) sum += cy(360.0 + i * hh) * ((i & 1) ? 4 : 2);
+            r->invNWY = float(1.0 / (NW * (sum * hh / 3)));
+        }
+        const uint64_t pixels = uint64_t(h.obs.width) * uint64_t(h.obs.height);
+        r->film = devAlloc<unsigned long long>(r, 3 * pixels);
+        OWE_HIP_CHECK(hipMemset(r->film, 0, 3 * pixels * sizeof(unsigned long long)));
+        allocPaths(r, uint32_t(std::min<uint64_t>(pixels * 64, 1u << 22)));
+        *out = r;
+    });
+    if (st != OWE_HIP_OK && r) {
+        for (void* p : r->allocations) (void)hipFree(p);
+        if (r->stream) (void)hipStreamDestroy(r->stream);
+        delete r;
+    }
+    return st;
+}
+
+int owe_hip_renderer_destroy(owe_hip_renderer* r) {
+    return guard([&] {
+        if (!r) return;
+        (void)hipSetDevice(r->device);
+        if (r->stream) (void)hipStreamSynchronize(r->stream);
+        for (void* p : r->allocations) OWE_HIP_CHECK(hipFree(p));
+        for (auto& row : r->ev)
+            for (hipEvent_t& e : row)
+                if (e) OWE_HIP_CHECK(hipEventDestroy(e));
+        if (r->stream) OWE_HIP_CHECK(hipStreamDestroy(r->stream));
+        delete r;
+    });
+}
+
+int owe_hip_renderer_set_timing(owe_hip_renderer* r, int enable) {
+    return guard([&] {
+        require(r != nullptr, "renderer is null");
+        OWE_HIP_CHECK(hipSetDevice(r->device));
+        if (enable && !r->ev[0][0])
+            for (auto& row : r->ev)
+                for (hipEvent_t& e : row) OWE_HIP_CHECK(hipEventCreate(&e));
+        r->timing = enable != 0;
+        for (int k = 0; k < 5; ++k) {
+            r->kernelMs[k] = 0;
+            r->kernelLaunches[k] = 0;
+        }
+    });
+}
+
+int owe_hip_renderer_kernel_times(const owe_hip_renderer* r, double* ms, uint64_t* launches) {
+    return guard([&] {
+        require(r != nullptr && ms != nullptr && launches != nullptr, "null argument");
+        for (int k = 0; k < 5; ++k) {
+            ms[k] = r->kernelMs[k];
+            launches[k] = r->kernelLaunches[k];
+        }
+    });
+}
+
+int owe_hip_renderer_pass(owe_hip_renderer* r, int spp) {
+    return guard([&] {
+        require(r != nullptr, "renderer is null");
+        if (spp <= 0) return;
+        OWE_HIP_CHECK(hipSetDevice(r->device));
+        const uint64_t total = uint64_t(r->host.obs.width) * uint64_t(r->host.obs.height) * uint64_t(spp);
+        for (uint64_t base = 0; base < total; base += r->capacity) {
+            const uint32_t count = uint32_t(std::min<uint64_t>(r->capacity, total - base));
+            runPaths(r, base, count, uint32_t(spp), uint32_t(r->passes), false, nullptr);
+        }
+        OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
+        r->passes++;
+        r->totalSpp += spp;
+    });
+}
+
+int owe_hip_renderer_resolve(owe_hip_renderer* r, double* xyz) {
+    return guard([&] {
+        require(r != nullptr && xyz != nullptr, "null argument");
+        OWE_HIP_CHECK(hipSetDevice(r->device));
+        const size_t n = 3 * size_t(r->host.obs.width) * size_t(r->host.obs.height);
+        std::vector<long long> film(n);
+        OWE_HIP_CHECK(hipMemcpyAsync(film.data(), r->film, n * sizeof(long long), hipMemcpyDeviceToHost, r->stream));
+        OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
+        const double norm = r->totalSpp > 0 ? 1.0 / (kFilmScale * double(r->totalSpp)) : 0.0;
+        for (size_t i = 0; i < n; ++i) xyz[i] = double(film[i]) * norm;
+    });
+}
+
+int owe_hip_renderer_stats(const owe_hip_renderer* r, owe_transport_stats* out) {
+    return guard([&] {
+        require(r != nullptr && out != nullptr, "null argument");
+        *out = r->stats;
+    });
+}
+
+int owe_hip_renderer_progress(const owe_hip_renderer* r, int* passes, long long* samples_per_pixel) {
+    return guard([&] {
+        require(r != nullptr, "renderer is null");
+        if (passes) *passes = r->passes;
+        if (samples_per_pixel) *samples_per_pixel = r->totalSpp;
+    });
+}
+
+int owe_hip_renderer_reset(owe_hip_renderer* r) {
+    return guard([&] {
+        require(r != nullptr, "renderer is null");
+        OWE_HIP_CHECK(hipSetDevice(r->device));
+        const size_t n = 3 * size_t(r->host.obs.width) * size_t(r->host.obs.height);
+        OWE_HIP_CHECK(hipMemsetAsync(r->film, 0, n * sizeof(unsigned long long), r->stream));
+        OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
+        r->passes = 0;
+        r->totalSpp = 0;
+        r->stats = {};
+    });
+}
+
+int owe_hip_renderer_camera_paths(owe_hip_renderer* r, int spp, int pass, const uint32_t* pixel, const uint32_t* sample,
+                                  uint64_t n, double* xyz) {
+    return guard([&] {
+        require(r != nullptr, "renderer is null");
+        require(spp > 0 && pass >= 0, "spp must be positive and pass non-negative");
+        if (n == 0) return;
+        require(pixel != nullptr && sample != nullptr && xyz != nullptr, "null buffer");
+        const uint64_t pixels = uint64_t(r->host.obs.width) * uint64_t(r->host.obs.height);
+        for (uint64_t i = 0; i < n; ++i)
+            require(pixel[i] < pixels && sample[i] < uint32_t(spp), "pixel or sample index out of range");
+        OWE_HIP_CHECK(hipSetDevice(r->device));
+        if (r->listCapacity < r->capacity) {
+            r->pathPixels = devAlloc<uint32_t>(r, r->capacity);
+            r->pathSamples = devAlloc<uint32_t>(r, r->capacity);
+            r->perPath = devAlloc<double>(r, 3 * size_t(r->capacity));
+            r->listCapacity = r->capacity;
+        }
+        for (uint64_t base = 0; base < n; base += r->capacity) {
+            const uint32_t count = uint32_t(std::min<uint64_t>(r->capacity, n - base));
+            OWE_HIP_CHECK(hipMemcpyAsync(r->pathPixels, pixel + base, count * sizeof(uint32_t), hipMemcpyHostToDevice, r->stream));
+            OWE_HIP_CHECK(hipMemcpyAsync(r->pathSamples, sample + base, count * sizeof(uint32_t), hipMemcpyHostToDevice, r->stream));
+            runPaths(r, 0, count, uint32_t(spp), uint32_t(pass), true, r->perPath);
+            OWE_HIP_CHECK(hipMemcpyAsync(xyz + 3 * base, r->perPath, 3 * size_t(count) * sizeof(double), hipMemcpyDeviceToHost,
+                                         r->stream));
+            OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,118 @@
+// The device-resident scene of the transport kernels: the flattened owe_scene rebased to the
+// observer (camera-relative) and rounded to float32, as the reference's float kernels hold it
+// (src/owe/backends/gpu/gpu_scene.hpp:1-10: "camera-relative float records").
+//
+// Layout in HBM: small tables (media, optics, emission, analytic boundaries, lights, spectra) live in
+// one read-only block of plain structs; every triangle of every mesh boundary is transformed to
+// world space once on the host and goes into one 4-wide BVH (bvh4.hpp) whose records carry the
+// owning boundary and the coplanar face id. Analytic shapes (planes, spheres) are few and are tested
+// in a loop after the triangle traversal.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "bvh4.hpp"
+#include "owe_hip_scene.h"
+
+namespace owehip {
+
+constexpr int NW = 4;
+constexpr uint32_t kRegionNone = 0xFFFFFFFEu;
+constexpr uint32_t kNoBoundary = 0xFFFFFFFEu;
+
+struct DSpec {
+    int32_t kind, count, offset, pad;  // tabulated: (λ, v) pairs at table[offset ..]
+    float value, kelvin, scale, pad2;
+};
+
+struct DMedium {
+    int32_t indexKind, terms, relAir, flags;  // flags: 1 absorbs, 2 scatters
+    float a[4], b[4];
+    DSpec absorption, scattering;
+    float g, pad[3];
+};
+
+struct DOptics {
+    int32_t type, backAbsorbs, delta, pad;
+    float roughness, pad2[3];
+    DSpec reflectance, condN, condK;
+};
+
+struct DEmission {
+    DSpec radiance;
+    int32_t front, back, pad[2];
+};
+
+struct DBoundary {
+    int32_t shape, aperture;
+    uint32_t front, back;
+    int32_t optics, emission;
+    float a, b;
+    float toLocal[12];  // world (camera-relative) → shape frame, row-major [R | t]
+    float toWorld[12];
+    float lo[3], area;
+    float hi[3];
+    int32_t light;  // index into the light table, -1 when next-event estimation does not sample it
+};
+
+struct DLight {
+    int32_t kind;  // 0 boundary, 1 sun
+    uint32_t boundary;
+    float pdf, cdf;
+};
+
+struct DEnv {
+    int32_t hasSun, sky, nLights, pad;
+    float sunDir[3], sunOneMinusCos;  // 1 − cos(angular radius), computed in double
+    float sunSolidAngle, sunPdf, pad2[2];
+    DSpec sunRadiance, skyRadiance;
+};
+
+struct DObserver {
+    int32_t width, height;
+    uint32_t region;
+    int32_t pad;
+    float position[3], tanX;  // position is the origin of the camera-relative frame: (0, 0, 0)
+    float fwd[3], tanY;
+    float right[3], pupil;
+    float up[3], focus;
+    float sigma, pad2[3];
+};
+
+// What a kernel needs to read the scene: device pointers and counts.
+struct DScene {
+    const DMedium* media;
+    const uint32_t* regionMedium;
+    const DOptics* optics;
+    const DEmission* emissions;
+    const DBoundary* boundaries;
+    const DLight* lights;
+    const float* table;
+    const WideNode* nodes;  // null when the scene has no triangles
+    const TriRecord* tris;
+    uint32_t nBoundaries, nLights, nRegions, stackCap;
+    DEnv env;
+    DObserver obs;
+};
+
+// Host side: the flattened, rebased, float32 scene and its upload.
+struct HostScene {
+    std::vector<DMedium> media;
+    std::vector<uint32_t> regionMedium;
+    std::vector<DOptics> optics;
+    std::vector<DEmission> emissions;
+    std::vector<DBoundary> boundaries;
+    std::vector<DLight> lights;
+    std::vector<float> table;
+    Bvh4 bvh;
+    bool hasTriangles = false;
+    DEnv env{};
+    DObserver obs{};
+    double origin[3] = {0, 0, 0};  // world position of the camera-relative frame's origin
+};
+
+// Validates and flattens an owe_scene (throws owehip::Failure on invalid input).
+HostScene flattenScene(const owe_scene& s);
+
+}  // namespace owehip

@@ -83,6 +83,16 @@ def load() -> ctypes.CDLL:
     _declare(lib, "owe_hip_mesh_destroy", c_int, [c_vp])
     _declare(lib, "owe_hip_mesh_stats_get", c_int, [c_vp, ctypes.POINTER(MeshStats)])
     _declare(lib, "owe_hip_trace_batch", c_int, [c_vp, c_int, c_vp, c_u64, c_vp, c_vp, c_vp])
+    _declare(lib, "owe_hip_renderer_create", c_int, [c_int, c_vp, c_vp, ctypes.POINTER(c_vp)])
+    _declare(lib, "owe_hip_renderer_destroy", c_int, [c_vp])
+    _declare(lib, "owe_hip_renderer_pass", c_int, [c_vp, c_int])
+    _declare(lib, "owe_hip_renderer_resolve", c_int, [c_vp, c_vp])
+    _declare(lib, "owe_hip_renderer_stats", c_int, [c_vp, c_vp])
+    _declare(lib, "owe_hip_renderer_progress", c_int, [c_vp, ctypes.POINTER(c_int), ctypes.POINTER(ctypes.c_longlong)])
+    _declare(lib, "owe_hip_renderer_reset", c_int, [c_vp])
+    _declare(lib, "owe_hip_renderer_camera_paths", c_int, [c_vp, c_int, c_int, c_vp, c_vp, c_u64, c_vp])
+    _declare(lib, "owe_hip_renderer_set_timing", c_int, [c_vp, c_int])
+    _declare(lib, "owe_hip_renderer_kernel_times", c_int, [c_vp, c_vp, c_vp])
     _lib = lib
     return lib
 

@@ -1,0 +1,78 @@
+"""The camera-path renderer through the C ABI (owe_hip_renderer_*).
+
+The Python face of the reference's Renderer (src/owe/render/renderer.hpp:19-79) for the hip
+backend: ``run_pass(spp)`` adds a pass, ``resolve()`` returns the XYZ image [H, W, 3] in double,
+``stats()`` the TransportStats. ``camera_paths`` returns single paths' XYZ for per-path parity.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .scene import CRenderSettings, CTransportStats, Scene, settings as make_settings
+
+
+class Renderer:
+    def __init__(self, scene: Scene, settings: CRenderSettings | None = None, device: int = 0):
+        self.scene = scene
+        self.settings = settings if settings is not None else make_settings()
+        c = scene.c()
+        h = ctypes.c_void_p()
+        _lib.check(_lib.load().owe_hip_renderer_create(device, ctypes.byref(c), ctypes.byref(self.settings),
+                                                       ctypes.byref(h)))
+        self._h = h
+        self.width, self.height = scene.observer["width"], scene.observer["height"]
+
+    def close(self) -> None:
+        if self._h:
+            _lib.check(_lib.load().owe_hip_renderer_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run_pass(self, spp: int) -> None:
+        _lib.check(_lib.load().owe_hip_renderer_pass(self._h, int(spp)))
+
+    def resolve(self) -> np.ndarray:
+        out = np.zeros((self.height, self.width, 3), dtype=np.float64)
+        _lib.check(_lib.load().owe_hip_renderer_resolve(self._h, out.ctypes.data))
+        return out
+
+    def stats(self) -> dict:
+        s = CTransportStats()
+        _lib.check(_lib.load().owe_hip_renderer_stats(self._h, ctypes.byref(s)))
+        return {"paths": s.paths, "segments": s.segments, "inconsistencies": s.inconsistencies, "leaks": s.leaks}
+
+    def progress(self) -> tuple[int, int]:
+        p, n = ctypes.c_int(), ctypes.c_longlong()
+        _lib.check(_lib.load().owe_hip_renderer_progress(self._h, ctypes.byref(p), ctypes.byref(n)))
+        return p.value, n.value
+
+    def reset(self) -> None:
+        _lib.check(_lib.load().owe_hip_renderer_reset(self._h))
+
+    KERNELS = ("generate", "intersect", "shade", "shadow", "splat")
+
+    def set_timing(self, enable: bool = True) -> None:
+        _lib.check(_lib.load().owe_hip_renderer_set_timing(self._h, int(enable)))
+
+    def kernel_times(self) -> dict:
+        """{kernel: (milliseconds, launches)} since timing was enabled (HIP events on the render stream)."""
+        ms = np.zeros(5)
+        n = np.zeros(5, dtype=np.uint64)
+        _lib.check(_lib.load().owe_hip_renderer_kernel_times(self._h, ms.ctypes.data, n.ctypes.data))
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(self.KERNELS)}
+
+    def camera_paths(self, spp: int, pixel, sample, pass_index: int = 0) -> np.ndarray:
+        pixel = np.ascontiguousarray(pixel, dtype=np.uint32)
+        sample = np.ascontiguousarray(sample, dtype=np.uint32)
+        out = np.zeros((len(pixel), 3), dtype=np.float64)
+        _lib.check(_lib.load().owe_hip_renderer_camera_paths(self._h, int(spp), int(pass_index), pixel.ctypes.data,
+                                                             sample.ctypes.data, len(pixel), out.ctypes.data))
+        return out

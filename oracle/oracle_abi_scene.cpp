@@ -1,0 +1,145 @@
+// TEST INFRASTRUCTURE ONLY — the double-precision CPU oracle (see oracle_scene.hpp).
+// C entry points over the world and the camera-path tracer: whole passes into an XYZ film
+// (CpuRenderer::pass, cpu_backend.cpp:52-94, in 16×16 tiles on host threads) and single paths by
+// (pixel, sample) for per-path comparisons with the hip backend.
+#include <atomic>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "oracle_scene.hpp"
+
+using namespace oracle;
+
+namespace {
+Settings toSettings(const owe_render_settings& r) {
+    Settings s;
+    s.seed = r.seed;
+    s.maxDepth = r.max_depth;
+    s.rrDepth = r.rr_depth;
+    s.spectral = r.spectral;
+    s.lambdaMin = r.lambda_min;
+    s.lambdaMax = r.lambda_max;
+    s.fresnelFloor = r.fresnel_floor;
+    return s;
+}
+// The stream of sample s of a pixel in the per-sample layout.
+Pcg sampleStream(uint64_t seed, uint64_t pixel, uint64_t s, uint64_t pass) {
+    return Pcg(hashPair(hashPair(seed, pixel), s), pass);
+}
+}  // namespace
+
+extern "C" {
+
+void* oracle_world_create(const owe_scene* s) { return new World(*s); }
+void oracle_world_destroy(void* w) { delete static_cast<World*>(w); }
+
+// Light-selection pdfs in table order (boundaries, then the sun); returns the count.
+int oracle_light_table(const void* wh, double* pdf, uint32_t* boundary, int cap) {
+    const World& w = *static_cast<const World*>(wh);
+    for (size_t i = 0; i < w.lights.size() && int(i) < cap; ++i) {
+        pdf[i] = w.lightPdf[i];
+        boundary[i] = w.lights[i].boundary;
+    }
+    return int(w.lights.size());
+}
+
+// Adds one pass of spp samples per pixel into xyz (W·H·3) and samples (W·H); stats += paths,
+// segments, inconsistencies, leaks. layout: 0 one stream per pixel (the reference's), 1 one per sample.
+void oracle_render(const void* wh, const owe_render_settings* rs, int spp, int pass, int layout, int threads,
+                   double* xyz, double* samples, uint64_t* stats) {
+    const World& w = *static_cast<const World*>(wh);
+    const Settings set = toSettings(*rs);
+    const Tracer tr(w, set);
+    const int W = w.observer.width, H = w.observer.height, tile = 16;
+    const int tilesX = (W + tile - 1) / tile, tilesY = (H + tile - 1) / tile;
+    if (threads <= 0) threads = int(std::max(1u, std::thread::hardware_concurrency()));
+    std::atomic<int> next{0};
+    std::vector<Stats> st{size_t(threads)};
+    auto worker = [&](int tid) {
+        for (;;) {
+            const int t = next.fetch_add(1);
+            if (t >= tilesX * tilesY) break;
+            const int tx = t % tilesX, ty = t / tilesX;
+            for (int y = ty * tile; y < std::min(H, (ty + 1) * tile); ++y)
+                for (int x = tx * tile; x < std::min(W, (tx + 1) * tile); ++x) {
+                    const uint64_t pixel = uint64_t(y) * W + x;
+                    Pcg rng(hashPair(set.seed, pixel), uint64_t(pass));
+                    const double rot = rng.uniform();
+                    double acc[3] = {0, 0, 0};
+                    for (int s = 0; s < spp; ++s) {
+                        double c[3];
+                        bool ok;
+                        if (layout == kLayoutPixel) {
+                            ok = tr.cameraSample(x, y, s, spp, rot, rng, st[size_t(tid)], c);
+                        } else {
+                            Pcg own = sampleStream(set.seed, pixel, uint64_t(s), uint64_t(pass));
+                            ok = tr.cameraSample(x, y, s, spp, rot, own, st[size_t(tid)], c);
+                        }
+                        if (!ok) continue;
+                        for (int k = 0; k < 3; ++k) acc[k] += c[k];
+                    }
+                    for (int k = 0; k < 3; ++k) xyz[3 * pixel + k] += acc[k];
+                    samples[pixel] += spp;
+                }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int k = 0; k < threads; ++k) pool.emplace_back(worker, k);
+    for (auto& th : pool) th.join();
+    Stats total;
+    for (const Stats& s : st) total.add(s);
+    stats[0] += total.paths;
+    stats[1] += total.segments;
+    stats[2] += total.inconsistencies;
+    stats[3] += total.leaks;
+}
+
+// Per-path XYZ of n camera samples (pixel[i], sample[i]) in the per-sample stream layout; segments
+// per path in segs.
+void oracle_camera_paths(const void* wh, const owe_render_settings* rs, int spp, int pass, uint64_t n,
+                         const uint32_t* pixel, const uint32_t* sample, double* xyz, uint32_t* segs, int threads) {
+    const World& w = *static_cast<const World*>(wh);
+    const Settings set = toSettings(*rs);
+    const Tracer tr(w, set);
+    const int W = w.observer.width;
+    if (threads <= 0) threads = int(std::max(1u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (int k = 0; k < threads; ++k)
+        pool.emplace_back([&, k] {
+            for (uint64_t i = n * uint64_t(k) / uint64_t(threads); i < n * uint64_t(k + 1) / uint64_t(threads); ++i) {
+                const uint64_t px = pixel[i];
+                Pcg pr(hashPair(set.seed, px), uint64_t(pass));
+                const double rot = pr.uniform();
+                Pcg own = sampleStream(set.seed, px, sample[i], uint64_t(pass));
+                Stats st;
+                double c[3];
+                if (!tr.cameraSample(int(px % uint64_t(W)), int(px / uint64_t(W)), int(sample[i]), spp, rot, own, st, c))
+                    c[0] = c[1] = c[2] = 0;
+                for (int q = 0; q < 3; ++q) xyz[3 * i + q] = c[q];
+                segs[i] = uint32_t(st.segments);
+            }
+        });
+    for (auto& th : pool) th.join();
+}
+
+double oracle_spectrum_luminance_blackbody(double kelvin) {
+    Spectrum s;
+    s.kind = OWE_SPEC_BLACKBODY;
+    s.kelvin = kelvin;
+    return s.luminance();
+}
+
+double oracle_medium_index(const owe_medium* m, double lnm) {
+    IndexModel x;
+    x.kind = m->index_kind;
+    x.terms = m->terms;
+    x.relAir = m->rel_air != 0;
+    for (int k = 0; k < 4; ++k) {
+        x.a[k] = m->coeff_a[k];
+        x.b[k] = m->coeff_b[k];
+    }
+    return x.n(lnm);
+}
+
+}  // extern "C"

@@ -1,0 +1,258 @@
+// TEST INFRASTRUCTURE ONLY — the double-precision CPU oracle (see oracle_core.hpp).
+//
+// The world, lights, observer and camera-path transport restated from the reference:
+//   spectra, blackbody normalisation, luminance     src/owe/core/spectrum.cpp:30-181
+//   n(λ) models, relative-to-air indices            src/owe/core/medium.cpp:46-66
+//   plane / sphere / mesh shapes, sampling          src/owe/scene/geometry.cpp:18-27, 391-477; mesh.cpp:12-113
+//   light table (by power, sun share)               src/owe/scene/world.cpp:285-377
+//   ideal observer (pinhole / pupil, pixel filter)  src/owe/scene/detector.cpp:36-91, detector.hpp:47-62
+//   camera path: radiance, NEE/MIS, media, RR       src/owe/transport/transport.cpp:35-529
+//   surface scattering                               src/owe/transport/scattering.cpp:65-222
+//   per-pixel passes and XYZ film                   src/owe/backends/cpu/cpu_backend.cpp:52-107
+//
+// World::intersect here is the nearest crossing over all boundaries, without the reference's
+// contact and tie machinery (world.cpp:440-550): scenes keep bodies apart. A spawned ray excludes
+// only what it provably cannot cross again: its own plane, its own mesh face (coplanar adjacent
+// triangles), or — on a sphere — the root at its origin (the other root is −2 o·d / d·d exactly).
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../include/owe_hip_scene.h"
+#include "oracle_bvh.hpp"
+
+namespace oracle {
+
+constexpr int NW = 4;
+constexpr uint32_t kNone = 0xFFFFFFFEu;
+
+struct Spec4 {
+    double v[NW] = {0, 0, 0, 0};
+    Spec4() = default;
+    explicit Spec4(double x) {
+        for (double& a : v) a = x;
+    }
+    double& operator[](int i) { return v[i]; }
+    double operator[](int i) const { return v[i]; }
+    Spec4& operator*=(const Spec4& o) {
+        for (int i = 0; i < NW; ++i) v[i] *= o.v[i];
+        return *this;
+    }
+    Spec4& operator*=(double s) {
+        for (double& a : v) a *= s;
+        return *this;
+    }
+    Spec4& operator+=(const Spec4& o) {
+        for (int i = 0; i < NW; ++i) v[i] += o.v[i];
+        return *this;
+    }
+    double maxValue() const {
+        double m = v[0];
+        for (double a : v) m = std::max(m, a);
+        return m;
+    }
+    bool isZero() const {
+        for (double a : v)
+            if (a != 0) return false;
+        return true;
+    }
+};
+inline Spec4 operator*(Spec4 a, const Spec4& b) { return a *= b; }
+inline Spec4 operator*(Spec4 a, double s) { return a *= s; }
+inline Spec4 operator+(Spec4 a, const Spec4& b) { return a += b; }
+
+struct Spectrum {
+    int kind = OWE_SPEC_CONSTANT;
+    double value = 0, kelvin = 0, scale = 1;
+    std::vector<double> lam, val;
+    double eval(double l) const;
+    bool isZero() const { return kind == OWE_SPEC_CONSTANT && (value == 0 || scale == 0); }
+    double luminance() const;  // ∫ s ȳ / ∫ ȳ
+};
+
+struct Wavelengths {
+    double lambda[NW] = {0, 0, 0, 0};
+    double pdf[NW] = {0, 0, 0, 0};
+    static Wavelengths hero(double u);
+    static Wavelengths uniform1(double u, double lo, double hi);
+    bool secondaryTerminated() const { return pdf[1] == 0 && pdf[2] == 0 && pdf[3] == 0; }
+    void terminateSecondary() {
+        if (secondaryTerminated()) return;
+        for (int i = 1; i < NW; ++i) pdf[i] = 0;
+        pdf[0] /= NW;
+    }
+    Spec4 eval(const Spectrum& s) const {
+        Spec4 r;
+        for (int i = 0; i < NW; ++i) r[i] = s.eval(lambda[i]);
+        return r;
+    }
+    void toXYZ(const Spec4& L, double xyz[3]) const;
+};
+
+struct IndexModel {
+    int kind = OWE_INDEX_CONSTANT, terms = 0;
+    bool relAir = false;
+    double a[4] = {1, 0, 0, 0}, b[4] = {0, 0, 0, 0};
+    double n(double lnm) const;
+};
+
+struct Medium {
+    IndexModel index;
+    Spectrum absorption, scattering;
+    double g = 0;
+    bool opaque = false;
+    bool scatters() const { return !scattering.isZero(); }
+    bool absorbs() const { return !absorption.isZero(); }
+};
+
+struct Optics {
+    int type = OWE_SURF_DIELECTRIC;
+    bool backAbsorbs = false;
+    double roughness = 0;
+    Spectrum reflectance, condN, condK;
+    bool isDelta() const {
+        return (type == OWE_SURF_DIELECTRIC || type == OWE_SURF_CONDUCTOR) ? roughness < 1e-3 : type == OWE_SURF_MIRROR;
+    }
+};
+
+struct Emission {
+    Spectrum radiance;
+    bool front = true, back = false, nee = true;
+};
+
+struct Xf {
+    double R[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    V3 t;
+    V3 point(V3 p) const { return vector(p) + t; }
+    V3 vector(V3 v) const {
+        return {R[0][0] * v.x + R[0][1] * v.y + R[0][2] * v.z, R[1][0] * v.x + R[1][1] * v.y + R[1][2] * v.z,
+                R[2][0] * v.x + R[2][1] * v.y + R[2][2] * v.z};
+    }
+    Xf inverse() const;
+};
+
+struct Boundary {
+    int shape = OWE_SHAPE_PLANE, aperture = OWE_APERTURE_RECT;
+    uint32_t front = 0, back = 0;
+    int optics = 0, emission = -1;
+    double a = 0, b = 0;
+    Xf toWorld, toLocal;
+    Box worldBox;
+    std::unique_ptr<Mesh> mesh;
+    std::vector<uint32_t> faceGroup;  // mesh: coplanar face id per triangle
+    double area() const;
+    bool canSample() const { return area() > 0; }
+};
+
+struct Ray {
+    V3 o, d;
+    uint32_t departed = kNone;   // boundary the ray leaves
+    uint32_t departedPrim = kNone;
+};
+
+struct Hit {
+    double t = kInf;
+    V3 p, n, pLocal, nLocal;
+    uint32_t boundary = kNone, prim = 0;
+};
+
+struct Environment {
+    bool hasSun = false;
+    int sky = OWE_SKY_NONE;
+    V3 sunDir{0, 0, 1};
+    double sunAngularRadius = 0;
+    Spectrum sunRadiance, skyRadiance;
+    double sunNeeShare = 0;
+    double skyAt(double l) const { return sky == OWE_SKY_UNIFORM ? skyRadiance.eval(l) : 0; }
+    double sunAt(V3 dir, double l) const {
+        return (hasSun && dot(dir, sunDir) >= std::cos(sunAngularRadius)) ? sunRadiance.eval(l) : 0;
+    }
+    double sunSolidAngle() const { return 2 * kPi * (1 - std::cos(sunAngularRadius)); }
+};
+
+struct Light {
+    int kind = 0;  // 0 boundary, 1 sun
+    uint32_t boundary = kNone;
+    double power = 0;
+};
+
+struct Observer {
+    int width = 0, height = 0;
+    uint32_t region = 0;
+    V3 position, fwd, right, up;
+    double tanX = 1, tanY = 1, pupil = 0, focus = kInf, sigma = 0.4;
+    bool generate(double px, double py, Pcg& rng, Ray& ray, double& weight) const;
+};
+
+class World {
+public:
+    explicit World(const owe_scene& s);
+    bool intersect(const Ray& ray, double tmax, Hit& hit) const;
+    Ray spawn(const Hit& h, V3 dir) const { return Ray{h.p, dir, h.boundary, h.prim}; }
+    const Medium& mediumOf(uint32_t region) const { return media[regionMedium[region]]; }
+    double indexOf(uint32_t region, double l) const { return mediumOf(region).index.n(l); }
+    size_t pickLight(double u) const;
+    double sceneRadius() const;
+    // A point on boundary b seen from ref (solid-angle pdf), and the pdf of a given point.
+    bool sampleFrom(const Boundary& b, V3 refLocal, double u1, double u2, double u3, V3& p, V3& n, double& pdfW) const;
+    double pdfFrom(const Boundary& b, V3 refLocal, V3 pLocal, V3 nLocal) const;
+
+    std::vector<Medium> media;
+    std::vector<uint32_t> regionMedium;
+    std::vector<Optics> optics;
+    std::vector<Emission> emissions;
+    std::vector<Boundary> boundaries;
+    Environment env;
+    Observer observer;
+    Box bounds;
+    std::vector<Light> lights;
+    std::vector<double> lightPdf, lightCdf;
+    std::vector<int> lightOfBoundary;
+};
+
+struct Settings {
+    uint64_t seed = 1;
+    int maxDepth = 64, rrDepth = 6;
+    int spectral = OWE_SPECTRAL_HERO4;
+    double lambdaMin = 380, lambdaMax = 780;
+    double fresnelFloor = 0;
+};
+
+struct Stats {
+    uint64_t paths = 0, segments = 0, inconsistencies = 0, leaks = 0;
+    void add(const Stats& o) {
+        paths += o.paths;
+        segments += o.segments;
+        inconsistencies += o.inconsistencies;
+        leaks += o.leaks;
+    }
+};
+
+class Tracer {
+public:
+    Tracer(const World& w, const Settings& s) : w_(w), s_(s) {}
+    // Tracer::radiance (transport.cpp:473-529).
+    Spec4 radiance(Ray ray, uint32_t region, Wavelengths& wl, Pcg& rng, Stats& st) const;
+    Spec4 transmittance(V3 p, const Ray& start, uint32_t region, V3 dir, double dist, const Wavelengths& wl,
+                        uint32_t* finalRegion, uint32_t excludeBoundary) const;
+    // One camera sample of pixel (x, y): the draws of cpu_backend.cpp:69-84 from `rng`.
+    bool cameraSample(int x, int y, int s, int spp, double rot, Pcg& rng, Stats& st, double xyz[3]) const;
+
+private:
+    struct Path;
+    Spec4 directLighting(const Hit& hit, bool surface, const Optics* o, double nFront, double nBack, V3 d,
+                         const Wavelengths& wl, Pcg& rng, double g, uint32_t region) const;
+    bool scatterAtSurface(const Hit& hit, const Optics& o, uint32_t other, int depth, Wavelengths& wl, Pcg& rng,
+                          Path& path) const;
+    const World& w_;
+    Settings s_;
+};
+
+// Stream layouts of a pass: the reference's one stream per (pixel, pass) drawn by every sample in
+// turn, or one stream per (pixel, sample, pass) so samples run independently (the hip backend's).
+constexpr int kLayoutPixel = 0;
+constexpr int kLayoutSample = 1;
+
+}  // namespace oracle

@@ -1,0 +1,474 @@
+// TEST INFRASTRUCTURE ONLY — the double-precision CPU oracle (see oracle_scene.hpp for citations).
+// Camera-path transport restated from src/owe/transport/transport.cpp:35-529 and
+// src/owe/transport/scattering.cpp:65-222, over the simplified World of oracle_scene.cpp.
+#include "oracle_scene.hpp"
+
+namespace oracle {
+
+namespace {
+
+constexpr bool kRadiance = true;
+
+// Hit-local optics state at one wavelength (scattering.hpp:31-44).
+struct Interface {
+    const Optics* o;
+    V3 n, p;
+    double nFront, nBack, lambda, floor;
+};
+
+struct Sample {
+    V3 wi;
+    double weight = 0, pdf = 0;
+    bool delta = false, transmitted = false;
+};
+
+C2 conductorEta(const Optics& o, double lambda, double nMed) {
+    const double inv = 1.0 / nMed;  // a Vec2 divided by a scalar multiplies by its reciprocal
+    return {o.condN.eval(lambda) * inv, o.condK.eval(lambda) * inv};
+}
+
+bool specularReflection(const Frame& f, V3 wo, double weight, Sample& s) {
+    s.wi = f.toWorld(V3(-wo.x, -wo.y, wo.z));
+    s.weight = weight;
+    s.delta = true;
+    return weight > 0;
+}
+
+void fromLobe(const Frame& f, const Lobe& l, Sample& s) {
+    s.wi = f.toWorld(l.wi);
+    s.weight = l.weight;
+    s.pdf = l.pdf;
+    s.delta = false;
+    s.transmitted = l.transmitted;
+}
+
+// sampleScatter (scattering.cpp:65-157): uc picks a branch, (u1, u2) the direction.
+bool sampleScatter(const Interface& it, V3 d, double uc, double u1, double u2, bool radiance, Sample& s) {
+    const Optics& o = *it.o;
+    const Frame frame(it.n);
+    const V3 wo = frame.toLocal(-d);
+    const bool fromFront = wo.z > 0;
+    s = Sample{};
+    switch (o.type) {
+        case OWE_SURF_DIELECTRIC: {
+            const double eta = it.nBack / it.nFront;
+            if (o.roughness >= kSmoothRoughness && eta != 1) {
+                Lobe l;
+                if (!sampleRoughDielectric(ggxAlpha(o.roughness), wo, eta, uc, u1, u2, radiance, l)) return false;
+                fromLobe(frame, l, s);
+                return true;
+            }
+            const double ni = fromFront ? it.nFront : it.nBack, nt = fromFront ? it.nBack : it.nFront;
+            const double R = fresnelDielectric(wo.z, eta), pR = branchProbability(R, it.floor);
+            s.delta = true;
+            if (uc < pR) {
+                specularReflection(frame, wo, R / pR, s);
+                return true;
+            }
+            if (!refractDir(d, it.n, ni, nt, s.wi)) return false;
+            s.weight = (1 - R) / (1 - pR) * (radiance ? sq(ni / nt) : 1.0);
+            s.transmitted = true;
+            return true;
+        }
+        case OWE_SURF_DIFFUSE: {
+            if (!fromFront && o.backAbsorbs) return false;
+            const double albedo = o.reflectance.eval(it.lambda);
+            if (albedo <= 0) return false;
+            V3 w = cosineHemisphere(u1, u2);
+            if (!fromFront) w.z = -w.z;
+            s.wi = frame.toWorld(w);
+            s.pdf = std::fabs(w.z) * kInvPi;
+            s.weight = albedo;
+            return s.pdf > 0;
+        }
+        case OWE_SURF_MIRROR:
+            if (!fromFront && o.backAbsorbs) return false;
+            return specularReflection(frame, wo, o.reflectance.eval(it.lambda), s);
+        case OWE_SURF_CONDUCTOR: {
+            if (!fromFront && o.backAbsorbs) return false;
+            const C2 eta = conductorEta(o, it.lambda, fromFront ? it.nFront : it.nBack);
+            const double tint = o.reflectance.eval(it.lambda);
+            const V3 w = fromFront ? wo : V3(wo.x, wo.y, -wo.z);
+            if (o.roughness < kSmoothRoughness) return specularReflection(frame, wo, fresnelConductor(w.z, eta) * tint, s);
+            Lobe l;
+            if (!sampleRoughConductor(ggxAlpha(o.roughness), w, eta, tint, u1, u2, l)) return false;
+            if (!fromFront) l.wi.z = -l.wi.z;
+            fromLobe(frame, l, s);
+            return true;
+        }
+        default: return false;
+    }
+}
+
+// evalScatter (scattering.cpp:159-188): f without the cosine, and the pdf of sampling wi.
+double evalScatter(const Interface& it, V3 d, V3 wiWorld, bool radiance, double& pdf) {
+    pdf = 0;
+    const Optics& o = *it.o;
+    if (o.isDelta()) return 0;
+    const Frame frame(it.n);
+    V3 wo = frame.toLocal(-d), wi = frame.toLocal(wiWorld);
+    const bool fromFront = wo.z > 0;
+    switch (o.type) {
+        case OWE_SURF_DIFFUSE:
+            if (!fromFront && o.backAbsorbs) return 0;
+            if (!sameHemi(wo, wi)) return 0;
+            pdf = std::fabs(wi.z) * kInvPi;
+            return o.reflectance.eval(it.lambda) * kInvPi;
+        case OWE_SURF_CONDUCTOR: {
+            if (!fromFront && o.backAbsorbs) return 0;
+            if (!sameHemi(wo, wi)) return 0;
+            if (!fromFront) {
+                wo.z = -wo.z;
+                wi.z = -wi.z;
+            }
+            const C2 eta = conductorEta(o, it.lambda, fromFront ? it.nFront : it.nBack);
+            return evalRoughConductor(ggxAlpha(o.roughness), wo, wi, eta, o.reflectance.eval(it.lambda), pdf);
+        }
+        case OWE_SURF_DIELECTRIC:
+            return evalRoughDielectric(ggxAlpha(o.roughness), wo, wi, it.nBack / it.nFront, radiance, pdf);
+        default: return 0;
+    }
+}
+
+bool isDispersive(const Interface& hero, const Interface& other) {
+    const double a = hero.nBack / hero.nFront, b = other.nBack / other.nFront;
+    return std::fabs(a - b) > 1e-12 * std::fabs(a);
+}
+
+double secondaryWeight(const Interface& it, V3 d, const Sample& s, bool radiance) {
+    const Optics& o = *it.o;
+    if (!s.delta) {
+        double pdf;
+        const double f = evalScatter(it, d, s.wi, radiance, pdf);
+        return s.pdf > 0 ? f * std::fabs(dot(s.wi, it.n)) / s.pdf : 0.0;
+    }
+    const double cosI = std::fabs(dot(d, it.n));
+    switch (o.type) {
+        case OWE_SURF_DIELECTRIC: return s.weight;
+        case OWE_SURF_MIRROR: return o.reflectance.eval(it.lambda);
+        case OWE_SURF_CONDUCTOR: {
+            const double nMed = dot(d, it.n) < 0 ? it.nFront : it.nBack;
+            return fresnelConductor(cosI, conductorEta(o, it.lambda, nMed)) * o.reflectance.eval(it.lambda);
+        }
+        default: return 0;
+    }
+}
+
+Spec4 sigmaT(const Medium& m, const Wavelengths& wl) {
+    Spec4 r;
+    for (int i = 0; i < NW; ++i) r[i] = m.absorption.eval(wl.lambda[i]) + m.scattering.eval(wl.lambda[i]);
+    return r;
+}
+
+void maskTerminated(const Wavelengths& wl, Spec4& beta) {
+    for (int i = 1; i < NW; ++i)
+        if (wl.pdf[i] == 0) beta[i] = 0;
+}
+
+enum class MediumEvent { Pass, Scatter, End };
+
+// sampleMedium (transport.cpp:95-113).
+MediumEvent sampleMedium(const Medium& m, const Wavelengths& wl, double seg, Pcg& rng, Spec4& beta, double& t) {
+    if (m.scatters()) {
+        const Spec4 sT = sigmaT(m, wl);
+        t = -std::log(1 - rng.uniform()) / sT[0];
+        if (t < seg) {
+            const double pdf0 = sT[0] * std::exp(-sT[0] * t);
+            for (int i = 0; i < NW; ++i) beta[i] *= m.scattering.eval(wl.lambda[i]) * std::exp(-sT[i] * t) / pdf0;
+            return MediumEvent::Scatter;
+        }
+        if (std::isinf(seg)) return MediumEvent::End;
+        const double pass0 = std::exp(-sT[0] * seg);
+        for (int i = 0; i < NW; ++i) beta[i] *= std::exp(-sT[i] * seg) / pass0;
+    } else if (m.absorbs()) {
+        if (std::isinf(seg)) return MediumEvent::End;
+        for (int i = 0; i < NW; ++i) beta[i] *= std::exp(-m.absorption.eval(wl.lambda[i]) * seg);
+    }
+    return MediumEvent::Pass;
+}
+
+}  // namespace
+
+struct Tracer::Path {
+    Ray ray;
+    uint32_t region;
+    Spec4 L, beta = Spec4(1.0);
+    bool specular = true;
+    double prevPdf = 0;
+    V3 prevP;
+    // Russian roulette against the throughput (transport.cpp:369-374).
+    bool survives(Pcg& rng) {
+        const double q = std::min(1.0, beta.maxValue());
+        if (rng.uniform() >= q) return false;
+        beta *= 1 / q;
+        return true;
+    }
+};
+
+// Tracer::transmittance (transport.cpp:237-285) along dir from a start point: through null
+// boundaries with Beer–Lambert on the way; zero at matter. `exclude` is the destination boundary
+// (an emitter's own surface), which the segment ends on.
+Spec4 Tracer::transmittance(V3 p, const Ray& start, uint32_t region, V3 dir, double dist, const Wavelengths& wl,
+                            uint32_t* finalRegion, uint32_t exclude) const {
+    Spec4 Tr(1.0);
+    Ray ray = start;
+    ray.o = p;
+    ray.d = dir;
+    double remaining = dist;
+    for (int guard = 0; guard < 256; ++guard) {
+        Hit h;
+        bool found = w_.intersect(ray, remaining, h);
+        while (found && h.boundary == exclude) found = false;  // the segment's own end
+        const double seg = found ? std::min(h.t, remaining) : remaining;
+        const Medium& m = w_.mediumOf(region);
+        if (m.absorbs() || m.scatters()) {
+            if (std::isinf(seg)) return Spec4(0.0);
+            const Spec4 sT = sigmaT(m, wl);
+            for (int i = 0; i < NW; ++i) Tr[i] *= std::exp(-sT[i] * seg);
+        }
+        if (!found) {
+            if (std::isinf(dist) && region != 0) return Spec4(0.0);
+            if (finalRegion) *finalRegion = region;
+            return Tr;
+        }
+        const Boundary& b = w_.boundaries[h.boundary];
+        const uint32_t incident = dot(dir, h.n) < 0 ? b.front : b.back;
+        if (incident != region) return Spec4(0.0);
+        if (w_.optics[size_t(b.optics)].type != OWE_SURF_NULL) return Spec4(0.0);
+        region = dot(dir, h.n) < 0 ? b.back : b.front;
+        ray = w_.spawn(h, dir);
+        if (!std::isinf(remaining)) remaining -= h.t;
+    }
+    return Spec4(0.0);
+}
+
+// Tracer::directLighting with sampleLightFrom (transport.cpp:35-70, 323-348).
+Spec4 Tracer::directLighting(const Hit& hit, bool surface, const Optics* o, double nFront, double nBack, V3 d,
+                             const Wavelengths& wl, Pcg& rng, double g, uint32_t region) const {
+    if (w_.lights.empty()) return Spec4(0.0);
+    const size_t li = w_.pickLight(rng.uniform());
+    const double sel = w_.lightPdf[li];
+    const double u1 = rng.uniform(), u2 = rng.uniform(), u3 = rng.uniform();
+    const Light& L = w_.lights[li];
+    V3 wi;
+    double dist = kInf, lpdf = 0;
+    Spec4 Le;
+    uint32_t lightRegion = kNone, lightBoundary = kNone;
+    if (L.kind == 1) {
+        wi = Frame(w_.env.sunDir).toWorld(uniformCone(u1, u2, std::cos(w_.env.sunAngularRadius)));
+        Le = wl.eval(w_.env.sunRadiance);
+        lpdf = sel / w_.env.sunSolidAngle();
+    } else {
+        const Boundary& b = w_.boundaries[L.boundary];
+        V3 pL, nL;
+        double pdfW;
+        if (!w_.sampleFrom(b, b.toLocal.point(hit.p), u1, u2, u3, pL, nL, pdfW)) return Spec4(0.0);
+        const V3 ep = b.toWorld.point(pL), en = normalize(b.toWorld.vector(nL));
+        const V3 dv = ep - hit.p;
+        dist = length(dv);
+        if (!(dist > 0)) return Spec4(0.0);
+        wi = dv / dist;
+        const Emission& e = w_.emissions[size_t(b.emission)];
+        const double cosL = dot(en, -wi);
+        Le = ((cosL > 0 && e.front) || (cosL < 0 && e.back)) ? wl.eval(e.radiance) : Spec4(0.0);
+        lpdf = sel * pdfW;
+        lightRegion = cosL > 0 ? b.front : b.back;
+        lightBoundary = L.boundary;
+    }
+    if (Le.isZero() || lpdf <= 0) return Spec4(0.0);
+    Spec4 f;
+    double pdfScatter = 0, cosFactor = 1;
+    uint32_t startRegion = region;
+    Ray start;
+    if (!surface) {
+        pdfScatter = hgPhase(dot(-d, wi), g);
+        f = Spec4(pdfScatter);
+    } else {
+        const Interface base{o, hit.n, hit.p, nFront, nBack, wl.lambda[0], s_.fresnelFloor};
+        const Boundary& b = w_.boundaries[hit.boundary];
+        for (int i = 0; i < NW; ++i) {
+            if (wl.pdf[i] == 0) continue;
+            Interface it = base;
+            if (i > 0) {
+                it.nFront = w_.indexOf(b.front, wl.lambda[i]);
+                it.nBack = w_.indexOf(b.back, wl.lambda[i]);
+                it.lambda = wl.lambda[i];
+            }
+            double pdf;
+            f[i] = evalScatter(it, d, wi, kRadiance, pdf);
+            if (i == 0) pdfScatter = pdf;
+        }
+        if (f.isZero()) return Spec4(0.0);
+        cosFactor = std::fabs(dot(wi, hit.n));
+        startRegion = dot(wi, hit.n) > 0 ? b.front : b.back;
+        start.departed = hit.boundary;
+        start.departedPrim = hit.prim;
+    }
+    Spec4 Tr;
+    if (std::isfinite(dist)) {
+        uint32_t arrival = kNone;
+        Tr = transmittance(hit.p, start, startRegion, wi, dist, wl, &arrival, lightBoundary);
+        if (arrival != lightRegion) Tr = Spec4(0.0);
+    } else {
+        Tr = transmittance(hit.p, start, startRegion, wi, kInf, wl, nullptr, kNone);
+    }
+    if (Tr.isZero()) return Spec4(0.0);
+    const double w = powerHeuristic(lpdf, pdfScatter);
+    return f * Tr * Le * (cosFactor * w / lpdf);
+}
+
+// Tracer::scatterAtSurface (transport.cpp:437-471).
+bool Tracer::scatterAtSurface(const Hit& hit, const Optics& o, uint32_t other, int depth, Wavelengths& wl, Pcg& rng,
+                              Path& path) const {
+    const Boundary& b = w_.boundaries[hit.boundary];
+    const double hero = wl.lambda[0];
+    Interface it{&o, hit.n, hit.p, w_.indexOf(b.front, hero), w_.indexOf(b.back, hero), hero, s_.fresnelFloor};
+    auto at = [&](int i) {
+        Interface r = it;
+        r.nFront = w_.indexOf(b.front, wl.lambda[i]);
+        r.nBack = w_.indexOf(b.back, wl.lambda[i]);
+        r.lambda = wl.lambda[i];
+        return r;
+    };
+    // At a dispersive dielectric the secondary wavelengths end; the hero carries the path.
+    if (o.type == OWE_SURF_DIELECTRIC && !wl.secondaryTerminated())
+        for (int i = 1; i < NW; ++i)
+            if (wl.pdf[i] != 0 && isDispersive(it, at(i))) {
+                wl.terminateSecondary();
+                maskTerminated(wl, path.beta);
+                break;
+            }
+    if (!o.isDelta()) {
+        const Spec4 dl = directLighting(hit, true, &o, it.nFront, it.nBack, path.ray.d, wl, rng, 0, path.region);
+        if (!dl.isZero()) path.L += path.beta * dl;
+    }
+    double u1, u2, u3;
+    rng.draw3(u1, u2, u3);
+    Sample s;
+    const bool ok = sampleScatter(it, path.ray.d, u1, u2, u3, kRadiance, s);
+    if (!ok || s.weight <= 0) return false;
+    path.beta[0] *= s.weight;
+    for (int i = 1; i < NW; ++i)
+        if (wl.pdf[i] != 0) path.beta[i] *= secondaryWeight(at(i), path.ray.d, s, kRadiance);
+    if (s.transmitted) path.region = other;
+    path.specular = s.delta;
+    path.prevPdf = s.pdf;
+    path.prevP = hit.p;
+    path.ray = w_.spawn(hit, s.wi);
+    if (depth >= s_.rrDepth && !path.survives(rng)) return false;
+    return true;
+}
+
+Spec4 Tracer::radiance(Ray ray, uint32_t region, Wavelengths& wl, Pcg& rng, Stats& st) const {
+    Path path{ray, region, Spec4(), Spec4(1.0), true, 0.0, V3()};
+    path.prevP = ray.o;
+    maskTerminated(wl, path.beta);
+    st.paths++;
+    for (int depth = 0;; ++depth) {
+        Hit hit;
+        const bool found = w_.intersect(path.ray, kInf, hit);
+        st.segments++;
+        const Medium& med = w_.mediumOf(path.region);
+        double t;
+        const MediumEvent ev = sampleMedium(med, wl, found ? hit.t : kInf, rng, path.beta, t);
+        if (ev == MediumEvent::End) break;
+        if (ev == MediumEvent::Scatter) {
+            // scatterInMedium (transport.cpp:379-397).
+            const V3 p = path.ray.o + path.ray.d * t;
+            if (depth >= s_.maxDepth) break;
+            Hit mh;
+            mh.p = p;
+            const Spec4 dl = directLighting(mh, false, nullptr, 1, 1, path.ray.d, wl, rng, med.g, path.region);
+            if (!dl.isZero()) path.L += path.beta * dl;
+            double u1, u2;
+            rng.draw2(u1, u2);
+            V3 wi;
+            path.prevPdf = sampleHg(-path.ray.d, med.g, u1, u2, wi);
+            path.specular = false;
+            path.prevP = p;
+            path.ray = Ray{p, wi, kNone, kNone};
+            if (!(depth < s_.rrDepth || path.survives(rng))) break;
+            continue;
+        }
+        if (!found) {
+            if (path.region == 0) {
+                // skyMet (transport.cpp:401-417): the uniform sky, and the sun weighted against NEE.
+                Spec4 sky, sun;
+                for (int i = 0; i < NW; ++i) {
+                    sky[i] = w_.env.skyAt(wl.lambda[i]);
+                    sun[i] = w_.env.sunAt(path.ray.d, wl.lambda[i]);
+                }
+                double w = 1;
+                if (!path.specular && !sun.isZero()) {
+                    const double sunPdf = w_.env.hasSun ? w_.lightPdf.back() / w_.env.sunSolidAngle() : 0;
+                    w = powerHeuristic(path.prevPdf, sunPdf);
+                }
+                path.L += path.beta * (sky + sun * w);
+            } else {
+                st.leaks++;
+            }
+            break;
+        }
+        const Boundary& b = w_.boundaries[hit.boundary];
+        const bool fromFront = dot(path.ray.d, hit.n) < 0;
+        const uint32_t inc = fromFront ? b.front : b.back, other = fromFront ? b.back : b.front;
+        if (inc != path.region) {
+            st.inconsistencies++;
+            path.region = inc;
+        }
+        const Optics& o = w_.optics[size_t(b.optics)];
+        // emissionMet (transport.cpp:420-433).
+        if (b.emission >= 0) {
+            const Emission& e = w_.emissions[size_t(b.emission)];
+            if (fromFront ? e.front : e.back) {
+                double w = 1;
+                if (!path.specular) {
+                    const int li = w_.lightOfBoundary[hit.boundary];
+                    const double pl = li < 0 ? 0
+                                             : w_.lightPdf[size_t(li)] *
+                                                   w_.pdfFrom(b, b.toLocal.point(path.prevP), hit.pLocal, hit.nLocal);
+                    if (pl > 0) w = powerHeuristic(path.prevPdf, pl);
+                }
+                path.L += path.beta * wl.eval(e.radiance) * w;
+            }
+        }
+        if (o.type == OWE_SURF_NULL) {
+            path.region = other;
+            path.ray = w_.spawn(hit, path.ray.d);
+            if (depth >= 4 * s_.maxDepth) break;
+            continue;
+        }
+        if (o.type == OWE_SURF_ABSORBER || o.type == OWE_SURF_DETECTOR) break;
+        if (depth >= s_.maxDepth) break;
+        if (!scatterAtSurface(hit, o, other, depth, wl, rng, path)) break;
+    }
+    return path.L;
+}
+
+// One sample of cpu_backend.cpp:69-84: wavelengths, position in the pixel, filter offset, ray.
+bool Tracer::cameraSample(int x, int y, int s, int spp, double rot, Pcg& rng, Stats& st, double xyz[3]) const {
+    xyz[0] = xyz[1] = xyz[2] = 0;
+    double u = (s + rng.uniform()) / spp + rot;
+    u -= std::floor(u);
+    Wavelengths wl = s_.spectral == OWE_SPECTRAL_HERO4 ? Wavelengths::hero(u)
+                                                        : Wavelengths::uniform1(u, s_.lambdaMin, s_.lambdaMax);
+    const double px = x + rng.uniform(), py = y + rng.uniform();
+    double fx = 0, fy = 0;
+    const Observer& ob = w_.observer;
+    if (ob.sigma > 0) {
+        double u1, u2;
+        rng.draw2(u1, u2);
+        standardNormal2(u1, u2, fx, fy);
+        fx *= ob.sigma;
+        fy *= ob.sigma;
+    }
+    Ray ray;
+    double weight;
+    if (!ob.generate(px + fx, py + fy, rng, ray, weight) || weight <= 0) return false;
+    const Spec4 L = radiance(ray, ob.region, wl, rng, st);
+    if (L.isZero()) return false;
+    wl.toXYZ(L * weight, xyz);
+    return std::isfinite(xyz[0] + xyz[1] + xyz[2]);
+}
+
+}  // namespace oracle

@@ -156,6 +156,13 @@ class Oracle(_Lib):
 
     def extra_sigs(self):
         return {
+            "world_create": (_vp, [_vp]),
+            "world_destroy": (None, [_vp]),
+            "light_table": (_i, [_vp, _vp, _vp, _i]),
+            "render": (None, [_vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp]),
+            "camera_paths": (None, [_vp, _vp, _i, _i, _u64, _vp, _vp, _vp, _vp, _i]),
+            "spectrum_luminance_blackbody": (_d, [_d]),
+            "medium_index": (_d, [_vp, _d]),
             "ciddor_air": (_d, [_d]),
             "mesh_any": (None, [_vp, _vp, _u64, _vp, _i]),
             "mesh_brute": (None, [_vp, _vp, _u64, _vp, _vp, _i]),
@@ -183,3 +190,46 @@ class Reference(_Lib):
 
 def reference_available() -> bool:
     return REF_LIB.exists()
+
+
+class World:
+    """The oracle's fp64 world built from a flattened scene (owe_hip.scene.Scene)."""
+
+    def __init__(self, oracle: Oracle, scene):
+        self.o = oracle
+        self.scene = scene
+        self.c = scene.c()
+        self.h = oracle.world_create(ctypes.byref(self.c))
+        ob = scene.observer
+        self.width, self.height = ob["width"], ob["height"]
+
+    def __del__(self):
+        try:
+            self.o.world_destroy(self.h)
+        except Exception:
+            pass
+
+    def light_table(self):
+        pdf = np.zeros(64)
+        b = np.zeros(64, dtype=np.uint32)
+        n = self.o.light_table(self.h, _ptr(pdf), _ptr(b), 64)
+        return pdf[:n], b[:n]
+
+    def render(self, settings, spp: int, passes: int = 1, layout: int = 1, threads: int = 0, first_pass: int = 0):
+        """Resolved XYZ image [H, W, 3] (film ÷ samples) and stats (paths, segments, inconsistencies, leaks)."""
+        xyz = np.zeros((self.height, self.width, 3))
+        samples = np.zeros((self.height, self.width))
+        stats = np.zeros(4, dtype=np.uint64)
+        for p in range(first_pass, first_pass + passes):
+            self.o.render(self.h, ctypes.byref(settings), spp, p, layout, threads, _ptr(xyz), _ptr(samples), _ptr(stats))
+        return xyz / np.maximum(samples, 1)[..., None], stats
+
+    def camera_paths(self, settings, spp: int, pixel, sample, pass_index: int = 0, threads: int = 0):
+        pixel = np.ascontiguousarray(pixel, dtype=np.uint32)
+        sample = np.ascontiguousarray(sample, dtype=np.uint32)
+        n = len(pixel)
+        xyz = np.zeros((n, 3))
+        segs = np.zeros(n, dtype=np.uint32)
+        self.o.camera_paths(self.h, ctypes.byref(settings), spp, pass_index, n, _ptr(pixel), _ptr(sample), _ptr(xyz),
+                            _ptr(segs), threads)
+        return xyz, segs
