@@ -256,12 +256,26 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="prism", choices=["prism", "rayquery"])
     ap.add_argument("--no-rayquery", action="store_true", help="prism: skip the ray-query record beside it")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy a 1 GiB device buffer 3 times (the known-size kernel that calibrates "
+                         "rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); every workload here runs on a single "
               "MI355X and no multi-GPU figure is produced.", file=sys.stderr)
         return 2
+    if args.calibrate_copy:
+        import torch
+
+        src = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read_per_copy": src.numel() * 4,
+                          "bytes_written_per_copy": dst.numel() * 4}), flush=True)
+        return 0
     if args.workload == "prism":
         result = bench_prism(args)
         if not args.no_rayquery:
