@@ -60,7 +60,8 @@ struct Paths {
     uint4* shInfo;    // start region, light's region, excluded boundary, departed boundary
     uint32_t* shFace; // departed face
     uint32_t* queue[2];
-    uint32_t* count;  // [2] queue lengths; [2] inconsistencies; [3] leaks (as 32-bit halves of 64-bit)
+    uint32_t* count;  // [0], [1] queue lengths; [2] inconsistencies; [3] leaks
+    unsigned long long* segments;  // segments traced in the current chunk (sum of live counts)
 };
 
 __device__ inline WL loadWL(const float4 l, uint32_t flags, const Settings& s) {
@@ -76,6 +77,7 @@ __device__ inline WL loadWL(const float4 l, uint32_t flags, const Settings& s) {
         w.pdf[0] = visiblePdfF(l.x) / NW;
         w.pdf[1] = w.pdf[2] = w.pdf[3] = 0.0f;
     } else {
+#pragma unroll
         for (int i = 0; i < NW; ++i) w.pdf[i] = visiblePdfF(w.lambda[i]);
     }
     return w;
@@ -122,6 +124,7 @@ __global__ __launch_bounds__(kBlock) void generateKernel(DScene sc, Settings set
         beta = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
     } else {
         float ls[NW];
+#pragma unroll
         for (int k = 0; k < NW; ++k) {
             float up = u + float(k) / NW;
             if (up >= 1.0f) up -= 1.0f;
@@ -163,9 +166,13 @@ __global__ __launch_bounds__(kBlock) void generateKernel(DScene sc, Settings set
 
 // ---------------------------------------------------------------- intersect
 
+// The live count is read from device memory, so rounds can be enqueued without a host round trip;
+// the grid covers a bound on it the host knows (live counts only shrink).
 template <int kCap>
-__global__ __launch_bounds__(kBlock) void intersectKernel(DScene sc, Paths P, int q, uint32_t count) {
+__global__ __launch_bounds__(kBlock) void intersectKernel(DScene sc, Paths P, int q) {
     __shared__ uint32_t stack[kCap * kBlock];
+    const uint32_t count = P.count[q];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && count > 0) atomicAdd(P.segments, (unsigned long long)count);
     const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= count) return;
     const uint32_t slot = P.queue[q][k];
@@ -408,6 +415,7 @@ __device__ inline void prepareDirect(ShadeCtx& c, f3 p, bool surface, const Ifac
     if (L.kind == 1) {
         const Frame f = makeFrame(ld3(sc.env.sunDir));
         wi = toWorld(f, uniformConeOmc(u1, u2, sc.env.sunOneMinusCos));
+#pragma unroll
         for (int i = 0; i < NW; ++i) Le[i] = evalSpec(sc.env.sunRadiance, sc.table, c.wl.lambda[i]);
         lpdf = L.pdf / sc.env.sunSolidAngle;
     } else {
@@ -423,6 +431,7 @@ __device__ inline void prepareDirect(ShadeCtx& c, f3 p, bool surface, const Ifac
         const DEmission& e = sc.emissions[b.emission];
         const float cosL = dot3(en, -wi);
         const bool lit = (cosL > 0.0f && e.front) || (cosL < 0.0f && e.back);
+#pragma unroll
         for (int i = 0; i < NW; ++i) Le[i] = lit ? evalSpec(e.radiance, sc.table, c.wl.lambda[i]) : 0.0f;
         lpdf = L.pdf * pdfW;
         lightRegion = cosL > 0.0f ? b.front : b.back;
@@ -430,6 +439,7 @@ __device__ inline void prepareDirect(ShadeCtx& c, f3 p, bool surface, const Ifac
     }
     if (!(lpdf > 0.0f)) return;
     bool any = false;
+#pragma unroll
     for (int i = 0; i < NW; ++i) any = any || Le[i] != 0.0f;
     if (!any) return;
     float f[NW] = {0, 0, 0, 0};
@@ -437,10 +447,12 @@ __device__ inline void prepareDirect(ShadeCtx& c, f3 p, bool surface, const Ifac
     uint32_t start = c.region;
     if (!surface) {
         pdfScatter = hgPhase(dot3(-c.d, wi), g);
+#pragma unroll
         for (int i = 0; i < NW; ++i) f[i] = pdfScatter;
     } else {
         const DBoundary& b = sc.boundaries[hitBoundary];
         bool nz = false;
+#pragma unroll
         for (int i = 0; i < NW; ++i) {
             if (c.wl.pdf[i] == 0.0f) continue;
             Iface at = *it;
@@ -488,12 +500,14 @@ __device__ inline bool shadeOne(ShadeCtx& c, const uint4 hraw, uint32_t* counter
     // sampleMedium (transport.cpp:95-113).
     if (med.flags & 2) {
         float sT[NW];
+#pragma unroll
         for (int i = 0; i < NW; ++i)
             sT[i] = evalSpec(med.absorption, sc.table, c.wl.lambda[i]) + evalSpec(med.scattering, sc.table, c.wl.lambda[i]);
         const float tm = -logf(1.0f - c.rng.uniform()) / sT[0];
         const float seg = found ? t : __builtin_inff();
         if (tm < seg) {
             const float pdf0 = sT[0] * expf(-sT[0] * tm);
+#pragma unroll
             for (int i = 0; i < NW; ++i)
                 setc(c.beta, i, get(c.beta, i) * evalSpec(med.scattering, sc.table, c.wl.lambda[i]) * expf(-sT[i] * tm) / pdf0);
             // scatterInMedium (transport.cpp:379-397).
@@ -513,9 +527,11 @@ __device__ inline bool shadeOne(ShadeCtx& c, const uint4 hraw, uint32_t* counter
         }
         if (!found) return false;
         const float pass0 = expf(-sT[0] * seg);
+#pragma unroll
         for (int i = 0; i < NW; ++i) setc(c.beta, i, get(c.beta, i) * expf(-sT[i] * seg) / pass0);
     } else if (med.flags & 1) {
         if (!found) return false;
+#pragma unroll
         for (int i = 0; i < NW; ++i) setc(c.beta, i, get(c.beta, i) * expf(-evalSpec(med.absorption, sc.table, c.wl.lambda[i]) * t));
     }
     if (!found) {
@@ -526,6 +542,7 @@ __device__ inline bool shadeOne(ShadeCtx& c, const uint4 hraw, uint32_t* counter
             const bool inSun = sc.env.hasSun && 0.5f * dot3(dd, dd) <= sc.env.sunOneMinusCos;
             float w = 1.0f;
             if (inSun && !(c.flags & kFlagSpecular)) w = powerHeuristic(c.prevPdf, sc.env.sunPdf);
+#pragma unroll
             for (int i = 0; i < NW; ++i) {
                 const float sky = sc.env.sky == OWE_SKY_UNIFORM ? evalSpec(sc.env.skyRadiance, sc.table, c.wl.lambda[i]) : 0.0f;
                 const float sun = inSun ? evalSpec(sc.env.sunRadiance, sc.table, c.wl.lambda[i]) : 0.0f;
@@ -573,6 +590,7 @@ __device__ inline bool shadeOne(ShadeCtx& c, const uint4 hraw, uint32_t* counter
                 const float pl2 = sc.lights[b.light].pdf * pdfFrom(b, xfPoint(b.toLocal, c.prevP), pl, nl);
                 if (pl2 > 0.0f) w = powerHeuristic(c.prevPdf, pl2);
             }
+#pragma unroll
             for (int i = 0; i < NW; ++i)
                 setc(c.L, i, get(c.L, i) + get(c.beta, i) * evalSpec(e.radiance, sc.table, c.wl.lambda[i]) * w);
         }
@@ -599,6 +617,7 @@ __device__ inline bool shadeOne(ShadeCtx& c, const uint4 hraw, uint32_t* counter
     };
     if (o.type == OWE_SURF_DIELECTRIC && !secondaryTerminated(c.wl)) {
         const float a0 = it.nBack / it.nFront;
+#pragma unroll
         for (int i = 1; i < NW; ++i) {
             if (c.wl.pdf[i] == 0.0f) continue;
             const Iface ai = at(i);
@@ -616,6 +635,7 @@ __device__ inline bool shadeOne(ShadeCtx& c, const uint4 hraw, uint32_t* counter
     SSample s;
     if (!sampleScatter(sc, it, c.d, u1, u2, u3, s) || s.weight <= 0.0f) return false;
     c.beta.x *= s.weight;
+#pragma unroll
     for (int i = 1; i < NW; ++i)
         if (c.wl.pdf[i] != 0.0f) setc(c.beta, i, get(c.beta, i) * secondaryWeight(sc, at(i), c.d, s));
     if (s.transmitted) c.region = other;
@@ -631,8 +651,8 @@ __device__ inline bool shadeOne(ShadeCtx& c, const uint4 hraw, uint32_t* counter
     return true;
 }
 
-__global__ __launch_bounds__(kBlock) void shadeKernel(DScene sc, Settings set, Paths P, int q, uint32_t count,
-                                                      uint64_t passInc) {
+__global__ __launch_bounds__(kBlock) void shadeKernel(DScene sc, Settings set, Paths P, int q, uint64_t passInc) {
+    const uint32_t count = P.count[q];
     const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
     bool alive = false;
     uint32_t slot = 0;
@@ -684,12 +704,12 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(DScene sc, Settings set, P
     if (mask == 0) return;
     const uint32_t lane = __lane_id();
     const int leader = __ffsll((long long)mask) - 1;
-    uint32_t base = 0;
-    if (int(lane) == leader) base = atomicAdd(&P.count[1 - q], uint32_t(__popcll(mask)));
-    base = __shfl(base, leader);
+    uint32_t at = 0;
+    if (int(lane) == leader) at = atomicAdd(&P.count[1 - q], uint32_t(__popcll(mask)));
+    at = __shfl(at, leader);
     if (alive) {
         const uint32_t below = __popcll(mask & ((1ull << lane) - 1ull));
-        P.queue[1 - q][base + below] = slot;
+        P.queue[1 - q][at + below] = slot;
     }
 }
 
@@ -697,12 +717,17 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(DScene sc, Settings set, P
 
 // Tracer::transmittance (transport.cpp:237-285): through null boundaries with Beer–Lambert.
 template <int kCap>
-__global__ __launch_bounds__(kBlock) void shadowKernel(DScene sc, Settings set, Paths P, int q, uint32_t count,
-                                                       int simple) {
+__device__ inline void shadowOne(const DScene& sc, const Paths& P, uint32_t slot, int simple, uint32_t* stack);
+
+template <int kCap>
+__global__ __launch_bounds__(kBlock) void shadowKernel(DScene sc, Settings set, Paths P, int q, int simple) {
     __shared__ uint32_t stack[kCap * kBlock];
     const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= count) return;
-    const uint32_t slot = P.queue[q][k];
+    if (k < P.count[q]) shadowOne<kCap>(sc, P, P.queue[q][k], simple, stack + threadIdx.x);
+}
+
+template <int kCap>
+__device__ inline void shadowOne(const DScene& sc, const Paths& P, uint32_t slot, int simple, uint32_t* stack) {
     const float4 pend = P.shPend[slot];
     if (pend.x == 0.0f && pend.y == 0.0f && pend.z == 0.0f && pend.w == 0.0f) return;
     const float4 so = P.shO[slot], sd = P.shD[slot];
@@ -716,7 +741,7 @@ __global__ __launch_bounds__(kBlock) void shadowKernel(DScene sc, Settings set, 
     const float4 lam = P.lam[slot];
     for (int guard = 0; guard < 64; ++guard) {
         Hit h;
-        const bool found = intersectWorld(sc, o, d, remaining, departed, face, si.z, stack + threadIdx.x, kBlock, h);
+        const bool found = intersectWorld(sc, o, d, remaining, departed, face, si.z, stack, kBlock, h);
         if (simple) {
             // No null boundaries or participating media in the scene: any crossing blocks.
             if (found) return;
@@ -727,6 +752,7 @@ __global__ __launch_bounds__(kBlock) void shadowKernel(DScene sc, Settings set, 
         const DMedium& m = sc.media[sc.regionMedium[region]];
         if (m.flags) {
             if (isinf(seg)) return;
+#pragma unroll
             for (int i = 0; i < NW; ++i) {
                 const float l = get(lam, i);
                 Tr[i] *= expf(-(evalSpec(m.absorption, sc.table, l) + evalSpec(m.scattering, sc.table, l)) * seg);
@@ -759,44 +785,77 @@ __global__ __launch_bounds__(kBlock) void shadowKernel(DScene sc, Settings set, 
 
 // ---------------------------------------------------------------- splat
 
+__device__ inline long long fixedPoint(float v) {
+    // |value| stays below 2^30 units of XYZ: far above any sample here, and no overflow of the
+    // 64-bit two's-complement sum.
+    return llrint(fmin(fmax(double(v) * kFilmScale, -4.6e18), 4.6e18));
+}
+
+// Each path's XYZ, then one film atomic per run of same-pixel lanes in the wave: a segmented suffix
+// sum of 64-bit integers (exact, so the film stays independent of order) over the lanes' pixels,
+// which are contiguous runs because path g belongs to pixel g / spp.
 __global__ __launch_bounds__(kBlock) void splatKernel(Settings set, Paths P, uint32_t count, float invNWY,
                                                       unsigned long long* film, double* perPath) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= count) return;
-    const float4 L = P.L[i];
-    const uint4 info = P.info[i];
-    const WL wl = loadWL(P.lam[i], info.y >> 16, set);
-    const float weight = P.ray1[i].w;
-    float X = 0.0f, Y = 0.0f, Z = 0.0f;
-    for (int k = 0; k < NW; ++k) {
-        const float v = get(L, k) * weight;
-        if (wl.pdf[k] == 0.0f || v == 0.0f) continue;
-        const float w = v / wl.pdf[k] * invNWY;
-        const f3 c = cieXYZ(wl.lambda[k]);
-        X += w * c.x;
-        Y += w * c.y;
-        Z += w * c.z;
-    }
-    if (perPath) {
+    long long v[3] = {0, 0, 0};
+    uint32_t pix = 0xFFFFFFFFu;
+    if (i < count) {
+        const float4 L = P.L[i];
+        const uint4 info = P.info[i];
+        const WL wl = loadWL(P.lam[i], info.y >> 16, set);
+        const float weight = P.ray1[i].w;
+        float X = 0.0f, Y = 0.0f, Z = 0.0f;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) {
+            const float val = get(L, k) * weight;
+            if (wl.pdf[k] == 0.0f || val == 0.0f) continue;
+            const float w = val / wl.pdf[k] * invNWY;
+            const f3 c = cieXYZ(wl.lambda[k]);
+            X += w * c.x;
+            Y += w * c.y;
+            Z += w * c.z;
+        }
         const bool ok = isfinite(X + Y + Z);
-        perPath[3 * size_t(i)] = ok ? X : 0.0;
-        perPath[3 * size_t(i) + 1] = ok ? Y : 0.0;
-        perPath[3 * size_t(i) + 2] = ok ? Z : 0.0;
-        return;
+        if (perPath) {
+            perPath[3 * size_t(i)] = ok ? X : 0.0;
+            perPath[3 * size_t(i) + 1] = ok ? Y : 0.0;
+            perPath[3 * size_t(i) + 2] = ok ? Z : 0.0;
+        } else if (ok) {
+            pix = info.x;
+            v[0] = fixedPoint(X);
+            v[1] = fixedPoint(Y);
+            v[2] = fixedPoint(Z);
+        }
     }
-    if (!isfinite(X + Y + Z) || (X == 0.0f && Y == 0.0f && Z == 0.0f)) return;
-    const float c[3] = {X, Y, Z};
-    for (int k = 0; k < 3; ++k) {
-        // |value| stays below 2^30 units of XYZ: far above any sample here, and no overflow of the
-        // 64-bit two's-complement sum.
-        const double scaled = fmin(fmax(double(c[k]) * kFilmScale, -4.6e18), 4.6e18);
-        const long long v = llrint(scaled);
-        atomicAdd(&film[3 * size_t(info.x) + k], (unsigned long long)v);
+    if (perPath) return;  // uniform across the grid
+    const uint32_t lane = __lane_id();
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t po = __shfl_down(pix, off);
+        long long o[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) o[c] = __shfl_down(v[c], off);
+        if (lane + off < 64 && po == pix) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v[c] += o[c];
+        }
     }
+    const uint32_t prev = __shfl_up(pix, 1);
+    const bool head = lane == 0 || prev != pix;
+    if (head && pix != 0xFFFFFFFFu)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            if (v[c] != 0) atomicAdd(&film[3 * size_t(pix) + c], (unsigned long long)v[c]);
 }
 
 __global__ void clearCounts(uint32_t* c, int n) {
     if (threadIdx.x < unsigned(n)) c[threadIdx.x] = 0;
+}
+
+// Starts a chunk: queue 0 holds `count` paths, queue 1 none, no segments, inconsistencies or leaks yet.
+__global__ void setCount(uint32_t* c, unsigned long long* segments, uint32_t count) {
+    if (threadIdx.x < 4) c[threadIdx.x] = threadIdx.x == 0 ? count : 0u;
+    if (threadIdx.x == 0) *segments = 0;
 }
 
 }  // namespace
@@ -826,6 +885,7 @@ struct owe_hip_renderer {
     float invNWY = 0;
     // Optional per-kernel timing with HIP events on the renderer's stream: generate, intersect,
     // shade, shadow, splat.
+    int roundsPerSync = 4;     // bounce rounds enqueued between host reads of the live counters
     bool timing = false;
     hipEvent_t ev[2][5] = {};
     double kernelMs[5] = {0, 0, 0, 0, 0};
@@ -869,6 +929,7 @@ void allocPaths(owe_hip_renderer* r, uint32_t n) {
     P.queue[0] = devAlloc<uint32_t>(r, n);
     P.queue[1] = devAlloc<uint32_t>(r, n);
     P.count = devAlloc<uint32_t>(r, 4);
+    P.segments = devAlloc<unsigned long long>(r, 1);
     r->capacity = n;
 }
 
@@ -898,25 +959,27 @@ void collectTimes(owe_hip_renderer* r, std::initializer_list<int> which) {
 
 // One segment of every live path; the shade kernel's PCG increment is the pass's ((pass << 1) | 1).
 template <int kCap>
-void launchBounce(owe_hip_renderer* r, int q, uint32_t count, uint64_t passInc) {
+void launchBounce(owe_hip_renderer* r, int q, uint32_t upper, uint64_t passInc) {
+    // upper: the live count at the host's last read, a bound on every later round's.
+    const dim3 grid = gridFor(upper);
     timed(r, kIntersect, [&] {
-        hipLaunchKernelGGL((intersectKernel<kCap>), gridFor(count), dim3{kBlock}, 0, r->stream, r->dev, r->P, q, count);
+        hipLaunchKernelGGL((intersectKernel<kCap>), grid, dim3{kBlock}, 0, r->stream, r->dev, r->P, q);
     });
     timed(r, kShade, [&] {
-        hipLaunchKernelGGL(shadeKernel, gridFor(count), dim3{kBlock}, 0, r->stream, r->dev, r->set, r->P, q, count,
-                           passInc);
+        hipLaunchKernelGGL(shadeKernel, grid, dim3{kBlock}, 0, r->stream, r->dev, r->set, r->P, q, passInc);
     });
     timed(r, kShadow, [&] {
-        hipLaunchKernelGGL((shadowKernel<kCap>), gridFor(count), dim3{kBlock}, 0, r->stream, r->dev, r->set, r->P, q,
-                           count, int(r->simpleShadows));
+        hipLaunchKernelGGL((shadowKernel<kCap>), grid, dim3{kBlock}, 0, r->stream, r->dev, r->set, r->P, q,
+                           int(r->simpleShadows));
     });
+    // The consumed queue's counter is cleared for the round after next (a kernel, not a memset).
+    hipLaunchKernelGGL(clearCounts, dim3{1}, dim3{64}, 0, r->stream, r->P.count + q, 1);
 }
 
 // Runs count paths: slots [0, count) with global path ids base + slot, or the (pixel, sample) lists.
 void runPaths(owe_hip_renderer* r, uint64_t base, uint32_t count, uint32_t spp, uint32_t pass, bool lists,
               double* perPath) {
     const uint64_t passInc = (uint64_t(pass) << 1) | 1u;
-    hipLaunchKernelGGL(clearCounts, dim3{1}, dim3{64}, 0, r->stream, r->P.count, 4);
     timed(r, kGenerate, [&] {
         hipLaunchKernelGGL(generateKernel, gridFor(count), dim3{kBlock}, 0, r->stream, r->dev, r->set, r->P, base, count,
                            spp, pass, lists ? r->pathPixels : nullptr, lists ? r->pathSamples : nullptr);
@@ -924,30 +987,46 @@ void runPaths(owe_hip_renderer* r, uint64_t base, uint32_t count, uint32_t spp, 
     OWE_HIP_CHECK(hipGetLastError());
     bool generated = true;
     r->stats.paths += count;
-    uint32_t live = count;
+    const uint64_t segmentsBefore = r->stats.segments;
+    // Rounds are enqueued in groups without waiting: each kernel reads its live count on the device.
+    // The host reads the counters back once per group to stop, and the bound it knew (`upper`, the
+    // live count at the last read) sizes the grids.
+    uint32_t upper = count;
+    uint32_t hostCounts[4] = {count, 0, 0, 0};
     int q = 0;
-    uint32_t hostCounts[4] = {0, 0, 0, 0};
-    // Every round raises each live path's depth by one and no path outlives depth 4·maxDepth (null
-    // crossings); the guard only bounds the loop should that ever fail.
     const int maxRounds = 4 * r->set.maxDepth + 4;
-    for (int round = 0; live > 0 && round <= maxRounds; ++round) {
-        r->stats.segments += live;
-        if (r->stackCap <= 16) launchBounce<16>(r, q, live, passInc);
-        else if (r->stackCap <= 32) launchBounce<32>(r, q, live, passInc);
-        else launchBounce<48>(r, q, live, passInc);
-        OWE_HIP_CHECK(hipGetLastError());
+    // Every round raises each live path's depth by one and no path outlives depth 4·maxDepth (null
+    // crossings); the round limit only bounds the loop should that ever fail.
+    hipLaunchKernelGGL(setCount, dim3{1}, dim3{64}, 0, r->stream, r->P.count, r->P.segments, count);
+    for (int round = 0; upper > 0 && round <= maxRounds;) {
+        const int group = round < 2 ? 1 : r->roundsPerSync;
+        for (int g = 0; g < group && round <= maxRounds; ++g, ++round) {
+            if (r->stackCap <= 16) launchBounce<16>(r, q, upper, passInc);
+            else if (r->stackCap <= 32) launchBounce<32>(r, q, upper, passInc);
+            else launchBounce<48>(r, q, upper, passInc);
+            OWE_HIP_CHECK(hipGetLastError());
+            q = 1 - q;
+            if (r->timing) {
+                OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
+                if (generated) collectTimes(r, {kGenerate});
+                generated = false;
+                collectTimes(r, {kIntersect, kShade, kShadow});
+            }
+        }
         OWE_HIP_CHECK(hipMemcpyAsync(hostCounts, r->P.count, sizeof hostCounts, hipMemcpyDeviceToHost, r->stream));
         OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
         if (generated) collectTimes(r, {kGenerate});
         generated = false;
-        collectTimes(r, {kIntersect, kShade, kShadow});
-        live = hostCounts[1 - q];
-        // Reset the consumed queue's counter for the next bounce (a kernel, not a memset).
-        hipLaunchKernelGGL(clearCounts, dim3{1}, dim3{64}, 0, r->stream, r->P.count + q, 1);
-        q = 1 - q;
+        upper = hostCounts[q];
     }
     r->stats.inconsistencies += hostCounts[2];
     r->stats.leaks += hostCounts[3];
+    {
+        unsigned long long segs = 0;
+        OWE_HIP_CHECK(hipMemcpyAsync(&segs, r->P.segments, sizeof segs, hipMemcpyDeviceToHost, r->stream));
+        OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
+        r->stats.segments = segmentsBefore + segs;
+    }
     timed(r, kSplat, [&] {
         hipLaunchKernelGGL(splatKernel, gridFor(count), dim3{kBlock}, 0, r->stream, r->set, r->P, count, r->invNWY, r->film,
                            perPath);

@@ -432,49 +432,50 @@ __device__ inline bool intersectWorld(const DScene& sc, f3 o, f3 d, float tmax, 
         }
     }
     const f3 inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    // Only (t, boundary) travel through the loop; the winner's point and normal are rebuilt after it.
+    uint32_t analytic = kNoBoundary;
     for (uint32_t i = 0; i < sc.nBoundaries; ++i) {
         const DBoundary& b = sc.boundaries[i];
         if (b.shape == OWE_SHAPE_MESH || i == exclude) continue;
         if (!boxHit(b, o, inv, h.t)) continue;
         const f3 lo = xfPoint(b.toLocal, o), ldv = xfVector(b.toLocal, d);
         float t;
-        f3 pl, nl;
         if (b.shape == OWE_SHAPE_PLANE) {
             if (i == departed || ldv.z == 0.0f) continue;
             t = -lo.z / ldv.z;
             if (!(t > 0.0f && t < h.t)) continue;
-            pl = lo + ldv * t;
+            f3 pl = lo + ldv * t;
             pl.z = 0.0f;
             if (!inAperture(b.aperture, b.a, b.b, pl)) continue;
-            nl = mk3(0, 0, 1);
+        } else if (i == departed) {
+            t = -2.0f * dot3(lo, ldv) / dot3(ldv, ldv);  // the root other than the origin's own
+            if (!(t > 0.0f && t < h.t)) continue;
         } else {
-            if (i == departed) {
-                t = -2.0f * dot3(lo, ldv) / dot3(ldv, ldv);  // the root other than the origin's own
-                if (!(t > 0.0f && t < h.t)) continue;
-            } else {
-                float t0, t1;
-                if (!sphereRoots(lo, ldv, b.a, t0, t1)) continue;
-                t = t0 > 0.0f ? t0 : t1;
-                if (!(t > 0.0f && t < h.t)) continue;
-            }
-            pl = lo + ldv * t;
-            nl = norm3(pl / b.a);
+            float t0, t1;
+            if (!sphereRoots(lo, ldv, b.a, t0, t1)) continue;
+            t = t0 > 0.0f ? t0 : t1;
+            if (!(t > 0.0f && t < h.t)) continue;
         }
         h.t = t;
-        h.boundary = i;
+        analytic = i;
+    }
+    if (analytic != kNoBoundary) {
+        const DBoundary& b = sc.boundaries[analytic];
+        const f3 lo = xfPoint(b.toLocal, o), ldv = xfVector(b.toLocal, d);
+        f3 pl = lo + ldv * h.t;
+        if (b.shape == OWE_SHAPE_PLANE) pl.z = 0.0f;
+        const f3 nl = b.shape == OWE_SHAPE_PLANE ? mk3(0, 0, 1) : norm3(pl / b.a);
+        h.boundary = analytic;
         h.face = 0;
         h.slot = kEmptyChild;
         h.pLocal = pl;
         h.nLocal = nl;
+        h.p = xfPoint(b.toWorld, pl);
+        h.n = norm3(xfVector(b.toWorld, nl));
+        return true;
     }
     if (h.boundary == kNoBoundary) return false;
-    if (h.slot == kEmptyChild) {
-        const DBoundary& b = sc.boundaries[h.boundary];
-        h.p = xfPoint(b.toWorld, h.pLocal);
-        h.n = norm3(xfVector(b.toWorld, h.nLocal));
-    } else {
-        h.p = o + d * h.t;
-    }
+    h.p = o + d * h.t;
     return true;
 }
 
