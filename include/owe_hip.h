@@ -100,7 +100,12 @@ int owe_hip_trace_batch(owe_hip_mesh* mesh, int mode, const float* rays, uint64_
  * The scene is copied at creation: the caller's arrays need not outlive the call. */
 typedef struct owe_hip_renderer owe_hip_renderer;
 
-/* Fails with OWE_HIP_ERR_UNSUPPORTED for an integrator other than path (renderer.hpp:22-36). */
+/* Integrators: OWE_INTEGRATOR_PATH (camera paths, Tracer::radiance, transport.cpp:473-529) and
+ * OWE_INTEGRATOR_LIGHT (particles emitted from the lights and the sky, connected to an observer or
+ * recorded by a surface sensor: the reference's "light" integrator, particle.slang:356-427, whose
+ * passes emit W·H·spp particles, gpu_renderer_passes.cpp:303-311, and whose film is divided by
+ * the particles emitted, gpu_renderer_film.cpp:17-20). Fails with OWE_HIP_ERR_UNSUPPORTED for
+ * any other integrator (renderer.hpp:22-36). */
 int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render_settings* settings,
                             owe_hip_renderer** out);
 int owe_hip_renderer_destroy(owe_hip_renderer* renderer);
@@ -119,11 +124,16 @@ int owe_hip_renderer_reset(owe_hip_renderer* renderer);
  * receives n × 3 doubles. */
 int owe_hip_renderer_camera_paths(owe_hip_renderer* renderer, int spp, int pass, const uint32_t* pixel,
                                   const uint32_t* sample, uint64_t n, double* xyz);
+/* The XYZ each particle index[i] of pass `pass` deposits, summed over its splats (not divided by
+ * the particle count), without touching the film: the light integrator's kernel, for per-particle
+ * comparison with the oracle. Host arrays; xyz receives n × 3 doubles. Light-integrator renderers only. */
+int owe_hip_renderer_particles(owe_hip_renderer* renderer, int pass, const uint64_t* index, uint64_t n, double* xyz);
 /* Per-kernel timing with HIP events on the renderer's stream (the reference's OWE_GPU_PROFILE,
  * src/owe/backends/gpu/vulkan_context.cpp:350-422). Enabling clears the totals. */
 int owe_hip_renderer_set_timing(owe_hip_renderer* renderer, int enable);
 /* Milliseconds and launches per kernel since timing was enabled, in the order generate, intersect,
- * shade, shadow, splat (5 entries each). */
+ * shade, shadow, splat (5 entries each); a light-integrator renderer counts its particle kernel
+ * as shade. */
 int owe_hip_renderer_kernel_times(const owe_hip_renderer* renderer, double* ms, uint64_t* launches);
 
 #ifdef __cplusplus

@@ -118,19 +118,36 @@ typedef struct owe_environment {
     double reserved2;
 } owe_environment;
 
-/* An ideal observer (owe::IdealObserver, scene/detector.hpp:84-116): pinhole when pupil_radius is 0. */
+#define OWE_DETECTOR_OBSERVER 0
+#define OWE_DETECTOR_SENSOR 1
+
+/* The detector a renderer measures with (owe::Detector, scene/detector.hpp:42-72):
+ *  - OWE_DETECTOR_OBSERVER: an ideal observer (IdealObserver, detector.hpp:84-116), pinhole when
+ *    pupil_radius is 0; it measures radiance.
+ *  - OWE_DETECTOR_SENSOR: a rectangular surface sensor (SurfaceSensor, detector.hpp:125-160) on
+ *    boundary sensor_boundary (a plane with DETECTOR or DIFFUSE optics), pixels over
+ *    [-half_x, half_x] x [-half_y, half_y] of its frame, measuring irradiance on its front face.
+ *    Camera paths leave it cosine-distributed, or toward the aim disk with probability aim_share
+ *    when has_aim is set; region is ignored (the boundary's front region is used). */
 typedef struct owe_observer {
     int32_t width;
     int32_t height;
     uint32_t region;  /* region containing the eye */
-    int32_t reserved;
+    int32_t kind;     /* OWE_DETECTOR_OBSERVER or OWE_DETECTOR_SENSOR */
     double position[3];
     double look_at[3];
     double up[3];
     double fov_y;          /* radians */
     double pupil_radius;
     double focus_distance; /* +inf: focus at infinity */
-    double pixel_sigma;    /* Gaussian pixel filter in pixels (0.4 by default) */
+    double pixel_sigma;    /* Gaussian pixel filter in pixels (0.4 for observers, 0 for sensors by default) */
+    /* surface sensor only */
+    uint32_t sensor_boundary;
+    int32_t flip_x, flip_y; /* readout: which physical corner is pixel (0, 0) */
+    int32_t has_aim;
+    double half_x, half_y;
+    double aim_center[3], aim_normal[3];
+    double aim_radius, aim_share;
 } owe_observer;
 
 typedef struct owe_scene {

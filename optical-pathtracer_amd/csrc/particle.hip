@@ -1,0 +1,505 @@
+// Particles (light tracing) in float32: one thread per particle, emitted by power from the sun, a
+// uniform sky or an emitting plane/sphere, walked through the world in importance transport, and
+// splatted exactly into fixed-point film bins.
+//
+//   stream and wavelengths of particle i       particle_kernel.slang:11-19 (salt 0x5bd1e995)
+//   emission: distant disc, area + cosine       particle.slang:184-344 (without photon guiding)
+//   observer link (pupil, focus, pixel filter)  particle.slang:59-95
+//   vertex connection, emitter seen directly    particle.slang:118-138, 267-279
+//   surface sensor readout                      particle.slang:98-114
+//   the walk: media, null crossings, sensor,    particle.slang:356-427
+//   non-delta connections, dispersion, scatter,
+//   roulette against the event's throughput
+//   exact splats                                particle.slang:23-49
+//
+// Connections walk from the vertex toward the eye (the reference walks from the eye toward the
+// vertex); both visit the same crossings, and the region reached must be the eye's. A distant
+// particle's first hit is found again from just before it, where float is precise
+// (refineDistantHit, particle.slang:143-147).
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "particle.hpp"
+#include "shading_dev.hpp"
+
+namespace owehip {
+namespace {
+
+constexpr int kPBlock = 128;
+
+struct Link {
+    f3 q, dir;
+    int px, py;
+    float factor;
+};
+
+struct PCtx {
+    const DScene& sc;
+    const Settings& set;
+    const ParticleBatch& pb;
+    uint32_t* stack;
+    Rng rng;
+    WL wl;
+    float beta[NW];
+    float sum[3];
+    uint32_t segments, inconsistencies, leaks;
+};
+
+__device__ inline void addFixedPoint(unsigned long long* bins, float v) {
+    if (!(v > 0.0f) || isinf(v)) return;  // zero, negative or not finite: nothing to add
+    const uint32_t bits = __float_as_uint(v);
+    const int e = int((bits >> 23) & 0xFFu);
+    if (e == 0) return;  // subnormal
+    uint32_t mant = (bits & 0x7FFFFFu) | 0x800000u;
+    int p = e - 150 - kSplatExp0;  // the significand's lowest bit, above bin 0's
+    if (p < 0) {
+        if (p <= -24) return;
+        mant >>= uint32_t(-p);
+        p = 0;
+    }
+    const int b = p >> 5, r = p & 31;
+    if (b >= kSplatBins) return;
+    const uint64_t shifted = uint64_t(mant) << r;
+    atomicAdd(&bins[b], (unsigned long long)(shifted & 0xFFFFFFFFull));
+    const uint64_t high = shifted >> 32;
+    if (high != 0 && b + 1 < kSplatBins) atomicAdd(&bins[b + 1], (unsigned long long)high);
+}
+
+__device__ inline void deposit(PCtx& c, int px, int py, const float v[NW]) {
+    float X = 0.0f, Y = 0.0f, Z = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+        if (c.wl.pdf[k] == 0.0f || v[k] == 0.0f) continue;
+        const float w = v[k] / c.wl.pdf[k] * c.pb.invNWY;
+        const f3 m = cieXYZ(c.wl.lambda[k]);
+        X += w * m.x;
+        Y += w * m.y;
+        Z += w * m.z;
+    }
+    if (c.pb.perParticle) {
+        const float xyz[3] = {X, Y, Z};
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            if (xyz[k] > 0.0f && !isinf(xyz[k])) c.sum[k] += xyz[k];
+        return;
+    }
+    unsigned long long* bins = c.pb.film + (size_t(py) * size_t(c.sc.obs.width) + size_t(px)) * (3 * kSplatBins);
+    addFixedPoint(bins, X);
+    addFixedPoint(bins + kSplatBins, Y);
+    addFixedPoint(bins + 2 * kSplatBins, Z);
+}
+
+__device__ inline void filterOffset(PCtx& c, float& fx, float& fy) {
+    fx = fy = 0.0f;
+    const float sigma = c.sc.obs.sigma;
+    if (!(sigma > 0.0f)) return;
+    const float u2 = c.rng.uniform(), u1 = c.rng.uniform();
+    const float r = sqrtf(-2.0f * logf(1.0f - u1)), phi = 2.0f * kPiF * u2;
+    fx = r * cosf(phi) * sigma;
+    fy = r * sinf(phi) * sigma;
+}
+
+// The eye sits at the camera-relative origin.
+__device__ inline bool linkObserver(PCtx& c, f3 x, Link& k) {
+    const DObserver& ob = c.sc.obs;
+    const f3 right = ld3(ob.right), up = ld3(ob.up), fwd = ld3(ob.fwd);
+    k.q = mk3(0.0f, 0.0f, 0.0f);
+    if (ob.pupil > 0.0f) {
+        const float u2 = c.rng.uniform(), u1 = c.rng.uniform();
+        float dx, dy;
+        concentricDisk(u1, u2, dx, dy);
+        k.q = (right * dx + up * dy) * ob.pupil;
+    }
+    const f3 w = x - k.q;
+    const float dist2 = dot3(w, w);
+    if (!(dist2 > 0.0f)) return false;
+    const f3 om = w / sqrtf(dist2);
+    const float cc = dot3(om, fwd);
+    if (cc <= 1e-9f) return false;
+    float sx, sy;
+    if (ob.pupil > 0.0f && isfinite(ob.focus)) {
+        const f3 F = k.q + om * (ob.focus / cc);
+        sx = dot3(F, right) / ob.focus;
+        sy = dot3(F, up) / ob.focus;
+    } else {
+        sx = dot3(om, right) / cc;
+        sy = dot3(om, up) / cc;
+    }
+    float ox, oy;
+    filterOffset(c, ox, oy);
+    const float W = float(ob.width), H = float(ob.height);
+    const float fx = (sx / ob.tanX + 1.0f) * 0.5f * W + ox, fy = (1.0f - sy / ob.tanY) * 0.5f * H + oy;
+    if (!(fx >= 0.0f && fy >= 0.0f && fx < W && fy < H)) return false;
+    k.px = min(int(fx), ob.width - 1);
+    k.py = min(int(fy), ob.height - 1);
+    const float pixelArea = (2.0f * ob.tanX / W) * (2.0f * ob.tanY / H);
+    k.factor = 1.0f / (dist2 * pixelArea * cc * cc * cc);
+    k.dir = norm3(k.q - x);
+    return true;
+}
+
+// A surface vertex (interface `it` at hit boundary bi, face) or a medium point (phase g) joined to
+// the eye, value β·f·|cos|·W·Tr.
+__device__ inline void connectVertex(PCtx& c, f3 p, bool surface, const Iface* it, uint32_t bi, uint32_t face, f3 n,
+                                     f3 d, float g, uint32_t region) {
+    Link k;
+    if (!linkObserver(c, p, k)) return;
+    const DScene& sc = c.sc;
+    float f[NW] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float cosX = 1.0f;
+    uint32_t start = region, departed = kNoBoundary, dface = ~0u;
+    if (!surface) {
+        const float ph = hgPhase(dot3(-d, k.dir), g);
+#pragma unroll
+        for (int i = 0; i < NW; ++i) f[i] = ph;
+    } else {
+        const DBoundary& b = sc.boundaries[bi];
+        bool any = false;
+#pragma unroll
+        for (int i = 0; i < NW; ++i) {
+            if (c.wl.pdf[i] == 0.0f) continue;
+            Iface at = *it;
+            if (i > 0) {
+                at.nFront = regionIndex(sc, b.front, c.wl.lambda[i]);
+                at.nBack = regionIndex(sc, b.back, c.wl.lambda[i]);
+                at.lambda = c.wl.lambda[i];
+            }
+            float pdf;
+            f[i] = evalScatter(sc, at, d, k.dir, false, pdf);
+            any = any || f[i] != 0.0f;
+        }
+        if (!any) return;
+        cosX = fabsf(dot3(k.dir, n));
+        start = dot3(k.dir, n) > 0.0f ? b.front : b.back;
+        departed = bi;
+        dface = face;
+    }
+    float Tr[NW];
+    if (!walkTransmittance(sc, p, k.dir, len3(k.q - p), start, departed, dface, kNoBoundary, sc.obs.region,
+                           c.wl.lambda, c.stack, kPBlock, Tr))
+        return;
+    float v[NW];
+#pragma unroll
+    for (int i = 0; i < NW; ++i) v[i] = c.beta[i] * f[i] * Tr[i] * (cosX * k.factor);
+    deposit(c, k.px, k.py, v);
+}
+
+struct PRay {
+    f3 o, d;
+    uint32_t region, departed, face;
+    bool distant;
+};
+
+// emitParticle (particle.slang:306-325): the sky or a light by power, then its emission.
+__device__ inline bool emitParticle(PCtx& c, bool observer, PRay& ray) {
+    const DScene& sc = c.sc;
+    const float pSky = sc.env.skyShare;
+    const bool fromSky = c.rng.uniform() < pSky;
+    uint32_t li = 0;
+    if (!fromSky) {
+        if (sc.nLights == 0) return false;
+        const float ul = c.rng.uniform();
+        while (li + 1 < sc.nLights && sc.lights[li].cdf <= ul) ++li;
+    }
+    float Le[NW];
+    if (fromSky || sc.lights[li].kind == 1) {
+        // emitDistant (particle.slang:234-265): a direction, then a point of a disc facing it.
+        const float sel = fromSky ? pSky : (1.0f - pSky) * sc.lights[li].pdf;
+        const float R = sc.env.emitRadius;
+        const float u2 = c.rng.uniform(), u1 = c.rng.uniform();
+        f3 toward;
+        float pdfDir;
+        if (fromSky) {
+            toward = uniformSphere(u1, u2);
+            pdfDir = 1.0f / (4.0f * kPiF);
+#pragma unroll
+            for (int i = 0; i < NW; ++i)
+                Le[i] = sc.env.sky == OWE_SKY_UNIFORM ? evalSpec(sc.env.skyRadiance, sc.table, c.wl.lambda[i]) : 0.0f;
+        } else {
+            toward = toWorld(makeFrame(ld3(sc.env.sunDir)), uniformConeOmc(u1, u2, sc.env.sunOneMinusCos));
+            pdfDir = 1.0f / sc.env.sunSolidAngle;
+#pragma unroll
+            for (int i = 0; i < NW; ++i) Le[i] = evalSpec(sc.env.sunRadiance, sc.table, c.wl.lambda[i]);
+        }
+        const f3 dir = -toward;
+        const Frame disc = makeFrame(dir);
+        const float v2 = c.rng.uniform(), v1 = c.rng.uniform();
+        float qx, qy;
+        concentricDisk(v1, v2, qx, qy);
+        ray.o = ld3(sc.env.center) - dir * (1.01f * R) + disc.s * (qx * R) + disc.t * (qy * R);
+        ray.d = dir;
+        ray.region = 0;
+        ray.departed = kNoBoundary;
+        ray.face = ~0u;
+        ray.distant = true;
+        const float w = kPiF * R * R / (sel * pdfDir);
+#pragma unroll
+        for (int i = 0; i < NW; ++i) c.beta[i] = Le[i] * w;
+    } else {
+        // emitFromArea (particle.slang:281-304): a point, a side, a cosine-distributed direction.
+        const float sel = (1.0f - pSky) * sc.lights[li].pdf;
+        const uint32_t bi = sc.lights[li].boundary;
+        const DBoundary& b = sc.boundaries[bi];
+        const DEmission& em = sc.emissions[b.emission];
+#pragma unroll
+        for (int i = 0; i < NW; ++i) Le[i] = evalSpec(em.radiance, sc.table, c.wl.lambda[i]);
+        const float u3 = c.rng.uniform(), u2 = c.rng.uniform(), u1 = c.rng.uniform();
+        (void)u3;  // a mesh's triangle choice; meshes are not in the light table
+        f3 pl, nl;
+        if (!sampleArea(b, u1, u2, pl, nl)) return false;
+        const f3 p = xfPoint(b.toWorld, pl), n = norm3(xfVector(b.toWorld, nl));
+        const float pdfA = 1.0f / b.area;
+        if (observer) {
+            // areaEmitterToObserver (particle.slang:267-279): the emitter seen directly.
+            Link k;
+            if (linkObserver(c, p, k)) {
+                const float cosL = dot3(n, k.dir);
+                if ((cosL > 0.0f && em.front) || (cosL < 0.0f && em.back)) {
+                    float Tr[NW];
+                    if (walkTransmittance(sc, p, k.dir, len3(k.q - p), cosL > 0.0f ? b.front : b.back, bi, ~0u,
+                                          kNoBoundary, sc.obs.region, c.wl.lambda, c.stack, kPBlock, Tr)) {
+                        const float s = fabsf(cosL) * k.factor / (sel * pdfA);
+                        float v[NW];
+#pragma unroll
+                        for (int i = 0; i < NW; ++i) v[i] = Le[i] * Tr[i] * s;
+                        deposit(c, k.px, k.py, v);
+                    }
+                }
+            }
+        }
+        float sidePdf = 1.0f;
+        bool front = em.front != 0;
+        if (em.front && em.back) {
+            front = c.rng.uniform() < 0.5f;
+            sidePdf = 0.5f;
+        }
+        const float c2 = c.rng.uniform(), c1 = c.rng.uniform();
+        ray.o = p;
+        ray.d = toWorld(makeFrame(front ? n : -n), cosineHemisphere(c1, c2));
+        ray.region = front ? b.front : b.back;
+        ray.departed = bi;
+        ray.face = ~0u;
+        ray.distant = false;
+        const float w = kPiF / (sel * pdfA * sidePdf);
+#pragma unroll
+        for (int i = 0; i < NW; ++i) c.beta[i] = Le[i] * w;
+    }
+    bool any = c.beta[0] != 0.0f;
+#pragma unroll
+    for (int i = 1; i < NW; ++i) {
+        if (c.wl.pdf[i] == 0.0f) c.beta[i] = 0.0f;
+        any = any || c.beta[i] != 0.0f;
+    }
+    return any;
+}
+
+__device__ inline bool roulette(PCtx& c, float q) {
+    if (c.rng.uniform() >= q) return false;
+    const float s = 1.0f / q;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) c.beta[i] *= s;
+    return true;
+}
+
+// traceParticle (particle.slang:356-427), without the hybrid partition.
+__device__ inline void traceParticle(PCtx& c) {
+    const DScene& sc = c.sc;
+    const DObserver& ob = sc.obs;
+    const bool observer = ob.kind == OWE_DETECTOR_OBSERVER;
+    const int maxDepth = c.set.maxDepth, rrDepth = c.set.rrDepth;
+    PRay ray;
+    if (!emitParticle(c, observer, ray)) return;
+    for (int depth = 0;; ++depth) {
+        Hit h;
+        bool found = intersectWorld(sc, ray.o, ray.d, __builtin_inff(), ray.departed, ray.face, kNoBoundary, c.stack,
+                                    kPBlock, h);
+        if (found && depth == 0 && ray.distant) {
+            const float back = 1e-5f * (1.0f + fmaxf(fabsf(h.p.x), fmaxf(fabsf(h.p.y), fabsf(h.p.z))));
+            Hit again;
+            if (intersectWorld(sc, h.p - ray.d * back, ray.d, 4.0f * back, kNoBoundary, ~0u, kNoBoundary, c.stack,
+                               kPBlock, again))
+                h = again;
+        }
+        c.segments++;
+        // sampleMedium (transport.slang:544-562).
+        const DMedium& med = sc.media[sc.regionMedium[ray.region]];
+        if (med.flags & 2) {
+            float sT[NW];
+#pragma unroll
+            for (int i = 0; i < NW; ++i)
+                sT[i] = evalSpec(med.absorption, sc.table, c.wl.lambda[i]) + evalSpec(med.scattering, sc.table, c.wl.lambda[i]);
+            const float tm = -logf(1.0f - c.rng.uniform()) / sT[0];
+            const float seg = found ? h.t : __builtin_inff();
+            if (tm < seg) {
+                const float pdf0 = sT[0] * expf(-sT[0] * tm);
+#pragma unroll
+                for (int i = 0; i < NW; ++i)
+                    c.beta[i] *= evalSpec(med.scattering, sc.table, c.wl.lambda[i]) * expf(-sT[i] * tm) / pdf0;
+                const f3 p = ray.o + ray.d * tm;
+                if (observer) connectVertex(c, p, false, nullptr, kNoBoundary, ~0u, mk3(0, 0, 0), ray.d, med.g, ray.region);
+                if (depth >= maxDepth) return;
+                const float u2 = c.rng.uniform(), u1 = c.rng.uniform();
+                f3 wi;
+                sampleHg(-ray.d, med.g, u1, u2, wi);
+                ray.o = p;
+                ray.d = wi;
+                ray.departed = kNoBoundary;
+                ray.face = ~0u;
+                if (depth >= rrDepth && !roulette(c, 0.95f)) return;
+                continue;
+            }
+            if (!found) return;
+            const float pass0 = expf(-sT[0] * seg);
+#pragma unroll
+            for (int i = 0; i < NW; ++i) c.beta[i] *= expf(-sT[i] * seg) / pass0;
+        } else if (med.flags & 1) {
+            if (!found) return;
+#pragma unroll
+            for (int i = 0; i < NW; ++i) c.beta[i] *= expf(-evalSpec(med.absorption, sc.table, c.wl.lambda[i]) * h.t);
+        }
+        if (!found) {
+            if (ray.region != 0) c.leaks++;
+            return;
+        }
+        // meetBoundary (transport.slang:600-615).
+        const uint32_t bi = h.boundary;
+        const DBoundary& b = sc.boundaries[bi];
+        const bool fromFront = dot3(ray.d, h.n) < 0.0f;
+        const uint32_t inc = fromFront ? b.front : b.back, other = fromFront ? b.back : b.front;
+        if (inc != ray.region) {
+            c.inconsistencies++;
+            ray.region = inc;
+        }
+        const DOptics& o = sc.optics[b.optics];
+        const uint32_t hitFace = h.slot != kEmptyChild ? h.face : ~0u;
+        if (o.type == OWE_SURF_NULL) {
+            ray.region = other;
+            ray.o = h.p;
+            ray.departed = bi;
+            ray.face = hitFace;
+            if (depth >= 4 * maxDepth) return;
+            continue;
+        }
+        if (!observer && fromFront && bi == ob.sensor) {
+            // sensorPixel (particle.slang:98-114): the flux per pixel area.
+            const float W = float(ob.width), H = float(ob.height);
+            float fx = (h.pLocal.x / ob.halfX + 1.0f) * 0.5f * W;
+            float fy = (1.0f - h.pLocal.y / ob.halfY) * 0.5f * H;
+            if (ob.flipX) fx = W - fx;
+            if (ob.flipY) fy = H - fy;
+            float ox, oy;
+            filterOffset(c, ox, oy);
+            fx += ox;
+            fy += oy;
+            if (fx >= 0.0f && fy >= 0.0f && fx < W && fy < H) {
+                const float inv = 1.0f / ((2.0f * ob.halfX / W) * (2.0f * ob.halfY / H));
+                float v[NW];
+#pragma unroll
+                for (int i = 0; i < NW; ++i) v[i] = c.beta[i] * inv;
+                deposit(c, min(int(fx), ob.width - 1), min(int(fy), ob.height - 1), v);
+            }
+        }
+        if (o.type == OWE_SURF_DETECTOR || o.type == OWE_SURF_ABSORBER) return;
+        const float hero = c.wl.lambda[0];
+        const Iface it{&o, h.n, regionIndex(sc, b.front, hero), regionIndex(sc, b.back, hero), hero, c.set.fresnelFloor};
+        if (!o.delta && observer) connectVertex(c, h.p, true, &it, bi, hitFace, h.n, ray.d, 0.0f, ray.region);
+        if (depth >= maxDepth) return;
+        auto at = [&](int i) {
+            Iface r = it;
+            r.nFront = regionIndex(sc, b.front, c.wl.lambda[i]);
+            r.nBack = regionIndex(sc, b.back, c.wl.lambda[i]);
+            r.lambda = c.wl.lambda[i];
+            return r;
+        };
+        // terminateIfDispersive (transport.slang:639-648).
+        if (o.type == OWE_SURF_DIELECTRIC && !secondaryTerminated(c.wl)) {
+            const float a0 = it.nBack / it.nFront;
+#pragma unroll
+            for (int i = 1; i < NW; ++i) {
+                if (c.wl.pdf[i] == 0.0f) continue;
+                const Iface ai = at(i);
+                if (fabsf(a0 - ai.nBack / ai.nFront) > 1e-12f * fabsf(a0)) {
+                    c.wl.pdf[0] /= NW;
+                    c.wl.pdf[1] = c.wl.pdf[2] = c.wl.pdf[3] = 0.0f;
+                    c.beta[1] = c.beta[2] = c.beta[3] = 0.0f;
+                    break;
+                }
+            }
+        }
+        // scatterAtSurface in importance mode (transport.slang:671-684).
+        const float u3 = c.rng.uniform(), u2 = c.rng.uniform(), u1 = c.rng.uniform();
+        SSample s;
+        if (!sampleScatter(sc, it, ray.d, u1, u2, u3, false, s) || s.weight <= 0.0f) return;
+        c.beta[0] *= s.weight;
+#pragma unroll
+        for (int i = 1; i < NW; ++i)
+            if (c.wl.pdf[i] != 0.0f) c.beta[i] *= secondaryWeight(sc, at(i), ray.d, s, false);
+        if (s.transmitted) ray.region = other;
+        ray.o = h.p;
+        ray.d = s.wi;
+        ray.departed = bi;
+        ray.face = hitFace;
+        // Particles carry flux: roulette against the event's own throughput.
+        if (depth >= rrDepth && s.weight < 1.0f && !roulette(c, s.weight)) return;
+    }
+}
+
+__device__ inline uint32_t waveSum(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+template <int kCap>
+__global__ __launch_bounds__(kPBlock) void particleKernel(DScene sc, Settings set, ParticleBatch pb) {
+    __shared__ uint32_t stack[kCap * kPBlock];
+    const uint32_t i = blockIdx.x * kPBlock + threadIdx.x;
+    PCtx c{sc, set, pb, stack + threadIdx.x};
+    c.sum[0] = c.sum[1] = c.sum[2] = 0.0f;
+    c.segments = c.inconsistencies = c.leaks = 0;
+    const bool live = i < pb.count;
+    if (live) {
+        const uint64_t index = pb.index ? pb.index[i] : pb.base + i;
+        c.rng.seed(hashPair(set.seed ^ 0x5bd1e995ull, index), pb.pass);
+        const float u = c.rng.uniform();
+        if (set.spectral == OWE_SPECTRAL_UNIFORM1) {
+            const float l = set.lambdaMin + (set.lambdaMax - set.lambdaMin) * u;
+#pragma unroll
+            for (int k = 0; k < NW; ++k) c.wl.lambda[k] = l;
+            c.wl.pdf[0] = 1.0f / (set.lambdaMax - set.lambdaMin) / NW;
+            c.wl.pdf[1] = c.wl.pdf[2] = c.wl.pdf[3] = 0.0f;
+        } else {
+#pragma unroll
+            for (int k = 0; k < NW; ++k) {
+                float up = u + float(k) / NW;
+                if (up >= 1.0f) up -= 1.0f;
+                c.wl.lambda[k] = sampleVisibleF(up);
+                c.wl.pdf[k] = visiblePdfF(c.wl.lambda[k]);
+            }
+        }
+        traceParticle(c);
+        if (pb.perParticle) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) pb.perParticle[3 * size_t(i) + k] = c.sum[k];
+        }
+    }
+    const uint32_t n[4] = {waveSum(live ? 1u : 0u), waveSum(c.segments), waveSum(c.inconsistencies), waveSum(c.leaks)};
+    if (__lane_id() == 0)
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (n[k]) atomicAdd(&pb.stats[k], (unsigned long long)n[k]);
+}
+
+}  // namespace
+
+void launchParticles(const DScene& sc, const Settings& set, const ParticleBatch& b, uint32_t stackCap,
+                     hipStream_t stream) {
+    if (b.count == 0) return;
+    const dim3 grid{(b.count + kPBlock - 1) / kPBlock}, block{kPBlock};
+    if (stackCap <= 16) hipLaunchKernelGGL(particleKernel<16>, grid, block, 0, stream, sc, set, b);
+    else if (stackCap <= 32) hipLaunchKernelGGL(particleKernel<32>, grid, block, 0, stream, sc, set, b);
+    else hipLaunchKernelGGL(particleKernel<48>, grid, block, 0, stream, sc, set, b);
+    OWE_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace owehip

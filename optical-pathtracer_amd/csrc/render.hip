@@ -24,7 +24,8 @@
 #include <vector>
 
 #include "common.hpp"
-#include "transport_dev.hpp"
+#include "particle.hpp"
+#include "shading_dev.hpp"
 
 using namespace owehip;
 
@@ -35,12 +36,6 @@ constexpr int kBlock = 256;
 constexpr double kFilmScale = 4294967296.0;  // 2^32 units per unit of XYZ
 
 enum : uint32_t { kFlagSpecular = 1u, kFlagTerminated = 2u };
-
-struct Settings {
-    uint64_t seed;
-    int maxDepth, rrDepth, spectral, pad;
-    float lambdaMin, lambdaMax, fresnelFloor, pad2;
-};
 
 // Structure-of-arrays path state, one slot per path of the current chunk.
 struct Paths {
@@ -83,13 +78,6 @@ __device__ inline WL loadWL(const float4 l, uint32_t flags, const Settings& s) {
     return w;
 }
 
-__device__ inline float get(const float4& v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w)); }
-__device__ inline void setc(float4& v, int i, float x) {
-    if (i == 0) v.x = x;
-    else if (i == 1) v.y = x;
-    else if (i == 2) v.z = x;
-    else v.w = x;
-}
 
 // ---------------------------------------------------------------- generate
 
@@ -139,6 +127,52 @@ __global__ __launch_bounds__(kBlock) void generateKernel(DScene sc, Settings set
         const float r = sqrtf(-2.0f * logf(1.0f - u1)), phi = 2.0f * kPiF * u2;
         fx = r * cosf(phi) * ob.sigma;
         fy = r * sinf(phi) * ob.sigma;
+    }
+    if (ob.kind == OWE_DETECTOR_SENSOR) {
+        // SurfaceSensor::generate (detector.cpp:110-149) without the focus disk: a point of the
+        // sensor, a direction over its front (cosine or toward the aim disk), weight cos/pdf.
+        float sxp = px + fx, syp = py + fy, weight = 0.0f;
+        const float W = float(ob.width), H = float(ob.height);
+        f3 p = mk3(0, 0, 0), dir = ld3(ob.normal);
+        if (sxp >= 0.0f && syp >= 0.0f && sxp < W && syp < H) {
+            if (ob.flipX) sxp = W - sxp;
+            if (ob.flipY) syp = H - syp;
+            p = xfPoint(ob.toWorld, mk3((2.0f * sxp / W - 1.0f) * ob.halfX, (1.0f - 2.0f * syp / H) * ob.halfY, 0.0f));
+            const f3 nrm = ld3(ob.normal);
+            if (ob.hasAim) {
+                const bool useAim = ob.aimShare >= 1.0f || rng.uniform() < ob.aimShare;
+                const float u2 = rng.uniform(), u1 = rng.uniform();
+                float dx, dy;
+                concentricDisk(u1, u2, dx, dy);
+                const Frame af = makeFrame(ld3(ob.aimNormal));
+                const f3 w = ld3(ob.aimCenter) + (af.s * dx + af.t * dy) * ob.aimRadius - p;
+                dir = w / len3(w);
+                if (!useAim) dir = toWorld(makeFrame(nrm), cosineHemisphere(u1, u2));
+                const float cosS = dot3(dir, nrm);
+                if (cosS > 0.0f) {
+                    const float pdfW = ob.aimShare * diskAimPdf(p, dir, ld3(ob.aimCenter), ld3(ob.aimNormal), ob.aimRadius,
+                                                                useAim) +
+                                       (1.0f - ob.aimShare) * cosS * kInvPiF;
+                    if (pdfW > 0.0f) weight = cosS / pdfW;
+                }
+            } else {
+                const float u2 = rng.uniform(), u1 = rng.uniform();
+                dir = toWorld(makeFrame(nrm), cosineHemisphere(u1, u2));
+                weight = kPiF;
+            }
+        }
+        // A sample the sensor does not measure (weight 0) still takes its slot and adds nothing.
+        P.ray0[i] = make_float4(p.x, p.y, p.z, 0.0f);
+        P.ray1[i] = make_float4(dir.x, dir.y, dir.z, weight);
+        P.beta[i] = weight > 0.0f ? beta : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        P.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        P.lam[i] = lam;
+        P.prevP[i] = make_float4(p.x, p.y, p.z, 0.0f);
+        P.info[i] = make_uint4(uint32_t(pixel), kFlagSpecular << 16, ob.region, ob.sensor);
+        P.face[i] = ~0u;
+        P.rng[i] = make_uint2(uint32_t(rng.state), uint32_t(rng.state >> 32));
+        P.queue[0][i] = i;
+        return;
     }
     const float sx = (2.0f * (px + fx) / float(ob.width) - 1.0f) * ob.tanX;
     const float sy = (1.0f - 2.0f * (py + fy) / float(ob.height)) * ob.tanY;
@@ -200,202 +234,6 @@ struct ShadeCtx {
     float shDist;
     uint32_t shStart, shLightRegion, shExclude, shDeparted, shFace;
 };
-
-__device__ inline float regionIndex(const DScene& sc, uint32_t region, float l) {
-    return mediumIndex(sc.media[sc.regionMedium[region]], l);
-}
-
-struct Iface {
-    const DOptics* o;
-    f3 n;
-    float nFront, nBack, lambda, floor;
-};
-
-struct SSample {
-    f3 wi;
-    float weight, pdf;
-    bool delta, transmitted;
-};
-
-__device__ inline Cx conductorEta(const DScene& sc, const DOptics& o, float lambda, float nMed) {
-    const float inv = 1.0f / nMed;
-    return {evalSpec(o.condN, sc.table, lambda) * inv, evalSpec(o.condK, sc.table, lambda) * inv};
-}
-
-__device__ inline bool specularReflection(const Frame& f, f3 wo, float weight, SSample& s) {
-    s.wi = toWorld(f, mk3(-wo.x, -wo.y, wo.z));
-    s.weight = weight;
-    s.delta = true;
-    return weight > 0.0f;
-}
-
-// sampleScatter (src/owe/transport/scattering.cpp:65-157), radiance transport.
-__device__ inline bool sampleScatter(const DScene& sc, const Iface& it, f3 d, float uc, float u1, float u2, SSample& s) {
-    const DOptics& o = *it.o;
-    const Frame frame = makeFrame(it.n);
-    const f3 wo = toLocal(frame, -d);
-    const bool fromFront = wo.z > 0.0f;
-    s = SSample{d, 0.0f, 0.0f, false, false};
-    switch (o.type) {
-        case OWE_SURF_DIELECTRIC: {
-            const float eta = it.nBack / it.nFront;
-            if (o.roughness >= 1e-3f && eta != 1.0f) {
-                Lobe l;
-                if (!sampleRoughDielectric(ggxAlpha(o.roughness), wo, eta, uc, u1, u2, true, l)) return false;
-                s.wi = toWorld(frame, l.wi);
-                s.weight = l.weight;
-                s.pdf = l.pdf;
-                s.transmitted = l.transmitted;
-                return true;
-            }
-            const float ni = fromFront ? it.nFront : it.nBack, nt = fromFront ? it.nBack : it.nFront;
-            const float R = fresnelDielectric(wo.z, eta);
-            const float pR = it.floor > 0.0f && R < 1.0f ? clampf(R, it.floor, 1.0f - it.floor) : R;
-            s.delta = true;
-            if (uc < pR) {
-                specularReflection(frame, wo, R / pR, s);
-                return true;
-            }
-            if (!refractDir(d, it.n, ni, nt, s.wi)) return false;
-            s.weight = (1.0f - R) / (1.0f - pR) * sqf(ni / nt);
-            s.transmitted = true;
-            return true;
-        }
-        case OWE_SURF_DIFFUSE: {
-            if (!fromFront && o.backAbsorbs) return false;
-            const float albedo = evalSpec(o.reflectance, sc.table, it.lambda);
-            if (albedo <= 0.0f) return false;
-            f3 w = cosineHemisphere(u1, u2);
-            if (!fromFront) w.z = -w.z;
-            s.wi = toWorld(frame, w);
-            s.pdf = fabsf(w.z) * kInvPiF;
-            s.weight = albedo;
-            return s.pdf > 0.0f;
-        }
-        case OWE_SURF_MIRROR:
-            if (!fromFront && o.backAbsorbs) return false;
-            return specularReflection(frame, wo, evalSpec(o.reflectance, sc.table, it.lambda), s);
-        case OWE_SURF_CONDUCTOR: {
-            if (!fromFront && o.backAbsorbs) return false;
-            const Cx eta = conductorEta(sc, o, it.lambda, fromFront ? it.nFront : it.nBack);
-            const float tint = evalSpec(o.reflectance, sc.table, it.lambda);
-            const f3 w = fromFront ? wo : mk3(wo.x, wo.y, -wo.z);
-            if (o.roughness < 1e-3f) return specularReflection(frame, wo, fresnelConductor(w.z, eta) * tint, s);
-            Lobe l;
-            if (!sampleRoughConductor(ggxAlpha(o.roughness), w, eta, tint, u1, u2, l)) return false;
-            if (!fromFront) l.wi.z = -l.wi.z;
-            s.wi = toWorld(frame, l.wi);
-            s.weight = l.weight;
-            s.pdf = l.pdf;
-            return true;
-        }
-        default: return false;
-    }
-}
-
-// evalScatter (scattering.cpp:159-188).
-__device__ inline float evalScatter(const DScene& sc, const Iface& it, f3 d, f3 wiW, float& pdf) {
-    pdf = 0.0f;
-    const DOptics& o = *it.o;
-    if (o.delta) return 0.0f;
-    const Frame frame = makeFrame(it.n);
-    f3 wo = toLocal(frame, -d), wi = toLocal(frame, wiW);
-    const bool fromFront = wo.z > 0.0f;
-    switch (o.type) {
-        case OWE_SURF_DIFFUSE:
-            if (!fromFront && o.backAbsorbs) return 0.0f;
-            if (!sameHemi(wo, wi)) return 0.0f;
-            pdf = fabsf(wi.z) * kInvPiF;
-            return evalSpec(o.reflectance, sc.table, it.lambda) * kInvPiF;
-        case OWE_SURF_CONDUCTOR: {
-            if (!fromFront && o.backAbsorbs) return 0.0f;
-            if (!sameHemi(wo, wi)) return 0.0f;
-            if (!fromFront) {
-                wo.z = -wo.z;
-                wi.z = -wi.z;
-            }
-            const Cx eta = conductorEta(sc, o, it.lambda, fromFront ? it.nFront : it.nBack);
-            return evalRoughConductor(ggxAlpha(o.roughness), wo, wi, eta, evalSpec(o.reflectance, sc.table, it.lambda), pdf);
-        }
-        case OWE_SURF_DIELECTRIC:
-            return evalRoughDielectric(ggxAlpha(o.roughness), wo, wi, it.nBack / it.nFront, true, pdf);
-        default: return 0.0f;
-    }
-}
-
-__device__ inline float secondaryWeight(const DScene& sc, const Iface& it, f3 d, const SSample& s) {
-    const DOptics& o = *it.o;
-    if (!s.delta) {
-        float pdf;
-        const float f = evalScatter(sc, it, d, s.wi, pdf);
-        return s.pdf > 0.0f ? f * fabsf(dot3(s.wi, it.n)) / s.pdf : 0.0f;
-    }
-    if (o.type == OWE_SURF_DIELECTRIC) return s.weight;
-    if (o.type == OWE_SURF_MIRROR) return evalSpec(o.reflectance, sc.table, it.lambda);
-    if (o.type == OWE_SURF_CONDUCTOR) {
-        const float nMed = dot3(d, it.n) < 0.0f ? it.nFront : it.nBack;
-        return fresnelConductor(fabsf(dot3(d, it.n)), conductorEta(sc, o, it.lambda, nMed)) *
-               evalSpec(o.reflectance, sc.table, it.lambda);
-    }
-    return 0.0f;
-}
-
-// Solid-angle pdf of a light point on analytic boundary b seen from ref (Shape::pdfFrom).
-__device__ inline float sphereConeSize(f3 ref, float R) {
-    const float dc2 = dot3(ref, ref);
-    if (dc2 <= R * R * (1.0f + 1e-6f)) return 0.0f;
-    const float s2 = R * R / dc2;
-    return s2 / (1.0f + sqrt0f(1.0f - s2));
-}
-__device__ inline bool areaToSolidAngle(f3 ref, f3 p, f3 n, float area, float& pdfW) {
-    pdfW = 0.0f;
-    const f3 w = p - ref;
-    const float d2 = dot3(w, w);
-    if (!(d2 > 0.0f)) return false;
-    const float cosL = fabsf(dot3(n, w)) / sqrtf(d2);
-    if (cosL <= 1e-7f) return false;
-    pdfW = d2 / (cosL * area);
-    return true;
-}
-__device__ inline float pdfFrom(const DBoundary& b, f3 refLocal, f3 pLocal, f3 nLocal) {
-    if (b.shape == OWE_SHAPE_SPHERE) {
-        const float omc = sphereConeSize(refLocal, b.a);
-        if (omc > 0.0f) return 1.0f / (2.0f * kPiF * omc);
-    }
-    float pdfW;
-    return b.area > 0.0f && areaToSolidAngle(refLocal, pLocal, nLocal, b.area, pdfW) ? pdfW : 0.0f;
-}
-__device__ inline bool sampleFrom(const DBoundary& b, f3 ref, float u1, float u2, float u3, f3& p, f3& n, float& pdfW) {
-    if (b.shape == OWE_SHAPE_SPHERE) {
-        const float omc = sphereConeSize(ref, b.a);
-        if (omc > 0.0f) {
-            const f3 axis = -ref / sqrtf(dot3(ref, ref));
-            const Frame f = makeFrame(axis);
-            const f3 wl = uniformConeOmc(u1, u2, omc);
-            const f3 w = f.s * wl.x + f.t * wl.y + axis * wl.z;
-            float t0, t1;
-            if (sphereRoots(ref, w, b.a, t0, t1) && (t0 > 0.0f || t1 > 0.0f)) p = ref + w * (t0 > 0.0f ? t0 : t1);
-            else p = norm3(ref + w * dot3(-ref, w)) * b.a;
-            n = norm3(p);
-            pdfW = 1.0f / (2.0f * kPiF * omc);
-            return true;
-        }
-        n = uniformSphere(u1, u2);
-        p = n * b.a;
-        return areaToSolidAngle(ref, p, n, b.area, pdfW);
-    }
-    n = mk3(0, 0, 1);
-    if (b.aperture == OWE_APERTURE_DISK) {
-        const float r = sqrtf(b.b * b.b + u1 * (b.a * b.a - b.b * b.b)), phi = 2.0f * kPiF * u2;
-        p = mk3(r * cosf(phi), r * sinf(phi), 0.0f);
-    } else if (b.aperture == OWE_APERTURE_RECT) {
-        p = mk3((2.0f * u1 - 1.0f) * b.a, (2.0f * u2 - 1.0f) * b.b, 0.0f);
-    } else {
-        const float r = sqrtf(u1), phi = 2.0f * kPiF * u2;
-        p = mk3(b.a * r * cosf(phi), b.b * r * sinf(phi), 0.0f);
-    }
-    return areaToSolidAngle(ref, p, n, b.area, pdfW);
-}
 
 // Next-event estimation at point p (sampleLightFrom + directLighting, transport.cpp:35-70, 323-348):
 // draws the light, evaluates everything but the transmittance, and leaves a shadow ray.
@@ -462,7 +300,7 @@ __device__ inline void prepareDirect(ShadeCtx& c, f3 p, bool surface, const Ifac
                 at.lambda = c.wl.lambda[i];
             }
             float pdf;
-            f[i] = evalScatter(sc, at, c.d, wi, pdf);
+            f[i] = evalScatter(sc, at, c.d, wi, true, pdf);
             if (i == 0) pdfScatter = pdf;
             nz = nz || f[i] != 0.0f;
         }
@@ -633,11 +471,11 @@ __device__ inline bool shadeOne(ShadeCtx& c, const uint4 hraw, uint32_t* counter
     if (!o.delta) prepareDirect(c, p, true, &it, bi, hraw.z, hraw.w, n, 0.0f);
     const float u3 = c.rng.uniform(), u2 = c.rng.uniform(), u1 = c.rng.uniform();
     SSample s;
-    if (!sampleScatter(sc, it, c.d, u1, u2, u3, s) || s.weight <= 0.0f) return false;
+    if (!sampleScatter(sc, it, c.d, u1, u2, u3, true, s) || s.weight <= 0.0f) return false;
     c.beta.x *= s.weight;
 #pragma unroll
     for (int i = 1; i < NW; ++i)
-        if (c.wl.pdf[i] != 0.0f) setc(c.beta, i, get(c.beta, i) * secondaryWeight(sc, at(i), c.d, s));
+        if (c.wl.pdf[i] != 0.0f) setc(c.beta, i, get(c.beta, i) * secondaryWeight(sc, at(i), c.d, s, true));
     if (s.transmitted) c.region = other;
     if (s.delta) c.flags |= kFlagSpecular;
     else c.flags &= ~kFlagSpecular;
@@ -878,6 +716,14 @@ struct owe_hip_renderer {
     uint32_t listCapacity = 0;
     int passes = 0;
     long long totalSpp = 0;
+    int integrator = OWE_INTEGRATOR_PATH;
+    // Light integrator: the exact splat bins, the pass's counters, and the particles emitted so far.
+    unsigned long long* lightFilm = nullptr;
+    unsigned long long* particleStats = nullptr;
+    double particleCount = 0;
+    uint64_t* particleIndex = nullptr;
+    double* perParticle = nullptr;
+    uint32_t particleListCapacity = 0;
     owe_transport_stats stats{};
     hipStream_t stream = nullptr;
     uint32_t stackCap = 16;
@@ -1039,6 +885,36 @@ void runPaths(owe_hip_renderer* r, uint64_t base, uint32_t count, uint32_t spp, 
     }
 }
 
+// One light-tracing pass of `total` particles (W·H·ppp), in launches of at most `capacity`; with a
+// particle list (device) and perParticle, the listed particles only, summed per particle.
+void runParticles(owe_hip_renderer* r, uint64_t total, uint32_t pass, const uint64_t* list = nullptr,
+                  double* perParticle = nullptr) {
+    if (!list) r->particleCount += double(total);
+    if (!(r->host.emittedPower > 0)) return;  // nothing emits: the particle estimate is zero
+    OWE_HIP_CHECK(hipMemsetAsync(r->particleStats, 0, 4 * sizeof(unsigned long long), r->stream));
+    for (uint64_t base = 0; base < total; base += r->capacity) {
+        ParticleBatch b{};
+        b.base = list ? 0 : base;
+        b.index = list ? list + base : nullptr;
+        b.count = uint32_t(std::min<uint64_t>(r->capacity, total - base));
+        b.pass = pass;
+        b.film = r->lightFilm;
+        b.perParticle = perParticle ? perParticle + 3 * base : nullptr;
+        b.stats = r->particleStats;
+        b.invNWY = r->invNWY;
+        timed(r, kShade, [&] { launchParticles(r->dev, r->set, b, r->stackCap, r->stream); });
+    }
+    unsigned long long st[4];
+    OWE_HIP_CHECK(hipMemcpyAsync(st, r->particleStats, sizeof st, hipMemcpyDeviceToHost, r->stream));
+    OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
+    if (r->timing) collectTimes(r, {kShade});
+    if (list) return;
+    r->stats.paths += st[0];
+    r->stats.segments += st[1];
+    r->stats.inconsistencies += st[2];
+    r->stats.leaks += st[3];
+}
+
 }  // namespace
 }  // namespace owehip
 
@@ -1055,8 +931,8 @@ int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render
         const owe_render_settings& s = *settings;
         require(s.max_depth >= 0 && s.rr_depth >= 0, "depths must be non-negative");
         require(s.max_depth <= 4096, "max depth above 4096 is not supported (depth is kept in 16 bits)");
-        if (s.integrator != OWE_INTEGRATOR_PATH)
-            fail(OWE_HIP_ERR_UNSUPPORTED, "unknown integrator (this backend implements path)");
+        if (s.integrator != OWE_INTEGRATOR_PATH && s.integrator != OWE_INTEGRATOR_LIGHT)
+            fail(OWE_HIP_ERR_UNSUPPORTED, "unknown integrator (this backend implements path and light)");
         require(s.spectral == OWE_SPECTRAL_HERO4 || s.spectral == OWE_SPECTRAL_UNIFORM1, "unknown spectral mode");
         if (s.spectral == OWE_SPECTRAL_UNIFORM1)
             require(s.lambda_min >= 360.0 && s.lambda_max <= 830.0 && s.lambda_min < s.lambda_max,
@@ -1065,6 +941,7 @@ int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render
         selectDevice(device);
         r = new owe_hip_renderer;
         r->device = device;
+        r->integrator = s.integrator;
         r->host = flattenScene(*scene);
         const HostScene& h = r->host;
         if (h.bvh.stackBound > 48)
@@ -1119,7 +996,14 @@ int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render
         const uint64_t pixels = uint64_t(h.obs.width) * uint64_t(h.obs.height);
         r->film = devAlloc<unsigned long long>(r, 3 * pixels);
         OWE_HIP_CHECK(hipMemset(r->film, 0, 3 * pixels * sizeof(unsigned long long)));
-        allocPaths(r, uint32_t(std::min<uint64_t>(pixels * 64, 1u << 22)));
+        if (r->integrator == OWE_INTEGRATOR_PATH) {
+            allocPaths(r, uint32_t(std::min<uint64_t>(pixels * 64, 1u << 22)));
+        } else {
+            r->capacity = uint32_t(std::min<uint64_t>(pixels * 64, 1u << 22));  // particles per launch
+            r->lightFilm = devAlloc<unsigned long long>(r, 3 * kSplatBins * pixels);
+            OWE_HIP_CHECK(hipMemset(r->lightFilm, 0, 3 * kSplatBins * pixels * sizeof(unsigned long long)));
+            r->particleStats = devAlloc<unsigned long long>(r, 4);
+        }
         *out = r;
     });
     if (st != OWE_HIP_OK && r) {
@@ -1175,6 +1059,12 @@ int owe_hip_renderer_pass(owe_hip_renderer* r, int spp) {
         if (spp <= 0) return;
         OWE_HIP_CHECK(hipSetDevice(r->device));
         const uint64_t total = uint64_t(r->host.obs.width) * uint64_t(r->host.obs.height) * uint64_t(spp);
+        if (r->integrator == OWE_INTEGRATOR_LIGHT) {
+            runParticles(r, total, uint32_t(r->passes));
+            r->passes++;
+            r->totalSpp += spp;
+            return;
+        }
         for (uint64_t base = 0; base < total; base += r->capacity) {
             const uint32_t count = uint32_t(std::min<uint64_t>(r->capacity, total - base));
             runPaths(r, base, count, uint32_t(spp), uint32_t(r->passes), false, nullptr);
@@ -1194,7 +1084,21 @@ int owe_hip_renderer_resolve(owe_hip_renderer* r, double* xyz) {
         OWE_HIP_CHECK(hipMemcpyAsync(film.data(), r->film, n * sizeof(long long), hipMemcpyDeviceToHost, r->stream));
         OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
         const double norm = r->totalSpp > 0 ? 1.0 / (kFilmScale * double(r->totalSpp)) : 0.0;
-        for (size_t i = 0; i < n; ++i) xyz[i] = double(film[i]) * norm;
+        for (size_t i = 0; i < n; ++i) xyz[i] = r->integrator == OWE_INTEGRATOR_PATH ? double(film[i]) * norm : 0.0;
+        if (r->lightFilm) {
+            // Particle splats ÷ particles emitted (gpu_renderer_film.cpp:17-20): each channel's bins
+            // are integers in units of 2^(kSplatExp0 + 32 b).
+            std::vector<unsigned long long> bins(n * kSplatBins);
+            OWE_HIP_CHECK(hipMemcpyAsync(bins.data(), r->lightFilm, bins.size() * sizeof(unsigned long long),
+                                         hipMemcpyDeviceToHost, r->stream));
+            OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
+            const double lightNorm = r->particleCount > 0 ? 1.0 / r->particleCount : 0.0;
+            for (size_t i = 0; i < n; ++i) {
+                double v = 0;
+                for (int b = kSplatBins - 1; b >= 0; --b) v += std::ldexp(double(bins[i * kSplatBins + b]), kSplatExp0 + 32 * b);
+                xyz[i] += v * lightNorm;
+            }
+        }
     });
 }
 
@@ -1219,9 +1123,12 @@ int owe_hip_renderer_reset(owe_hip_renderer* r) {
         OWE_HIP_CHECK(hipSetDevice(r->device));
         const size_t n = 3 * size_t(r->host.obs.width) * size_t(r->host.obs.height);
         OWE_HIP_CHECK(hipMemsetAsync(r->film, 0, n * sizeof(unsigned long long), r->stream));
+        if (r->lightFilm)
+            OWE_HIP_CHECK(hipMemsetAsync(r->lightFilm, 0, n * kSplatBins * sizeof(unsigned long long), r->stream));
         OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
         r->passes = 0;
         r->totalSpp = 0;
+        r->particleCount = 0;
         r->stats = {};
     });
 }
@@ -1230,6 +1137,7 @@ int owe_hip_renderer_camera_paths(owe_hip_renderer* r, int spp, int pass, const 
                                   uint64_t n, double* xyz) {
     return guard([&] {
         require(r != nullptr, "renderer is null");
+        require(r->integrator == OWE_INTEGRATOR_PATH, "camera paths need a path-integrator renderer");
         require(spp > 0 && pass >= 0, "spp must be positive and pass non-negative");
         if (n == 0) return;
         require(pixel != nullptr && sample != nullptr && xyz != nullptr, "null buffer");
@@ -1250,6 +1158,33 @@ int owe_hip_renderer_camera_paths(owe_hip_renderer* r, int spp, int pass, const 
             runPaths(r, 0, count, uint32_t(spp), uint32_t(pass), true, r->perPath);
             OWE_HIP_CHECK(hipMemcpyAsync(xyz + 3 * base, r->perPath, 3 * size_t(count) * sizeof(double), hipMemcpyDeviceToHost,
                                          r->stream));
+            OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
+        }
+    });
+}
+
+int owe_hip_renderer_particles(owe_hip_renderer* r, int pass, const uint64_t* index, uint64_t n, double* xyz) {
+    return guard([&] {
+        require(r != nullptr, "renderer is null");
+        require(r->integrator == OWE_INTEGRATOR_LIGHT, "particles need a light-integrator renderer");
+        require(pass >= 0, "pass must be non-negative");
+        if (n == 0) return;
+        require(index != nullptr && xyz != nullptr, "null buffer");
+        OWE_HIP_CHECK(hipSetDevice(r->device));
+        if (r->particleListCapacity < r->capacity) {
+            r->particleIndex = devAlloc<uint64_t>(r, r->capacity);
+            r->perParticle = devAlloc<double>(r, 3 * size_t(r->capacity));
+            r->particleListCapacity = r->capacity;
+        }
+        std::fill(xyz, xyz + 3 * n, 0.0);
+        for (uint64_t base = 0; base < n; base += r->capacity) {
+            const uint32_t count = uint32_t(std::min<uint64_t>(r->capacity, n - base));
+            OWE_HIP_CHECK(hipMemcpyAsync(r->particleIndex, index + base, count * sizeof(uint64_t), hipMemcpyHostToDevice,
+                                         r->stream));
+            OWE_HIP_CHECK(hipMemsetAsync(r->perParticle, 0, 3 * size_t(count) * sizeof(double), r->stream));
+            runParticles(r, count, uint32_t(pass), r->particleIndex, r->perParticle);
+            OWE_HIP_CHECK(hipMemcpyAsync(xyz + 3 * base, r->perParticle, 3 * size_t(count) * sizeof(double),
+                                         hipMemcpyDeviceToHost, r->stream));
             OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
         }
     });

@@ -138,12 +138,28 @@ HostScene flattenScene(const owe_scene& s) {
             "null scene arrays");
     require(s.n_table == 0 || s.table, "null spectrum table");
     require(s.n_triangles == 0 || (s.triangles && s.vertices), "null mesh arrays");
-    // Observer and rebase origin.
+    // Detector and rebase origin: the eye, or the sensor's centre.
     const owe_observer& ob = s.observer;
-    require(ob.width > 0 && ob.height > 0, "observer resolution must be positive");
-    require(ob.region < s.n_regions, "observer region out of range");
-    require(ob.fov_y > 0 && ob.fov_y < kPi, "observer field of view must lie in (0, pi)");
-    for (int k = 0; k < 3; ++k) h.origin[k] = ob.position[k];
+    require(ob.width > 0 && ob.height > 0, "detector resolution must be positive");
+    require(ob.kind == OWE_DETECTOR_OBSERVER || ob.kind == OWE_DETECTOR_SENSOR, "unknown detector kind");
+    require(ob.pixel_sigma >= 0, "pixel filter width must be non-negative");
+    if (ob.kind == OWE_DETECTOR_OBSERVER) {
+        require(ob.region < s.n_regions, "observer region out of range");
+        require(ob.fov_y > 0 && ob.fov_y < kPi, "observer field of view must lie in (0, pi)");
+        for (int k = 0; k < 3; ++k) h.origin[k] = ob.position[k];
+    } else {
+        // SurfaceSensor::prepare (detector.cpp:100-108): a detector surface, or a diffuse screen.
+        require(ob.sensor_boundary < s.n_boundaries, "sensor boundary out of range");
+        const owe_boundary& sb = s.boundaries[ob.sensor_boundary];
+        require(sb.shape == OWE_SHAPE_PLANE, "a sensor is a plane boundary");
+        require(sb.optics >= 0 && uint32_t(sb.optics) < s.n_optics &&
+                    (s.optics[sb.optics].type == OWE_SURF_DETECTOR || s.optics[sb.optics].type == OWE_SURF_DIFFUSE),
+                "sensor boundary is not a detector surface");
+        require(ob.half_x > 0 && ob.half_y > 0, "sensor half sizes must be positive");
+        require(!ob.has_aim || (ob.aim_radius > 0 && ob.aim_share > 0 && ob.aim_share <= 1),
+                "sensor aim needs a positive radius and a share in (0, 1]");
+        for (int k = 0; k < 3; ++k) h.origin[k] = sb.to_world[4 * k + 3];
+    }
     const V origin{h.origin[0], h.origin[1], h.origin[2]};
 
     for (uint32_t i = 0; i < s.n_media; ++i) {
@@ -384,15 +400,51 @@ HostScene flattenScene(const owe_scene& s) {
     }
     h.env.nLights = int32_t(h.lights.size());
     h.env.sunPdf = e.has_sun ? float(h.lights.back().pdf / (2 * kPi * omc)) : 0.0f;
+    // Particles: computeEnvironmentPower (world.cpp:352-371) for a uniform sky, against the lights'
+    // total power (gpu_scene.cpp:753-758).
+    double skyPower = 0;
+    if (e.sky == OWE_SKY_UNIFORM) {
+        double y = 0;
+        for (double l = 400; l <= 700; l += 50) y += evalHost(e.sky_radiance, s.table, l);
+        skyPower = std::max(0.0, y / 7) * kPi * radius * radius * 4 * kPi;
+    }
+    h.emittedPower = skyPower + (lights.empty() ? 0.0 : total);
+    h.env.skyShare = h.emittedPower > 0 ? float(skyPower / h.emittedPower) : 0.0f;
+    h.env.emitRadius = float(radius * 1.001);
+    for (int k = 0; k < 3; ++k) h.env.center[k] = boundsValid ? float(0.5 * (blo[k] + bhi[k])) : 0.0f;
 
+    h.obs.width = ob.width;
+    h.obs.height = ob.height;
+    h.obs.kind = ob.kind;
+    h.obs.sigma = float(ob.pixel_sigma);
+    if (ob.kind == OWE_DETECTOR_SENSOR) {
+        const DBoundary& sb = h.boundaries[ob.sensor_boundary];
+        h.obs.sensor = ob.sensor_boundary;
+        h.obs.region = sb.front;
+        h.obs.halfX = float(ob.half_x);
+        h.obs.halfY = float(ob.half_y);
+        h.obs.flipX = ob.flip_x != 0;
+        h.obs.flipY = ob.flip_y != 0;
+        h.obs.hasAim = ob.has_aim != 0;
+        h.obs.aimShare = float(ob.aim_share);
+        h.obs.aimRadius = float(ob.aim_radius);
+        const owe_boundary& b = s.boundaries[ob.sensor_boundary];
+        const V nrmW = nrm({b.to_world[2], b.to_world[6], b.to_world[10]});
+        const V an = nrm({ob.aim_normal[0], ob.aim_normal[1], ob.aim_normal[2]});
+        for (int k = 0; k < 3; ++k) {
+            h.obs.normal[k] = float(k == 0 ? nrmW.x : (k == 1 ? nrmW.y : nrmW.z));
+            h.obs.aimCenter[k] = float(ob.aim_center[k] - h.origin[k]);
+            h.obs.aimNormal[k] = float(k == 0 ? an.x : (k == 1 ? an.y : an.z));
+        }
+        std::copy(sb.toWorld, sb.toWorld + 12, h.obs.toWorld);
+        return h;
+    }
     // Observer (IdealObserver::prepare, src/owe/scene/detector.cpp:36-45), in the rebased frame.
     const V pos{ob.position[0], ob.position[1], ob.position[2]};
     const V fwd = nrm(sub({ob.look_at[0], ob.look_at[1], ob.look_at[2]}, pos));
     V u{ob.up[0], ob.up[1], ob.up[2]};
     if (std::fabs(dt(nrm(u), fwd)) > 0.9999) u = std::fabs(fwd.z) < 0.9 ? V{0, 0, 1} : V{0, 1, 0};
     const V right = nrm(crs(fwd, u)), up = crs(right, fwd);
-    h.obs.width = ob.width;
-    h.obs.height = ob.height;
     h.obs.region = ob.region;
     const double tanY = std::tan(0.5 * ob.fov_y);
     h.obs.tanY = float(tanY);
@@ -406,7 +458,6 @@ HostScene flattenScene(const owe_scene& s) {
     }
     h.obs.pupil = float(ob.pupil_radius);
     h.obs.focus = float(ob.focus_distance);
-    h.obs.sigma = float(ob.pixel_sigma);
     (void)origin;
     return h;
 }

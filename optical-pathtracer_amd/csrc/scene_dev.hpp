@@ -67,17 +67,29 @@ struct DEnv {
     float sunDir[3], sunOneMinusCos;  // 1 − cos(angular radius), computed in double
     float sunSolidAngle, sunPdf, pad2[2];
     DSpec sunRadiance, skyRadiance;
+    // Particles: the sky's share of the emitted power, the radius of the disc distant light leaves
+    // from (1.001 × the scene radius) and the scene's centre (gpu_scene.cpp:750-758).
+    float skyShare, emitRadius, pad3[2];
+    float center[3], pad4;
 };
 
+// The detector (owe_observer): an ideal observer at the camera-relative origin, or a surface sensor
+// on plane boundary `sensor` (whose centre is then the origin).
 struct DObserver {
     int32_t width, height;
     uint32_t region;
-    int32_t pad;
+    int32_t kind;             // OWE_DETECTOR_OBSERVER or OWE_DETECTOR_SENSOR
     float position[3], tanX;  // position is the origin of the camera-relative frame: (0, 0, 0)
     float fwd[3], tanY;
     float right[3], pupil;
     float up[3], focus;
-    float sigma, pad2[3];
+    float sigma, halfX, halfY, aimShare;
+    uint32_t sensor;
+    int32_t flipX, flipY, hasAim;
+    float normal[3], aimRadius;
+    float aimCenter[3], pad;
+    float aimNormal[3], pad2;
+    float toWorld[12];        // the sensor's frame → camera-relative world
 };
 
 // What a kernel needs to read the scene: device pointers and counts.
@@ -110,6 +122,7 @@ struct HostScene {
     DEnv env{};
     DObserver obs{};
     double origin[3] = {0, 0, 0};  // world position of the camera-relative frame's origin
+    double emittedPower = 0;       // sky + lights: particles carry something when positive
 };
 
 // Validates and flattens an owe_scene (throws owehip::Failure on invalid input).

@@ -91,6 +91,7 @@ def load() -> ctypes.CDLL:
     _declare(lib, "owe_hip_renderer_progress", c_int, [c_vp, ctypes.POINTER(c_int), ctypes.POINTER(ctypes.c_longlong)])
     _declare(lib, "owe_hip_renderer_reset", c_int, [c_vp])
     _declare(lib, "owe_hip_renderer_camera_paths", c_int, [c_vp, c_int, c_int, c_vp, c_vp, c_u64, c_vp])
+    _declare(lib, "owe_hip_renderer_particles", c_int, [c_vp, c_int, c_vp, c_u64, c_vp])
     _declare(lib, "owe_hip_renderer_set_timing", c_int, [c_vp, c_int])
     _declare(lib, "owe_hip_renderer_kernel_times", c_int, [c_vp, c_vp, c_vp])
     _lib = lib

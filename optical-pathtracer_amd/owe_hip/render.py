@@ -1,8 +1,10 @@
-"""The camera-path renderer through the C ABI (owe_hip_renderer_*).
+"""The renderer through the C ABI (owe_hip_renderer_*).
 
 The Python face of the reference's Renderer (src/owe/render/renderer.hpp:19-79) for the hip
-backend: ``run_pass(spp)`` adds a pass, ``resolve()`` returns the XYZ image [H, W, 3] in double,
-``stats()`` the TransportStats. ``camera_paths`` returns single paths' XYZ for per-path parity.
+backend, with the path or the light integrator: ``run_pass(spp)`` adds a pass (spp samples, or
+spp particles, per pixel), ``resolve()`` returns the XYZ image [H, W, 3] in double, ``stats()`` the
+TransportStats. ``camera_paths`` and ``particles`` return single paths' or particles' XYZ for
+per-path parity.
 """
 from __future__ import annotations
 
@@ -75,4 +77,12 @@ class Renderer:
         out = np.zeros((len(pixel), 3), dtype=np.float64)
         _lib.check(_lib.load().owe_hip_renderer_camera_paths(self._h, int(spp), int(pass_index), pixel.ctypes.data,
                                                              sample.ctypes.data, len(pixel), out.ctypes.data))
+        return out
+
+    def particles(self, index, pass_index: int = 0) -> np.ndarray:
+        """XYZ each particle index[i] of a light-tracing pass deposits (summed over its splats)."""
+        index = np.ascontiguousarray(index, dtype=np.uint64)
+        out = np.zeros((len(index), 3), dtype=np.float64)
+        _lib.check(_lib.load().owe_hip_renderer_particles(self._h, int(pass_index), index.ctypes.data, len(index),
+                                                          out.ctypes.data))
         return out

@@ -29,6 +29,7 @@ APERTURE_DISK, APERTURE_RECT, APERTURE_ELLIPSE = 0, 1, 2
 SKY_NONE, SKY_UNIFORM = 0, 1
 INTEGRATOR_PATH, INTEGRATOR_LIGHT = 0, 1
 SPECTRAL_HERO4, SPECTRAL_UNIFORM1 = 0, 1
+DETECTOR_OBSERVER, DETECTOR_SENSOR = 0, 1
 
 
 class CSpectrum(ctypes.Structure):
@@ -70,9 +71,13 @@ class CEnvironment(ctypes.Structure):
 
 class CObserver(ctypes.Structure):
     _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("region", ctypes.c_uint32),
-                ("reserved", ctypes.c_int32), ("position", ctypes.c_double * 3), ("look_at", ctypes.c_double * 3),
+                ("kind", ctypes.c_int32), ("position", ctypes.c_double * 3), ("look_at", ctypes.c_double * 3),
                 ("up", ctypes.c_double * 3), ("fov_y", ctypes.c_double), ("pupil_radius", ctypes.c_double),
-                ("focus_distance", ctypes.c_double), ("pixel_sigma", ctypes.c_double)]
+                ("focus_distance", ctypes.c_double), ("pixel_sigma", ctypes.c_double),
+                ("sensor_boundary", ctypes.c_uint32), ("flip_x", ctypes.c_int32), ("flip_y", ctypes.c_int32),
+                ("has_aim", ctypes.c_int32), ("half_x", ctypes.c_double), ("half_y", ctypes.c_double),
+                ("aim_center", ctypes.c_double * 3), ("aim_normal", ctypes.c_double * 3),
+                ("aim_radius", ctypes.c_double), ("aim_share", ctypes.c_double)]
 
 
 class CScene(ctypes.Structure):
@@ -286,9 +291,18 @@ class Scene:
 
     def set_observer(self, width, height, position, look_at, up=(0.0, 0.0, 1.0), fov_y=math.radians(40),
                      pupil_radius=0.0, focus_distance=math.inf, pixel_sigma=0.4, region=0):
-        self.observer = dict(width=width, height=height, position=position, look_at=look_at, up=up, fov_y=fov_y,
-                             pupil_radius=pupil_radius, focus_distance=focus_distance, pixel_sigma=pixel_sigma,
-                             region=region)
+        self.observer = dict(kind=DETECTOR_OBSERVER, width=width, height=height, position=position, look_at=look_at,
+                             up=up, fov_y=fov_y, pupil_radius=pupil_radius, focus_distance=focus_distance,
+                             pixel_sigma=pixel_sigma, region=region)
+
+    def set_sensor(self, boundary, width, height, half_x, half_y, flip_x=False, flip_y=False, pixel_sigma=0.0,
+                   aim_center=None, aim_normal=(0.0, 0.0, 1.0), aim_radius=0.0, aim_share=1.0):
+        """The detector is the surface sensor on plane boundary `boundary` (SurfaceSensor,
+        src/owe/scene/detector.hpp:125-160): irradiance on its front face over a pixel grid."""
+        self.observer = dict(kind=DETECTOR_SENSOR, width=width, height=height, sensor_boundary=boundary,
+                             half_x=half_x, half_y=half_y, flip_x=flip_x, flip_y=flip_y, pixel_sigma=pixel_sigma,
+                             has_aim=aim_center is not None, aim_center=aim_center or (0.0, 0.0, 0.0),
+                             aim_normal=aim_normal, aim_radius=aim_radius, aim_share=aim_share)
 
     # ---- flattening
 
@@ -353,11 +367,18 @@ class Scene:
         tab = np.ascontiguousarray(table if table else [0.0], dtype=np.float64)
         ob = CObserver()
         o = self.observer
-        ob.width, ob.height, ob.region = o["width"], o["height"], o["region"]
-        for k in range(3):
-            ob.position[k], ob.look_at[k], ob.up[k] = o["position"][k], o["look_at"][k], o["up"][k]
-        ob.fov_y, ob.pupil_radius, ob.focus_distance, ob.pixel_sigma = (o["fov_y"], o["pupil_radius"],
-                                                                        o["focus_distance"], o["pixel_sigma"])
+        ob.kind, ob.width, ob.height, ob.pixel_sigma = o["kind"], o["width"], o["height"], o["pixel_sigma"]
+        if o["kind"] == DETECTOR_OBSERVER:
+            ob.region = o["region"]
+            for k in range(3):
+                ob.position[k], ob.look_at[k], ob.up[k] = o["position"][k], o["look_at"][k], o["up"][k]
+            ob.fov_y, ob.pupil_radius, ob.focus_distance = o["fov_y"], o["pupil_radius"], o["focus_distance"]
+        else:
+            ob.sensor_boundary, ob.half_x, ob.half_y = o["sensor_boundary"], o["half_x"], o["half_y"]
+            ob.flip_x, ob.flip_y, ob.has_aim = int(o["flip_x"]), int(o["flip_y"]), int(o["has_aim"])
+            for k in range(3):
+                ob.aim_center[k], ob.aim_normal[k] = o["aim_center"][k], o["aim_normal"][k]
+            ob.aim_radius, ob.aim_share = o["aim_radius"], o["aim_share"]
         s = CScene()
         s.n_media, s.n_regions, s.n_optics = len(self.media), len(regions), len(self.optics)
         s.n_emissions, s.n_boundaries = len(self.emissions), len(self.boundaries)
@@ -482,4 +503,103 @@ def dielectric_sphere_scene(size: int = 24) -> Scene:
     sc.add_sphere(1.0, transform(), amb, inside, diel)
     sc.set_uniform_sky(Spectrum.constant(1.0))
     sc.set_observer(size, size, position=(0, -4, 0), look_at=(0, 0, 0), fov_y=math.radians(30))
+    return sc
+
+
+def disk_irradiance(L: float, a: float, h: float, r: float) -> float:
+    """Irradiance at distance r from the axis on a plane h below a Lambertian disk of radius a and
+    radiance L (the exact off-axis formula of tests/analytic.hpp:10-14)."""
+    s = h * h + r * r + a * a
+    return math.pi * L / 2 * (1 - (h * h + r * r - a * a) / math.sqrt(s * s - 4 * r * r * a * a))
+
+
+def mean_disk_irradiance(L: float, a: float, h: float, half: float, n: int = 200) -> float:
+    """disk_irradiance averaged over a centred square of half-size `half` (tests/analytic.hpp:16-25)."""
+    c = -half + (np.arange(n) + 0.5) * 2 * half / n
+    x, y = np.meshgrid(c, c)
+    r = np.sqrt(x * x + y * y)
+    s = h * h + r * r + a * a
+    e = math.pi * L / 2 * (1 - (h * h + r * r - a * a) / np.sqrt(s * s - 4 * r * r * a * a))
+    return float(e.mean())
+
+
+def disk_irradiance_scene(a: float = 0.3, h: float = 0.5, half: float = 0.15, aimed: bool = False,
+                          screen: bool = False, size: int = 2) -> Scene:
+    """A Lambertian disk emitter (radiance 2, radius a) facing down from height h onto an upward
+    square sensor of half-size `half` (tests/analytic.hpp:28-60): a detector surface, or a diffuse
+    screen of albedo 0.7; optionally aimed at the disk."""
+    sc = Scene()
+    air = sc.add_medium(Medium.air())
+    amb = sc.add_region(air)
+    black = sc.add_optics(Optics(SURF_ABSORBER))
+    e = sc.add_emission(Emission(Spectrum.constant(2.0)))
+    sc.add_plane(APERTURE_DISK, a, 0.0, transform(rotation((1, 0, 0), math.pi), (0.0, 0.0, h)), amb, amb, black, e)
+    so = sc.add_optics(Optics(SURF_DIFFUSE, reflectance=Spectrum.constant(0.7)) if screen else Optics(SURF_DETECTOR))
+    sb = sc.add_plane(APERTURE_RECT, half, half, transform(), amb, amb, so)
+    if aimed:
+        sc.set_sensor(sb, size, size, half, half, aim_center=(0.0, 0.0, h), aim_normal=(0.0, 0.0, 1.0), aim_radius=a,
+                      aim_share=0.95 if screen else 1.0)
+    else:
+        sc.set_sensor(sb, size, size, half, half)
+    return sc
+
+
+def ball_caustic_scene(size: int = 16, sun_radius_deg: float = 3.0, dispersive: bool = True) -> Scene:
+    """A glass ball (radius 20 mm, centre 40 mm up) focusing a sun straight overhead onto a diffuse
+    floor (half-size 0.1 m, albedo 0.8); a pinhole eye 15 mm above the floor, below the ball, looks
+    straight down at the caustic. Everything the eye sees is diffuse, so camera paths and particles
+    estimate the same image; the light reaching it has crossed two refractions (importance transport
+    drops the n² radiance factor that camera paths carry). The sun is widened to `sun_radius_deg` so
+    camera paths find it through the ball."""
+    sc = Scene()
+    air = sc.add_medium(Medium.constant(1.000277))
+    glass = (sc.add_medium(Medium.sellmeier((1.03961212, 0.231792344, 1.01046945), (0.00600069867, 0.0200179144,
+                                                                                     103.560653)))
+             if dispersive else sc.add_medium(Medium.constant(1.5)))
+    amb = sc.add_region(air)
+    inside = sc.add_region(glass)
+    floor = sc.add_optics(Optics(SURF_DIFFUSE, reflectance=Spectrum.constant(0.8)))
+    diel = sc.add_optics(Optics(SURF_DIELECTRIC))
+    sc.add_plane(APERTURE_RECT, 0.1, 0.1, transform(), amb, amb, floor)
+    sc.add_sphere(0.02, transform(None, (0.0, 0.0, 0.04)), amb, inside, diel)
+    sc.set_sun((0.0, 0.0, 1.0), math.radians(sun_radius_deg), Spectrum.constant(1.0))
+    sc.set_observer(size, size, position=(0.0, 0.0, 0.015), look_at=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0),
+                    fov_y=math.radians(90))
+    return sc
+
+
+def box_mesh(lo, hi):
+    """An axis-aligned box as 12 triangles wound outward (the front side faces out)."""
+    lo, hi = np.asarray(lo, float), np.asarray(hi, float)
+    pos = np.array([[(hi if (k >> a) & 1 else lo)[a] for a in range(3)] for k in range(8)])
+    quads = [(0, 2, 6, 4), (1, 5, 7, 3), (0, 4, 5, 1), (2, 3, 7, 6), (0, 1, 3, 2), (4, 6, 7, 5)]
+    tri = []
+    c = pos.mean(axis=0)
+    for q in quads:
+        for t in ((q[0], q[1], q[2]), (q[0], q[2], q[3])):
+            n = np.cross(pos[t[1]] - pos[t[0]], pos[t[2]] - pos[t[0]])
+            tri.append(t if np.dot(n, pos[list(t)].mean(axis=0) - c) > 0 else (t[0], t[2], t[1]))
+    return pos, np.array(tri, dtype=np.uint32)
+
+
+def submerged_sensor_scene(size: int = 2, n_water: float = 1.333) -> Scene:
+    """The disk emitter of disk_irradiance_scene (radius 0.3, radiance 2, 0.5 m up, facing down) over
+    a water box (0.4 × 0.4 × 0.1 m, n = 1.333) holding an upward detector sensor (half-size 0.1,
+    30 mm under the surface). The light reaching the sensor has refracted into the water: camera
+    paths carry radiance (the n² factor at the surface) and particles carry flux (no factor), so
+    the two integrators agree only if both transport modes are right."""
+    sc = Scene()
+    air = sc.add_medium(Medium.air())
+    water = sc.add_medium(Medium.constant(n_water))
+    amb = sc.add_region(air)
+    wet = sc.add_region(water)
+    black = sc.add_optics(Optics(SURF_ABSORBER))
+    diel = sc.add_optics(Optics(SURF_DIELECTRIC))
+    det = sc.add_optics(Optics(SURF_DETECTOR))
+    e = sc.add_emission(Emission(Spectrum.constant(2.0)))
+    sc.add_plane(APERTURE_DISK, 0.3, 0.0, transform(rotation((1, 0, 0), math.pi), (0.0, 0.0, 0.5)), amb, amb, black, e)
+    pos, tri = box_mesh((-0.2, -0.2, -0.1), (0.2, 0.2, 0.0))
+    sc.add_mesh(pos, tri, transform(), amb, wet, diel)
+    sb = sc.add_plane(APERTURE_RECT, 0.1, 0.1, transform(None, (0.0, 0.0, -0.03)), wet, wet, det)
+    sc.set_sensor(sb, size, size, 0.1, 0.1)
     return sc

@@ -16,7 +16,7 @@ namespace {
 
 template <class F>
 void forRanges(size_t n, int threads, F&& f) {
-    if (threads <= 0) threads = int(std::max(1u, std::thread::hardware_concurrency()));
+    if (threads <= 0) threads = int(std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
     threads = int(std::min<size_t>(size_t(threads), std::max<size_t>(1, n / 1024 + 1)));
     if (threads <= 1) {
         f(size_t(0), n);
@@ -190,7 +190,7 @@ void oracle_mesh_brute(const void* mh, const float* rays, uint64_t n, uint32_t* 
 void oracle_mesh_traversal_stats(const void* mh, const float* rays, uint64_t n, double* out, int threads) {
     const Mesh& m = *static_cast<const Mesh*>(mh);
     std::vector<uint64_t> boxes(64, 0), tris(64, 0);
-    int used = threads <= 0 ? int(std::max(1u, std::thread::hardware_concurrency())) : threads;
+    int used = threads <= 0 ? int(std::min(16u, std::max(1u, std::thread::hardware_concurrency()))) : threads;
     used = std::min(used, 64);
     std::vector<std::pair<size_t, size_t>> ranges;
     for (int k = 0; k < used; ++k) ranges.push_back({size_t(n) * k / used, size_t(n) * (k + 1) / used});

@@ -3,6 +3,7 @@
 // (CpuRenderer::pass, cpu_backend.cpp:52-94, in 16×16 tiles on host threads) and single paths by
 // (pixel, sample) for per-path comparisons with the hip backend.
 #include <atomic>
+#include <cmath>
 #include <cstring>
 #include <thread>
 #include <vector>
@@ -53,7 +54,7 @@ void oracle_render(const void* wh, const owe_render_settings* rs, int spp, int p
     const Tracer tr(w, set);
     const int W = w.observer.width, H = w.observer.height, tile = 16;
     const int tilesX = (W + tile - 1) / tile, tilesY = (H + tile - 1) / tile;
-    if (threads <= 0) threads = int(std::max(1u, std::thread::hardware_concurrency()));
+    if (threads <= 0) threads = int(std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
     std::atomic<int> next{0};
     std::vector<Stats> st{size_t(threads)};
     auto worker = [&](int tid) {
@@ -103,7 +104,7 @@ void oracle_camera_paths(const void* wh, const owe_render_settings* rs, int spp,
     const Settings set = toSettings(*rs);
     const Tracer tr(w, set);
     const int W = w.observer.width;
-    if (threads <= 0) threads = int(std::max(1u, std::thread::hardware_concurrency()));
+    if (threads <= 0) threads = int(std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
     std::vector<std::thread> pool;
     for (int k = 0; k < threads; ++k)
         pool.emplace_back([&, k] {
@@ -121,6 +122,93 @@ void oracle_camera_paths(const void* wh, const owe_render_settings* rs, int spp,
             }
         });
     for (auto& th : pool) th.join();
+}
+
+// Adds one light-tracing pass into xyz (W·H·3): particles [0, W·H·ppp), particle i on the stream
+// (hashCombine(seed ^ 0x5bd1e995, i), pass) as particle_kernel.slang:15-16, each thread into its own
+// film, summed in thread order. The caller divides by the particle count (gpu_renderer_film.cpp:17-20).
+void oracle_render_particles(const void* wh, const owe_render_settings* rs, int ppp, int pass, int threads,
+                             double* xyz, uint64_t* stats) {
+    const World& w = *static_cast<const World*>(wh);
+    const Settings set = toSettings(*rs);
+    const Tracer tr(w, set);
+    const int W = w.observer.width, H = w.observer.height;
+    const uint64_t n = uint64_t(W) * uint64_t(H) * uint64_t(ppp);
+    if (threads <= 0) threads = int(std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
+    struct Film final : Tracer::Splat {
+        int width = 0;
+        std::vector<double> v;
+        void add(int px, int py, const double c[3]) override {
+            if (!std::isfinite(c[0] + c[1] + c[2])) return;
+            for (int k = 0; k < 3; ++k) v[3 * (size_t(py) * size_t(width) + size_t(px)) + size_t(k)] += c[k];
+        }
+    };
+    std::vector<Film> films(static_cast<size_t>(threads));
+    std::vector<Stats> st(static_cast<size_t>(threads));
+    std::vector<std::thread> pool;
+    for (int k = 0; k < threads; ++k)
+        pool.emplace_back([&, k] {
+            Film& f = films[size_t(k)];
+            f.width = W;
+            f.v.assign(size_t(W) * size_t(H) * 3, 0.0);
+            for (uint64_t i = n * uint64_t(k) / uint64_t(threads); i < n * uint64_t(k + 1) / uint64_t(threads); ++i) {
+                Pcg rng(hashPair(set.seed ^ 0x5bd1e995ULL, i), uint64_t(pass));
+                tr.particle(rng, st[size_t(k)], f);
+            }
+        });
+    for (auto& th : pool) th.join();
+    Stats total;
+    for (int k = 0; k < threads; ++k) {
+        for (size_t j = 0; j < films[size_t(k)].v.size(); ++j) xyz[j] += films[size_t(k)].v[j];
+        total.add(st[size_t(k)]);
+    }
+    stats[0] += total.paths;
+    stats[1] += total.segments;
+    stats[2] += total.inconsistencies;
+    stats[3] += total.leaks;
+}
+
+// The XYZ each of n particles (by index, of pass `pass`) deposits, summed over its splats; segments
+// per particle in segs.
+void oracle_particles(const void* wh, const owe_render_settings* rs, int pass, uint64_t n, const uint64_t* index,
+                      double* xyz, uint32_t* segs, int threads) {
+    const World& w = *static_cast<const World*>(wh);
+    const Settings set = toSettings(*rs);
+    const Tracer tr(w, set);
+    if (threads <= 0) threads = int(std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
+    struct Sum final : Tracer::Splat {
+        double c[3] = {0, 0, 0};
+        void add(int, int, const double v[3]) override {
+            if (!std::isfinite(v[0] + v[1] + v[2])) return;
+            for (int k = 0; k < 3; ++k) c[k] += v[k];
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int k = 0; k < threads; ++k)
+        pool.emplace_back([&, k] {
+            for (uint64_t i = n * uint64_t(k) / uint64_t(threads); i < n * uint64_t(k + 1) / uint64_t(threads); ++i) {
+                Pcg rng(hashPair(set.seed ^ 0x5bd1e995ULL, index[i]), uint64_t(pass));
+                Stats st;
+                Sum sum;
+                tr.particle(rng, st, sum);
+                for (int q = 0; q < 3; ++q) xyz[3 * i + q] = sum.c[q];
+                segs[i] = uint32_t(st.segments);
+            }
+        });
+    for (auto& th : pool) th.join();
+}
+
+// The particle emission set-up: the sky's share of emitted power, the scene radius and centre.
+void oracle_emission_info(const void* wh, double* out) {
+    const World& w = *static_cast<const World*>(wh);
+    const V3 c = w.sceneCenter();
+    out[0] = w.skyShare;
+    out[1] = w.sceneRadius();
+    out[2] = c.x;
+    out[3] = c.y;
+    out[4] = c.z;
+    out[5] = w.environmentPower;
+    out[6] = w.totalLightPower;
 }
 
 double oracle_spectrum_luminance_blackbody(double kelvin) {

@@ -153,6 +153,17 @@ inline void standardNormal2(double u1, double u2, double& g1, double& g2) {
     g1 = r * std::cos(phi);
     g2 = r * std::sin(phi);
 }
+// Solid-angle density at p of directions toward a uniform point of a disc (optics.hpp:248-256);
+// generatingDisk: wi is known to come from the disc, so no rim test.
+inline double diskAimPdf(V3 p, V3 wi, V3 center, V3 normal, double radius, bool generatingDisk) {
+    if (!(radius > 0)) return 0;
+    const double dn = dot(wi, normal);
+    if (std::fabs(dn) < 1e-12) return 0;
+    const double t = dot(center - p, normal) / dn;
+    const V3 delta = p + wi * t - center;
+    if (t <= 0 || (!generatingDisk && dot(delta, delta) > radius * radius)) return 0;
+    return t * t / (std::fabs(dn) * kPi * radius * radius);
+}
 inline double powerHeuristic(double f, double g) {
     const double a = f * f, b = g * g;
     if (std::isinf(a)) return 1;

@@ -326,11 +326,41 @@ World::World(const owe_scene& s) {
         lightCdf.push_back(acc);
     }
     if (!lightCdf.empty()) lightCdf.back() = 1.0;
+    totalLightPower = total;
+    // computeEnvironmentPower (world.cpp:352-371): a uniform sky's radiance averaged over 400..700 nm
+    // in 50 nm steps (the same over every direction), times π R² · 4π.
+    if (env.sky == OWE_SKY_UNIFORM) {
+        double y = 0;
+        for (double l = 400; l <= 700; l += 50) y += env.skyRadiance.eval(l);
+        environmentPower = std::max(0.0, y / 7) * kPi * radius * radius * 4 * kPi;
+    }
+    const double emitted = environmentPower + (lights.empty() ? 0.0 : totalLightPower);
+    skyShare = emitted > 0 ? environmentPower / emitted : 0;
 
-    // Ideal observer (IdealObserver::prepare, detector.cpp:36-45).
     const owe_observer& ob = s.observer;
+    observer.kind = ob.kind;
     observer.width = ob.width;
     observer.height = ob.height;
+    observer.sigma = ob.pixel_sigma;
+    if (ob.kind == OWE_DETECTOR_SENSOR) {
+        // SurfaceSensor::prepare (detector.cpp:100-108).
+        const Boundary& b = boundaries.at(ob.sensor_boundary);
+        observer.boundary = ob.sensor_boundary;
+        observer.toWorld = b.toWorld;
+        observer.normal = normalize(b.toWorld.vector(V3(0, 0, 1)));
+        observer.region = b.front;
+        observer.halfX = ob.half_x;
+        observer.halfY = ob.half_y;
+        observer.flipX = ob.flip_x != 0;
+        observer.flipY = ob.flip_y != 0;
+        observer.hasAim = ob.has_aim != 0;
+        observer.aimCenter = V3(ob.aim_center[0], ob.aim_center[1], ob.aim_center[2]);
+        observer.aimNormal = V3(ob.aim_normal[0], ob.aim_normal[1], ob.aim_normal[2]);
+        observer.aimRadius = ob.aim_radius;
+        observer.aimShare = ob.aim_share;
+        return;
+    }
+    // Ideal observer (IdealObserver::prepare, detector.cpp:36-45).
     observer.region = ob.region;
     observer.position = V3(ob.position[0], ob.position[1], ob.position[2]);
     observer.fwd = normalize(V3(ob.look_at[0], ob.look_at[1], ob.look_at[2]) - observer.position);
@@ -343,7 +373,6 @@ World::World(const owe_scene& s) {
     observer.tanX = observer.tanY * double(ob.width) / double(ob.height);
     observer.pupil = ob.pupil_radius;
     observer.focus = ob.focus_distance;
-    observer.sigma = ob.pixel_sigma;
 }
 
 double World::sceneRadius() const {
@@ -423,9 +452,18 @@ bool World::sampleFrom(const Boundary& b, V3 ref, double u1, double u2, double u
             pdfW = 1 / (2 * kPi * omc);
             return true;
         }
+    }
+    uint32_t prim;
+    if (!sampleArea(b, u1, u2, u3, p, n, prim)) return false;
+    return areaToSolidAngle(ref, p, n, b.area(), pdfW);
+}
+
+bool World::sampleArea(const Boundary& b, double u1, double u2, double u3, V3& p, V3& n, uint32_t& prim) const {
+    prim = 0;
+    if (b.shape == OWE_SHAPE_SPHERE) {
         n = uniformSphere(u1, u2);
         p = n * b.a;
-        return areaToSolidAngle(ref, p, n, b.area(), pdfW);
+        return true;
     }
     if (!b.canSample()) return false;
     if (b.shape == OWE_SHAPE_PLANE) {
@@ -448,8 +486,9 @@ bool World::sampleFrom(const Boundary& b, V3 ref, double u1, double u2, double u
         const double su = std::sqrt(u1), b0 = 1 - su, b1 = u2 * su;
         p = P[t[0]] * b0 + P[t[1]] * b1 + P[t[2]] * (1 - b0 - b1);
         n = b.mesh->normalOf(uint32_t(i));
+        prim = uint32_t(i);
     }
-    return areaToSolidAngle(ref, p, n, b.area(), pdfW);
+    return true;
 }
 
 double World::pdfFrom(const Boundary& b, V3 ref, V3 p, V3 n) const {
@@ -463,7 +502,49 @@ double World::pdfFrom(const Boundary& b, V3 ref, V3 p, V3 n) const {
 
 // ---------------------------------------------------------------- observer
 
+void Observer::filterOffset(Pcg& rng, double& fx, double& fy) const {
+    fx = fy = 0;
+    if (!(sigma > 0)) return;
+    double u1, u2;
+    rng.draw2(u1, u2);
+    standardNormal2(u1, u2, fx, fy);
+    fx *= sigma;
+    fy *= sigma;
+}
+
 bool Observer::generate(double px, double py, Pcg& rng, Ray& ray, double& weight) const {
+    if (sensor()) {
+        // SurfaceSensor::generate (detector.cpp:110-149), without the focus disk.
+        weight = 0;
+        if (px < 0 || py < 0 || px >= width || py >= height) return false;
+        if (flipX) px = width - px;
+        if (flipY) py = height - py;
+        const V3 p = toWorld.point(V3((2 * px / width - 1) * halfX, (1 - 2 * py / height) * halfY, 0));
+        double u1, u2;
+        V3 dir;
+        if (hasAim) {
+            const Frame aim(aimNormal);
+            const bool useAim = aimShare >= 1 || rng.uniform() < aimShare;
+            rng.draw2(u1, u2);
+            double dx, dy;
+            concentricDisk(u1, u2, dx, dy);
+            const V3 w = aimCenter + (aim.s * dx + aim.t * dy) * aimRadius - p;
+            dir = w / length(w);
+            if (!useAim) dir = Frame(normal).toWorld(cosineHemisphere(u1, u2));
+            const double cosS = dot(dir, normal);
+            if (cosS <= 0) return false;
+            const double pdfW = aimShare * diskAimPdf(p, dir, aimCenter, aimNormal, aimRadius, useAim) +
+                                (1 - aimShare) * cosS * kInvPi;
+            if (!(pdfW > 0)) return false;
+            weight = cosS / pdfW;
+        } else {
+            rng.draw2(u1, u2);
+            dir = Frame(normal).toWorld(cosineHemisphere(u1, u2));
+            weight = kPi;
+        }
+        ray = Ray{p, dir, boundary, kNone};
+        return true;
+    }
     const double sx = (2 * px / width - 1) * tanX, sy = (1 - 2 * py / height) * tanY;
     const V3 dirPinhole = normalize(right * sx + up * sy + fwd);
     V3 q = position;

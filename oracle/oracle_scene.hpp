@@ -178,11 +178,23 @@ struct Light {
     double power = 0;
 };
 
+// The detector: an ideal observer (detector.cpp:36-91) or a surface sensor (detector.cpp:100-149).
 struct Observer {
+    int kind = OWE_DETECTOR_OBSERVER;
     int width = 0, height = 0;
     uint32_t region = 0;
     V3 position, fwd, right, up;
     double tanX = 1, tanY = 1, pupil = 0, focus = kInf, sigma = 0.4;
+    // surface sensor
+    uint32_t boundary = kNone;
+    double halfX = 0, halfY = 0;
+    bool flipX = false, flipY = false, hasAim = false;
+    V3 aimCenter, aimNormal{0, 0, 1}, normal;
+    double aimRadius = 0, aimShare = 1;
+    Xf toWorld;
+    bool sensor() const { return kind == OWE_DETECTOR_SENSOR; }
+    // Pixel filter offset (Detector::filterOffset, detector.hpp:57-62).
+    void filterOffset(Pcg& rng, double& fx, double& fy) const;
     bool generate(double px, double py, Pcg& rng, Ray& ray, double& weight) const;
 };
 
@@ -198,6 +210,8 @@ public:
     // A point on boundary b seen from ref (solid-angle pdf), and the pdf of a given point.
     bool sampleFrom(const Boundary& b, V3 refLocal, double u1, double u2, double u3, V3& p, V3& n, double& pdfW) const;
     double pdfFrom(const Boundary& b, V3 refLocal, V3 pLocal, V3 nLocal) const;
+    // A uniform point of b in its own frame (Shape::sampleArea; geometry.slang:708-752).
+    bool sampleArea(const Boundary& b, double u1, double u2, double u3, V3& p, V3& n, uint32_t& prim) const;
 
     std::vector<Medium> media;
     std::vector<uint32_t> regionMedium;
@@ -210,6 +224,10 @@ public:
     std::vector<Light> lights;
     std::vector<double> lightPdf, lightCdf;
     std::vector<int> lightOfBoundary;
+    // Particles (gpu_scene.cpp:753-758): the sky's share of the emitted power (environment power,
+    // world.cpp:352-371, against the lights' total) and the scene's centre.
+    double environmentPower = 0, totalLightPower = 0, skyShare = 0;
+    V3 sceneCenter() const { return bounds.valid() ? (bounds.lo + bounds.hi) * 0.5 : V3(); }
 };
 
 struct Settings {
@@ -240,7 +258,23 @@ public:
     // One camera sample of pixel (x, y): the draws of cpu_backend.cpp:69-84 from `rng`.
     bool cameraSample(int x, int y, int s, int spp, double rot, Pcg& rng, Stats& st, double xyz[3]) const;
 
+    // Where a particle's contributions go: pixel (px, py) receives XYZ.
+    struct Splat {
+        virtual void add(int px, int py, const double xyz[3]) = 0;
+        virtual ~Splat() = default;
+    };
+    // One particle of a light-tracing pass (particle_kernel.slang:11-19, particle.slang:356-427):
+    // its wavelengths from the first uniform of `rng`, then emission, walk and splats.
+    void particle(Pcg& rng, Stats& st, Splat& out) const;
+
 private:
+    struct Link;
+    struct Emitted;
+    bool linkObserver(V3 x, Pcg& rng, Link& k) const;
+    void deposit(Splat& out, int px, int py, const Spec4& v, const Wavelengths& wl) const;
+    bool emitParticle(Wavelengths& wl, Pcg& rng, bool connect, Emitted& e, Splat& out) const;
+    void connectVertex(const Hit& hit, bool surface, const Optics* o, V3 d, double g, uint32_t region,
+                       const Spec4& beta, const Wavelengths& wl, Pcg& rng, Splat& out) const;
     struct Path;
     Spec4 directLighting(const Hit& hit, bool surface, const Optics* o, double nFront, double nBack, V3 d,
                          const Wavelengths& wl, Pcg& rng, double g, uint32_t region) const;

@@ -1,6 +1,8 @@
 // TEST INFRASTRUCTURE ONLY — the double-precision CPU oracle (see oracle_scene.hpp for citations).
 // Camera-path transport restated from src/owe/transport/transport.cpp:35-529 and
-// src/owe/transport/scattering.cpp:65-222, over the simplified World of oracle_scene.cpp.
+// src/owe/transport/scattering.cpp:65-222, over the simplified World of oracle_scene.cpp; particle
+// transport (light tracing) restated in double from the reference's only implementation, the float
+// kernel src/owe/backends/gpu/shaders/particle.slang:23-427 with particle_kernel.slang:11-19.
 #include "oracle_scene.hpp"
 
 namespace oracle {
@@ -469,6 +471,288 @@ bool Tracer::cameraSample(int x, int y, int s, int spp, double rot, Pcg& rng, St
     if (L.isZero()) return false;
     wl.toXYZ(L * weight, xyz);
     return std::isfinite(xyz[0] + xyz[1] + xyz[2]);
+}
+
+// ---------------------------------------------------------------- particles (light tracing)
+
+// IdealObserver::connect as the particle kernel does it (particle.slang:59-95): a pupil point q,
+// the pixel, and the factor W with which the pixel receives β·f·|cos|·W·Tr / N.
+struct Tracer::Link {
+    V3 q, dir;
+    int px = 0, py = 0;
+    double factor = 0;
+};
+
+// A particle leaving an emitter (particle.slang:184-196).
+struct Tracer::Emitted {
+    Ray ray;
+    uint32_t region = 0;
+    Spec4 beta;
+    bool distant = false;
+};
+
+bool Tracer::linkObserver(V3 x, Pcg& rng, Link& k) const {
+    const Observer& ob = w_.observer;
+    k = Link{};
+    k.q = ob.position;
+    if (ob.pupil > 0) {
+        double u1, u2, dx, dy;
+        rng.draw2(u1, u2);
+        concentricDisk(u1, u2, dx, dy);
+        k.q = ob.position + (ob.right * dx + ob.up * dy) * ob.pupil;
+    }
+    const V3 w = x - k.q;
+    const double dist2 = dot(w, w);
+    if (dist2 <= 0) return false;
+    const V3 om = w / std::sqrt(dist2);
+    const double c = dot(om, ob.fwd);
+    if (c <= 1e-9) return false;
+    double sx, sy;
+    if (ob.pupil > 0 && std::isfinite(ob.focus)) {
+        const V3 F = (k.q - ob.position) + om * (ob.focus / c);
+        sx = dot(F, ob.right) / ob.focus;
+        sy = dot(F, ob.up) / ob.focus;
+    } else {
+        sx = dot(om, ob.right) / c;
+        sy = dot(om, ob.up) / c;
+    }
+    double ox, oy;
+    ob.filterOffset(rng, ox, oy);
+    const double W = ob.width, H = ob.height;
+    const double fx = (sx / ob.tanX + 1) * 0.5 * W + ox, fy = (1 - sy / ob.tanY) * 0.5 * H + oy;
+    if (!(fx >= 0 && fy >= 0 && fx < W && fy < H)) return false;
+    k.px = std::min(int(fx), ob.width - 1);
+    k.py = std::min(int(fy), ob.height - 1);
+    const double pixelArea = (2 * ob.tanX / W) * (2 * ob.tanY / H);
+    k.factor = 1 / (dist2 * pixelArea * c * c * c);
+    k.dir = normalize(k.q - x);
+    return true;
+}
+
+void Tracer::deposit(Splat& out, int px, int py, const Spec4& v, const Wavelengths& wl) const {
+    double xyz[3];
+    wl.toXYZ(v, xyz);
+    out.add(px, py, xyz);
+}
+
+// connectVertex (particle.slang:118-138): a surface vertex (its optics o at the hit) or a medium
+// point (phase anisotropy g) joined to the observer; the connection's walk runs from the vertex.
+void Tracer::connectVertex(const Hit& hit, bool surface, const Optics* o, V3 d, double g, uint32_t region,
+                           const Spec4& beta, const Wavelengths& wl, Pcg& rng, Splat& out) const {
+    Link k;
+    if (!linkObserver(hit.p, rng, k)) return;
+    Spec4 f;
+    double cosX = 1;
+    uint32_t start = region;
+    Ray from;
+    if (!surface) {
+        f = Spec4(hgPhase(dot(-d, k.dir), g));
+    } else {
+        const Boundary& b = w_.boundaries[hit.boundary];
+        for (int i = 0; i < NW; ++i) {
+            if (wl.pdf[i] == 0) continue;
+            const Interface it{o, hit.n, hit.p, w_.indexOf(b.front, wl.lambda[i]), w_.indexOf(b.back, wl.lambda[i]),
+                               wl.lambda[i], s_.fresnelFloor};
+            double pdf;
+            f[i] = evalScatter(it, d, k.dir, !kRadiance, pdf);
+        }
+        if (f.isZero()) return;
+        cosX = std::fabs(dot(k.dir, hit.n));
+        start = dot(k.dir, hit.n) > 0 ? b.front : b.back;
+        from.departed = hit.boundary;
+        from.departedPrim = hit.prim;
+    }
+    uint32_t arrival = kNone;
+    Spec4 Tr = transmittance(hit.p, from, start, k.dir, length(k.q - hit.p), wl, &arrival, kNone);
+    if (arrival != w_.observer.region || Tr.isZero()) return;
+    deposit(out, k.px, k.py, beta * f * Tr * (cosX * k.factor), wl);
+}
+
+// emitParticle with emitDistant / emitFromArea (particle.slang:234-344), without photon guiding.
+bool Tracer::emitParticle(Wavelengths& wl, Pcg& rng, bool connect, Emitted& e, Splat& out) const {
+    e = Emitted{};
+    const double pSky = w_.skyShare;
+    const bool fromSky = rng.uniform() < pSky;
+    size_t li = 0;
+    if (!fromSky) {
+        if (w_.lights.empty()) return false;
+        li = w_.pickLight(rng.uniform());
+    }
+    if (fromSky || w_.lights[li].kind == 1) {
+        // A direction, then a point of a disc of radius 1.001·R across the scene, facing it.
+        const double sel = fromSky ? pSky : (1 - pSky) * w_.lightPdf[li];
+        const double R = w_.sceneRadius() * 1.001;
+        double u1, u2;
+        rng.draw2(u1, u2);
+        V3 toward;
+        double pdfDir;
+        Spec4 Le;
+        if (fromSky) {
+            toward = uniformSphere(u1, u2);
+            pdfDir = 1 / (4 * kPi);
+            for (int i = 0; i < NW; ++i) Le[i] = w_.env.skyAt(wl.lambda[i]);
+        } else {
+            toward = Frame(w_.env.sunDir).toWorld(uniformCone(u1, u2, std::cos(w_.env.sunAngularRadius)));
+            pdfDir = 1 / w_.env.sunSolidAngle();
+            Le = wl.eval(w_.env.sunRadiance);
+        }
+        const V3 dir = -toward;
+        const Frame disc(dir);
+        double v1, v2, qx, qy;
+        rng.draw2(v1, v2);
+        concentricDisk(v1, v2, qx, qy);
+        e.ray = Ray{w_.sceneCenter() - dir * (1.01 * R) + disc.s * (qx * R) + disc.t * (qy * R), dir, kNone, kNone};
+        e.region = 0;
+        e.beta = Le * (1 / (sel * pdfDir * (1 / (kPi * R * R))));
+        e.distant = true;
+    } else {
+        const double sel = (1 - pSky) * w_.lightPdf[li];
+        const uint32_t bi = w_.lights[li].boundary;
+        const Boundary& b = w_.boundaries[bi];
+        const Emission& em = w_.emissions[size_t(b.emission)];
+        const Spec4 Le = wl.eval(em.radiance);
+        double u1, u2, u3;
+        rng.draw3(u1, u2, u3);
+        V3 pL, nL;
+        uint32_t prim;
+        if (!w_.sampleArea(b, u1, u2, u3, pL, nL, prim)) return false;
+        const V3 p = b.toWorld.point(pL), n = normalize(b.toWorld.vector(nL));
+        const double pdfA = 1 / b.area();
+        if (connect) {
+            // areaEmitterToObserver (particle.slang:267-279): the emitter seen directly.
+            Link k;
+            if (linkObserver(p, rng, k)) {
+                const double cosL = dot(n, k.dir);
+                if ((cosL > 0 && em.front) || (cosL < 0 && em.back)) {
+                    uint32_t arrival = kNone;
+                    const Ray from{p, k.dir, bi, prim};
+                    const Spec4 Tr = transmittance(p, from, cosL > 0 ? b.front : b.back, k.dir, length(k.q - p), wl,
+                                                   &arrival, kNone);
+                    if (arrival == w_.observer.region && !Tr.isZero())
+                        deposit(out, k.px, k.py, Le * Tr * (std::fabs(cosL) * k.factor / (sel * pdfA)), wl);
+                }
+            }
+        }
+        double sidePdf = 1;
+        bool front = em.front;
+        if (em.front && em.back) {
+            front = rng.uniform() < 0.5;
+            sidePdf = 0.5;
+        }
+        double c1, c2;
+        rng.draw2(c1, c2);
+        const V3 dir = Frame(front ? n : -n).toWorld(cosineHemisphere(c1, c2));
+        e.ray = Ray{p, dir, bi, prim};
+        e.region = front ? b.front : b.back;
+        e.beta = Le * (kPi / (sel * pdfA * sidePdf));
+    }
+    maskTerminated(wl, e.beta);
+    return !e.beta.isZero();
+}
+
+void Tracer::particle(Pcg& rng, Stats& st, Splat& out) const {
+    const double u = rng.uniform();
+    Wavelengths wl = s_.spectral == OWE_SPECTRAL_HERO4 ? Wavelengths::hero(u)
+                                                        : Wavelengths::uniform1(u, s_.lambdaMin, s_.lambdaMax);
+    const Observer& ob = w_.observer;
+    const bool observer = !ob.sensor();
+    st.paths++;
+    Emitted em;
+    if (!emitParticle(wl, rng, observer, em, out)) return;
+    Ray ray = em.ray;
+    uint32_t region = em.region;
+    Spec4 beta = em.beta;
+    for (int depth = 0;; ++depth) {
+        Hit hit;
+        const bool found = w_.intersect(ray, kInf, hit);
+        st.segments++;
+        const Medium& med = w_.mediumOf(region);
+        double t;
+        const MediumEvent ev = sampleMedium(med, wl, found ? hit.t : kInf, rng, beta, t);
+        if (ev == MediumEvent::End) break;
+        if (ev == MediumEvent::Scatter) {
+            Hit mh;
+            mh.p = ray.o + ray.d * t;
+            if (observer) connectVertex(mh, false, nullptr, ray.d, med.g, region, beta, wl, rng, out);
+            if (depth >= s_.maxDepth) break;
+            double u1, u2;
+            rng.draw2(u1, u2);
+            V3 wi;
+            sampleHg(-ray.d, med.g, u1, u2, wi);
+            ray = Ray{mh.p, wi, kNone, kNone};
+            if (depth >= s_.rrDepth) {
+                if (rng.uniform() >= 0.95) break;
+                beta *= 1 / 0.95;
+            }
+            continue;
+        }
+        if (!found) {
+            if (region != 0) st.leaks++;
+            break;
+        }
+        const Boundary& b = w_.boundaries[hit.boundary];
+        const bool fromFront = dot(ray.d, hit.n) < 0;
+        const uint32_t inc = fromFront ? b.front : b.back, other = fromFront ? b.back : b.front;
+        if (inc != region) {
+            st.inconsistencies++;
+            region = inc;
+        }
+        const Optics& o = w_.optics[size_t(b.optics)];
+        if (o.type == OWE_SURF_NULL) {
+            region = other;
+            ray = w_.spawn(hit, ray.d);
+            if (depth >= 4 * s_.maxDepth) break;
+            continue;
+        }
+        // The sensor records the flux reaching its front (sensorPixel, particle.slang:98-114).
+        if (ob.sensor() && fromFront && hit.boundary == ob.boundary) {
+            double fx = (hit.pLocal.x / ob.halfX + 1) * 0.5 * ob.width;
+            double fy = (1 - hit.pLocal.y / ob.halfY) * 0.5 * ob.height;
+            if (ob.flipX) fx = ob.width - fx;
+            if (ob.flipY) fy = ob.height - fy;
+            double ox, oy;
+            ob.filterOffset(rng, ox, oy);
+            fx += ox;
+            fy += oy;
+            const double area = (2 * ob.halfX / ob.width) * (2 * ob.halfY / ob.height);
+            if (fx >= 0 && fy >= 0 && fx < ob.width && fy < ob.height)
+                deposit(out, std::min(int(fx), ob.width - 1), std::min(int(fy), ob.height - 1), beta * (1 / area), wl);
+        }
+        if (o.type == OWE_SURF_DETECTOR || o.type == OWE_SURF_ABSORBER) break;
+        if (!o.isDelta() && observer) connectVertex(hit, true, &o, ray.d, 0, region, beta, wl, rng, out);
+        if (depth >= s_.maxDepth) break;
+        // terminateIfDispersive, then the importance-mode scatter (transport.slang:639-684).
+        Interface it{&o, hit.n, hit.p, w_.indexOf(b.front, wl.lambda[0]), w_.indexOf(b.back, wl.lambda[0]),
+                     wl.lambda[0], s_.fresnelFloor};
+        auto at = [&](int i) {
+            Interface r = it;
+            r.nFront = w_.indexOf(b.front, wl.lambda[i]);
+            r.nBack = w_.indexOf(b.back, wl.lambda[i]);
+            r.lambda = wl.lambda[i];
+            return r;
+        };
+        if (o.type == OWE_SURF_DIELECTRIC && !wl.secondaryTerminated())
+            for (int i = 1; i < NW; ++i)
+                if (wl.pdf[i] != 0 && isDispersive(it, at(i))) {
+                    wl.terminateSecondary();
+                    maskTerminated(wl, beta);
+                    break;
+                }
+        double u1, u2, u3;
+        rng.draw3(u1, u2, u3);
+        Sample s;
+        if (!sampleScatter(it, ray.d, u1, u2, u3, !kRadiance, s) || s.weight <= 0) break;
+        beta[0] *= s.weight;
+        for (int i = 1; i < NW; ++i)
+            if (wl.pdf[i] != 0) beta[i] *= secondaryWeight(at(i), ray.d, s, !kRadiance);
+        if (s.transmitted) region = other;
+        ray = w_.spawn(hit, s.wi);
+        // Particles carry flux: roulette against the event's own throughput.
+        if (depth >= s_.rrDepth && s.weight < 1) {
+            if (rng.uniform() >= s.weight) break;
+            beta *= 1 / s.weight;
+        }
+    }
 }
 
 }  // namespace oracle

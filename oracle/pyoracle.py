@@ -161,6 +161,9 @@ class Oracle(_Lib):
             "light_table": (_i, [_vp, _vp, _vp, _i]),
             "render": (None, [_vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp]),
             "camera_paths": (None, [_vp, _vp, _i, _i, _u64, _vp, _vp, _vp, _vp, _i]),
+            "render_particles": (None, [_vp, _vp, _i, _i, _i, _vp, _vp]),
+            "particles": (None, [_vp, _vp, _i, _u64, _vp, _vp, _vp, _i]),
+            "emission_info": (None, [_vp, _vp]),
             "spectrum_luminance_blackbody": (_d, [_d]),
             "medium_index": (_d, [_vp, _d]),
             "ciddor_air": (_d, [_d]),
@@ -233,3 +236,27 @@ class World:
         self.o.camera_paths(self.h, ctypes.byref(settings), spp, pass_index, n, _ptr(pixel), _ptr(sample), _ptr(xyz),
                             _ptr(segs), threads)
         return xyz, segs
+
+    def render_particles(self, settings, ppp: int, passes: int = 1, threads: int = 0, first_pass: int = 0):
+        """Light tracing: resolved XYZ image [H, W, 3] (splats ÷ particles emitted) and stats."""
+        xyz = np.zeros((self.height, self.width, 3))
+        stats = np.zeros(4, dtype=np.uint64)
+        for p in range(first_pass, first_pass + passes):
+            self.o.render_particles(self.h, ctypes.byref(settings), ppp, p, threads, _ptr(xyz), _ptr(stats))
+        return xyz / float(self.width * self.height * ppp * passes), stats
+
+    def particles(self, settings, index, pass_index: int = 0, threads: int = 0):
+        """Per-particle XYZ (summed over its splats, not normalised) and segments."""
+        index = np.ascontiguousarray(index, dtype=np.uint64)
+        n = len(index)
+        xyz = np.zeros((n, 3))
+        segs = np.zeros(n, dtype=np.uint32)
+        self.o.particles(self.h, ctypes.byref(settings), pass_index, n, _ptr(index), _ptr(xyz), _ptr(segs), threads)
+        return xyz, segs
+
+    def emission_info(self) -> dict:
+        out = np.zeros(7)
+        self.o.emission_info(self.h, _ptr(out))
+        return dict(sky_share=out[0], scene_radius=out[1], center=out[2:5].copy(), environment_power=out[5],
+                    light_power=out[6])
+

@@ -139,7 +139,7 @@ void ref_mesh_order(const void* m, uint32_t* out) {
 }
 void ref_mesh_closest(const void* mh, const float* rays, uint64_t n, uint32_t* prim, double* t, int threads) {
     const MeshShape& m = *static_cast<const MeshShape*>(mh);
-    if (threads <= 0) threads = int(std::max(1u, std::thread::hardware_concurrency()));
+    if (threads <= 0) threads = int(std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
     std::vector<std::thread> pool;
     for (int k = 0; k < threads; ++k)
         pool.emplace_back([&, k] {

@@ -151,5 +151,5 @@ def test_unsupported_integrator_is_refused(cuda):
     from owe_hip import _lib
 
     with pytest.raises(_lib.OweHipError) as e:
-        render.Renderer(S.white_furnace_scene(0.5), S.settings(integrator=S.INTEGRATOR_LIGHT))
+        render.Renderer(S.white_furnace_scene(0.5), S.settings(integrator=7))
     assert e.value.code == _lib.ERR_UNSUPPORTED
