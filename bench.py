@@ -12,7 +12,8 @@ bytes per live path (path state 64 B read and written, hit record 16 B, shadow r
 whole pass is also priced at E_seg·224 + 24 B per path with E_seg the measured segments per path.
 cpu_baseline: the oracle's fp64 restatement of Tracer::radiance (the reference's Tracer needs
 stb_image and cannot be built here) on the host threads, full frames of the same scene and spp.
-The ray-query workload is reported beside it under ``rayquery``.
+The ray-query workload is reported beside it under ``rayquery``, and light tracing of the same
+prism scene (256×256×64 particles per pass, the light integrator) under ``light``.
 
 Workload ``rayquery`` (batch ray queries on a BVH): 1,048,820-triangle displaced icosphere
 (frequency 229, the nearest 20·n² count at or above 2^20), 8,388,608 seeded rays resident in HBM.
@@ -249,13 +250,69 @@ def bench_rayquery(args):
     return result
 
 
+def cpu_baseline_light(sc, st, seconds_budget: float = 10.0):
+    sys.path.insert(0, str(ROOT / "oracle"))
+    from pyoracle import Oracle, World
+
+    threads = min(16, os.cpu_count() or 1)
+    w = World(Oracle(), sc)
+    frames, elapsed = 0, 0.0
+    while elapsed < seconds_budget and frames < 64:
+        t0 = time.perf_counter()
+        w.render_particles(st, PRISM_SPP, passes=1, threads=threads, first_pass=frames)
+        elapsed += time.perf_counter() - t0
+        frames += 1
+    n = frames * PRISM_SIZE * PRISM_SIZE * PRISM_SPP
+    return {"value": n / elapsed, "unit": "particles/s", "cores": threads, "kind": "port",
+            "sample": f"{frames} full passes ({PRISM_SIZE}x{PRISM_SIZE}x{PRISM_SPP} particles, {n} particles, {elapsed:.1f} s) "
+                      f"of the same scene through the oracle's fp64 particle restatement, {threads} threads"}
+
+
+def bench_light(args):
+    """Light tracing of the prism scene: 256×256×64 particles per pass from the sun, connected to
+    the eye from every diffuse vertex, splatted exactly."""
+    import torch
+
+    from owe_hip import render
+    from owe_hip import scene as S
+
+    sc = S.dispersion_prism_scene(PRISM_SIZE, PRISM_SIZE)
+    st = S.settings(seed=1, max_depth=16, rr_depth=4, spectral=S.SPECTRAL_UNIFORM1, lambda_min=380.0, lambda_max=780.0,
+                    integrator=S.INTEGRATOR_LIGHT)
+    r = render.Renderer(sc, st)
+    for _ in range(args.warmup):
+        r.run_pass(PRISM_SPP)
+    r.reset()
+    r.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r.run_pass(PRISM_SPP)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    stats = r.stats()
+    ms, launches = r.kernel_times()["shade"]
+    n = stats["paths"]
+    log(f"light: {args.steps} passes in {wall:.3f}s; {n / wall / 1e6:.1f} M particles/s; "
+        f"segments/particle {stats['segments'] / max(1, n):.3f}; particle kernel {ms / args.steps:.3f} ms/pass")
+    out = {"value": n / wall, "unit": "particles/s", "ms_per_step": wall * 1e3 / args.steps,
+           "config": {"workload": "light tracing (particles), dispersion prism", "resolution": [PRISM_SIZE, PRISM_SIZE],
+                      "particles_per_step": PRISM_SIZE * PRISM_SIZE * PRISM_SPP, "max_depth": 16, "rr_depth": 4,
+                      "segments_per_particle": stats["segments"] / max(1, n),
+                      "kernel_ms_per_step": ms / args.steps, "mean_Y": float(r.resolve()[..., 1].mean())}}
+    out["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline_light(sc, st)
+    r.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="prism", choices=["prism", "rayquery"])
+    ap.add_argument("--workload", default="prism", choices=["prism", "rayquery", "light"])
     ap.add_argument("--no-rayquery", action="store_true", help="prism: skip the ray-query record beside it")
+    ap.add_argument("--no-light", action="store_true", help="prism: skip the light-tracing record beside it")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 1 GiB device buffer 3 times (the known-size kernel that calibrates "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -282,6 +339,11 @@ def main() -> int:
             rq = bench_rayquery(argparse.Namespace(steps=args.steps, warmup=args.warmup,
                                                    no_cpu_baseline=args.no_cpu_baseline))
             result["rayquery"] = {k: rq[k] for k in ("value", "unit", "ms_per_step", "config", "roofline", "cpu_baseline")}
+        if not args.no_light:
+            result["light"] = bench_light(argparse.Namespace(steps=args.steps, warmup=args.warmup,
+                                                             no_cpu_baseline=args.no_cpu_baseline))
+    elif args.workload == "light":
+        result = bench_light(args)
     else:
         result = bench_rayquery(args)
     print(json.dumps(result), flush=True)
