@@ -45,7 +45,7 @@ struct PCtx {
     uint32_t segments, inconsistencies, leaks;
 };
 
-__device__ inline void addFixedPoint(unsigned long long* bins, float v) {
+__device__ __forceinline__ void addFixedPoint(unsigned long long* bins, float v) {
     if (!(v > 0.0f) || isinf(v)) return;  // zero, negative or not finite: nothing to add
     const uint32_t bits = __float_as_uint(v);
     const int e = int((bits >> 23) & 0xFFu);
@@ -65,7 +65,7 @@ __device__ inline void addFixedPoint(unsigned long long* bins, float v) {
     if (high != 0 && b + 1 < kSplatBins) atomicAdd(&bins[b + 1], (unsigned long long)high);
 }
 
-__device__ inline void deposit(PCtx& c, int px, int py, const float v[NW]) {
+__device__ __forceinline__ void deposit(PCtx& c, int px, int py, const float v[NW]) {
     float X = 0.0f, Y = 0.0f, Z = 0.0f;
 #pragma unroll
     for (int k = 0; k < NW; ++k) {
@@ -89,7 +89,7 @@ __device__ inline void deposit(PCtx& c, int px, int py, const float v[NW]) {
     addFixedPoint(bins + 2 * kSplatBins, Z);
 }
 
-__device__ inline void filterOffset(PCtx& c, float& fx, float& fy) {
+__device__ __forceinline__ void filterOffset(PCtx& c, float& fx, float& fy) {
     fx = fy = 0.0f;
     const float sigma = c.sc.obs.sigma;
     if (!(sigma > 0.0f)) return;
@@ -100,7 +100,7 @@ __device__ inline void filterOffset(PCtx& c, float& fx, float& fy) {
 }
 
 // The eye sits at the camera-relative origin.
-__device__ inline bool linkObserver(PCtx& c, f3 x, Link& k) {
+__device__ __forceinline__ bool linkObserver(PCtx& c, f3 x, Link& k) {
     const DObserver& ob = c.sc.obs;
     const f3 right = ld3(ob.right), up = ld3(ob.up), fwd = ld3(ob.fwd);
     k.q = mk3(0.0f, 0.0f, 0.0f);
@@ -140,7 +140,7 @@ __device__ inline bool linkObserver(PCtx& c, f3 x, Link& k) {
 
 // A surface vertex (interface `it` at hit boundary bi, face) or a medium point (phase g) joined to
 // the eye, value β·f·|cos|·W·Tr.
-__device__ inline void connectVertex(PCtx& c, f3 p, bool surface, const Iface* it, uint32_t bi, uint32_t face, f3 n,
+__device__ __forceinline__ void connectVertex(PCtx& c, f3 p, bool surface, const Iface* it, uint32_t bi, uint32_t face, f3 n,
                                      f3 d, float g, uint32_t region) {
     Link k;
     if (!linkObserver(c, p, k)) return;
@@ -191,7 +191,7 @@ struct PRay {
 };
 
 // emitParticle (particle.slang:306-325): the sky or a light by power, then its emission.
-__device__ inline bool emitParticle(PCtx& c, bool observer, PRay& ray) {
+__device__ __forceinline__ bool emitParticle(PCtx& c, bool observer, PRay& ray) {
     const DScene& sc = c.sc;
     const float pSky = sc.env.skyShare;
     const bool fromSky = c.rng.uniform() < pSky;
@@ -293,7 +293,7 @@ __device__ inline bool emitParticle(PCtx& c, bool observer, PRay& ray) {
     return any;
 }
 
-__device__ inline bool roulette(PCtx& c, float q) {
+__device__ __forceinline__ bool roulette(PCtx& c, float q) {
     if (c.rng.uniform() >= q) return false;
     const float s = 1.0f / q;
 #pragma unroll
@@ -302,7 +302,7 @@ __device__ inline bool roulette(PCtx& c, float q) {
 }
 
 // traceParticle (particle.slang:356-427), without the hybrid partition.
-__device__ inline void traceParticle(PCtx& c) {
+__device__ __forceinline__ void traceParticle(PCtx& c) {
     const DScene& sc = c.sc;
     const DObserver& ob = sc.obs;
     const bool observer = ob.kind == OWE_DETECTOR_OBSERVER;
@@ -444,7 +444,7 @@ __device__ inline void traceParticle(PCtx& c) {
     }
 }
 
-__device__ inline uint32_t waveSum(uint32_t v) {
+__device__ __forceinline__ uint32_t waveSum(uint32_t v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
     return v;

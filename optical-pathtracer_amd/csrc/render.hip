@@ -212,7 +212,7 @@ __global__ __launch_bounds__(kBlock) void intersectKernel(DScene sc, Paths P, in
     const uint32_t slot = P.queue[q][k];
     const float4 r0 = P.ray0[slot], r1 = P.ray1[slot];
     Hit h;
-    const bool found = intersectWorld(sc, mk3(r0.x, r0.y, r0.z), mk3(r1.x, r1.y, r1.z), __builtin_inff(), P.info[slot].w,
+    const bool found = intersectWorld<false>(sc, mk3(r0.x, r0.y, r0.z), mk3(r1.x, r1.y, r1.z), __builtin_inff(), P.info[slot].w,
                                       P.face[slot], kNoBoundary, stack + threadIdx.x, kBlock, h);
     P.hit[slot] = found ? make_uint4(__float_as_uint(h.t), h.boundary, h.face, h.slot)
                         : make_uint4(__float_as_uint(__builtin_inff()), kNoBoundary, 0u, kEmptyChild);

@@ -414,7 +414,9 @@ __device__ inline bool sphereRoots(f3 o, f3 d, float R, float& t0, float& t1) {
 }
 
 // The nearest crossing in (0, tmax). departed/departedFace: the spawning boundary and (mesh) face;
-// exclude: an analytic boundary the segment ends on (a sampled light point).
+// exclude: an analytic boundary the segment ends on (a sampled light point). kGeometry = false
+// leaves the point and normals unset (a caller that keeps only t, boundary, face and slot).
+template <bool kGeometry = true>
 __device__ inline bool intersectWorld(const DScene& sc, f3 o, f3 d, float tmax, uint32_t departed, uint32_t departedFace,
                                       uint32_t exclude, uint32_t* stack, uint32_t stride, Hit& h) {
     h.boundary = kNoBoundary;
@@ -428,7 +430,7 @@ __device__ inline bool intersectWorld(const DScene& sc, f3 o, f3 d, float tmax, 
             h.slot = rh.slot;
             h.face = __float_as_uint(b.w);
             h.boundary = __float_as_uint(c.w);
-            h.n = norm3(triNormal(mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z)));
+            if (kGeometry) h.n = norm3(triNormal(mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z)));
         }
     }
     const f3 inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
@@ -460,14 +462,15 @@ __device__ inline bool intersectWorld(const DScene& sc, f3 o, f3 d, float tmax, 
         analytic = i;
     }
     if (analytic != kNoBoundary) {
+        h.boundary = analytic;
+        h.face = 0;
+        h.slot = kEmptyChild;
+        if (!kGeometry) return true;
         const DBoundary& b = sc.boundaries[analytic];
         const f3 lo = xfPoint(b.toLocal, o), ldv = xfVector(b.toLocal, d);
         f3 pl = lo + ldv * h.t;
         if (b.shape == OWE_SHAPE_PLANE) pl.z = 0.0f;
         const f3 nl = b.shape == OWE_SHAPE_PLANE ? mk3(0, 0, 1) : norm3(pl / b.a);
-        h.boundary = analytic;
-        h.face = 0;
-        h.slot = kEmptyChild;
         h.pLocal = pl;
         h.nLocal = nl;
         h.p = xfPoint(b.toWorld, pl);
@@ -475,7 +478,7 @@ __device__ inline bool intersectWorld(const DScene& sc, f3 o, f3 d, float tmax, 
         return true;
     }
     if (h.boundary == kNoBoundary) return false;
-    h.p = o + d * h.t;
+    if (kGeometry) h.p = o + d * h.t;
     return true;
 }
 
