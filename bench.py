@@ -53,11 +53,16 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def measured_traffic(path: Path):
-    """HBM bytes per launch recorded by the committed rocprofv3 --pmc summary, if present."""
+def measured_traffic(path: Path, kernel: str | None = None):
+    """HBM bytes per launch of `kernel` recorded by the committed rocprofv3 --pmc summary
+    (a line `traffic_bytes_per_launch[kernel]: B`), if present."""
     if not path.exists():
         return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", path.read_text())
+    text = path.read_text()
+    if kernel is not None:
+        m = re.search(r"traffic_bytes_per_launch\[" + re.escape(kernel) + r"\]:\s*([0-9.eE+]+)", text)
+        return float(m.group(1)) if m else None
+    m = re.search(r"traffic_bytes_per_launch(?:\[[^\]]*\])?:\s*([0-9.eE+]+)", text)
     return float(m.group(1)) if m else None
 
 
@@ -108,40 +113,63 @@ def cpu_baseline_prism(sc, st, seconds_budget: float = 12.0):
                       f"of the same scene through the oracle's fp64 Tracer::radiance, {threads} threads"}
 
 
-def bench_prism(args):
+def _time_passes(r, steps, warmup):
     import torch
 
+    for _ in range(warmup):
+        r.run_pass(PRISM_SPP)
+    r.reset()
+    r.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r.run_pass(PRISM_SPP)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def bench_prism(args):
     from owe_hip import render
     from owe_hip import scene as S
 
     log("prism: building scene")
     sc = S.dispersion_prism_scene(PRISM_SIZE, PRISM_SIZE)
     st = S.settings(seed=1, max_depth=16, rr_depth=4, spectral=S.SPECTRAL_UNIFORM1, lambda_min=380.0, lambda_max=780.0)
-    r = render.Renderer(sc, st)
-    for _ in range(args.warmup):
-        r.run_pass(PRISM_SPP)
-    r.reset()
-    r.set_timing(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        r.run_pass(PRISM_SPP)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    stats = r.stats()
-    times = r.kernel_times()
+    forms = {}
+    # The plain wavefront form, then the persistent batched form with its autotuned launch shape
+    # and graph replay; the two images are equal bit for bit (checked here on the timed passes).
+    for form in ("wavefront", "persistent"):
+        r = render.Renderer(sc, st)
+        if form == "persistent":
+            r.set_mode(render.Renderer.PERSISTENT, use_graph=True)
+        tuned_ms = r.autotune(PRISM_SPP)
+        wall = _time_passes(r, args.steps, args.warmup)
+        stats, times = r.stats(), r.kernel_times()
+        forms[form] = dict(wall=wall, stats=stats, times=times, mode=r.mode(), tuned_ms=tuned_ms, img=r.resolve())
+        paths = stats["paths"]
+        log(f"prism {form}: {args.steps} passes in {wall:.3f}s; {paths / wall / 1e6:.1f} M paths/s; "
+            f"E_seg {stats['segments'] / max(1, paths):.3f}; shape {r.mode()}; "
+            + ", ".join(f"{k} {ms / args.steps:.3f} ms/pass ({n})" for k, (ms, n) in times.items() if n))
+        r.close()
+    same = bool((forms["wavefront"]["img"] == forms["persistent"]["img"]).all())
+    best = max(forms, key=lambda f: forms[f]["stats"]["paths"] / forms[f]["wall"])
+    F = forms[best]
+    wall, stats, times = F["wall"], F["stats"], F["times"]
     paths = stats["paths"]
     eseg = stats["segments"] / max(1, paths)
-    total_ms = sum(ms for ms, _ in times.values())
-    dominant = max(times, key=lambda k: times[k][0])
-    dom_ms, dom_launches = times[dominant]
-    live = stats["segments"] if dominant in ("intersect", "shade", "shadow") else paths
-    dom_bytes = KERNEL_BYTES[dominant] * live
-    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     pass_bytes = paths * (eseg * 224 + 24)
-    log(f"prism: {args.steps} passes in {wall:.3f}s; {paths / wall / 1e6:.1f} M paths/s; E_seg {eseg:.3f}; "
-        + ", ".join(f"{k} {ms / args.steps:.3f} ms/pass ({n} launches)" for k, (ms, n) in times.items()))
-    img = r.resolve()
+    if best == "persistent":
+        dominant = "persistent"
+        dom_ms, dom_launches = times["persistent"]
+        dom_bytes = pass_bytes
+    else:
+        dominant = max(("generate", "intersect", "shade", "shadow", "splat"), key=lambda k: times[k][0])
+        dom_ms, dom_launches = times[dominant]
+        live = stats["segments"] if dominant in ("intersect", "shade", "shadow") else paths
+        dom_bytes = KERNEL_BYTES[dominant] * live
+    total_ms = sum(ms for ms, _ in times.values())
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    img = F["img"]
     result = {
         "metric": "spectral paths/s at fixed rel. RMSE vs fp64 ref; adjoint grads/s; HBM GB/s",
         "value": paths / wall,
@@ -155,19 +183,27 @@ def bench_prism(args):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (procedural dispersion-prism scene, seeded streams)",
-        "config": {"workload": "plain spectral path tracing, dispersion prism (wavefront)", "resolution": [PRISM_SIZE, PRISM_SIZE],
-                   "spp": PRISM_SPP, "paths_per_step": PRISM_SIZE * PRISM_SIZE * PRISM_SPP, "max_depth": 16,
-                   "rr_depth": 4, "spectral": "1 wavelength per path, uniform 380-780 nm", "segments_per_path": eseg,
-                   "kernel_ms_per_step": {k: ms / args.steps for k, (ms, _) in times.items()},
-                   "inconsistencies": stats["inconsistencies"], "leaks": stats["leaks"], "mean_Y": float(img[..., 1].mean())},
+        "config": {"workload": f"plain spectral path tracing, dispersion prism ({best} form)",
+                   "resolution": [PRISM_SIZE, PRISM_SIZE], "spp": PRISM_SPP,
+                   "paths_per_step": PRISM_SIZE * PRISM_SIZE * PRISM_SPP, "max_depth": 16, "rr_depth": 4,
+                   "spectral": "1 wavelength per path, uniform 380-780 nm", "segments_per_path": eseg,
+                   "kernel_ms_per_step": {k: ms / args.steps for k, (ms, n) in times.items() if n},
+                   "inconsistencies": stats["inconsistencies"], "leaks": stats["leaks"],
+                   "mean_Y": float(img[..., 1].mean()),
+                   "forms": {f: {"paths_per_s": forms[f]["stats"]["paths"] / forms[f]["wall"],
+                                 "ms_per_step": forms[f]["wall"] * 1e3 / args.steps, "launch": forms[f]["mode"],
+                                 "autotuned_pass_ms": forms[f]["tuned_ms"]} for f in forms},
+                   "persistent_over_wavefront": (forms["persistent"]["stats"]["paths"] / forms["persistent"]["wall"]) /
+                                                (forms["wavefront"]["stats"]["paths"] / forms["wavefront"]["wall"]),
+                   "forms_bit_identical": same},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(PROFILE_PRISM_TRAFFIC),
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(PROFILE_PRISM_TRAFFIC, f"{dominant}Kernel"),
                      "kernel": f"{dominant}Kernel", "algorithmic_bytes_per_launch": dom_bytes / max(1, dom_launches),
                      "pass": {"algorithmic_bytes": pass_bytes / args.steps, "kernel_ms": total_ms / args.steps,
                               "achieved_GBs": pass_bytes / (total_ms * 1e-3) / 1e9}},
     }
     result["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline_prism(sc, st)
-    r.close()
     return result
 
 
@@ -238,7 +274,7 @@ def bench_rayquery(args):
                    "closest_ms": closest_ms, "any_ms": any_ms,
                    "closest_rays_per_s": n / (closest_ms * 1e-3), "any_rays_per_s": n / (any_ms * 1e-3)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(PROFILE_TRAFFIC),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(PROFILE_TRAFFIC, "traceBatchKernel<closest>"),
                      "kernel": "traceBatchKernel<closest>", "algorithmic_bytes_per_launch": algo_bytes},
     }
     if not args.no_cpu_baseline:
@@ -306,6 +342,8 @@ def bench_light(args):
 
 
 def main() -> int:
+    import torch  # noqa: F401  (first: its HIP runtime is the process's; see owe_hip/_lib.py)
+
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)

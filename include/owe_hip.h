@@ -128,12 +128,28 @@ int owe_hip_renderer_camera_paths(owe_hip_renderer* renderer, int spp, int pass,
  * the particle count), without touching the film: the light integrator's kernel, for per-particle
  * comparison with the oracle. Host arrays; xyz receives n × 3 doubles. Light-integrator renderers only. */
 int owe_hip_renderer_particles(owe_hip_renderer* renderer, int pass, const uint64_t* index, uint64_t n, double* xyz);
+/* The two forms of the camera-path integrator: the wavefront (one kernel per stage over SoA queues,
+ * the default) and the persistent batched form (a grid sized to the chip drains a device queue of
+ * path ids, each lane tracing a path to its end in registers with the same device functions). The
+ * film is integer, so both give the same image bit for bit. blocks_per_cu and chunk (path ids per
+ * lane fetch) shape the persistent launch (0 keeps the current value); use_graph replays each pass
+ * from a captured HIP graph. No counterpart in the reference (whose GPU backend has one form). */
+#define OWE_HIP_MODE_WAVEFRONT 0
+#define OWE_HIP_MODE_PERSISTENT 1
+int owe_hip_renderer_set_mode(owe_hip_renderer* renderer, int mode, int blocks_per_cu, int chunk, int use_graph);
+int owe_hip_renderer_get_mode(const owe_hip_renderer* renderer, int* mode, int* blocks_per_cu, int* chunk,
+                              int* use_graph);
+/* Times passes of spp samples per pixel under candidate launch shapes of the current form
+ * (persistent: blocks per compute unit × chunk; wavefront: bounce rounds between host reads of the
+ * live counters), keeps the fastest, and clears the film and statistics again. Only on a renderer
+ * without passes. best_ms (optional) receives the winning pass time. */
+int owe_hip_renderer_autotune(owe_hip_renderer* renderer, int spp, double* best_ms);
 /* Per-kernel timing with HIP events on the renderer's stream (the reference's OWE_GPU_PROFILE,
  * src/owe/backends/gpu/vulkan_context.cpp:350-422). Enabling clears the totals. */
 int owe_hip_renderer_set_timing(owe_hip_renderer* renderer, int enable);
 /* Milliseconds and launches per kernel since timing was enabled, in the order generate, intersect,
- * shade, shadow, splat (5 entries each); a light-integrator renderer counts its particle kernel
- * as shade. */
+ * shade, shadow, splat, persistent (6 entries each); a light-integrator renderer counts its particle
+ * kernel as shade. */
 int owe_hip_renderer_kernel_times(const owe_hip_renderer* renderer, double* ms, uint64_t* launches);
 
 #ifdef __cplusplus

@@ -59,7 +59,7 @@ struct Paths {
     unsigned long long* segments;  // segments traced in the current chunk (sum of live counts)
 };
 
-__device__ inline WL loadWL(const float4 l, uint32_t flags, const Settings& s) {
+__device__ __forceinline__ WL loadWL(const float4 l, uint32_t flags, const Settings& s) {
     WL w;
     w.lambda[0] = l.x;
     w.lambda[1] = l.y;
@@ -81,21 +81,20 @@ __device__ inline WL loadWL(const float4 l, uint32_t flags, const Settings& s) {
 
 // ---------------------------------------------------------------- generate
 
-__global__ __launch_bounds__(kBlock) void generateKernel(DScene sc, Settings set, Paths P, uint64_t base, uint32_t count,
-                                                          uint32_t spp, uint32_t pass, const uint32_t* pixelList,
-                                                          const uint32_t* sampleList) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= count) return;
+// A camera path's first state (CpuRenderer::pass per sample, cpu_backend.cpp:66-81, and
+// IdealObserver/SurfaceSensor::generate, detector.cpp:61-149), shared by the wavefront and the
+// persistent kernels so both trace the same float operations.
+struct PathStart {
+    f3 o, d;
+    float weight;
+    float4 lam, beta;
+    uint32_t region, departed;
+    uint64_t rngState;
+};
+
+__device__ __forceinline__ void startPath(const DScene& sc, const Settings& set, uint64_t pixel, uint64_t s, uint32_t spp,
+                                 uint32_t pass, PathStart& out) {
     const DObserver& ob = sc.obs;
-    uint64_t pixel, s;
-    if (pixelList) {
-        pixel = pixelList[i];
-        s = sampleList[i];
-    } else {
-        const uint64_t g = base + i;
-        pixel = g / spp;
-        s = g % spp;
-    }
     const uint32_t x = uint32_t(pixel % uint64_t(ob.width)), y = uint32_t(pixel / uint64_t(ob.width));
     Rng pr;
     pr.seed(hashPair(set.seed, pixel), pass);
@@ -128,6 +127,7 @@ __global__ __launch_bounds__(kBlock) void generateKernel(DScene sc, Settings set
         fx = r * cosf(phi) * ob.sigma;
         fy = r * sinf(phi) * ob.sigma;
     }
+    out.lam = lam;
     if (ob.kind == OWE_DETECTOR_SENSOR) {
         // SurfaceSensor::generate (detector.cpp:110-149) without the focus disk: a point of the
         // sensor, a direction over its front (cosine or toward the aim disk), weight cos/pdf.
@@ -162,16 +162,13 @@ __global__ __launch_bounds__(kBlock) void generateKernel(DScene sc, Settings set
             }
         }
         // A sample the sensor does not measure (weight 0) still takes its slot and adds nothing.
-        P.ray0[i] = make_float4(p.x, p.y, p.z, 0.0f);
-        P.ray1[i] = make_float4(dir.x, dir.y, dir.z, weight);
-        P.beta[i] = weight > 0.0f ? beta : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        P.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        P.lam[i] = lam;
-        P.prevP[i] = make_float4(p.x, p.y, p.z, 0.0f);
-        P.info[i] = make_uint4(uint32_t(pixel), kFlagSpecular << 16, ob.region, ob.sensor);
-        P.face[i] = ~0u;
-        P.rng[i] = make_uint2(uint32_t(rng.state), uint32_t(rng.state >> 32));
-        P.queue[0][i] = i;
+        out.o = p;
+        out.d = dir;
+        out.weight = weight;
+        out.beta = weight > 0.0f ? beta : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        out.region = ob.region;
+        out.departed = ob.sensor;
+        out.rngState = rng.state;
         return;
     }
     const float sx = (2.0f * (px + fx) / float(ob.width) - 1.0f) * ob.tanX;
@@ -186,15 +183,40 @@ __global__ __launch_bounds__(kBlock) void generateKernel(DScene sc, Settings set
         q = (right * dx + up * dy) * ob.pupil;
         if (isfinite(ob.focus)) dir = norm3(aim * ob.focus - q);
     }
-    P.ray0[i] = make_float4(q.x, q.y, q.z, 0.0f);
-    P.ray1[i] = make_float4(dir.x, dir.y, dir.z, 1.0f);
-    P.beta[i] = beta;
+    out.o = q;
+    out.d = dir;
+    out.weight = 1.0f;
+    out.beta = beta;
+    out.region = ob.region;
+    out.departed = kNoBoundary;
+    out.rngState = rng.state;
+}
+
+__global__ __launch_bounds__(kBlock) void generateKernel(DScene sc, Settings set, Paths P, uint64_t base, uint32_t count,
+                                                          uint32_t spp, uint32_t pass, const uint32_t* pixelList,
+                                                          const uint32_t* sampleList) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= count) return;
+    uint64_t pixel, s;
+    if (pixelList) {
+        pixel = pixelList[i];
+        s = sampleList[i];
+    } else {
+        const uint64_t g = base + i;
+        pixel = g / spp;
+        s = g % spp;
+    }
+    PathStart ps;
+    startPath(sc, set, pixel, s, spp, pass, ps);
+    P.ray0[i] = make_float4(ps.o.x, ps.o.y, ps.o.z, 0.0f);
+    P.ray1[i] = make_float4(ps.d.x, ps.d.y, ps.d.z, ps.weight);
+    P.beta[i] = ps.beta;
     P.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    P.lam[i] = lam;
-    P.prevP[i] = make_float4(q.x, q.y, q.z, 0.0f);
-    P.info[i] = make_uint4(uint32_t(pixel), kFlagSpecular << 16, ob.region, kNoBoundary);
+    P.lam[i] = ps.lam;
+    P.prevP[i] = make_float4(ps.o.x, ps.o.y, ps.o.z, 0.0f);
+    P.info[i] = make_uint4(uint32_t(pixel), kFlagSpecular << 16, ps.region, ps.departed);
     P.face[i] = ~0u;
-    P.rng[i] = make_uint2(uint32_t(rng.state), uint32_t(rng.state >> 32));
+    P.rng[i] = make_uint2(uint32_t(ps.rngState), uint32_t(ps.rngState >> 32));
     P.queue[0][i] = i;
 }
 
@@ -237,7 +259,7 @@ struct ShadeCtx {
 
 // Next-event estimation at point p (sampleLightFrom + directLighting, transport.cpp:35-70, 323-348):
 // draws the light, evaluates everything but the transmittance, and leaves a shadow ray.
-__device__ inline void prepareDirect(ShadeCtx& c, f3 p, bool surface, const Iface* it, uint32_t hitBoundary,
+__device__ __forceinline__ void prepareDirect(ShadeCtx& c, f3 p, bool surface, const Iface* it, uint32_t hitBoundary,
                                      uint32_t hitFace, uint32_t hitSlot, f3 n, float g) {
     const DScene& sc = c.sc;
     if (sc.nLights == 0) return;
@@ -321,7 +343,7 @@ __device__ inline void prepareDirect(ShadeCtx& c, f3 p, bool surface, const Ifac
     c.shFace = surface && hitSlot != kEmptyChild ? hitFace : ~0u;
 }
 
-__device__ inline bool survives(ShadeCtx& c) {
+__device__ __forceinline__ bool survives(ShadeCtx& c) {
     const float q = fminf(1.0f, fmaxf(fmaxf(c.beta.x, c.beta.y), fmaxf(c.beta.z, c.beta.w)));
     if (c.rng.uniform() >= q) return false;
     const float s = 1.0f / q;
@@ -330,7 +352,7 @@ __device__ inline bool survives(ShadeCtx& c) {
 }
 
 // One iteration of Tracer::radiance's loop after the intersection; false when the path ends.
-__device__ inline bool shadeOne(ShadeCtx& c, const uint4 hraw, uint32_t* counters) {
+__device__ __forceinline__ bool shadeOne(ShadeCtx& c, const uint4 hraw, uint32_t* counters) {
     const DScene& sc = c.sc;
     const bool found = hraw.y != kNoBoundary;
     const float t = __uint_as_float(hraw.x);
@@ -554,42 +576,28 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(DScene sc, Settings set, P
 // ---------------------------------------------------------------- shadow rays
 
 // Tracer::transmittance (transport.cpp:237-285): through null boundaries with Beer–Lambert.
-template <int kCap>
-__device__ inline void shadowOne(const DScene& sc, const Paths& P, uint32_t slot, int simple, uint32_t* stack);
-
-template <int kCap>
-__global__ __launch_bounds__(kBlock) void shadowKernel(DScene sc, Settings set, Paths P, int q, int simple) {
-    __shared__ uint32_t stack[kCap * kBlock];
-    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-    if (k < P.count[q]) shadowOne<kCap>(sc, P, P.queue[q][k], simple, stack + threadIdx.x);
-}
-
-template <int kCap>
-__device__ inline void shadowOne(const DScene& sc, const Paths& P, uint32_t slot, int simple, uint32_t* stack) {
-    const float4 pend = P.shPend[slot];
-    if (pend.x == 0.0f && pend.y == 0.0f && pend.z == 0.0f && pend.w == 0.0f) return;
-    const float4 so = P.shO[slot], sd = P.shD[slot];
-    const uint4 si = P.shInfo[slot];
-    f3 o = mk3(so.x, so.y, so.z);
-    const f3 d = mk3(sd.x, sd.y, sd.z);
-    float remaining = so.w;
+// Tracer::transmittance of one pending NEE ray (transport.cpp:237-285): through null boundaries
+// with Beer–Lambert; false when blocked. With `simple` (no null boundaries, no media in the scene)
+// any crossing blocks. Shared by the wavefront and the persistent kernels.
+__device__ __forceinline__ bool shadowTransmittance(const DScene& sc, f3 o, f3 d, float remaining, uint32_t region,
+                                           uint32_t arrival, uint32_t exclude, uint32_t departed, uint32_t face,
+                                           const float4 lam, int simple, uint32_t* stack, float Tr[NW]) {
     const bool infinite = isinf(remaining);
-    uint32_t departed = si.w, face = P.shFace[slot], region = si.x;
-    float Tr[NW] = {1.0f, 1.0f, 1.0f, 1.0f};
-    const float4 lam = P.lam[slot];
+#pragma unroll
+    for (int i = 0; i < NW; ++i) Tr[i] = 1.0f;
     for (int guard = 0; guard < 64; ++guard) {
         Hit h;
-        const bool found = intersectWorld(sc, o, d, remaining, departed, face, si.z, stack, kBlock, h);
+        const bool found = intersectWorld(sc, o, d, remaining, departed, face, exclude, stack, kBlock, h);
         if (simple) {
             // No null boundaries or participating media in the scene: any crossing blocks.
-            if (found) return;
-            if (infinite && region != 0) return;
+            if (found) return false;
+            if (infinite && region != 0) return false;
             break;
         }
         const float seg = found ? fminf(h.t, remaining) : remaining;
         const DMedium& m = sc.media[sc.regionMedium[region]];
         if (m.flags) {
-            if (isinf(seg)) return;
+            if (isinf(seg)) return false;
 #pragma unroll
             for (int i = 0; i < NW; ++i) {
                 const float l = get(lam, i);
@@ -597,33 +605,52 @@ __device__ inline void shadowOne(const DScene& sc, const Paths& P, uint32_t slot
             }
         }
         if (!found) {
-            if (infinite && region != 0) return;
-            if (!infinite && region != si.y) return;
+            if (infinite && region != 0) return false;
+            if (!infinite && region != arrival) return false;
             break;
         }
         const DBoundary& b = sc.boundaries[h.boundary];
         const bool front = dot3(d, h.n) < 0.0f;
-        if ((front ? b.front : b.back) != region) return;
-        if (sc.optics[b.optics].type != OWE_SURF_NULL) return;
+        if ((front ? b.front : b.back) != region) return false;
+        if (sc.optics[b.optics].type != OWE_SURF_NULL) return false;
         region = front ? b.back : b.front;
         o = h.p;
         departed = h.boundary;
         face = h.slot != kEmptyChild ? h.face : ~0u;
         if (!infinite) remaining -= h.t;
-        if (guard == 63) return;  // exhausting the guard is not evidence the segment is clear
+        if (guard == 63) return false;  // exhausting the guard is not evidence the segment is clear
     }
-    if (simple && !infinite && region != si.y) return;
-    float4 L = P.L[slot];
+    return !(simple && !infinite && region != arrival);
+}
+
+__device__ __forceinline__ float4 addShadow(float4 L, const float4 pend, const float Tr[NW]) {
     L.x += pend.x * Tr[0];
     L.y += pend.y * Tr[1];
     L.z += pend.z * Tr[2];
     L.w += pend.w * Tr[3];
-    P.L[slot] = L;
+    return L;
+}
+
+template <int kCap>
+__global__ __launch_bounds__(kBlock) void shadowKernel(DScene sc, Settings set, Paths P, int q, int simple) {
+    __shared__ uint32_t stack[kCap * kBlock];
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= P.count[q]) return;
+    const uint32_t slot = P.queue[q][k];
+    const float4 pend = P.shPend[slot];
+    if (pend.x == 0.0f && pend.y == 0.0f && pend.z == 0.0f && pend.w == 0.0f) return;
+    const float4 so = P.shO[slot], sd = P.shD[slot];
+    const uint4 si = P.shInfo[slot];
+    float Tr[NW];
+    if (!shadowTransmittance(sc, mk3(so.x, so.y, so.z), mk3(sd.x, sd.y, sd.z), so.w, si.x, si.y, si.z, si.w,
+                             P.shFace[slot], P.lam[slot], simple, stack + threadIdx.x, Tr))
+        return;
+    P.L[slot] = addShadow(P.L[slot], pend, Tr);
 }
 
 // ---------------------------------------------------------------- splat
 
-__device__ inline long long fixedPoint(float v) {
+__device__ __forceinline__ long long fixedPoint(float v) {
     // |value| stays below 2^30 units of XYZ: far above any sample here, and no overflow of the
     // 64-bit two's-complement sum.
     return llrint(fmin(fmax(double(v) * kFilmScale, -4.6e18), 4.6e18));
@@ -632,28 +659,32 @@ __device__ inline long long fixedPoint(float v) {
 // Each path's XYZ, then one film atomic per run of same-pixel lanes in the wave: a segmented suffix
 // sum of 64-bit integers (exact, so the film stays independent of order) over the lanes' pixels,
 // which are contiguous runs because path g belongs to pixel g / spp.
+// A path's XYZ (Wavelengths::toXYZ, core/wavelengths.hpp:71-83) from its estimate per wavelength,
+// its camera weight and its final wavelengths; false when not finite (the sample is dropped).
+__device__ __forceinline__ bool pathXYZ(const float4 L, float weight, const WL& wl, float invNWY, float& X, float& Y, float& Z) {
+    X = Y = Z = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+        const float val = get(L, k) * weight;
+        if (wl.pdf[k] == 0.0f || val == 0.0f) continue;
+        const float w = val / wl.pdf[k] * invNWY;
+        const f3 c = cieXYZ(wl.lambda[k]);
+        X += w * c.x;
+        Y += w * c.y;
+        Z += w * c.z;
+    }
+    return isfinite(X + Y + Z);
+}
+
 __global__ __launch_bounds__(kBlock) void splatKernel(Settings set, Paths P, uint32_t count, float invNWY,
                                                       unsigned long long* film, double* perPath) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     long long v[3] = {0, 0, 0};
     uint32_t pix = 0xFFFFFFFFu;
     if (i < count) {
-        const float4 L = P.L[i];
         const uint4 info = P.info[i];
-        const WL wl = loadWL(P.lam[i], info.y >> 16, set);
-        const float weight = P.ray1[i].w;
-        float X = 0.0f, Y = 0.0f, Z = 0.0f;
-#pragma unroll
-        for (int k = 0; k < NW; ++k) {
-            const float val = get(L, k) * weight;
-            if (wl.pdf[k] == 0.0f || val == 0.0f) continue;
-            const float w = val / wl.pdf[k] * invNWY;
-            const f3 c = cieXYZ(wl.lambda[k]);
-            X += w * c.x;
-            Y += w * c.y;
-            Z += w * c.z;
-        }
-        const bool ok = isfinite(X + Y + Z);
+        float X, Y, Z;
+        const bool ok = pathXYZ(P.L[i], P.ray1[i].w, loadWL(P.lam[i], info.y >> 16, set), invNWY, X, Y, Z);
         if (perPath) {
             perPath[3 * size_t(i)] = ok ? X : 0.0;
             perPath[3 * size_t(i) + 1] = ok ? Y : 0.0;
@@ -696,6 +727,130 @@ __global__ void setCount(uint32_t* c, unsigned long long* segments, uint32_t cou
     if (threadIdx.x == 0) *segments = 0;
 }
 
+// ---------------------------------------------------------------- persistent form
+
+// The batched persistent form of the same camera loop: a grid sized to the chip keeps one path per
+// lane in registers and advances it one segment per iteration (intersect, shade, shadow — the very
+// device functions of the wavefront kernels, so every path's float operations are the same); a lane
+// whose path ends splats it and takes the next path id from its chunk, or a new chunk of `chunk`
+// consecutive ids from a device counter (one atomic per wave per refill). Splats of consecutive
+// paths of one pixel accumulate in the lane and reach the film as one 64-bit atomic per channel.
+// The film is integer, so it equals the wavefront's bit for bit.
+struct Persist {
+    uint64_t total;            // paths of the pass: path g is sample g % spp of pixel g / spp
+    uint32_t spp, chunk;
+    const uint32_t* pass;      // the pass index, read on the device (a graph replays with a new one)
+    unsigned long long* next;  // path ids handed out so far
+    unsigned long long* film;
+    unsigned long long* segments;
+    uint32_t* counters;        // [2] inconsistencies, [3] leaks (as the wavefront's)
+    float invNWY;
+    int simple;
+};
+
+__global__ void persistBegin(Persist a) {
+    if (threadIdx.x == 0) {
+        *a.next = 0;
+        *a.segments = 0;
+    }
+    if (threadIdx.x < 4) a.counters[threadIdx.x] = 0;
+}
+
+__device__ __forceinline__ void flushFilm(unsigned long long* film, uint32_t pix, long long acc[3]) {
+    if (pix == 0xFFFFFFFFu) return;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        if (acc[c] != 0) atomicAdd(&film[3 * size_t(pix) + c], (unsigned long long)acc[c]);
+    acc[0] = acc[1] = acc[2] = 0;
+}
+
+template <int kCap>
+__global__ __launch_bounds__(kBlock) void persistentKernel(DScene sc, Settings set, Persist a) {
+    __shared__ uint32_t stack[kCap * kBlock];
+    uint32_t* st = stack + threadIdx.x;
+    const uint32_t pass = *a.pass;
+    const uint64_t passInc = (uint64_t(pass) << 1) | 1u;
+    const uint32_t lane = __lane_id();
+    ShadeCtx c{sc, set};
+    bool live = false, done = false;
+    uint64_t id = 0, end = 0;
+    uint32_t accPix = 0xFFFFFFFFu, segments = 0;
+    long long acc[3] = {0, 0, 0};
+    for (;;) {
+        const bool needChunk = !live && !done && id >= end;
+        const uint64_t ask = __ballot(needChunk);
+        if (ask) {
+            const int leader = __ffsll((long long)ask) - 1;
+            unsigned long long base = 0;
+            if (int(lane) == leader) base = atomicAdd(a.next, (unsigned long long)__popcll(ask) * a.chunk);
+            base = __shfl(base, leader);
+            if (needChunk) {
+                id = base + uint64_t(__popcll(ask & ((1ull << lane) - 1ull))) * a.chunk;
+                end = id + a.chunk < a.total ? id + a.chunk : a.total;
+                done = id >= a.total;
+            }
+        }
+        if (!live && !done) {
+            const uint64_t pixel = id / a.spp;
+            PathStart ps;
+            startPath(sc, set, pixel, id % a.spp, a.spp, pass, ps);
+            c.rng.state = ps.rngState;
+            c.rng.inc = passInc;
+            c.flags = kFlagSpecular;
+            c.depth = 0;
+            c.wl = loadWL(ps.lam, c.flags, set);
+            c.beta = ps.beta;
+            c.L = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            c.o = ps.o;
+            c.prevPdf = 0.0f;
+            c.d = ps.d;
+            c.weight = ps.weight;
+            c.prevP = ps.o;
+            c.pixel = uint32_t(pixel);
+            c.region = ps.region;
+            c.departed = ps.departed;
+            c.face = ~0u;
+            live = true;
+            ++id;
+        }
+        if (__ballot(live) == 0) break;  // every lane of the wave is done: the queue is empty
+        if (!live) continue;
+        Hit h;
+        const bool found = intersectWorld<false>(sc, c.o, c.d, __builtin_inff(), c.departed, c.face, kNoBoundary, st,
+                                                 kBlock, h);
+        ++segments;
+        const uint4 hraw = found ? make_uint4(__float_as_uint(h.t), h.boundary, h.face, h.slot)
+                                 : make_uint4(__float_as_uint(__builtin_inff()), kNoBoundary, 0u, kEmptyChild);
+        c.shPend = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const bool alive = shadeOne(c, hraw, a.counters);
+        if (c.shPend.x != 0.0f || c.shPend.y != 0.0f || c.shPend.z != 0.0f || c.shPend.w != 0.0f) {
+            float Tr[NW];
+            const float4 lam = make_float4(c.wl.lambda[0], c.wl.lambda[1], c.wl.lambda[2], c.wl.lambda[3]);
+            if (shadowTransmittance(sc, c.shO, c.shD, c.shDist, c.shStart, c.shLightRegion, c.shExclude, c.shDeparted,
+                                    c.shFace, lam, a.simple, st, Tr))
+                c.L = addShadow(c.L, c.shPend, Tr);
+        }
+        c.depth += 1;
+        if (!alive) {
+            live = false;
+            float X, Y, Z;
+            if (pathXYZ(c.L, c.weight, c.wl, a.invNWY, X, Y, Z)) {
+                if (c.pixel != accPix) {
+                    flushFilm(a.film, accPix, acc);
+                    accPix = c.pixel;
+                }
+                acc[0] += fixedPoint(X);
+                acc[1] += fixedPoint(Y);
+                acc[2] += fixedPoint(Z);
+            }
+        }
+    }
+    flushFilm(a.film, accPix, acc);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) segments += __shfl_xor(segments, off);
+    if (lane == 0 && segments) atomicAdd(a.segments, (unsigned long long)segments);
+}
+
 }  // namespace
 }  // namespace owehip
 
@@ -733,9 +888,22 @@ struct owe_hip_renderer {
     // shade, shadow, splat.
     int roundsPerSync = 4;     // bounce rounds enqueued between host reads of the live counters
     bool timing = false;
-    hipEvent_t ev[2][5] = {};
-    double kernelMs[5] = {0, 0, 0, 0, 0};
-    uint64_t kernelLaunches[5] = {0, 0, 0, 0, 0};
+    hipEvent_t ev[2][6] = {};
+    double kernelMs[6] = {0, 0, 0, 0, 0, 0};
+    uint64_t kernelLaunches[6] = {0, 0, 0, 0, 0, 0};
+    // The persistent form: mode, launch shape, its device counters and the captured pass graph.
+    int mode = OWE_HIP_MODE_WAVEFRONT;
+    int computeUnits = 256;
+    int blocksPerCU = 2;
+    uint32_t chunk = 4;
+    bool useGraph = false;
+    unsigned long long* persistNext = nullptr;
+    unsigned long long* persistSegments = nullptr;
+    uint32_t* persistCounters = nullptr;
+    uint32_t* passSlot = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+    uint64_t graphTotal = 0;  // the pass size the graph was captured for
 };
 
 namespace owehip {
@@ -781,7 +949,8 @@ void allocPaths(owe_hip_renderer* r, uint32_t n) {
 
 dim3 gridFor(uint64_t n) { return dim3{unsigned((n + kBlock - 1) / kBlock)}; }
 
-enum { kGenerate = 0, kIntersect = 1, kShade = 2, kShadow = 3, kSplat = 4 };
+enum { kGenerate = 0, kIntersect = 1, kShade = 2, kShadow = 3, kSplat = 4, kPersistent = 5 };
+constexpr int kTimers = 6;
 
 // Brackets a launch with events when timing is on; the elapsed time is collected after the next
 // stream synchronisation (collectTimes).
@@ -883,6 +1052,76 @@ void runPaths(owe_hip_renderer* r, uint64_t base, uint32_t count, uint32_t spp, 
         if (generated) collectTimes(r, {kGenerate});
         collectTimes(r, {kSplat});
     }
+}
+
+__global__ void setPass(uint32_t* slot, uint32_t pass) {
+    if (threadIdx.x == 0) *slot = pass;
+}
+
+Persist persistArgs(owe_hip_renderer* r, uint64_t total, uint32_t spp) {
+    Persist a{};
+    a.total = total;
+    a.spp = spp;
+    a.chunk = r->chunk;
+    a.pass = r->passSlot;
+    a.next = r->persistNext;
+    a.film = r->film;
+    a.segments = r->persistSegments;
+    a.counters = r->persistCounters;
+    a.invNWY = r->invNWY;
+    a.simple = int(r->simpleShadows);
+    return a;
+}
+
+void launchPersistent(owe_hip_renderer* r, const Persist& a) {
+    hipLaunchKernelGGL(persistBegin, dim3{1}, dim3{64}, 0, r->stream, a);
+    const dim3 grid{unsigned(std::max(1, r->computeUnits * r->blocksPerCU))};
+    if (r->stackCap <= 16) hipLaunchKernelGGL(persistentKernel<16>, grid, dim3{kBlock}, 0, r->stream, r->dev, r->set, a);
+    else if (r->stackCap <= 32) hipLaunchKernelGGL(persistentKernel<32>, grid, dim3{kBlock}, 0, r->stream, r->dev, r->set, a);
+    else hipLaunchKernelGGL(persistentKernel<48>, grid, dim3{kBlock}, 0, r->stream, r->dev, r->set, a);
+}
+
+void dropGraph(owe_hip_renderer* r) {
+    if (r->graphExec) OWE_HIP_CHECK(hipGraphExecDestroy(r->graphExec));
+    if (r->graph) OWE_HIP_CHECK(hipGraphDestroy(r->graph));
+    r->graphExec = nullptr;
+    r->graph = nullptr;
+    r->graphTotal = 0;
+}
+
+// One pass of the persistent form: the pass index goes to the device, then the counters are cleared
+// and the persistent grid drains the pass's path ids (both in one captured graph when enabled; the
+// graph holds kernels only, no memset nodes).
+void runPersistent(owe_hip_renderer* r, uint32_t spp, uint32_t pass) {
+    const uint64_t total = uint64_t(r->host.obs.width) * uint64_t(r->host.obs.height) * uint64_t(spp);
+    const Persist a = persistArgs(r, total, spp);
+    hipLaunchKernelGGL(setPass, dim3{1}, dim3{64}, 0, r->stream, r->passSlot, pass);
+    timed(r, kPersistent, [&] {
+        if (!r->useGraph) {
+            launchPersistent(r, a);
+            return;
+        }
+        if (r->graphExec && r->graphTotal != total) dropGraph(r);
+        if (!r->graphExec) {
+            OWE_HIP_CHECK(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
+            launchPersistent(r, a);
+            OWE_HIP_CHECK(hipStreamEndCapture(r->stream, &r->graph));
+            OWE_HIP_CHECK(hipGraphInstantiate(&r->graphExec, r->graph, nullptr, nullptr, 0));
+            r->graphTotal = total;
+        }
+        OWE_HIP_CHECK(hipGraphLaunch(r->graphExec, r->stream));
+    });
+    OWE_HIP_CHECK(hipGetLastError());
+    unsigned long long segs = 0;
+    uint32_t counters[4] = {0, 0, 0, 0};
+    OWE_HIP_CHECK(hipMemcpyAsync(&segs, r->persistSegments, sizeof segs, hipMemcpyDeviceToHost, r->stream));
+    OWE_HIP_CHECK(hipMemcpyAsync(counters, r->persistCounters, sizeof counters, hipMemcpyDeviceToHost, r->stream));
+    OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
+    if (r->timing) collectTimes(r, {kPersistent});
+    r->stats.paths += total;
+    r->stats.segments += segs;
+    r->stats.inconsistencies += counters[2];
+    r->stats.leaks += counters[3];
 }
 
 // One light-tracing pass of `total` particles (W·H·ppp), in launches of at most `capacity`; with a
@@ -998,6 +1237,13 @@ int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render
         OWE_HIP_CHECK(hipMemset(r->film, 0, 3 * pixels * sizeof(unsigned long long)));
         if (r->integrator == OWE_INTEGRATOR_PATH) {
             allocPaths(r, uint32_t(std::min<uint64_t>(pixels * 64, 1u << 22)));
+            hipDeviceProp_t prop;
+            OWE_HIP_CHECK(hipGetDeviceProperties(&prop, device));
+            r->computeUnits = prop.multiProcessorCount;
+            r->persistNext = devAlloc<unsigned long long>(r, 1);
+            r->persistSegments = devAlloc<unsigned long long>(r, 1);
+            r->persistCounters = devAlloc<uint32_t>(r, 4);
+            r->passSlot = devAlloc<uint32_t>(r, 1);
         } else {
             r->capacity = uint32_t(std::min<uint64_t>(pixels * 64, 1u << 22));  // particles per launch
             r->lightFilm = devAlloc<unsigned long long>(r, 3 * kSplatBins * pixels);
@@ -1019,6 +1265,7 @@ int owe_hip_renderer_destroy(owe_hip_renderer* r) {
         if (!r) return;
         (void)hipSetDevice(r->device);
         if (r->stream) (void)hipStreamSynchronize(r->stream);
+        dropGraph(r);
         for (void* p : r->allocations) OWE_HIP_CHECK(hipFree(p));
         for (auto& row : r->ev)
             for (hipEvent_t& e : row)
@@ -1036,7 +1283,7 @@ int owe_hip_renderer_set_timing(owe_hip_renderer* r, int enable) {
             for (auto& row : r->ev)
                 for (hipEvent_t& e : row) OWE_HIP_CHECK(hipEventCreate(&e));
         r->timing = enable != 0;
-        for (int k = 0; k < 5; ++k) {
+        for (int k = 0; k < kTimers; ++k) {
             r->kernelMs[k] = 0;
             r->kernelLaunches[k] = 0;
         }
@@ -1046,7 +1293,7 @@ int owe_hip_renderer_set_timing(owe_hip_renderer* r, int enable) {
 int owe_hip_renderer_kernel_times(const owe_hip_renderer* r, double* ms, uint64_t* launches) {
     return guard([&] {
         require(r != nullptr && ms != nullptr && launches != nullptr, "null argument");
-        for (int k = 0; k < 5; ++k) {
+        for (int k = 0; k < kTimers; ++k) {
             ms[k] = r->kernelMs[k];
             launches[k] = r->kernelLaunches[k];
         }
@@ -1065,9 +1312,13 @@ int owe_hip_renderer_pass(owe_hip_renderer* r, int spp) {
             r->totalSpp += spp;
             return;
         }
-        for (uint64_t base = 0; base < total; base += r->capacity) {
-            const uint32_t count = uint32_t(std::min<uint64_t>(r->capacity, total - base));
-            runPaths(r, base, count, uint32_t(spp), uint32_t(r->passes), false, nullptr);
+        if (r->mode == OWE_HIP_MODE_PERSISTENT) {
+            runPersistent(r, uint32_t(spp), uint32_t(r->passes));
+        } else {
+            for (uint64_t base = 0; base < total; base += r->capacity) {
+                const uint32_t count = uint32_t(std::min<uint64_t>(r->capacity, total - base));
+                runPaths(r, base, count, uint32_t(spp), uint32_t(r->passes), false, nullptr);
+            }
         }
         OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
         r->passes++;
@@ -1187,6 +1438,109 @@ int owe_hip_renderer_particles(owe_hip_renderer* r, int pass, const uint64_t* in
                                          hipMemcpyDeviceToHost, r->stream));
             OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
         }
+    });
+}
+
+int owe_hip_renderer_set_mode(owe_hip_renderer* r, int mode, int blocks_per_cu, int chunk, int use_graph) {
+    return guard([&] {
+        require(r != nullptr, "renderer is null");
+        require(mode == OWE_HIP_MODE_WAVEFRONT || mode == OWE_HIP_MODE_PERSISTENT, "unknown mode");
+        require(mode == OWE_HIP_MODE_WAVEFRONT || r->integrator == OWE_INTEGRATOR_PATH,
+                "the persistent form traces camera paths");
+        require(blocks_per_cu >= 0 && blocks_per_cu <= 64, "blocks per compute unit must lie in [0, 64]");
+        require(chunk >= 0 && chunk <= 1024, "chunk must lie in [0, 1024]");
+        OWE_HIP_CHECK(hipSetDevice(r->device));
+        r->mode = mode;
+        if (blocks_per_cu > 0) r->blocksPerCU = blocks_per_cu;
+        if (chunk > 0) r->chunk = uint32_t(chunk);
+        r->useGraph = use_graph != 0;
+        dropGraph(r);
+    });
+}
+
+int owe_hip_renderer_get_mode(const owe_hip_renderer* r, int* mode, int* blocks_per_cu, int* chunk, int* use_graph) {
+    return guard([&] {
+        require(r != nullptr, "renderer is null");
+        if (mode) *mode = r->mode;
+        if (blocks_per_cu) *blocks_per_cu = r->blocksPerCU;
+        if (chunk) *chunk = int(r->chunk);
+        if (use_graph) *use_graph = r->useGraph ? 1 : 0;
+    });
+}
+
+int owe_hip_renderer_autotune(owe_hip_renderer* r, int spp, double* best_ms) {
+    return guard([&] {
+        require(r != nullptr, "renderer is null");
+        require(spp > 0, "spp must be positive");
+        require(r->integrator == OWE_INTEGRATOR_PATH, "the autotuner tunes the camera-path forms");
+        require(r->passes == 0, "autotune a fresh (or reset) renderer: its passes would enter the film");
+        OWE_HIP_CHECK(hipSetDevice(r->device));
+        hipEvent_t e0, e1;
+        OWE_HIP_CHECK(hipEventCreate(&e0));
+        OWE_HIP_CHECK(hipEventCreate(&e1));
+        const bool wasTiming = r->timing;
+        r->timing = false;
+        double best = INFINITY;
+        int bestA = 0, bestB = 0;
+        auto timePass = [&](auto&& run) {
+            OWE_HIP_CHECK(hipEventRecord(e0, r->stream));
+            run();
+            OWE_HIP_CHECK(hipEventRecord(e1, r->stream));
+            OWE_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0;
+            OWE_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            return double(ms);
+        };
+        const uint64_t total = uint64_t(r->host.obs.width) * uint64_t(r->host.obs.height) * uint64_t(spp);
+        if (r->mode == OWE_HIP_MODE_PERSISTENT) {
+            // Resident blocks per compute unit × path ids per lane fetch.
+            const bool graph = r->useGraph;
+            r->useGraph = false;
+            for (int b : {1, 2, 3, 4, 6, 8})
+                for (int c : {1, 2, 4, 8, 16}) {
+                    r->blocksPerCU = b;
+                    r->chunk = uint32_t(c);
+                    hipLaunchKernelGGL(setPass, dim3{1}, dim3{64}, 0, r->stream, r->passSlot, 0u);
+                    const double ms = timePass([&] { launchPersistent(r, persistArgs(r, total, uint32_t(spp))); });
+                    if (ms < best) {
+                        best = ms;
+                        bestA = b;
+                        bestB = c;
+                    }
+                }
+            r->blocksPerCU = bestA;
+            r->chunk = uint32_t(bestB);
+            r->useGraph = graph;
+            dropGraph(r);
+        } else {
+            // Bounce rounds enqueued between host reads of the live counters.
+            for (int k : {1, 2, 4, 8}) {
+                r->roundsPerSync = k;
+                const double ms = timePass([&] {
+                    for (uint64_t base = 0; base < total; base += r->capacity)
+                        runPaths(r, base, uint32_t(std::min<uint64_t>(r->capacity, total - base)), uint32_t(spp), 0u,
+                                 false, nullptr);
+                });
+                if (ms < best) {
+                    best = ms;
+                    bestA = k;
+                }
+            }
+            r->roundsPerSync = bestA;
+        }
+        OWE_HIP_CHECK(hipEventDestroy(e0));
+        OWE_HIP_CHECK(hipEventDestroy(e1));
+        r->timing = wasTiming;
+        // The tuning passes leave nothing behind.
+        const size_t n = 3 * size_t(r->host.obs.width) * size_t(r->host.obs.height);
+        OWE_HIP_CHECK(hipMemsetAsync(r->film, 0, n * sizeof(unsigned long long), r->stream));
+        OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
+        r->stats = {};
+        for (int k = 0; k < kTimers; ++k) {
+            r->kernelMs[k] = 0;
+            r->kernelLaunches[k] = 0;
+        }
+        if (best_ms) *best_ms = best;
     });
 }
 

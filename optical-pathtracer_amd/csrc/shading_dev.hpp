@@ -19,15 +19,15 @@ struct Settings {
     float lambdaMin, lambdaMax, fresnelFloor, pad2;
 };
 
-__device__ inline float get(const float4& v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w)); }
-__device__ inline void setc(float4& v, int i, float x) {
+__device__ __forceinline__ float get(const float4& v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w)); }
+__device__ __forceinline__ void setc(float4& v, int i, float x) {
     if (i == 0) v.x = x;
     else if (i == 1) v.y = x;
     else if (i == 2) v.z = x;
     else v.w = x;
 }
 
-__device__ inline float regionIndex(const DScene& sc, uint32_t region, float l) {
+__device__ __forceinline__ float regionIndex(const DScene& sc, uint32_t region, float l) {
     return mediumIndex(sc.media[sc.regionMedium[region]], l);
 }
 
@@ -43,12 +43,12 @@ struct SSample {
     bool delta, transmitted;
 };
 
-__device__ inline Cx conductorEta(const DScene& sc, const DOptics& o, float lambda, float nMed) {
+__device__ __forceinline__ Cx conductorEta(const DScene& sc, const DOptics& o, float lambda, float nMed) {
     const float inv = 1.0f / nMed;
     return {evalSpec(o.condN, sc.table, lambda) * inv, evalSpec(o.condK, sc.table, lambda) * inv};
 }
 
-__device__ inline bool specularReflection(const Frame& f, f3 wo, float weight, SSample& s) {
+__device__ __forceinline__ bool specularReflection(const Frame& f, f3 wo, float weight, SSample& s) {
     s.wi = toWorld(f, mk3(-wo.x, -wo.y, wo.z));
     s.weight = weight;
     s.delta = true;
@@ -56,7 +56,7 @@ __device__ inline bool specularReflection(const Frame& f, f3 wo, float weight, S
 }
 
 // sampleScatter (src/owe/transport/scattering.cpp:65-157).
-__device__ inline bool sampleScatter(const DScene& sc, const Iface& it, f3 d, float uc, float u1, float u2, bool radiance,
+__device__ __forceinline__ bool sampleScatter(const DScene& sc, const Iface& it, f3 d, float uc, float u1, float u2, bool radiance,
                                      SSample& s) {
     const DOptics& o = *it.o;
     const Frame frame = makeFrame(it.n);
@@ -121,7 +121,7 @@ __device__ inline bool sampleScatter(const DScene& sc, const Iface& it, f3 d, fl
 }
 
 // evalScatter (scattering.cpp:159-188).
-__device__ inline float evalScatter(const DScene& sc, const Iface& it, f3 d, f3 wiW, bool radiance, float& pdf) {
+__device__ __forceinline__ float evalScatter(const DScene& sc, const Iface& it, f3 d, f3 wiW, bool radiance, float& pdf) {
     pdf = 0.0f;
     const DOptics& o = *it.o;
     if (o.delta) return 0.0f;
@@ -150,7 +150,7 @@ __device__ inline float evalScatter(const DScene& sc, const Iface& it, f3 d, f3 
     }
 }
 
-__device__ inline float secondaryWeight(const DScene& sc, const Iface& it, f3 d, const SSample& s, bool radiance) {
+__device__ __forceinline__ float secondaryWeight(const DScene& sc, const Iface& it, f3 d, const SSample& s, bool radiance) {
     const DOptics& o = *it.o;
     if (!s.delta) {
         float pdf;
@@ -168,13 +168,13 @@ __device__ inline float secondaryWeight(const DScene& sc, const Iface& it, f3 d,
 }
 
 // Solid-angle pdf of a light point on analytic boundary b seen from ref (Shape::pdfFrom).
-__device__ inline float sphereConeSize(f3 ref, float R) {
+__device__ __forceinline__ float sphereConeSize(f3 ref, float R) {
     const float dc2 = dot3(ref, ref);
     if (dc2 <= R * R * (1.0f + 1e-6f)) return 0.0f;
     const float s2 = R * R / dc2;
     return s2 / (1.0f + sqrt0f(1.0f - s2));
 }
-__device__ inline bool areaToSolidAngle(f3 ref, f3 p, f3 n, float area, float& pdfW) {
+__device__ __forceinline__ bool areaToSolidAngle(f3 ref, f3 p, f3 n, float area, float& pdfW) {
     pdfW = 0.0f;
     const f3 w = p - ref;
     const float d2 = dot3(w, w);
@@ -184,7 +184,7 @@ __device__ inline bool areaToSolidAngle(f3 ref, f3 p, f3 n, float area, float& p
     pdfW = d2 / (cosL * area);
     return true;
 }
-__device__ inline float pdfFrom(const DBoundary& b, f3 refLocal, f3 pLocal, f3 nLocal) {
+__device__ __forceinline__ float pdfFrom(const DBoundary& b, f3 refLocal, f3 pLocal, f3 nLocal) {
     if (b.shape == OWE_SHAPE_SPHERE) {
         const float omc = sphereConeSize(refLocal, b.a);
         if (omc > 0.0f) return 1.0f / (2.0f * kPiF * omc);
@@ -192,7 +192,7 @@ __device__ inline float pdfFrom(const DBoundary& b, f3 refLocal, f3 pLocal, f3 n
     float pdfW;
     return b.area > 0.0f && areaToSolidAngle(refLocal, pLocal, nLocal, b.area, pdfW) ? pdfW : 0.0f;
 }
-__device__ inline bool sampleFrom(const DBoundary& b, f3 ref, float u1, float u2, float u3, f3& p, f3& n, float& pdfW) {
+__device__ __forceinline__ bool sampleFrom(const DBoundary& b, f3 ref, float u1, float u2, float u3, f3& p, f3& n, float& pdfW) {
     if (b.shape == OWE_SHAPE_SPHERE) {
         const float omc = sphereConeSize(ref, b.a);
         if (omc > 0.0f) {
@@ -226,7 +226,7 @@ __device__ inline bool sampleFrom(const DBoundary& b, f3 ref, float u1, float u2
 
 // The solid-angle density at p of directions toward a uniform point of a disc (diskAimPdf,
 // core/shared/optics.hpp:248-256); generatingDisk: wi is known to come from the disc.
-__device__ inline float diskAimPdf(f3 p, f3 wi, f3 center, f3 normal, float radius, bool generatingDisk) {
+__device__ __forceinline__ float diskAimPdf(f3 p, f3 wi, f3 center, f3 normal, float radius, bool generatingDisk) {
     if (!(radius > 0.0f)) return 0.0f;
     const float dn = dot3(wi, normal);
     if (fabsf(dn) < 1e-12f) return 0.0f;
@@ -238,7 +238,7 @@ __device__ inline float diskAimPdf(f3 p, f3 wi, f3 center, f3 normal, float radi
 
 // A uniform point of emitter b in its own frame (sampleArea, geometry.slang:708-752): planes by
 // aperture (annulus, rectangle, ellipse), spheres over their whole surface. Meshes are not sampled.
-__device__ inline bool sampleArea(const DBoundary& b, float u1, float u2, f3& p, f3& n) {
+__device__ __forceinline__ bool sampleArea(const DBoundary& b, float u1, float u2, f3& p, f3& n) {
     n = mk3(0, 0, 1);
     if (b.shape == OWE_SHAPE_SPHERE) {
         n = uniformSphere(u1, u2);
@@ -264,7 +264,7 @@ __device__ inline bool sampleArea(const DBoundary& b, float u1, float u2, f3& p,
 // (mesh face `face`, ~0 for none) and never meets `exclude` (the far end's own boundary). For a
 // finite segment the region it arrives in must be `arrival`. The traversal stack is stack[k·stride]. Tr holds the transmittance per
 // wavelength on return; false when the segment is blocked.
-__device__ inline bool walkTransmittance(const DScene& sc, f3 o, f3 d, float dist, uint32_t region, uint32_t departed,
+__device__ __forceinline__ bool walkTransmittance(const DScene& sc, f3 o, f3 d, float dist, uint32_t region, uint32_t departed,
                                          uint32_t face, uint32_t exclude, uint32_t arrival, const float lam[NW],
                                          uint32_t* stack, uint32_t stride, float Tr[NW]) {
     float remaining = dist;

@@ -59,16 +59,16 @@ struct Rng {
 
 // ---------------------------------------------------------------- small math
 
-__device__ inline float sqf(float x) { return x * x; }
-__device__ inline float sqrt0f(float x) { return sqrtf(fmaxf(0.0f, x)); }
-__device__ inline float clampf(float x, float lo, float hi) { return fminf(hi, fmaxf(lo, x)); }
-__device__ inline f3 reflect3(f3 v, f3 n) { return v - n * (2.0f * dot3(v, n)); }
-__device__ inline f3 ld3(const float* p) { return mk3(p[0], p[1], p[2]); }
-__device__ inline f3 xfPoint(const float* m, f3 p) {
+__device__ __forceinline__ float sqf(float x) { return x * x; }
+__device__ __forceinline__ float sqrt0f(float x) { return sqrtf(fmaxf(0.0f, x)); }
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(hi, fmaxf(lo, x)); }
+__device__ __forceinline__ f3 reflect3(f3 v, f3 n) { return v - n * (2.0f * dot3(v, n)); }
+__device__ __forceinline__ f3 ld3(const float* p) { return mk3(p[0], p[1], p[2]); }
+__device__ __forceinline__ f3 xfPoint(const float* m, f3 p) {
     return mk3(m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3], m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7],
                m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11]);
 }
-__device__ inline f3 xfVector(const float* m, f3 v) {
+__device__ __forceinline__ f3 xfVector(const float* m, f3 v) {
     return mk3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[4] * v.x + m[5] * v.y + m[6] * v.z,
                m[8] * v.x + m[9] * v.y + m[10] * v.z);
 }
@@ -76,16 +76,16 @@ __device__ inline f3 xfVector(const float* m, f3 v) {
 struct Frame {
     f3 s, t, n;
 };
-__device__ inline Frame makeFrame(f3 n) {
+__device__ __forceinline__ Frame makeFrame(f3 n) {
     const float sg = copysignf(1.0f, n.z);
     const float a = -1.0f / (sg + n.z);
     const float b = n.x * n.y * a;
     return {mk3(1.0f + sg * n.x * n.x * a, sg * b, -sg * n.x), mk3(b, sg + n.y * n.y * a, -n.y), n};
 }
-__device__ inline f3 toLocal(const Frame& f, f3 v) { return mk3(dot3(v, f.s), dot3(v, f.t), dot3(v, f.n)); }
-__device__ inline f3 toWorld(const Frame& f, f3 v) { return f.s * v.x + f.t * v.y + f.n * v.z; }
+__device__ __forceinline__ f3 toLocal(const Frame& f, f3 v) { return mk3(dot3(v, f.s), dot3(v, f.t), dot3(v, f.n)); }
+__device__ __forceinline__ f3 toWorld(const Frame& f, f3 v) { return f.s * v.x + f.t * v.y + f.n * v.z; }
 
-__device__ inline void concentricDisk(float u1, float u2, float& x, float& y) {
+__device__ __forceinline__ void concentricDisk(float u1, float u2, float& x, float& y) {
     const float a = 2.0f * u1 - 1.0f, b = 2.0f * u2 - 1.0f;
     x = y = 0.0f;
     if (a == 0.0f && b == 0.0f) return;
@@ -100,31 +100,31 @@ __device__ inline void concentricDisk(float u1, float u2, float& x, float& y) {
     x = r * cosf(phi);
     y = r * sinf(phi);
 }
-__device__ inline f3 cosineHemisphere(float u1, float u2) {
+__device__ __forceinline__ f3 cosineHemisphere(float u1, float u2) {
     float x, y;
     concentricDisk(u1, u2, x, y);
     return mk3(x, y, sqrt0f(1.0f - x * x - y * y));
 }
-__device__ inline f3 uniformSphere(float u1, float u2) {
+__device__ __forceinline__ f3 uniformSphere(float u1, float u2) {
     const float z = 1.0f - 2.0f * u1, r = sqrt0f(1.0f - z * z), phi = 2.0f * kPiF * u2;
     return mk3(r * cosf(phi), r * sinf(phi), z);
 }
 // A direction in the cone of 1 − cos = omc about +z, uniform (cancellation-free for tiny cones).
-__device__ inline f3 uniformConeOmc(float u1, float u2, float omc) {
+__device__ __forceinline__ f3 uniformConeOmc(float u1, float u2, float omc) {
     const float o = u1 * omc, c = 1.0f - o, s = sqrt0f(o * (2.0f - o)), phi = 2.0f * kPiF * u2;
     return mk3(cosf(phi) * s, sinf(phi) * s, c);
 }
-__device__ inline float powerHeuristic(float f, float g) {
+__device__ __forceinline__ float powerHeuristic(float f, float g) {
     const float a = f * f, b = g * g;
     if (isinf(a)) return 1.0f;
     if (a + b == 0.0f) return 0.0f;
     return a / (a + b);
 }
-__device__ inline float hgPhase(float c, float g) {
+__device__ __forceinline__ float hgPhase(float c, float g) {
     const float d = 1.0f + g * g + 2.0f * g * c;
     return (1.0f - g * g) / (4.0f * kPiF * d * sqrtf(fmaxf(d, 1e-30f)));
 }
-__device__ inline float sampleHg(f3 wo, float g, float u1, float u2, f3& wi) {
+__device__ __forceinline__ float sampleHg(f3 wo, float g, float u1, float u2, f3& wi) {
     float c;
     if (fabsf(g) < 1e-3f) c = 1.0f - 2.0f * u1;
     else c = -1.0f / (2.0f * g) * (1.0f + g * g - sqf((1.0f - g * g) / (1.0f + g - 2.0f * g * u1)));
@@ -136,12 +136,12 @@ __device__ inline float sampleHg(f3 wo, float g, float u1, float u2, f3& wi) {
 
 // ---------------------------------------------------------------- spectra
 
-__device__ inline float planckF(float lnm, float kelvin) {
+__device__ __forceinline__ float planckF(float lnm, float kelvin) {
     const float c1 = 1.191042972e8f, c2 = 1.438776877e4f;
     const float l = lnm * 1e-3f, l2 = l * l;
     return 1e6f * c1 / (l2 * l2 * l * (expf(c2 / (l * kelvin)) - 1.0f));
 }
-__device__ inline float evalSpec(const DSpec& s, const float* table, float l) {
+__device__ __forceinline__ float evalSpec(const DSpec& s, const float* table, float l) {
     if (s.kind == OWE_SPEC_CONSTANT) return s.scale * s.value;
     if (s.kind == OWE_SPEC_BLACKBODY) return s.scale * planckF(l, s.kelvin);
     const float* t = table + s.offset;
@@ -153,28 +153,28 @@ __device__ inline float evalSpec(const DSpec& s, const float* table, float l) {
     const float u = (l - t[2 * (i - 1)]) / (t[2 * i] - t[2 * (i - 1)]);
     return s.scale * (t[2 * (i - 1) + 1] * (1.0f - u) + t[2 * i + 1] * u);
 }
-__device__ inline float cieLobe(float x, float mu, float s1, float s2) {
+__device__ __forceinline__ float cieLobe(float x, float mu, float s1, float s2) {
     const float t = (x - mu) / (x < mu ? s1 : s2);
     return expf(-0.5f * t * t);
 }
-__device__ inline f3 cieXYZ(float l) {
+__device__ __forceinline__ f3 cieXYZ(float l) {
     return mk3(1.056f * cieLobe(l, 599.8f, 37.9f, 31.0f) + 0.362f * cieLobe(l, 442.0f, 16.0f, 26.7f) -
                    0.065f * cieLobe(l, 501.1f, 20.4f, 26.2f),
                0.821f * cieLobe(l, 568.8f, 46.9f, 40.5f) + 0.286f * cieLobe(l, 530.9f, 16.3f, 31.1f),
                1.217f * cieLobe(l, 437.0f, 11.8f, 36.0f) + 0.681f * cieLobe(l, 459.0f, 26.0f, 13.8f));
 }
-__device__ inline float sampleVisibleF(float u) {
+__device__ __forceinline__ float sampleVisibleF(float u) {
     return clampf(538.0f - 138.888889f * atanhf(0.85691062f - 1.82750197f * u), 360.0f, 830.0f);
 }
-__device__ inline float visiblePdfF(float l) {
+__device__ __forceinline__ float visiblePdfF(float l) {
     if (l < 360.0f || l > 830.0f) return 0.0f;
     return 0.0039398042f / sqf(coshf(0.0072f * (l - 538.0f)));
 }
-__device__ inline float ciddorAirF(float l2) {
+__device__ __forceinline__ float ciddorAirF(float l2) {
     const float s2 = 1.0f / l2;
     return 1.0f + 0.05792105f / (238.0185f - s2) + 0.00167917f / (57.362f - s2);
 }
-__device__ inline float mediumIndex(const DMedium& m, float lnm) {
+__device__ __forceinline__ float mediumIndex(const DMedium& m, float lnm) {
     const float l = lnm * 1e-3f, l2 = l * l;
     float v = 1.0f;
     if (m.indexKind == OWE_INDEX_CONSTANT) v = m.a[0];
@@ -192,11 +192,11 @@ struct WL {
     float lambda[NW];
     float pdf[NW];
 };
-__device__ inline bool secondaryTerminated(const WL& w) { return w.pdf[1] == 0.0f && w.pdf[2] == 0.0f && w.pdf[3] == 0.0f; }
+__device__ __forceinline__ bool secondaryTerminated(const WL& w) { return w.pdf[1] == 0.0f && w.pdf[2] == 0.0f && w.pdf[3] == 0.0f; }
 
 // ---------------------------------------------------------------- interface optics
 
-__device__ inline bool refractDir(f3 d, f3 n, float ni, float nt, f3& out) {
+__device__ __forceinline__ bool refractDir(f3 d, f3 n, float ni, float nt, f3& out) {
     const f3 nn = dot3(d, n) < 0.0f ? n : -n;
     const float ci = -dot3(d, nn), eta = ni / nt;
     const float k = 1.0f - eta * eta * (1.0f - ci * ci);
@@ -205,7 +205,7 @@ __device__ inline bool refractDir(f3 d, f3 n, float ni, float nt, f3& out) {
     out = norm3(d * eta + nn * (eta * ci - sqrtf(k)));
     return true;
 }
-__device__ inline float fresnelDielectric(float cosI, float eta) {
+__device__ __forceinline__ float fresnelDielectric(float cosI, float eta) {
     cosI = clampf(cosI, -1.0f, 1.0f);
     if (cosI < 0.0f) {
         eta = 1.0f / eta;
@@ -220,16 +220,16 @@ __device__ inline float fresnelDielectric(float cosI, float eta) {
 struct Cx {
     float re, im;
 };
-__device__ inline Cx cmul(Cx a, Cx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
-__device__ inline Cx cdiv(Cx a, Cx b) {
+__device__ __forceinline__ Cx cmul(Cx a, Cx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ __forceinline__ Cx cdiv(Cx a, Cx b) {
     const float d = b.re * b.re + b.im * b.im;
     return {(a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d};
 }
-__device__ inline Cx csqrtx(Cx z) {
+__device__ __forceinline__ Cx csqrtx(Cx z) {
     const float r = sqrtf(z.re * z.re + z.im * z.im);
     return {sqrtf(fmaxf(0.0f, (r + z.re) * 0.5f)), copysignf(sqrtf(fmaxf(0.0f, (r - z.re) * 0.5f)), z.im)};
 }
-__device__ inline float fresnelConductor(float cosI, Cx eta) {
+__device__ __forceinline__ float fresnelConductor(float cosI, Cx eta) {
     cosI = clampf(cosI, 0.0f, 1.0f);
     const Cx s2t = cdiv({1.0f - cosI * cosI, 0.0f}, cmul(eta, eta));
     const Cx ct = csqrtx({1.0f - s2t.re, -s2t.im});
@@ -238,8 +238,8 @@ __device__ inline float fresnelConductor(float cosI, Cx eta) {
     const Cx b = cdiv({cosI - ect.re, -ect.im}, {cosI + ect.re, ect.im});
     return (a.re * a.re + a.im * a.im + b.re * b.re + b.im * b.im) * 0.5f;
 }
-__device__ inline bool sameHemi(f3 a, f3 b) { return a.z * b.z > 0.0f; }
-__device__ inline bool refractLocal(f3 wi, f3 n, float eta, float& etap, f3& wt) {
+__device__ __forceinline__ bool sameHemi(f3 a, f3 b) { return a.z * b.z > 0.0f; }
+__device__ __forceinline__ bool refractLocal(f3 wi, f3 n, float eta, float& etap, f3& wt) {
     float ci = dot3(n, wi);
     etap = 1.0f;
     wt = wi;
@@ -255,8 +255,8 @@ __device__ inline bool refractLocal(f3 wi, f3 n, float eta, float& etap, f3& wt)
     etap = eta;
     return true;
 }
-__device__ inline float ggxAlpha(float r) { return fmaxf(r, 1e-4f); }
-__device__ inline float ggxD(float a, f3 m) {
+__device__ __forceinline__ float ggxAlpha(float r) { return fmaxf(r, 1e-4f); }
+__device__ __forceinline__ float ggxD(float a, f3 m) {
     const float c2 = m.z * m.z;
     if (c2 <= 0.0f) return 0.0f;
     const float c4 = c2 * c2;
@@ -264,18 +264,18 @@ __device__ inline float ggxD(float a, f3 m) {
     const float e = (1.0f - c2) / c2 / (a * a);
     return 1.0f / (kPiF * a * a * c4 * sqf(1.0f + e));
 }
-__device__ inline float ggxLambda(float a, f3 w) {
+__device__ __forceinline__ float ggxLambda(float a, f3 w) {
     const float c2 = w.z * w.z;
     if (c2 <= 0.0f) return 0.0f;
     return (sqrtf(1.0f + a * a * ((1.0f - c2) / c2)) - 1.0f) * 0.5f;
 }
-__device__ inline float ggxG1(float a, f3 w) { return 1.0f / (1.0f + ggxLambda(a, w)); }
-__device__ inline float ggxG(float a, f3 wo, f3 wi) { return 1.0f / (1.0f + ggxLambda(a, wo) + ggxLambda(a, wi)); }
-__device__ inline float ggxPdf(float a, f3 w, f3 m) {
+__device__ __forceinline__ float ggxG1(float a, f3 w) { return 1.0f / (1.0f + ggxLambda(a, w)); }
+__device__ __forceinline__ float ggxG(float a, f3 wo, f3 wi) { return 1.0f / (1.0f + ggxLambda(a, wo) + ggxLambda(a, wi)); }
+__device__ __forceinline__ float ggxPdf(float a, f3 w, f3 m) {
     if (w.z == 0.0f) return 0.0f;
     return ggxG1(a, w) / fabsf(w.z) * ggxD(a, m) * fabsf(dot3(w, m));
 }
-__device__ inline f3 ggxSampleWm(float a, f3 w, float u1, float u2) {
+__device__ __forceinline__ f3 ggxSampleWm(float a, f3 w, float u1, float u2) {
     f3 wh = norm3(mk3(a * w.x, a * w.y, w.z));
     if (wh.z < 0.0f) wh = -wh;
     const f3 t1 = wh.z < 0.99999f ? norm3(cross3(mk3(0, 0, 1), wh)) : mk3(1, 0, 0);
@@ -294,7 +294,7 @@ struct Lobe {
     float weight, pdf;
     bool transmitted;
 };
-__device__ inline bool sampleRoughDielectric(float a, f3 wo, float eta, float uc, float u1, float u2, bool radiance, Lobe& s) {
+__device__ __forceinline__ bool sampleRoughDielectric(float a, f3 wo, float eta, float uc, float u1, float u2, bool radiance, Lobe& s) {
     const f3 wm = ggxSampleWm(a, wo, u1, u2);
     const float R = fresnelDielectric(dot3(wo, wm), eta), T = 1.0f - R;
     s = Lobe{wo, 0.0f, 0.0f, false};
@@ -319,7 +319,7 @@ __device__ inline bool sampleRoughDielectric(float a, f3 wo, float eta, float uc
     }
     return isfinite(s.weight) && s.pdf > 0.0f;
 }
-__device__ inline float evalRoughDielectric(float a, f3 wo, f3 wi, float eta, bool radiance, float& pdf) {
+__device__ __forceinline__ float evalRoughDielectric(float a, f3 wo, f3 wi, float eta, bool radiance, float& pdf) {
     pdf = 0.0f;
     const float co = wo.z, ci = wi.z;
     if (co == 0.0f || ci == 0.0f) return 0.0f;
@@ -342,7 +342,7 @@ __device__ inline float evalRoughDielectric(float a, f3 wo, f3 wi, float eta, bo
     if (radiance) ft /= sqf(etap);
     return ft;
 }
-__device__ inline bool sampleRoughConductor(float a, f3 w, Cx eta, float tint, float u1, float u2, Lobe& s) {
+__device__ __forceinline__ bool sampleRoughConductor(float a, f3 w, Cx eta, float tint, float u1, float u2, Lobe& s) {
     const f3 wm = ggxSampleWm(a, w, u1, u2);
     s = Lobe{reflect3(-w, wm), 0.0f, 0.0f, false};
     const float F = fresnelConductor(fabsf(dot3(w, wm)), eta) * tint;
@@ -353,7 +353,7 @@ __device__ inline bool sampleRoughConductor(float a, f3 w, Cx eta, float tint, f
     s.weight = f * s.wi.z / s.pdf;
     return isfinite(s.weight);
 }
-__device__ inline float evalRoughConductor(float a, f3 wo, f3 wi, Cx eta, float tint, float& pdf) {
+__device__ __forceinline__ float evalRoughConductor(float a, f3 wo, f3 wi, Cx eta, float tint, float& pdf) {
     pdf = 0.0f;
     f3 wm = wo + wi;
     if (dot3(wm, wm) == 0.0f) return 0.0f;
@@ -374,7 +374,7 @@ struct Hit {
     f3 pLocal, nLocal;  // in the boundary's own frame (analytic shapes)
 };
 
-__device__ inline bool inAperture(int kind, float a, float b, f3 p) {
+__device__ __forceinline__ bool inAperture(int kind, float a, float b, f3 p) {
     if (kind == OWE_APERTURE_DISK) {
         const float r2 = p.x * p.x + p.y * p.y;
         return r2 <= a * a && r2 >= b * b;
@@ -383,7 +383,7 @@ __device__ inline bool inAperture(int kind, float a, float b, f3 p) {
     return sqf(p.x / a) + sqf(p.y / b) <= 1.0f;
 }
 
-__device__ inline bool boxHit(const DBoundary& b, f3 o, f3 inv, float tmax) {
+__device__ __forceinline__ bool boxHit(const DBoundary& b, f3 o, f3 inv, float tmax) {
     const float x0 = (b.lo[0] - o.x) * inv.x, x1 = (b.hi[0] - o.x) * inv.x;
     const float y0 = (b.lo[1] - o.y) * inv.y, y1 = (b.hi[1] - o.y) * inv.y;
     const float z0 = (b.lo[2] - o.z) * inv.z, z1 = (b.hi[2] - o.z) * inv.z;
@@ -394,7 +394,7 @@ __device__ inline bool boxHit(const DBoundary& b, f3 o, f3 inv, float tmax) {
 }
 
 // Sphere roots in the geometric form (core/shared/surfaces.hpp:98-117), ascending.
-__device__ inline bool sphereRoots(f3 o, f3 d, float R, float& t0, float& t1) {
+__device__ __forceinline__ bool sphereRoots(f3 o, f3 d, float R, float& t0, float& t1) {
     const float dd = dot3(d, d);
     const float b = dot3(o, d) / dd;
     const f3 perp = o - d * b;
@@ -417,7 +417,7 @@ __device__ inline bool sphereRoots(f3 o, f3 d, float R, float& t0, float& t1) {
 // exclude: an analytic boundary the segment ends on (a sampled light point). kGeometry = false
 // leaves the point and normals unset (a caller that keeps only t, boundary, face and slot).
 template <bool kGeometry = true>
-__device__ inline bool intersectWorld(const DScene& sc, f3 o, f3 d, float tmax, uint32_t departed, uint32_t departedFace,
+__device__ __forceinline__ bool intersectWorld(const DScene& sc, f3 o, f3 d, float tmax, uint32_t departed, uint32_t departedFace,
                                       uint32_t exclude, uint32_t* stack, uint32_t stride, Hit& h) {
     h.boundary = kNoBoundary;
     h.t = tmax;

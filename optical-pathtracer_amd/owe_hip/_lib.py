@@ -70,6 +70,13 @@ def load() -> ctypes.CDLL:
         raise FileNotFoundError(
             f"{LIB_PATH} is missing: build it with `make -C optical-pathtracer_amd` "
             "(or __graft_entry__.build()); the hip backend has no fallback")
+    # PyTorch ships its own HIP runtime. Loading it first makes the process use one runtime for both
+    # (the library resolves libamdhip64 to the copy already loaded); loading this library first
+    # would leave torch unable to see the device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(os.fspath(LIB_PATH))
     c_int, c_u32, c_u64, c_vp = ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_void_p
     _declare(lib, "owe_hip_last_error", ctypes.c_char_p, [])
@@ -92,6 +99,9 @@ def load() -> ctypes.CDLL:
     _declare(lib, "owe_hip_renderer_reset", c_int, [c_vp])
     _declare(lib, "owe_hip_renderer_camera_paths", c_int, [c_vp, c_int, c_int, c_vp, c_vp, c_u64, c_vp])
     _declare(lib, "owe_hip_renderer_particles", c_int, [c_vp, c_int, c_vp, c_u64, c_vp])
+    _declare(lib, "owe_hip_renderer_set_mode", c_int, [c_vp, c_int, c_int, c_int, c_int])
+    _declare(lib, "owe_hip_renderer_get_mode", c_int, [c_vp, c_vp, c_vp, c_vp, c_vp])
+    _declare(lib, "owe_hip_renderer_autotune", c_int, [c_vp, c_int, c_vp])
     _declare(lib, "owe_hip_renderer_set_timing", c_int, [c_vp, c_int])
     _declare(lib, "owe_hip_renderer_kernel_times", c_int, [c_vp, c_vp, c_vp])
     _lib = lib

@@ -59,15 +59,32 @@ class Renderer:
     def reset(self) -> None:
         _lib.check(_lib.load().owe_hip_renderer_reset(self._h))
 
-    KERNELS = ("generate", "intersect", "shade", "shadow", "splat")
+    KERNELS = ("generate", "intersect", "shade", "shadow", "splat", "persistent")
+    WAVEFRONT, PERSISTENT = 0, 1
+
+    def set_mode(self, mode: int, blocks_per_cu: int = 0, chunk: int = 0, use_graph: bool = False) -> None:
+        """The wavefront (0) or persistent batched (1) form of the camera loop; 0 keeps a shape value."""
+        _lib.check(_lib.load().owe_hip_renderer_set_mode(self._h, int(mode), int(blocks_per_cu), int(chunk),
+                                                         int(use_graph)))
+
+    def mode(self) -> dict:
+        v = [ctypes.c_int() for _ in range(4)]
+        _lib.check(_lib.load().owe_hip_renderer_get_mode(self._h, *[ctypes.byref(x) for x in v]))
+        return dict(mode=v[0].value, blocks_per_cu=v[1].value, chunk=v[2].value, use_graph=bool(v[3].value))
+
+    def autotune(self, spp: int) -> float:
+        """Picks the fastest launch shape of the current form over passes of spp; returns its ms."""
+        ms = ctypes.c_double()
+        _lib.check(_lib.load().owe_hip_renderer_autotune(self._h, int(spp), ctypes.byref(ms)))
+        return ms.value
 
     def set_timing(self, enable: bool = True) -> None:
         _lib.check(_lib.load().owe_hip_renderer_set_timing(self._h, int(enable)))
 
     def kernel_times(self) -> dict:
         """{kernel: (milliseconds, launches)} since timing was enabled (HIP events on the render stream)."""
-        ms = np.zeros(5)
-        n = np.zeros(5, dtype=np.uint64)
+        ms = np.zeros(6)
+        n = np.zeros(6, dtype=np.uint64)
         _lib.check(_lib.load().owe_hip_renderer_kernel_times(self._h, ms.ctypes.data, n.ctypes.data))
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(self.KERNELS)}
 
