@@ -102,6 +102,15 @@ def load() -> ctypes.CDLL:
     _declare(lib, "owe_hip_renderer_set_mode", c_int, [c_vp, c_int, c_int, c_int, c_int])
     _declare(lib, "owe_hip_renderer_get_mode", c_int, [c_vp, c_vp, c_vp, c_vp, c_vp])
     _declare(lib, "owe_hip_renderer_autotune", c_int, [c_vp, c_int, c_vp])
+    _declare(lib, "owe_hip_lens_batch_create", c_int, [c_int, c_vp, c_u32, c_vp, c_u32, c_vp, c_u32, c_vp,
+                                                        ctypes.POINTER(c_vp)])
+    _declare(lib, "owe_hip_lens_batch_destroy", c_int, [c_vp])
+    _declare(lib, "owe_hip_lens_batch_run", c_int, [c_vp])
+    _declare(lib, "owe_hip_lens_batch_results", c_int, [c_vp, c_vp, c_vp, c_vp])
+    _declare(lib, "owe_hip_lens_batch_set_shape", c_int, [c_vp, c_int, c_int])
+    _declare(lib, "owe_hip_lens_batch_autotune", c_int, [c_vp, c_vp])
+    _declare(lib, "owe_hip_lens_batch_trace", c_int, [c_vp, c_u32, c_int, c_vp, c_u64, c_vp])
+    _declare(lib, "owe_hip_lens_batch_last_ms", c_int, [c_vp, c_vp])
     _declare(lib, "owe_hip_renderer_set_timing", c_int, [c_vp, c_int])
     _declare(lib, "owe_hip_renderer_kernel_times", c_int, [c_vp, c_vp, c_vp])
     _lib = lib

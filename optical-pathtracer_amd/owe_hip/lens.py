@@ -1,0 +1,281 @@
+"""Sequential lenses for the batched lens analysis (include/owe_hip_lens.h).
+
+* ``CATALOG``: the reference catalog's glasses (Schott Sellmeier, relative to air) and Ciddor air
+  (src/owe/core/medium.cpp:109-127);
+* ``Surface``/``Prescription``: rows of a lens table in millimetres, as the reference's ``.lens``
+  text (src/owe/scene/prescription.hpp:16-44), converted to metres for the C ABI;
+* ``SINGLET``/``DOUBLET``: the reference's own test prescriptions (tests/test_instruments.cpp:209-221);
+* ``seeded_achromats``: cemented achromat doublets from a fixed seed — focal length U(50, 150) mm,
+  front radius ±U(30, 120) mm, element thicknesses U(2, 8) mm, a crown and a flint from the catalog,
+  the inner and rear radii from the thin-lens achromat condition at the F, d and C lines;
+* ``LensBatch``: the persistent GPU analysis over the C ABI;
+* ``spot``: focusedSpot (src/owe/analysis/analysis.cpp:85-113) from the nine sums a run returns.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .scene import CMedium, INDEX_CIDDOR, INDEX_SELLMEIER
+
+LAMBDA_F, LAMBDA_D, LAMBDA_C = 486.13, 587.56, 656.27
+# Fraunhofer h, g, F, e, d, C, r lines (nm): the seven wavelengths of the batched analysis.
+FRAUNHOFER7 = (404.66, 435.83, 486.13, 546.07, 587.56, 656.27, 706.52)
+
+CATALOG = {
+    "air": None,
+    "N-BK7": ((1.03961212, 0.231792344, 1.01046945), (0.00600069867, 0.0200179144, 103.560653)),
+    "N-SK16": ((1.34317774, 0.241144399, 0.994317969), (0.00704687339, 0.0229005, 92.7508526)),
+    "N-BAF10": ((1.5851495, 0.143559385, 1.08521269), (0.00926681282, 0.0424489805, 105.613573)),
+    "F2": ((1.34533359, 0.209073176, 0.937357162), (0.00997743871, 0.0470450767, 111.886764)),
+    "N-F2": ((1.39757037, 0.159201403, 1.2686543), (0.00995906143, 0.0546931752, 119.248346)),
+    "N-SF6": ((1.77931763, 0.338149866, 2.08734474), (0.0133714182, 0.0617533621, 174.01759)),
+    "N-SF11": ((1.73759695, 0.313747346, 1.89878101), (0.013188707, 0.0623068142, 155.23629)),
+}
+CROWNS = ("N-BK7", "N-SK16", "N-BAF10")
+FLINTS = ("F2", "N-F2", "N-SF6", "N-SF11")
+MEDIA_ORDER = tuple(CATALOG)
+
+
+def media_array():
+    """The catalog as owe_medium records, in MEDIA_ORDER (index 0 is air)."""
+    arr = (CMedium * len(MEDIA_ORDER))()
+    for i, name in enumerate(MEDIA_ORDER):
+        m = arr[i]
+        if CATALOG[name] is None:
+            m.index_kind = INDEX_CIDDOR
+            continue
+        B, C = CATALOG[name]
+        m.index_kind, m.terms, m.rel_air = INDEX_SELLMEIER, len(B), 1
+        for k in range(len(B)):
+            m.coeff_a[k], m.coeff_b[k] = B[k], C[k]
+    return arr
+
+
+def ciddor_air(lnm: float) -> float:
+    s2 = (1e3 / lnm) ** 2
+    return 1 + 0.05792105 / (238.0185 - s2) + 0.00167917 / (57.362 - s2)
+
+
+def index(name: str, lnm: float) -> float:
+    """Absolute refractive index (catalog glasses are relative to air)."""
+    if CATALOG[name] is None:
+        return ciddor_air(lnm)
+    B, C = CATALOG[name]
+    l2 = (lnm * 1e-3) ** 2
+    n2 = 1 + sum(b * l2 / (l2 - c) for b, c in zip(B, C))
+    return math.sqrt(n2) * ciddor_air(lnm)
+
+
+@dataclass
+class Surface:
+    R: float                  # mm, math.inf for flat
+    t: float                  # mm to the next vertex
+    medium: str               # after the surface
+    sd: float                 # semi-diameter, mm
+    k: float = 0.0
+    A: tuple = ()             # A4, A6, ... in SI units
+    stop: bool = False
+
+
+@dataclass
+class Prescription:
+    surfaces: list
+    ambient: str = "air"
+    rays: int = 4096
+    name: str = ""
+
+
+SINGLET = Prescription([Surface(50.0, 8.0, "N-BK7", 12.5), Surface(-50.0, 40.0, "air", 12.5)], name="biconvex singlet")
+DOUBLET = Prescription([Surface(44.78, 6.0, "N-BK7", 12.5), Surface(-44.78, 2.5, "F2", 12.5),
+                        Surface(-810.8, 95.0, "air", 12.5)], name="cemented achromat f=100")
+
+
+class CLensSurface(ctypes.Structure):
+    _fields_ = [("curvature", ctypes.c_double), ("conic", ctypes.c_double), ("asphere", ctypes.c_double * 4),
+                ("thickness", ctypes.c_double), ("semi_diameter", ctypes.c_double), ("medium", ctypes.c_int32),
+                ("stop", ctypes.c_int32)]
+
+
+class CLens(ctypes.Structure):
+    _fields_ = [("first_surface", ctypes.c_uint32), ("surface_count", ctypes.c_uint32), ("rays", ctypes.c_uint64),
+                ("ambient", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class CLensSettings(ctypes.Structure):
+    _fields_ = [("field_deg", ctypes.c_double), ("wavelengths", ctypes.c_double * 16), ("n_wavelengths", ctypes.c_int32),
+                ("detector_pixels", ctypes.c_int32), ("detector_half_size", ctypes.c_double),
+                ("reference_wavelength", ctypes.c_double)]
+
+
+class CLensResult(ctypes.Structure):
+    _fields_ = [("launched", ctypes.c_uint64), ("arrived", ctypes.c_uint64), ("efl", ctypes.c_double),
+                ("bfl", ctypes.c_double), ("entrance_pupil_z", ctypes.c_double),
+                ("entrance_pupil_radius", ctypes.c_double), ("last_vertex_z", ctypes.c_double),
+                ("best_focus_z", ctypes.c_double), ("rms_best", ctypes.c_double), ("rms_paraxial", ctypes.c_double),
+                ("centroid_y", ctypes.c_double), ("transmitted", ctypes.c_double)]
+
+
+MOMENTS = 9
+
+
+def c_surface(s: Surface) -> CLensSurface:
+    c = CLensSurface()
+    c.curvature = 0.0 if math.isinf(s.R) else 1.0 / (s.R * 1e-3)
+    c.conic = s.k
+    for i, a in enumerate(s.A):
+        c.asphere[i] = a
+    c.thickness = s.t * 1e-3
+    c.semi_diameter = s.sd * 1e-3
+    c.medium = MEDIA_ORDER.index(s.medium)
+    c.stop = int(s.stop)
+    return c
+
+
+def c_batch(lenses):
+    """(surfaces, lenses) C arrays for a list of Prescriptions."""
+    rows = [c_surface(s) for p in lenses for s in p.surfaces]
+    surf = (CLensSurface * max(1, len(rows)))(*rows)
+    arr = (CLens * len(lenses))()
+    first = 0
+    for i, p in enumerate(lenses):
+        arr[i].first_surface, arr[i].surface_count = first, len(p.surfaces)
+        arr[i].rays, arr[i].ambient = int(p.rays), MEDIA_ORDER.index(p.ambient)
+        first += len(p.surfaces)
+    return surf, arr
+
+
+def c_settings(field_deg=0.5, wavelengths=FRAUNHOFER7, pixels=128, half_size=0.0, reference=LAMBDA_D):
+    s = CLensSettings()
+    s.field_deg = field_deg
+    for i, l in enumerate(wavelengths):
+        s.wavelengths[i] = l
+    s.n_wavelengths = len(wavelengths)
+    s.detector_pixels, s.detector_half_size, s.reference_wavelength = pixels, half_size, reference
+    return s
+
+
+def sag_mm(R: float, r: float) -> float:
+    return 0.0 if math.isinf(R) else R - math.copysign(math.sqrt(R * R - r * r), R)
+
+
+def seeded_achromats(n: int = 256, seed: int = 2024, rays_min: int = 4096, rays_max: int = 1 << 20):
+    """n cemented achromats: f U(50, 150) mm, front radius ±U(30, 120) mm, element thicknesses
+    U(2, 8) mm, crown/flint pairs from the catalog, f/8 clear aperture, rays log-uniform in
+    [rays_min, rays_max]. Thin-lens achromat powers φ₁ = φ V₁/(V₁ − V₂), φ₂ = −φ V₂/(V₁ − V₂);
+    a design whose radii leave no room for the aperture is drawn again."""
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < n:
+        f = rng.uniform(50.0, 150.0)
+        R1 = rng.uniform(30.0, 120.0) * rng.choice((-1.0, 1.0))
+        t1, t2 = rng.uniform(2.0, 8.0), rng.uniform(2.0, 8.0)
+        crown, flint = CROWNS[rng.integers(len(CROWNS))], FLINTS[rng.integers(len(FLINTS))]
+        rays = int(round(math.exp(rng.uniform(math.log(rays_min), math.log(rays_max)))))
+        n1 = {l: index(crown, l) / ciddor_air(l) for l in (LAMBDA_F, LAMBDA_D, LAMBDA_C)}
+        n2 = {l: index(flint, l) / ciddor_air(l) for l in (LAMBDA_F, LAMBDA_D, LAMBDA_C)}
+        V1 = (n1[LAMBDA_D] - 1) / (n1[LAMBDA_F] - n1[LAMBDA_C])
+        V2 = (n2[LAMBDA_D] - 1) / (n2[LAMBDA_F] - n2[LAMBDA_C])
+        phi = 1.0 / f
+        phi1, phi2 = phi * V1 / (V1 - V2), -phi * V2 / (V1 - V2)
+        c1 = 1.0 / R1
+        c2 = c1 - phi1 / (n1[LAMBDA_D] - 1)
+        c3 = c2 - phi2 / (n2[LAMBDA_D] - 1)
+        sd = f / 16.0
+        radii = [1.0 / c if abs(c) > 1e-12 else math.inf for c in (c1, c2, c3)]
+        if any(abs(R) < 1.2 * sd for R in radii):
+            continue
+        # Edge thickness of each element at least 1 mm.
+        e1 = t1 + sag_mm(radii[1], sd) - sag_mm(radii[0], sd)
+        e2 = t2 + sag_mm(radii[2], sd) - sag_mm(radii[1], sd)
+        t1 += max(0.0, 1.0 - e1)
+        t2 += max(0.0, 1.0 - e2)
+        out.append(Prescription([Surface(radii[0], t1, crown, sd, stop=True), Surface(radii[1], t2, flint, sd),
+                                 Surface(radii[2], f, "air", sd)], rays=rays,
+                                name=f"achromat f={f:.1f} {crown}/{flint}"))
+    return out
+
+
+def spot(m, z_img: float, last_vertex_z: float) -> dict:
+    """focusedSpot from the nine sums (count, ΣA, ΣB, Σ|A|², Σ|B|², ΣA·B, ΣT): the best-focus plane
+    z = −cov/var, its RMS radius, and the RMS radius and centroid height at z_img."""
+    m = np.asarray(m, dtype=np.float64)
+    n = m[0]
+    if n < 3:
+        return dict(arrived=int(n), best_focus_z=math.nan, rms_best=math.nan, rms_paraxial=math.nan,
+                    centroid_y=math.nan)
+    ma, mb = m[1:3] / n, m[3:5] / n
+    var_a = m[5] / n - ma @ ma
+    var_b = m[6] / n - mb @ mb
+    cov = m[7] / n - ma @ mb
+    z_best = -cov / var_b if var_b > 0 else z_img
+
+    def rms(z):
+        return math.sqrt(max(0.0, var_a + 2 * z * cov + z * z * var_b))
+
+    return dict(arrived=int(n), best_focus_z=z_best - last_vertex_z, rms_best=rms(z_best), rms_paraxial=rms(z_img),
+                centroid_y=ma[1] + mb[1] * z_img)
+
+
+class LensBatch:
+    """Many lenses traced in one persistent launch on the GPU (owe_hip_lens_batch_*)."""
+
+    def __init__(self, lenses, settings: CLensSettings | None = None, device: int = 0):
+        self.lenses = list(lenses)
+        self.settings = settings if settings is not None else c_settings()
+        self._media = media_array()
+        self._surf, self._arr = c_batch(self.lenses)
+        h = ctypes.c_void_p()
+        lib = _lib.load()
+        _lib.check(lib.owe_hip_lens_batch_create(device, self._media, len(MEDIA_ORDER), self._surf,
+                                                 sum(len(p.surfaces) for p in self.lenses), self._arr, len(self.lenses),
+                                                 ctypes.byref(self.settings), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            _lib.check(_lib.load().owe_hip_lens_batch_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self):
+        _lib.check(_lib.load().owe_hip_lens_batch_run(self._h))
+
+    def results(self, irradiance: bool = False):
+        n, nw, px = len(self.lenses), self.settings.n_wavelengths, self.settings.detector_pixels
+        res = (CLensResult * n)()
+        mom = np.zeros((n, nw, MOMENTS))
+        irr = np.zeros((n, px, px), dtype=np.float32) if irradiance and px > 0 else None
+        _lib.check(_lib.load().owe_hip_lens_batch_results(self._h, res, mom.ctypes.data,
+                                                          irr.ctypes.data if irr is not None else None))
+        per = [{f: getattr(r, f) for f, _ in CLensResult._fields_} for r in res]
+        return per, mom, irr
+
+    def set_shape(self, blocks_per_cu: int = 0, rays_per_item: int = 0):
+        _lib.check(_lib.load().owe_hip_lens_batch_set_shape(self._h, int(blocks_per_cu), int(rays_per_item)))
+
+    def autotune(self) -> float:
+        ms = ctypes.c_double()
+        _lib.check(_lib.load().owe_hip_lens_batch_autotune(self._h, ctypes.byref(ms)))
+        return ms.value
+
+    def last_ms(self) -> float:
+        ms = ctypes.c_double()
+        _lib.check(_lib.load().owe_hip_lens_batch_last_ms(self._h, ctypes.byref(ms)))
+        return ms.value
+
+    def trace(self, lens: int, wavelength: int, index) -> np.ndarray:
+        index = np.ascontiguousarray(index, dtype=np.uint64)
+        out = np.zeros((len(index), 7))
+        _lib.check(_lib.load().owe_hip_lens_batch_trace(self._h, int(lens), int(wavelength), index.ctypes.data,
+                                                        len(index), out.ctypes.data))
+        return out

@@ -1,0 +1,472 @@
+// TEST INFRASTRUCTURE ONLY — the double-precision CPU oracle (see oracle_core.hpp).
+//
+// Sequential lens analysis restated in double from the reference:
+//   quadric roots, refinement from the approximate hit       src/owe/core/shared/surfaces.hpp:13-94
+//   conic sag, slope, normal, conic sheet; slab and cylinder  src/owe/core/shared/surfaces.hpp:140-171
+//   SagSurface: quadric path and numeric asphere (96 samples  src/owe/scene/geometry.cpp:26-166
+//   + bisection), aperture acceptance
+//   transmission through a smooth dielectric (weight T)       Tracer::walk PrimaryTransmission,
+//                                                             src/owe/transport/transport.cpp:531-595,
+//                                                             scattering.cpp:65-108, optics.hpp:14-54
+//   ray acceptance (every surface refracts, then escape)      traceThrough, src/owe/analysis/analysis.cpp:49-70
+//   paraxial system matrix, EFL, BFL, stop, entrance pupil    src/owe/scene/prescription.cpp:193-321
+//   spot: best focus −cov/var, RMS radius, centroid           focusedSpot, src/owe/analysis/analysis.cpp:85-113
+// The lens is traced surface after surface (each ray meets the next row's sag): for the rays
+// traceThrough accepts — refracted once by every surface, then escaping — that is the order the
+// world intersection finds them in.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "../include/owe_hip_lens.h"
+#include "oracle_scene.hpp"
+
+using namespace oracle;
+
+namespace {
+
+constexpr double kTinyD = 1e-300, kRefineD = 1e-6, kBisectD = 1e-15, kRelTolD = 1e-9;
+constexpr int kBisectStepsD = 80;
+
+int solveQuadratic(double a, double b, double c, double& t0, double& t1) {
+    t0 = t1 = 0;
+    if (std::fabs(a) < kTinyD || std::fabs(a) < 1e-14 * std::fabs(b)) {
+        if (b == 0) return 0;
+        t0 = t1 = -c / b;
+        return 1;
+    }
+    const double disc = b * b - 4 * a * c;
+    if (disc < 0) return 0;
+    const double q = -(b + std::copysign(std::sqrt(disc), b)) / 2;
+    t0 = q / a;
+    t1 = q != 0 ? c / q : t0;
+    if (t0 > t1) std::swap(t0, t1);
+    return 2;
+}
+
+// The sag quadric c(x² + y²) + c(1 + k) z² − 2z = 0.
+void sagCoeffs(double c, double ck, V3 o, V3 d, double& A, double& B, double& C) {
+    A = c * (d.x * d.x + d.y * d.y) + ck * d.z * d.z;
+    B = 2 * (c * (o.x * d.x + o.y * d.y) + ck * o.z * d.z - d.z);
+    C = c * (o.x * o.x + o.y * o.y) + ck * o.z * o.z - 2 * o.z;
+}
+double refineRoot(double c, double ck, V3 o, V3 d, double t) {
+    double A, B, C, u0, u1;
+    sagCoeffs(c, ck, o + d * t, d, A, B, C);
+    if (solveQuadratic(A, B, C, u0, u1) == 0) return t;
+    const double du = std::fabs(u0) < std::fabs(u1) ? u0 : u1;
+    return std::fabs(du) < kRefineD * (1 + std::fabs(t)) ? t + du : t;
+}
+bool mayLieIn(double t, double tmin, double tmax) {
+    const double m = kRefineD * (1 + std::fabs(t));
+    return !(t + m <= tmin || t - m >= tmax);
+}
+
+struct Surf {
+    double c, k, A[4], sd, z, zMin, zMax;
+    int nA;
+};
+
+double sag(const Surf& s, double r) {
+    const double r2 = r * r;
+    double z = s.c * r2 / (1 + sqrt0(1 - (1 + s.k) * s.c * s.c * r2));
+    double rp = r2 * r2;
+    for (int i = 0; i < s.nA; ++i) {
+        z += s.A[i] * rp;
+        rp *= r2;
+    }
+    return z;
+}
+double dsag(const Surf& s, double r) {
+    const double root = sqrt0(1 - (1 + s.k) * s.c * s.c * (r * r));
+    double d = root > 0 ? s.c * r / root : (s.c != 0 ? kInf : 0.0);
+    double m = 4, rp = r * r * r;
+    for (int i = 0; i < s.nA; ++i) {
+        d += s.A[i] * m * rp;
+        rp *= r * r;
+        m += 2;
+    }
+    return d;
+}
+V3 sagNormal(const Surf& s, V3 p) {
+    const double r = std::sqrt(p.x * p.x + p.y * p.y);
+    if (r < kTinyD) return V3(0, 0, 1);
+    const double sl = dsag(s, r);
+    return normalize(V3(-sl * p.x / r, -sl * p.y / r, 1));
+}
+bool accept(const Surf& s, V3 p) { return p.x * p.x + p.y * p.y <= s.sd * s.sd; }
+bool onConicSheet(const Surf& s, double z) { return s.c == 0 || (1 + s.k) * s.c * z <= 1 + kRelTolD; }
+
+Surf makeSurf(const owe_lens_surface& r, double z) {
+    Surf s{};
+    s.c = r.curvature;
+    s.k = r.conic;
+    s.nA = 4;
+    for (int i = 0; i < 4; ++i) s.A[i] = r.asphere[i];
+    while (s.nA > 0 && s.A[s.nA - 1] == 0) --s.nA;
+    s.sd = r.semi_diameter;
+    s.z = z;
+    s.zMin = kInf;
+    s.zMax = -kInf;
+    for (int i = 0; i <= 512; ++i) {
+        const double v = sag(s, s.sd * i / 512);
+        s.zMin = std::min(s.zMin, v);
+        s.zMax = std::max(s.zMax, v);
+    }
+    const double pad = 1e-9 + 1e-3 * (s.zMax - s.zMin);
+    s.zMin -= pad;
+    s.zMax += pad;
+    return s;
+}
+
+// SagSurface::intersect in the surface's frame (vertex at the origin), t in (tmin, tmax).
+bool intersectSag(const Surf& s, V3 o, V3 d, double tmin, double tmax, double& t, V3& p, V3& n) {
+    if (s.nA == 0) {
+        if (s.c == 0) {
+            if (d.z == 0) return false;
+            t = -o.z / d.z;
+            if (t <= tmin || t >= tmax) return false;
+            p = o + d * t;
+            if (!accept(s, p)) return false;
+            n = sagNormal(s, p);
+            return true;
+        }
+        const double ck = s.c * (1 + s.k);
+        double A, B, C, t0, t1;
+        sagCoeffs(s.c, ck, o, d, A, B, C);
+        if (solveQuadratic(A, B, C, t0, t1) == 0) return false;
+        for (double tc : {t0, t1}) {
+            if (!mayLieIn(tc, tmin, tmax)) continue;
+            const double tr = refineRoot(s.c, ck, o, d, tc);
+            if (!(tr > tmin && tr < tmax)) continue;
+            const V3 q = o + d * tr;
+            if (!accept(s, q) || !onConicSheet(s, q.z)) continue;
+            t = tr;
+            p = q;
+            n = sagNormal(s, q);
+            return true;
+        }
+        return false;
+    }
+    // Numeric: the ray's stretch inside the aperture cylinder and the sag's slab, 96 samples of
+    // f(t) = sag(r) − z, the first sign change bisected.
+    double ta = tmin, tb = tmax;
+    {
+        const double a = d.x * d.x + d.y * d.y;
+        if (a < kTinyD) {
+            if (!(o.x * o.x + o.y * o.y <= s.sd * s.sd && ta < tb)) return false;
+        } else {
+            double q0, q1;
+            if (solveQuadratic(a, 2 * (o.x * d.x + o.y * d.y), o.x * o.x + o.y * o.y - s.sd * s.sd, q0, q1) < 2) return false;
+            ta = std::max(ta, q0);
+            tb = std::min(tb, q1);
+            if (!(ta < tb)) return false;
+        }
+        if (std::fabs(d.z) < kTinyD) {
+            if (!(o.z >= s.zMin && o.z <= s.zMax && ta < tb)) return false;
+        } else {
+            const double s0 = (s.zMin - o.z) / d.z, s1 = (s.zMax - o.z) / d.z;
+            ta = std::max(ta, std::min(s0, s1));
+            tb = std::min(tb, std::max(s0, s1));
+            if (!(ta < tb)) return false;
+        }
+    }
+    auto f = [&](double tt) {
+        const V3 q = o + d * tt;
+        return sag(s, std::sqrt(q.x * q.x + q.y * q.y)) - q.z;
+    };
+    const int N = 96;
+    double tPrev = ta, fPrev = f(ta);
+    for (int i = 1; i <= N; ++i) {
+        const double tCur = ta + (tb - ta) * i / N, fCur = f(tCur);
+        if ((fPrev <= 0 && fCur >= 0) || (fPrev >= 0 && fCur <= 0)) {
+            if (fPrev == 0 && tPrev <= tmin) {
+                tPrev = tCur;
+                fPrev = fCur;
+                continue;
+            }
+            double lo = tPrev, hi = tCur, flo = fPrev;
+            for (int it = 0; it < kBisectStepsD && hi - lo > kBisectD * (1 + std::fabs(hi)); ++it) {
+                const double mid = 0.5 * (lo + hi), fm = f(mid);
+                if ((fm <= 0) == (flo <= 0)) {
+                    lo = mid;
+                    flo = fm;
+                } else {
+                    hi = mid;
+                }
+            }
+            const double tr = 0.5 * (lo + hi);
+            const V3 q = o + d * tr;
+            if (tr > tmin && tr < tmax && accept(s, q)) {
+                t = tr;
+                p = q;
+                n = sagNormal(s, q);
+                return true;
+            }
+        }
+        tPrev = tCur;
+        fPrev = fCur;
+    }
+    return false;
+}
+
+struct Lens {
+    std::vector<Surf> surf;
+    std::vector<double> nAfter;  // per surface at the wavelength
+    double nAmbient = 1;
+};
+
+// One ray through the lens: true when every surface refracts it (traceThrough's acceptance);
+// p, d the last surface's point and the outgoing direction, T the product of 1 − R.
+bool traceRay(const Lens& L, V3 o, V3 d, V3& p, V3& dOut, double& T) {
+    T = 1;
+    double nPrev = L.nAmbient;
+    for (size_t i = 0; i < L.surf.size(); ++i) {
+        const Surf& s = L.surf[i];
+        double t;
+        V3 q, n;
+        const V3 ol(o.x, o.y, o.z - s.z);
+        if (!intersectSag(s, ol, d, 0, kInf, t, q, n)) return false;
+        const double nNext = L.nAfter[i];
+        const double R = fresnelDielectric(std::fabs(dot(d, n)), nNext / nPrev);
+        V3 w;
+        if (!refractDir(d, n, nPrev, nNext, w)) return false;
+        T *= 1 - R;
+        o = V3(q.x, q.y, q.z + s.z);
+        d = w;
+        nPrev = nNext;
+    }
+    p = o;
+    dOut = d;
+    return d.z > 0;
+}
+
+// Fibonacci lattice point i of n over a disc of radius r (the batch's entrance-pupil sampling).
+void pupilPoint(uint64_t i, uint64_t n, double r, double& x, double& y) {
+    const double rr = r * std::sqrt((double(i) + 0.5) / double(n));
+    double f = double(i) * 0.6180339887498949;
+    f -= std::floor(f);
+    const double a = 2 * kPi * f;
+    x = rr * std::cos(a);
+    y = rr * std::sin(a);
+}
+
+struct Paraxial {
+    double efl = kInf, bfl = kInf, epz = 0, epr = 0, length = 0;
+};
+
+// paraxialAnalysis for refracting rows with the object at infinity (prescription.cpp:193-321).
+Paraxial paraxial(const std::vector<owe_lens_surface>& s, const std::vector<double>& nAfter, double n0) {
+    const size_t N = s.size();
+    std::vector<double> nBefore(N), phi(N), t(N);
+    double nPrev = n0;
+    for (size_t i = 0; i < N; ++i) {
+        nBefore[i] = nPrev;
+        phi[i] = (nAfter[i] - nPrev) * s[i].curvature;
+        t[i] = std::fabs(s[i].thickness);
+        nPrev = nAfter[i];
+    }
+    double a = 1, b = 0, c = 0, d = 1;
+    for (size_t i = 0; i < N; ++i) {
+        // R = [1 0; −φ 1], then T = [1 t/n; 0 1]
+        const double a2 = a, b2 = b, c2 = c - phi[i] * a, d2 = d - phi[i] * b;
+        a = a2;
+        b = b2;
+        c = c2;
+        d = d2;
+        if (i + 1 < N) {
+            const double k = t[i] / nAfter[i];
+            a = a + k * c;
+            b = b + k * d;
+        }
+    }
+    Paraxial r;
+    for (size_t i = 0; i + 1 < N; ++i) r.length += t[i];
+    const double nk = nAfter[N - 1];
+    r.efl = -nk / c;
+    r.bfl = -nk * a / c;
+    int stop = -1;
+    for (size_t i = 0; i < N; ++i)
+        if (s[i].stop) stop = int(i);
+    if (stop < 0) {
+        double best = -1, y = 1, nu = 0;
+        for (size_t i = 0; i < N; ++i) {
+            const double ratio = std::fabs(y) / std::max(s[i].semi_diameter, 1e-12);
+            if (ratio > best) {
+                best = ratio;
+                stop = int(i);
+            }
+            nu -= phi[i] * y;
+            if (i + 1 < N) y += t[i] * nu / nAfter[i];
+        }
+    }
+    const double stopR = s[size_t(stop)].semi_diameter;
+    double y = 0, nu = 1e-3 * nBefore[size_t(stop)], yM = stopR, nuM = 0;
+    for (int i = stop - 1; i >= 0; --i) {
+        y -= t[size_t(i)] * nu / nAfter[size_t(i)];
+        yM -= t[size_t(i)] * nuM / nAfter[size_t(i)];
+        nu += phi[size_t(i)] * y;
+        nuM += phi[size_t(i)] * yM;
+    }
+    if (stop == 0) {
+        r.epz = 0;
+        r.epr = stopR;
+    } else {
+        const double u = nu / n0;
+        if (u != 0) {
+            const double dz = -y / u;
+            r.epz = dz;
+            r.epr = std::fabs(yM + dz * nuM / n0);
+        }
+    }
+    return r;
+}
+
+IndexModel indexOf(const owe_medium& m) {
+    IndexModel x;
+    x.kind = m.index_kind;
+    x.terms = m.terms;
+    x.relAir = m.rel_air != 0;
+    for (int k = 0; k < 4; ++k) {
+        x.a[k] = m.coeff_a[k];
+        x.b[k] = m.coeff_b[k];
+    }
+    return x;
+}
+
+struct Setup {
+    Lens L;
+    V3 dir;
+    double zStart = 0, zEP = 0, rEP = 0;
+    Paraxial px;
+};
+
+Setup setup(const owe_medium* media, const owe_lens_surface* surfaces, const owe_lens& lens, double fieldDeg,
+            double lambda, double lambdaRef) {
+    Setup u;
+    std::vector<owe_lens_surface> rows(surfaces + lens.first_surface, surfaces + lens.first_surface + lens.surface_count);
+    std::vector<double> nRef, nAt;
+    double z = 0;
+    for (const owe_lens_surface& r : rows) {
+        u.L.surf.push_back(makeSurf(r, z));
+        z += r.thickness;
+        nRef.push_back(indexOf(media[r.medium]).n(lambdaRef));
+        nAt.push_back(indexOf(media[r.medium]).n(lambda));
+    }
+    u.L.nAfter = nAt;
+    u.L.nAmbient = indexOf(media[lens.ambient]).n(lambda);
+    u.px = paraxial(rows, nRef, indexOf(media[lens.ambient]).n(lambdaRef));
+    const double f = fieldDeg * kPi / 180;
+    u.dir = V3(0, std::sin(f), std::cos(f));
+    u.zStart = -0.05 - 0.1 * u.px.length;
+    u.zEP = u.px.epz;
+    u.rEP = u.px.epr;
+    return u;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Traces rays index[i] of `lens` at wavelength lambda: out n × 7 = p, d, T (T = 0 and NaN p, d when
+// the ray does not arrive).
+void oracle_lens_trace(const owe_medium* media, const owe_lens_surface* surfaces, const owe_lens* lens, double fieldDeg,
+                       double lambda, double lambdaRef, const uint64_t* index, uint64_t n, double* out) {
+    const Setup u = setup(media, surfaces, *lens, fieldDeg, lambda, lambdaRef);
+    for (uint64_t k = 0; k < n; ++k) {
+        double x, y;
+        pupilPoint(index[k], lens->rays, 0.999 * u.rEP, x, y);
+        const V3 P(x, y, u.zEP);
+        const V3 o = P + u.dir * ((u.zStart - u.zEP) / u.dir.z);
+        V3 p, d;
+        double T;
+        double* r = out + 7 * k;
+        if (traceRay(u.L, o, u.dir, p, d, T)) {
+            r[0] = p.x, r[1] = p.y, r[2] = p.z, r[3] = d.x, r[4] = d.y, r[5] = d.z, r[6] = T;
+        } else {
+            for (int q = 0; q < 6; ++q) r[q] = std::nan("");
+            r[6] = 0;
+        }
+    }
+}
+
+// The spot sums of one lens at one wavelength over all its rays (OWE_LENS_MOMENTS doubles), and the
+// paraxial quantities (efl, bfl, entrance pupil z and radius, length) at lambdaRef.
+void oracle_lens_moments(const owe_medium* media, const owe_lens_surface* surfaces, const owe_lens* lens,
+                         double fieldDeg, double lambda, double lambdaRef, int threads, double* moments,
+                         double* paraxialOut) {
+    const Setup u = setup(media, surfaces, *lens, fieldDeg, lambda, lambdaRef);
+    if (threads <= 0) threads = int(std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
+    std::vector<std::vector<double>> part(size_t(threads), std::vector<double>(OWE_LENS_MOMENTS, 0.0));
+    std::vector<std::thread> pool;
+    const uint64_t n = lens->rays;
+    for (int t = 0; t < threads; ++t)
+        pool.emplace_back([&, t] {
+            std::vector<double>& m = part[size_t(t)];
+            for (uint64_t i = n * uint64_t(t) / uint64_t(threads); i < n * uint64_t(t + 1) / uint64_t(threads); ++i) {
+                double x, y;
+                pupilPoint(i, n, 0.999 * u.rEP, x, y);
+                const V3 P(x, y, u.zEP);
+                const V3 o = P + u.dir * ((u.zStart - u.zEP) / u.dir.z);
+                V3 p, d;
+                double T;
+                if (!traceRay(u.L, o, u.dir, p, d, T)) continue;
+                const double bx = d.x / d.z, by = d.y / d.z, ax = p.x - bx * p.z, ay = p.y - by * p.z;
+                m[0] += 1;
+                m[1] += ax;
+                m[2] += ay;
+                m[3] += bx;
+                m[4] += by;
+                m[5] += ax * ax + ay * ay;
+                m[6] += bx * bx + by * by;
+                m[7] += ax * bx + ay * by;
+                m[8] += T;
+            }
+        });
+    for (auto& th : pool) th.join();
+    for (int q = 0; q < OWE_LENS_MOMENTS; ++q) {
+        moments[q] = 0;
+        for (int t = 0; t < threads; ++t) moments[q] += part[size_t(t)][size_t(q)];
+    }
+    if (paraxialOut) {
+        paraxialOut[0] = u.px.efl;
+        paraxialOut[1] = u.px.bfl;
+        paraxialOut[2] = u.px.epz;
+        paraxialOut[3] = u.px.epr;
+        paraxialOut[4] = u.px.length;
+    }
+}
+
+// SagSurface::intersect alone, in the surface's frame: t, point, normal (NaN t when no hit).
+void oracle_sag_intersect(const owe_lens_surface* row, const double* o, const double* d, double tmin, double tmax,
+                          double* out) {
+    const Surf s = makeSurf(*row, 0);
+    double t;
+    V3 p, n;
+    if (intersectSag(s, V3(o[0], o[1], o[2]), V3(d[0], d[1], d[2]), tmin, tmax, t, p, n)) {
+        out[0] = t, out[1] = p.x, out[2] = p.y, out[3] = p.z, out[4] = n.x, out[5] = n.y, out[6] = n.z;
+    } else {
+        for (int q = 0; q < 7; ++q) out[q] = std::nan("");
+    }
+}
+
+double oracle_sag_value(const owe_lens_surface* row, double r) { return sag(makeSurf(*row, 0), r); }
+
+// One ray (origin o, unit direction d, lens frame) through `lens` at wavelength lambda: out = p, d, T
+// (T = 0 and NaN p, d when it does not arrive).
+void oracle_lens_trace_ray(const owe_medium* media, const owe_lens_surface* surfaces, const owe_lens* lens, double lambda,
+                           const double* o, const double* d, double* out) {
+    const Setup u = setup(media, surfaces, *lens, 0, lambda, lambda);
+    V3 p, w;
+    double T;
+    if (traceRay(u.L, V3(o[0], o[1], o[2]), V3(d[0], d[1], d[2]), p, w, T)) {
+        out[0] = p.x, out[1] = p.y, out[2] = p.z, out[3] = w.x, out[4] = w.y, out[5] = w.z, out[6] = T;
+    } else {
+        for (int q = 0; q < 6; ++q) out[q] = std::nan("");
+        out[6] = 0;
+    }
+}
+
+}  // extern "C"

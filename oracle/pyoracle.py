@@ -164,6 +164,11 @@ class Oracle(_Lib):
             "render_particles": (None, [_vp, _vp, _i, _i, _i, _vp, _vp]),
             "particles": (None, [_vp, _vp, _i, _u64, _vp, _vp, _vp, _i]),
             "emission_info": (None, [_vp, _vp]),
+            "lens_trace": (None, [_vp, _vp, _vp, _d, _d, _d, _vp, _u64, _vp]),
+            "lens_moments": (None, [_vp, _vp, _vp, _d, _d, _d, _i, _vp, _vp]),
+            "sag_intersect": (None, [_vp, _vp, _vp, _d, _d, _vp]),
+            "sag_value": (_d, [_vp, _d]),
+            "lens_trace_ray": (None, [_vp, _vp, _vp, _d, _vp, _vp, _vp]),
             "spectrum_luminance_blackbody": (_d, [_d]),
             "medium_index": (_d, [_vp, _d]),
             "ciddor_air": (_d, [_d]),
@@ -259,4 +264,46 @@ class World:
         self.o.emission_info(self.h, _ptr(out))
         return dict(sky_share=out[0], scene_radius=out[1], center=out[2:5].copy(), environment_power=out[5],
                     light_power=out[6])
+
+
+class LensOracle:
+    """The oracle's fp64 sequential lens trace over the same C records as the GPU batch
+    (owe_hip.lens): per-ray traces, spot sums and the paraxial quantities."""
+
+    def __init__(self, oracle: Oracle, lenses, field_deg: float = 0.5, reference: float = 587.56):
+        from owe_hip import lens as L
+
+        self.o = oracle
+        self.lenses = list(lenses)
+        self.field_deg, self.reference = field_deg, reference
+        self.media = L.media_array()
+        self.surf, self.arr = L.c_batch(self.lenses)
+
+    def trace(self, lens: int, lam: float, index) -> np.ndarray:
+        index = np.ascontiguousarray(index, dtype=np.uint64)
+        out = np.zeros((len(index), 7))
+        self.o.lens_trace(self.media, self.surf, ctypes.byref(self.arr[lens]), self.field_deg, lam, self.reference,
+                          _ptr(index), len(index), _ptr(out))
+        return out
+
+    def trace_ray(self, lens: int, lam: float, o, d) -> np.ndarray:
+        o = np.ascontiguousarray(o, dtype=np.float64)
+        d = np.ascontiguousarray(d, dtype=np.float64)
+        out = np.zeros(7)
+        self.o.lens_trace_ray(self.media, self.surf, ctypes.byref(self.arr[lens]), lam, _ptr(o), _ptr(d), _ptr(out))
+        return out
+
+    def sag_intersect(self, row, o, d, tmin: float = 0.0, tmax: float = np.inf) -> np.ndarray:
+        o = np.ascontiguousarray(o, dtype=np.float64)
+        d = np.ascontiguousarray(d, dtype=np.float64)
+        out = np.zeros(7)
+        self.o.sag_intersect(ctypes.byref(row), _ptr(o), _ptr(d), tmin, tmax, _ptr(out))
+        return out
+
+    def moments(self, lens: int, lam: float, threads: int = 0):
+        m = np.zeros(9)
+        px = np.zeros(5)
+        self.o.lens_moments(self.media, self.surf, ctypes.byref(self.arr[lens]), self.field_deg, lam, self.reference,
+                            threads, _ptr(m), _ptr(px))
+        return m, dict(efl=px[0], bfl=px[1], entrance_pupil_z=px[2], entrance_pupil_radius=px[3], length=px[4])
 

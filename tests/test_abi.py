@@ -1,4 +1,4 @@
-"""The C-ABI library loads and exports every entry point include/owe_hip.h declares (no GPU calls)."""
+"""The C-ABI library loads and exports every entry point include/*.h declares (no GPU calls)."""
 from __future__ import annotations
 
 import ctypes
@@ -10,18 +10,21 @@ import pytest
 from conftest import ROOT
 from owe_hip import _lib
 
-HEADER = ROOT / "include" / "owe_hip.h"
+HEADERS = sorted((ROOT / "include").glob("*.h"))
 
 
 def declared_functions() -> list[str]:
-    text = HEADER.read_text()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(owe_hip_[a-z0-9_]+)\s*\(", text)))
+    names = set()
+    for h in HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        names |= set(re.findall(r"\b(owe_hip_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
 
 
 def test_header_declares_entry_points():
     names = declared_functions()
     assert "owe_hip_trace_batch" in names and "owe_hip_last_error" in names
+    assert "owe_hip_lens_batch_run" in names and "owe_hip_renderer_particles" in names
     assert len(names) >= 10
 
 
