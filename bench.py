@@ -13,7 +13,8 @@ whole pass is also priced at E_seg·224 + 24 B per path with E_seg the measured 
 cpu_baseline: the oracle's fp64 restatement of Tracer::radiance (the reference's Tracer needs
 stb_image and cannot be built here) on the host threads, full frames of the same scene and spp.
 The ray-query workload is reported beside it under ``rayquery``, and light tracing of the same
-prism scene (256×256×64 particles per pass, the light integrator) under ``light``.
+prism scene (256×256×64 particles per pass, the light integrator) under ``light``, and the batched
+lens analysis (256 seeded achromats × 7 wavelengths in one persistent launch) under ``lens``.
 
 Workload ``rayquery`` (batch ray queries on a BVH): 1,048,820-triangle displaced icosphere
 (frequency 229, the nearest 20·n² count at or above 2^20), 8,388,608 seeded rays resident in HBM.
@@ -341,6 +342,59 @@ def bench_light(args):
     return out
 
 
+def bench_lens(args):
+    """Batched lens analysis: 256 seeded cemented achromats, each tracing its log-uniform number of
+    rays (4,096 to 1,048,576) at all seven Fraunhofer wavelengths in one persistent launch."""
+    from owe_hip import lens as L
+
+    lenses = L.seeded_achromats(256)
+    traces = 7 * sum(p.rays for p in lenses)
+    b = L.LensBatch(lenses)
+    tuned = b.autotune()
+    for _ in range(args.warmup):
+        b.run()
+    t0 = time.perf_counter()
+    kernel_ms = 0.0
+    for _ in range(args.steps):
+        b.run()
+        kernel_ms += b.last_ms()
+    wall = time.perf_counter() - t0
+    per, _, _ = b.results()
+    arrived = sum(r["arrived"] for r in per)
+    log(f"lens: {args.steps} runs in {wall:.3f}s; {traces * args.steps / wall / 1e9:.2f} G ray traces/s; "
+        f"kernel {kernel_ms / args.steps:.2f} ms/run; arrived {arrived / traces:.3f}")
+    # The grid's histogram atomics are the only algorithmic HBM traffic: 8 B per arrived ray.
+    hist_bytes = 8.0 * arrived
+    out = {"value": traces * args.steps / wall, "unit": "ray traces/s (lens x wavelength x ray)",
+           "ms_per_step": wall * 1e3 / args.steps,
+           "config": {"workload": "batched lens analysis, 256 seeded cemented achromats, 7 wavelengths, 0.5 deg field",
+                      "lenses": len(lenses), "traces_per_step": traces, "arrived_fraction": arrived / traces,
+                      "kernel_ms_per_step": kernel_ms / args.steps, "autotuned_ms": tuned,
+                      "rays_min": min(p.rays for p in lenses), "rays_max": max(p.rays for p in lenses)},
+           "roofline": {"bound": "hbm", "achieved": hist_bytes / (kernel_ms / args.steps * 1e-3) / 1e9,
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": hist_bytes / (kernel_ms / args.steps * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                        "kernel": "lensKernel", "algorithmic_bytes_per_launch": hist_bytes}}
+    b.close()
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        from pyoracle import LensOracle, Oracle
+
+        threads = min(16, os.cpu_count() or 1)
+        sample = lenses[:4]
+        lo = LensOracle(Oracle(), sample)
+        t0 = time.perf_counter()
+        for li in range(len(sample)):
+            for lam in L.FRAUNHOFER7:
+                lo.moments(li, lam, threads=threads)
+        el = time.perf_counter() - t0
+        n = 7 * sum(p.rays for p in sample)
+        out["cpu_baseline"] = {"value": n / el, "unit": "ray traces/s", "cores": threads, "kind": "port",
+                               "sample": f"the first 4 lenses ({n} traces, {el:.1f} s) through the oracle's fp64 "
+                                         f"sequential trace, {threads} threads"}
+    return out
+
+
 def main() -> int:
     import torch  # noqa: F401  (first: its HIP runtime is the process's; see owe_hip/_lib.py)
 
@@ -348,9 +402,10 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="prism", choices=["prism", "rayquery", "light"])
+    ap.add_argument("--workload", default="prism", choices=["prism", "rayquery", "light", "lens"])
     ap.add_argument("--no-rayquery", action="store_true", help="prism: skip the ray-query record beside it")
     ap.add_argument("--no-light", action="store_true", help="prism: skip the light-tracing record beside it")
+    ap.add_argument("--no-lens", action="store_true", help="prism: skip the lens-analysis record beside it")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 1 GiB device buffer 3 times (the known-size kernel that calibrates "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -380,8 +435,13 @@ def main() -> int:
         if not args.no_light:
             result["light"] = bench_light(argparse.Namespace(steps=args.steps, warmup=args.warmup,
                                                              no_cpu_baseline=args.no_cpu_baseline))
+        if not args.no_lens:
+            result["lens"] = bench_lens(argparse.Namespace(steps=args.steps, warmup=args.warmup,
+                                                           no_cpu_baseline=args.no_cpu_baseline))
     elif args.workload == "light":
         result = bench_light(args)
+    elif args.workload == "lens":
+        result = bench_lens(args)
     else:
         result = bench_rayquery(args)
     print(json.dumps(result), flush=True)

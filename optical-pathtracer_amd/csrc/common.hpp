@@ -52,4 +52,15 @@ int guard(F&& body) {
 // Makes `device` current after checking that it exists and is a gfx950.
 void selectDevice(int device);
 
+// Fails unless p points into device memory (of `device` when it is not negative): kernels must not
+// be handed host pointers.
+inline void requireDevicePointer(const void* p, const char* what, int device = -1) {
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess || a.type != hipMemoryTypeDevice ||
+        (device >= 0 && a.device != device)) {
+        (void)hipGetLastError();
+        fail(OWE_HIP_ERR_INVALID, std::string(what) + " must point to device memory");
+    }
+}
+
 }  // namespace owehip

@@ -111,6 +111,10 @@ def load() -> ctypes.CDLL:
     _declare(lib, "owe_hip_lens_batch_autotune", c_int, [c_vp, c_vp])
     _declare(lib, "owe_hip_lens_batch_trace", c_int, [c_vp, c_u32, c_int, c_vp, c_u64, c_vp])
     _declare(lib, "owe_hip_lens_batch_last_ms", c_int, [c_vp, c_vp])
+    _declare(lib, "owe_hip_caustic_create", c_int, [c_int, c_vp, ctypes.POINTER(c_vp)])
+    _declare(lib, "owe_hip_caustic_destroy", c_int, [c_vp])
+    _declare(lib, "owe_hip_caustic_forward", c_int, [c_vp, c_vp, c_vp, c_u64, c_vp])
+    _declare(lib, "owe_hip_caustic_backward", c_int, [c_vp, c_vp, c_vp, c_vp, c_u64, c_vp])
     _declare(lib, "owe_hip_renderer_set_timing", c_int, [c_vp, c_int])
     _declare(lib, "owe_hip_renderer_kernel_times", c_int, [c_vp, c_vp, c_vp])
     _lib = lib

@@ -169,6 +169,8 @@ class Oracle(_Lib):
             "sag_intersect": (None, [_vp, _vp, _vp, _d, _d, _vp]),
             "sag_value": (_d, [_vp, _d]),
             "lens_trace_ray": (None, [_vp, _vp, _vp, _d, _vp, _vp, _vp]),
+            "caustic_forward": (None, [_vp, _vp, _u64, _vp, _i]),
+            "caustic_fd": (None, [_vp, _vp, _vp, _u64, _vp, _i, _d, _vp]),
             "spectrum_luminance_blackbody": (_d, [_d]),
             "medium_index": (_d, [_vp, _d]),
             "ciddor_air": (_d, [_d]),
@@ -306,4 +308,26 @@ class LensOracle:
         self.o.lens_moments(self.media, self.surf, ctypes.byref(self.arr[lens]), self.field_deg, lam, self.reference,
                             threads, _ptr(m), _ptr(px))
         return m, dict(efl=px[0], bfl=px[1], entrance_pupil_z=px[2], entrance_pupil_radius=px[3], length=px[4])
+
+
+class CausticOracle:
+    """The oracle's fp64 caustic slab over the same settings record as the GPU (owe_hip.caustic)."""
+
+    def __init__(self, oracle: Oracle, settings):
+        self.o, self.s = oracle, settings
+
+    def forward(self, heights, pass_index: int = 0, threads: int = 0) -> np.ndarray:
+        h = np.ascontiguousarray(heights, dtype=np.float64)
+        E = np.zeros((self.s.pixels, self.s.pixels))
+        self.o.caustic_forward(ctypes.byref(self.s), _ptr(h), pass_index, _ptr(E), threads)
+        return E
+
+    def fd(self, heights, target, nodes, h: float = 1e-7, pass_index: int = 0) -> np.ndarray:
+        """Central differences of mean((E − target)²) for the heights at flat indices `nodes`."""
+        hh = np.ascontiguousarray(heights, dtype=np.float64)
+        tt = np.ascontiguousarray(target, dtype=np.float64)
+        nn = np.ascontiguousarray(nodes, dtype=np.uint32)
+        g = np.zeros(len(nn))
+        self.o.caustic_fd(ctypes.byref(self.s), _ptr(hh), _ptr(tt), pass_index, _ptr(nn), len(nn), h, _ptr(g))
+        return g
 
