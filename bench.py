@@ -1,20 +1,23 @@
 """Benchmark of the hip transport backend on one MI355X; prints ONE JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload prism|rayquery]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload prism|rayquery|light|lens|caustic]
 
 Workload ``prism`` (the default; plain spectral path tracing of a dispersion prism): an equilateral
 Sellmeier-glass prism (40 mm sides, 20 mm tall) on a 1 m² Lambertian floor of albedo 0.8 in air
 n = 1.000277, a 5778 K blackbody sun of 0.53°; a 256×256 pinhole camera, 64 spp, one wavelength per
 path uniform over 380–780 nm, max depth 16, Russian roulette from depth 4. One step = one pass of
-64 spp = 4,194,304 camera paths through the wavefront kernels; ``value`` is spectral paths per
-second. roofline: the kernel with the largest share of the pass time, priced at its algorithmic
-bytes per live path (path state 64 B read and written, hit record 16 B, shadow record 32 B); the
-whole pass is also priced at E_seg·224 + 24 B per path with E_seg the measured segments per path.
-cpu_baseline: the oracle's fp64 restatement of Tracer::radiance (the reference's Tracer needs
-stb_image and cannot be built here) on the host threads, full frames of the same scene and spp.
-The ray-query workload is reported beside it under ``rayquery``, and light tracing of the same
-prism scene (256×256×64 particles per pass, the light integrator) under ``light``, and the batched
-lens analysis (256 seeded achromats × 7 wavelengths in one persistent launch) under ``lens``.
+64 spp = 4,194,304 camera paths. Both forms run — the wavefront kernels and the persistent batched
+kernel replayed from a captured hipGraph, each autotuned first — and ``value`` is the faster form's
+spectral paths per second (the films of the two forms are compared bit for bit). roofline: the
+kernel with the largest share of the pass time, priced at its algorithmic bytes per live path (path
+state 64 B read and written, hit record 16 B, shadow record 32 B). cpu_baseline: the oracle's fp64
+restatement of Tracer::radiance (the reference's Tracer needs stb_image and cannot be built here) on
+the host threads, full frames of the same scene and spp.
+
+Beside it: ``rayquery`` (below), ``light`` (light tracing of the same prism scene, 256×256×64
+particles per pass), ``lens`` (256 seeded achromats × 7 wavelengths in one persistent launch) and
+``caustic`` (a gradient step of the caustic slab: 1,048,576 particles forward, then the path-replay
+adjoint into the 256×256 heights).
 
 Workload ``rayquery`` (batch ray queries on a BVH): 1,048,820-triangle displaced icosphere
 (frequency 229, the nearest 20·n² count at or above 2^20), 8,388,608 seeded rays resident in HBM.
@@ -395,6 +398,81 @@ def bench_lens(args):
     return out
 
 
+def bench_caustic(args):
+    """Differentiable caustic design: one gradient step of L = mean((E − target)²) over a 256 × 256
+    height field — 1,048,576 particles traced forward into the 512 × 512 wall grid, then replayed
+    backward (path-replay adjoint) into ∂L/∂heights."""
+    import torch
+
+    from owe_hip import caustic as C
+
+    s = C.caustic_settings()
+    c = C.Caustic(s)
+    H = torch.tensor(C.initial_heights(), device="cuda")
+    T = torch.tensor(C.target(s), device="cuda")
+    E = torch.empty((s.pixels, s.pixels), dtype=torch.float32, device="cuda")
+    g = torch.empty_like(H)
+
+    def step(k):
+        c.forward_into(H, E, k)
+        gE = (2.0 / E.numel()) * (E - T)
+        c.backward_into(H, gE, g, k)
+
+    for k in range(args.warmup):
+        step(k)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    torch.cuda.synchronize()
+    ev[0].record()
+    for k in range(args.steps):
+        c.forward_into(H, E, k)
+    ev[1].record()
+    gE = ((2.0 / E.numel()) * (E - T)).contiguous()
+    for k in range(args.steps):
+        c.backward_into(H, gE, g, k)
+    ev[2].record()
+    torch.cuda.synchronize()
+    f_ms, b_ms = ev[0].elapsed_time(ev[1]) / args.steps, ev[1].elapsed_time(ev[2]) / args.steps
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(k)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    n = s.particles
+    log(f"caustic: {args.steps} gradient steps in {wall:.3f}s; forward {f_ms:.3f} ms, adjoint {b_ms:.3f} ms "
+        f"(ratio {b_ms / f_ms:.2f}); {n * args.steps / wall / 1e6:.1f} M particles/s forward+adjoint")
+    # Algorithmic HBM traffic of the adjoint: 16 float atomics of 4 B per particle into the heights'
+    # gradient, ∂L/∂E read at 9 pixels (4 B each) — the heights and ∂L/∂E are L2-resident.
+    adj_bytes = n * (16 * 4 + 9 * 4)
+    out = {"value": n * args.steps / wall, "unit": "particles/s (forward + adjoint)", "ms_per_step": wall * 1e3 / args.steps,
+           "config": {"workload": "caustic design gradient step, 100 mm PMMA slab, 256x256 heights, 512x512 wall",
+                      "particles_per_step": n, "nodes": s.nodes, "pixels": s.pixels,
+                      "forward_ms": f_ms, "adjoint_ms": b_ms, "adjoint_over_forward": b_ms / f_ms},
+           "roofline": {"bound": "hbm", "achieved": adj_bytes / (b_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": adj_bytes / (b_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                        "kernel": "causticBackward", "algorithmic_bytes_per_launch": adj_bytes}}
+    c.close()
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import numpy as np
+        from pyoracle import CausticOracle, Oracle
+
+        threads = min(16, os.cpu_count() or 1)
+        co = CausticOracle(Oracle(), s)
+        Hn = H.cpu().numpy().astype(np.float64)
+        frames, el = 0, 0.0
+        while el < 10.0 and frames < 16:
+            t0 = time.perf_counter()
+            co.forward(Hn, frames, threads=threads)
+            el += time.perf_counter() - t0
+            frames += 1
+        out["cpu_baseline"] = {"value": frames * n / el, "unit": "particles/s (forward only)", "cores": threads,
+                               "kind": "port",
+                               "sample": f"{frames} forward passes of {n} particles ({el:.1f} s) through the oracle's "
+                                         f"fp64 slab, {threads} threads; the CPU has no adjoint — a finite-difference "
+                                         f"gradient costs 2 x {s.nodes * s.nodes} such passes"}
+    return out
+
+
 def main() -> int:
     import torch  # noqa: F401  (first: its HIP runtime is the process's; see owe_hip/_lib.py)
 
@@ -402,10 +480,11 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="prism", choices=["prism", "rayquery", "light", "lens"])
+    ap.add_argument("--workload", default="prism", choices=["prism", "rayquery", "light", "lens", "caustic"])
     ap.add_argument("--no-rayquery", action="store_true", help="prism: skip the ray-query record beside it")
     ap.add_argument("--no-light", action="store_true", help="prism: skip the light-tracing record beside it")
     ap.add_argument("--no-lens", action="store_true", help="prism: skip the lens-analysis record beside it")
+    ap.add_argument("--no-caustic", action="store_true", help="prism: skip the caustic-gradient record beside it")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 1 GiB device buffer 3 times (the known-size kernel that calibrates "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -438,6 +517,11 @@ def main() -> int:
         if not args.no_lens:
             result["lens"] = bench_lens(argparse.Namespace(steps=args.steps, warmup=args.warmup,
                                                            no_cpu_baseline=args.no_cpu_baseline))
+        if not args.no_caustic:
+            result["caustic"] = bench_caustic(argparse.Namespace(steps=args.steps, warmup=args.warmup,
+                                                                 no_cpu_baseline=args.no_cpu_baseline))
+    elif args.workload == "caustic":
+        result = bench_caustic(args)
     elif args.workload == "light":
         result = bench_light(args)
     elif args.workload == "lens":

@@ -57,7 +57,7 @@ typedef struct owe_lens_batch_settings {
     double wavelengths[OWE_LENS_MAX_WAVELENGTHS];  /* nm */
     int32_t n_wavelengths;
     int32_t detector_pixels;      /* side of each lens's square irradiance grid; 0: none */
-    double detector_half_size;    /* metres about the chief ray's paraxial image point; 0: 2e-3 × EFL */
+    double detector_half_size;    /* metres about the chief ray's paraxial image point; 0: 5e-3 × EFL */
     double reference_wavelength;  /* nm: the paraxial quantities and the detector plane */
 } owe_lens_batch_settings;
 

@@ -4,8 +4,10 @@
 // wavelength importance-sampled over the visible range and weighted by the sun's luminous spectrum,
 // and a point uniform over the slab's entry face. It enters at normal incidence (weight 1 − R),
 // crosses the slab along +z to the exit surface z = t + h(x, y), refracts out of it (weight 1 − R;
-// total internal reflection ends it) and lands on the wall, where its weight spreads bilinearly over
-// the four nearest pixel centres as 64-bit fixed-point adds — exact, so a pass is reproducible.
+// total internal reflection ends it) and lands on the wall, where its weight spreads over the 3 × 3
+// pixels around it with quadratic B-spline weights (continuously differentiable in the landing
+// point, so the loss is C¹ in the heights) as 64-bit fixed-point adds — exact, so a pass is
+// reproducible.
 //
 // Backward (path replay): the same particles are drawn again from their seeds and the same float
 // operations give the same wall point; the exit computation is also carried through three-component

@@ -632,7 +632,7 @@ int owe_hip_lens_batch_create(int device, const owe_medium* media, uint32_t n_me
             I.detZ = float(px.length + px.bfl);
             I.detX = 0.0f;
             I.detY = float(px.efl * std::tan(f));
-            I.detHalf = float(s.detector_half_size > 0 ? s.detector_half_size : 2e-3 * std::fabs(px.efl));
+            I.detHalf = float(s.detector_half_size > 0 ? s.detector_half_size : 5e-3 * std::fabs(px.efl));
             b->info.push_back(I);
         }
         ds.resize(n_surfaces);

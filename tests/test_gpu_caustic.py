@@ -101,14 +101,23 @@ def test_adjoint_matches_finite_differences(setup, oracle):
     fd = fd_gpu(3e-9)
     secant = fd_gpu(1e-7)
     a = g[nodes]
-    for name, ref, tol in (("gpu fd", fd, 0.01), ("fp64 cpu fd", fd_cpu, 0.01)):
+    results = {}
+    for name, ref in (("gpu fd", fd), ("fp64 cpu fd", fd_cpu)):
         rel = np.linalg.norm(a - ref) / np.linalg.norm(ref)
-        worst = (np.abs(a - ref) / (np.abs(ref) + 0.01 * np.abs(ref).max())).max()
-        print(f"adjoint vs {name}: relative error {rel:.2e}, worst component {worst:.2e}", flush=True)
-        assert rel < tol and worst < 2 * tol
+        worst = (np.abs(a - ref) / (np.abs(ref) + 0.02 * np.abs(ref).max())).max()
+        results[name] = (rel, worst)
+        print(f"adjoint vs {name}: relative error {rel:.2e} (norm over 64 heights), worst component {worst:.2e}",
+              flush=True)
     sec = np.linalg.norm(a - secant) / np.linalg.norm(secant)
     agree = np.linalg.norm(secant - secant_cpu) / np.linalg.norm(secant_cpu)
     print(f"1e-4 mm step: adjoint vs secant {sec:.2e}; gpu secant vs fp64 secant {agree:.2e}", flush=True)
+    # Within 1 % of the GPU's own finite differences over the 64 heights, each component within 5 %
+    # (the float forward's rounding makes single small differences noisy); the fp64 differences are
+    # free of that noise and pin the adjoint ten times tighter.
+    rel, worst = results["gpu fd"]
+    assert rel < 0.01 and worst < 0.05
+    rel, worst = results["fp64 cpu fd"]
+    assert rel < 1e-3 and worst < 5e-3
     assert agree < 0.01
 
 
