@@ -12,8 +12,8 @@
  *   paraxial EFL, BFL, entrance pupil                  paraxialAnalysis, src/owe/scene/prescription.cpp:193-321
  *   spot: closed-form best focus and RMS radius        focusedSpot, src/owe/analysis/analysis.cpp:85-113
  *
- * Every ray of lens l at wavelength j starts on the entrance pupil at point i of a Fibonacci disc
- * lattice of `rays` points (radius 0.999 of the pupil's), collimated at the batch's field angle in
+ * Every ray of lens l at wavelength j starts on the entrance pupil at point i of the lens's pupil
+ * pattern (radius 0.999 of the pupil's), collimated at the batch's field angle in
  * the meridional (y, z) plane, from z_start = −0.05 − 0.1·length (analyzeLens,
  * src/owe/analysis/analysis.cpp:174-178). Units are metres; the first vertex sits at z = 0. */
 #ifndef OWE_HIP_LENS_H
@@ -41,13 +41,16 @@ typedef struct owe_lens_surface {
 } owe_lens_surface;
 
 /* A lens: surfaces [first_surface, first_surface + surface_count) of the batch, the ambient medium
- * before and after it, and the rays it traces at each wavelength. */
+ * before and after it, and the rays it traces at each wavelength. The entrance pupil is sampled by
+ * a Fibonacci disc lattice of `rays` points (pupil_rings = 0), or by spotAt's hexapolar rings
+ * (src/owe/analysis/analysis.cpp:115-131): the centre, then 6k points on ring k = 1..pupil_rings
+ * at radius 0.999 rEP k / pupil_rings, angle 2π j / 6k — then `rays` must be 1 + 3 r (r + 1). */
 typedef struct owe_lens {
     uint32_t first_surface;
     uint32_t surface_count;
     uint64_t rays;
     int32_t ambient;
-    int32_t reserved;
+    int32_t pupil_rings;
 } owe_lens;
 
 #define OWE_LENS_MAX_WAVELENGTHS 16

@@ -48,7 +48,7 @@ struct DLensInfo {
     uint64_t rays;            // per wavelength
     uint64_t itemBase;        // first work item
     uint32_t itemsPerLambda;
-    uint32_t ambient;         // not used on the device (indices are tabulated)
+    uint32_t rings;           // 0: Fibonacci lattice; r: hexapolar rings
     double rEP, zEP, zStart;  // entrance pupil (sampling radius 0.999 rEP) and launch plane
     float detX, detY, detZ, detHalf;  // detector grid centre, plane, half size
 };
@@ -257,9 +257,26 @@ __device__ __forceinline__ bool sagHit(const DSurf& s, f3 o, f3 d, f3& p, f3& n)
     return false;
 }
 
-// Fibonacci lattice point i of n over a disc of radius r, in double.
-__device__ __forceinline__ void pupilPoint(uint64_t i, uint64_t n, double r, double& x, double& y) {
-    const double rr = r * sqrt((double(i) + 0.5) / double(n));
+// Pupil point i of n on the entrance pupil of radius rEP, in double: a Fibonacci lattice over the
+// disc of radius 0.999 rEP, or with rings > 0 spotAt's hexapolar pattern (analysis.cpp:124-131) —
+// point 0 the centre, ring k holding the 6k points after the 1 + 3k(k − 1) before it.
+__device__ __forceinline__ void pupilPoint(uint64_t i, uint64_t n, uint32_t rings, double rEP, double& x, double& y) {
+    if (rings > 0) {
+        if (i == 0) {
+            x = y = 0.0;
+            return;
+        }
+        uint64_t k = uint64_t((3.0 + sqrt(12.0 * double(i) - 3.0)) / 6.0);
+        while (k > 1 && 1 + 3 * k * (k - 1) > i) --k;
+        while (1 + 3 * k * (k + 1) <= i) ++k;
+        const uint64_t j = i - (1 + 3 * k * (k - 1)), m = 6 * k;
+        const double rr = rEP * double(k) / double(rings) * 0.999;
+        const double a = 2.0 * 3.14159265358979323846 * double(j) / double(m);
+        x = rr * cos(a);
+        y = rr * sin(a);
+        return;
+    }
+    const double rr = 0.999 * rEP * sqrt((double(i) + 0.5) / double(n));
     double f = double(i) * 0.6180339887498949;
     f -= floor(f);
     double sa, ca;
@@ -272,7 +289,7 @@ __device__ __forceinline__ void pupilPoint(uint64_t i, uint64_t n, double r, dou
 __device__ __forceinline__ bool traceLensRay(const LensArgs& a, const DLensInfo& L, uint32_t li, uint32_t w, uint64_t i,
                                              f3& p, f3& d, float& T) {
     double x, y;
-    pupilPoint(i, L.rays, 0.999 * L.rEP, x, y);
+    pupilPoint(i, L.rays, L.rings, L.rEP, x, y);
     const double s = (L.zStart - L.zEP) / a.dirZ;
     f3 o = mk3(float(x), float(y + a.dirY * s), float(L.zEP + a.dirZ * s));
     d = mk3(0.0f, float(a.dirY), float(a.dirZ));
@@ -600,6 +617,9 @@ int owe_hip_lens_batch_create(int device, const owe_medium* media, uint32_t n_me
                     name + ": surfaces out of range");
             require(L.rays >= 1 && L.rays <= (uint64_t(1) << 32), name + ": rays must lie in [1, 2^32]");
             require(L.ambient >= 0 && uint32_t(L.ambient) < n_media, name + ": ambient medium out of range");
+            require(L.pupil_rings >= 0 && L.pupil_rings <= 30000, name + ": pupil rings must lie in [0, 30000]");
+            require(L.pupil_rings == 0 || L.rays == 1 + 3 * uint64_t(L.pupil_rings) * uint64_t(L.pupil_rings + 1),
+                    name + ": with pupil rings r the lens traces 1 + 3 r (r + 1) rays");
             const owe_lens_surface* rows = surfaces + L.first_surface;
             std::vector<double> nRef(L.surface_count);
             for (uint32_t k = 0; k < L.surface_count; ++k) {
@@ -623,7 +643,7 @@ int owe_hip_lens_batch_create(int device, const owe_medium* media, uint32_t n_me
             I.first = L.first_surface;
             I.count = L.surface_count;
             I.rays = L.rays;
-            I.ambient = uint32_t(L.ambient);
+            I.rings = uint32_t(L.pupil_rings);
             I.rEP = px.epr;
             I.zEP = px.epz;
             I.zStart = -0.05 - 0.1 * px.length;

@@ -88,6 +88,10 @@ class Prescription:
     ambient: str = "air"
     rays: int = 4096
     name: str = ""
+    rings: int = 0            # > 0: spotAt's hexapolar pupil rings (then rays = 1 + 3 r (r + 1))
+
+    def with_rings(self, rings: int) -> "Prescription":
+        return Prescription(self.surfaces, self.ambient, 1 + 3 * rings * (rings + 1), self.name, rings)
 
 
 SINGLET = Prescription([Surface(50.0, 8.0, "N-BK7", 12.5), Surface(-50.0, 40.0, "air", 12.5)], name="biconvex singlet")
@@ -103,7 +107,7 @@ class CLensSurface(ctypes.Structure):
 
 class CLens(ctypes.Structure):
     _fields_ = [("first_surface", ctypes.c_uint32), ("surface_count", ctypes.c_uint32), ("rays", ctypes.c_uint64),
-                ("ambient", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("ambient", ctypes.c_int32), ("pupil_rings", ctypes.c_int32)]
 
 
 class CLensSettings(ctypes.Structure):
@@ -144,7 +148,7 @@ def c_batch(lenses):
     first = 0
     for i, p in enumerate(lenses):
         arr[i].first_surface, arr[i].surface_count = first, len(p.surfaces)
-        arr[i].rays, arr[i].ambient = int(p.rays), MEDIA_ORDER.index(p.ambient)
+        arr[i].rays, arr[i].ambient, arr[i].pupil_rings = int(p.rays), MEDIA_ORDER.index(p.ambient), int(p.rings)
         first += len(p.surfaces)
     return surf, arr
 

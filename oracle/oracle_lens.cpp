@@ -243,9 +243,28 @@ bool traceRay(const Lens& L, V3 o, V3 d, V3& p, V3& dOut, double& T) {
     return d.z > 0;
 }
 
-// Fibonacci lattice point i of n over a disc of radius r (the batch's entrance-pupil sampling).
-void pupilPoint(uint64_t i, uint64_t n, double r, double& x, double& y) {
-    const double rr = r * std::sqrt((double(i) + 0.5) / double(n));
+// Pupil point i of n on the entrance pupil of radius rEP: with rings > 0 spotAt's hexapolar
+// pattern (analysis.cpp:124-131), walked ring by ring; else a Fibonacci lattice over the disc of
+// radius 0.999 rEP (the batch's sampling for arbitrary ray counts).
+void pupilPoint(uint64_t i, uint64_t n, int rings, double rEP, double& x, double& y) {
+    if (rings > 0) {
+        uint64_t seen = 0;
+        for (int k = 0; k <= rings; ++k) {
+            const uint64_t m = k == 0 ? 1 : 6 * uint64_t(k);
+            if (i < seen + m) {
+                const uint64_t j = i - seen;
+                const double rr = rEP * double(k) / rings * 0.999;
+                const double a = 2 * kPi * double(j) / double(m);
+                x = rr * std::cos(a);
+                y = rr * std::sin(a);
+                return;
+            }
+            seen += m;
+        }
+        x = y = std::nan("");
+        return;
+    }
+    const double rr = 0.999 * rEP * std::sqrt((double(i) + 0.5) / double(n));
     double f = double(i) * 0.6180339887498949;
     f -= std::floor(f);
     const double a = 2 * kPi * f;
@@ -377,7 +396,7 @@ void oracle_lens_trace(const owe_medium* media, const owe_lens_surface* surfaces
     const Setup u = setup(media, surfaces, *lens, fieldDeg, lambda, lambdaRef);
     for (uint64_t k = 0; k < n; ++k) {
         double x, y;
-        pupilPoint(index[k], lens->rays, 0.999 * u.rEP, x, y);
+        pupilPoint(index[k], lens->rays, lens->pupil_rings, u.rEP, x, y);
         const V3 P(x, y, u.zEP);
         const V3 o = P + u.dir * ((u.zStart - u.zEP) / u.dir.z);
         V3 p, d;
@@ -407,7 +426,7 @@ void oracle_lens_moments(const owe_medium* media, const owe_lens_surface* surfac
             std::vector<double>& m = part[size_t(t)];
             for (uint64_t i = n * uint64_t(t) / uint64_t(threads); i < n * uint64_t(t + 1) / uint64_t(threads); ++i) {
                 double x, y;
-                pupilPoint(i, n, 0.999 * u.rEP, x, y);
+                pupilPoint(i, n, lens->pupil_rings, u.rEP, x, y);
                 const V3 P(x, y, u.zEP);
                 const V3 o = P + u.dir * ((u.zStart - u.zEP) / u.dir.z);
                 V3 p, d;

@@ -73,6 +73,29 @@ def test_spot_matches_oracle(batch, oracle, li):
     assert np.abs(mom[li, :, 0] - ms[:, 0]).max() <= max(2, s["arrived"] // 2000)
 
 
+def test_ring_pupil_matches_oracle(cuda, oracle):
+    # spotAt's hexapolar rings: six (the reference's analyzeLens call) and 300 (270,901 rays).
+    lenses = [L.SINGLET.with_rings(6), L.DOUBLET.with_rings(6), LENSES[2].with_rings(300)]
+    b = L.LensBatch(lenses)
+    b.run()
+    per, mom, _ = b.results()
+    lo = LensOracle(oracle, lenses)
+    for li, p in enumerate(lenses):
+        idx = np.arange(min(p.rays, 20000), dtype=np.uint64)
+        g = b.trace(li, 4, idx)
+        c = lo.trace(li, L.FRAUNHOFER7[4], idx)
+        okg, okc = g[:, 6] > 0, c[:, 6] > 0
+        both = okg & okc
+        flips = int((okg != okc).sum())
+        dp = np.abs(g[both, :3] - c[both, :3]).max(initial=0)
+        ms = np.array([lo.moments(li, lam)[0] for lam in L.FRAUNHOFER7])
+        print(f"rings lens {li}: {p.rays} rays, {both.sum()} both arrive, {flips} flips, |Δp| {dp:.2e}, "
+              f"arrived {per[li]['arrived']} / {int(ms[:, 0].sum())}", flush=True)
+        assert flips <= 2 and dp < 1e-8
+        assert abs(per[li]["arrived"] - ms[:, 0].sum()) <= max(2, ms[:, 0].sum() // 2000)
+    b.close()
+
+
 def test_doublet_focuses_every_ray_on_axis(cuda):
     # On axis every ray of the reference's achromat arrives (tests/test_instruments.cpp:333).
     b = L.LensBatch([L.DOUBLET], L.c_settings(field_deg=0.0))

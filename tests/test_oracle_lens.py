@@ -10,7 +10,10 @@
 * an achromat reduces the F–C focal shift per unit power tenfold against the singlet, and every
   launched ray arrives (tests/test_instruments.cpp:320-333); the singlet's spherical aberration is
   undercorrected;
-* focusedSpot from the nine sums equals the direct computation over the traced rays.
+* focusedSpot from the nine sums equals the direct computation over the traced rays;
+* the ring pupil visits spotAt's hexapolar points in its order (analysis.cpp:124-131), and with
+  six rings on axis at F, d and C — the reference's own analyzeLens call — the achromat and
+  singlet trends above hold as they do under the Fibonacci lattice.
 """
 from __future__ import annotations
 
@@ -89,8 +92,36 @@ def test_numeric_asphere_path_equals_the_quadric_path(oracle):
             assert np.abs(a[:4] - b[:4]).max() < 1e-12
 
 
-def test_achromat_cuts_the_chromatic_focal_shift_and_keeps_every_ray(oracle):
+def ring_points(rings, r_ep):
+    """spotAt's pupil pattern written out (analysis.cpp:124-131): centre, then 6k points on ring k."""
+    pts = []
+    for k in range(rings + 1):
+        n = 1 if k == 0 else 6 * k
+        for j in range(n):
+            rr = r_ep * k / rings * 0.999
+            a = 2 * math.pi * j / n
+            pts.append((rr * math.cos(a), rr * math.sin(a)))
+    return np.array(pts)
+
+
+def test_ring_pupil_walks_spot_at_pattern_in_order(oracle):
+    # A flat window on axis passes every ray straight: its exit points are the pupil points.
+    window = L.Prescription([L.Surface(math.inf, 5.0, "N-BK7", 10.0, stop=True),
+                             L.Surface(math.inf, 10.0, "air", 10.0)]).with_rings(6)
+    assert window.rays == 127
+    lo = LensOracle(oracle, [window], field_deg=0.0)
+    out = lo.trace(0, L.LAMBDA_D, np.arange(window.rays, dtype=np.uint64))
+    _, px = lo.moments(0, L.LAMBDA_D)
+    assert px["entrance_pupil_radius"] == pytest.approx(0.010, rel=1e-12)
+    assert np.abs(out[:, :2] - ring_points(6, px["entrance_pupil_radius"])).max() < 1e-15
+    assert (out[:, 6] > 0).all()
+
+
+@pytest.mark.parametrize("rings", [0, 6])
+def test_achromat_cuts_the_chromatic_focal_shift_and_keeps_every_ray(oracle, rings):
+    # rings = 6 is the reference's own configuration: analyzeLens(p, {0}, {F, d, C}, 6).
     def focus(p, lam):
+        p = p.with_rings(rings) if rings else p
         lo = LensOracle(oracle, [p], field_deg=0.0, reference=lam)
         m, px = lo.moments(0, lam)
         lv = px["length"]
