@@ -50,7 +50,10 @@ RQ_NODES_PER_QUERY = 50.684757232666016  # slab tests per closest-hit query, ref
 RQ_TRIS_PER_QUERY = 7.060239195823669    # triangle tests per closest-hit query, same
 RQ_BYTES_PER_QUERY = 40 + 32 * RQ_NODES_PER_QUERY + 36 * RQ_TRIS_PER_QUERY
 PROFILE_TRAFFIC = ROOT / "profiles" / "rayquery_closest_traffic.md"
-PROFILE_PRISM_TRAFFIC = ROOT / "profiles" / "prism_wavefront_traffic.md"
+PROFILE_PRISM_TRAFFIC = ROOT / "profiles" / "prism_camera_paths_traffic.md"
+PROFILE_LIGHT_TRAFFIC = ROOT / "profiles" / "light_particles_traffic.md"
+PROFILE_LENS_TRAFFIC = ROOT / "profiles" / "lens_batch_traffic.md"
+PROFILE_CAUSTIC_TRAFFIC = ROOT / "profiles" / "caustic_adjoint_traffic.md"
 
 
 def log(msg: str) -> None:
@@ -335,11 +338,19 @@ def bench_light(args):
     n = stats["paths"]
     log(f"light: {args.steps} passes in {wall:.3f}s; {n / wall / 1e6:.1f} M particles/s; "
         f"segments/particle {stats['segments'] / max(1, n):.3f}; particle kernel {ms / args.steps:.3f} ms/pass")
+    # Algorithmic HBM bytes: each particle vertex connects to the eye at most once and deposits three
+    # 64-bit bin adds (24 B); the particle's own state never leaves registers.
+    splat_bytes = 24.0 * stats["segments"] / args.steps
+    k_ms = ms / args.steps
     out = {"value": n / wall, "unit": "particles/s", "ms_per_step": wall * 1e3 / args.steps,
            "config": {"workload": "light tracing (particles), dispersion prism", "resolution": [PRISM_SIZE, PRISM_SIZE],
                       "particles_per_step": PRISM_SIZE * PRISM_SIZE * PRISM_SPP, "max_depth": 16, "rr_depth": 4,
                       "segments_per_particle": stats["segments"] / max(1, n),
-                      "kernel_ms_per_step": ms / args.steps, "mean_Y": float(r.resolve()[..., 1].mean())}}
+                      "kernel_ms_per_step": k_ms, "mean_Y": float(r.resolve()[..., 1].mean())},
+           "roofline": {"bound": "hbm", "achieved": splat_bytes / (k_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": splat_bytes / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                        "traffic": measured_traffic(PROFILE_LIGHT_TRAFFIC, "particleKernel"),
+                        "kernel": "particleKernel", "algorithmic_bytes_per_launch": splat_bytes}}
     out["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline_light(sc, st)
     r.close()
     return out
@@ -376,7 +387,8 @@ def bench_lens(args):
                       "rays_min": min(p.rays for p in lenses), "rays_max": max(p.rays for p in lenses)},
            "roofline": {"bound": "hbm", "achieved": hist_bytes / (kernel_ms / args.steps * 1e-3) / 1e9,
                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": hist_bytes / (kernel_ms / args.steps * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                        "frac": hist_bytes / (kernel_ms / args.steps * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                        "traffic": measured_traffic(PROFILE_LENS_TRAFFIC, "lensKernel"),
                         "kernel": "lensKernel", "algorithmic_bytes_per_launch": hist_bytes}}
     b.close()
     if not args.no_cpu_baseline:
@@ -448,7 +460,8 @@ def bench_caustic(args):
                       "particles_per_step": n, "nodes": s.nodes, "pixels": s.pixels,
                       "forward_ms": f_ms, "adjoint_ms": b_ms, "adjoint_over_forward": b_ms / f_ms},
            "roofline": {"bound": "hbm", "achieved": adj_bytes / (b_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": adj_bytes / (b_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                        "unit": "GB/s", "frac": adj_bytes / (b_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                        "traffic": measured_traffic(PROFILE_CAUSTIC_TRAFFIC, "causticBackward"),
                         "kernel": "causticBackward", "algorithmic_bytes_per_launch": adj_bytes}}
     c.close()
     if not args.no_cpu_baseline:

@@ -80,9 +80,19 @@ typedef struct owe_emission {
 } owe_emission;
 
 /* Shapes, each in its own frame (scene/geometry.hpp). */
-#define OWE_SHAPE_MESH 0   /* triangles [first_triangle, first_triangle + triangle_count) */
-#define OWE_SHAPE_PLANE 1  /* z = 0 inside an aperture, normal +z */
-#define OWE_SHAPE_SPHERE 2 /* radius a about the origin, normal outward */
+#define OWE_SHAPE_MESH 0     /* triangles [first_triangle, first_triangle + triangle_count) */
+#define OWE_SHAPE_PLANE 1    /* z = 0 inside an aperture, normal +z */
+#define OWE_SHAPE_SPHERE 2   /* radius a about the origin, normal outward */
+#define OWE_SHAPE_CYLINDER 3 /* open cylinder x² + y² = a², params[0] ≤ z ≤ params[1], normal outward
+                                (CylinderShape, geometry.hpp:191-209) */
+#define OWE_SHAPE_SAG 4      /* z = sag(r) for b ≤ r ≤ a, vertex at the origin, normal +z (SagSurface,
+                                geometry.hpp:46-82): params = curvature 1/R, conic k, A4, A6, A8, A10 */
+#define OWE_SHAPE_WAVE 5     /* z = h(x, y) over |x| ≤ a, |y| ≤ b, normal +z (WaveSurface,
+                                geometry.hpp:90-142): the plane waves [first_wave, first_wave +
+                                wave_count) of the scene tapered to zero within params[0] of the
+                                edges; params[1] is the node spacing (0: the analytic sum; < 0:
+                                automatic, the sum up to six waves, else a bicubic Hermite spline
+                                sampled every twelfth of the shortest wavelength) */
 
 #define OWE_APERTURE_DISK 0    /* a = outer radius, b = inner radius */
 #define OWE_APERTURE_RECT 1    /* a, b = half sizes along x, y */
@@ -96,12 +106,21 @@ typedef struct owe_boundary {
     uint32_t front;   /* region on the +normal side */
     uint32_t back;
     int32_t optics;
-    int32_t emission; /* -1: none */
+    int32_t emission; /* -1: none; cylinders, sags and waves do not emit */
     uint32_t first_triangle;
     uint32_t triangle_count;
     double a, b;
     double to_world[12]; /* row-major [R | t]: x_world = R x_local + t (rigid) */
+    double params[6];    /* per shape, see OWE_SHAPE_* */
+    uint32_t first_wave; /* OWE_SHAPE_WAVE: its records in owe_scene.waves */
+    uint32_t wave_count;
 } owe_boundary;
+
+/* One plane wave of a liquid surface (PlaneWave, geometry.hpp:84-89): amplitude (m), wave vector
+ * (rad/m), phase (rad). */
+typedef struct owe_plane_wave {
+    double amplitude, kx, ky, phase;
+} owe_plane_wave;
 
 #define OWE_SKY_NONE 0
 #define OWE_SKY_UNIFORM 1
@@ -162,6 +181,9 @@ typedef struct owe_scene {
     const double* table;        /* tabulated spectra */
     owe_environment env;
     owe_observer observer;
+    const owe_plane_wave* waves; /* n_waves records for OWE_SHAPE_WAVE boundaries */
+    uint32_t n_waves;
+    uint32_t reserved;
 } owe_scene;
 
 #define OWE_INTEGRATOR_PATH 0

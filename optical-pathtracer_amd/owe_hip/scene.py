@@ -24,7 +24,7 @@ import numpy as np
 SPEC_CONSTANT, SPEC_BLACKBODY, SPEC_TABULATED = 0, 1, 2
 INDEX_CONSTANT, INDEX_CAUCHY, INDEX_SELLMEIER, INDEX_CIDDOR = 0, 1, 2, 3
 SURF_NULL, SURF_DIELECTRIC, SURF_DIFFUSE, SURF_CONDUCTOR, SURF_MIRROR, SURF_ABSORBER, SURF_DETECTOR = range(7)
-SHAPE_MESH, SHAPE_PLANE, SHAPE_SPHERE = 0, 1, 2
+SHAPE_MESH, SHAPE_PLANE, SHAPE_SPHERE, SHAPE_CYLINDER, SHAPE_SAG, SHAPE_WAVE = 0, 1, 2, 3, 4, 5
 APERTURE_DISK, APERTURE_RECT, APERTURE_ELLIPSE = 0, 1, 2
 SKY_NONE, SKY_UNIFORM = 0, 1
 INTEGRATOR_PATH, INTEGRATOR_LIGHT = 0, 1
@@ -59,7 +59,13 @@ class CBoundary(ctypes.Structure):
     _fields_ = [("shape", ctypes.c_int32), ("aperture", ctypes.c_int32), ("front", ctypes.c_uint32),
                 ("back", ctypes.c_uint32), ("optics", ctypes.c_int32), ("emission", ctypes.c_int32),
                 ("first_triangle", ctypes.c_uint32), ("triangle_count", ctypes.c_uint32), ("a", ctypes.c_double),
-                ("b", ctypes.c_double), ("to_world", ctypes.c_double * 12)]
+                ("b", ctypes.c_double), ("to_world", ctypes.c_double * 12), ("params", ctypes.c_double * 6),
+                ("first_wave", ctypes.c_uint32), ("wave_count", ctypes.c_uint32)]
+
+
+class CPlaneWave(ctypes.Structure):
+    _fields_ = [("amplitude", ctypes.c_double), ("kx", ctypes.c_double), ("ky", ctypes.c_double),
+                ("phase", ctypes.c_double)]
 
 
 class CEnvironment(ctypes.Structure):
@@ -88,7 +94,8 @@ class CScene(ctypes.Structure):
                 ("optics", ctypes.POINTER(COptics)), ("emissions", ctypes.POINTER(CEmission)),
                 ("boundaries", ctypes.POINTER(CBoundary)), ("vertices", ctypes.POINTER(ctypes.c_double)),
                 ("triangles", ctypes.POINTER(ctypes.c_uint32)), ("table", ctypes.POINTER(ctypes.c_double)),
-                ("env", CEnvironment), ("observer", CObserver)]
+                ("env", CEnvironment), ("observer", CObserver), ("waves", ctypes.POINTER(CPlaneWave)),
+                ("n_waves", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 class CRenderSettings(ctypes.Structure):
@@ -233,6 +240,7 @@ class Scene:
         self.optics: list[Optics] = []
         self.emissions: list[Emission] = []
         self.boundaries: list[dict] = []
+        self.waves: list[tuple] = []
         self.vertices: list[np.ndarray] = []
         self.triangles: list[np.ndarray] = []
         self.n_vertices = 0
@@ -267,6 +275,34 @@ class Scene:
     def add_sphere(self, radius: float, xf, front: int, back: int, optics: int, emission: int = -1) -> int:
         self.boundaries.append(dict(shape=SHAPE_SPHERE, aperture=0, a=radius, b=0.0, xf=np.asarray(xf, float),
                                     front=front, back=back, optics=optics, emission=emission, first=0, count=0))
+        return len(self.boundaries) - 1
+
+    def add_cylinder(self, radius: float, z0: float, z1: float, xf, front: int, back: int, optics: int) -> int:
+        """Open cylinder x² + y² = radius², z0 ≤ z ≤ z1, normal outward (CylinderShape)."""
+        self.boundaries.append(dict(shape=SHAPE_CYLINDER, aperture=0, a=radius, b=0.0, xf=np.asarray(xf, float),
+                                    front=front, back=back, optics=optics, emission=-1, first=0, count=0,
+                                    params=(z0, z1)))
+        return len(self.boundaries) - 1
+
+    def add_sag(self, curvature: float, conic: float, asphere, r_max: float, r_min: float, xf, front: int, back: int,
+                optics: int) -> int:
+        """z = sag(r) for r_min ≤ r ≤ r_max, vertex at the origin, normal +z (SagSurface)."""
+        A = list(asphere) + [0.0] * (4 - len(asphere))
+        self.boundaries.append(dict(shape=SHAPE_SAG, aperture=0, a=r_max, b=r_min, xf=np.asarray(xf, float),
+                                    front=front, back=back, optics=optics, emission=-1, first=0, count=0,
+                                    params=(curvature, conic, *A[:4])))
+        return len(self.boundaries) - 1
+
+    def add_wave(self, waves, half_x: float, half_y: float, margin: float, xf, front: int, back: int, optics: int,
+                 spacing: float = -1.0) -> int:
+        """z = h(x, y) over the rectangle, a sum of plane waves (amplitude, kx, ky, phase) tapered
+        to zero within `margin` of the edges (WaveSurface); spacing > 0 samples it as a bicubic
+        Hermite spline, < 0 chooses as the reference does (analytic up to six waves)."""
+        first = len(self.waves)
+        self.waves.extend(tuple(map(float, w)) for w in waves)
+        self.boundaries.append(dict(shape=SHAPE_WAVE, aperture=0, a=half_x, b=half_y, xf=np.asarray(xf, float),
+                                    front=front, back=back, optics=optics, emission=-1, first=0, count=0,
+                                    params=(margin, spacing), waves=(first, len(waves))))
         return len(self.boundaries) - 1
 
     def add_mesh(self, positions, triangles, xf, front: int, back: int, optics: int, emission: int = -1) -> int:
@@ -351,6 +387,12 @@ class Scene:
             cb.a, cb.b = b["a"], b["b"]
             for k, v in enumerate(b["xf"].reshape(-1)):
                 cb.to_world[k] = v
+            for k, v in enumerate(b.get("params", ())):
+                cb.params[k] = v
+            cb.first_wave, cb.wave_count = b.get("waves", (0, 0))
+        waves = (CPlaneWave * max(1, len(self.waves)))()
+        for i, w in enumerate(self.waves):
+            waves[i].amplitude, waves[i].kx, waves[i].ky, waves[i].phase = w
         regions = np.ascontiguousarray(self.region_medium, dtype=np.uint32)
         verts = (np.ascontiguousarray(np.concatenate(self.vertices), dtype=np.float64) if self.vertices
                  else np.zeros((1, 3)))
@@ -392,7 +434,8 @@ class Scene:
         s.triangles = tris.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
         s.table = tab.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         s.env, s.observer = env, ob
-        self._keep = [media, optics, emissions, bounds, regions, verts, tris, tab]
+        s.waves, s.n_waves = ctypes.cast(waves, ctypes.POINTER(CPlaneWave)), len(self.waves)
+        self._keep = [media, optics, emissions, bounds, regions, verts, tris, tab, waves]
         return s
 
 

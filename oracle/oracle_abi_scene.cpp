@@ -211,6 +211,29 @@ void oracle_emission_info(const void* wh, double* out) {
     out[6] = w.totalLightPower;
 }
 
+// World::intersect for n rays (origins and directions in world coordinates, the boundary each
+// departs or ~0u): out n × 8 = t, point, normal, boundary (t = inf and boundary ~0u on a miss).
+void oracle_world_intersect(const void* wh, uint64_t n, const double* o, const double* d, const uint32_t* departed,
+                            double* out) {
+    const World& w = *static_cast<const World*>(wh);
+    for (uint64_t i = 0; i < n; ++i) {
+        Ray r;
+        r.o = V3(o[3 * i], o[3 * i + 1], o[3 * i + 2]);
+        r.d = V3(d[3 * i], d[3 * i + 1], d[3 * i + 2]);
+        r.departed = departed ? departed[i] : kNone;
+        Hit h;
+        double* q = out + 8 * i;
+        if (w.intersect(r, kInf, h)) {
+            q[0] = h.t, q[1] = h.p.x, q[2] = h.p.y, q[3] = h.p.z, q[4] = h.n.x, q[5] = h.n.y, q[6] = h.n.z;
+            q[7] = double(h.boundary);
+        } else {
+            q[0] = kInf;
+            for (int k = 1; k < 7; ++k) q[k] = std::nan("");
+            q[7] = double(kNone);
+        }
+    }
+}
+
 double oracle_spectrum_luminance_blackbody(double kelvin) {
     Spectrum s;
     s.kind = OWE_SPEC_BLACKBODY;

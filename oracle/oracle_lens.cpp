@@ -1,10 +1,6 @@
 // TEST INFRASTRUCTURE ONLY — the double-precision CPU oracle (see oracle_core.hpp).
 //
-// Sequential lens analysis restated in double from the reference:
-//   quadric roots, refinement from the approximate hit       src/owe/core/shared/surfaces.hpp:13-94
-//   conic sag, slope, normal, conic sheet; slab and cylinder  src/owe/core/shared/surfaces.hpp:140-171
-//   SagSurface: quadric path and numeric asphere (96 samples  src/owe/scene/geometry.cpp:26-166
-//   + bisection), aperture acceptance
+// Sequential lens analysis restated in double from the reference (sag surfaces: oracle_shapes.hpp):
 //   transmission through a smooth dielectric (weight T)       Tracer::walk PrimaryTransmission,
 //                                                             src/owe/transport/transport.cpp:531-595,
 //                                                             scattering.cpp:65-108, optics.hpp:14-54
@@ -27,189 +23,19 @@ using namespace oracle;
 
 namespace {
 
-constexpr double kTinyD = 1e-300, kRefineD = 1e-6, kBisectD = 1e-15, kRelTolD = 1e-9;
-constexpr int kBisectStepsD = 80;
+using shapes::intersectSag;
+using shapes::sag;
 
-int solveQuadratic(double a, double b, double c, double& t0, double& t1) {
-    t0 = t1 = 0;
-    if (std::fabs(a) < kTinyD || std::fabs(a) < 1e-14 * std::fabs(b)) {
-        if (b == 0) return 0;
-        t0 = t1 = -c / b;
-        return 1;
-    }
-    const double disc = b * b - 4 * a * c;
-    if (disc < 0) return 0;
-    const double q = -(b + std::copysign(std::sqrt(disc), b)) / 2;
-    t0 = q / a;
-    t1 = q != 0 ? c / q : t0;
-    if (t0 > t1) std::swap(t0, t1);
-    return 2;
-}
-
-// The sag quadric c(x² + y²) + c(1 + k) z² − 2z = 0.
-void sagCoeffs(double c, double ck, V3 o, V3 d, double& A, double& B, double& C) {
-    A = c * (d.x * d.x + d.y * d.y) + ck * d.z * d.z;
-    B = 2 * (c * (o.x * d.x + o.y * d.y) + ck * o.z * d.z - d.z);
-    C = c * (o.x * o.x + o.y * o.y) + ck * o.z * o.z - 2 * o.z;
-}
-double refineRoot(double c, double ck, V3 o, V3 d, double t) {
-    double A, B, C, u0, u1;
-    sagCoeffs(c, ck, o + d * t, d, A, B, C);
-    if (solveQuadratic(A, B, C, u0, u1) == 0) return t;
-    const double du = std::fabs(u0) < std::fabs(u1) ? u0 : u1;
-    return std::fabs(du) < kRefineD * (1 + std::fabs(t)) ? t + du : t;
-}
-bool mayLieIn(double t, double tmin, double tmax) {
-    const double m = kRefineD * (1 + std::fabs(t));
-    return !(t + m <= tmin || t - m >= tmax);
-}
-
-struct Surf {
-    double c, k, A[4], sd, z, zMin, zMax;
-    int nA;
+// A lens row: its sag (no central hole) and the vertex position along the axis.
+struct Surf : shapes::Sag {
+    double z = 0;
 };
 
-double sag(const Surf& s, double r) {
-    const double r2 = r * r;
-    double z = s.c * r2 / (1 + sqrt0(1 - (1 + s.k) * s.c * s.c * r2));
-    double rp = r2 * r2;
-    for (int i = 0; i < s.nA; ++i) {
-        z += s.A[i] * rp;
-        rp *= r2;
-    }
-    return z;
-}
-double dsag(const Surf& s, double r) {
-    const double root = sqrt0(1 - (1 + s.k) * s.c * s.c * (r * r));
-    double d = root > 0 ? s.c * r / root : (s.c != 0 ? kInf : 0.0);
-    double m = 4, rp = r * r * r;
-    for (int i = 0; i < s.nA; ++i) {
-        d += s.A[i] * m * rp;
-        rp *= r * r;
-        m += 2;
-    }
-    return d;
-}
-V3 sagNormal(const Surf& s, V3 p) {
-    const double r = std::sqrt(p.x * p.x + p.y * p.y);
-    if (r < kTinyD) return V3(0, 0, 1);
-    const double sl = dsag(s, r);
-    return normalize(V3(-sl * p.x / r, -sl * p.y / r, 1));
-}
-bool accept(const Surf& s, V3 p) { return p.x * p.x + p.y * p.y <= s.sd * s.sd; }
-bool onConicSheet(const Surf& s, double z) { return s.c == 0 || (1 + s.k) * s.c * z <= 1 + kRelTolD; }
-
 Surf makeSurf(const owe_lens_surface& r, double z) {
-    Surf s{};
-    s.c = r.curvature;
-    s.k = r.conic;
-    s.nA = 4;
-    for (int i = 0; i < 4; ++i) s.A[i] = r.asphere[i];
-    while (s.nA > 0 && s.A[s.nA - 1] == 0) --s.nA;
-    s.sd = r.semi_diameter;
+    Surf s;
+    static_cast<shapes::Sag&>(s) = shapes::makeSag(r.curvature, r.conic, r.asphere, r.semi_diameter, 0.0);
     s.z = z;
-    s.zMin = kInf;
-    s.zMax = -kInf;
-    for (int i = 0; i <= 512; ++i) {
-        const double v = sag(s, s.sd * i / 512);
-        s.zMin = std::min(s.zMin, v);
-        s.zMax = std::max(s.zMax, v);
-    }
-    const double pad = 1e-9 + 1e-3 * (s.zMax - s.zMin);
-    s.zMin -= pad;
-    s.zMax += pad;
     return s;
-}
-
-// SagSurface::intersect in the surface's frame (vertex at the origin), t in (tmin, tmax).
-bool intersectSag(const Surf& s, V3 o, V3 d, double tmin, double tmax, double& t, V3& p, V3& n) {
-    if (s.nA == 0) {
-        if (s.c == 0) {
-            if (d.z == 0) return false;
-            t = -o.z / d.z;
-            if (t <= tmin || t >= tmax) return false;
-            p = o + d * t;
-            if (!accept(s, p)) return false;
-            n = sagNormal(s, p);
-            return true;
-        }
-        const double ck = s.c * (1 + s.k);
-        double A, B, C, t0, t1;
-        sagCoeffs(s.c, ck, o, d, A, B, C);
-        if (solveQuadratic(A, B, C, t0, t1) == 0) return false;
-        for (double tc : {t0, t1}) {
-            if (!mayLieIn(tc, tmin, tmax)) continue;
-            const double tr = refineRoot(s.c, ck, o, d, tc);
-            if (!(tr > tmin && tr < tmax)) continue;
-            const V3 q = o + d * tr;
-            if (!accept(s, q) || !onConicSheet(s, q.z)) continue;
-            t = tr;
-            p = q;
-            n = sagNormal(s, q);
-            return true;
-        }
-        return false;
-    }
-    // Numeric: the ray's stretch inside the aperture cylinder and the sag's slab, 96 samples of
-    // f(t) = sag(r) − z, the first sign change bisected.
-    double ta = tmin, tb = tmax;
-    {
-        const double a = d.x * d.x + d.y * d.y;
-        if (a < kTinyD) {
-            if (!(o.x * o.x + o.y * o.y <= s.sd * s.sd && ta < tb)) return false;
-        } else {
-            double q0, q1;
-            if (solveQuadratic(a, 2 * (o.x * d.x + o.y * d.y), o.x * o.x + o.y * o.y - s.sd * s.sd, q0, q1) < 2) return false;
-            ta = std::max(ta, q0);
-            tb = std::min(tb, q1);
-            if (!(ta < tb)) return false;
-        }
-        if (std::fabs(d.z) < kTinyD) {
-            if (!(o.z >= s.zMin && o.z <= s.zMax && ta < tb)) return false;
-        } else {
-            const double s0 = (s.zMin - o.z) / d.z, s1 = (s.zMax - o.z) / d.z;
-            ta = std::max(ta, std::min(s0, s1));
-            tb = std::min(tb, std::max(s0, s1));
-            if (!(ta < tb)) return false;
-        }
-    }
-    auto f = [&](double tt) {
-        const V3 q = o + d * tt;
-        return sag(s, std::sqrt(q.x * q.x + q.y * q.y)) - q.z;
-    };
-    const int N = 96;
-    double tPrev = ta, fPrev = f(ta);
-    for (int i = 1; i <= N; ++i) {
-        const double tCur = ta + (tb - ta) * i / N, fCur = f(tCur);
-        if ((fPrev <= 0 && fCur >= 0) || (fPrev >= 0 && fCur <= 0)) {
-            if (fPrev == 0 && tPrev <= tmin) {
-                tPrev = tCur;
-                fPrev = fCur;
-                continue;
-            }
-            double lo = tPrev, hi = tCur, flo = fPrev;
-            for (int it = 0; it < kBisectStepsD && hi - lo > kBisectD * (1 + std::fabs(hi)); ++it) {
-                const double mid = 0.5 * (lo + hi), fm = f(mid);
-                if ((fm <= 0) == (flo <= 0)) {
-                    lo = mid;
-                    flo = fm;
-                } else {
-                    hi = mid;
-                }
-            }
-            const double tr = 0.5 * (lo + hi);
-            const V3 q = o + d * tr;
-            if (tr > tmin && tr < tmax && accept(s, q)) {
-                t = tr;
-                p = q;
-                n = sagNormal(s, q);
-                return true;
-            }
-        }
-        tPrev = tCur;
-        fPrev = fCur;
-    }
-    return false;
 }
 
 struct Lens {

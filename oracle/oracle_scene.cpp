@@ -195,6 +195,64 @@ std::vector<uint32_t> faceGroups(const Mesh& m) {
     return g;
 }
 
+// WaveSurface's constructor (geometry.cpp:188-225): amplitude and slope bounds, and with a node
+// spacing (given, or automatic beyond six waves) the sampled bicubic Hermite nodes and the slope
+// bound of the spline.
+shapes::Wave makeWave(const owe_boundary& ob, const owe_plane_wave* records) {
+    shapes::Wave w;
+    w.hx = ob.a;
+    w.hy = ob.b;
+    w.margin = std::max(0.0, std::min({ob.params[0], w.hx, w.hy}));
+    for (uint32_t i = 0; i < ob.wave_count; ++i) {
+        const owe_plane_wave& r = records[ob.first_wave + i];
+        w.waves.push_back({r.amplitude, r.kx, r.ky, r.phase});
+    }
+    double slope = 0, curvature = 0, kMax = 0;
+    for (const auto& p : w.waves) {
+        const double k = std::hypot(p.kx, p.ky);
+        w.amp += std::fabs(p.amplitude);
+        slope += std::fabs(p.amplitude) * k;
+        curvature += std::fabs(p.amplitude) * k * k;
+        kMax = std::max(kMax, k);
+    }
+    const double taper = w.margin > 0 ? 1.5 * std::sqrt(2.0) / w.margin : 0;
+    w.slope = slope + w.amp * taper;
+    double spacing = ob.params[1];
+    if (spacing < 0) spacing = w.waves.size() <= 6 || kMax == 0 ? 0 : (2 * kPi / kMax) / 12;
+    if (!(spacing > 0)) return w;
+    w.nx = std::clamp(int(std::ceil(2 * w.hx / spacing)) + 1, 2, 4096);
+    w.ny = std::clamp(int(std::ceil(2 * w.hy / spacing)) + 1, 2, 4096);
+    w.dx = 2 * w.hx / (w.nx - 1);
+    w.dy = 2 * w.hy / (w.ny - 1);
+    w.nodes.resize(4 * size_t(w.nx) * w.ny);
+    double nodeSlope = 0;
+    for (int j = 0; j < w.ny; ++j)
+        for (int i = 0; i < w.nx; ++i) {
+            double* n = &w.nodes[4 * (size_t(j) * w.nx + i)];
+            const double x = -w.hx + i * w.dx, y = -w.hy + j * w.dy;
+            double g = 0, gx = 0, gy = 0, gxy = 0;
+            for (const auto& p : w.waves) {
+                const double arg = p.kx * x + p.ky * y + p.phase, sn = std::sin(arg), cs = std::cos(arg);
+                g += p.amplitude * sn;
+                gx += p.amplitude * cs * p.kx;
+                gy += p.amplitude * cs * p.ky;
+                gxy -= p.amplitude * sn * p.kx * p.ky;
+            }
+            double wx, dwx, wy, dwy;
+            shapes::waveTaper(x, w.hx, w.margin, wx, dwx);
+            shapes::waveTaper(y, w.hy, w.margin, wy, dwy);
+            if (std::fabs(x) > w.hx || std::fabs(y) > w.hy) wx = wy = dwx = dwy = 0;
+            n[0] = wx * wy * g;
+            n[1] = dwx * wy * g + wx * wy * gx;
+            n[2] = wx * dwy * g + wx * wy * gy;
+            n[3] = dwx * dwy * g + dwx * wy * gy + wx * dwy * gx + wx * wy * gxy;
+            nodeSlope = std::max(nodeSlope, std::hypot(n[1], n[2]));
+        }
+    const double hessian = curvature + 2 * taper * slope + (w.margin > 0 ? 6 * w.amp / (w.margin * w.margin) : 0);
+    w.slope = std::min(w.slope, 1.05 * (nodeSlope + hessian * 0.5 * std::hypot(w.dx, w.dy)));
+    return w;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- world
@@ -278,6 +336,19 @@ World::World(const owe_scene& s) {
             const double pad = 1e-9 * (1 + std::max(ex, ey));
             local.grow(V3(-ex, -ey, -pad));
             local.grow(V3(ex, ey, pad));
+        } else if (b.shape == OWE_SHAPE_CYLINDER) {
+            b.z0 = std::min(ob.params[0], ob.params[1]);
+            b.z1 = std::max(ob.params[0], ob.params[1]);
+            local.grow(V3(-b.a, -b.a, b.z0));
+            local.grow(V3(b.a, b.a, b.z1));
+        } else if (b.shape == OWE_SHAPE_SAG) {
+            b.sag = shapes::makeSag(ob.params[0], ob.params[1], ob.params + 2, b.a, b.b);
+            local.grow(V3(-b.a, -b.a, b.sag.zMin));
+            local.grow(V3(b.a, b.a, b.sag.zMax));
+        } else if (b.shape == OWE_SHAPE_WAVE) {
+            b.wave = std::make_shared<shapes::Wave>(makeWave(ob, s.waves));
+            local.grow(V3(-b.a, -b.b, -b.wave->amp));
+            local.grow(V3(b.a, b.b, b.wave->amp));
         } else {
             local.grow(V3(-b.a, -b.a, -b.a));
             local.grow(V3(b.a, b.a, b.a));
@@ -385,6 +456,15 @@ size_t World::pickLight(double u) const {
     return std::min(i, lights.size() - 1);
 }
 
+// The root of a quadric other than the one a departing ray stands on (C ≈ 0 at its origin):
+// −B/A, polished from the approximate point.
+double otherRoot(const shapes::Quadric& q, V3 o, V3 d) {
+    double A, B, C;
+    shapes::quadricCoeffs(q, o, d, A, B, C);
+    if (std::fabs(A) < 1e-300) return -1;
+    return shapes::refineRoot(q, o, d, -B / A);
+}
+
 bool World::intersect(const Ray& ray, double tmax, Hit& hit) const {
     bool found = false;
     double best = tmax;
@@ -418,6 +498,32 @@ bool World::intersect(const Ray& ray, double tmax, Hit& hit) const {
             }
             pl = lo + ld * t;
             nl = normalize(pl / b.a);
+        } else if (b.shape == OWE_SHAPE_CYLINDER) {
+            if (departed) {
+                t = otherRoot(shapes::Quadric{1, b.a * b.a, 0}, lo, ld);
+                if (!(t > 0 && t < best)) continue;
+                pl = lo + ld * t;
+                if (!(pl.z >= b.z0 && pl.z <= b.z1)) continue;
+                nl = normalize(V3(pl.x, pl.y, 0));
+            } else if (!shapes::intersectCylinder(b.a, b.z0, b.z1, lo, ld, 0, best, t, pl, nl)) {
+                continue;
+            }
+        } else if (b.shape == OWE_SHAPE_SAG) {
+            const shapes::Sag& sg = b.sag;
+            if (departed && sg.nA == 0) {
+                if (sg.c == 0) continue;  // a plane: nothing beyond its own point
+                t = otherRoot(shapes::Quadric{0, sg.c, sg.c * (1 + sg.k)}, lo, ld);
+                if (!(t > 0 && t < best)) continue;
+                pl = lo + ld * t;
+                if (!shapes::sagAccept(sg, pl) || !shapes::onConicSheet(sg, pl.z)) continue;
+                nl = shapes::sagNormal(sg, pl);
+            } else {
+                // A numeric sag left by this ray: roots within 1e-9 of its scale are its own point.
+                const double tmin = departed ? 1e-9 * (sg.rMax + std::max(std::fabs(sg.zMin), std::fabs(sg.zMax))) : 0.0;
+                if (!shapes::intersectSag(sg, lo, ld, tmin, best, t, pl, nl)) continue;
+            }
+        } else if (b.shape == OWE_SHAPE_WAVE) {
+            if (!shapes::intersectWave(*b.wave, lo, ld, 0, best, departed, t, pl, nl)) continue;
         } else {
             const uint32_t skip = departed && ray.departedPrim < b.faceGroup.size() ? ray.departedPrim : ~0u;
             if (!b.mesh->closest(lo, ld, 0, best, t, prim, skip, nullptr, nullptr, skip != ~0u ? b.faceGroup.data() : nullptr))

@@ -22,6 +22,7 @@
 
 #include "../include/owe_hip_scene.h"
 #include "oracle_bvh.hpp"
+#include "oracle_shapes.hpp"
 
 namespace oracle {
 
@@ -142,6 +143,9 @@ struct Boundary {
     Box worldBox;
     std::unique_ptr<Mesh> mesh;
     std::vector<uint32_t> faceGroup;  // mesh: coplanar face id per triangle
+    double z0 = 0, z1 = 0;            // cylinder: axial extent
+    shapes::Sag sag;                  // sag surface
+    std::shared_ptr<shapes::Wave> wave;
     double area() const;
     bool canSample() const { return area() > 0; }
 };

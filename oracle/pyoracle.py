@@ -164,6 +164,7 @@ class Oracle(_Lib):
             "render_particles": (None, [_vp, _vp, _i, _i, _i, _vp, _vp]),
             "particles": (None, [_vp, _vp, _i, _u64, _vp, _vp, _vp, _i]),
             "emission_info": (None, [_vp, _vp]),
+            "world_intersect": (None, [_vp, _u64, _vp, _vp, _vp, _vp]),
             "lens_trace": (None, [_vp, _vp, _vp, _d, _d, _d, _vp, _u64, _vp]),
             "lens_moments": (None, [_vp, _vp, _vp, _d, _d, _d, _i, _vp, _vp]),
             "sag_intersect": (None, [_vp, _vp, _vp, _d, _d, _vp]),
@@ -243,6 +244,15 @@ class World:
         self.o.camera_paths(self.h, ctypes.byref(settings), spp, pass_index, n, _ptr(pixel), _ptr(sample), _ptr(xyz),
                             _ptr(segs), threads)
         return xyz, segs
+
+    def intersect(self, origins, directions, departed=None) -> np.ndarray:
+        """World::intersect per ray: [n, 8] = t, point, normal, boundary (inf, NaN, 2^32 − 2 on a miss)."""
+        o = np.ascontiguousarray(origins, dtype=np.float64).reshape(-1, 3)
+        d = np.ascontiguousarray(directions, dtype=np.float64).reshape(-1, 3)
+        dep = None if departed is None else np.ascontiguousarray(departed, dtype=np.uint32)
+        out = np.zeros((len(o), 8))
+        self.o.world_intersect(self.h, len(o), _ptr(o), _ptr(d), None if dep is None else _ptr(dep), _ptr(out))
+        return out
 
     def render_particles(self, settings, ppp: int, passes: int = 1, threads: int = 0, first_pass: int = 0):
         """Light tracing: resolved XYZ image [H, W, 3] (splats ÷ particles emitted) and stats."""
