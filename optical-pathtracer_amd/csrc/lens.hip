@@ -23,6 +23,7 @@
 
 #include "common.hpp"
 #include "owe_hip_lens.h"
+#include "shapes_dev.hpp"
 #include "transport_dev.hpp"
 
 using namespace owehip;
@@ -33,14 +34,9 @@ namespace {
 constexpr int kLensBlock = 256;
 constexpr double kHistScale = 4294967296.0;  // 2^32 units of weight per unit
 
-struct DSurf {
-    double c, ck;             // curvature, c (1 + k): the quadric, in double for the polish
-    float cf, kf, ckf, sd2;   // the same in float, and the squared semi-diameter
-    float A[4];               // even asphere A4..A10
-    int nA;                   // 0: quadric path
-    float z;                  // vertex position along the axis
-    float zMin, zMax;         // the numeric path's slab, relative to the vertex
-    float sd;
+// A lens row: its sag surface (shapes_dev.hpp) and the vertex position along the axis.
+struct DSurf : DSag {
+    float z;
 };
 
 struct DLensInfo {
@@ -65,197 +61,6 @@ struct LensArgs {
     double* moments;          // [lens × nλ × OWE_LENS_MOMENTS]
     unsigned long long* hist; // [lens × pixels²]
 };
-
-// ---------------------------------------------------------------- roots
-
-__device__ __forceinline__ int solveQuadraticF(float a, float b, float c, float& t0, float& t1) {
-    t0 = t1 = 0.0f;
-    if (fabsf(a) < 1e-30f || fabsf(a) < 1e-14f * fabsf(b)) {
-        if (b == 0.0f) return 0;
-        t0 = t1 = -c / b;
-        return 1;
-    }
-    const float disc = b * b - 4.0f * a * c;
-    if (disc < 0.0f) return 0;
-    const float q = -(b + copysignf(sqrtf(disc), b)) * 0.5f;
-    t0 = q / a;
-    t1 = q != 0.0f ? c / q : t0;
-    if (t0 > t1) {
-        const float s = t0;
-        t0 = t1;
-        t1 = s;
-    }
-    return 2;
-}
-__device__ __forceinline__ int solveQuadraticD(double a, double b, double c, double& t0, double& t1) {
-    t0 = t1 = 0.0;
-    if (fabs(a) < 1e-300 || fabs(a) < 1e-14 * fabs(b)) {
-        if (b == 0.0) return 0;
-        t0 = t1 = -c / b;
-        return 1;
-    }
-    const double disc = b * b - 4.0 * a * c;
-    if (disc < 0.0) return 0;
-    const double q = -(b + copysign(sqrt(disc), b)) * 0.5;
-    t0 = q / a;
-    t1 = q != 0.0 ? c / q : t0;
-    if (t0 > t1) {
-        const double s = t0;
-        t0 = t1;
-        t1 = s;
-    }
-    return 2;
-}
-
-// refineRoot in double: the quadric re-solved from the approximate hit point o + t d.
-__device__ __forceinline__ double refineSagD(const DSurf& s, f3 of, f3 df, double t) {
-    const double ox = double(of.x) + double(df.x) * t, oy = double(of.y) + double(df.y) * t,
-                 oz = double(of.z) + double(df.z) * t;
-    const double dx = df.x, dy = df.y, dz = df.z;
-    const double A = s.c * (dx * dx + dy * dy) + s.ck * dz * dz;
-    const double B = 2.0 * (s.c * (ox * dx + oy * dy) + s.ck * oz * dz - dz);
-    const double C = s.c * (ox * ox + oy * oy) + s.ck * oz * oz - 2.0 * oz;
-    double u0, u1;
-    if (solveQuadraticD(A, B, C, u0, u1) == 0) return t;
-    const double du = fabs(u0) < fabs(u1) ? u0 : u1;
-    return fabs(du) < 1e-3 * (1.0 + fabs(t)) ? t + du : t;
-}
-
-__device__ __forceinline__ float sagF(const DSurf& s, float r) {
-    const float r2 = r * r;
-    float z = s.cf * r2 / (1.0f + sqrt0f(1.0f - (1.0f + s.kf) * s.cf * s.cf * r2));
-    float rp = r2 * r2;
-    for (int i = 0; i < s.nA; ++i) {
-        z += s.A[i] * rp;
-        rp *= r2;
-    }
-    return z;
-}
-__device__ __forceinline__ double sagD(const DSurf& s, double r) {
-    const double r2 = r * r, k = double(s.kf);
-    double z = s.c * r2 / (1.0 + sqrt(fmax(0.0, 1.0 - (1.0 + k) * s.c * s.c * r2)));
-    double rp = r2 * r2;
-    for (int i = 0; i < s.nA; ++i) {
-        z += double(s.A[i]) * rp;
-        rp *= r2;
-    }
-    return z;
-}
-__device__ __forceinline__ double dsagD(const DSurf& s, double r) {
-    const double k = double(s.kf), root = sqrt(fmax(0.0, 1.0 - (1.0 + k) * s.c * s.c * r * r));
-    double d = root > 0.0 ? s.c * r / root : 0.0;
-    double m = 4.0, rp = r * r * r;
-    for (int i = 0; i < s.nA; ++i) {
-        d += double(s.A[i]) * m * rp;
-        rp *= r * r;
-        m += 2.0;
-    }
-    return d;
-}
-__device__ __forceinline__ f3 sagNormalF(const DSurf& s, f3 p) {
-    const float r = sqrtf(p.x * p.x + p.y * p.y);
-    if (r < 1e-30f) return mk3(0.0f, 0.0f, 1.0f);
-    const float sl = float(dsagD(s, double(r)));
-    return norm3(mk3(-sl * p.x / r, -sl * p.y / r, 1.0f));
-}
-
-// SagSurface::intersect in the surface's frame for t > 0.
-__device__ __forceinline__ bool sagHit(const DSurf& s, f3 o, f3 d, f3& p, f3& n) {
-    if (s.nA == 0) {
-        if (s.cf == 0.0f) {
-            if (d.z == 0.0f) return false;
-            const float t = -o.z / d.z;
-            if (!(t > 0.0f)) return false;
-            p = o + d * t;
-            p.z = 0.0f;
-            if (p.x * p.x + p.y * p.y > s.sd2) return false;
-            n = mk3(0.0f, 0.0f, 1.0f);
-            return true;
-        }
-        const float A = s.cf * (d.x * d.x + d.y * d.y) + s.ckf * d.z * d.z;
-        const float B = 2.0f * (s.cf * (o.x * d.x + o.y * d.y) + s.ckf * o.z * d.z - d.z);
-        const float C = s.cf * (o.x * o.x + o.y * o.y) + s.ckf * o.z * o.z - 2.0f * o.z;
-        float t0, t1;
-        if (solveQuadraticF(A, B, C, t0, t1) == 0) return false;
-        for (int k = 0; k < 2; ++k) {
-            const float tc = k == 0 ? t0 : t1;
-            if (tc + 1e-3f * (1.0f + fabsf(tc)) <= 0.0f) continue;  // cannot be refined into (0, ∞)
-            const double tr = refineSagD(s, o, d, double(tc));
-            if (!(tr > 0.0)) continue;
-            const f3 q = mk3(float(double(o.x) + double(d.x) * tr), float(double(o.y) + double(d.y) * tr),
-                             float(double(o.z) + double(d.z) * tr));
-            if (q.x * q.x + q.y * q.y > s.sd2) continue;
-            if (!((1.0f + s.kf) * s.cf * q.z <= 1.0f + 1e-6f)) continue;  // the conic's other sheet
-            p = q;
-            n = sagNormalF(s, q);
-            return true;
-        }
-        return false;
-    }
-    // Numeric asphere: clip to the aperture cylinder and the sag's slab, 96 samples, bisection in
-    // float, then Newton steps on f(t) = sag(r) − z in double.
-    float ta = 0.0f, tb = __builtin_inff();
-    const float a = d.x * d.x + d.y * d.y;
-    if (a < 1e-30f) {
-        if (o.x * o.x + o.y * o.y > s.sd2) return false;
-    } else {
-        float q0, q1;
-        if (solveQuadraticF(a, 2.0f * (o.x * d.x + o.y * d.y), o.x * o.x + o.y * o.y - s.sd2, q0, q1) < 2) return false;
-        ta = fmaxf(ta, q0);
-        tb = fminf(tb, q1);
-    }
-    if (fabsf(d.z) < 1e-30f) {
-        if (o.z < s.zMin || o.z > s.zMax) return false;
-    } else {
-        const float s0 = (s.zMin - o.z) / d.z, s1 = (s.zMax - o.z) / d.z;
-        ta = fmaxf(ta, fminf(s0, s1));
-        tb = fminf(tb, fmaxf(s0, s1));
-    }
-    if (!(ta < tb)) return false;
-    auto f = [&](float tt) {
-        const f3 q = o + d * tt;
-        return sagF(s, sqrtf(q.x * q.x + q.y * q.y)) - q.z;
-    };
-    float tPrev = ta, fPrev = f(ta);
-    for (int i = 1; i <= 96; ++i) {
-        const float tCur = ta + (tb - ta) * float(i) / 96.0f, fCur = f(tCur);
-        if ((fPrev <= 0.0f && fCur >= 0.0f) || (fPrev >= 0.0f && fCur <= 0.0f)) {
-            if (!(fPrev == 0.0f && tPrev <= 0.0f)) {
-                float lo = tPrev, hi = tCur, flo = fPrev;
-                for (int it = 0; it < 40 && hi - lo > 2e-7f * (1.0f + fabsf(hi)); ++it) {
-                    const float mid = 0.5f * (lo + hi), fm = f(mid);
-                    if ((fm <= 0.0f) == (flo <= 0.0f)) {
-                        lo = mid;
-                        flo = fm;
-                    } else {
-                        hi = mid;
-                    }
-                }
-                double t = 0.5 * (double(lo) + double(hi));
-                for (int it = 0; it < 2; ++it) {
-                    const double x = double(o.x) + double(d.x) * t, y = double(o.y) + double(d.y) * t,
-                                 z = double(o.z) + double(d.z) * t, r = sqrt(x * x + y * y);
-                    const double fv = sagD(s, r) - z;
-                    const double fd = (r > 0.0 ? dsagD(s, r) * (x * d.x + y * d.y) / r : 0.0) - double(d.z);
-                    if (fd == 0.0) break;
-                    const double step = fv / fd;
-                    if (!(fabs(step) <= double(hi - lo) + 1e-12)) break;
-                    t -= step;
-                }
-                const f3 q = mk3(float(double(o.x) + double(d.x) * t), float(double(o.y) + double(d.y) * t),
-                                 float(double(o.z) + double(d.z) * t));
-                if (t > 0.0 && q.x * q.x + q.y * q.y <= s.sd2) {
-                    p = q;
-                    n = sagNormalF(s, q);
-                    return true;
-                }
-            }
-        }
-        tPrev = tCur;
-        fPrev = fCur;
-    }
-    return false;
-}
 
 // Pupil point i of n on the entrance pupil of radius rEP, in double: a Fibonacci lattice over the
 // disc of radius 0.999 rEP, or with rings > 0 spotAt's hexapolar pattern (analysis.cpp:124-131) —
@@ -298,7 +103,9 @@ __device__ __forceinline__ bool traceLensRay(const LensArgs& a, const DLensInfo&
     for (uint32_t k = 0; k < L.count; ++k) {
         const DSurf& sf = a.surf[L.first + k];
         f3 q, n;
-        if (!sagHit(sf, mk3(o.x, o.y, o.z - sf.z), d, q, n)) return false;
+        float th;
+        if (!sagHit(sf, mk3(o.x, o.y, o.z - sf.z), d, 0.0f, __builtin_inff(), false, th, q)) return false;
+        n = sagNormalF(sf, q);
         const float nNext = a.nAfter[(L.first + k) * a.nLambda + w];
         const float R = fresnelDielectric(fabsf(dot3(d, n)), nNext / nPrev);
         f3 wv;
@@ -493,17 +300,6 @@ ParaxialOut paraxialHost(const owe_lens_surface* s, size_t N, const std::vector<
     return r;
 }
 
-double sagHost(const owe_lens_surface& s, double r) {
-    const double r2 = r * r;
-    double z = s.curvature * r2 / (1 + std::sqrt(std::max(0.0, 1 - (1 + s.conic) * s.curvature * s.curvature * r2)));
-    double rp = r2 * r2;
-    for (double A : s.asphere) {
-        z += A * rp;
-        rp *= r2;
-    }
-    return z;
-}
-
 }  // namespace
 }  // namespace owehip
 
@@ -662,26 +458,8 @@ int owe_hip_lens_batch_create(int device, const owe_medium* media, uint32_t n_me
             for (uint32_t k = 0; k < L.surface_count; ++k) {
                 const owe_lens_surface& r = surfaces[L.first_surface + k];
                 DSurf d{};
-                d.c = r.curvature;
-                d.ck = r.curvature * (1 + r.conic);
-                d.cf = float(r.curvature);
-                d.kf = float(r.conic);
-                d.ckf = float(d.ck);
-                d.sd = float(r.semi_diameter);
-                d.sd2 = float(r.semi_diameter * r.semi_diameter);
-                d.nA = 4;
-                for (int q = 0; q < 4; ++q) d.A[q] = float(r.asphere[q]);
-                while (d.nA > 0 && r.asphere[d.nA - 1] == 0) --d.nA;
+                static_cast<DSag&>(d) = makeSag(r.curvature, r.conic, r.asphere, r.semi_diameter, 0.0);
                 d.z = float(z);
-                double zmin = INFINITY, zmax = -INFINITY;
-                for (int q = 0; q <= 512; ++q) {
-                    const double v = sagHost(r, r.semi_diameter * q / 512);
-                    zmin = std::min(zmin, v);
-                    zmax = std::max(zmax, v);
-                }
-                const double pad = 1e-9 + 1e-3 * (zmax - zmin);
-                d.zMin = float(zmin - pad);
-                d.zMax = float(zmax + pad);
                 ds[L.first_surface + k] = d;
                 z += r.thickness;
             }

@@ -424,13 +424,7 @@ __device__ __forceinline__ bool shadeOne(ShadeCtx& c, const uint4 hraw, uint32_t
         p = c.o + c.d * t;
     } else {
         const f3 lo = xfPoint(b.toLocal, c.o), ldv = xfVector(b.toLocal, c.d);
-        pl = lo + ldv * t;
-        if (b.shape == OWE_SHAPE_PLANE) {
-            pl.z = 0.0f;
-            nl = mk3(0, 0, 1);
-        } else {
-            nl = norm3(pl / b.a);
-        }
+        analyticGeometry(sc, b, lo, ldv, t, pl, nl);
         p = xfPoint(b.toWorld, pl);
         n = norm3(xfVector(b.toWorld, nl));
     }
@@ -1196,6 +1190,7 @@ int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render
         d.boundaries = upload(r, h.boundaries);
         d.lights = upload(r, h.lights);
         d.table = upload(r, h.table);
+        d.waveData = reinterpret_cast<const float4*>(upload(r, h.waveData));
         if (h.hasTriangles) {
             d.nodes = upload(r, h.bvh.nodes);
             d.tris = upload(r, h.bvh.tris);

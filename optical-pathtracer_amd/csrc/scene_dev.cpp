@@ -130,6 +130,131 @@ float upf(double v) {
 
 }  // namespace
 
+namespace {
+
+// WaveSurface's construction (src/owe/scene/geometry.cpp:188-225): amplitude and slope bounds; with
+// a node spacing (given, or automatic beyond six waves: a twelfth of the shortest wavelength) the
+// bicubic Hermite nodes (h, ∂x, ∂y, ∂xy) of the tapered sum and the spline's slope bound. Records
+// are appended to `data` as float quadruples — waves (amplitude, kx, ky, phase), or nodes with
+// their derivatives pre-multiplied by the spacings.
+DWave buildWave(const owe_boundary& b, const owe_plane_wave* waves, std::vector<float>& data) {
+    DWave w{};
+    const double hx = b.a, hy = b.b, margin = std::max(0.0, std::min({b.params[0], hx, hy}));
+    double amp = 0, slope = 0, curvature = 0, kMax = 0;
+    for (uint32_t i = 0; i < b.wave_count; ++i) {
+        const owe_plane_wave& p = waves[b.first_wave + i];
+        const double k = std::hypot(p.kx, p.ky);
+        amp += std::fabs(p.amplitude);
+        slope += std::fabs(p.amplitude) * k;
+        curvature += std::fabs(p.amplitude) * k * k;
+        kMax = std::max(kMax, k);
+    }
+    const double taper = margin > 0 ? 1.5 * std::sqrt(2.0) / margin : 0;
+    double slopeBound = slope + amp * taper;
+    double spacing = b.params[1];
+    if (spacing < 0) spacing = b.wave_count <= 6 || kMax == 0 ? 0 : (2 * kPi / kMax) / 12;
+    w.hx = float(hx);
+    w.hy = float(hy);
+    w.margin = float(margin);
+    w.amp = float(amp);
+    require(data.size() / 4 < 0xFFFFFFFFull, "wave data too large");
+    w.data = uint32_t(data.size() / 4);
+    if (!(spacing > 0)) {
+        w.nx = 0;
+        w.ny = b.wave_count;
+        for (uint32_t i = 0; i < b.wave_count; ++i) {
+            const owe_plane_wave& p = waves[b.first_wave + i];
+            data.insert(data.end(), {float(p.amplitude), float(p.kx), float(p.ky), float(p.phase)});
+        }
+        w.slope = float(slopeBound);
+        return w;
+    }
+    const int nx = std::clamp(int(std::ceil(2 * hx / spacing)) + 1, 2, 4096);
+    const int ny = std::clamp(int(std::ceil(2 * hy / spacing)) + 1, 2, 4096);
+    const double dx = 2 * hx / (nx - 1), dy = 2 * hy / (ny - 1);
+    auto taperOf = [&](double u, double half, double& wv, double& dw) {
+        wv = 1;
+        dw = 0;
+        if (!(margin > 0)) return;
+        const double e = half - std::fabs(u);
+        if (e >= margin) return;
+        if (e <= 0) {
+            wv = 0;
+            return;
+        }
+        const double t = e / margin;
+        wv = t * t * (3 - 2 * t);
+        dw = -(u < 0 ? -1.0 : 1.0) * 6 * t * (1 - t) / margin;
+    };
+    double nodeSlope = 0;
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            const double x = -hx + i * dx, y = -hy + j * dy;
+            double g = 0, gx = 0, gy = 0, gxy = 0;
+            for (uint32_t q = 0; q < b.wave_count; ++q) {
+                const owe_plane_wave& p = waves[b.first_wave + q];
+                const double arg = p.kx * x + p.ky * y + p.phase, sn = std::sin(arg), cs = std::cos(arg);
+                g += p.amplitude * sn;
+                gx += p.amplitude * cs * p.kx;
+                gy += p.amplitude * cs * p.ky;
+                gxy -= p.amplitude * sn * p.kx * p.ky;
+            }
+            double wx, dwx, wy, dwy;
+            taperOf(x, hx, wx, dwx);
+            taperOf(y, hy, wy, dwy);
+            if (std::fabs(x) > hx || std::fabs(y) > hy) wx = wy = dwx = dwy = 0;
+            const double h = wx * wy * g, hX = dwx * wy * g + wx * wy * gx, hY = wx * dwy * g + wx * wy * gy;
+            const double hXY = dwx * dwy * g + dwx * wy * gy + wx * dwy * gx + wx * wy * gxy;
+            nodeSlope = std::max(nodeSlope, std::hypot(hX, hY));
+            data.insert(data.end(), {float(h), float(hX * dx), float(hY * dy), float(hXY * dx * dy)});
+        }
+    const double hessian = curvature + 2 * taper * slope + (margin > 0 ? 6 * amp / (margin * margin) : 0);
+    slopeBound = std::min(slopeBound, 1.05 * (nodeSlope + hessian * 0.5 * std::hypot(dx, dy)));
+    w.nx = uint32_t(nx);
+    w.ny = uint32_t(ny);
+    w.dx = float(dx);
+    w.dy = float(dy);
+    w.slope = float(slopeBound);
+    return w;
+}
+
+}  // namespace
+
+DSag makeSag(double c, double k, const double asphere[4], double rMax, double rMin) {
+    DSag d{};
+    d.c = c;
+    d.ck = c * (1 + k);
+    d.cf = float(c);
+    d.kf = float(k);
+    d.ckf = float(d.ck);
+    d.rMax = float(rMax);
+    d.rMax2 = float(rMax * rMax);
+    d.rMin2 = float(rMin * rMin);
+    d.nA = 4;
+    for (int q = 0; q < 4; ++q) d.A[q] = float(asphere[q]);
+    while (d.nA > 0 && asphere[d.nA - 1] == 0) --d.nA;
+    auto sag = [&](double r) {
+        const double r2 = r * r;
+        double z = c * r2 / (1 + std::sqrt(std::max(0.0, 1 - (1 + k) * c * c * r2)));
+        double rp = r2 * r2;
+        for (int q = 0; q < 4; ++q) {
+            z += asphere[q] * rp;
+            rp *= r2;
+        }
+        return z;
+    };
+    double zmin = INFINITY, zmax = -INFINITY;
+    for (int q = 0; q <= 512; ++q) {
+        const double v = sag(rMin + (rMax - rMin) * q / 512);
+        zmin = std::min(zmin, v);
+        zmax = std::max(zmax, v);
+    }
+    const double pad = 1e-9 + 1e-3 * (zmax - zmin);
+    d.zMin = float(zmin - pad);
+    d.zMax = float(zmax + pad);
+    return d;
+}
+
 HostScene flattenScene(const owe_scene& s) {
     HostScene h;
     require(s.n_regions >= 1, "a scene needs at least the ambient region 0");
@@ -264,6 +389,34 @@ HostScene flattenScene(const owe_scene& s) {
             require(b.a > 0, name + ": sphere radius must be positive");
             for (int c = 0; c < 8; ++c) grow(xf.point({(c & 1) ? b.a : -b.a, (c & 2) ? b.a : -b.a, (c & 4) ? b.a : -b.a}));
             area = 4 * kPi * b.a * b.a;
+        } else if (b.shape == OWE_SHAPE_CYLINDER) {
+            const double z0 = std::min(b.params[0], b.params[1]), z1 = std::max(b.params[0], b.params[1]);
+            require(b.a > 0 && z1 > z0, name + ": a cylinder needs a positive radius and z0 != z1");
+            require(b.emission < 0, name + ": cylinders do not emit on this backend");
+            for (int c = 0; c < 8; ++c) grow(xf.point({(c & 1) ? b.a : -b.a, (c & 2) ? b.a : -b.a, (c & 4) ? z1 : z0}));
+            d.z0 = float(z0);
+            d.z1 = float(z1);
+            area = 2 * kPi * b.a * (z1 - z0);
+        } else if (b.shape == OWE_SHAPE_SAG) {
+            const double c = b.params[0], k = b.params[1];
+            require(b.a > 0 && b.b >= 0 && b.b < b.a, name + ": a sag surface needs 0 <= r_min < r_max");
+            require(1 - (1 + k) * c * c * b.a * b.a >= 0, name + ": aperture exceeds the domain of the conic");
+            require(b.emission < 0, name + ": sag surfaces do not emit on this backend");
+            d.sag = makeSag(c, k, b.params + 2, b.a, b.b);
+            for (int q = 0; q < 8; ++q)
+                grow(xf.point({(q & 1) ? b.a : -b.a, (q & 2) ? b.a : -b.a, double((q & 4) ? d.sag.zMax : d.sag.zMin)}));
+            d.depTol = float(1e-5 * (b.a + std::max(std::fabs(double(d.sag.zMin)), std::fabs(double(d.sag.zMax)))));
+            area = kPi * (b.a * b.a - b.b * b.b);
+        } else if (b.shape == OWE_SHAPE_WAVE) {
+            require(b.a > 0 && b.b > 0, name + ": a wave surface needs a positive extent");
+            require(b.wave_count >= 1 && s.waves && uint64_t(b.first_wave) + b.wave_count <= s.n_waves,
+                    name + ": waves out of range");
+            require(b.emission < 0, name + ": wave surfaces do not emit on this backend");
+            d.wave = buildWave(b, s.waves, h.waveData);
+            const double amp = d.wave.amp;
+            for (int q = 0; q < 8; ++q) grow(xf.point({(q & 1) ? b.a : -b.a, (q & 2) ? b.b : -b.b, (q & 4) ? amp : -amp}));
+            d.depTol = float(1e-5 * (std::max(b.a, b.b) + amp));
+            area = 4 * b.a * b.b;
         } else if (b.shape == OWE_SHAPE_MESH) {
             require(uint64_t(b.first_triangle) + b.triangle_count <= s.n_triangles, name + ": triangles out of range");
             // Face groups in the mesh's own frame (double, exact coplanarity).

@@ -44,6 +44,26 @@ struct DEmission {
     int32_t front, back, pad[2];
 };
 
+// Sag surface (SagSurface) and liquid surface (WaveSurface) parameters; the device code is in
+// shapes_dev.hpp.
+struct DSag {
+    double c, ck;                 // curvature, c (1 + k): the quadric, in double for the polish
+    float cf, kf, ckf;            // the same in float
+    float rMax2, rMin2, rMax;     // the annulus
+    float zMin, zMax;             // the numeric path's slab
+    float A[4];                   // even asphere A4..A10
+    int32_t nA;                   // 0: quadric path
+    int32_t pad;
+};
+
+struct DWave {
+    float hx, hy, margin;
+    float amp, slope;             // |h| ≤ amp, |∇h| ≤ slope
+    float dx, dy;                 // sampled node spacing
+    uint32_t data;                // first record in the scene's wave data
+    uint32_t nx, ny;              // nx = 0: analytic, ny waves; else nx × ny nodes
+};
+
 struct DBoundary {
     int32_t shape, aperture;
     uint32_t front, back;
@@ -54,6 +74,11 @@ struct DBoundary {
     float lo[3], area;
     float hi[3];
     int32_t light;  // index into the light table, -1 when next-event estimation does not sample it
+    float z0, z1;   // cylinder: axial extent
+    float depTol;   // numeric sag, wave: a departing ray's own root lies within this of its origin
+    int32_t pad;
+    DSag sag;
+    DWave wave;
 };
 
 struct DLight {
@@ -101,6 +126,7 @@ struct DScene {
     const DBoundary* boundaries;
     const DLight* lights;
     const float* table;
+    const float4* waveData;  // wave records (OWE_SHAPE_WAVE)
     const WideNode* nodes;  // null when the scene has no triangles
     const TriRecord* tris;
     uint32_t nBoundaries, nLights, nRegions, stackCap;
@@ -117,6 +143,7 @@ struct HostScene {
     std::vector<DBoundary> boundaries;
     std::vector<DLight> lights;
     std::vector<float> table;
+    std::vector<float> waveData;  // 4 floats per record
     Bvh4 bvh;
     bool hasTriangles = false;
     DEnv env{};
@@ -127,5 +154,10 @@ struct HostScene {
 
 // Validates and flattens an owe_scene (throws owehip::Failure on invalid input).
 HostScene flattenScene(const owe_scene& s);
+
+// SagSurface's construction (src/owe/scene/geometry.cpp:69-89): trailing zero asphere terms
+// dropped, the slab [zMin, zMax] from 513 samples of the annulus padded by 1e-9 + 1e-3 of its
+// height.
+DSag makeSag(double curvature, double conic, const double asphere[4], double rMax, double rMin);
 
 }  // namespace owehip

@@ -17,6 +17,7 @@
 
 #include "dmath.hpp"
 #include "scene_dev.hpp"
+#include "shapes_dev.hpp"
 #include "traverse.hpp"
 
 namespace owehip {
@@ -416,6 +417,24 @@ __device__ __forceinline__ bool sphereRoots(f3 o, f3 d, float R, float& t0, floa
 // The nearest crossing in (0, tmax). departed/departedFace: the spawning boundary and (mesh) face;
 // exclude: an analytic boundary the segment ends on (a sampled light point). kGeometry = false
 // leaves the point and normals unset (a caller that keeps only t, boundary, face and slot).
+// The local point and normal of an analytic boundary met at distance t along the local ray.
+__device__ __forceinline__ void analyticGeometry(const DScene& sc, const DBoundary& b, f3 lo, f3 ldv, float t, f3& pl,
+                                                 f3& nl) {
+    pl = lo + ldv * t;
+    if (b.shape == OWE_SHAPE_PLANE) {
+        pl.z = 0.0f;
+        nl = mk3(0, 0, 1);
+    } else if (b.shape == OWE_SHAPE_SPHERE) {
+        nl = norm3(pl / b.a);
+    } else if (b.shape == OWE_SHAPE_CYLINDER) {
+        nl = norm3(mk3(pl.x, pl.y, 0.0f));
+    } else if (b.shape == OWE_SHAPE_SAG) {
+        nl = sagNormalF(b.sag, pl);
+    } else {
+        nl = waveNormalF(b.wave, sc.waveData, pl.x, pl.y);
+    }
+}
+
 template <bool kGeometry = true>
 __device__ __forceinline__ bool intersectWorld(const DScene& sc, f3 o, f3 d, float tmax, uint32_t departed, uint32_t departedFace,
                                       uint32_t exclude, uint32_t* stack, uint32_t stride, Hit& h) {
@@ -449,14 +468,24 @@ __device__ __forceinline__ bool intersectWorld(const DScene& sc, f3 o, f3 d, flo
             f3 pl = lo + ldv * t;
             pl.z = 0.0f;
             if (!inAperture(b.aperture, b.a, b.b, pl)) continue;
-        } else if (i == departed) {
-            t = -2.0f * dot3(lo, ldv) / dot3(ldv, ldv);  // the root other than the origin's own
-            if (!(t > 0.0f && t < h.t)) continue;
+        } else if (b.shape == OWE_SHAPE_SPHERE) {
+            if (i == departed) {
+                t = -2.0f * dot3(lo, ldv) / dot3(ldv, ldv);  // the root other than the origin's own
+                if (!(t > 0.0f && t < h.t)) continue;
+            } else {
+                float t0, t1;
+                if (!sphereRoots(lo, ldv, b.a, t0, t1)) continue;
+                t = t0 > 0.0f ? t0 : t1;
+                if (!(t > 0.0f && t < h.t)) continue;
+            }
+        } else if (b.shape == OWE_SHAPE_CYLINDER) {
+            if (!cylinderHit(b.a, b.z0, b.z1, lo, ldv, h.t, i == departed, t)) continue;
+        } else if (b.shape == OWE_SHAPE_SAG) {
+            f3 q;
+            const float tmin = i == departed && b.sag.nA != 0 ? b.depTol : 0.0f;
+            if (!sagHit(b.sag, lo, ldv, tmin, h.t, i == departed, t, q)) continue;
         } else {
-            float t0, t1;
-            if (!sphereRoots(lo, ldv, b.a, t0, t1)) continue;
-            t = t0 > 0.0f ? t0 : t1;
-            if (!(t > 0.0f && t < h.t)) continue;
+            if (!waveHit(b.wave, sc.waveData, lo, ldv, h.t, i == departed, b.depTol, t)) continue;
         }
         h.t = t;
         analytic = i;
@@ -468,9 +497,8 @@ __device__ __forceinline__ bool intersectWorld(const DScene& sc, f3 o, f3 d, flo
         if (!kGeometry) return true;
         const DBoundary& b = sc.boundaries[analytic];
         const f3 lo = xfPoint(b.toLocal, o), ldv = xfVector(b.toLocal, d);
-        f3 pl = lo + ldv * h.t;
-        if (b.shape == OWE_SHAPE_PLANE) pl.z = 0.0f;
-        const f3 nl = b.shape == OWE_SHAPE_PLANE ? mk3(0, 0, 1) : norm3(pl / b.a);
+        f3 pl, nl;
+        analyticGeometry(sc, b, lo, ldv, h.t, pl, nl);
         h.pLocal = pl;
         h.nLocal = nl;
         h.p = xfPoint(b.toWorld, pl);

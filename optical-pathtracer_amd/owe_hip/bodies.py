@@ -378,3 +378,36 @@ def ripple_analytic(a: float = 0.0005, wavelength: float = 0.02, depth: float = 
     col = ((u[keep] + 0.02) / 0.04 * bins).astype(int)
     hist = np.bincount(np.clip(col, 0, bins - 1), weights=w, minlength=bins)
     return hist / ((0.04 / bins) * 0.02)
+
+
+def uniform_sky_optics_scene(kind: str = "lens", size: int = 32) -> S.Scene:
+    """Lossless dielectric optics under a uniform sky of unit radiance, seen by a camera: a thick
+    biconvex lens with polished (dielectric) rims and an annular step, a rippled water pool
+    (n = 1.333, no absorption) with a sampled or analytic wave surface, or a concave dielectric
+    meniscus. Fresnel reflection and refraction only redistribute a uniform radiance field, so every
+    pixel sees radiance 1 (the invisible-sphere argument of tests/test_transport.cpp:116-142)."""
+    sc = S.Scene()
+    air = sc.add_region(sc.add_medium(S.Medium.air()))
+    sc.set_uniform_sky(S.Spectrum.constant(1.0))
+    diel = sc.add_optics(S.Optics(S.SURF_DIELECTRIC))
+    if kind == "lens":
+        n = relative_index(S.N_BK7, 587.56)
+        R1, R2 = design_simple_lens(0.05, n, 0.012)
+        add_lens(sc, (R1, R2), (0.015, 0.012), (0.012,), [glass_medium()], placement((0, 0, -0.006)), air, diel,
+                 dielectric=diel)
+        sc.set_observer(size, size, (0.02, 0.01, 0.12), (0, 0, 0), up=(0, 1, 0), fov_y=math.radians(20), region=air)
+    elif kind in ("waves", "waves-sampled"):
+        waves = [(0.002, 2 * math.pi / 0.03, 0.0, 0.0), (0.001, 60.0, 120.0, 0.7)]
+        add_water(sc, waves, 0.05, 0.05, 0.01, 0.03, S.Medium.constant(1.333), placement(), air,
+                  spacing=0.001 if kind == "waves-sampled" else 0.0)
+        sc.set_observer(size, size, (0.03, -0.05, 0.12), (0, 0, -0.01), up=(0, 0, 1), fov_y=math.radians(45),
+                        region=air)
+    else:
+        # An aspheric meniscus: a concave conic face over a convex one, with a cylindrical rim.
+        glass = sc.add_region(sc.add_medium(S.Medium.constant(1.6)))
+        sc.add_sag(1 / 0.05, 0.0, (), 0.02, 0.0, S.transform(), glass, air, diel)
+        sc.add_sag(1 / 0.08, -0.5, (0.0, 2.0e3), 0.02, 0.0, S.transform(None, (0, 0, 0.004)), air, glass, diel)
+        z0, z1 = sag_of(1 / 0.05, 0.0, 0.02), 0.004 + sag_of(1 / 0.08, -0.5, 0.02) + 2.0e3 * 0.02 ** 6
+        sc.add_cylinder(0.02, z0, z1, S.transform(), air, glass, diel)
+        sc.set_observer(size, size, (0.0, 0.03, 0.1), (0, 0, 0), up=(0, 1, 0), fov_y=math.radians(30), region=air)
+    return sc

@@ -194,3 +194,13 @@ def test_ripple_caustics_follow_the_ray_map_jacobian(oracle):
     da = abs(int(A.argmax()) - int(R.argmax()))
     assert da <= 1 or abs(da - 80) <= 1
     assert A.max() > 3 * E0
+
+
+@pytest.mark.parametrize("kind", ["lens", "waves", "waves-sampled", "meniscus"])
+def test_uniform_sky_through_lossless_optics_stays_uniform(oracle, kind):
+    img, stats = World(oracle, B.uniform_sky_optics_scene(kind, 16)).render(S.settings(seed=3, max_depth=64, rr_depth=8),
+                                                                             256)
+    y = img[..., 1]
+    print(f"{kind}: mean {y.mean():.4f}, range [{y.min():.3f}, {y.max():.3f}], stats {stats}")
+    assert y.mean() == pytest.approx(1.0, abs=0.01)
+    assert int(stats[2]) == 0 and int(stats[3]) == 0
