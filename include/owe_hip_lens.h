@@ -27,9 +27,17 @@
 extern "C" {
 #endif
 
-/* One refracting surface (PrescriptionSurface): vertex curvature 1/R (0: flat), conic constant,
- * even asphere A4..A10 (SI units), distance to the next vertex, semi-diameter, the medium after the
- * surface (an index into the batch's media) and whether it is the aperture stop. */
+/* One surface (PrescriptionSurface, prescription.hpp:16-34): vertex curvature 1/R (0: flat),
+ * conic constant, even asphere A4..A10 (SI units), the signed distance to the next vertex along the
+ * fixed chart (negative on a leg a mirror has turned back), semi-diameter, the medium after the
+ * surface (an index into the batch's media), whether it is the aperture stop, and for a reflecting
+ * row (a mirror: the light reflects into the same medium) the radius of its central hole.
+ *
+ * Mirrors also obstruct: a ray crossing the vertex plane of a mirror other than the two surfaces it
+ * travels between is stopped there if it lies on the mirror's body (hole_radius ≤ r ≤
+ * semi_diameter) — the bodies buildPrescription places (prescription.cpp:365-388), approximated by
+ * their vertex-plane annulus. Spider vanes lie behind a secondary, beyond every launch plane, and
+ * are not modelled. */
 typedef struct owe_lens_surface {
     double curvature;
     double conic;
@@ -38,6 +46,9 @@ typedef struct owe_lens_surface {
     double semi_diameter;
     int32_t medium;
     int32_t stop;
+    int32_t reflect;
+    int32_t reserved;
+    double hole_radius;
 } owe_lens_surface;
 
 /* A lens: surfaces [first_surface, first_surface + surface_count) of the batch, the ambient medium
@@ -64,21 +75,43 @@ typedef struct owe_lens_batch_settings {
     double reference_wavelength;  /* nm: the paraxial quantities and the detector plane */
 } owe_lens_batch_settings;
 
-/* Per lens, over every wavelength together (focusedSpot over all arrived rays). */
+/* Per lens, over every wavelength together: focusedSpot over all arrived rays, or for an afocal
+ * lens (paraxial power zero at the reference wavelength) afocalSpot (analysis.cpp:70-81): the
+ * apparent direction of the mean ray and the angular RMS about it. */
 typedef struct owe_lens_result {
     uint64_t launched, arrived;
-    double efl, bfl;                 /* paraxial, at the reference wavelength */
+    double efl, bfl;                 /* paraxial, at the reference wavelength (inf when afocal) */
     double entrance_pupil_z, entrance_pupil_radius;
     double last_vertex_z;
     double best_focus_z;             /* from the last vertex */
     double rms_best, rms_paraxial;   /* RMS spot radius at best focus and at the paraxial image plane */
     double centroid_y;               /* at the paraxial image plane */
     double transmitted;              /* mean weight T over launched rays */
+    int32_t afocal;
+    int32_t reserved;
+    double angular_magnification;    /* afocal: D n0 / nk of the system matrix */
+    double apparent_angle, angular_rms;  /* afocal: radians */
 } owe_lens_result;
 
-/* The eight sums a spot is computed from, per lens and wavelength, with A = p_xy − B p_z and
- * B = d_xy / d_z of each arrived ray: count, ΣAx, ΣAy, ΣBx, ΣBy, Σ|A|², Σ|B|², ΣA·B; then ΣT. */
+/* The eight sums a spot is computed from, per lens and wavelength, with B = d_xy / d_z of each
+ * arrived ray and A = p_xy − B p_z (focused) or A = atan(B) per component (afocal): count, ΣAx, ΣAy,
+ * ΣBx, ΣBy, Σ|A|², Σ|B|², ΣA·B; then ΣT. */
 #define OWE_LENS_MOMENTS 9
+
+/* Differentiable lens parameters: up to OWE_LENS_MAX_PARAMS (surface, field) pairs, the surface
+ * counted within each lens, shared by every lens of the batch. A run then also returns, per lens,
+ * the derivative of its spot metric — rms_best when focused, angular_rms when afocal — with respect
+ * to each parameter, for the rays the run traced (forward-mode dual numbers through every surface;
+ * the launch rays are held fixed). No counterpart in the reference, which has no lens derivatives. */
+#define OWE_LENS_MAX_PARAMS 8
+#define OWE_LENS_PARAM_CURVATURE 0
+#define OWE_LENS_PARAM_CONIC 1
+#define OWE_LENS_PARAM_THICKNESS 2  /* the gap after the surface: every later vertex moves */
+#define OWE_LENS_PARAM_A4 3         /* A4, A6, A8, A10: 3, 4, 5, 6 */
+typedef struct owe_lens_param {
+    uint32_t surface;
+    int32_t field;
+} owe_lens_param;
 
 typedef struct owe_hip_lens_batch owe_hip_lens_batch;
 
@@ -106,6 +139,10 @@ int owe_hip_lens_batch_trace(owe_hip_lens_batch* batch, uint32_t lens, int wavel
                              uint64_t n, double* out);
 /* Milliseconds of the last run's persistent kernel (HIP events on the batch's stream). */
 int owe_hip_lens_batch_last_ms(const owe_hip_lens_batch* batch, double* ms);
+/* Sets the differentiable parameters (n ≤ OWE_LENS_MAX_PARAMS; 0 clears them). */
+int owe_hip_lens_batch_set_params(owe_hip_lens_batch* batch, const owe_lens_param* params, uint32_t n);
+/* The last run's derivatives: n_lenses × n_params doubles, d(metric)/dθ in SI units. */
+int owe_hip_lens_batch_gradients(owe_hip_lens_batch* batch, double* grad);
 
 #ifdef __cplusplus
 }

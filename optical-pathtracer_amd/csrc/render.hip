@@ -758,8 +758,10 @@ __device__ __forceinline__ void flushFilm(unsigned long long* film, uint32_t pix
     acc[0] = acc[1] = acc[2] = 0;
 }
 
-template <int kCap>
-__global__ __launch_bounds__(kBlock) void persistentKernel(DScene sc, Settings set, Persist a) {
+// kWaves: the waves per SIMD the register allocation must allow (2: no spills; 3: more waves to
+// hide latency, at the price of some register spills) — the autotuner measures both.
+template <int kCap, int kWaves>
+__global__ __launch_bounds__(kBlock, kWaves) void persistentKernel(DScene sc, Settings set, Persist a) {
     __shared__ uint32_t stack[kCap * kBlock];
     uint32_t* st = stack + threadIdx.x;
     const uint32_t pass = *a.pass;
@@ -889,6 +891,7 @@ struct owe_hip_renderer {
     int mode = OWE_HIP_MODE_WAVEFRONT;
     int computeUnits = 256;
     int blocksPerCU = 2;
+    int persistWaves = 2;
     uint32_t chunk = 4;
     bool useGraph = false;
     unsigned long long* persistNext = nullptr;
@@ -1070,9 +1073,16 @@ Persist persistArgs(owe_hip_renderer* r, uint64_t total, uint32_t spp) {
 void launchPersistent(owe_hip_renderer* r, const Persist& a) {
     hipLaunchKernelGGL(persistBegin, dim3{1}, dim3{64}, 0, r->stream, a);
     const dim3 grid{unsigned(std::max(1, r->computeUnits * r->blocksPerCU))};
-    if (r->stackCap <= 16) hipLaunchKernelGGL(persistentKernel<16>, grid, dim3{kBlock}, 0, r->stream, r->dev, r->set, a);
-    else if (r->stackCap <= 32) hipLaunchKernelGGL(persistentKernel<32>, grid, dim3{kBlock}, 0, r->stream, r->dev, r->set, a);
-    else hipLaunchKernelGGL(persistentKernel<48>, grid, dim3{kBlock}, 0, r->stream, r->dev, r->set, a);
+    auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, grid, dim3{kBlock}, 0, r->stream, r->dev, r->set, a); };
+    if (r->persistWaves == 3) {
+        if (r->stackCap <= 16) go(persistentKernel<16, 3>);
+        else if (r->stackCap <= 32) go(persistentKernel<32, 3>);
+        else go(persistentKernel<48, 3>);
+    } else {
+        if (r->stackCap <= 16) go(persistentKernel<16, 2>);
+        else if (r->stackCap <= 32) go(persistentKernel<32, 2>);
+        else go(persistentKernel<48, 2>);
+    }
 }
 
 void dropGraph(owe_hip_renderer* r) {
@@ -1436,7 +1446,8 @@ int owe_hip_renderer_particles(owe_hip_renderer* r, int pass, const uint64_t* in
     });
 }
 
-int owe_hip_renderer_set_mode(owe_hip_renderer* r, int mode, int blocks_per_cu, int chunk, int use_graph) {
+int owe_hip_renderer_set_mode(owe_hip_renderer* r, int mode, int blocks_per_cu, int chunk, int use_graph,
+                              int waves_per_simd) {
     return guard([&] {
         require(r != nullptr, "renderer is null");
         require(mode == OWE_HIP_MODE_WAVEFRONT || mode == OWE_HIP_MODE_PERSISTENT, "unknown mode");
@@ -1444,22 +1455,26 @@ int owe_hip_renderer_set_mode(owe_hip_renderer* r, int mode, int blocks_per_cu, 
                 "the persistent form traces camera paths");
         require(blocks_per_cu >= 0 && blocks_per_cu <= 64, "blocks per compute unit must lie in [0, 64]");
         require(chunk >= 0 && chunk <= 1024, "chunk must lie in [0, 1024]");
+        require(waves_per_simd == 0 || waves_per_simd == 2 || waves_per_simd == 3, "waves per SIMD must be 0, 2 or 3");
         OWE_HIP_CHECK(hipSetDevice(r->device));
         r->mode = mode;
         if (blocks_per_cu > 0) r->blocksPerCU = blocks_per_cu;
         if (chunk > 0) r->chunk = uint32_t(chunk);
+        if (waves_per_simd > 0) r->persistWaves = waves_per_simd;
         r->useGraph = use_graph != 0;
         dropGraph(r);
     });
 }
 
-int owe_hip_renderer_get_mode(const owe_hip_renderer* r, int* mode, int* blocks_per_cu, int* chunk, int* use_graph) {
+int owe_hip_renderer_get_mode(const owe_hip_renderer* r, int* mode, int* blocks_per_cu, int* chunk, int* use_graph,
+                              int* waves_per_simd) {
     return guard([&] {
         require(r != nullptr, "renderer is null");
         if (mode) *mode = r->mode;
         if (blocks_per_cu) *blocks_per_cu = r->blocksPerCU;
         if (chunk) *chunk = int(r->chunk);
         if (use_graph) *use_graph = r->useGraph ? 1 : 0;
+        if (waves_per_simd) *waves_per_simd = r->persistWaves;
     });
 }
 
@@ -1488,21 +1503,26 @@ int owe_hip_renderer_autotune(owe_hip_renderer* r, int spp, double* best_ms) {
         };
         const uint64_t total = uint64_t(r->host.obs.width) * uint64_t(r->host.obs.height) * uint64_t(spp);
         if (r->mode == OWE_HIP_MODE_PERSISTENT) {
-            // Resident blocks per compute unit × path ids per lane fetch.
+            // Register budget (waves per SIMD) × blocks per compute unit × path ids per lane fetch.
             const bool graph = r->useGraph;
             r->useGraph = false;
-            for (int b : {1, 2, 3, 4, 6, 8})
-                for (int c : {1, 2, 4, 8, 16}) {
-                    r->blocksPerCU = b;
-                    r->chunk = uint32_t(c);
-                    hipLaunchKernelGGL(setPass, dim3{1}, dim3{64}, 0, r->stream, r->passSlot, 0u);
-                    const double ms = timePass([&] { launchPersistent(r, persistArgs(r, total, uint32_t(spp))); });
-                    if (ms < best) {
-                        best = ms;
-                        bestA = b;
-                        bestB = c;
+            int bestW = 2;
+            for (int w : {2, 3})
+                for (int b : {1, 2, 3, 4, 6, 8})
+                    for (int c : {1, 2, 4, 8, 16}) {
+                        r->persistWaves = w;
+                        r->blocksPerCU = b;
+                        r->chunk = uint32_t(c);
+                        hipLaunchKernelGGL(setPass, dim3{1}, dim3{64}, 0, r->stream, r->passSlot, 0u);
+                        const double ms = timePass([&] { launchPersistent(r, persistArgs(r, total, uint32_t(spp))); });
+                        if (ms < best) {
+                            best = ms;
+                            bestA = b;
+                            bestB = c;
+                            bestW = w;
+                        }
                     }
-                }
+            r->persistWaves = bestW;
             r->blocksPerCU = bestA;
             r->chunk = uint32_t(bestB);
             r->useGraph = graph;

@@ -20,7 +20,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from .scene import CMedium, INDEX_CIDDOR, INDEX_SELLMEIER
+from .scene import CMedium, INDEX_CAUCHY, INDEX_CIDDOR, INDEX_SELLMEIER
 
 LAMBDA_F, LAMBDA_D, LAMBDA_C = 486.13, 587.56, 656.27
 # Fraunhofer h, g, F, e, d, C, r lines (nm): the seven wavelengths of the batched analysis.
@@ -35,6 +35,7 @@ CATALOG = {
     "N-F2": ((1.39757037, 0.159201403, 1.2686543), (0.00995906143, 0.0546931752, 119.248346)),
     "N-SF6": ((1.77931763, 0.338149866, 2.08734474), (0.0133714182, 0.0617533621, 174.01759)),
     "N-SF11": ((1.73759695, 0.313747346, 1.89878101), (0.013188707, 0.0623068142, 155.23629)),
+    "crown": ("cauchy", (1.5046, 0.00420)),   # IndexModel::cauchy, absolute (medium.cpp:145-147)
 }
 CROWNS = ("N-BK7", "N-SK16", "N-BAF10")
 FLINTS = ("F2", "N-F2", "N-SF6", "N-SF11")
@@ -50,6 +51,11 @@ def media_array():
             m.index_kind = INDEX_CIDDOR
             continue
         B, C = CATALOG[name]
+        if B == "cauchy":
+            m.index_kind, m.rel_air = INDEX_CAUCHY, 0
+            for k, v in enumerate(C):
+                m.coeff_a[k] = v
+            continue
         m.index_kind, m.terms, m.rel_air = INDEX_SELLMEIER, len(B), 1
         for k in range(len(B)):
             m.coeff_a[k], m.coeff_b[k] = B[k], C[k]
@@ -66,6 +72,9 @@ def index(name: str, lnm: float) -> float:
     if CATALOG[name] is None:
         return ciddor_air(lnm)
     B, C = CATALOG[name]
+    if B == "cauchy":
+        l2 = (lnm * 1e-3) ** 2
+        return C[0] + C[1] / l2
     l2 = (lnm * 1e-3) ** 2
     n2 = 1 + sum(b * l2 / (l2 - c) for b, c in zip(B, C))
     return math.sqrt(n2) * ciddor_air(lnm)
@@ -74,12 +83,14 @@ def index(name: str, lnm: float) -> float:
 @dataclass
 class Surface:
     R: float                  # mm, math.inf for flat
-    t: float                  # mm to the next vertex
+    t: float                  # mm to the next vertex, signed along the fixed chart
     medium: str               # after the surface
     sd: float                 # semi-diameter, mm
     k: float = 0.0
     A: tuple = ()             # A4, A6, ... in SI units
     stop: bool = False
+    reflect: bool = False     # a mirror row
+    hole: float = 0.0         # mirror: central hole radius, mm
 
 
 @dataclass
@@ -97,12 +108,22 @@ class Prescription:
 SINGLET = Prescription([Surface(50.0, 8.0, "N-BK7", 12.5), Surface(-50.0, 40.0, "air", 12.5)], name="biconvex singlet")
 DOUBLET = Prescription([Surface(44.78, 6.0, "N-BK7", 12.5), Surface(-44.78, 2.5, "F2", 12.5),
                         Surface(-810.8, 95.0, "air", 12.5)], name="cemented achromat f=100")
+# The reference's Gregorian telescope (lenses/gregorian_54x.lens): a paraboloidal primary with a
+# 26 mm hole, an ellipsoidal secondary, a Huygenian ocular of two plano-convex crown lenses ("crown"
+# is the catalog's Cauchy glass, medium.cpp:145-150). The secondary's spider lies behind it.
+GREGORIAN = Prescription([
+    Surface(-900.0, -600.0, "air", 54.0, k=-1.0, stop=True, reflect=True, hole=13.0),
+    Surface(247.368421053, 667.255310082, "air", 21.0, k=-0.42136041859, reflect=True),
+    Surface(41.3076651664, 3.0, "crown", 13.0), Surface(math.inf, 50.3333333333, "air", 13.0),
+    Surface(13.7692217221, 2.5, "crown", 6.0), Surface(math.inf, 14.7400784369, "air", 6.0)],
+    name="Gregorian 54x")
 
 
 class CLensSurface(ctypes.Structure):
     _fields_ = [("curvature", ctypes.c_double), ("conic", ctypes.c_double), ("asphere", ctypes.c_double * 4),
                 ("thickness", ctypes.c_double), ("semi_diameter", ctypes.c_double), ("medium", ctypes.c_int32),
-                ("stop", ctypes.c_int32)]
+                ("stop", ctypes.c_int32), ("reflect", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("hole_radius", ctypes.c_double)]
 
 
 class CLens(ctypes.Structure):
@@ -121,7 +142,16 @@ class CLensResult(ctypes.Structure):
                 ("bfl", ctypes.c_double), ("entrance_pupil_z", ctypes.c_double),
                 ("entrance_pupil_radius", ctypes.c_double), ("last_vertex_z", ctypes.c_double),
                 ("best_focus_z", ctypes.c_double), ("rms_best", ctypes.c_double), ("rms_paraxial", ctypes.c_double),
-                ("centroid_y", ctypes.c_double), ("transmitted", ctypes.c_double)]
+                ("centroid_y", ctypes.c_double), ("transmitted", ctypes.c_double), ("afocal", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("angular_magnification", ctypes.c_double),
+                ("apparent_angle", ctypes.c_double), ("angular_rms", ctypes.c_double)]
+
+
+class CLensParam(ctypes.Structure):
+    _fields_ = [("surface", ctypes.c_uint32), ("field", ctypes.c_int32)]
+
+
+PARAM_CURVATURE, PARAM_CONIC, PARAM_THICKNESS, PARAM_A4 = 0, 1, 2, 3
 
 
 MOMENTS = 9
@@ -137,6 +167,8 @@ def c_surface(s: Surface) -> CLensSurface:
     c.semi_diameter = s.sd * 1e-3
     c.medium = MEDIA_ORDER.index(s.medium)
     c.stop = int(s.stop)
+    c.reflect = int(s.reflect)
+    c.hole_radius = s.hole * 1e-3
     return c
 
 

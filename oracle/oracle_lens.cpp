@@ -26,15 +26,19 @@ namespace {
 using shapes::intersectSag;
 using shapes::sag;
 
-// A lens row: its sag (no central hole) and the vertex position along the axis.
+// A lens row: its sag (a mirror's annulus outside its hole), the vertex position along the fixed
+// chart, and whether it reflects.
 struct Surf : shapes::Sag {
     double z = 0;
+    bool reflect = false;
 };
 
 Surf makeSurf(const owe_lens_surface& r, double z) {
     Surf s;
-    static_cast<shapes::Sag&>(s) = shapes::makeSag(r.curvature, r.conic, r.asphere, r.semi_diameter, 0.0);
+    static_cast<shapes::Sag&>(s) =
+        shapes::makeSag(r.curvature, r.conic, r.asphere, r.semi_diameter, r.reflect ? r.hole_radius : 0.0);
     s.z = z;
+    s.reflect = r.reflect != 0;
     return s;
 }
 
@@ -44,26 +48,51 @@ struct Lens {
     double nAmbient = 1;
 };
 
-// One ray through the lens: true when every surface refracts it (traceThrough's acceptance);
-// p, d the last surface's point and the outgoing direction, T the product of 1 − R.
+// Whether the segment a → b (b's z may be infinite along d) crosses the body of a mirror other than
+// surfaces `from` and `to`: its vertex-plane annulus hole ≤ r ≤ semi-diameter.
+bool obstructed(const Lens& L, V3 a, V3 d, double zEnd, size_t from, size_t to) {
+    for (size_t j = 0; j < L.surf.size(); ++j) {
+        const Surf& m = L.surf[j];
+        if (!m.reflect || j == from || j == to) continue;
+        const double lo = std::min(a.z, zEnd), hi = std::max(a.z, zEnd);
+        if (!(m.z > lo && m.z < hi) || d.z == 0) continue;
+        const V3 x = a + d * ((m.z - a.z) / d.z);
+        const double r2 = x.x * x.x + x.y * x.y;
+        if (r2 >= m.rMin * m.rMin && r2 <= m.rMax * m.rMax) return true;
+    }
+    return false;
+}
+
+// One ray through the lens, surface after surface: true when every surface takes it — a
+// refracting row refracts it (traceThrough's acceptance), a mirror row reflects it — no mirror body
+// stops it on the way, and it leaves along +z; p, d the last surface's point and the outgoing
+// direction, T the product of 1 − R over the refractions.
 bool traceRay(const Lens& L, V3 o, V3 d, V3& p, V3& dOut, double& T) {
     T = 1;
     double nPrev = L.nAmbient;
-    for (size_t i = 0; i < L.surf.size(); ++i) {
+    const size_t N = L.surf.size();
+    for (size_t i = 0; i < N; ++i) {
         const Surf& s = L.surf[i];
         double t;
         V3 q, n;
         const V3 ol(o.x, o.y, o.z - s.z);
         if (!intersectSag(s, ol, d, 0, kInf, t, q, n)) return false;
-        const double nNext = L.nAfter[i];
-        const double R = fresnelDielectric(std::fabs(dot(d, n)), nNext / nPrev);
+        const V3 qw(q.x, q.y, q.z + s.z);
+        if (obstructed(L, o, d, qw.z, i == 0 ? N : i - 1, i)) return false;
         V3 w;
-        if (!refractDir(d, n, nPrev, nNext, w)) return false;
-        T *= 1 - R;
-        o = V3(q.x, q.y, q.z + s.z);
+        if (s.reflect) {
+            w = d - n * (2 * dot(d, n));
+        } else {
+            const double nNext = L.nAfter[i];
+            const double R = fresnelDielectric(std::fabs(dot(d, n)), nNext / nPrev);
+            if (!refractDir(d, n, nPrev, nNext, w)) return false;
+            T *= 1 - R;
+            nPrev = nNext;
+        }
+        o = qw;
         d = w;
-        nPrev = nNext;
     }
+    if (obstructed(L, o, d, d.z > 0 ? kInf : -kInf, N - 1, N)) return false;
     p = o;
     dOut = d;
     return d.z > 0;
@@ -100,16 +129,20 @@ void pupilPoint(uint64_t i, uint64_t n, int rings, double rEP, double& x, double
 
 struct Paraxial {
     double efl = kInf, bfl = kInf, epz = 0, epr = 0, length = 0;
+    bool afocal = false;
+    double angularMagnification = 0;
 };
 
-// paraxialAnalysis for refracting rows with the object at infinity (prescription.cpp:193-321).
+// paraxialAnalysis with the object at infinity (prescription.cpp:193-321): a refracting row has
+// power (n' − n) c, a mirror −2 dir n c with dir the travel along the fixed chart, which it reverses.
 Paraxial paraxial(const std::vector<owe_lens_surface>& s, const std::vector<double>& nAfter, double n0) {
     const size_t N = s.size();
     std::vector<double> nBefore(N), phi(N), t(N);
-    double nPrev = n0;
+    double nPrev = n0, dir = 1;
     for (size_t i = 0; i < N; ++i) {
         nBefore[i] = nPrev;
-        phi[i] = (nAfter[i] - nPrev) * s[i].curvature;
+        phi[i] = s[i].reflect ? -2 * dir * nAfter[i] * s[i].curvature : (nAfter[i] - nPrev) * s[i].curvature;
+        if (s[i].reflect) dir = -dir;
         t[i] = std::fabs(s[i].thickness);
         nPrev = nAfter[i];
     }
@@ -130,8 +163,14 @@ Paraxial paraxial(const std::vector<owe_lens_surface>& s, const std::vector<doub
     Paraxial r;
     for (size_t i = 0; i + 1 < N; ++i) r.length += t[i];
     const double nk = nAfter[N - 1];
-    r.efl = -nk / c;
-    r.bfl = -nk * a / c;
+    const double scale = 1e-12 * (1 + std::fabs(a) + std::fabs(d));
+    r.afocal = std::fabs(c) * std::max(1.0, r.length) < scale * 1e3 || std::fabs(c) < 1e-9;
+    if (!r.afocal) {
+        r.efl = -nk / c;
+        r.bfl = -nk * a / c;
+    } else {
+        r.angularMagnification = d * n0 / nk;
+    }
     int stop = -1;
     for (size_t i = 0; i < N; ++i)
         if (s[i].stop) stop = int(i);
@@ -258,7 +297,9 @@ void oracle_lens_moments(const owe_medium* media, const owe_lens_surface* surfac
                 V3 p, d;
                 double T;
                 if (!traceRay(u.L, o, u.dir, p, d, T)) continue;
-                const double bx = d.x / d.z, by = d.y / d.z, ax = p.x - bx * p.z, ay = p.y - by * p.z;
+                const double bx = d.x / d.z, by = d.y / d.z;
+                const double ax = u.px.afocal ? std::atan(bx) : p.x - bx * p.z;
+                const double ay = u.px.afocal ? std::atan(by) : p.y - by * p.z;
                 m[0] += 1;
                 m[1] += ax;
                 m[2] += ay;
@@ -281,6 +322,48 @@ void oracle_lens_moments(const owe_medium* media, const owe_lens_surface* surfac
         paraxialOut[2] = u.px.epz;
         paraxialOut[3] = u.px.epr;
         paraxialOut[4] = u.px.length;
+        paraxialOut[5] = u.px.afocal ? 1 : 0;
+        paraxialOut[6] = u.px.angularMagnification;
+    }
+}
+
+// Rays index[i] of `lens` traced with its surfaces perturbed — delta[k] added to parameter k
+// (surface params[2k], field params[2k + 1]: curvature, conic, gap after the surface, A4..A10) —
+// from the launch of the unperturbed lens: out n × 7 as oracle_lens_trace.
+void oracle_lens_trace_perturbed(const owe_medium* media, const owe_lens_surface* surfaces, const owe_lens* lens,
+                                 double fieldDeg, double lambda, double lambdaRef, uint32_t nParams,
+                                 const int32_t* params, const double* delta, const uint64_t* index, uint64_t n,
+                                 double* out) {
+    const Setup u = setup(media, surfaces, *lens, fieldDeg, lambda, lambdaRef);
+    std::vector<owe_lens_surface> rows(surfaces + lens->first_surface, surfaces + lens->first_surface + lens->surface_count);
+    for (uint32_t k = 0; k < nParams; ++k) {
+        owe_lens_surface& r = rows[size_t(params[2 * k])];
+        const int f = params[2 * k + 1];
+        if (f == OWE_LENS_PARAM_CURVATURE) r.curvature += delta[k];
+        else if (f == OWE_LENS_PARAM_CONIC) r.conic += delta[k];
+        else if (f == OWE_LENS_PARAM_THICKNESS) r.thickness += delta[k];
+        else r.asphere[f - OWE_LENS_PARAM_A4] += delta[k];
+    }
+    Lens L = u.L;
+    double z = 0;
+    for (size_t i = 0; i < rows.size(); ++i) {
+        L.surf[i] = makeSurf(rows[i], z);
+        z += rows[i].thickness;
+    }
+    for (uint64_t k = 0; k < n; ++k) {
+        double x, y;
+        pupilPoint(index[k], lens->rays, lens->pupil_rings, u.rEP, x, y);
+        const V3 P(x, y, u.zEP);
+        const V3 o = P + u.dir * ((u.zStart - u.zEP) / u.dir.z);
+        V3 p, d;
+        double T;
+        double* r = out + 7 * k;
+        if (traceRay(L, o, u.dir, p, d, T)) {
+            r[0] = p.x, r[1] = p.y, r[2] = p.z, r[3] = d.x, r[4] = d.y, r[5] = d.z, r[6] = T;
+        } else {
+            for (int q = 0; q < 6; ++q) r[q] = std::nan("");
+            r[6] = 0;
+        }
     }
 }
 

@@ -170,6 +170,7 @@ class Oracle(_Lib):
             "sag_intersect": (None, [_vp, _vp, _vp, _d, _d, _vp]),
             "sag_value": (_d, [_vp, _d]),
             "lens_trace_ray": (None, [_vp, _vp, _vp, _d, _vp, _vp, _vp]),
+            "lens_trace_perturbed": (None, [_vp, _vp, _vp, _d, _d, _d, _u32, _vp, _vp, _vp, _u64, _vp]),
             "caustic_forward": (None, [_vp, _vp, _u64, _vp, _i]),
             "caustic_fd": (None, [_vp, _vp, _vp, _u64, _vp, _i, _d, _vp]),
             "spectrum_luminance_blackbody": (_d, [_d]),
@@ -314,10 +315,22 @@ class LensOracle:
 
     def moments(self, lens: int, lam: float, threads: int = 0):
         m = np.zeros(9)
-        px = np.zeros(5)
+        px = np.zeros(7)
         self.o.lens_moments(self.media, self.surf, ctypes.byref(self.arr[lens]), self.field_deg, lam, self.reference,
                             threads, _ptr(m), _ptr(px))
-        return m, dict(efl=px[0], bfl=px[1], entrance_pupil_z=px[2], entrance_pupil_radius=px[3], length=px[4])
+        return m, dict(efl=px[0], bfl=px[1], entrance_pupil_z=px[2], entrance_pupil_radius=px[3], length=px[4],
+                       afocal=bool(px[5]), angular_magnification=px[6])
+
+    def trace_perturbed(self, lens: int, lam: float, params, delta, index) -> np.ndarray:
+        """trace() with parameter k (surface, field) of the lens moved by delta[k], from the
+        unperturbed lens's launch rays."""
+        index = np.ascontiguousarray(index, dtype=np.uint64)
+        pr = np.ascontiguousarray(np.asarray(params, dtype=np.int32).reshape(-1))
+        dl = np.ascontiguousarray(delta, dtype=np.float64)
+        out = np.zeros((len(index), 7))
+        self.o.lens_trace_perturbed(self.media, self.surf, ctypes.byref(self.arr[lens]), self.field_deg, lam,
+                                    self.reference, len(dl), _ptr(pr), _ptr(dl), _ptr(index), len(index), _ptr(out))
+        return out
 
 
 class CausticOracle:

@@ -45,15 +45,18 @@ def render_with(sc, st, spp, passes, mode, **kw):
     return img, stats
 
 
+@pytest.mark.parametrize("waves", [2, 3])
 @pytest.mark.parametrize("case", sorted(CASES))
-def test_persistent_form_equals_wavefront_bit_for_bit(cuda, case):
+def test_persistent_form_equals_wavefront_bit_for_bit(cuda, case, waves):
+    # Both register budgets of the persistent kernel (3 waves per SIMD spills registers) compute
+    # every path exactly as the wavefront kernels do.
     make, st, spp = CASES[case]
     sc = make()
     a, sa = render_with(sc, st, spp, 2, False)
-    b, sb = render_with(sc, st, spp, 2, True, blocks_per_cu=2, chunk=4)
+    b, sb = render_with(sc, st, spp, 2, True, blocks_per_cu=2, chunk=4, waves_per_simd=waves)
     diff = np.abs(a - b)
-    print(f"{case}: max |diff| {diff.max():.3e}, pixels differing {(diff > 0).any(axis=2).sum()}, stats {sa} / {sb}",
-          flush=True)
+    print(f"{case} ({waves} waves/SIMD): max |diff| {diff.max():.3e}, pixels differing "
+          f"{(diff > 0).any(axis=2).sum()}, stats {sa} / {sb}", flush=True)
     assert np.array_equal(a, b)
     assert sa == sb
 
