@@ -1,6 +1,6 @@
 """Benchmark of the hip transport backend on one MI355X; prints ONE JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload prism|rayquery|light|lens|caustic]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload prism|rayquery|light|lens|caustic|telescope]
 
 Workload ``prism`` (the default; plain spectral path tracing of a dispersion prism): an equilateral
 Sellmeier-glass prism (40 mm sides, 20 mm tall) on a 1 m² Lambertian floor of albedo 0.8 in air
@@ -17,7 +17,8 @@ the host threads, full frames of the same scene and spp.
 Beside it: ``rayquery`` (below), ``light`` (light tracing of the same prism scene, 256×256×64
 particles per pass), ``lens`` (256 seeded achromats × 7 wavelengths in one persistent launch) and
 ``caustic`` (a gradient step of the caustic slab: 1,048,576 particles forward, then the path-replay
-adjoint into the 256×256 heights).
+adjoint into the 256×256 heights) and ``telescope`` (64 perturbed Gregorian telescopes, the angular
+RMS of each afocal spot and its derivatives with respect to 8 prescription parameters).
 
 Workload ``rayquery`` (batch ray queries on a BVH): 1,048,820-triangle displaced icosphere
 (frequency 229, the nearest 20·n² count at or above 2^20), 8,388,608 seeded rays resident in HBM.
@@ -486,6 +487,91 @@ def bench_caustic(args):
     return out
 
 
+def telescope_variants(n: int = 64, rings: int = 200, seed: int = 5):
+    """n perturbed copies of the reference's Gregorian (lenses/gregorian_54x.lens): both mirrors'
+    radii ±0.2 %, the secondary's conic ±0.002, the ocular's radii ±0.5 %, each re-solved afocal."""
+    import numpy as np
+
+    from owe_hip import lens as L
+
+    base = L.solve_afocal(L.GREGORIAN, 587.5618).with_rings(rings)
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        sf = [L.Surface(**{**q.__dict__}) for q in base.surfaces]
+        sf[0].R *= 1 + rng.uniform(-2e-3, 2e-3)
+        sf[1].R *= 1 + rng.uniform(-2e-3, 2e-3)
+        sf[1].k += rng.uniform(-2e-3, 2e-3)
+        sf[2].R *= 1 + rng.uniform(-5e-3, 5e-3)
+        sf[4].R *= 1 + rng.uniform(-5e-3, 5e-3)
+        out.append(L.solve_afocal(L.Prescription(sf, base.ambient, base.rays, base.name, base.rings), 587.5618))
+    return out
+
+
+TELESCOPE_PARAMS = [(0, 0), (0, 1), (1, 0), (1, 1), (0, 2), (1, 2), (2, 0), (4, 0)]
+
+
+def bench_telescope(args):
+    """Differentiable telescope: 64 perturbed Gregorians, each traced with 120,601 pupil rays at the F,
+    d and C lines; per variant the angular RMS of its afocal spot and its derivatives with respect
+    to 8 parameters (both mirrors' curvatures and conics, both mirror gaps, two ocular curvatures)
+    by forward-mode dual numbers."""
+    import numpy as np
+
+    from owe_hip import lens as L
+
+    variants = telescope_variants()
+    lams = (486.13, 587.56, 656.27)
+    st = L.c_settings(field_deg=0.05, wavelengths=lams, pixels=0, reference=587.5618)
+    b = L.LensBatch(variants, st)
+    b.set_params(TELESCOPE_PARAMS)
+    for _ in range(args.warmup):
+        b.run()
+    t0 = time.perf_counter()
+    kernel_ms = 0.0
+    for _ in range(args.steps):
+        b.run()
+        kernel_ms += b.last_ms()
+    wall = time.perf_counter() - t0
+    g = b.gradients()
+    per, _, _ = b.results()
+    traces = len(variants) * variants[0].rays * len(lams)
+    grads = len(variants) * len(TELESCOPE_PARAMS)
+    log(f"telescope: {args.steps} runs in {wall:.3f}s; {grads * args.steps / wall:.0f} derivatives/s; kernel "
+        f"{kernel_ms / args.steps:.2f} ms/run; angular RMS {np.mean([r['angular_rms'] for r in per]):.3e} rad")
+    out = {"value": grads * args.steps / wall, "unit": "spot-metric derivatives/s (variant x parameter)",
+           "ms_per_step": wall * 1e3 / args.steps,
+           "config": {"workload": "differentiable Gregorian telescope: 64 perturbed variants x 8 parameters, "
+                                  "120,601 pupil rays x 3 wavelengths each, afocal angular RMS",
+                      "variants": len(variants), "parameters": len(TELESCOPE_PARAMS), "traces_per_step": traces,
+                      "kernel_ms_per_step": kernel_ms / args.steps,
+                      "ray_traces_per_s": traces * args.steps / wall,
+                      "mean_angular_rms_rad": float(np.mean([r["angular_rms"] for r in per])),
+                      "gradient_norm_mean": float(np.linalg.norm(g, axis=1).mean())}}
+    b.close()
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        from pyoracle import LensOracle, Oracle
+
+        lo = LensOracle(Oracle(), variants[:1], field_deg=0.05, reference=587.5618)
+        idx = np.arange(variants[0].rays, dtype=np.uint64)
+        t0 = time.perf_counter()
+        # One variant's gradient by central differences: two perturbed traces per parameter.
+        for k in range(len(TELESCOPE_PARAMS)):
+            for sgn in (1.0, -1.0):
+                delta = np.zeros(len(TELESCOPE_PARAMS))
+                delta[k] = sgn * 1e-6
+                for lam in lams:
+                    lo.trace_perturbed(0, lam, TELESCOPE_PARAMS, delta, idx)
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": len(TELESCOPE_PARAMS) / el, "unit": "spot-metric derivatives/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"one variant's 8 derivatives by central differences of the oracle's fp64 "
+                                         f"trace (16 perturbed traces of {variants[0].rays} rays x 3 wavelengths, "
+                                         f"{el:.1f} s, 1 thread)"}
+    return out
+
+
 def main() -> int:
     import torch  # noqa: F401  (first: its HIP runtime is the process's; see owe_hip/_lib.py)
 
@@ -493,11 +579,13 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="prism", choices=["prism", "rayquery", "light", "lens", "caustic"])
+    ap.add_argument("--workload", default="prism", choices=["prism", "rayquery", "light", "lens", "caustic",
+                                                             "telescope"])
     ap.add_argument("--no-rayquery", action="store_true", help="prism: skip the ray-query record beside it")
     ap.add_argument("--no-light", action="store_true", help="prism: skip the light-tracing record beside it")
     ap.add_argument("--no-lens", action="store_true", help="prism: skip the lens-analysis record beside it")
     ap.add_argument("--no-caustic", action="store_true", help="prism: skip the caustic-gradient record beside it")
+    ap.add_argument("--no-telescope", action="store_true", help="prism: skip the telescope-gradient record beside it")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 1 GiB device buffer 3 times (the known-size kernel that calibrates "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -533,8 +621,13 @@ def main() -> int:
         if not args.no_caustic:
             result["caustic"] = bench_caustic(argparse.Namespace(steps=args.steps, warmup=args.warmup,
                                                                  no_cpu_baseline=args.no_cpu_baseline))
+        if not args.no_telescope:
+            result["telescope"] = bench_telescope(argparse.Namespace(steps=args.steps, warmup=args.warmup,
+                                                                     no_cpu_baseline=args.no_cpu_baseline))
     elif args.workload == "caustic":
         result = bench_caustic(args)
+    elif args.workload == "telescope":
+        result = bench_telescope(args)
     elif args.workload == "light":
         result = bench_light(args)
     elif args.workload == "lens":

@@ -34,10 +34,17 @@ namespace {
 constexpr int kLensBlock = 256;
 constexpr double kHistScale = 4294967296.0;  // 2^32 units of weight per unit
 
-// A lens row: its sag surface (shapes_dev.hpp) and the vertex position along the axis.
+// A lens row: its sag surface (shapes_dev.hpp; a mirror's annulus outside its hole), the vertex
+// position along the fixed chart, and whether it reflects.
 struct DSurf : DSag {
     float z;
+    int32_t reflect;
 };
+
+// Parameters differentiated per gradient pass (of up to OWE_LENS_MAX_PARAMS), and the sums a pass
+// adds per (lens, wavelength, parameter): ΣA', ΣB' (x, y), ΣA·A', ΣB·B', Σ(A'·B + A·B').
+constexpr int kGroup = 4;
+constexpr int kGradSums = 7;
 
 struct DLensInfo {
     uint32_t first, count;    // surfaces
@@ -46,7 +53,9 @@ struct DLensInfo {
     uint32_t itemsPerLambda;
     uint32_t rings;           // 0: Fibonacci lattice; r: hexapolar rings
     double rEP, zEP, zStart;  // entrance pupil (sampling radius 0.999 rEP) and launch plane
-    float detX, detY, detZ, detHalf;  // detector grid centre, plane, half size
+    float detX, detY, detZ, detHalf;  // detector grid centre, plane, half size (0: no grid)
+    int32_t afocal;           // spot sums of angles (afocalSpot) instead of (A, B) at an image
+    int32_t mirrors;          // reflecting rows: their vertex planes obstruct
 };
 
 struct LensArgs {
@@ -60,6 +69,12 @@ struct LensArgs {
     unsigned long long* next;
     double* moments;          // [lens × nλ × OWE_LENS_MOMENTS]
     unsigned long long* hist; // [lens × pixels²]
+    // Gradient passes: parameters [paramBase, paramBase + kGroup) of nParams, each (surface within
+    // the lens, field); their sums [lens × nλ × OWE_LENS_MAX_PARAMS × kGradSums].
+    uint32_t nParams, paramBase;
+    int32_t paramSurface[OWE_LENS_MAX_PARAMS], paramField[OWE_LENS_MAX_PARAMS];
+    double* grads;
+    int32_t sums;             // a gradient pass also adds the moments and histogram
 };
 
 // Pupil point i of n on the entrance pupil of radius rEP, in double: a Fibonacci lattice over the
@@ -90,31 +105,153 @@ __device__ __forceinline__ void pupilPoint(uint64_t i, uint64_t n, uint32_t ring
     y = rr * sa;
 }
 
-// One ray i of lens L at wavelength w: true when it crosses every surface; p, d after the last one.
+// Whether the ray o + s d, s in (0, sEnd) (sEnd < 0: unbounded), crosses the vertex-plane annulus
+// hole ≤ r ≤ semi-diameter of a mirror of the lens other than surfaces `from` and `to`.
+__device__ __forceinline__ bool obstructed(const LensArgs& a, const DLensInfo& L, f3 o, f3 d, float zEnd, uint32_t from,
+                                           uint32_t to) {
+    for (uint32_t j = 0; j < L.count; ++j) {
+        const DSurf& m = a.surf[L.first + j];
+        if (!m.reflect || j == from || j == to || d.z == 0.0f) continue;
+        if (!(m.z > fminf(o.z, zEnd) && m.z < fmaxf(o.z, zEnd))) continue;
+        const float s = (m.z - o.z) / d.z;
+        const float x = o.x + d.x * s, y = o.y + d.y * s, r2 = x * x + y * y;
+        if (r2 >= m.rMin2 && r2 <= m.rMax2) return true;
+    }
+    return false;
+}
+
+// Derivatives of the ray with respect to the kGroup parameters of a gradient pass.
+struct RayGrad {
+    float o[3][kGroup], d[3][kGroup];
+};
+
+// One ray i of lens L at wavelength w: true when every surface takes it — a refracting row refracts
+// it, a mirror reflects it — no mirror body stops it and it leaves along +z; p, d after the last
+// surface. With kGrad the derivatives of p and d ride along (launch rays held fixed): at each
+// surface the implicit function theorem on F = sag(r²) − z gives the root's derivative
+// dt = −(∇F·(do + t dd) + dz_v + F_θ)/F_t, the hit point dp = do + t dd + dt d, the normal's through
+// dN of N = (−2g x, −2g y, 1) with g = d sag / d r², and the refracted or reflected direction's by
+// the chain rule of its formula.
+template <bool kGrad>
 __device__ __forceinline__ bool traceLensRay(const LensArgs& a, const DLensInfo& L, uint32_t li, uint32_t w, uint64_t i,
-                                             f3& p, f3& d, float& T) {
+                                             f3& p, f3& d, float& T, RayGrad& G) {
     double x, y;
     pupilPoint(i, L.rays, L.rings, L.rEP, x, y);
     const double s = (L.zStart - L.zEP) / a.dirZ;
     f3 o = mk3(float(x), float(y + a.dirY * s), float(L.zEP + a.dirZ * s));
     d = mk3(0.0f, float(a.dirY), float(a.dirZ));
     T = 1.0f;
+    if (kGrad) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int k = 0; k < kGroup; ++k) G.o[c][k] = G.d[c][k] = 0.0f;
+    }
     float nPrev = a.nAmbient[li * a.nLambda + w];
     for (uint32_t k = 0; k < L.count; ++k) {
         const DSurf& sf = a.surf[L.first + k];
         f3 q, n;
         float th;
         if (!sagHit(sf, mk3(o.x, o.y, o.z - sf.z), d, 0.0f, __builtin_inff(), false, th, q)) return false;
+        if (L.mirrors && obstructed(a, L, o, d, q.z + sf.z, k == 0 ? L.count : k - 1, k)) return false;
         n = sagNormalF(sf, q);
-        const float nNext = a.nAfter[(L.first + k) * a.nLambda + w];
-        const float R = fresnelDielectric(fabsf(dot3(d, n)), nNext / nPrev);
         f3 wv;
-        if (!refractDir(d, n, nPrev, nNext, wv)) return false;
-        T *= 1.0f - R;
+        float eta = 1.0f;
+        if (sf.reflect) {
+            wv = d - n * (2.0f * dot3(d, n));
+        } else {
+            const float nNext = a.nAfter[(L.first + k) * a.nLambda + w];
+            const float R = fresnelDielectric(fabsf(dot3(d, n)), nNext / nPrev);
+            if (!refractDir(d, n, nPrev, nNext, wv)) return false;
+            T *= 1.0f - R;
+            eta = nPrev / nNext;
+            nPrev = nNext;
+        }
+        if (kGrad) {
+            // The surface's partials at the local hit point q.
+            const float r2 = q.x * q.x + q.y * q.y, c = sf.cf, kk = sf.kf;
+            const float S = sqrtf(fmaxf(1e-30f, 1.0f - (1.0f + kk) * c * c * r2));
+            float g = c / (2.0f * S), gr = c * (1.0f + kk) * c * c / (4.0f * S * S * S);
+            const float Fc = r2 * ((1.0f + S) + (1.0f + kk) * r2 * c * c / S) / ((1.0f + S) * (1.0f + S));
+            const float Fk = c * c * c * r2 * r2 / (2.0f * S * (1.0f + S) * (1.0f + S));
+            const float gc = 1.0f / (2.0f * S) + (1.0f + kk) * c * c * r2 / (2.0f * S * S * S);
+            const float gk = c * c * c * r2 / (4.0f * S * S * S);
+            float FA[4], gA[4], rj = 1.0f;  // rj = r2^j
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                FA[j] = rj * r2 * r2;                   // ∂sag/∂A_j = r2^(2+j)
+                gA[j] = float(2 + j) * rj * r2;         // ∂g/∂A_j = (2+j) r2^(1+j)
+                if (j < sf.nA) {
+                    g += sf.A[j] * gA[j];
+                    gr += sf.A[j] * float(2 + j) * float(1 + j) * rj;
+                }
+                rj *= r2;
+            }
+            const f3 gradF = mk3(2.0f * g * q.x, 2.0f * g * q.y, -1.0f);
+            const float Ft = dot3(gradF, d);
+            const f3 N = mk3(-2.0f * g * q.x, -2.0f * g * q.y, 1.0f);
+            const float invN = rsqrtf(dot3(N, N));
+            const float sig = dot3(d, n) < 0.0f ? 1.0f : -1.0f;  // the refraction's facing normal
+            f3 W = wv;
+            float invW = 1.0f, ci = 0.0f, sq = 0.0f;
+            if (!sf.reflect) {
+                const f3 nn = n * sig;
+                ci = -dot3(d, nn);
+                sq = sqrtf(fmaxf(0.0f, 1.0f - eta * eta * (1.0f - ci * ci)));
+                W = d * eta + nn * (eta * ci - sq);
+                invW = rsqrtf(dot3(W, W));
+            }
+#pragma unroll
+            for (int kp = 0; kp < kGroup; ++kp) {
+                const uint32_t pi = a.paramBase + uint32_t(kp);
+                float dc = 0.0f, dk = 0.0f, dA[4] = {0.0f, 0.0f, 0.0f, 0.0f}, dzv = 0.0f;
+                if (pi < a.nParams) {
+                    const uint32_t ps = uint32_t(a.paramSurface[pi]);
+                    const int pf = a.paramField[pi];
+                    if (ps == k) {
+                        if (pf == OWE_LENS_PARAM_CURVATURE) dc = 1.0f;
+                        else if (pf == OWE_LENS_PARAM_CONIC) dk = 1.0f;
+                        else if (pf >= OWE_LENS_PARAM_A4) dA[pf - OWE_LENS_PARAM_A4] = 1.0f;
+                    }
+                    if (pf == OWE_LENS_PARAM_THICKNESS && ps < k) dzv = 1.0f;
+                }
+                const f3 dO = mk3(G.o[0][kp], G.o[1][kp], G.o[2][kp]), dD = mk3(G.d[0][kp], G.d[1][kp], G.d[2][kp]);
+                float Fth = Fc * dc + Fk * dk, gth = gc * dc + gk * dk;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    Fth += FA[j] * dA[j];
+                    gth += gA[j] * dA[j];
+                }
+                const f3 dx = dO + dD * th;
+                const float dt = -(dot3(gradF, dx) + dzv + Fth) / Ft;
+                const f3 dP = dx + d * dt;
+                // dN from dg = gr dr² + g_θ and the local point's move.
+                const float dg = gr * 2.0f * (q.x * dP.x + q.y * dP.y) + gth;
+                const f3 dN = mk3(-2.0f * (dg * q.x + g * dP.x), -2.0f * (dg * q.y + g * dP.y), 0.0f);
+                const f3 dn = (dN - n * dot3(n, dN)) * invN;
+                f3 dW;
+                if (sf.reflect) {
+                    dW = dD - (n * (dot3(dD, n) + dot3(d, dn)) + dn * dot3(d, n)) * 2.0f;
+                } else {
+                    const f3 nn = n * sig, dnn = dn * sig;
+                    const float dci = -(dot3(dD, nn) + dot3(d, dnn));
+                    const float dsq = sq > 0.0f ? eta * eta * ci * dci / sq : 0.0f;
+                    const f3 dWu = dD * eta + nn * (eta * dci - dsq) + dnn * (eta * ci - sq);
+                    dW = (dWu - wv * dot3(wv, dWu)) * invW;
+                }
+                G.o[0][kp] = dP.x;
+                G.o[1][kp] = dP.y;
+                G.o[2][kp] = dP.z;
+                G.d[0][kp] = dW.x;
+                G.d[1][kp] = dW.y;
+                G.d[2][kp] = dW.z;
+            }
+        }
         o = mk3(q.x, q.y, q.z + sf.z);
         d = wv;
-        nPrev = nNext;
     }
+    if (L.mirrors && obstructed(a, L, o, d, d.z > 0.0f ? __builtin_inff() : -__builtin_inff(), L.count - 1, L.count))
+        return false;
     p = o;
     return d.z > 0.0f;
 }
@@ -125,8 +262,13 @@ __device__ __forceinline__ double waveSumD(double v) {
     return v;
 }
 
+// The persistent batch: a wave takes the next (lens, wavelength, block of rays) item, its lanes
+// trace the block, the spot sums (and with kGrad the derivative sums of the pass's parameters) are
+// reduced across the wave in double and added with one atomic each.
+template <bool kGrad>
 __global__ __launch_bounds__(kLensBlock) void lensKernel(LensArgs a) {
     const uint32_t lane = __lane_id();
+    const bool sums = !kGrad || a.sums;
     for (;;) {
         unsigned long long item = 0;
         if (lane == 0) item = atomicAdd(a.next, 1ull);
@@ -149,14 +291,41 @@ __global__ __launch_bounds__(kLensBlock) void lensKernel(LensArgs a) {
         const uint32_t w = uint32_t(local / L.itemsPerLambda);
         const uint64_t first = (local % L.itemsPerLambda) * a.raysPerItem;
         double m[OWE_LENS_MOMENTS] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        float gs[kGrad ? kGroup : 1][kGradSums] = {};
         for (uint32_t k = lane; k < a.raysPerItem; k += 64) {
             const uint64_t i = first + k;
             if (i >= L.rays) break;
             f3 p, d;
             float T;
-            if (!traceLensRay(a, L, li, w, i, p, d, T)) continue;
+            RayGrad G;
+            if (!traceLensRay<kGrad>(a, L, li, w, i, p, d, T, G)) continue;
             const double bx = double(d.x) / double(d.z), by = double(d.y) / double(d.z);
-            const double ax = double(p.x) - bx * double(p.z), ay = double(p.y) - by * double(p.z);
+            const double ax = L.afocal ? double(atanf(float(bx))) : double(p.x) - bx * double(p.z);
+            const double ay = L.afocal ? double(atanf(float(by))) : double(p.y) - by * double(p.z);
+            if (kGrad) {
+                const float iz = 1.0f / d.z;
+#pragma unroll
+                for (int kp = 0; kp < kGroup; ++kp) {
+                    const float dbx = (G.d[0][kp] * d.z - d.x * G.d[2][kp]) * iz * iz;
+                    const float dby = (G.d[1][kp] * d.z - d.y * G.d[2][kp]) * iz * iz;
+                    float dax, day;
+                    if (L.afocal) {
+                        dax = dbx / (1.0f + float(bx * bx));
+                        day = dby / (1.0f + float(by * by));
+                    } else {
+                        dax = G.o[0][kp] - dbx * p.z - float(bx) * G.o[2][kp];
+                        day = G.o[1][kp] - dby * p.z - float(by) * G.o[2][kp];
+                    }
+                    gs[kp][0] += dax;
+                    gs[kp][1] += day;
+                    gs[kp][2] += dbx;
+                    gs[kp][3] += dby;
+                    gs[kp][4] += float(ax) * dax + float(ay) * day;
+                    gs[kp][5] += float(bx) * dbx + float(by) * dby;
+                    gs[kp][6] += dax * float(bx) + day * float(by) + float(ax) * dbx + float(ay) * dby;
+                }
+            }
+            if (!sums) continue;
             m[0] += 1.0;
             m[1] += ax;
             m[2] += ay;
@@ -166,7 +335,7 @@ __global__ __launch_bounds__(kLensBlock) void lensKernel(LensArgs a) {
             m[6] += bx * bx + by * by;
             m[7] += ax * bx + ay * by;
             m[8] += double(T);
-            if (a.pixels > 0) {
+            if (a.pixels > 0 && L.detHalf > 0.0f) {
                 const float s = (L.detZ - p.z) / d.z;
                 const float gx = (p.x + d.x * s - L.detX) / L.detHalf, gy = (p.y + d.y * s - L.detY) / L.detHalf;
                 const float fx = (gx + 1.0f) * 0.5f * float(a.pixels), fy = (1.0f - gy) * 0.5f * float(a.pixels);
@@ -177,20 +346,40 @@ __global__ __launch_bounds__(kLensBlock) void lensKernel(LensArgs a) {
                 }
             }
         }
-        double* dst = a.moments + (size_t(li) * a.nLambda + w) * OWE_LENS_MOMENTS;
+        if (sums) {
+            double* dst = a.moments + (size_t(li) * a.nLambda + w) * OWE_LENS_MOMENTS;
 #pragma unroll
-        for (int q = 0; q < OWE_LENS_MOMENTS; ++q) {
-            const double v = waveSumD(m[q]);
-            if (lane == 0 && v != 0.0) atomicAdd(dst + q, v);
+            for (int q = 0; q < OWE_LENS_MOMENTS; ++q) {
+                const double v = waveSumD(m[q]);
+                if (lane == 0 && v != 0.0) atomicAdd(dst + q, v);
+            }
+        }
+        if (kGrad) {
+            double* dst = a.grads + (size_t(li) * a.nLambda + w) * (OWE_LENS_MAX_PARAMS * kGradSums);
+#pragma unroll
+            for (int kp = 0; kp < kGroup; ++kp) {
+                if (a.paramBase + uint32_t(kp) >= a.nParams) continue;
+#pragma unroll
+                for (int q = 0; q < kGradSums; ++q) {
+                    const double v = waveSumD(double(gs[kp][q]));
+                    if (lane == 0 && v != 0.0) atomicAdd(dst + (a.paramBase + kp) * kGradSums + q, v);
+                }
+            }
         }
     }
 }
 
-__global__ void lensBegin(LensArgs a, size_t nMoments, size_t nHist) {
+__global__ void lensBegin(LensArgs a, size_t nMoments, size_t nHist, size_t nGrads) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x, stride = size_t(gridDim.x) * blockDim.x;
     if (i == 0) *a.next = 0;
     for (size_t k = i; k < nMoments; k += stride) a.moments[k] = 0.0;
     for (size_t k = i; k < nHist; k += stride) a.hist[k] = 0;
+    for (size_t k = i; k < nGrads; k += stride) a.grads[k] = 0.0;
+}
+
+// Resets the item counter between the gradient passes of one run.
+__global__ void lensNext(LensArgs a) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a.next = 0;
 }
 
 __global__ void lensProbe(LensArgs a, uint32_t li, uint32_t w, const uint64_t* index, uint64_t n, double* out) {
@@ -198,8 +387,9 @@ __global__ void lensProbe(LensArgs a, uint32_t li, uint32_t w, const uint64_t* i
     if (k >= n) return;
     f3 p, d;
     float T;
+    RayGrad G;
     double* r = out + 7 * k;
-    if (traceLensRay(a, a.lens[li], li, w, index[k], p, d, T)) {
+    if (traceLensRay<false>(a, a.lens[li], li, w, index[k], p, d, T, G)) {
         r[0] = p.x, r[1] = p.y, r[2] = p.z, r[3] = d.x, r[4] = d.y, r[5] = d.z, r[6] = T;
     } else {
         for (int q = 0; q < 6; ++q) r[q] = __builtin_nan("");
@@ -234,17 +424,23 @@ double mediumIndexHost(const owe_medium& m, double lnm) {
 
 struct ParaxialOut {
     double efl, bfl, epz, epr, length;
+    double lastZ;  // the last vertex along the fixed chart (signed gaps)
+    bool afocal;
+    double angMag;
 };
 
-// paraxialAnalysis for refracting rows, object at infinity (src/owe/scene/prescription.cpp:193-321):
-// the (y, n·u) system matrix, EFL −n'/C, BFL −n'A/C, the stop (marked, else the surface that limits
-// a marginal ray) and its image through the rows before it, the entrance pupil.
+// paraxialAnalysis, object at infinity (src/owe/scene/prescription.cpp:193-321): the (y, n·u) system
+// matrix with a refracting row's power (n' − n) c and a mirror's −2 dir n c (dir the travel along
+// the fixed chart, which a mirror reverses; gaps unfolded), EFL −n'/C and BFL −n'A/C or — when C
+// vanishes at the reference's scale — afocal with angular magnification D n/n', the stop (marked,
+// else the surface that limits a marginal ray) and its image through the rows before it.
 ParaxialOut paraxialHost(const owe_lens_surface* s, size_t N, const std::vector<double>& nAfter, double n0) {
     std::vector<double> nBefore(N), phi(N), t(N);
-    double nPrev = n0;
+    double nPrev = n0, dir = 1;
     for (size_t i = 0; i < N; ++i) {
         nBefore[i] = nPrev;
-        phi[i] = (nAfter[i] - nPrev) * s[i].curvature;
+        phi[i] = s[i].reflect ? -2 * dir * nAfter[i] * s[i].curvature : (nAfter[i] - nPrev) * s[i].curvature;
+        if (s[i].reflect) dir = -dir;
         t[i] = std::fabs(s[i].thickness);
         nPrev = nAfter[i];
     }
@@ -259,11 +455,20 @@ ParaxialOut paraxialHost(const owe_lens_surface* s, size_t N, const std::vector<
         }
     }
     ParaxialOut r{};
-    for (size_t i = 0; i + 1 < N; ++i) r.length += t[i];
-    require(std::fabs(c) > 1e-9, "lens is afocal (no finite focus for the spot analysis)");
+    for (size_t i = 0; i + 1 < N; ++i) {
+        r.length += t[i];
+        r.lastZ += s[i].thickness;
+    }
     const double nk = nAfter[N - 1];
-    r.efl = -nk / c;
-    r.bfl = -nk * a / c;
+    const double scale = 1e-12 * (1 + std::fabs(a) + std::fabs(d));
+    r.afocal = std::fabs(c) * std::max(1.0, r.length) < scale * 1e3 || std::fabs(c) < 1e-9;
+    if (r.afocal) {
+        r.efl = r.bfl = INFINITY;
+        r.angMag = d * n0 / nk;
+    } else {
+        r.efl = -nk / c;
+        r.bfl = -nk * a / c;
+    }
     int stop = -1;
     for (size_t i = 0; i < N; ++i)
         if (s[i].stop) stop = int(i);
@@ -312,7 +517,7 @@ struct owe_hip_lens_batch {
     std::vector<DLensInfo> info;
     owe_lens_batch_settings settings{};
     LensArgs args{};
-    size_t nMoments = 0, nHist = 0;
+    size_t nMoments = 0, nHist = 0, nGrads = 0;
     int computeUnits = 256;
     int blocksPerCU = 4;
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -345,11 +550,23 @@ void layoutItems(owe_hip_lens_batch* b, uint32_t raysPerItem) {
     b->args.totalItems = base;
 }
 
+// One run: the plain kernel, or with parameters set one gradient pass per kGroup parameters (the
+// first also adding the spot sums and the histogram).
 double runOnce(owe_hip_lens_batch* b) {
-    hipLaunchKernelGGL(lensBegin, dim3{1024}, dim3{256}, 0, b->stream, b->args, b->nMoments, b->nHist);
+    hipLaunchKernelGGL(lensBegin, dim3{1024}, dim3{256}, 0, b->stream, b->args, b->nMoments, b->nHist, b->nGrads);
     OWE_HIP_CHECK(hipEventRecord(b->ev[0], b->stream));
-    hipLaunchKernelGGL(lensKernel, dim3{unsigned(std::max(1, b->computeUnits * b->blocksPerCU))}, dim3{kLensBlock}, 0,
-                       b->stream, b->args);
+    const dim3 grid{unsigned(std::max(1, b->computeUnits * b->blocksPerCU))};
+    if (b->args.nParams == 0) {
+        hipLaunchKernelGGL(lensKernel<false>, grid, dim3{kLensBlock}, 0, b->stream, b->args);
+    } else {
+        for (uint32_t base = 0; base < b->args.nParams; base += kGroup) {
+            LensArgs a = b->args;
+            a.paramBase = base;
+            a.sums = base == 0;
+            if (base > 0) hipLaunchKernelGGL(lensNext, dim3{1}, dim3{64}, 0, b->stream, a);
+            hipLaunchKernelGGL(lensKernel<true>, grid, dim3{kLensBlock}, 0, b->stream, a);
+        }
+    }
     OWE_HIP_CHECK(hipGetLastError());
     OWE_HIP_CHECK(hipEventRecord(b->ev[1], b->stream));
     OWE_HIP_CHECK(hipEventSynchronize(b->ev[1]));
@@ -390,6 +607,8 @@ int owe_hip_lens_batch_create(int device, const owe_medium* media, uint32_t n_me
                     name + ": values must be finite");
             require(1 - (1 + r.conic) * r.curvature * r.curvature * r.semi_diameter * r.semi_diameter >= 0,
                     name + ": aperture exceeds the domain of the conic");
+            require(r.hole_radius >= 0 && (r.hole_radius == 0 || (r.reflect && r.hole_radius < r.semi_diameter)),
+                    name + ": only a mirror has a hole, smaller than its semi-diameter");
         }
         selectDevice(device);
         b = new owe_hip_lens_batch;
@@ -418,9 +637,15 @@ int owe_hip_lens_batch_create(int device, const owe_medium* media, uint32_t n_me
                     name + ": with pupil rings r the lens traces 1 + 3 r (r + 1) rays");
             const owe_lens_surface* rows = surfaces + L.first_surface;
             std::vector<double> nRef(L.surface_count);
+            int mirrors = 0;
             for (uint32_t k = 0; k < L.surface_count; ++k) {
                 require(!used[L.first_surface + k], name + ": surfaces are shared with another lens");
                 used[L.first_surface + k] = 1;
+                if (rows[k].reflect) {
+                    ++mirrors;
+                    require(rows[k].medium == (k == 0 ? L.ambient : rows[k - 1].medium),
+                            name + ": a mirror reflects into the medium it is met in");
+                }
                 nRef[k] = mediumIndexHost(media[rows[k].medium], s.reference_wavelength);
                 for (uint32_t w = 0; w < nw; ++w)
                     nAfter[size_t(L.first_surface + k) * nw + w] = float(mediumIndexHost(media[rows[k].medium], s.wavelengths[w]));
@@ -433,7 +658,9 @@ int owe_hip_lens_batch_create(int device, const owe_medium* media, uint32_t n_me
             r.bfl = px.bfl;
             r.entrance_pupil_z = px.epz;
             r.entrance_pupil_radius = px.epr;
-            r.last_vertex_z = px.length;
+            r.last_vertex_z = px.lastZ;
+            r.afocal = px.afocal ? 1 : 0;
+            r.angular_magnification = px.angMag;
             b->paraxial.push_back(r);
             DLensInfo I{};
             I.first = L.first_surface;
@@ -445,10 +672,15 @@ int owe_hip_lens_batch_create(int device, const owe_medium* media, uint32_t n_me
             I.zStart = -0.05 - 0.1 * px.length;
             // The detector: the paraxial image plane at the reference wavelength, centred on the
             // chief ray's paraxial image height EFL·tan θ.
-            I.detZ = float(px.length + px.bfl);
+            I.detZ = float(px.lastZ + px.bfl);
             I.detX = 0.0f;
             I.detY = float(px.efl * std::tan(f));
             I.detHalf = float(s.detector_half_size > 0 ? s.detector_half_size : 5e-3 * std::fabs(px.efl));
+            I.afocal = px.afocal ? 1 : 0;
+            I.mirrors = mirrors;
+            if (px.afocal) {  // no image plane: no irradiance grid
+                I.detZ = I.detY = I.detHalf = 0.0f;
+            }
             b->info.push_back(I);
         }
         ds.resize(n_surfaces);
@@ -458,8 +690,10 @@ int owe_hip_lens_batch_create(int device, const owe_medium* media, uint32_t n_me
             for (uint32_t k = 0; k < L.surface_count; ++k) {
                 const owe_lens_surface& r = surfaces[L.first_surface + k];
                 DSurf d{};
-                static_cast<DSag&>(d) = makeSag(r.curvature, r.conic, r.asphere, r.semi_diameter, 0.0);
+                static_cast<DSag&>(d) = makeSag(r.curvature, r.conic, r.asphere, r.semi_diameter,
+                                                r.reflect ? r.hole_radius : 0.0);
                 d.z = float(z);
+                d.reflect = r.reflect ? 1 : 0;
                 ds[L.first_surface + k] = d;
                 z += r.thickness;
             }
@@ -486,6 +720,9 @@ int owe_hip_lens_batch_create(int device, const owe_medium* media, uint32_t n_me
         a.moments = lensAlloc<double>(b, b->nMoments);
         b->nHist = size_t(n_lenses) * a.pixels * a.pixels;
         a.hist = lensAlloc<unsigned long long>(b, b->nHist);
+        b->nGrads = 0;
+        a.grads = lensAlloc<double>(b, size_t(n_lenses) * nw * OWE_LENS_MAX_PARAMS * kGradSums);
+        a.nParams = 0;
         layoutItems(b, 1024);
         *out = b;
     });
@@ -517,6 +754,76 @@ int owe_hip_lens_batch_run(owe_hip_lens_batch* b) {
         require(b != nullptr, "batch is null");
         OWE_HIP_CHECK(hipSetDevice(b->device));
         runOnce(b);
+    });
+}
+
+int owe_hip_lens_batch_set_params(owe_hip_lens_batch* b, const owe_lens_param* params, uint32_t n) {
+    return guard([&] {
+        require(b != nullptr, "batch is null");
+        require(n <= OWE_LENS_MAX_PARAMS, "at most OWE_LENS_MAX_PARAMS parameters");
+        require(n == 0 || params != nullptr, "null parameters");
+        uint32_t minCount = ~0u;
+        for (const owe_lens& L : b->lenses) minCount = std::min(minCount, L.surface_count);
+        for (uint32_t k = 0; k < n; ++k) {
+            require(params[k].surface < minCount, "parameter surface beyond a lens's surfaces");
+            require(params[k].field >= OWE_LENS_PARAM_CURVATURE && params[k].field <= OWE_LENS_PARAM_A4 + 3,
+                    "unknown parameter field");
+            b->args.paramSurface[k] = int32_t(params[k].surface);
+            b->args.paramField[k] = params[k].field;
+        }
+        b->args.nParams = n;
+        b->nGrads = n ? size_t(b->lenses.size()) * b->args.nLambda * OWE_LENS_MAX_PARAMS * kGradSums : 0;
+        b->ran = false;
+    });
+}
+
+int owe_hip_lens_batch_gradients(owe_hip_lens_batch* b, double* grad) {
+    return guard([&] {
+        require(b != nullptr && grad != nullptr, "null argument");
+        require(b->ran, "run the batch first");
+        require(b->args.nParams > 0, "no parameters are set");
+        const size_t nl = b->lenses.size(), nw = b->args.nLambda, np = b->args.nParams;
+        std::vector<double> mom(nl * nw * OWE_LENS_MOMENTS), gs(nl * nw * OWE_LENS_MAX_PARAMS * kGradSums);
+        OWE_HIP_CHECK(hipMemcpy(mom.data(), b->args.moments, mom.size() * sizeof(double), hipMemcpyDeviceToHost));
+        OWE_HIP_CHECK(hipMemcpy(gs.data(), b->args.grads, gs.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t li = 0; li < nl; ++li) {
+            double m[OWE_LENS_MOMENTS] = {};
+            std::vector<double> g(np * kGradSums, 0.0);
+            for (size_t w = 0; w < nw; ++w) {
+                for (int q = 0; q < OWE_LENS_MOMENTS; ++q) m[q] += mom[(li * nw + w) * OWE_LENS_MOMENTS + q];
+                for (size_t k = 0; k < np * kGradSums; ++k) g[k] += gs[(li * nw + w) * OWE_LENS_MAX_PARAMS * kGradSums + k];
+            }
+            const double n = m[0];
+            for (size_t k = 0; k < np; ++k) {
+                double out = NAN;
+                const double* G = &g[k * kGradSums];  // ΣA'x, ΣA'y, ΣB'x, ΣB'y, ΣA·A', ΣB·B', Σ(A'·B + A·B')
+                if (n >= 3) {
+                    const double max = m[1] / n, may = m[2] / n, mbx = m[3] / n, mby = m[4] / n;
+                    const double dmax = G[0] / n, dmay = G[1] / n, dmbx = G[2] / n, dmby = G[3] / n;
+                    if (b->paraxial[li].afocal) {
+                        // v = Σ|A|²/n − 2 c·ΣA/n + |c|², c = atan(mean B): afocalSpot's RMS².
+                        const double cx = std::atan(mbx), cy = std::atan(mby);
+                        const double dcx = dmbx / (1 + mbx * mbx), dcy = dmby / (1 + mby * mby);
+                        const double v = m[5] / n - 2 * (cx * max + cy * may) + cx * cx + cy * cy;
+                        const double dv = 2 * G[4] / n - 2 * (dcx * max + dcy * may) - 2 * (cx * dmax + cy * dmay) +
+                                          2 * (cx * dcx + cy * dcy);
+                        out = v > 0 ? dv / (2 * std::sqrt(v)) : 0.0;
+                    } else {
+                        // rms² at best focus = var_A − cov²/var_B (focusedSpot).
+                        const double va = m[5] / n - (max * max + may * may);
+                        const double vb = m[6] / n - (mbx * mbx + mby * mby);
+                        const double cv = m[7] / n - (max * mbx + may * mby);
+                        const double dva = 2 * G[4] / n - 2 * (max * dmax + may * dmay);
+                        const double dvb = 2 * G[5] / n - 2 * (mbx * dmbx + mby * dmby);
+                        const double dcv = G[6] / n - (dmax * mbx + dmay * mby + max * dmbx + may * dmby);
+                        const double r2 = vb > 0 ? va - cv * cv / vb : va;
+                        const double dr2 = vb > 0 ? dva - (2 * cv * dcv * vb - cv * cv * dvb) / (vb * vb) : dva;
+                        out = r2 > 0 ? dr2 / (2 * std::sqrt(r2)) : 0.0;
+                    }
+                }
+                grad[li * np + k] = out;
+            }
+        }
     });
 }
 
@@ -581,9 +888,19 @@ int owe_hip_lens_batch_results(owe_hip_lens_batch* b, owe_lens_result* lenses, d
                     for (int q = 0; q < OWE_LENS_MOMENTS; ++q) s[q] += m[(size_t(li) * nw + w) * OWE_LENS_MOMENTS + q];
                 r.arrived = uint64_t(s[0]);
                 r.transmitted = s[8] / double(r.launched);
-                // focusedSpot (analysis.cpp:85-113) from the sums.
+                // focusedSpot (analysis.cpp:85-113) from the sums, or afocalSpot (70-81) from the
+                // sums of angles.
                 const double zImg = r.last_vertex_z + r.bfl;
-                if (s[0] >= 3) {
+                r.apparent_angle = r.angular_rms = NAN;
+                if (r.afocal) {
+                    r.best_focus_z = r.rms_best = r.rms_paraxial = r.centroid_y = NAN;
+                    if (s[0] >= 3) {
+                        const double n = s[0], cx = std::atan(s[3] / n), cy = std::atan(s[4] / n);
+                        const double v = s[5] / n - 2 * (cx * s[1] + cy * s[2]) / n + cx * cx + cy * cy;
+                        r.apparent_angle = cy;
+                        r.angular_rms = std::sqrt(std::max(0.0, v));
+                    }
+                } else if (s[0] >= 3) {
                     const double n = s[0], max_ = s[1] / n, may = s[2] / n, mbx = s[3] / n, mby = s[4] / n;
                     const double varA = s[5] / n - (max_ * max_ + may * may), varB = s[6] / n - (mbx * mbx + mby * mby);
                     const double cov = s[7] / n - (max_ * mbx + may * mby);
@@ -606,6 +923,10 @@ int owe_hip_lens_batch_results(owe_hip_lens_batch* b, owe_lens_result* lenses, d
             for (uint32_t li = 0; li < nl; ++li) {
                 // Unit irradiance across the beam: each ray carries π r² cos θ / (rays · nλ) of flux.
                 const DLensInfo& I = b->info[li];
+                if (!(I.detHalf > 0)) {
+                    std::fill(irradiance + size_t(li) * px * px, irradiance + size_t(li + 1) * px * px, 0.0f);
+                    continue;
+                }
                 const double pix = 2.0 * I.detHalf / px;
                 const double k = 3.14159265358979323846 * I.rEP * I.rEP * 0.999 * 0.999 * cosT /
                                  (double(I.rays) * nw * kHistScale * pix * pix);

@@ -109,6 +109,8 @@ def load() -> ctypes.CDLL:
     _declare(lib, "owe_hip_lens_batch_results", c_int, [c_vp, c_vp, c_vp, c_vp])
     _declare(lib, "owe_hip_lens_batch_set_shape", c_int, [c_vp, c_int, c_int])
     _declare(lib, "owe_hip_lens_batch_autotune", c_int, [c_vp, c_vp])
+    _declare(lib, "owe_hip_lens_batch_set_params", c_int, [c_vp, c_vp, ctypes.c_uint32])
+    _declare(lib, "owe_hip_lens_batch_gradients", c_int, [c_vp, c_vp])
     _declare(lib, "owe_hip_lens_batch_trace", c_int, [c_vp, c_u32, c_int, c_vp, c_u64, c_vp])
     _declare(lib, "owe_hip_lens_batch_last_ms", c_int, [c_vp, c_vp])
     _declare(lib, "owe_hip_caustic_create", c_int, [c_int, c_vp, ctypes.POINTER(c_vp)])

@@ -91,6 +91,7 @@ class Surface:
     stop: bool = False
     reflect: bool = False     # a mirror row
     hole: float = 0.0         # mirror: central hole radius, mm
+    solve: bool = False       # the gap solve_afocal adjusts
 
 
 @dataclass
@@ -113,7 +114,7 @@ DOUBLET = Prescription([Surface(44.78, 6.0, "N-BK7", 12.5), Surface(-44.78, 2.5,
 # is the catalog's Cauchy glass, medium.cpp:145-150). The secondary's spider lies behind it.
 GREGORIAN = Prescription([
     Surface(-900.0, -600.0, "air", 54.0, k=-1.0, stop=True, reflect=True, hole=13.0),
-    Surface(247.368421053, 667.255310082, "air", 21.0, k=-0.42136041859, reflect=True),
+    Surface(247.368421053, 667.255310082, "air", 21.0, k=-0.42136041859, reflect=True, solve=True),
     Surface(41.3076651664, 3.0, "crown", 13.0), Surface(math.inf, 50.3333333333, "air", 13.0),
     Surface(13.7692217221, 2.5, "crown", 6.0), Surface(math.inf, 14.7400784369, "air", 6.0)],
     name="Gregorian 54x")
@@ -155,6 +156,62 @@ PARAM_CURVATURE, PARAM_CONIC, PARAM_THICKNESS, PARAM_A4 = 0, 1, 2, 3
 
 
 MOMENTS = 9
+
+
+def system_matrix(p: Prescription, lam: float):
+    """The paraxial (y, n u) matrix (A, B, C, D) of paraxialAnalysis (prescription.cpp:193-230): a
+    refracting row's power (n' − n) c, a mirror's −2 dir n c (dir the travel, which it reverses),
+    the gaps unfolded."""
+    a, b, c, d = 1.0, 0.0, 0.0, 1.0
+    n_prev, direction = index(p.ambient, lam), 1.0
+    for i, sf in enumerate(p.surfaces):
+        cur = 0.0 if math.isinf(sf.R) else 1.0 / (sf.R * 1e-3)
+        n = index(sf.medium, lam)
+        phi = -2 * direction * n * cur if sf.reflect else (n - n_prev) * cur
+        if sf.reflect:
+            direction = -direction
+        c, d = c - phi * a, d - phi * b
+        if i + 1 < len(p.surfaces):
+            k = abs(sf.t * 1e-3) / n
+            a, b = a + k * c, b + k * d
+        n_prev = n
+    return a, b, c, d
+
+
+def solve_afocal(p: Prescription, lam: float = LAMBDA_D) -> Prescription:
+    """solveAfocal (prescription.cpp:324-352): the marked gap set so the system's power C vanishes
+    (C is affine in the gap), its sign (the leg's travel) kept."""
+    gaps = [i for i, sf in enumerate(p.surfaces) if sf.solve]
+    if len(gaps) != 1:
+        raise ValueError("mark exactly one gap with solve")
+    g = gaps[0]
+    old = p.surfaces[g].t
+    sign = -1.0 if old < 0 else 1.0
+
+    def power(t):
+        q = [Surface(**{**sf.__dict__}) for sf in p.surfaces]
+        q[g].t = t
+        return system_matrix(Prescription(q, p.ambient, p.rays, p.name, p.rings), lam)[2]
+
+    c0, c1 = power(0.0), power(sign)
+    t = -c0 / (c1 - c0)
+    if not t > 0:
+        raise ValueError("no positive afocal separation")
+    q = [Surface(**{**sf.__dict__}) for sf in p.surfaces]
+    q[g].t = sign * t
+    return Prescription(q, p.ambient, p.rays, p.name, p.rings)
+
+
+def afocal_spot(m) -> dict:
+    """afocalSpot (analysis.cpp:70-81) from the nine sums of angles: the apparent angle of the mean
+    ray and the angular RMS about it (radians)."""
+    m = np.asarray(m, dtype=np.float64)
+    n = m[0]
+    if n < 3:
+        return dict(arrived=int(n), apparent_angle=math.nan, angular_rms=math.nan)
+    c = np.arctan(m[3:5] / n)
+    v = m[5] / n - 2 * (c @ m[1:3]) / n + c @ c
+    return dict(arrived=int(n), apparent_angle=float(c[1]), angular_rms=math.sqrt(max(0.0, v)))
 
 
 def c_surface(s: Surface) -> CLensSurface:
@@ -271,6 +328,7 @@ class LensBatch:
                                                  sum(len(p.surfaces) for p in self.lenses), self._arr, len(self.lenses),
                                                  ctypes.byref(self.settings), ctypes.byref(h)))
         self._h = h
+        self.n_params = 0
 
     def close(self):
         if self._h:
@@ -308,6 +366,21 @@ class LensBatch:
         ms = ctypes.c_double()
         _lib.check(_lib.load().owe_hip_lens_batch_last_ms(self._h, ctypes.byref(ms)))
         return ms.value
+
+    def set_params(self, params):
+        """Differentiable parameters: (surface within each lens, PARAM_*) pairs, at most 8."""
+        arr = (CLensParam * max(1, len(params)))()
+        for k, (surface, field) in enumerate(params):
+            arr[k].surface, arr[k].field = int(surface), int(field)
+        _lib.check(_lib.load().owe_hip_lens_batch_set_params(self._h, arr, len(params)))
+        self.n_params = len(params)
+
+    def gradients(self) -> np.ndarray:
+        """d(spot metric)/dθ of the last run, [lens, parameter] (SI units): rms_best when focused,
+        angular_rms when afocal."""
+        out = np.zeros((len(self.lenses), self.n_params))
+        _lib.check(_lib.load().owe_hip_lens_batch_gradients(self._h, out.ctypes.data))
+        return out
 
     def trace(self, lens: int, wavelength: int, index) -> np.ndarray:
         index = np.ascontiguousarray(index, dtype=np.uint64)

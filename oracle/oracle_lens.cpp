@@ -129,6 +129,7 @@ void pupilPoint(uint64_t i, uint64_t n, int rings, double rEP, double& x, double
 
 struct Paraxial {
     double efl = kInf, bfl = kInf, epz = 0, epr = 0, length = 0;
+    double lastZ = 0;  // the last vertex along the fixed chart (signed gaps)
     bool afocal = false;
     double angularMagnification = 0;
 };
@@ -161,7 +162,10 @@ Paraxial paraxial(const std::vector<owe_lens_surface>& s, const std::vector<doub
         }
     }
     Paraxial r;
-    for (size_t i = 0; i + 1 < N; ++i) r.length += t[i];
+    for (size_t i = 0; i + 1 < N; ++i) {
+        r.length += t[i];
+        r.lastZ += s[i].thickness;
+    }
     const double nk = nAfter[N - 1];
     const double scale = 1e-12 * (1 + std::fabs(a) + std::fabs(d));
     r.afocal = std::fabs(c) * std::max(1.0, r.length) < scale * 1e3 || std::fabs(c) < 1e-9;
@@ -324,6 +328,7 @@ void oracle_lens_moments(const owe_medium* media, const owe_lens_surface* surfac
         paraxialOut[4] = u.px.length;
         paraxialOut[5] = u.px.afocal ? 1 : 0;
         paraxialOut[6] = u.px.angularMagnification;
+        paraxialOut[7] = u.px.lastZ;
     }
 }
 

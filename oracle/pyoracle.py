@@ -315,11 +315,11 @@ class LensOracle:
 
     def moments(self, lens: int, lam: float, threads: int = 0):
         m = np.zeros(9)
-        px = np.zeros(7)
+        px = np.zeros(8)
         self.o.lens_moments(self.media, self.surf, ctypes.byref(self.arr[lens]), self.field_deg, lam, self.reference,
                             threads, _ptr(m), _ptr(px))
         return m, dict(efl=px[0], bfl=px[1], entrance_pupil_z=px[2], entrance_pupil_radius=px[3], length=px[4],
-                       afocal=bool(px[5]), angular_magnification=px[6])
+                       afocal=bool(px[5]), angular_magnification=px[6], last_vertex_z=px[7])
 
     def trace_perturbed(self, lens: int, lam: float, params, delta, index) -> np.ndarray:
         """trace() with parameter k (surface, field) of the lens moved by delta[k], from the

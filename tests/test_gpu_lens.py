@@ -58,7 +58,7 @@ def test_spot_matches_oracle(batch, oracle, li):
     lo = LensOracle(oracle, LENSES)
     ms = np.array([lo.moments(li, lam)[0] for lam in L.FRAUNHOFER7])
     _, px = lo.moments(li, L.LAMBDA_D)
-    lv = px["length"]
+    lv = px["last_vertex_z"]
     s = L.spot(ms.sum(axis=0), lv + px["bfl"], lv)
     r = per[li]
     print(f"lens {li}: arrived {r['arrived']} / {s['arrived']}, rms {r['rms_best']:.6e} / {s['rms_best']:.6e}, "

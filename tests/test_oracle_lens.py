@@ -124,7 +124,7 @@ def test_achromat_cuts_the_chromatic_focal_shift_and_keeps_every_ray(oracle, rin
         p = p.with_rings(rings) if rings else p
         lo = LensOracle(oracle, [p], field_deg=0.0, reference=lam)
         m, px = lo.moments(0, lam)
-        lv = px["length"]
+        lv = px["last_vertex_z"]
         return L.spot(m, lv + px["bfl"], lv), px, int(p.rays)
 
     shift = {}
@@ -160,7 +160,7 @@ def test_spot_from_sums_equals_direct_computation(oracle):
     A = P[:, :2] - B * P[:, 2:3]
     m, px = lo.moments(0, L.LAMBDA_D)
     assert m[0] == ok.sum()
-    lv = px["length"]
+    lv = px["last_vertex_z"]
     z_img = lv + px["bfl"]
     s = L.spot(m, z_img, lv)
     cov = ((A - A.mean(0)) * (B - B.mean(0))).sum()
