@@ -572,6 +572,72 @@ def bench_telescope(args):
     return out
 
 
+def bench_telescope_render(args, variants_rendered: int = 8):
+    """The differentiable telescope's rendering half: perturbed Gregorians as matter, each seen
+    through its ocular in a closed room with a 2,009,780-triangle statue, 512×512 at 256 spp, depth
+    32, by the persistent form (autotuned on the first variant). One step renders
+    `variants_rendered` variants; the full workload is 64."""
+    import numpy as np
+
+    from owe_hip import bodies as B
+    from owe_hip import render
+    from owe_hip import scene as S
+
+    variants = telescope_variants(n=variants_rendered)
+    st = S.settings(seed=1, max_depth=32, rr_depth=4)
+    size, spp = 512, 256
+    total_ms, paths, segments, shape = 0.0, 0, 0, None
+    build_s = 0.0
+    for vi, v in enumerate(variants):
+        t0 = time.perf_counter()
+        sc = B.telescope_room_scene(v, size=size)
+        r = render.Renderer(sc, st)
+        build_s += time.perf_counter() - t0
+        r.set_mode(render.Renderer.PERSISTENT, use_graph=True)
+        if shape is None:
+            r.autotune(64)
+            shape = r.mode()
+        else:
+            r.set_mode(render.Renderer.PERSISTENT, shape["blocks_per_cu"], shape["chunk"], True, shape["waves_per_simd"])
+        r.set_timing(True)
+        r.run_pass(spp)
+        ms, _ = r.kernel_times()["persistent"]
+        total_ms += ms
+        stt = r.stats()
+        paths += stt["paths"]
+        segments += stt["segments"]
+        r.close()
+    log(f"telescope render: {variants_rendered} variants x {size}x{size}x{spp}: {paths / (total_ms * 1e-3) / 1e6:.1f} M "
+        f"paths/s; E_seg {segments / paths:.2f}; {total_ms / variants_rendered:.1f} ms/variant; scenes built in "
+        f"{build_s:.1f} s; shape {shape}")
+    out = {"value": paths / (total_ms * 1e-3), "unit": "spectral paths/s",
+           "ms_per_step": total_ms,
+           "config": {"workload": "Gregorian telescope as matter in a 2,009,780-triangle room, 512x512x256 spp, "
+                                  "depth 32, persistent form", "variants_rendered": variants_rendered,
+                      "variants_full_workload": 64, "paths_per_variant": size * size * spp,
+                      "segments_per_path": segments / paths, "ms_per_variant": total_ms / variants_rendered,
+                      "launch": shape}}
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        from pyoracle import Oracle, World
+
+        threads = min(16, os.cpu_count() or 1)
+        sc = B.telescope_room_scene(variants[0], size=128)
+        w = World(Oracle(), sc)
+        el, n, passes = 0.0, 0, 0
+        while el < 8.0 and passes < 64:
+            t0 = time.perf_counter()
+            _, ost = w.render(st, 16, threads=threads, first_pass=passes)
+            el += time.perf_counter() - t0
+            n += int(ost[0])
+            passes += 1
+        out["cpu_baseline"] = {"value": n / el, "unit": "spectral paths/s", "cores": threads, "kind": "port",
+                               "sample": f"{passes} passes of 128x128x16 spp of the first variant's scene (the same "
+                                         f"room and statue) through the oracle's fp64 Tracer::radiance, {threads} "
+                                         f"threads, {el:.1f} s"}
+    return out
+
+
 def main() -> int:
     import torch  # noqa: F401  (first: its HIP runtime is the process's; see owe_hip/_lib.py)
 
@@ -586,6 +652,8 @@ def main() -> int:
     ap.add_argument("--no-lens", action="store_true", help="prism: skip the lens-analysis record beside it")
     ap.add_argument("--no-caustic", action="store_true", help="prism: skip the caustic-gradient record beside it")
     ap.add_argument("--no-telescope", action="store_true", help="prism: skip the telescope-gradient record beside it")
+    ap.add_argument("--no-telescope-render", action="store_true",
+                    help="telescope: skip the rendering of the telescopes in the room")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 1 GiB device buffer 3 times (the known-size kernel that calibrates "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -622,12 +690,16 @@ def main() -> int:
             result["caustic"] = bench_caustic(argparse.Namespace(steps=args.steps, warmup=args.warmup,
                                                                  no_cpu_baseline=args.no_cpu_baseline))
         if not args.no_telescope:
-            result["telescope"] = bench_telescope(argparse.Namespace(steps=args.steps, warmup=args.warmup,
-                                                                     no_cpu_baseline=args.no_cpu_baseline))
+            ns = argparse.Namespace(steps=args.steps, warmup=args.warmup, no_cpu_baseline=args.no_cpu_baseline)
+            result["telescope"] = bench_telescope(ns)
+            if not args.no_telescope_render:
+                result["telescope"]["render"] = bench_telescope_render(ns)
     elif args.workload == "caustic":
         result = bench_caustic(args)
     elif args.workload == "telescope":
         result = bench_telescope(args)
+        if not args.no_telescope_render:
+            result["render"] = bench_telescope_render(args)
     elif args.workload == "light":
         result = bench_light(args)
     elif args.workload == "lens":

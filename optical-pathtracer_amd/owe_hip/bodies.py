@@ -18,6 +18,7 @@ Units are metres.
 """
 from __future__ import annotations
 
+import functools
 import math
 
 import numpy as np
@@ -410,4 +411,118 @@ def uniform_sky_optics_scene(kind: str = "lens", size: int = 32) -> S.Scene:
         z0, z1 = sag_of(1 / 0.05, 0.0, 0.02), 0.004 + sag_of(1 / 0.08, -0.5, 0.02) + 2.0e3 * 0.02 ** 6
         sc.add_cylinder(0.02, z0, z1, S.transform(), air, glass, diel)
         sc.set_observer(size, size, (0.0, 0.03, 0.1), (0, 0, 0), up=(0, 1, 0), fov_y=math.radians(30), region=air)
+    return sc
+
+
+# ---------------------------------------------------------------- instruments as matter
+
+
+def add_prescription(sc: S.Scene, p, xf, ambient: int, mirror_optics: int, rim_optics: int, tube_radius: float = 0.0,
+                     media_of=None) -> dict:
+    """buildPrescription (src/owe/scene/prescription.cpp:354-432): a lens prescription (owe_hip.lens,
+    millimetres) as matter placed by xf — each mirror row a mirror body (its frame turned over when
+    met travelling +z, so the face meets the light; substrate, rim, bore, spider struts behind it),
+    each group of glass rows a lens with rims, each stop row in air a black annulus. Returns the
+    vertex positions (metres, along the body's z) and the index of the last vertex."""
+    from . import lens as Lm
+
+    if media_of is None:
+        def media_of(name):
+            if Lm.CATALOG[name] is None:
+                return S.Medium.air()
+            B, C = Lm.CATALOG[name]
+            if B == "cauchy":
+                return S.Medium.cauchy(C[0], C[1])
+            return S.Medium.sellmeier(B, C, True)
+
+    rows = p.surfaces
+    z = [0.0]
+    for r in rows[:-1]:
+        z.append(z[-1] + r.t * 1e-3)
+    black = sc.add_optics(S.Optics(S.SURF_ABSORBER))
+    direction, i = 1.0, 0
+    while i < len(rows):
+        r = rows[i]
+        before = p.ambient if i == 0 else rows[i - 1].medium
+        if r.reflect:
+            R = (-r.R if direction > 0 else r.R) * 1e-3
+            at = S.transform(None, (0, 0, z[i]))
+            if direction > 0:
+                at = compose(at, S.transform(S.rotation((1, 0, 0), math.pi)))
+            substrate = getattr(r, "substrate", 0.0) or 4.0
+            add_mirror(sc, R, r.k, r.sd * 1e-3, substrate * 1e-3, mirror_optics, compose(xf, at), ambient,
+                       hole=r.hole * 1e-3)
+            for k in range(getattr(r, "spider_count", 0)):
+                a = getattr(r, "spider_angle", 0.0) + 2 * math.pi * k / r.spider_count
+                outer = r.spider_outer * 1e-3
+                z_back = min(0.0, sag_of(1 / R, r.k, r.sd * 1e-3)) - substrate * 1e-3
+                strut = compose(at, compose(S.transform(S.rotation((0, 0, 1), a)),
+                                            S.transform(None, (0.5 * outer, 0, z_back - 0.5 * substrate * 1e-3 - 1e-6))))
+                pos, tri = S.box_mesh((-0.5 * outer, -0.5 * r.spider_width * 1e-3, -0.5 * substrate * 1e-3),
+                                      (0.5 * outer, 0.5 * r.spider_width * 1e-3, 0.5 * substrate * 1e-3))
+                inside = sc.add_region(sc.add_medium(S.Medium(S.INDEX_CONSTANT, (1.0,), opaque=True)))
+                sc.add_mesh(pos, tri, compose(xf, strut), ambient, inside, black)
+            direction = -direction
+            i += 1
+            continue
+        if before == p.ambient and r.medium == p.ambient:
+            if r.stop and r.sd > 0:
+                outer = tube_radius if tube_radius > r.sd * 1e-3 else 3 * r.sd * 1e-3
+                sc.add_plane(S.APERTURE_DISK, outer, r.sd * 1e-3, compose(xf, S.transform(None, (0, 0, z[i]))),
+                             ambient, ambient, black)
+            i += 1
+            continue
+        j = i
+        while j < len(rows) and rows[j].medium != p.ambient:
+            j += 1
+        radii = [(math.inf if math.isinf(q.R) else q.R * 1e-3) for q in rows[i:j + 1]]
+        add_lens(sc, radii, [q.sd * 1e-3 for q in rows[i:j + 1]], [q.t * 1e-3 for q in rows[i:j]],
+                 [media_of(q.medium) for q in rows[i:j]], compose(xf, S.transform(None, (0, 0, z[i]))), ambient,
+                 rim_optics)
+        i = j + 1
+    return dict(vertex_z=z, last=len(rows) - 1)
+
+
+@functools.lru_cache(maxsize=2)
+def _statue(frequency: int, seed: int):
+    """The room's statue mesh (a displaced icosphere), generated once per (frequency, seed)."""
+    from .workloads import icosphere
+
+    return icosphere(frequency, seed=seed, amplitude=0.05, frequency=3.0)
+
+
+def telescope_room_scene(p, size: int = 512, statue_frequency: int = 317, fov_deg: float = 50.0,
+                         seed: int = 1) -> S.Scene:
+    """The differentiable-telescope scene: telescope p (owe_hip.lens, solved afocal) at the origin
+    looking along −z into a closed room — a diffuse floor and walls, a rough aluminium wall, an
+    emitting ceiling panel, and a displaced icosphere statue of 20 · statue_frequency² triangles
+    (2,009,780 at 317) in rough gold 5 m in front — seen by an eye at the ocular's eye point (the
+    last row's gap) through a pinhole, apparent field fov_deg."""
+    sc = S.Scene()
+    air = sc.add_region(sc.add_medium(S.Medium.air()))
+    mirror = sc.add_optics(S.Optics(S.SURF_CONDUCTOR, conductor_n=S.Spectrum.constant(1.2),
+                                    conductor_k=S.Spectrum.constant(4.0)))
+    black = sc.add_optics(S.Optics(S.SURF_ABSORBER))
+    info = add_prescription(sc, p, S.transform(), air, mirror, black)
+    # The room: x in [−2, 2], y in [−1, 2], z in [−8, 1].
+    floor = sc.add_optics(S.Optics(S.SURF_DIFFUSE, reflectance=S.Spectrum.constant(0.5)))
+    wall = sc.add_optics(S.Optics(S.SURF_DIFFUSE, reflectance=S.Spectrum.constant(0.7)))
+    alu = sc.add_optics(S.Optics(S.SURF_CONDUCTOR, roughness=0.15, conductor_n=S.Spectrum.constant(1.0),
+                                 conductor_k=S.Spectrum.constant(6.5)))
+    gold = sc.add_optics(S.Optics(S.SURF_CONDUCTOR, roughness=0.2, conductor_n=S.Spectrum.constant(0.3),
+                                  conductor_k=S.Spectrum.constant(3.0)))
+    light = sc.add_emission(S.Emission(S.Spectrum.constant(8.0)))
+    sc.add_plane(S.APERTURE_RECT, 2.0, 4.5, placement((0, -1, -3.5), (0, 1, 0), (0, 0, 1)), air, air, floor)
+    sc.add_plane(S.APERTURE_RECT, 2.0, 4.5, placement((0, 2, -3.5), (0, -1, 0), (0, 0, 1)), air, air, wall)
+    sc.add_plane(S.APERTURE_RECT, 1.0, 1.0, placement((0, 1.99, -4.0), (0, -1, 0), (0, 0, 1)), air, air, black, light)
+    sc.add_plane(S.APERTURE_RECT, 2.0, 1.5, placement((0, 0.5, -8.0), (0, 0, 1), (0, 1, 0)), air, air, alu)
+    sc.add_plane(S.APERTURE_RECT, 2.0, 1.5, placement((0, 0.5, 1.0), (0, 0, -1), (0, 1, 0)), air, air, wall)
+    sc.add_plane(S.APERTURE_RECT, 4.5, 1.5, placement((-2, 0.5, -3.5), (1, 0, 0), (0, 1, 0)), air, air, wall)
+    sc.add_plane(S.APERTURE_RECT, 4.5, 1.5, placement((2, 0.5, -3.5), (-1, 0, 0), (0, 1, 0)), air, air, wall)
+    pos, tri = _statue(statue_frequency, seed)
+    inside = sc.add_region(sc.add_medium(S.Medium(S.INDEX_CONSTANT, (1.0,), opaque=True)))
+    sc.add_mesh(np.asarray(pos, np.float64) * 0.4, tri, S.transform(None, (0.0, -0.2, -5.0)), air, inside, gold)
+    eye = info["vertex_z"][info["last"]] + p.surfaces[-1].t * 1e-3
+    sc.set_observer(size, size, (0.0, 0.0, eye), (0.0, 0.0, eye - 1.0), up=(0, 1, 0), fov_y=math.radians(fov_deg),
+                    region=air)
     return sc

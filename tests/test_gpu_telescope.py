@@ -159,3 +159,29 @@ def test_batch_of_perturbed_telescopes_keeps_each_variants_gradient(cuda):
         one.close()
     assert np.isfinite(g).all()
     b.close()
+
+
+def test_telescope_in_a_room_matches_oracle_path_by_path(cuda, oracle):
+    # The instrument as matter (mirrors with bore and spider, ocular lenses with rims) in a room with
+    # a 2,880-triangle statue; camera paths from the eye point through the ocular.
+    from owe_hip import bodies as B
+    from owe_hip import render
+    from owe_hip import scene as S
+    from pyoracle import World
+
+    sc = B.telescope_room_scene(gregorian(), size=32, statue_frequency=12)
+    st = S.settings(seed=3, max_depth=32, rr_depth=4)
+    spp = 8
+    pixel = np.repeat(np.arange(32 * 32, dtype=np.uint32), spp)
+    sample = np.tile(np.arange(spp, dtype=np.uint32), 32 * 32)
+    r = render.Renderer(sc, st)
+    gpu = r.camera_paths(spp, pixel, sample)
+    ora, _ = World(oracle, sc).camera_paths(st, spp, pixel, sample)
+    gy, oy = gpu[:, 1], ora[:, 1]
+    close = np.abs(gy - oy) <= 1e-3 * np.maximum(np.abs(oy), np.abs(gy)) + 1e-6
+    print(f"telescope room: {close.mean():.5f} of {len(gy)} paths agree; mean Y gpu {gy.mean():.5f} oracle "
+          f"{oy.mean():.5f}", flush=True)
+    assert close.mean() > 0.99
+    assert abs(gy.mean() - oy.mean()) < 0.03 * abs(oy.mean())
+    r.run_pass(64)
+    assert r.stats()["inconsistencies"] <= 1e-4 * r.stats()["paths"]
