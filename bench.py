@@ -572,7 +572,7 @@ def bench_telescope(args):
     return out
 
 
-def bench_telescope_render(args, variants_rendered: int = 8):
+def bench_telescope_render(args, variants_rendered: int = 4):
     """The differentiable telescope's rendering half: perturbed Gregorians as matter, each seen
     through its ocular in a closed room with a 2,009,780-triangle statue, 512×512 at 256 spp, depth
     32, by the persistent form (autotuned on the first variant). One step renders
