@@ -121,19 +121,34 @@ def cpu_baseline_prism(sc, st, seconds_budget: float = 12.0):
                       f"of the same scene through the oracle's fp64 Tracer::radiance, {threads} threads"}
 
 
-def _time_passes(r, steps, warmup):
+def _time_passes(r, steps, warmup, events_in_timed: bool):
+    """Wall time of `steps` passes after `warmup`, the renderer's statistics and image over them,
+    and per-kernel HIP-event times. With events_in_timed the events run inside the timed passes
+    (one pair per pass for the persistent form's single launch); otherwise — the wavefront form,
+    whose events would sit between its many small kernels — the same passes are repeated with
+    events after the timed region and the breakdown comes from that repeat."""
     import torch
 
     for _ in range(warmup):
         r.run_pass(PRISM_SPP)
     r.reset()
-    r.set_timing(True)
+    r.set_timing(events_in_timed)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         r.run_pass(PRISM_SPP)
     torch.cuda.synchronize()
-    return time.perf_counter() - t0
+    wall = time.perf_counter() - t0
+    stats, img = r.stats(), r.resolve()
+    if not events_in_timed:
+        r.reset()
+        r.set_timing(True)
+        for _ in range(steps):
+            r.run_pass(PRISM_SPP)
+        torch.cuda.synchronize()
+    times = r.kernel_times()
+    r.set_timing(False)
+    return wall, stats, times, img
 
 
 def bench_prism(args):
@@ -151,9 +166,8 @@ def bench_prism(args):
         if form == "persistent":
             r.set_mode(render.Renderer.PERSISTENT, use_graph=True)
         tuned_ms = r.autotune(PRISM_SPP)
-        wall = _time_passes(r, args.steps, args.warmup)
-        stats, times = r.stats(), r.kernel_times()
-        forms[form] = dict(wall=wall, stats=stats, times=times, mode=r.mode(), tuned_ms=tuned_ms, img=r.resolve())
+        wall, stats, times, img = _time_passes(r, args.steps, args.warmup, events_in_timed=form == "persistent")
+        forms[form] = dict(wall=wall, stats=stats, times=times, mode=r.mode(), tuned_ms=tuned_ms, img=img)
         paths = stats["paths"]
         log(f"prism {form}: {args.steps} passes in {wall:.3f}s; {paths / wall / 1e6:.1f} M paths/s; "
             f"E_seg {stats['segments'] / max(1, paths):.3f}; shape {r.mode()}; "
@@ -598,7 +612,8 @@ def bench_telescope_render(args, variants_rendered: int = 4):
             r.autotune(64)
             shape = r.mode()
         else:
-            r.set_mode(render.Renderer.PERSISTENT, shape["blocks_per_cu"], shape["chunk"], True, shape["waves_per_simd"])
+            r.set_mode(render.Renderer.PERSISTENT, shape["blocks_per_cu"], shape["chunk"], True, shape["waves_per_simd"],
+                       shape["refill"])
         r.set_timing(True)
         r.run_pass(spp)
         ms, _ = r.kernel_times()["persistent"]

@@ -134,16 +134,18 @@ int owe_hip_renderer_particles(owe_hip_renderer* renderer, int pass, const uint6
  * film is integer, so both give the same image bit for bit. blocks_per_cu and chunk (path ids per
  * lane fetch) shape the persistent launch (0 keeps the current value); waves_per_simd (2 or 3; 0
  * keeps it) selects the persistent kernel compiled for that many resident waves per SIMD (3 trades
- * register spills for latency hiding); use_graph replays each pass from a captured HIP graph. No
- * counterpart in the reference (whose GPU backend has one form). */
+ * register spills for latency hiding); refill (1..64; 0 keeps it) is how many of a wave's lanes must
+ * have finished their paths before the wave starts new ones together; use_graph replays each pass
+ * from a captured HIP graph. No counterpart in the reference (whose GPU backend has one form). */
 #define OWE_HIP_MODE_WAVEFRONT 0
 #define OWE_HIP_MODE_PERSISTENT 1
 int owe_hip_renderer_set_mode(owe_hip_renderer* renderer, int mode, int blocks_per_cu, int chunk, int use_graph,
-                              int waves_per_simd);
+                              int waves_per_simd, int refill);
 int owe_hip_renderer_get_mode(const owe_hip_renderer* renderer, int* mode, int* blocks_per_cu, int* chunk,
-                              int* use_graph, int* waves_per_simd);
+                              int* use_graph, int* waves_per_simd, int* refill);
 /* Times passes of spp samples per pixel under candidate launch shapes of the current form
- * (persistent: waves per SIMD × blocks per compute unit × chunk; wavefront: bounce rounds between host reads of the
+ * (persistent: waves per SIMD × refill threshold × blocks per compute unit × chunk; wavefront: bounce
+ * rounds between host reads of the
  * live counters), keeps the fastest, and clears the film and statistics again. Only on a renderer
  * without passes. best_ms (optional) receives the winning pass time. */
 int owe_hip_renderer_autotune(owe_hip_renderer* renderer, int spp, double* best_ms);

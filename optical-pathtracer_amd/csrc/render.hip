@@ -740,6 +740,7 @@ struct Persist {
     uint32_t* counters;        // [2] inconsistencies, [3] leaks (as the wavefront's)
     float invNWY;
     int simple;
+    uint32_t regen;            // refill a wave's finished lanes once at least this many are idle
 };
 
 __global__ void persistBegin(Persist a) {
@@ -773,7 +774,11 @@ __global__ __launch_bounds__(kBlock, kWaves) void persistentKernel(DScene sc, Se
     uint32_t accPix = 0xFFFFFFFFu, segments = 0;
     long long acc[3] = {0, 0, 0};
     for (;;) {
-        const bool needChunk = !live && !done && id >= end;
+        // Finished lanes wait until at least a.regen of them (or the whole wave) are idle, then all
+        // start new paths together: the wave runs the path set-up code once for many lanes.
+        const uint64_t idle = __ballot(!live && !done);
+        const bool refill = uint32_t(__popcll(idle)) >= a.regen || __ballot(live) == 0;
+        const bool needChunk = refill && !live && !done && id >= end;
         const uint64_t ask = __ballot(needChunk);
         if (ask) {
             const int leader = __ffsll((long long)ask) - 1;
@@ -786,7 +791,7 @@ __global__ __launch_bounds__(kBlock, kWaves) void persistentKernel(DScene sc, Se
                 done = id >= a.total;
             }
         }
-        if (!live && !done) {
+        if (refill && !live && !done) {
             const uint64_t pixel = id / a.spp;
             PathStart ps;
             startPath(sc, set, pixel, id % a.spp, a.spp, pass, ps);
@@ -892,6 +897,7 @@ struct owe_hip_renderer {
     int computeUnits = 256;
     int blocksPerCU = 2;
     int persistWaves = 2;
+    uint32_t regen = 1;
     uint32_t chunk = 4;
     bool useGraph = false;
     unsigned long long* persistNext = nullptr;
@@ -1060,6 +1066,7 @@ Persist persistArgs(owe_hip_renderer* r, uint64_t total, uint32_t spp) {
     a.total = total;
     a.spp = spp;
     a.chunk = r->chunk;
+    a.regen = r->regen;
     a.pass = r->passSlot;
     a.next = r->persistNext;
     a.film = r->film;
@@ -1447,7 +1454,7 @@ int owe_hip_renderer_particles(owe_hip_renderer* r, int pass, const uint64_t* in
 }
 
 int owe_hip_renderer_set_mode(owe_hip_renderer* r, int mode, int blocks_per_cu, int chunk, int use_graph,
-                              int waves_per_simd) {
+                              int waves_per_simd, int refill) {
     return guard([&] {
         require(r != nullptr, "renderer is null");
         require(mode == OWE_HIP_MODE_WAVEFRONT || mode == OWE_HIP_MODE_PERSISTENT, "unknown mode");
@@ -1456,18 +1463,20 @@ int owe_hip_renderer_set_mode(owe_hip_renderer* r, int mode, int blocks_per_cu, 
         require(blocks_per_cu >= 0 && blocks_per_cu <= 64, "blocks per compute unit must lie in [0, 64]");
         require(chunk >= 0 && chunk <= 1024, "chunk must lie in [0, 1024]");
         require(waves_per_simd == 0 || waves_per_simd == 2 || waves_per_simd == 3, "waves per SIMD must be 0, 2 or 3");
+        require(refill >= 0 && refill <= 64, "refill threshold must lie in [0, 64]");
         OWE_HIP_CHECK(hipSetDevice(r->device));
         r->mode = mode;
         if (blocks_per_cu > 0) r->blocksPerCU = blocks_per_cu;
         if (chunk > 0) r->chunk = uint32_t(chunk);
         if (waves_per_simd > 0) r->persistWaves = waves_per_simd;
+        if (refill > 0) r->regen = uint32_t(refill);
         r->useGraph = use_graph != 0;
         dropGraph(r);
     });
 }
 
 int owe_hip_renderer_get_mode(const owe_hip_renderer* r, int* mode, int* blocks_per_cu, int* chunk, int* use_graph,
-                              int* waves_per_simd) {
+                              int* waves_per_simd, int* refill) {
     return guard([&] {
         require(r != nullptr, "renderer is null");
         if (mode) *mode = r->mode;
@@ -1475,6 +1484,7 @@ int owe_hip_renderer_get_mode(const owe_hip_renderer* r, int* mode, int* blocks_
         if (chunk) *chunk = int(r->chunk);
         if (use_graph) *use_graph = r->useGraph ? 1 : 0;
         if (waves_per_simd) *waves_per_simd = r->persistWaves;
+        if (refill) *refill = int(r->regen);
     });
 }
 
@@ -1506,23 +1516,28 @@ int owe_hip_renderer_autotune(owe_hip_renderer* r, int spp, double* best_ms) {
             // Register budget (waves per SIMD) × blocks per compute unit × path ids per lane fetch.
             const bool graph = r->useGraph;
             r->useGraph = false;
-            int bestW = 2;
+            int bestW = 2, bestR = 1;
             for (int w : {2, 3})
-                for (int b : {1, 2, 3, 4, 6, 8})
-                    for (int c : {1, 2, 4, 8, 16}) {
-                        r->persistWaves = w;
-                        r->blocksPerCU = b;
-                        r->chunk = uint32_t(c);
-                        hipLaunchKernelGGL(setPass, dim3{1}, dim3{64}, 0, r->stream, r->passSlot, 0u);
-                        const double ms = timePass([&] { launchPersistent(r, persistArgs(r, total, uint32_t(spp))); });
-                        if (ms < best) {
-                            best = ms;
-                            bestA = b;
-                            bestB = c;
-                            bestW = w;
+                for (int g : {1, 16, 32, 48})
+                    for (int b : {1, 2, 3, 4, 8})
+                        for (int c : {1, 4, 16}) {
+                            r->persistWaves = w;
+                            r->regen = uint32_t(g);
+                            r->blocksPerCU = b;
+                            r->chunk = uint32_t(c);
+                            hipLaunchKernelGGL(setPass, dim3{1}, dim3{64}, 0, r->stream, r->passSlot, 0u);
+                            const double ms =
+                                timePass([&] { launchPersistent(r, persistArgs(r, total, uint32_t(spp))); });
+                            if (ms < best) {
+                                best = ms;
+                                bestA = b;
+                                bestB = c;
+                                bestW = w;
+                                bestR = g;
+                            }
                         }
-                    }
             r->persistWaves = bestW;
+            r->regen = uint32_t(bestR);
             r->blocksPerCU = bestA;
             r->chunk = uint32_t(bestB);
             r->useGraph = graph;

@@ -63,16 +63,16 @@ class Renderer:
     WAVEFRONT, PERSISTENT = 0, 1
 
     def set_mode(self, mode: int, blocks_per_cu: int = 0, chunk: int = 0, use_graph: bool = False,
-                 waves_per_simd: int = 0) -> None:
+                 waves_per_simd: int = 0, refill: int = 0) -> None:
         """The wavefront (0) or persistent batched (1) form of the camera loop; 0 keeps a shape value."""
         _lib.check(_lib.load().owe_hip_renderer_set_mode(self._h, int(mode), int(blocks_per_cu), int(chunk),
-                                                         int(use_graph), int(waves_per_simd)))
+                                                         int(use_graph), int(waves_per_simd), int(refill)))
 
     def mode(self) -> dict:
-        v = [ctypes.c_int() for _ in range(5)]
+        v = [ctypes.c_int() for _ in range(6)]
         _lib.check(_lib.load().owe_hip_renderer_get_mode(self._h, *[ctypes.byref(x) for x in v]))
         return dict(mode=v[0].value, blocks_per_cu=v[1].value, chunk=v[2].value, use_graph=bool(v[3].value),
-                    waves_per_simd=v[4].value)
+                    waves_per_simd=v[4].value, refill=v[5].value)
 
     def autotune(self, spp: int) -> float:
         """Picks the fastest launch shape of the current form over passes of spp; returns its ms."""

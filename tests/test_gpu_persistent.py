@@ -45,17 +45,18 @@ def render_with(sc, st, spp, passes, mode, **kw):
     return img, stats
 
 
-@pytest.mark.parametrize("waves", [2, 3])
+@pytest.mark.parametrize("waves,refill", [(2, 1), (3, 1), (2, 32)])
 @pytest.mark.parametrize("case", sorted(CASES))
-def test_persistent_form_equals_wavefront_bit_for_bit(cuda, case, waves):
-    # Both register budgets of the persistent kernel (3 waves per SIMD spills registers) compute
-    # every path exactly as the wavefront kernels do.
+def test_persistent_form_equals_wavefront_bit_for_bit(cuda, case, waves, refill):
+    # Both register budgets of the persistent kernel (3 waves per SIMD spills registers) and a
+    # refill threshold (finished lanes wait for half the wave) compute every path exactly as the
+    # wavefront kernels do.
     make, st, spp = CASES[case]
     sc = make()
     a, sa = render_with(sc, st, spp, 2, False)
-    b, sb = render_with(sc, st, spp, 2, True, blocks_per_cu=2, chunk=4, waves_per_simd=waves)
+    b, sb = render_with(sc, st, spp, 2, True, blocks_per_cu=2, chunk=4, waves_per_simd=waves, refill=refill)
     diff = np.abs(a - b)
-    print(f"{case} ({waves} waves/SIMD): max |diff| {diff.max():.3e}, pixels differing "
+    print(f"{case} ({waves} waves/SIMD, refill {refill}): max |diff| {diff.max():.3e}, pixels differing "
           f"{(diff > 0).any(axis=2).sum()}, stats {sa} / {sb}", flush=True)
     assert np.array_equal(a, b)
     assert sa == sb
