@@ -769,7 +769,7 @@ __global__ __launch_bounds__(kBlock, kWaves) void persistentKernel(DScene sc, Se
     const uint64_t passInc = (uint64_t(pass) << 1) | 1u;
     const uint32_t lane = __lane_id();
     ShadeCtx c{sc, set};
-    bool live = false, done = false;
+    bool live = false, done = false, pending = false;  // pending: a finished path not yet splatted
     uint64_t id = 0, end = 0;
     uint32_t accPix = 0xFFFFFFFFu, segments = 0;
     long long acc[3] = {0, 0, 0};
@@ -778,6 +778,20 @@ __global__ __launch_bounds__(kBlock, kWaves) void persistentKernel(DScene sc, Se
         // start new paths together: the wave runs the path set-up code once for many lanes.
         const uint64_t idle = __ballot(!live && !done);
         const bool refill = uint32_t(__popcll(idle)) >= a.regen || __ballot(live) == 0;
+        // Finished paths are splatted together at the refill, from the state they ended with.
+        if (refill && pending) {
+            pending = false;
+            float X, Y, Z;
+            if (pathXYZ(c.L, c.weight, c.wl, a.invNWY, X, Y, Z)) {
+                if (c.pixel != accPix) {
+                    flushFilm(a.film, accPix, acc);
+                    accPix = c.pixel;
+                }
+                acc[0] += fixedPoint(X);
+                acc[1] += fixedPoint(Y);
+                acc[2] += fixedPoint(Z);
+            }
+        }
         const bool needChunk = refill && !live && !done && id >= end;
         const uint64_t ask = __ballot(needChunk);
         if (ask) {
@@ -834,16 +848,7 @@ __global__ __launch_bounds__(kBlock, kWaves) void persistentKernel(DScene sc, Se
         c.depth += 1;
         if (!alive) {
             live = false;
-            float X, Y, Z;
-            if (pathXYZ(c.L, c.weight, c.wl, a.invNWY, X, Y, Z)) {
-                if (c.pixel != accPix) {
-                    flushFilm(a.film, accPix, acc);
-                    accPix = c.pixel;
-                }
-                acc[0] += fixedPoint(X);
-                acc[1] += fixedPoint(Y);
-                acc[2] += fixedPoint(Z);
-            }
+            pending = true;
         }
     }
     flushFilm(a.film, accPix, acc);
