@@ -69,8 +69,15 @@ def placement(position=(0.0, 0.0, 0.0), axis=(0.0, 0.0, 1.0), up=(0.0, 1.0, 0.0)
     return compose(S.transform(None, position), align_z(axis, up))
 
 
-def sag_of(curvature: float, conic: float, r: float) -> float:
-    return curvature * r * r / (1 + math.sqrt(max(0.0, 1 - (1 + conic) * curvature * curvature * r * r)))
+def sag_of(curvature: float, conic: float, r: float, asphere=()) -> float:
+    """z = c r² / (1 + √(1 − (1 + k) c² r²)) + Σ A₂ₘ r²ᵐ (m = 2, 3, …), the sag of SagSurface."""
+    z = curvature * r * r / (1 + math.sqrt(max(0.0, 1 - (1 + conic) * curvature * curvature * r * r)))
+    r2 = r * r
+    rp = r2
+    for a in asphere:
+        rp *= r2
+        z += a * rp
+    return z
 
 
 def design_simple_lens(f: float, n: float, t: float, form: str = "bi"):
@@ -85,16 +92,19 @@ def design_simple_lens(f: float, n: float, t: float, form: str = "bi"):
 
 
 def add_lens(sc: S.Scene, radii, semi_diameters, thicknesses, media, xf, ambient: int, rim_optics: int,
-             dielectric: int | None = None) -> list[int]:
+             dielectric: int | None = None, conics=None, aspheres=None, edge_radius: float = 0.0) -> list[int]:
     """buildLens: len(media) elements between len(media) + 1 surfaces (radii in metres, inf for
-    flat; semi-diameters; centre thicknesses); the body placed by xf, its vertex 0 at the origin and
-    axis +z. Returns the boundaries added."""
+    flat; semi-diameters; centre thicknesses; optional conic constants and asphere terms A4, A6, … in
+    SI units per surface); the body placed by xf, its vertex 0 at the origin and axis +z; the rim at
+    the largest of the semi-diameters and edge_radius. Returns the boundaries added."""
     n = len(media)
     if len(radii) != n + 1 or len(semi_diameters) != n + 1 or len(thicknesses) != n:
         raise ValueError("a lens needs n media, n thicknesses and n + 1 surfaces")
+    conics = [0.0] * (n + 1) if conics is None else list(conics)
+    aspheres = [()] * (n + 1) if aspheres is None else [tuple(a) for a in aspheres]
     if dielectric is None:
         dielectric = sc.add_optics(S.Optics(S.SURF_DIELECTRIC))
-    edge = max(semi_diameters)
+    edge = max(max(semi_diameters), edge_radius)
     regions = [sc.add_region(sc.add_medium(m)) for m in media]
     zv = [0.0]
     for t in thicknesses:
@@ -103,11 +113,11 @@ def add_lens(sc: S.Scene, radii, semi_diameters, thicknesses, media, xf, ambient
     for i in range(n + 1):
         c = 0.0 if math.isinf(radii[i]) else 1.0 / radii[i]
         sd = semi_diameters[i]
-        z_edge.append(zv[i] + sag_of(c, 0.0, sd))
+        z_edge.append(zv[i] + sag_of(c, conics[i], sd, aspheres[i]))
         front = regions[i] if i < n else ambient
         back = regions[i - 1] if i > 0 else ambient
-        out.append(sc.add_sag(c, 0.0, (), sd, 0.0, compose(xf, S.transform(None, (0, 0, zv[i]))), front, back,
-                              dielectric))
+        out.append(sc.add_sag(c, conics[i], aspheres[i], sd, 0.0, compose(xf, S.transform(None, (0, 0, zv[i]))),
+                              front, back, dielectric))
         if sd < edge:
             out.append(sc.add_plane(S.APERTURE_DISK, edge, sd, compose(xf, S.transform(None, (0, 0, z_edge[i]))),
                                     front, back, rim_optics))
@@ -119,20 +129,20 @@ def add_lens(sc: S.Scene, radii, semi_diameters, thicknesses, media, xf, ambient
 
 
 def add_mirror(sc: S.Scene, radius: float, conic: float, semi_diameter: float, thickness: float, optics: int, xf,
-               ambient: int, hole: float = 0.0) -> list[int]:
+               ambient: int, hole: float = 0.0, asphere=()) -> list[int]:
     """buildMirror: the reflecting sag face (normal +z, the substrate behind it on −z), a black
     back disc and rim (and bore, with a central hole) around an opaque substrate."""
     sub = sc.add_region(sc.add_medium(S.Medium(S.INDEX_CONSTANT, (1.0,), opaque=True)))
     black = sc.add_optics(S.Optics(S.SURF_ABSORBER))
     c = 0.0 if math.isinf(radius) else 1.0 / radius
-    out = [sc.add_sag(c, conic, (), semi_diameter, hole, xf, ambient, sub, optics)]
-    z_edge = sag_of(c, conic, semi_diameter)
+    out = [sc.add_sag(c, conic, tuple(asphere), semi_diameter, hole, xf, ambient, sub, optics)]
+    z_edge = sag_of(c, conic, semi_diameter, asphere)
     z_back = min(0.0, z_edge) - thickness
     flip = S.transform(S.rotation((1, 0, 0), math.pi), (0, 0, z_back))
     out.append(sc.add_plane(S.APERTURE_DISK, semi_diameter, hole, compose(xf, flip), ambient, sub, black))
     out.append(sc.add_cylinder(semi_diameter, z_back, z_edge, xf, ambient, sub, black))
     if hole > 0:
-        out.append(sc.add_cylinder(hole, z_back, sag_of(c, conic, hole), xf, sub, ambient, black))
+        out.append(sc.add_cylinder(hole, z_back, sag_of(c, conic, hole, asphere), xf, sub, ambient, black))
     return out
 
 
@@ -422,8 +432,10 @@ def add_prescription(sc: S.Scene, p, xf, ambient: int, mirror_optics: int, rim_o
     """buildPrescription (src/owe/scene/prescription.cpp:354-432): a lens prescription (owe_hip.lens,
     millimetres) as matter placed by xf — each mirror row a mirror body (its frame turned over when
     met travelling +z, so the face meets the light; substrate, rim, bore, spider struts behind it),
-    each group of glass rows a lens with rims, each stop row in air a black annulus. Returns the
-    vertex positions (metres, along the body's z) and the index of the last vertex."""
+    each group of glass rows a lens with rims (and, inside a tube, a black mount from its rim to the
+    tube), each stop row in air a black annulus. ``mirror_optics`` is an optics index, or a function
+    of a mirror row's material name. Returns the vertex positions (metres, along the body's z), the
+    index of the last vertex, and the rear group's nearest z and edge radius."""
     from . import lens as Lm
 
     if media_of is None:
@@ -436,6 +448,7 @@ def add_prescription(sc: S.Scene, p, xf, ambient: int, mirror_optics: int, rim_o
             return S.Medium.sellmeier(B, C, True)
 
     rows = p.surfaces
+    rear_nearest, rear_edge = 0.0, 0.0
     z = [0.0]
     for r in rows[:-1]:
         z.append(z[-1] + r.t * 1e-3)
@@ -450,8 +463,10 @@ def add_prescription(sc: S.Scene, p, xf, ambient: int, mirror_optics: int, rim_o
             if direction > 0:
                 at = compose(at, S.transform(S.rotation((1, 0, 0), math.pi)))
             substrate = getattr(r, "substrate", 0.0) or 4.0
-            add_mirror(sc, R, r.k, r.sd * 1e-3, substrate * 1e-3, mirror_optics, compose(xf, at), ambient,
-                       hole=r.hole * 1e-3)
+            optics = mirror_optics(getattr(r, "material", "")) if callable(mirror_optics) else mirror_optics
+            A = tuple((-a if direction > 0 else a) for a in r.A)
+            add_mirror(sc, R, r.k, r.sd * 1e-3, substrate * 1e-3, optics, compose(xf, at), ambient,
+                       hole=r.hole * 1e-3, asphere=A)
             for k in range(getattr(r, "spider_count", 0)):
                 a = getattr(r, "spider_angle", 0.0) + 2 * math.pi * k / r.spider_count
                 outer = r.spider_outer * 1e-3
@@ -475,12 +490,27 @@ def add_prescription(sc: S.Scene, p, xf, ambient: int, mirror_optics: int, rim_o
         j = i
         while j < len(rows) and rows[j].medium != p.ambient:
             j += 1
-        radii = [(math.inf if math.isinf(q.R) else q.R * 1e-3) for q in rows[i:j + 1]]
-        add_lens(sc, radii, [q.sd * 1e-3 for q in rows[i:j + 1]], [q.t * 1e-3 for q in rows[i:j]],
-                 [media_of(q.medium) for q in rows[i:j]], compose(xf, S.transform(None, (0, 0, z[i]))), ambient,
-                 rim_optics)
+        if j >= len(rows):
+            raise ValueError("prescription: last surface must exit into the ambient medium")
+        group = rows[i:j + 1]
+        radii = [(math.inf if math.isinf(q.R) else q.R * 1e-3) for q in group]
+        sds = [q.sd * 1e-3 for q in group]
+        add_lens(sc, radii, sds, [q.t * 1e-3 for q in rows[i:j]], [media_of(q.medium) for q in rows[i:j]],
+                 compose(xf, S.transform(None, (0, 0, z[i]))), ambient, rim_optics,
+                 conics=[q.k for q in group], aspheres=[q.A for q in group])
+        edge = max(sds)
+        c0 = 0.0 if math.isinf(radii[0]) else 1 / radii[0]
+        c1 = 0.0 if math.isinf(radii[-1]) else 1 / radii[-1]
+        z_first = z[i] + sag_of(c0, group[0].k, sds[0], group[0].A)
+        z_last = z[j] + sag_of(c1, group[-1].k, sds[-1], group[-1].A)
+        rear_nearest, rear_edge = max(z[j], z_last, z[i], z_first), edge
+        if tube_radius > edge:
+            # The mount: an annulus from just outside the rim (1e-9 relative, so the two bodies share
+            # no edge) to the tube, halfway between the group's edges.
+            sc.add_plane(S.APERTURE_DISK, tube_radius, edge * (1 + 1e-9),
+                         compose(xf, S.transform(None, (0, 0, 0.5 * (z_first + z_last)))), ambient, ambient, black)
         i = j + 1
-    return dict(vertex_z=z, last=len(rows) - 1)
+    return dict(vertex_z=z, last=len(rows) - 1, rear_nearest_z=rear_nearest, rear_edge_radius=rear_edge)
 
 
 @functools.lru_cache(maxsize=2)

@@ -14,6 +14,7 @@
 from __future__ import annotations
 
 import ctypes
+import dataclasses
 import math
 from dataclasses import dataclass, field
 
@@ -92,6 +93,13 @@ class Surface:
     reflect: bool = False     # a mirror row
     hole: float = 0.0         # mirror: central hole radius, mm
     solve: bool = False       # the gap solve_afocal adjusts
+    material: str = ""        # mirror: the scene material of its face
+    substrate: float = 0.0    # mirror: axial depth of its backing, mm
+    spider_count: int = 0     # mirror: opaque struts behind it
+    spider_width: float = 0.0
+    spider_angle: float = 0.0  # radians
+    spider_outer: float = 0.0
+    label: str = ""
 
 
 @dataclass
@@ -101,9 +109,11 @@ class Prescription:
     rays: int = 4096
     name: str = ""
     rings: int = 0            # > 0: spotAt's hexapolar pupil rings (then rays = 1 + 3 r (r + 1))
+    wavelength: float = 587.5618   # design wavelength for paraxial data, nm
+    object_distance: float = math.inf  # mm
 
     def with_rings(self, rings: int) -> "Prescription":
-        return Prescription(self.surfaces, self.ambient, 1 + 3 * rings * (rings + 1), self.name, rings)
+        return dataclasses.replace(self, rays=1 + 3 * rings * (rings + 1), rings=rings)
 
 
 SINGLET = Prescription([Surface(50.0, 8.0, "N-BK7", 12.5), Surface(-50.0, 40.0, "air", 12.5)], name="biconvex singlet")
@@ -158,48 +168,283 @@ PARAM_CURVATURE, PARAM_CONIC, PARAM_THICKNESS, PARAM_A4 = 0, 1, 2, 3
 MOMENTS = 9
 
 
-def system_matrix(p: Prescription, lam: float):
-    """The paraxial (y, n u) matrix (A, B, C, D) of paraxialAnalysis (prescription.cpp:193-230): a
-    refracting row's power (n' − n) c, a mirror's −2 dir n c (dir the travel, which it reverses),
-    the gaps unfolded."""
-    a, b, c, d = 1.0, 0.0, 0.0, 1.0
-    n_prev, direction = index(p.ambient, lam), 1.0
-    for i, sf in enumerate(p.surfaces):
+_LENS_UNITS = {"m": 1e3, "cm": 10.0, "mm": 1.0, "um": 1e-3, "nm": 1e-6, "km": 1e6, "in": 25.4}
+
+
+def _lens_number(tok: str, ctx: str) -> float:
+    if tok in ("inf", "infinity", "INF", "Infinity"):
+        return math.inf
+    if tok == "-inf":
+        return -math.inf
+    try:
+        return float(tok)
+    except ValueError:
+        raise ValueError(f"{ctx}: expected a number, got '{tok}'") from None
+
+
+def _lens_option(x: str, ctx: str, unit: float, s: Surface) -> None:
+    if x in ("stop", "reflect", "solve"):
+        setattr(s, x, True)
+        return
+    if "=" not in x:
+        raise ValueError(f"{ctx}: unexpected token '{x}'")
+    k, v = x.split("=", 1)
+    if k == "material":
+        s.material = v
+    elif k == "out_axis":
+        raise ValueError(f"{ctx}: out_axis (folded, non-coaxial mirrors) is not supported")
+    elif k == "support":
+        if not (v.startswith("spider(") and v.endswith(")")):
+            raise ValueError(f"{ctx}: support must be spider(...)")
+        for kv in v[7:-1].split(","):
+            if "=" not in kv:
+                raise ValueError(f"{ctx}: spider takes name=value pairs")
+            n, val = kv.split("=", 1)
+            val = _lens_number(val, ctx)
+            if n == "count":
+                s.spider_count = int(val)
+            elif n == "width":
+                s.spider_width = val * unit
+            elif n == "angle":
+                s.spider_angle = math.radians(val)
+            elif n == "outer_radius":
+                s.spider_outer = val * unit
+            else:
+                raise ValueError(f"{ctx}: unknown spider parameter '{n}'")
+        if s.spider_count < 1 or not s.spider_width > 0 or not s.spider_outer > 0:
+            raise ValueError(f"{ctx}: spider needs count >= 1, width > 0 and outer_radius > 0")
+    else:
+        val = _lens_number(v, ctx)
+        if k == "substrate":
+            s.substrate = val * unit
+        elif k == "hole":
+            s.hole = val * unit / 2       # the file gives the bore's diameter
+        elif k == "k":
+            s.k = val
+        elif len(k) > 1 and k[0] == "A" and k[1:].isdigit():
+            order = int(k[1:])
+            if order < 4 or order % 2:
+                raise ValueError(f"{ctx}: aspheric terms are A4, A6, A8, ...")
+            slot = (order - 4) // 2
+            A = list(s.A) + [0.0] * max(0, slot + 1 - len(s.A))
+            A[slot] = val * (unit * 1e-3) ** (1 - order)   # coefficient of r^order in SI units
+            s.A = tuple(A)
+        else:
+            raise ValueError(f"{ctx}: unknown key '{k}'")
+
+
+def parse_prescription(text: str, origin: str = "<string>") -> Prescription:
+    """A ``.lens`` table (src/owe/scene/prescription.cpp:97-140): ``name``, ``units``, ``ambient``,
+    ``object_distance``, ``wavelength`` lines and one row per surface — label, R, t, the medium after
+    it, semi-diameter, then options (``k=``, ``A4=``…, ``stop``, ``reflect``, ``material=``,
+    ``substrate=``, ``hole=`` (a diameter), ``support=spider(…)``, ``solve``). Lengths are converted to
+    millimetres, asphere terms to SI units; a mirror system's legs are checked (t signed with the
+    travel, glass only on +z legs)."""
+    p = Prescription([])
+    unit = 1.0
+    for n, raw in enumerate(text.splitlines(), 1):
+        line = raw.split("#", 1)[0]
+        tok = line.split()
+        if not tok:
+            continue
+        ctx = f"{origin}:{n}"
+        key = tok[0]
+        if key == "name":
+            p.name = line[line.index("name") + 4:].strip().strip('"').strip()
+        elif key == "units":
+            if len(tok) < 2 or tok[1] not in _LENS_UNITS:
+                raise ValueError(f"{ctx}: unknown unit")
+            unit = _LENS_UNITS[tok[1]]
+        elif key == "ambient":
+            p.ambient = tok[1]
+        elif key == "object_distance":
+            p.object_distance = _lens_number(tok[1], ctx) * unit
+        elif key == "wavelength":
+            p.wavelength = _lens_number(tok[1], ctx)
+        else:
+            if len(tok) < 5:
+                raise ValueError(f"{ctx}: surface lines need: label R t medium semi-diameter")
+            s = Surface(_lens_number(tok[1], ctx) * unit, _lens_number(tok[2], ctx) * unit, tok[3],
+                        _lens_number(tok[4], ctx) * unit, label=tok[0])
+            for x in tok[5:]:
+                _lens_option(x, ctx, unit, s)
+            if math.isinf(s.t):
+                raise ValueError(f"{ctx}: thickness must be finite")
+            if s.reflect and (not s.material or not s.substrate > 0):
+                raise ValueError(f"{ctx}: a reflecting row needs material= and substrate= > 0")
+            if s.reflect and s.hole >= s.sd:
+                raise ValueError(f"{ctx}: the hole must be smaller than the mirror")
+            p.surfaces.append(s)
+    if not p.surfaces:
+        raise ValueError(f"{origin}: prescription has no surfaces")
+    if any(s.reflect for s in p.surfaces):
+        direction = 1.0
+        for i, s in enumerate(p.surfaces):
+            if s.reflect:
+                if s.medium != p.ambient:
+                    raise ValueError(f"{origin}: {s.label}: a mirror reflects into the ambient medium")
+                direction = -direction
+            elif direction < 0:
+                raise ValueError(f"{origin}: {s.label}: refraction on a leg travelling -z is not supported")
+            if i + 1 < len(p.surfaces) and s.t * direction <= 0:
+                raise ValueError(f"{origin}: {s.label}: t must have the sign of the travel after this row")
+    return p
+
+
+def load_prescription(path: str) -> Prescription:
+    with open(path) as f:
+        return parse_prescription(f.read(), path)
+
+
+def _unfolded(p: Prescription, lam: float, index_fn):
+    """Per surface: curvature (1/m), index after, power, gap to the next vertex along the travel (m),
+    and the travel direction after the last surface (makeSystem, prescription.cpp:197-215)."""
+    n_prev, direction = index_fn(p.ambient, lam), 1.0
+    rows = []
+    for sf in p.surfaces:
         cur = 0.0 if math.isinf(sf.R) else 1.0 / (sf.R * 1e-3)
-        n = index(sf.medium, lam)
+        n = index_fn(sf.medium, lam)
         phi = -2 * direction * n * cur if sf.reflect else (n - n_prev) * cur
         if sf.reflect:
             direction = -direction
-        c, d = c - phi * a, d - phi * b
-        if i + 1 < len(p.surfaces):
-            k = abs(sf.t * 1e-3) / n
-            a, b = a + k * c, b + k * d
+        rows.append((n_prev, n, phi, abs(sf.t * 1e-3)))
         n_prev = n
+    return rows, direction
+
+
+def system_matrix(p: Prescription, lam: float, index_fn=None):
+    """The paraxial (y, n u) matrix (A, B, C, D) of paraxialAnalysis (prescription.cpp:193-230): a
+    refracting row's power (n' − n) c, a mirror's −2 dir n c (dir the travel, which it reverses),
+    the gaps unfolded."""
+    rows, _ = _unfolded(p, lam, index_fn or index)
+    a, b, c, d = 1.0, 0.0, 0.0, 1.0
+    for i, (_, n, phi, t) in enumerate(rows):
+        c, d = c - phi * a, d - phi * b
+        if i + 1 < len(rows):
+            k = t / n
+            a, b = a + k * c, b + k * d
     return a, b, c, d
 
 
-def solve_afocal(p: Prescription, lam: float = LAMBDA_D) -> Prescription:
-    """solveAfocal (prescription.cpp:324-352): the marked gap set so the system's power C vanishes
-    (C is affine in the gap), its sign (the leg's travel) kept."""
-    gaps = [i for i, sf in enumerate(p.surfaces) if sf.solve]
-    if len(gaps) != 1:
-        raise ValueError("mark exactly one gap with solve")
-    g = gaps[0]
-    old = p.surfaces[g].t
+def stop_index(p: Prescription) -> int:
+    """The last row marked stop, or −1."""
+    stop = -1
+    for i, sf in enumerate(p.surfaces):
+        if sf.stop:
+            stop = i
+    return stop
+
+
+def paraxial(p: Prescription, lam: float | None = None, index_fn=None) -> dict:
+    """paraxialAnalysis (prescription.cpp:232-322), lengths in metres along the fixed chart: the system
+    matrix; efl, bfl, ffl, or the angular magnification when afocal; the image distance of the object;
+    the aperture stop (marked, else the surface that limits a marginal ray from the object); the exit
+    pupil (the stop imaged by the surfaces after it: its z from the first vertex and radius) and the
+    entrance pupil (imaged backwards by the surfaces before it); the f-number."""
+    index_fn = index_fn or index
+    lam = p.wavelength if lam is None else lam
+    rows, dir_last = _unfolded(p, lam, index_fn)
+    N = len(rows)
+    A, B, C, D = system_matrix(p, lam, index_fn)
+    n0, nk = rows[0][0], rows[-1][1]
+    length = sum(r[3] for r in rows[:-1])
+    z_chart = [0.0]
+    for sf in p.surfaces[:-1]:
+        z_chart.append(z_chart[-1] + sf.t * 1e-3)
+    scale = 1e-12 * (1 + abs(A) + abs(D))
+    out = dict(A=A, B=B, C=C, D=D, length=length, efl=0.0, bfl=0.0, ffl=0.0, angular_magnification=0.0,
+               exit_pupil_z=0.0, exit_pupil_radius=0.0, entrance_pupil_z=0.0, entrance_pupil_radius=0.0, f_number=0.0)
+    out["afocal"] = abs(C) * max(1.0, length) < scale * 1e3 or abs(C) < 1e-9
+    if not out["afocal"]:
+        out["efl"], out["bfl"], out["ffl"] = -nk / C, -nk * A / C, n0 * D / C
+    else:
+        out["angular_magnification"] = D * n0 / nk
+    obj = p.object_distance * 1e-3
+    if math.isinf(obj):
+        out["image_distance"] = math.inf if out["afocal"] else out["bfl"]
+    else:
+        u = 1e-3
+        y0 = obj * u
+        yk, nuk = A * y0 + B * n0 * u, C * y0 + D * n0 * u
+        out["image_distance"] = math.inf if nuk == 0 else -yk * nk / nuk
+    stop = stop_index(p)
+    if stop < 0:
+        best, y = -1.0, 1.0
+        nu = 0.0 if math.isinf(obj) else n0 / obj
+        for i in range(N):
+            ratio = abs(y) / max(p.surfaces[i].sd * 1e-3, 1e-12)
+            if ratio > best:
+                best, stop = ratio, i
+            nu -= rows[i][2] * y
+            if i + 1 < N:
+                y += rows[i][3] * nu / rows[i][1]
+    out["stop_index"] = stop
+    stop_r = p.surfaces[stop].sd * 1e-3
+    y, nu, ym, num = 0.0, 1e-3 * rows[stop][1], stop_r, 0.0
+    for i in range(stop + 1, N):
+        y += rows[i - 1][3] * nu / rows[i - 1][1]
+        ym += rows[i - 1][3] * num / rows[i - 1][1]
+        nu -= rows[i][2] * y
+        num -= rows[i][2] * ym
+    u = nu / nk
+    if stop == N - 1:
+        out["exit_pupil_z"], out["exit_pupil_radius"] = z_chart[-1], stop_r
+    elif u != 0:
+        dz = -y / u
+        out["exit_pupil_z"] = z_chart[-1] + dir_last * dz
+        out["exit_pupil_radius"] = abs(ym + dz * num / nk)
+    y, nu, ym, num = 0.0, 1e-3 * rows[stop][0], stop_r, 0.0
+    for i in range(stop - 1, -1, -1):
+        y -= rows[i][3] * nu / rows[i][1]
+        ym -= rows[i][3] * num / rows[i][1]
+        nu += rows[i][2] * y
+        num += rows[i][2] * ym
+    if stop == 0:
+        out["entrance_pupil_z"], out["entrance_pupil_radius"] = 0.0, stop_r
+    else:
+        u = nu / n0
+        if u != 0:
+            dz = -y / u
+            out["entrance_pupil_z"], out["entrance_pupil_radius"] = dz, abs(ym + dz * num / n0)
+    if not out["afocal"] and out["entrance_pupil_radius"] > 0:
+        out["f_number"] = abs(out["efl"]) / (2 * out["entrance_pupil_radius"])
+    return out
+
+
+def solve_afocal(p: Prescription, lam: float = LAMBDA_D, index_fn=None) -> Prescription:
+    """solveAfocal (prescription.cpp:324-352): the gap marked solve (a mirror system needs exactly
+    one), else the largest air gap, set so the system's power C vanishes (C is affine in the gap),
+    its sign (the leg's travel) kept."""
+    rows = p.surfaces
+    marked = [i for i, sf in enumerate(rows[:-1]) if sf.solve]
+    if len(marked) > 1:
+        raise ValueError("solveAfocal: more than one gap is marked solve")
+    if not marked and any(sf.reflect for sf in rows):
+        raise ValueError("solveAfocal: mark the gap to adjust with solve")
+    if marked:
+        g = marked[0]
+    else:
+        air = [i for i, sf in enumerate(rows[:-1]) if sf.medium == p.ambient]
+        if not air:
+            raise ValueError("solveAfocal: no air gap to adjust")
+        g = max(air, key=lambda i: (rows[i].t, -i))
+    old = rows[g].t
     sign = -1.0 if old < 0 else 1.0
 
     def power(t):
-        q = [Surface(**{**sf.__dict__}) for sf in p.surfaces]
+        q = [dataclasses.replace(sf) for sf in rows]
         q[g].t = t
-        return system_matrix(Prescription(q, p.ambient, p.rays, p.name, p.rings), lam)[2]
+        return system_matrix(dataclasses.replace(p, surfaces=q), lam, index_fn)[2]
 
     c0, c1 = power(0.0), power(sign)
+    if c1 == c0:
+        raise ValueError("solveAfocal: gap does not affect power")
     t = -c0 / (c1 - c0)
     if not t > 0:
-        raise ValueError("no positive afocal separation")
-    q = [Surface(**{**sf.__dict__}) for sf in p.surfaces]
+        raise ValueError("solveAfocal: no positive afocal separation")
+    q = [dataclasses.replace(sf) for sf in rows]
     q[g].t = sign * t
-    return Prescription(q, p.ambient, p.rays, p.name, p.rings)
+    return dataclasses.replace(p, surfaces=q)
 
 
 def afocal_spot(m) -> dict:
