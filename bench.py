@@ -653,6 +653,25 @@ def bench_telescope_render(args, variants_rendered: int = 4):
     return out
 
 
+def bench_scenes(args):
+    """The scene-file suite (owe bench): the package's example scenes read from .owe files, one pass of
+    16 camera paths per pixel at 256x192 each in the persistent form, best of 3 fresh renderers; the
+    aggregate paths/s and each scene's digest (FNV-1a of its raw XYZ image)."""
+    from owe_hip import render, suite
+
+    res = suite.run(spp=16, resolution=(256, 192), repeat=3, mode=render.Renderer.PERSISTENT)
+    for r in res["rows"]:
+        print(f"scenes: {Path(r['scene']).name:<20s} {r['mpaths_per_s']:9.1f} M paths/s  {r['msegs_per_s']:9.1f} "
+              f"M segments/s  mean Y {r['mean_Y']:.6g}  digest {r['digest']}", file=sys.stderr, flush=True)
+    return {"value": res["mpaths_per_s"] * 1e6, "unit": "paths/s (suite aggregate)",
+            "config": {"workload": "scene files through the loader: prism, lens caustic through glass, caustic in a "
+                                   "mirror, rippled pool, slabs behind a stop, refractor at its exit pupil",
+                       "resolution": [256, 192], "spp": 16, "form": "persistent", "timing": "best of 3 passes"},
+            "rows": [{k: r[k] for k in ("scene", "detector", "mpaths_per_s", "msegs_per_s", "mean_Y", "digest",
+                                        "inconsistencies", "leaks")} | {"scene": Path(r["scene"]).name}
+                     for r in res["rows"]]}
+
+
 def main() -> int:
     import torch  # noqa: F401  (first: its HIP runtime is the process's; see owe_hip/_lib.py)
 
@@ -661,12 +680,13 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="prism", choices=["prism", "rayquery", "light", "lens", "caustic",
-                                                             "telescope"])
+                                                             "telescope", "scenes"])
     ap.add_argument("--no-rayquery", action="store_true", help="prism: skip the ray-query record beside it")
     ap.add_argument("--no-light", action="store_true", help="prism: skip the light-tracing record beside it")
     ap.add_argument("--no-lens", action="store_true", help="prism: skip the lens-analysis record beside it")
     ap.add_argument("--no-caustic", action="store_true", help="prism: skip the caustic-gradient record beside it")
     ap.add_argument("--no-telescope", action="store_true", help="prism: skip the telescope-gradient record beside it")
+    ap.add_argument("--no-scenes", action="store_true", help="prism: skip the scene-file suite record")
     ap.add_argument("--no-telescope-render", action="store_true",
                     help="telescope: skip the rendering of the telescopes in the room")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -709,6 +729,10 @@ def main() -> int:
             result["telescope"] = bench_telescope(ns)
             if not args.no_telescope_render:
                 result["telescope"]["render"] = bench_telescope_render(ns)
+        if not args.no_scenes:
+            result["scenes"] = bench_scenes(args)
+    elif args.workload == "scenes":
+        result = bench_scenes(args)
     elif args.workload == "caustic":
         result = bench_caustic(args)
     elif args.workload == "telescope":

@@ -76,3 +76,15 @@ def test_loaded_scene_forms_are_bit_identical(cuda, name):
     print(f"{name}: digests {digests}, mean Y {imgs[0][..., 1].mean():.6g}", flush=True)
     assert np.array_equal(imgs[0], imgs[1])
     assert digests[0] == digests[1]
+
+
+def test_suite_digests_are_reproducible(cuda):
+    from owe_hip import suite
+
+    runs = [suite.run(spp=4, resolution=(64, 48), mode=m) for m in
+            (render.Renderer.PERSISTENT, render.Renderer.PERSISTENT, render.Renderer.WAVEFRONT)]
+    for rows in zip(*(r["rows"] for r in runs)):
+        print(f"{Path(rows[0]['scene']).name}: {rows[0]['digest']} {rows[0]['mpaths_per_s']:.1f} M paths/s", flush=True)
+        assert rows[0]["digest"] == rows[1]["digest"] == rows[2]["digest"]
+        assert rows[0]["paths"] == 64 * 48 * 4 and rows[0]["inconsistencies"] == 0
+    print(suite.format_table(runs[0]), flush=True)

@@ -301,3 +301,17 @@ def test_white_rgb_is_flat_and_colours_reproduce():
     red = LD.rgb_spectrum(0.8, 0.1, 0.1, False)
     lam, val = zip(*red.table)
     assert np.interp(650, lam, val) > 0.5 and np.interp(450, lam, val) < 0.2
+
+
+def test_suite_specs_and_table():
+    from owe_hip import suite
+
+    assert suite.split_spec("scenes/the_study.owe:Cam") == ("scenes/the_study.owe", "Cam")
+    assert suite.split_spec("a/b.owe") == ("a/b.owe", "")
+    assert suite.split_spec("C:/x/b.owe") == ("C:/x/b.owe", "")
+    rows = [dict(scene="x/p.owe", detector="Eye", resolution="8x6", spp=2, mpaths_per_s=1.5, msegs_per_s=3.0,
+                 seconds=0.01, mean_Y=0.25, digest="0123456789abcdef")]
+    t = suite.format_table(dict(rows=rows, total_seconds=0.01, mpaths_per_s=1.5))
+    assert "p.owe" in t and "0123456789abcdef" in t and t.splitlines()[-1].startswith("total: 0.010 s")
+    for path, _ in suite.DEFAULT_CASES:
+        assert LD.load_scene(path).scene.c() is not None
