@@ -55,6 +55,7 @@ PROFILE_PRISM_TRAFFIC = ROOT / "profiles" / "prism_camera_paths_traffic.md"
 PROFILE_LIGHT_TRAFFIC = ROOT / "profiles" / "light_particles_traffic.md"
 PROFILE_LENS_TRAFFIC = ROOT / "profiles" / "lens_batch_traffic.md"
 PROFILE_CAUSTIC_TRAFFIC = ROOT / "profiles" / "caustic_adjoint_traffic.md"
+PROFILE_COUNTERS = ROOT / "profiles" / "occupancy_valu_l2_counters.md"
 
 
 def log(msg: str) -> None:
@@ -72,6 +73,17 @@ def measured_traffic(path: Path, kernel: str | None = None):
         return float(m.group(1)) if m else None
     m = re.search(r"traffic_bytes_per_launch(?:\[[^\]]*\])?:\s*([0-9.eE+]+)", text)
     return float(m.group(1)) if m else None
+
+
+def measured_counters(kernel: str):
+    """Occupancy, VALU and L2 figures of `kernel` recorded by the committed rocprofv3 counter summary
+    (a line `counters[kernel]: name=value ...`), if present."""
+    if not PROFILE_COUNTERS.exists():
+        return None
+    m = re.search(r"counters\[" + re.escape(kernel) + r"\]:([^\n]*)", PROFILE_COUNTERS.read_text())
+    if not m:
+        return None
+    return {k: float(v) for k, v in re.findall(r"(\w+)=([0-9.eE+-]+)", m.group(1))}
 
 
 def cpu_baseline_rayquery(pos, tri, rays, seconds_budget: float = 15.0):
@@ -221,6 +233,7 @@ def bench_prism(args):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(PROFILE_PRISM_TRAFFIC, f"{dominant}Kernel"),
+                     "counters": measured_counters(f"{dominant}Kernel"),
                      "kernel": f"{dominant}Kernel", "algorithmic_bytes_per_launch": dom_bytes / max(1, dom_launches),
                      "pass": {"algorithmic_bytes": pass_bytes / args.steps, "kernel_ms": total_ms / args.steps,
                               "achieved_GBs": pass_bytes / (total_ms * 1e-3) / 1e9}},
@@ -297,6 +310,7 @@ def bench_rayquery(args):
                    "closest_rays_per_s": n / (closest_ms * 1e-3), "any_rays_per_s": n / (any_ms * 1e-3)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(PROFILE_TRAFFIC, "traceBatchKernel<closest>"),
+                     "counters": measured_counters("traceBatchKernel<closest>"),
                      "kernel": "traceBatchKernel<closest>", "algorithmic_bytes_per_launch": algo_bytes},
     }
     if not args.no_cpu_baseline:
