@@ -275,6 +275,43 @@ def bench_rayquery(args):
         if ev is not None:
             ev[2].record(stream)
 
+    # Launch forms: one lane per ray, and persistent lanes with refill over a sweep of shapes; the
+    # fastest closest + any pair is kept. Both forms must return the same hits bit for bit.
+    def pair_ms(reps=3):
+        for _ in range(2):
+            step()
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(reps)]
+        for k in range(reps):
+            step(ev[k])
+        torch.cuda.synchronize()
+        return (sum(e[0].elapsed_time(e[1]) for e in ev) / reps, sum(e[1].elapsed_time(e[2]) for e in ev) / reps)
+
+    mesh.set_launch(rayquery.Mesh.PER_LANE)
+    per_lane = pair_ms()
+    ref_prim, ref_t = prim.clone(), tt.clone()
+    ref_prim2, ref_t2 = prim2.clone(), tt2.clone()
+    shapes = []
+    for bpc in (6, 8, 10):
+        for refill in (1, 8, 16, 32):
+            mesh.set_launch(rayquery.Mesh.REFILL, bpc, refill)
+            shapes.append((pair_ms(), bpc, refill))
+    # Each query mode keeps its own fastest form and shape.
+    launch = {}
+    refill_ms = [0.0, 0.0]
+    for q, name in ((_lib.QUERY_CLOSEST, "closest"), (_lib.QUERY_ANY, "any")):
+        ms, bpc, refill = min(shapes, key=lambda x: x[0][q])
+        refill_ms[q] = ms[q]
+        if ms[q] < per_lane[q]:
+            mesh.set_launch(rayquery.Mesh.REFILL, bpc, refill, mode=q)
+        else:
+            mesh.set_launch(rayquery.Mesh.PER_LANE, mode=q)
+        launch[name] = mesh.launch(q) | {"refill_ms": ms[q], "refill_blocks_per_cu": bpc, "refill_refill": refill}
+    step()
+    torch.cuda.synchronize()
+    identical = bool(torch.equal(prim, ref_prim) and torch.equal(tt, ref_t) and torch.equal(prim2, ref_prim2)
+                     and torch.equal(tt2, ref_t2))
+    log(f"rayquery: per lane closest {per_lane[0]:.3f} + any {per_lane[1]:.3f} ms; best refill closest "
+        f"{refill_ms[0]:.3f} + any {refill_ms[1]:.3f} ms; identical {identical}; using {launch}")
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -291,6 +328,7 @@ def bench_rayquery(args):
     log(f"rayquery: closest {closest_ms:.3f} ms, any {any_ms:.3f} ms per launch, wall {wall:.3f}s, hit {hits / n:.4f}")
     algo_bytes = RQ_BYTES_PER_QUERY * n
     achieved = algo_bytes / (closest_ms * 1e-3) / 1e9
+    kernel = "traceRefillKernel" if launch["closest"]["form"] == rayquery.Mesh.REFILL else "traceBatchKernel"
     result = {
         "metric": "spectral paths/s at fixed rel. RMSE vs fp64 ref; adjoint grads/s; HBM GB/s",
         "value": 2 * n * args.steps / wall,
@@ -307,11 +345,15 @@ def bench_rayquery(args):
         "config": {"workload": "batch ray queries on a BVH: closest + any hit", "triangles": int(st.triangles),
                    "rays": n, "mesh": "icosphere frequency 229, radial value noise 0.02", "bvh": "4-wide SAH",
                    "closest_ms": closest_ms, "any_ms": any_ms,
-                   "closest_rays_per_s": n / (closest_ms * 1e-3), "any_rays_per_s": n / (any_ms * 1e-3)},
+                   "closest_rays_per_s": n / (closest_ms * 1e-3), "any_rays_per_s": n / (any_ms * 1e-3),
+                   "launch": launch,
+                   "forms": {"per_lane": {"closest_ms": per_lane[0], "any_ms": per_lane[1]},
+                             "refill_best": {"closest_ms": refill_ms[0], "any_ms": refill_ms[1]},
+                             "bit_identical": identical}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(PROFILE_TRAFFIC, "traceBatchKernel<closest>"),
-                     "counters": measured_counters("traceBatchKernel<closest>"),
-                     "kernel": "traceBatchKernel<closest>", "algorithmic_bytes_per_launch": algo_bytes},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(PROFILE_TRAFFIC, f"{kernel}<closest>"),
+                     "counters": measured_counters(f"{kernel}<closest>"),
+                     "kernel": f"{kernel}<closest>", "algorithmic_bytes_per_launch": algo_bytes},
     }
     if not args.no_cpu_baseline:
         log("rayquery: timing the CPU baseline")

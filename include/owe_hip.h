@@ -91,6 +91,18 @@ int owe_hip_mesh_stats_get(const owe_hip_mesh* mesh, owe_hip_mesh_stats* out);
 int owe_hip_trace_batch(owe_hip_mesh* mesh, int mode, const float* rays, uint64_t n, uint32_t* prim, float* t,
                         void* stream);
 
+/* Launch form of owe_hip_trace_batch per query mode (both forms return the same hits bit for bit):
+ * OWE_HIP_TRACE_PER_LANE — one lane per ray, a wave lives until its longest ray ends;
+ * OWE_HIP_TRACE_REFILL (default) — a persistent grid of blocks_per_cu blocks per compute unit whose
+ * waves each own a contiguous range of rays; lanes whose rays ended take new ones together once
+ * `refill` lanes of the wave are idle. mode: OWE_HIP_QUERY_CLOSEST, OWE_HIP_QUERY_ANY, or −1 for
+ * both; 0 keeps a shape value (blocks_per_cu ≤ 10, refill ≤ 64). No counterpart in the reference:
+ * the batch query is this backend's own stand-in for many MeshShape::intersect calls. */
+#define OWE_HIP_TRACE_PER_LANE 0
+#define OWE_HIP_TRACE_REFILL 1
+int owe_hip_mesh_set_launch(owe_hip_mesh* mesh, int mode, int form, int blocks_per_cu, int refill);
+int owe_hip_mesh_get_launch(const owe_hip_mesh* mesh, int mode, int* form, int* blocks_per_cu, int* refill);
+
 /* ------------------------------------------------------------------ camera-path renderer
  * owe::Backend::createRenderer (backend.hpp:45-46) and owe::Renderer (renderer.hpp:19-79) over the
  * flattened scene of owe_hip_scene.h. Each pass adds spp samples per pixel of the scene's observer

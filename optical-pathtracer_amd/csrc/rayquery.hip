@@ -2,8 +2,13 @@
 // Stands in for MeshShape::intersect over many rays (src/owe/scene/mesh.cpp:73-97) and the float
 // kernels' any-hit occludedMesh (src/owe/backends/gpu/shaders/geometry.slang:577-615).
 //
-// One lane per ray, 128 lanes per block; the traversal stack lives in LDS with the capacity the
-// build proved sufficient (bvh4.hpp), chosen from a few compiled sizes.
+// Two launch forms. One lane per ray (traceBatchKernel), 128 lanes per block: a wave lives until its
+// longest ray ends. Lane refill (traceRefillKernel): a persistent grid whose waves each own a
+// contiguous range of rays; a lane whose ray ends waits until `refill` lanes of its wave are idle,
+// then the idle lanes take the next rays of the range together, so the wave keeps most of its lanes
+// busy instead of waiting on its slowest ray. Both forms walk each ray identically and return the
+// same hits bit for bit. The traversal stack lives in LDS with the capacity the build proved
+// sufficient (bvh4.hpp), chosen from a few compiled sizes.
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -19,6 +24,12 @@ struct owe_hip_mesh {
     WideNode* nodes = nullptr;
     TriRecord* tris = nullptr;
     owe_hip_mesh_stats stats{};
+    int cus = 0;                 // compute units of the device
+    // Per query mode (closest, any): the launch form, and the refill form's resident blocks per
+    // compute unit and idle lanes that trigger a refill.
+    int form[2] = {OWE_HIP_TRACE_REFILL, OWE_HIP_TRACE_REFILL};
+    int blocksPerCu[2] = {8, 8};
+    int refill[2] = {32, 32};
 };
 
 namespace owehip {
@@ -47,9 +58,65 @@ __global__ __launch_bounds__(kQueryBlock) void traceBatchKernel(const WideNode* 
     }
 }
 
+// Persistent lanes with refill: wave w owns rays [w·per, min(n, (w + 1)·per)). Every branch that
+// decides a refill is wave-uniform (ballots and the range); no barrier is used, so waves run
+// independently and each exits once its range is drained and its lanes are idle.
+template <int kCap, bool kAny>
+__global__ __launch_bounds__(kQueryBlock) void traceRefillKernel(const WideNode* __restrict__ nodes,
+                                                                 const TriRecord* __restrict__ tris,
+                                                                 const float4* __restrict__ rays, uint64_t n,
+                                                                 uint64_t per, uint32_t refill,
+                                                                 uint32_t* __restrict__ primOut,
+                                                                 float* __restrict__ tOut) {
+    __shared__ uint32_t stack[kCap * kQueryBlock];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t wave = (uint64_t(blockIdx.x) * kQueryBlock + threadIdx.x) >> 6;
+    uint64_t next = wave * per;
+    const uint64_t end = next < n ? (n - next < per ? n : next + per) : next;
+    TravState s;
+    uint64_t ray = 0;
+    bool active = false;
+    for (;;) {
+        const uint64_t idle = __ballot(!active);
+        const uint32_t nIdle = uint32_t(__popcll(idle));
+        if (next < end && (nIdle >= refill || nIdle == 64u)) {
+            const uint32_t rank = uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
+            if (!active && next + rank < end) {
+                ray = next + rank;
+                const float4 r0 = rays[2 * ray], r1 = rays[2 * ray + 1];
+                beginBvh4(s, mk3(r0.x, r0.y, r0.z), mk3(r1.x, r1.y, r1.z), r0.w, r1.w);
+                active = true;
+            }
+            next = end - next > nIdle ? next + nIdle : end;
+        }
+        if (__ballot(active) == 0) break;
+        if (active && stepBvh4<kAny>(s, nodes, tris, stack + threadIdx.x, kQueryBlock)) {
+            primOut[ray] = s.hit.slot == kEmptyChild ? 0xFFFFFFFFu : tris[s.hit.slot].prim;
+            tOut[ray] = s.hit.slot == kEmptyChild ? __builtin_inff() : s.hit.t;
+            active = false;
+        }
+    }
+}
+
 template <int kCap>
 void launchTrace(int mode, const owe_hip_mesh* m, const float* rays, uint64_t n, uint32_t* prim, float* t,
                  hipStream_t s) {
+    const float4* r4 = reinterpret_cast<const float4*>(rays);
+    const int q = mode == OWE_HIP_QUERY_ANY ? 1 : 0;
+    if (m->form[q] == OWE_HIP_TRACE_REFILL) {
+        const uint64_t blocks = uint64_t(m->cus) * uint64_t(m->blocksPerCu[q]);
+        const uint64_t waves = blocks * (kQueryBlock / 64);
+        const uint64_t per = (n + waves - 1) / waves;
+        const dim3 grid{unsigned(blocks)}, block{unsigned(kQueryBlock)};
+        if (mode == OWE_HIP_QUERY_ANY)
+            hipLaunchKernelGGL((traceRefillKernel<kCap, true>), grid, block, 0, s, m->nodes, m->tris, r4, n, per,
+                               uint32_t(m->refill[q]), prim, t);
+        else
+            hipLaunchKernelGGL((traceRefillKernel<kCap, false>), grid, block, 0, s, m->nodes, m->tris, r4, n, per,
+                               uint32_t(m->refill[q]), prim, t);
+        OWE_HIP_CHECK(hipGetLastError());
+        return;
+    }
     const uint64_t blocks = (n + kQueryBlock - 1) / kQueryBlock;
     if (blocks > 0x7FFFFFFFull) fail(OWE_HIP_ERR_INVALID, "batch too large for one launch");
     const dim3 grid{unsigned(blocks)}, block{unsigned(kQueryBlock)};
@@ -81,6 +148,12 @@ int owe_hip_mesh_create(int device, const float* positions, uint32_t nVertices, 
                                               " entries; the kernels hold at most 64");
         auto* m = new owe_hip_mesh;
         m->device = device;
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) != hipSuccess || prop.multiProcessorCount <= 0) {
+            delete m;
+            fail(OWE_HIP_ERR_DEVICE, "cannot read the device's compute-unit count");
+        }
+        m->cus = prop.multiProcessorCount;
         const size_t nb = b.nodes.size() * sizeof(WideNode), tb = std::max<size_t>(1, b.tris.size()) * sizeof(TriRecord);
         hipError_t e = hipMalloc(&m->nodes, nb);
         if (e == hipSuccess) e = hipMalloc(&m->tris, tb);
@@ -117,6 +190,33 @@ int owe_hip_mesh_stats_get(const owe_hip_mesh* m, owe_hip_mesh_stats* out) {
     return guard([&] {
         require(m != nullptr && out != nullptr, "null argument");
         *out = m->stats;
+    });
+}
+
+int owe_hip_mesh_set_launch(owe_hip_mesh* m, int mode, int form, int blocks_per_cu, int refill) {
+    return guard([&] {
+        require(m != nullptr, "mesh is null");
+        require(mode == -1 || mode == OWE_HIP_QUERY_CLOSEST || mode == OWE_HIP_QUERY_ANY,
+                "mode must be closest (0), any (1) or both (-1)");
+        require(form == OWE_HIP_TRACE_PER_LANE || form == OWE_HIP_TRACE_REFILL, "form must be 0 (per lane) or 1 (refill)");
+        require(blocks_per_cu >= 0 && blocks_per_cu <= 10, "blocks_per_cu must be 0 (keep) to 10");
+        require(refill >= 0 && refill <= 64, "refill must be 0 (keep) to 64");
+        for (int q = 0; q < 2; ++q) {
+            if (mode != -1 && mode != q) continue;
+            m->form[q] = form;
+            if (blocks_per_cu) m->blocksPerCu[q] = blocks_per_cu;
+            if (refill) m->refill[q] = refill;
+        }
+    });
+}
+
+int owe_hip_mesh_get_launch(const owe_hip_mesh* m, int mode, int* form, int* blocks_per_cu, int* refill) {
+    return guard([&] {
+        require(m != nullptr && form != nullptr && blocks_per_cu != nullptr && refill != nullptr, "null argument");
+        require(mode == OWE_HIP_QUERY_CLOSEST || mode == OWE_HIP_QUERY_ANY, "mode must be closest (0) or any (1)");
+        *form = m->form[mode];
+        *blocks_per_cu = m->blocksPerCu[mode];
+        *refill = m->refill[mode];
     });
 }
 

@@ -90,6 +90,9 @@ def load() -> ctypes.CDLL:
     _declare(lib, "owe_hip_mesh_destroy", c_int, [c_vp])
     _declare(lib, "owe_hip_mesh_stats_get", c_int, [c_vp, ctypes.POINTER(MeshStats)])
     _declare(lib, "owe_hip_trace_batch", c_int, [c_vp, c_int, c_vp, c_u64, c_vp, c_vp, c_vp])
+    _declare(lib, "owe_hip_mesh_set_launch", c_int, [c_vp, c_int, c_int, c_int, c_int])
+    _declare(lib, "owe_hip_mesh_get_launch", c_int, [c_vp, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int),
+                                                     ctypes.POINTER(c_int)])
     _declare(lib, "owe_hip_renderer_create", c_int, [c_int, c_vp, c_vp, ctypes.POINTER(c_vp)])
     _declare(lib, "owe_hip_renderer_destroy", c_int, [c_vp])
     _declare(lib, "owe_hip_renderer_pass", c_int, [c_vp, c_int])

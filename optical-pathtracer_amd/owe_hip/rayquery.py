@@ -36,6 +36,19 @@ class Mesh:
         _lib.check(_lib.load().owe_hip_mesh_stats_get(self._h, ctypes.byref(s)))
         return s
 
+    PER_LANE, REFILL = 0, 1
+
+    def set_launch(self, form: int, blocks_per_cu: int = 0, refill: int = 0, mode: int = -1) -> None:
+        """One lane per ray (0) or persistent lanes with refill (1) for query mode `mode` (closest 0,
+        any 1, both −1); 0 keeps a shape value."""
+        _lib.check(_lib.load().owe_hip_mesh_set_launch(self._h, int(mode), int(form), int(blocks_per_cu),
+                                                       int(refill)))
+
+    def launch(self, mode: int = _lib.QUERY_CLOSEST) -> dict:
+        v = [ctypes.c_int() for _ in range(3)]
+        _lib.check(_lib.load().owe_hip_mesh_get_launch(self._h, int(mode), *[ctypes.byref(x) for x in v]))
+        return dict(form=v[0].value, blocks_per_cu=v[1].value, refill=v[2].value)
+
     def close(self) -> None:
         if self._h:
             _lib.check(_lib.load().owe_hip_mesh_destroy(self._h))

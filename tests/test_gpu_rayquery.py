@@ -141,3 +141,24 @@ def test_full_size_batch(cuda, oracle):
     assert_parity(classify(pos, tri, closest[sub], gp[sub], gt[sub], op, ot))
     gp2, gt2 = run(mesh, closest)
     assert np.array_equal(gp, gp2) and np.array_equal(gt, gt2)
+
+
+@pytest.mark.parametrize("n", [1, 777, 200_000])
+def test_refill_form_returns_the_per_lane_forms_hits_bit_for_bit(cuda, n):
+    """Persistent lanes with refill walk each ray as one lane per ray does: the same hits, any shape."""
+    pos, tri = workloads.icosphere(n=64, seed=5)
+    closest, any_rays = workloads.query_rays(n, seed=31)
+    mesh = rayquery.Mesh(pos, tri)
+    mesh.set_launch(rayquery.Mesh.PER_LANE)
+    ref = [run(mesh, closest), run(mesh, any_rays, _lib.QUERY_ANY)]
+    for bpc, refill in ((1, 1), (6, 16), (10, 64), (8, 32)):
+        mesh.set_launch(rayquery.Mesh.REFILL, bpc, refill)
+        assert mesh.launch() == mesh.launch(_lib.QUERY_ANY) == dict(form=1, blocks_per_cu=bpc, refill=refill)
+        got = [run(mesh, closest), run(mesh, any_rays, _lib.QUERY_ANY)]
+        for (rp, rt), (gp, gt) in zip(ref, got):
+            assert np.array_equal(rp, gp) and np.array_equal(rt, gt), (bpc, refill)
+    # Each mode keeps its own form.
+    mesh.set_launch(rayquery.Mesh.PER_LANE, mode=_lib.QUERY_ANY)
+    assert mesh.launch(_lib.QUERY_ANY)["form"] == 0 and mesh.launch(_lib.QUERY_CLOSEST)["form"] == 1
+    assert [run(mesh, closest), run(mesh, any_rays, _lib.QUERY_ANY)][1][0].tolist() == ref[1][0].tolist()
+    print(f"n {n}: refill form identical over 4 shapes", flush=True)
