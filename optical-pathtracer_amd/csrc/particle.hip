@@ -140,6 +140,7 @@ __device__ __forceinline__ bool linkObserver(PCtx& c, f3 x, Link& k) {
 
 // A surface vertex (interface `it` at hit boundary bi, face) or a medium point (phase g) joined to
 // the eye, value β·f·|cos|·W·Tr.
+template <bool kContacts>
 __device__ __forceinline__ void connectVertex(PCtx& c, f3 p, bool surface, const Iface* it, uint32_t bi, uint32_t face, f3 n,
                                      f3 d, float g, uint32_t region) {
     Link k;
@@ -175,7 +176,7 @@ __device__ __forceinline__ void connectVertex(PCtx& c, f3 p, bool surface, const
         dface = face;
     }
     float Tr[NW];
-    if (!walkTransmittance(sc, p, k.dir, len3(k.q - p), start, departed, dface, kNoBoundary, sc.obs.region,
+    if (!walkTransmittance<kContacts>(sc, p, k.dir, len3(k.q - p), start, departed, dface, kNoBoundary, sc.obs.region,
                            c.wl.lambda, c.stack, kPBlock, Tr))
         return;
     float v[NW];
@@ -191,6 +192,7 @@ struct PRay {
 };
 
 // emitParticle (particle.slang:306-325): the sky or a light by power, then its emission.
+template <bool kContacts>
 __device__ __forceinline__ bool emitParticle(PCtx& c, bool observer, PRay& ray) {
     const DScene& sc = c.sc;
     const float pSky = sc.env.skyShare;
@@ -256,7 +258,7 @@ __device__ __forceinline__ bool emitParticle(PCtx& c, bool observer, PRay& ray) 
                 const float cosL = dot3(n, k.dir);
                 if ((cosL > 0.0f && em.front) || (cosL < 0.0f && em.back)) {
                     float Tr[NW];
-                    if (walkTransmittance(sc, p, k.dir, len3(k.q - p), cosL > 0.0f ? b.front : b.back, bi, ~0u,
+                    if (walkTransmittance<kContacts>(sc, p, k.dir, len3(k.q - p), cosL > 0.0f ? b.front : b.back, bi, ~0u,
                                           kNoBoundary, sc.obs.region, c.wl.lambda, c.stack, kPBlock, Tr)) {
                         const float s = fabsf(cosL) * k.factor / (sel * pdfA);
                         float v[NW];
@@ -302,17 +304,18 @@ __device__ __forceinline__ bool roulette(PCtx& c, float q) {
 }
 
 // traceParticle (particle.slang:356-427), without the hybrid partition.
+template <bool kContacts>
 __device__ __forceinline__ void traceParticle(PCtx& c) {
     const DScene& sc = c.sc;
     const DObserver& ob = sc.obs;
     const bool observer = ob.kind == OWE_DETECTOR_OBSERVER;
     const int maxDepth = c.set.maxDepth, rrDepth = c.set.rrDepth;
     PRay ray;
-    if (!emitParticle(c, observer, ray)) return;
+    if (!emitParticle<kContacts>(c, observer, ray)) return;
     for (int depth = 0;; ++depth) {
         Hit h;
-        bool found = intersectWorld(sc, ray.o, ray.d, __builtin_inff(), ray.departed, ray.face, kNoBoundary, c.stack,
-                                    kPBlock, h);
+        bool found = intersectContact<true, kContacts>(sc, ray.o, ray.d, __builtin_inff(), ray.departed, ray.face, kNoBoundary, ray.region,
+                                      c.stack, kPBlock, h);
         if (found && depth == 0 && ray.distant) {
             const float back = 1e-5f * (1.0f + fmaxf(fabsf(h.p.x), fmaxf(fabsf(h.p.y), fabsf(h.p.z))));
             Hit again;
@@ -336,7 +339,7 @@ __device__ __forceinline__ void traceParticle(PCtx& c) {
                 for (int i = 0; i < NW; ++i)
                     c.beta[i] *= evalSpec(med.scattering, sc.table, c.wl.lambda[i]) * expf(-sT[i] * tm) / pdf0;
                 const f3 p = ray.o + ray.d * tm;
-                if (observer) connectVertex(c, p, false, nullptr, kNoBoundary, ~0u, mk3(0, 0, 0), ray.d, med.g, ray.region);
+                if (observer) connectVertex<kContacts>(c, p, false, nullptr, kNoBoundary, ~0u, mk3(0, 0, 0), ray.d, med.g, ray.region);
                 if (depth >= maxDepth) return;
                 const float u2 = c.rng.uniform(), u1 = c.rng.uniform();
                 f3 wi;
@@ -402,7 +405,7 @@ __device__ __forceinline__ void traceParticle(PCtx& c) {
         if (o.type == OWE_SURF_DETECTOR || o.type == OWE_SURF_ABSORBER) return;
         const float hero = c.wl.lambda[0];
         const Iface it{&o, h.n, regionIndex(sc, b.front, hero), regionIndex(sc, b.back, hero), hero, c.set.fresnelFloor};
-        if (!o.delta && observer) connectVertex(c, h.p, true, &it, bi, hitFace, h.n, ray.d, 0.0f, ray.region);
+        if (!o.delta && observer) connectVertex<kContacts>(c, h.p, true, &it, bi, hitFace, h.n, ray.d, 0.0f, ray.region);
         if (depth >= maxDepth) return;
         auto at = [&](int i) {
             Iface r = it;
@@ -450,7 +453,7 @@ __device__ __forceinline__ uint32_t waveSum(uint32_t v) {
     return v;
 }
 
-template <int kCap>
+template <int kCap, bool kContacts>
 __global__ __launch_bounds__(kPBlock) void particleKernel(DScene sc, Settings set, ParticleBatch pb) {
     __shared__ uint32_t stack[kCap * kPBlock];
     const uint32_t i = blockIdx.x * kPBlock + threadIdx.x;
@@ -477,7 +480,7 @@ __global__ __launch_bounds__(kPBlock) void particleKernel(DScene sc, Settings se
                 c.wl.pdf[k] = visiblePdfF(c.wl.lambda[k]);
             }
         }
-        traceParticle(c);
+        traceParticle<kContacts>(c);
         if (pb.perParticle) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) pb.perParticle[3 * size_t(i) + k] = c.sum[k];
@@ -496,9 +499,15 @@ void launchParticles(const DScene& sc, const Settings& set, const ParticleBatch&
                      hipStream_t stream) {
     if (b.count == 0) return;
     const dim3 grid{(b.count + kPBlock - 1) / kPBlock}, block{kPBlock};
-    if (stackCap <= 16) hipLaunchKernelGGL(particleKernel<16>, grid, block, 0, stream, sc, set, b);
-    else if (stackCap <= 32) hipLaunchKernelGGL(particleKernel<32>, grid, block, 0, stream, sc, set, b);
-    else hipLaunchKernelGGL(particleKernel<48>, grid, block, 0, stream, sc, set, b);
+    if (sc.contacts) {
+        if (stackCap <= 16) hipLaunchKernelGGL((particleKernel<16, true>), grid, block, 0, stream, sc, set, b);
+        else if (stackCap <= 32) hipLaunchKernelGGL((particleKernel<32, true>), grid, block, 0, stream, sc, set, b);
+        else hipLaunchKernelGGL((particleKernel<48, true>), grid, block, 0, stream, sc, set, b);
+    } else {
+        if (stackCap <= 16) hipLaunchKernelGGL((particleKernel<16, false>), grid, block, 0, stream, sc, set, b);
+        else if (stackCap <= 32) hipLaunchKernelGGL((particleKernel<32, false>), grid, block, 0, stream, sc, set, b);
+        else hipLaunchKernelGGL((particleKernel<48, false>), grid, block, 0, stream, sc, set, b);
+    }
     OWE_HIP_CHECK(hipGetLastError());
 }
 

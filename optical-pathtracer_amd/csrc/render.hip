@@ -224,7 +224,7 @@ __global__ __launch_bounds__(kBlock) void generateKernel(DScene sc, Settings set
 
 // The live count is read from device memory, so rounds can be enqueued without a host round trip;
 // the grid covers a bound on it the host knows (live counts only shrink).
-template <int kCap>
+template <int kCap, bool kContacts>
 __global__ __launch_bounds__(kBlock) void intersectKernel(DScene sc, Paths P, int q) {
     __shared__ uint32_t stack[kCap * kBlock];
     const uint32_t count = P.count[q];
@@ -233,9 +233,10 @@ __global__ __launch_bounds__(kBlock) void intersectKernel(DScene sc, Paths P, in
     if (k >= count) return;
     const uint32_t slot = P.queue[q][k];
     const float4 r0 = P.ray0[slot], r1 = P.ray1[slot];
+    const uint4 info = P.info[slot];
     Hit h;
-    const bool found = intersectWorld<false>(sc, mk3(r0.x, r0.y, r0.z), mk3(r1.x, r1.y, r1.z), __builtin_inff(), P.info[slot].w,
-                                      P.face[slot], kNoBoundary, stack + threadIdx.x, kBlock, h);
+    const bool found = intersectContact<false, kContacts>(sc, mk3(r0.x, r0.y, r0.z), mk3(r1.x, r1.y, r1.z), __builtin_inff(), info.w,
+                                               P.face[slot], kNoBoundary, info.z, stack + threadIdx.x, kBlock, h);
     P.hit[slot] = found ? make_uint4(__float_as_uint(h.t), h.boundary, h.face, h.slot)
                         : make_uint4(__float_as_uint(__builtin_inff()), kNoBoundary, 0u, kEmptyChild);
 }
@@ -573,6 +574,7 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(DScene sc, Settings set, P
 // Tracer::transmittance of one pending NEE ray (transport.cpp:237-285): through null boundaries
 // with Beer–Lambert; false when blocked. With `simple` (no null boundaries, no media in the scene)
 // any crossing blocks. Shared by the wavefront and the persistent kernels.
+template <bool kContacts>
 __device__ __forceinline__ bool shadowTransmittance(const DScene& sc, f3 o, f3 d, float remaining, uint32_t region,
                                            uint32_t arrival, uint32_t exclude, uint32_t departed, uint32_t face,
                                            const float4 lam, int simple, uint32_t* stack, float Tr[NW]) {
@@ -581,7 +583,8 @@ __device__ __forceinline__ bool shadowTransmittance(const DScene& sc, f3 o, f3 d
     for (int i = 0; i < NW; ++i) Tr[i] = 1.0f;
     for (int guard = 0; guard < 64; ++guard) {
         Hit h;
-        const bool found = intersectWorld(sc, o, d, remaining, departed, face, exclude, stack, kBlock, h);
+        const bool found = intersectContact<true, kContacts>(sc, o, d, remaining, departed, face, exclude, region, stack,
+                                                             kBlock, h);
         if (simple) {
             // No null boundaries or participating media in the scene: any crossing blocks.
             if (found) return false;
@@ -625,7 +628,7 @@ __device__ __forceinline__ float4 addShadow(float4 L, const float4 pend, const f
     return L;
 }
 
-template <int kCap>
+template <int kCap, bool kContacts>
 __global__ __launch_bounds__(kBlock) void shadowKernel(DScene sc, Settings set, Paths P, int q, int simple) {
     __shared__ uint32_t stack[kCap * kBlock];
     const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
@@ -636,7 +639,7 @@ __global__ __launch_bounds__(kBlock) void shadowKernel(DScene sc, Settings set, 
     const float4 so = P.shO[slot], sd = P.shD[slot];
     const uint4 si = P.shInfo[slot];
     float Tr[NW];
-    if (!shadowTransmittance(sc, mk3(so.x, so.y, so.z), mk3(sd.x, sd.y, sd.z), so.w, si.x, si.y, si.z, si.w,
+    if (!shadowTransmittance<kContacts>(sc, mk3(so.x, so.y, so.z), mk3(sd.x, sd.y, sd.z), so.w, si.x, si.y, si.z, si.w,
                              P.shFace[slot], P.lam[slot], simple, stack + threadIdx.x, Tr))
         return;
     P.L[slot] = addShadow(P.L[slot], pend, Tr);
@@ -761,7 +764,7 @@ __device__ __forceinline__ void flushFilm(unsigned long long* film, uint32_t pix
 
 // kWaves: the waves per SIMD the register allocation must allow (2: no spills; 3: more waves to
 // hide latency, at the price of some register spills) — the autotuner measures both.
-template <int kCap, int kWaves>
+template <int kCap, int kWaves, bool kContacts>
 __global__ __launch_bounds__(kBlock, kWaves) void persistentKernel(DScene sc, Settings set, Persist a) {
     __shared__ uint32_t stack[kCap * kBlock];
     uint32_t* st = stack + threadIdx.x;
@@ -831,8 +834,8 @@ __global__ __launch_bounds__(kBlock, kWaves) void persistentKernel(DScene sc, Se
         if (__ballot(live) == 0) break;  // every lane of the wave is done: the queue is empty
         if (!live) continue;
         Hit h;
-        const bool found = intersectWorld<false>(sc, c.o, c.d, __builtin_inff(), c.departed, c.face, kNoBoundary, st,
-                                                 kBlock, h);
+        const bool found = intersectContact<false, kContacts>(sc, c.o, c.d, __builtin_inff(), c.departed, c.face,
+                                                              kNoBoundary, c.region, st, kBlock, h);
         ++segments;
         const uint4 hraw = found ? make_uint4(__float_as_uint(h.t), h.boundary, h.face, h.slot)
                                  : make_uint4(__float_as_uint(__builtin_inff()), kNoBoundary, 0u, kEmptyChild);
@@ -841,7 +844,7 @@ __global__ __launch_bounds__(kBlock, kWaves) void persistentKernel(DScene sc, Se
         if (c.shPend.x != 0.0f || c.shPend.y != 0.0f || c.shPend.z != 0.0f || c.shPend.w != 0.0f) {
             float Tr[NW];
             const float4 lam = make_float4(c.wl.lambda[0], c.wl.lambda[1], c.wl.lambda[2], c.wl.lambda[3]);
-            if (shadowTransmittance(sc, c.shO, c.shD, c.shDist, c.shStart, c.shLightRegion, c.shExclude, c.shDeparted,
+            if (shadowTransmittance<kContacts>(sc, c.shO, c.shD, c.shDist, c.shStart, c.shLightRegion, c.shExclude, c.shDeparted,
                                     c.shFace, lam, a.simple, st, Tr))
                 c.L = addShadow(c.L, c.shPend, Tr);
         }
@@ -981,18 +984,18 @@ void collectTimes(owe_hip_renderer* r, std::initializer_list<int> which) {
 }
 
 // One segment of every live path; the shade kernel's PCG increment is the pass's ((pass << 1) | 1).
-template <int kCap>
+template <int kCap, bool kContacts>
 void launchBounce(owe_hip_renderer* r, int q, uint32_t upper, uint64_t passInc) {
     // upper: the live count at the host's last read, a bound on every later round's.
     const dim3 grid = gridFor(upper);
     timed(r, kIntersect, [&] {
-        hipLaunchKernelGGL((intersectKernel<kCap>), grid, dim3{kBlock}, 0, r->stream, r->dev, r->P, q);
+        hipLaunchKernelGGL((intersectKernel<kCap, kContacts>), grid, dim3{kBlock}, 0, r->stream, r->dev, r->P, q);
     });
     timed(r, kShade, [&] {
         hipLaunchKernelGGL(shadeKernel, grid, dim3{kBlock}, 0, r->stream, r->dev, r->set, r->P, q, passInc);
     });
     timed(r, kShadow, [&] {
-        hipLaunchKernelGGL((shadowKernel<kCap>), grid, dim3{kBlock}, 0, r->stream, r->dev, r->set, r->P, q,
+        hipLaunchKernelGGL((shadowKernel<kCap, kContacts>), grid, dim3{kBlock}, 0, r->stream, r->dev, r->set, r->P, q,
                            int(r->simpleShadows));
     });
     // The consumed queue's counter is cleared for the round after next (a kernel, not a memset).
@@ -1024,9 +1027,15 @@ void runPaths(owe_hip_renderer* r, uint64_t base, uint32_t count, uint32_t spp, 
     for (int round = 0; upper > 0 && round <= maxRounds;) {
         const int group = round < 2 ? 1 : r->roundsPerSync;
         for (int g = 0; g < group && round <= maxRounds; ++g, ++round) {
-            if (r->stackCap <= 16) launchBounce<16>(r, q, upper, passInc);
-            else if (r->stackCap <= 32) launchBounce<32>(r, q, upper, passInc);
-            else launchBounce<48>(r, q, upper, passInc);
+            if (r->dev.contacts) {
+                if (r->stackCap <= 16) launchBounce<16, true>(r, q, upper, passInc);
+                else if (r->stackCap <= 32) launchBounce<32, true>(r, q, upper, passInc);
+                else launchBounce<48, true>(r, q, upper, passInc);
+            } else {
+                if (r->stackCap <= 16) launchBounce<16, false>(r, q, upper, passInc);
+                else if (r->stackCap <= 32) launchBounce<32, false>(r, q, upper, passInc);
+                else launchBounce<48, false>(r, q, upper, passInc);
+            }
             OWE_HIP_CHECK(hipGetLastError());
             q = 1 - q;
             if (r->timing) {
@@ -1086,14 +1095,15 @@ void launchPersistent(owe_hip_renderer* r, const Persist& a) {
     hipLaunchKernelGGL(persistBegin, dim3{1}, dim3{64}, 0, r->stream, a);
     const dim3 grid{unsigned(std::max(1, r->computeUnits * r->blocksPerCU))};
     auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, grid, dim3{kBlock}, 0, r->stream, r->dev, r->set, a); };
+    const bool c = r->dev.contacts != 0u;
     if (r->persistWaves == 3) {
-        if (r->stackCap <= 16) go(persistentKernel<16, 3>);
-        else if (r->stackCap <= 32) go(persistentKernel<32, 3>);
-        else go(persistentKernel<48, 3>);
+        if (r->stackCap <= 16) c ? go(persistentKernel<16, 3, true>) : go(persistentKernel<16, 3, false>);
+        else if (r->stackCap <= 32) c ? go(persistentKernel<32, 3, true>) : go(persistentKernel<32, 3, false>);
+        else c ? go(persistentKernel<48, 3, true>) : go(persistentKernel<48, 3, false>);
     } else {
-        if (r->stackCap <= 16) go(persistentKernel<16, 2>);
-        else if (r->stackCap <= 32) go(persistentKernel<32, 2>);
-        else go(persistentKernel<48, 2>);
+        if (r->stackCap <= 16) c ? go(persistentKernel<16, 2, true>) : go(persistentKernel<16, 2, false>);
+        else if (r->stackCap <= 32) c ? go(persistentKernel<32, 2, true>) : go(persistentKernel<32, 2, false>);
+        else c ? go(persistentKernel<48, 2, true>) : go(persistentKernel<48, 2, false>);
     }
 }
 
@@ -1221,6 +1231,7 @@ int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render
         d.nLights = uint32_t(h.lights.size());
         d.nRegions = uint32_t(h.regionMedium.size());
         d.stackCap = r->stackCap;
+        d.contacts = h.contacts ? 1u : 0u;
         d.env = h.env;
         d.obs = h.obs;
         bool anyNull = false, anyMedium = false;

@@ -8,6 +8,7 @@
 // Face groups: adjacent, exactly coplanar, equally oriented triangles of one mesh
 // (MeshShape::sameFace, src/owe/scene/mesh.cpp:50-71), joined transitively.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <map>
 #include <numeric>
@@ -341,6 +342,7 @@ HostScene flattenScene(const owe_scene& s) {
     std::vector<uint32_t> triIdx, triOwner, triFace;
     uint32_t faceBase = 0;
     double blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    std::vector<std::array<double, 6>> boxes;  // per boundary, camera-relative: lo, hi
     for (uint32_t i = 0; i < s.n_boundaries; ++i) {
         const owe_boundary& b = s.boundaries[i];
         const std::string name = "boundary " + std::to_string(i);
@@ -496,10 +498,23 @@ HostScene flattenScene(const owe_scene& s) {
             blo[k] = std::min(blo[k], lo[k]);
             bhi[k] = std::max(bhi[k], hi[k]);
         }
+        boxes.push_back({lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]});
         d.area = float(area);
         h.boundaries.push_back(d);
     }
     h.hasTriangles = !triOwner.empty();
+    // Contacts are possible only where two distinct boundaries' boxes meet, within the contact
+    // tolerance (2^-19 of the largest coordinate of either box).
+    for (size_t i = 0; i < boxes.size() && !h.contacts; ++i)
+        for (size_t j = i + 1; j < boxes.size() && !h.contacts; ++j) {
+            double m = 0;
+            for (int k = 0; k < 6; ++k) m = std::max({m, std::fabs(boxes[i][k]), std::fabs(boxes[j][k])});
+            const double tol = std::ldexp(m, -19);
+            bool meet = true;
+            for (int k = 0; k < 3; ++k)
+                meet = meet && boxes[i][k] - tol <= boxes[j][k + 3] && boxes[j][k] - tol <= boxes[i][k + 3];
+            h.contacts = meet;
+        }
     h.bvh = buildBvh4(triPos.data(), uint32_t(triPos.size() / 3), triIdx.data(), uint32_t(triOwner.size()),
                       triOwner.data(), triFace.data());
 

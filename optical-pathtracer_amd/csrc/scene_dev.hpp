@@ -130,6 +130,7 @@ struct DScene {
     const WideNode* nodes;  // null when the scene has no triangles
     const TriRecord* tris;
     uint32_t nBoundaries, nLights, nRegions, stackCap;
+    uint32_t contacts;  // two distinct boundaries touch or overlap: rays resolve contacts (intersectContact)
     DEnv env;
     DObserver obs;
 };
@@ -146,6 +147,7 @@ struct HostScene {
     std::vector<float> waveData;  // 4 floats per record
     Bvh4 bvh;
     bool hasTriangles = false;
+    bool contacts = false;  // some pair of distinct boundaries' boxes meet within the contact tolerance
     DEnv env{};
     DObserver obs{};
     double origin[3] = {0, 0, 0};  // world position of the camera-relative frame's origin

@@ -264,6 +264,7 @@ __device__ __forceinline__ bool sampleArea(const DBoundary& b, float u1, float u
 // (mesh face `face`, ~0 for none) and never meets `exclude` (the far end's own boundary). For a
 // finite segment the region it arrives in must be `arrival`. The traversal stack is stack[k·stride]. Tr holds the transmittance per
 // wavelength on return; false when the segment is blocked.
+template <bool kContacts = true>
 __device__ __forceinline__ bool walkTransmittance(const DScene& sc, f3 o, f3 d, float dist, uint32_t region, uint32_t departed,
                                          uint32_t face, uint32_t exclude, uint32_t arrival, const float lam[NW],
                                          uint32_t* stack, uint32_t stride, float Tr[NW]) {
@@ -273,7 +274,8 @@ __device__ __forceinline__ bool walkTransmittance(const DScene& sc, f3 o, f3 d, 
     for (int i = 0; i < NW; ++i) Tr[i] = 1.0f;
     for (int guard = 0; guard < 64; ++guard) {
         Hit h;
-        const bool found = intersectWorld(sc, o, d, remaining, departed, face, exclude, stack, stride, h);
+        const bool found = intersectContact<true, kContacts>(sc, o, d, remaining, departed, face, exclude, region, stack,
+                                                             stride, h);
         const float seg = found ? fminf(h.t, remaining) : remaining;
         const DMedium& m = sc.media[sc.regionMedium[region]];
         if (m.flags) {

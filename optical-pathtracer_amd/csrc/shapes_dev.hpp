@@ -238,8 +238,8 @@ __device__ __forceinline__ bool sagHit(const DSag& s, f3 o, f3 d, float tmin, fl
 
 // CylinderShape::intersect in float: geometric-form roots, the z trim; a ray leaving the wall
 // takes the other root −2 (o·d)_xy / |d_xy|².
-__device__ __forceinline__ bool cylinderHit(float R, float z0, float z1, f3 o, f3 d, float tmax, bool departed,
-                                            float& tOut) {
+__device__ __forceinline__ bool cylinderHit(float R, float z0, float z1, f3 o, f3 d, float tmin, float tmax,
+                                            bool departed, float& tOut) {
     const float A = d.x * d.x + d.y * d.y;
     if (A < 1e-30f) return false;
     const float b = o.x * d.x + o.y * d.y;
@@ -262,7 +262,7 @@ __device__ __forceinline__ bool cylinderHit(float R, float z0, float z1, f3 o, f
     }
     for (int k = 0; k < 2; ++k) {
         const float t = k == 0 ? t0 : t1;
-        if (!(t > 0.0f && t < tmax)) continue;
+        if (!(t > tmin && t < tmax)) continue;
         const float z = o.z + d.z * t;
         if (z < z0 || z > z1) continue;
         tOut = t;
@@ -344,14 +344,15 @@ __device__ __forceinline__ f3 waveNormalF(const DWave& w, const float4* data, fl
     return norm3(mk3(-gx, -gy, 1.0f));
 }
 
-// WaveSurface::intersect in float: march f(t) = z − h in steps |f| / L (L the Lipschitz bound of f
-// along the ray), Newton near the surface, bisection of a sign change inside a step. Every loop is
-// bounded. A departing ray skips roots within `depTol` of its origin along the surface normal.
-__device__ __forceinline__ bool waveHit(const DWave& w, const float4* data, f3 o, f3 d, float tmax, bool departed,
-                                        float depTol, float& tOut) {
+// WaveSurface::intersect in float for t in (tlo, tmax): march f(t) = z − h in steps |f| / L (L the
+// Lipschitz bound of f along the ray), Newton near the surface, bisection of a sign change inside a
+// step. Every loop is bounded. A departing ray skips roots within `depTol` of its origin along the
+// surface normal.
+__device__ __forceinline__ bool waveHit(const DWave& w, const float4* data, f3 o, f3 d, float tlo, float tmax,
+                                        bool departed, float depTol, float& tOut) {
     const float L = fabsf(d.z) + w.slope * sqrtf(d.x * d.x + d.y * d.y);
     if (!(L > 0.0f)) return false;
-    const float tmin = departed ? depTol / L : 0.0f;
+    const float tmin = departed ? fmaxf(depTol / L, tlo) : tlo;
     float ta = tmin, tb = tmax;
     auto clip = [&](float oc, float dc, float lo, float hi) {
         if (fabsf(dc) < 1e-30f) return oc >= lo && oc <= hi;

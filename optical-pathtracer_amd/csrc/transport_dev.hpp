@@ -435,13 +435,20 @@ __device__ __forceinline__ void analyticGeometry(const DScene& sc, const DBounda
     }
 }
 
-template <bool kGeometry = true>
+template <bool kGeometry = true, bool kContact = false>
 __device__ __forceinline__ bool intersectWorld(const DScene& sc, f3 o, f3 d, float tmax, uint32_t departed, uint32_t departedFace,
-                                      uint32_t exclude, uint32_t* stack, uint32_t stride, Hit& h) {
+                                      uint32_t exclude, uint32_t* stack, uint32_t stride, Hit& h, float lo = 0.0f,
+                                      uint32_t skipB = kNoBoundary, uint32_t skipFace = ~0u) {
+    // Crossings count for t in (lo, tmax): the departed boundary's own from max(lo, 0) by its own
+    // rules, every other boundary's from lo (below zero for a spawned ray among touching bodies).
+    // skipB (an analytic boundary) and skipFace (a mesh face group) are not crossings.
+    const float loOwn = fmaxf(lo, 0.0f);
     h.boundary = kNoBoundary;
     h.t = tmax;
     if (sc.nodes) {
-        const RayHit rh = traverseBvh4<false>(sc.nodes, sc.tris, o, d, 0.0f, tmax, departedFace, stack, stride);
+        const RayHit rh = kContact ? traverseBvh4<false>(sc.nodes, sc.tris, o, d, loOwn, tmax, departedFace, stack, stride,
+                                                         departed, lo, skipFace)
+                                   : traverseBvh4<false>(sc.nodes, sc.tris, o, d, loOwn, tmax, departedFace, stack, stride);
         if (rh.slot != kEmptyChild) {
             const float4* tr = reinterpret_cast<const float4*>(sc.tris + rh.slot);
             const float4 a = tr[0], b = tr[1], c = tr[2];
@@ -457,35 +464,37 @@ __device__ __forceinline__ bool intersectWorld(const DScene& sc, f3 o, f3 d, flo
     uint32_t analytic = kNoBoundary;
     for (uint32_t i = 0; i < sc.nBoundaries; ++i) {
         const DBoundary& b = sc.boundaries[i];
-        if (b.shape == OWE_SHAPE_MESH || i == exclude) continue;
+        if (b.shape == OWE_SHAPE_MESH || i == exclude || i == skipB) continue;
         if (!boxHit(b, o, inv, h.t)) continue;
-        const f3 lo = xfPoint(b.toLocal, o), ldv = xfVector(b.toLocal, d);
+        const f3 lo3 = xfPoint(b.toLocal, o), ldv = xfVector(b.toLocal, d);
+        const bool own = i == departed;
+        const float tmin = own ? loOwn : lo;
         float t;
         if (b.shape == OWE_SHAPE_PLANE) {
-            if (i == departed || ldv.z == 0.0f) continue;
-            t = -lo.z / ldv.z;
-            if (!(t > 0.0f && t < h.t)) continue;
-            f3 pl = lo + ldv * t;
+            if (own || ldv.z == 0.0f) continue;
+            t = -lo3.z / ldv.z;
+            if (!(t > tmin && t < h.t)) continue;
+            f3 pl = lo3 + ldv * t;
             pl.z = 0.0f;
             if (!inAperture(b.aperture, b.a, b.b, pl)) continue;
         } else if (b.shape == OWE_SHAPE_SPHERE) {
-            if (i == departed) {
-                t = -2.0f * dot3(lo, ldv) / dot3(ldv, ldv);  // the root other than the origin's own
-                if (!(t > 0.0f && t < h.t)) continue;
+            if (own) {
+                t = -2.0f * dot3(lo3, ldv) / dot3(ldv, ldv);  // the root other than the origin's own
+                if (!(t > tmin && t < h.t)) continue;
             } else {
                 float t0, t1;
-                if (!sphereRoots(lo, ldv, b.a, t0, t1)) continue;
-                t = t0 > 0.0f ? t0 : t1;
-                if (!(t > 0.0f && t < h.t)) continue;
+                if (!sphereRoots(lo3, ldv, b.a, t0, t1)) continue;
+                t = t0 > tmin ? t0 : t1;
+                if (!(t > tmin && t < h.t)) continue;
             }
         } else if (b.shape == OWE_SHAPE_CYLINDER) {
-            if (!cylinderHit(b.a, b.z0, b.z1, lo, ldv, h.t, i == departed, t)) continue;
+            if (!cylinderHit(b.a, b.z0, b.z1, lo3, ldv, tmin, h.t, own, t)) continue;
         } else if (b.shape == OWE_SHAPE_SAG) {
             f3 q;
-            const float tmin = i == departed && b.sag.nA != 0 ? b.depTol : 0.0f;
-            if (!sagHit(b.sag, lo, ldv, tmin, h.t, i == departed, t, q)) continue;
+            const float smin = own && b.sag.nA != 0 ? fmaxf(b.depTol, tmin) : tmin;
+            if (!sagHit(b.sag, lo3, ldv, smin, h.t, own, t, q)) continue;
         } else {
-            if (!waveHit(b.wave, sc.waveData, lo, ldv, h.t, i == departed, b.depTol, t)) continue;
+            if (!waveHit(b.wave, sc.waveData, lo3, ldv, tmin, h.t, own, b.depTol, t)) continue;
         }
         h.t = t;
         analytic = i;
@@ -496,9 +505,9 @@ __device__ __forceinline__ bool intersectWorld(const DScene& sc, f3 o, f3 d, flo
         h.slot = kEmptyChild;
         if (!kGeometry) return true;
         const DBoundary& b = sc.boundaries[analytic];
-        const f3 lo = xfPoint(b.toLocal, o), ldv = xfVector(b.toLocal, d);
+        const f3 lo3 = xfPoint(b.toLocal, o), ldv = xfVector(b.toLocal, d);
         f3 pl, nl;
-        analyticGeometry(sc, b, lo, ldv, h.t, pl, nl);
+        analyticGeometry(sc, b, lo3, ldv, h.t, pl, nl);
         h.pLocal = pl;
         h.nLocal = nl;
         h.p = xfPoint(b.toWorld, pl);
@@ -508,6 +517,54 @@ __device__ __forceinline__ bool intersectWorld(const DScene& sc, f3 o, f3 d, flo
     if (h.boundary == kNoBoundary) return false;
     if (kGeometry) h.p = o + d * h.t;
     return true;
+}
+
+// Contact tolerance at a point (camera-relative coordinates): 2^-19 of its largest coordinate.
+__device__ __forceinline__ float contactTol(f3 p) {
+    return ldexpf(fmaxf(fmaxf(fabsf(p.x), fabsf(p.y)), fabsf(p.z)), -19);
+}
+
+// World::intersect with contact resolution, for scenes whose bodies touch (sc.contacts): a spawned
+// ray meets other boundaries from t > −tol (tol the contact tolerance at its origin), and a crossing
+// met from another region than the ray's, or a detector crossing, gives way to a crossing at the same
+// place (within tol of its point; anywhere ahead for a contact at the ray's own origin) that is met
+// from the ray's region and is not a detector — two rounds at most (one inlined query in a loop of
+// at most three). Without contacts it is the plain intersectWorld.
+// kContacts: compiled for scenes with contacts (the launch picks it from sc.contacts); without it
+// this is the plain intersectWorld, with no contact code in the kernel.
+template <bool kGeometry = true, bool kContacts = true>
+__device__ __forceinline__ bool intersectContact(const DScene& sc, f3 o, f3 d, float tmax, uint32_t departed,
+                                                 uint32_t departedFace, uint32_t exclude, uint32_t region,
+                                                 uint32_t* stack, uint32_t stride, Hit& h) {
+    if (!kContacts || !sc.contacts)
+        return intersectWorld<kGeometry>(sc, o, d, tmax, departed, departedFace, exclude, stack, stride, h);
+    const bool contacts = true;
+    const float dep = departed != kNoBoundary ? contactTol(o) : 0.0f;
+    float lo = -dep, hi = tmax;
+    uint32_t skipB = kNoBoundary, skipF = ~0u;
+    bool found = false, fits = false;
+    for (int round = 0; round < 3; ++round) {
+        Hit x;
+        if (!intersectWorld<true, true>(sc, o, d, hi, departed, departedFace, exclude, stack, stride, x, lo, skipB, skipF))
+            break;
+        const DBoundary& xb = sc.boundaries[x.boundary];
+        const bool xfits = (dot3(d, x.n) < 0.0f ? xb.front : xb.back) == region;
+        const bool xdet = sc.optics[xb.optics].type == OWE_SURF_DETECTOR;
+        // An alternative replaces the first crossing only if it fits and is not a detector where
+        // the first one fitted.
+        if (round > 0 && (!xfits || (xdet && fits))) break;
+        h = x;
+        found = true;
+        fits = xfits;
+        if (!contacts || region == kRegionNone || (xfits && !xdet)) break;
+        const bool atOrigin = x.t <= dep;
+        const float tol = contactTol(x.p);
+        lo = atOrigin ? -dep : x.t - tol;
+        hi = atOrigin ? tmax : fminf(tmax, x.t + tol);
+        skipB = x.slot == kEmptyChild ? x.boundary : kNoBoundary;
+        skipF = x.slot == kEmptyChild ? ~0u : x.face;
+    }
+    return found;
 }
 
 }  // namespace owehip

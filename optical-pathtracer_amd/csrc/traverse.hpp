@@ -25,9 +25,15 @@ struct RayHit {
 
 // skipFace: a spawned ray's own face (its coplanar triangles are not crossings); ~0u for none.
 // stack: this lane's first entry in LDS; stride: lanes per block.
+// Contacts (world queries only; a triangle record's last word names its boundary): triangles of
+// boundary `own` count from tmin, those of every other boundary from tminOther (below zero for a
+// spawned ray: a touching surface at its origin); skipFace2 is a second face group that is skipped.
 template <bool kAny>
 __device__ inline RayHit traverseBvh4(const WideNode* __restrict__ nodes, const TriRecord* __restrict__ tris, f3 o, f3 d,
-                                      float tmin, float tmax, uint32_t skipFace, uint32_t* stack, uint32_t stride) {
+                                      float tmin, float tmax, uint32_t skipFace, uint32_t* stack, uint32_t stride,
+                                      uint32_t own = ~0u, float tminOther = 0.0f, uint32_t skipFace2 = ~0u) {
+    const bool contact = own != ~0u;
+    const float tlow = contact ? fminf(tmin, tminOther) : tmin;
     const Shear sh = makeShear(d);
     const f3 inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
     const bool nx = inv.x < 0.0f, ny = inv.y < 0.0f, nz = inv.z < 0.0f;
@@ -49,7 +55,7 @@ __device__ inline RayHit traverseBvh4(const WideNode* __restrict__ nodes, const 
             const float x0 = ((nx ? hxs[c] : lxs[c]) - o.x) * inv.x, x1 = ((nx ? lxs[c] : hxs[c]) - o.x) * inv.x;
             const float y0 = ((ny ? hys[c] : lys[c]) - o.y) * inv.y, y1 = ((ny ? lys[c] : hys[c]) - o.y) * inv.y;
             const float z0 = ((nz ? hzs[c] : lzs[c]) - o.z) * inv.z, z1 = ((nz ? lzs[c] : hzs[c]) - o.z) * inv.z;
-            const float near = fmaxf(fmaxf(tmin, x0), fmaxf(y0, z0));
+            const float near = fmaxf(fmaxf(tlow, x0), fmaxf(y0, z0));
             float far = fminf(fminf(x1, y1), z1);
             far = far * (far >= 0.0f ? 1.0000004f : 0.9999996f);
             far = fminf(far, hit.t);
@@ -67,8 +73,10 @@ __device__ inline RayHit traverseBvh4(const WideNode* __restrict__ nodes, const 
                 const float4* tr = reinterpret_cast<const float4*>(tris + slot);
                 const float4 a = tr[0], b = tr[1], cc = tr[2];
                 if (skipFace != ~0u && __float_as_uint(b.w) == skipFace) continue;
+                if (skipFace2 != ~0u && __float_as_uint(b.w) == skipFace2) continue;
+                const float tl = contact && __float_as_uint(cc.w) != own ? tminOther : tmin;
                 float t;
-                if (!hitTriangle(sh, o, d, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(cc.x, cc.y, cc.z), tmin, hit.t, t))
+                if (!hitTriangle(sh, o, d, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(cc.x, cc.y, cc.z), tl, hit.t, t))
                     continue;
                 hit.t = t;
                 hit.slot = slot;

@@ -34,6 +34,8 @@ extern "C" {
 
 void* oracle_world_create(const owe_scene* s) { return new World(*s); }
 void oracle_world_destroy(void* w) { delete static_cast<World*>(w); }
+// Whether rays of this world resolve contacts (two distinct boundaries' boxes meet).
+int oracle_world_contacts(const void* wh) { return static_cast<const World*>(wh)->contacts ? 1 : 0; }
 
 // Light-selection pdfs in table order (boundaries, then the sun); returns the count.
 int oracle_light_table(const void* wh, double* pdf, uint32_t* boundary, int cap) {

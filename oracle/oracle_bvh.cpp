@@ -212,7 +212,7 @@ bool Mesh::triangleT(uint32_t prim, V3 o, V3 d, double tmin, double tmax, double
 }
 
 bool Mesh::closest(V3 o, V3 d, double tmin, double tmax, double& tOut, uint32_t& primOut, uint32_t skipFace,
-                   uint64_t* boxTests, uint64_t* triTests, const uint32_t* group) const {
+                   uint64_t* boxTests, uint64_t* triTests, const uint32_t* group, uint32_t skipFace2) const {
     bool found = false;
     double best = tmax;
     const auto& ord = bvh_.order();
@@ -223,6 +223,7 @@ bool Mesh::closest(V3 o, V3 d, double tmin, double tmax, double& tOut, uint32_t&
             double t;
             if (!triangleT(i, o, d, tmin, tm, t)) return;
             if (skipFace != ~0u && (group ? group[i] == group[skipFace] : sameFace(skipFace, i))) return;
+            if (skipFace2 != ~0u && (group ? group[i] == group[skipFace2] : sameFace(skipFace2, i))) return;
             tm = t;
             found = true;
             tOut = t;

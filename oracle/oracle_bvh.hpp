@@ -136,7 +136,8 @@ public:
     // as this one are not crossings (a spawned ray's own face); ~0u for none. With `group` (a face id
     // per triangle), every triangle of skipFace's group is skipped instead of its adjacent ones.
     bool closest(V3 o, V3 d, double tmin, double tmax, double& t, uint32_t& prim, uint32_t skipFace = ~0u,
-                 uint64_t* boxTests = nullptr, uint64_t* triTests = nullptr, const uint32_t* group = nullptr) const;
+                 uint64_t* boxTests = nullptr, uint64_t* triTests = nullptr, const uint32_t* group = nullptr,
+                 uint32_t skipFace2 = ~0u) const;
     bool any(V3 o, V3 d, double tmin, double tmax, uint32_t skipFace = ~0u) const;
     bool bruteClosest(V3 o, V3 d, double tmin, double tmax, double& t, uint32_t& prim) const;
     // Adjacent, exactly coplanar triangles with the same orientation form one face.

@@ -429,11 +429,15 @@ World::World(const owe_scene& s) {
         observer.aimNormal = V3(ob.aim_normal[0], ob.aim_normal[1], ob.aim_normal[2]);
         observer.aimRadius = ob.aim_radius;
         observer.aimShare = ob.aim_share;
+        rebase = b.toWorld.point(V3(0, 0, 0));
+        findContacts();
         return;
     }
     // Ideal observer (IdealObserver::prepare, detector.cpp:36-45).
     observer.region = ob.region;
     observer.position = V3(ob.position[0], ob.position[1], ob.position[2]);
+    rebase = observer.position;
+    findContacts();
     observer.fwd = normalize(V3(ob.look_at[0], ob.look_at[1], ob.look_at[2]) - observer.position);
     V3 u(ob.up[0], ob.up[1], ob.up[2]);
     if (std::fabs(dot(normalize(u), observer.fwd)) > 0.9999)
@@ -465,70 +469,96 @@ double otherRoot(const shapes::Quadric& q, V3 o, V3 d) {
     return shapes::refineRoot(q, o, d, -B / A);
 }
 
-bool World::intersect(const Ray& ray, double tmax, Hit& hit) const {
+void World::findContacts() {
+    contacts = false;
+    for (size_t i = 0; i < boundaries.size() && !contacts; ++i)
+        for (size_t j = i + 1; j < boundaries.size() && !contacts; ++j) {
+            const Box& a = boundaries[i].worldBox;
+            const Box& c = boundaries[j].worldBox;
+            const V3 al = a.lo - rebase, ah = a.hi - rebase, cl = c.lo - rebase, ch = c.hi - rebase;
+            double m = 0;
+            for (const V3& v : {al, ah, cl, ch}) m = std::max({m, std::fabs(v.x), std::fabs(v.y), std::fabs(v.z)});
+            const double tol = std::ldexp(m, -19);
+            contacts = al.x - tol <= ch.x && cl.x - tol <= ah.x && al.y - tol <= ch.y && cl.y - tol <= ah.y &&
+                       al.z - tol <= ch.z && cl.z - tol <= ah.z;
+        }
+}
+
+double World::contactTol(V3 p) const {
+    const V3 q = p - rebase;
+    return std::ldexp(std::max({std::fabs(q.x), std::fabs(q.y), std::fabs(q.z)}), -19);
+}
+
+bool World::intersectRaw(const Ray& ray, double lo, double hi, Hit& hit, uint32_t skipB, uint32_t skipPrim) const {
     bool found = false;
-    double best = tmax;
+    double best = hi;
     const Slabs rs(ray.o, ray.d);
     for (uint32_t i = 0; i < boundaries.size(); ++i) {
         const Boundary& b = boundaries[i];
+        if (i == skipB && b.shape != OWE_SHAPE_MESH) continue;
         double tEnter;
-        if (!rs.hit(b.worldBox, 0, best, tEnter)) continue;
+        if (!rs.hit(b.worldBox, std::min(lo, 0.0), best, tEnter)) continue;
         const bool departed = i == ray.departed;
-        const V3 lo = b.toLocal.point(ray.o), ld = b.toLocal.vector(ray.d);
+        // Other boundaries count from lo (below zero at a departure: a contact at the origin); the
+        // departed one by its own rules, ahead of the origin.
+        const double tmin = departed ? std::max(lo, 0.0) : lo;
+        const V3 lo3 = b.toLocal.point(ray.o), ld = b.toLocal.vector(ray.d);
         double t;
         V3 pl, nl;
         uint32_t prim = 0;
         if (b.shape == OWE_SHAPE_PLANE) {
             if (departed || ld.z == 0) continue;
-            t = -lo.z / ld.z;
-            if (!(t > 0 && t < best)) continue;
-            pl = lo + ld * t;
+            t = -lo3.z / ld.z;
+            if (!(t > tmin && t < best)) continue;
+            pl = lo3 + ld * t;
             pl.z = 0;
             if (!inAperture(b.aperture, b.a, b.b, pl)) continue;
             nl = V3(0, 0, 1);
         } else if (b.shape == OWE_SHAPE_SPHERE) {
             if (departed) {
-                t = -2 * dot(lo, ld) / dot(ld, ld);  // the root other than the origin's own
-                if (!(t > 0 && t < best)) continue;
+                t = -2 * dot(lo3, ld) / dot(ld, ld);  // the root other than the origin's own
+                if (!(t > tmin && t < best)) continue;
             } else {
                 double t0, t1;
-                if (!sphereRoots(lo, ld, b.a, t0, t1)) continue;
-                t = t0 > 0 ? t0 : t1;
-                if (!(t > 0 && t < best)) continue;
+                if (!sphereRoots(lo3, ld, b.a, t0, t1)) continue;
+                t = t0 > tmin ? t0 : t1;
+                if (!(t > tmin && t < best)) continue;
             }
-            pl = lo + ld * t;
+            pl = lo3 + ld * t;
             nl = normalize(pl / b.a);
         } else if (b.shape == OWE_SHAPE_CYLINDER) {
             if (departed) {
-                t = otherRoot(shapes::Quadric{1, b.a * b.a, 0}, lo, ld);
-                if (!(t > 0 && t < best)) continue;
-                pl = lo + ld * t;
+                t = otherRoot(shapes::Quadric{1, b.a * b.a, 0}, lo3, ld);
+                if (!(t > tmin && t < best)) continue;
+                pl = lo3 + ld * t;
                 if (!(pl.z >= b.z0 && pl.z <= b.z1)) continue;
                 nl = normalize(V3(pl.x, pl.y, 0));
-            } else if (!shapes::intersectCylinder(b.a, b.z0, b.z1, lo, ld, 0, best, t, pl, nl)) {
+            } else if (!shapes::intersectCylinder(b.a, b.z0, b.z1, lo3, ld, tmin, best, t, pl, nl)) {
                 continue;
             }
         } else if (b.shape == OWE_SHAPE_SAG) {
             const shapes::Sag& sg = b.sag;
             if (departed && sg.nA == 0) {
                 if (sg.c == 0) continue;  // a plane: nothing beyond its own point
-                t = otherRoot(shapes::Quadric{0, sg.c, sg.c * (1 + sg.k)}, lo, ld);
-                if (!(t > 0 && t < best)) continue;
-                pl = lo + ld * t;
+                t = otherRoot(shapes::Quadric{0, sg.c, sg.c * (1 + sg.k)}, lo3, ld);
+                if (!(t > tmin && t < best)) continue;
+                pl = lo3 + ld * t;
                 if (!shapes::sagAccept(sg, pl) || !shapes::onConicSheet(sg, pl.z)) continue;
                 nl = shapes::sagNormal(sg, pl);
             } else {
                 // A numeric sag left by this ray: roots within 1e-9 of its scale are its own point.
-                const double tmin = departed ? 1e-9 * (sg.rMax + std::max(std::fabs(sg.zMin), std::fabs(sg.zMax))) : 0.0;
-                if (!shapes::intersectSag(sg, lo, ld, tmin, best, t, pl, nl)) continue;
+                const double own = departed ? 1e-9 * (sg.rMax + std::max(std::fabs(sg.zMin), std::fabs(sg.zMax))) : tmin;
+                if (!shapes::intersectSag(sg, lo3, ld, std::max(own, tmin), best, t, pl, nl)) continue;
             }
         } else if (b.shape == OWE_SHAPE_WAVE) {
-            if (!shapes::intersectWave(*b.wave, lo, ld, 0, best, departed, t, pl, nl)) continue;
+            if (!shapes::intersectWave(*b.wave, lo3, ld, tmin, best, departed, t, pl, nl)) continue;
         } else {
             const uint32_t skip = departed && ray.departedPrim < b.faceGroup.size() ? ray.departedPrim : ~0u;
-            if (!b.mesh->closest(lo, ld, 0, best, t, prim, skip, nullptr, nullptr, skip != ~0u ? b.faceGroup.data() : nullptr))
+            const uint32_t skip2 = i == skipB && skipPrim < b.faceGroup.size() ? skipPrim : ~0u;
+            if (!b.mesh->closest(lo3, ld, tmin, best, t, prim, skip, nullptr, nullptr,
+                                 b.faceGroup.empty() ? nullptr : b.faceGroup.data(), skip2))
                 continue;
-            pl = lo + ld * t;
+            pl = lo3 + ld * t;
             nl = b.mesh->normalOf(prim);
         }
         best = t;
@@ -543,6 +573,31 @@ bool World::intersect(const Ray& ray, double tmax, Hit& hit) const {
     const Boundary& b = boundaries[hit.boundary];
     hit.p = b.toWorld.point(hit.pLocal);
     hit.n = normalize(b.toWorld.vector(hit.nLocal));
+    return true;
+}
+
+bool World::intersect(const Ray& ray, double tmax, Hit& hit, uint32_t region) const {
+    const double dep = contacts && ray.departed != kNone ? contactTol(ray.o) : 0.0;
+    if (!intersectRaw(ray, -dep, tmax, hit, kNone, kNone)) return false;
+    if (region == kNone || !contacts) return true;
+    auto incident = [&](const Hit& h) {
+        const Boundary& b = boundaries[h.boundary];
+        return dot(ray.d, h.n) < 0 ? b.front : b.back;
+    };
+    auto detector = [&](const Hit& h) { return optics[size_t(boundaries[h.boundary].optics)].type == OWE_SURF_DETECTOR; };
+    // Two rounds at most: a crossing that does not fit the ray gives way to one that does at the same
+    // place — anywhere ahead when the misfit is a contact at the ray's own origin.
+    for (int round = 0; round < 2; ++round) {
+        const bool fits = incident(hit) == region;
+        if (fits && !detector(hit)) return true;
+        const bool atOrigin = hit.t <= dep;
+        const double tol = contactTol(hit.p);
+        const double lo = atOrigin ? -dep : hit.t - tol, hi = atOrigin ? tmax : std::min(tmax, hit.t + tol);
+        Hit alt;
+        if (!intersectRaw(ray, lo, hi, alt, hit.boundary, hit.prim)) return true;
+        if (incident(alt) != region || (detector(alt) && fits)) return true;
+        hit = alt;
+    }
     return true;
 }
 

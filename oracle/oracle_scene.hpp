@@ -205,7 +205,17 @@ struct Observer {
 class World {
 public:
     explicit World(const owe_scene& s);
-    bool intersect(const Ray& ray, double tmax, Hit& hit) const;
+    // The nearest crossing of `ray` before tmax, with contact resolution when the ray's region is
+    // known (region != kNone): a spawned ray meets other boundaries from t > −tol (tol the contact
+    // tolerance at its origin), and a crossing met from another region than the ray's, or a detector
+    // crossing, gives way to a crossing at the same place (within tol of its point; anywhere ahead for
+    // a contact at the ray's own origin) that is met from the ray's region and is not a detector.
+    bool intersect(const Ray& ray, double tmax, Hit& hit, uint32_t region = kNone) const;
+    // Contact tolerance at a point: 2^-19 of its largest coordinate relative to the rebase origin.
+    double contactTol(V3 p) const;
+    // The nearest crossing with t in (lo, hi) — the departed boundary's own crossings only for t > max(lo, 0)
+    // by its own rules — skipping boundary `skipB` (for a mesh, only the face of triangle `skipPrim`).
+    bool intersectRaw(const Ray& ray, double lo, double hi, Hit& hit, uint32_t skipB, uint32_t skipPrim) const;
     Ray spawn(const Hit& h, V3 dir) const { return Ray{h.p, dir, h.boundary, h.prim}; }
     const Medium& mediumOf(uint32_t region) const { return media[regionMedium[region]]; }
     double indexOf(uint32_t region, double l) const { return mediumOf(region).index.n(l); }
@@ -224,6 +234,11 @@ public:
     std::vector<Boundary> boundaries;
     Environment env;
     Observer observer;
+    V3 rebase;  // the eye, or the sensor's centre
+    // Two distinct boundaries' boxes meet within the contact tolerance: only then do rays resolve
+    // contacts (a spawned ray's window below zero, and the region rule).
+    bool contacts = false;
+    void findContacts();
     Box bounds;
     std::vector<Light> lights;
     std::vector<double> lightPdf, lightCdf;

@@ -219,7 +219,7 @@ Spec4 Tracer::transmittance(V3 p, const Ray& start, uint32_t region, V3 dir, dou
     double remaining = dist;
     for (int guard = 0; guard < 256; ++guard) {
         Hit h;
-        bool found = w_.intersect(ray, remaining, h);
+        bool found = w_.intersect(ray, remaining, h, region);
         while (found && h.boundary == exclude) found = false;  // the segment's own end
         const double seg = found ? std::min(h.t, remaining) : remaining;
         const Medium& m = w_.mediumOf(region);
@@ -368,7 +368,7 @@ Spec4 Tracer::radiance(Ray ray, uint32_t region, Wavelengths& wl, Pcg& rng, Stat
     st.paths++;
     for (int depth = 0;; ++depth) {
         Hit hit;
-        const bool found = w_.intersect(path.ray, kInf, hit);
+        const bool found = w_.intersect(path.ray, kInf, hit, path.region);
         st.segments++;
         const Medium& med = w_.mediumOf(path.region);
         double t;
@@ -664,7 +664,7 @@ void Tracer::particle(Pcg& rng, Stats& st, Splat& out) const {
     Spec4 beta = em.beta;
     for (int depth = 0;; ++depth) {
         Hit hit;
-        const bool found = w_.intersect(ray, kInf, hit);
+        const bool found = w_.intersect(ray, kInf, hit, region);
         st.segments++;
         const Medium& med = w_.mediumOf(region);
         double t;

@@ -158,6 +158,7 @@ class Oracle(_Lib):
         return {
             "world_create": (_vp, [_vp]),
             "world_destroy": (None, [_vp]),
+            "world_contacts": (_i, [_vp]),
             "light_table": (_i, [_vp, _vp, _vp, _i]),
             "render": (None, [_vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp]),
             "camera_paths": (None, [_vp, _vp, _i, _i, _u64, _vp, _vp, _vp, _vp, _i]),
@@ -220,6 +221,10 @@ class World:
             self.o.world_destroy(self.h)
         except Exception:
             pass
+
+    @property
+    def contacts(self) -> bool:
+        return bool(self.o.world_contacts(self.h))
 
     def light_table(self):
         pdf = np.zeros(64)
