@@ -504,12 +504,12 @@ HostScene flattenScene(const owe_scene& s) {
     }
     h.hasTriangles = !triOwner.empty();
     // Contacts are possible only where two distinct boundaries' boxes meet, within the contact
-    // tolerance (2^-19 of the largest coordinate of either box).
+    // tolerance (2^-21 of the largest coordinate of either box).
     for (size_t i = 0; i < boxes.size() && !h.contacts; ++i)
         for (size_t j = i + 1; j < boxes.size() && !h.contacts; ++j) {
             double m = 0;
             for (int k = 0; k < 6; ++k) m = std::max({m, std::fabs(boxes[i][k]), std::fabs(boxes[j][k])});
-            const double tol = std::ldexp(m, -19);
+            const double tol = std::ldexp(m, -21);
             bool meet = true;
             for (int k = 0; k < 3; ++k)
                 meet = meet && boxes[i][k] - tol <= boxes[j][k + 3] && boxes[j][k] - tol <= boxes[i][k + 3];

@@ -519,9 +519,10 @@ __device__ __forceinline__ bool intersectWorld(const DScene& sc, f3 o, f3 d, flo
     return true;
 }
 
-// Contact tolerance at a point (camera-relative coordinates): 2^-19 of its largest coordinate.
+// Contact tolerance at a point (camera-relative coordinates): 2^-21 of its largest coordinate, two to
+// four float ulps.
 __device__ __forceinline__ float contactTol(f3 p) {
-    return ldexpf(fmaxf(fmaxf(fabsf(p.x), fabsf(p.y)), fabsf(p.z)), -19);
+    return ldexpf(fmaxf(fmaxf(fabsf(p.x), fabsf(p.y)), fabsf(p.z)), -21);
 }
 
 // World::intersect with contact resolution, for scenes whose bodies touch (sc.contacts): a spawned
@@ -558,7 +559,8 @@ __device__ __forceinline__ bool intersectContact(const DScene& sc, f3 o, f3 d, f
         fits = xfits;
         if (!contacts || region == kRegionNone || (xfits && !xdet)) break;
         const bool atOrigin = x.t <= dep;
-        const float tol = contactTol(x.p);
+        // The window about the crossing: the point's tolerance plus a few ulps of t itself.
+        const float tol = contactTol(x.p) + ldexpf(x.t, -21);
         lo = atOrigin ? -dep : x.t - tol;
         hi = atOrigin ? tmax : fminf(tmax, x.t + tol);
         skipB = x.slot == kEmptyChild ? x.boundary : kNoBoundary;

@@ -478,7 +478,7 @@ void World::findContacts() {
             const V3 al = a.lo - rebase, ah = a.hi - rebase, cl = c.lo - rebase, ch = c.hi - rebase;
             double m = 0;
             for (const V3& v : {al, ah, cl, ch}) m = std::max({m, std::fabs(v.x), std::fabs(v.y), std::fabs(v.z)});
-            const double tol = std::ldexp(m, -19);
+            const double tol = std::ldexp(m, -21);
             contacts = al.x - tol <= ch.x && cl.x - tol <= ah.x && al.y - tol <= ch.y && cl.y - tol <= ah.y &&
                        al.z - tol <= ch.z && cl.z - tol <= ah.z;
         }
@@ -486,7 +486,7 @@ void World::findContacts() {
 
 double World::contactTol(V3 p) const {
     const V3 q = p - rebase;
-    return std::ldexp(std::max({std::fabs(q.x), std::fabs(q.y), std::fabs(q.z)}), -19);
+    return std::ldexp(std::max({std::fabs(q.x), std::fabs(q.y), std::fabs(q.z)}), -21);
 }
 
 bool World::intersectRaw(const Ray& ray, double lo, double hi, Hit& hit, uint32_t skipB, uint32_t skipPrim) const {
@@ -591,7 +591,7 @@ bool World::intersect(const Ray& ray, double tmax, Hit& hit, uint32_t region) co
         const bool fits = incident(hit) == region;
         if (fits && !detector(hit)) return true;
         const bool atOrigin = hit.t <= dep;
-        const double tol = contactTol(hit.p);
+        const double tol = contactTol(hit.p) + std::ldexp(hit.t, -21);
         const double lo = atOrigin ? -dep : hit.t - tol, hi = atOrigin ? tmax : std::min(tmax, hit.t + tol);
         Hit alt;
         if (!intersectRaw(ray, lo, hi, alt, hit.boundary, hit.prim)) return true;

@@ -211,7 +211,8 @@ public:
     // crossing, gives way to a crossing at the same place (within tol of its point; anywhere ahead for
     // a contact at the ray's own origin) that is met from the ray's region and is not a detector.
     bool intersect(const Ray& ray, double tmax, Hit& hit, uint32_t region = kNone) const;
-    // Contact tolerance at a point: 2^-19 of its largest coordinate relative to the rebase origin.
+    // Contact tolerance at a point: 2^-21 (two to four float ulps) of its largest coordinate relative to
+    // the rebase origin.
     double contactTol(V3 p) const;
     // The nearest crossing with t in (lo, hi) — the departed boundary's own crossings only for t > max(lo, 0)
     // by its own rules — skipping boundary `skipB` (for a mesh, only the face of triangle `skipPrim`).
