@@ -112,12 +112,15 @@ int owe_hip_mesh_get_launch(const owe_hip_mesh* mesh, int mode, int* form, int* 
  * The scene is copied at creation: the caller's arrays need not outlive the call. */
 typedef struct owe_hip_renderer owe_hip_renderer;
 
-/* Integrators: OWE_INTEGRATOR_PATH (camera paths, Tracer::radiance, transport.cpp:473-529) and
+/* Integrators: OWE_INTEGRATOR_PATH (camera paths, Tracer::radiance, transport.cpp:473-529),
  * OWE_INTEGRATOR_LIGHT (particles emitted from the lights and the sky, connected to an observer or
  * recorded by a surface sensor: the reference's "light" integrator, particle.slang:356-427, whose
  * passes emit W·H·spp particles, gpu_renderer_passes.cpp:303-311, and whose film is divided by
- * the particles emitted, gpu_renderer_film.cpp:17-20). Fails with OWE_HIP_ERR_UNSUPPORTED for
- * any other integrator (renderer.hpp:22-36). */
+ * the particles emitted, gpu_renderer_film.cpp:17-20) and OWE_INTEGRATOR_SPPM (stochastic
+ * progressive photon mapping, sppm.slang; a pass of spp runs spp iterations, each one camera path
+ * per pixel and photons_per_pixel·W·H photons, GpuRenderer::passSppm, gpu_renderer_passes.cpp:259-290;
+ * resolve gives L_direct / iterations + τ / (N_emitted π r²)). Fails with OWE_HIP_ERR_UNSUPPORTED
+ * for any other integrator (renderer.hpp:22-36). */
 int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render_settings* settings,
                             owe_hip_renderer** out);
 int owe_hip_renderer_destroy(owe_hip_renderer* renderer);
@@ -125,6 +128,11 @@ int owe_hip_renderer_destroy(owe_hip_renderer* renderer);
 int owe_hip_renderer_pass(owe_hip_renderer* renderer, int spp);
 /* Renderer::resolve: width × height × 3 doubles of XYZ (row-major), film ÷ samples per pixel. */
 int owe_hip_renderer_resolve(owe_hip_renderer* renderer, double* xyz);
+/* The SPPM state per pixel (SppmPixel, sppm.slang:43-46): width × height × 5 doubles of τ (XYZ), N and
+ * the gather radius r; the photons emitted per iteration and the iterations so far. Any output may be
+ * null. Fails unless the renderer's integrator is OWE_INTEGRATOR_SPPM. */
+int owe_hip_renderer_sppm_state(owe_hip_renderer* renderer, double* state, uint64_t* photons_per_iteration,
+                                long long* iterations);
 /* Renderer::stats: paths, segments, inconsistencies, leaks (transport_stats.hpp:12-31). */
 int owe_hip_renderer_stats(const owe_hip_renderer* renderer, owe_transport_stats* out);
 /* Renderer::passes() and samplesPerPixel() (renderer.hpp:48-49). */

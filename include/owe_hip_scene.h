@@ -188,6 +188,7 @@ typedef struct owe_scene {
 
 #define OWE_INTEGRATOR_PATH 0
 #define OWE_INTEGRATOR_LIGHT 1
+#define OWE_INTEGRATOR_SPPM 2
 
 /* Wavelength sampling: the reference's four stratified hero wavelengths importance-sampled over
  * 360–830 nm (core/wavelengths.hpp:36-45), or one wavelength per path uniform over
@@ -203,6 +204,13 @@ typedef struct owe_render_settings {
     int32_t spectral;
     double lambda_min, lambda_max;
     double fresnel_floor;
+    /* Stochastic progressive photon mapping (RenderSettings, scene/scene.hpp:27-35): the initial
+     * gather radius in pixel footprints, the radius-reduction alpha in (0, 1], photons per pixel and
+     * iteration, and wavelength groups per iteration (1–255). A zero selects the reference's
+     * default (2, 2/3, 1, 4); the other integrators ignore them. */
+    double photon_radius, photon_alpha, photons_per_pixel;
+    int32_t photon_groups;
+    int32_t reserved;
 } owe_render_settings;
 
 /* TransportStats (scene/transport_stats.hpp:12-31). */

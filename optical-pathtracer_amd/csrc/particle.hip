@@ -21,6 +21,7 @@
 #include "common.hpp"
 #include "particle.hpp"
 #include "shading_dev.hpp"
+#include "sppm.hpp"
 
 namespace owehip {
 namespace {
@@ -303,13 +304,85 @@ __device__ __forceinline__ bool roulette(PCtx& c, float q) {
     return true;
 }
 
-// traceParticle (particle.slang:356-427), without the hybrid partition.
-template <bool kContacts>
-__device__ __forceinline__ void traceParticle(PCtx& c) {
+// An SPPM photon arriving at a non-specular surface point along d (gatherPhoton, sppm.slang:88-148):
+// adds its contribution to every visible point within reach, on a disc in the point's tangent plane
+// with a matching surface orientation, whose wavelengths agree with the photon's.
+__device__ void sppmGather(PCtx& c, const SppmDev& s, const Hit& h, f3 d, uint32_t group) {
+    const DScene& sc = c.sc;
+    const int lo = s.levels[0], hi = s.levels[1];
+    if (lo > hi) return;  // no visible points this iteration
+    const f3 wi = -d;
+    for (int level = lo; level <= hi; ++level) {
+        const float size = ldexpf(1.0f, level);
+        const int bx = sppmCell(h.p.x - 0.5f * size, size), by = sppmCell(h.p.y - 0.5f * size, size),
+                  bz = sppmCell(h.p.z - 0.5f * size, size);
+        for (int k = 0; k < 8; ++k) {
+            const int cx = bx + (k & 1), cy = by + ((k >> 1) & 1), cz = bz + ((k >> 2) & 1);
+            for (uint32_t i = s.head[sppmHash(cx, cy, cz, level, s.gridMask)]; i != kGridNone; i = s.next[i]) {
+                const float4 vp = s.vpP[i];
+                const float r = vp.w;
+                const f3 dp = h.p - mk3(vp.x, vp.y, vp.z);
+                if (dot3(dp, dp) > r * r) continue;
+                // The point's own cell (a bucket is shared by hash collisions): counted once.
+                if (sppmLevel(r) != level || sppmCell(vp.x, size) != cx || sppmCell(vp.y, size) != cy ||
+                    sppmCell(vp.z, size) != cz)
+                    continue;
+                const float4 vn = s.vpN[i];
+                const f3 n = mk3(vn.x, vn.y, vn.z);
+                if (fabsf(dot3(dp, n)) > 0.25f * r || dot3(n, h.n) < 0.5f) continue;
+                const float4 vd = s.vpD[i];
+                const uint32_t flags = __float_as_uint(vd.w);
+                const bool free = (flags & kVpFree) != 0;
+                if (!free && ((flags >> 8) & 0xFFu) != group) continue;  // wavelengths must agree
+                WL w = c.wl;  // the photon's (a free point is evaluated at them)
+                if (!free && (flags & kVpTerminated) && !secondaryTerminated(w)) {
+                    w.pdf[0] /= NW;
+                    w.pdf[1] = w.pdf[2] = w.pdf[3] = 0.0f;
+                }
+                const float4 vb = s.vpBeta[i];
+                const DBoundary& b = sc.boundaries[__float_as_uint(vn.w)];
+                const DOptics& o = sc.optics[b.optics];
+                const f3 wo = mk3(vd.x, vd.y, vd.z);
+                float X = 0.0f, Y = 0.0f, Z = 0.0f;
+                bool any = false;
+#pragma unroll
+                for (int j = 0; j < NW; ++j) {
+                    if (w.pdf[j] == 0.0f) continue;
+                    const float l = w.lambda[j];
+                    const Iface it{&o, n, regionIndex(sc, b.front, l), regionIndex(sc, b.back, l), l, c.set.fresnelFloor};
+                    float pdf;
+                    const float f = evalScatter(sc, it, wo, wi, true, pdf);
+                    // A point bound to its group sees 1/groups of the photons: its share is scaled back up.
+                    const float v = (free ? vb.x : get(vb, j) * float(s.groups)) * f * c.beta[j];
+                    if (v == 0.0f) continue;
+                    any = true;
+                    const float wt = v / w.pdf[j] * s.invNWY;
+                    const f3 cm = cieXYZ(l);
+                    X += wt * cm.x;
+                    Y += wt * cm.y;
+                    Z += wt * cm.z;
+                }
+                if (!any || !isfinite(X + Y + Z)) continue;
+                unsigned long long* bins = s.flux + size_t(i) * (3 * kSplatBins);
+                addFixedPoint(bins, X);
+                addFixedPoint(bins + kSplatBins, Y);
+                addFixedPoint(bins + 2 * kSplatBins, Z);
+                atomicAdd(&s.count[i], 1u);
+            }
+        }
+    }
+}
+
+// traceParticle (particle.slang:356-427), without the hybrid partition; with kSppm an SPPM photon
+// (traceSppmPhoton, sppm.slang:150-201): no observer connections or sensor splats, a gather at every
+// non-specular surface after the first scattering event.
+template <bool kContacts, bool kSppm = false>
+__device__ __forceinline__ void traceParticle(PCtx& c, const SppmDev* sp = nullptr, uint32_t group = 0) {
     const DScene& sc = c.sc;
     const DObserver& ob = sc.obs;
-    const bool observer = ob.kind == OWE_DETECTOR_OBSERVER;
+    const bool observer = !kSppm && ob.kind == OWE_DETECTOR_OBSERVER;
     const int maxDepth = c.set.maxDepth, rrDepth = c.set.rrDepth;
+    int events = 0;  // scattering events so far (SPPM: direct light is the camera pass's)
     PRay ray;
     if (!emitParticle<kContacts>(c, observer, ray)) return;
     for (int depth = 0;; ++depth) {
@@ -348,6 +421,7 @@ __device__ __forceinline__ void traceParticle(PCtx& c) {
                 ray.d = wi;
                 ray.departed = kNoBoundary;
                 ray.face = ~0u;
+                ++events;
                 if (depth >= rrDepth && !roulette(c, 0.95f)) return;
                 continue;
             }
@@ -383,7 +457,7 @@ __device__ __forceinline__ void traceParticle(PCtx& c) {
             if (depth >= 4 * maxDepth) return;
             continue;
         }
-        if (!observer && fromFront && bi == ob.sensor) {
+        if (!kSppm && !observer && fromFront && bi == ob.sensor) {
             // sensorPixel (particle.slang:98-114): the flux per pixel area.
             const float W = float(ob.width), H = float(ob.height);
             float fx = (h.pLocal.x / ob.halfX + 1.0f) * 0.5f * W;
@@ -406,6 +480,7 @@ __device__ __forceinline__ void traceParticle(PCtx& c) {
         const float hero = c.wl.lambda[0];
         const Iface it{&o, h.n, regionIndex(sc, b.front, hero), regionIndex(sc, b.back, hero), hero, c.set.fresnelFloor};
         if (!o.delta && observer) connectVertex<kContacts>(c, h.p, true, &it, bi, hitFace, h.n, ray.d, 0.0f, ray.region);
+        if (kSppm && !o.delta && events >= 1) sppmGather(c, *sp, h, ray.d, group);
         if (depth >= maxDepth) return;
         auto at = [&](int i) {
             Iface r = it;
@@ -442,6 +517,7 @@ __device__ __forceinline__ void traceParticle(PCtx& c) {
         ray.d = s.wi;
         ray.departed = bi;
         ray.face = hitFace;
+        ++events;
         // Particles carry flux: roulette against the event's own throughput.
         if (depth >= rrDepth && s.weight < 1.0f && !roulette(c, s.weight)) return;
     }
@@ -493,7 +569,49 @@ __global__ __launch_bounds__(kPBlock) void particleKernel(DScene sc, Settings se
             if (n[k]) atomicAdd(&pb.stats[k], (unsigned long long)n[k]);
 }
 
+// One thread per SPPM photon (sppm_photon_kernel): photon i of the iteration has its own stream
+// (seed, i, iteration) and the wavelengths of its group, i mod G.
+template <int kCap, bool kContacts>
+__global__ __launch_bounds__(kPBlock) void sppmPhotonKernel(DScene sc, Settings set, SppmDev s, uint64_t base,
+                                                            uint32_t count) {
+    __shared__ uint32_t stack[kCap * kPBlock];
+    const uint32_t i = blockIdx.x * kPBlock + threadIdx.x;
+    const ParticleBatch none{};
+    PCtx c{sc, set, none, stack + threadIdx.x};
+    c.sum[0] = c.sum[1] = c.sum[2] = 0.0f;
+    c.segments = c.inconsistencies = c.leaks = 0;
+    const bool live = i < count;
+    if (live) {
+        const uint64_t index = base + i;
+        c.rng.seed(hashPair(set.seed ^ kSppmPhoton, index), s.iteration);
+        const uint32_t group = uint32_t(index % uint64_t(s.groups));
+        c.wl = sppmWavelengths(set, s.rotation, group, s.groups);
+        traceParticle<kContacts, true>(c, &s, group);
+    }
+    const uint32_t n[4] = {waveSum(live ? 1u : 0u), waveSum(c.segments), waveSum(c.inconsistencies), waveSum(c.leaks)};
+    if (__lane_id() == 0)
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (n[k]) atomicAdd(&s.stats[k], (unsigned long long)n[k]);
+}
+
 }  // namespace
+
+void launchSppmPhotons(const DScene& sc, const Settings& set, const SppmDev& s, uint64_t base, uint32_t count,
+                       uint32_t stackCap, hipStream_t stream) {
+    if (count == 0) return;
+    const dim3 grid{(count + kPBlock - 1) / kPBlock}, block{kPBlock};
+    if (sc.contacts) {
+        if (stackCap <= 16) hipLaunchKernelGGL((sppmPhotonKernel<16, true>), grid, block, 0, stream, sc, set, s, base, count);
+        else if (stackCap <= 32) hipLaunchKernelGGL((sppmPhotonKernel<32, true>), grid, block, 0, stream, sc, set, s, base, count);
+        else hipLaunchKernelGGL((sppmPhotonKernel<48, true>), grid, block, 0, stream, sc, set, s, base, count);
+    } else {
+        if (stackCap <= 16) hipLaunchKernelGGL((sppmPhotonKernel<16, false>), grid, block, 0, stream, sc, set, s, base, count);
+        else if (stackCap <= 32) hipLaunchKernelGGL((sppmPhotonKernel<32, false>), grid, block, 0, stream, sc, set, s, base, count);
+        else hipLaunchKernelGGL((sppmPhotonKernel<48, false>), grid, block, 0, stream, sc, set, s, base, count);
+    }
+    OWE_HIP_CHECK(hipGetLastError());
+}
 
 void launchParticles(const DScene& sc, const Settings& set, const ParticleBatch& b, uint32_t stackCap,
                      hipStream_t stream) {

@@ -26,6 +26,7 @@
 #include "common.hpp"
 #include "particle.hpp"
 #include "shading_dev.hpp"
+#include "sppm.hpp"
 
 using namespace owehip;
 
@@ -92,6 +93,9 @@ struct PathStart {
     uint64_t rngState;
 };
 
+__device__ __forceinline__ void cameraRay(const DScene& sc, uint32_t x, uint32_t y, Rng& rng, float4 lam, float4 beta,
+                                          PathStart& out);
+
 __device__ __forceinline__ void startPath(const DScene& sc, const Settings& set, uint64_t pixel, uint64_t s, uint32_t spp,
                                  uint32_t pass, PathStart& out) {
     const DObserver& ob = sc.obs;
@@ -119,6 +123,14 @@ __device__ __forceinline__ void startPath(const DScene& sc, const Settings& set,
         }
         lam = make_float4(ls[0], ls[1], ls[2], ls[3]);
     }
+    cameraRay(sc, x, y, rng, lam, beta, out);
+}
+
+// The camera ray of a pixel from `rng`: position in the pixel, filter offset, then the detector's
+// own draws (IdealObserver/SurfaceSensor::generate, detector.cpp:61-149).
+__device__ __forceinline__ void cameraRay(const DScene& sc, uint32_t x, uint32_t y, Rng& rng, float4 lam, float4 beta,
+                                          PathStart& out) {
+    const DObserver& ob = sc.obs;
     const float px = float(x) + rng.uniform(), py = float(y) + rng.uniform();
     float fx = 0.0f, fy = 0.0f;
     if (ob.sigma > 0.0f) {
@@ -352,8 +364,22 @@ __device__ __forceinline__ bool survives(ShadeCtx& c) {
     return true;
 }
 
-// One iteration of Tracer::radiance's loop after the intersection; false when the path ends.
-__device__ __forceinline__ bool shadeOne(ShadeCtx& c, const uint4 hraw, uint32_t* counters) {
+// An SPPM camera path's visible point and what decides it (transport.slang:749-762, 783-791, 846-858).
+struct SppmRec {
+    bool valid = false;
+    bool chromatic = false;  // an absorbing, non-scattering medium on the way
+    uint32_t events = 0, nonDelta = 0, boundary = 0, flags = 0;
+    float pathLength = 0.0f;  // unfolded length from the eye
+    f3 p, n, d;
+    float4 beta;
+};
+
+// One iteration of Tracer::radiance's loop after the intersection; false when the path ends. With
+// kSppm the SPPM camera pass's rule: the first non-specular surface reached through specular events
+// only is the visible point; the path takes next-event estimation and one scatter sample there and
+// ends at the next matter it meets (light scattered by a medium on that segment is the photons').
+template <bool kSppm = false>
+__device__ __forceinline__ bool shadeOne(ShadeCtx& c, const uint4 hraw, uint32_t* counters, SppmRec* rec = nullptr) {
     const DScene& sc = c.sc;
     const bool found = hraw.y != kNoBoundary;
     const float t = __uint_as_float(hraw.x);
@@ -373,11 +399,16 @@ __device__ __forceinline__ bool shadeOne(ShadeCtx& c, const uint4 hraw, uint32_t
                 setc(c.beta, i, get(c.beta, i) * evalSpec(med.scattering, sc.table, c.wl.lambda[i]) * expf(-sT[i] * tm) / pdf0);
             // scatterInMedium (transport.cpp:379-397).
             const f3 p = c.o + c.d * tm;
+            if (kSppm && rec->valid) return false;
             if (int(c.depth) >= c.set.maxDepth) return false;
             prepareDirect(c, p, false, nullptr, kNoBoundary, 0, kEmptyChild, mk3(0, 0, 0), med.g);
             const float u2 = c.rng.uniform(), u1 = c.rng.uniform();
             f3 wi;
             c.prevPdf = sampleHg(-c.d, med.g, u1, u2, wi);
+            if (kSppm) {
+                rec->events++;
+                rec->nonDelta++;
+            }
             c.flags &= ~kFlagSpecular;
             c.prevP = p;
             c.o = p;
@@ -395,6 +426,7 @@ __device__ __forceinline__ bool shadeOne(ShadeCtx& c, const uint4 hraw, uint32_t
 #pragma unroll
         for (int i = 0; i < NW; ++i) setc(c.beta, i, get(c.beta, i) * expf(-evalSpec(med.absorption, sc.table, c.wl.lambda[i]) * t));
     }
+    if (kSppm && (med.flags & 1) && !(med.flags & 2)) rec->chromatic = true;
     if (!found) {
         if (c.region == 0) {
             // skyMet (transport.cpp:401-417); the sun's disc test in the cancellation-free form
@@ -435,6 +467,7 @@ __device__ __forceinline__ bool shadeOne(ShadeCtx& c, const uint4 hraw, uint32_t
         atomicAdd(&counters[2], 1u);
         c.region = inc;
     }
+    if (kSppm && !rec->valid) rec->pathLength += t;
     const DOptics& o = sc.optics[b.optics];
     // emissionMet (transport.cpp:420-433).
     if (b.emission >= 0) {
@@ -458,6 +491,7 @@ __device__ __forceinline__ bool shadeOne(ShadeCtx& c, const uint4 hraw, uint32_t
         c.face = hitFace;
         return int(c.depth) < 4 * c.set.maxDepth;
     }
+    if (kSppm && rec->valid) return false;  // the visible point's scatter segment has met matter
     if (o.type == OWE_SURF_ABSORBER || o.type == OWE_SURF_DETECTOR) return false;
     if (int(c.depth) >= c.set.maxDepth) return false;
     // scatterAtSurface (transport.cpp:437-471).
@@ -485,10 +519,24 @@ __device__ __forceinline__ bool shadeOne(ShadeCtx& c, const uint4 hraw, uint32_t
             }
         }
     }
+    if (kSppm && !o.delta && rec->nonDelta == 0) {
+        rec->valid = true;
+        rec->p = p;
+        rec->n = n;
+        rec->d = c.d;
+        rec->boundary = bi;
+        rec->beta = c.beta;
+        rec->flags = kVpValid | (secondaryTerminated(c.wl) ? kVpTerminated : 0u) |
+                     (rec->events == 0 && !rec->chromatic ? kVpFree : 0u);
+    }
     if (!o.delta) prepareDirect(c, p, true, &it, bi, hraw.z, hraw.w, n, 0.0f);
     const float u3 = c.rng.uniform(), u2 = c.rng.uniform(), u1 = c.rng.uniform();
     SSample s;
     if (!sampleScatter(sc, it, c.d, u1, u2, u3, true, s) || s.weight <= 0.0f) return false;
+    if (kSppm) {
+        rec->events++;
+        if (!s.delta) rec->nonDelta++;
+    }
     c.beta.x *= s.weight;
 #pragma unroll
     for (int i = 1; i < NW; ++i)
@@ -860,6 +908,139 @@ __global__ __launch_bounds__(kBlock, kWaves) void persistentKernel(DScene sc, Se
     if (lane == 0 && segments) atomicAdd(a.segments, (unsigned long long)segments);
 }
 
+// ---------------------------------------------------------------- SPPM
+
+// The SPPM camera pass (sppm_camera_kernel): one thread per pixel and one camera path per iteration,
+// at the wavelengths of the pixel's group. The path's own radiance goes to the film (one sample per
+// pixel per iteration); its visible point, if any, to the pixel's slot, with the pixel's radius (set
+// at its first visible point from the unfolded path length).
+template <int kCap, bool kContacts>
+__global__ __launch_bounds__(kBlock) void sppmCameraKernel(DScene sc, Settings set, SppmDev s, unsigned long long* film,
+                                                           uint32_t* counters, int simple) {
+    __shared__ uint32_t stack[kCap * kBlock];
+    uint32_t* st = stack + threadIdx.x;
+    const uint32_t pixel = blockIdx.x * kBlock + threadIdx.x;
+    if (pixel >= s.pixels) return;
+    const uint32_t group = uint32_t(hashPair(hashPair(set.seed ^ kSppmGroup, pixel), s.iteration) % uint64_t(s.groups));
+    ShadeCtx c{sc, set};
+    c.wl = sppmWavelengths(set, s.rotation, group, s.groups);
+    Rng rng;
+    rng.seed(hashPair(set.seed ^ kSppmCamera, pixel), s.iteration);
+    const float4 lam = make_float4(c.wl.lambda[0], c.wl.lambda[1], c.wl.lambda[2], c.wl.lambda[3]);
+    const float4 beta0 = set.spectral == OWE_SPECTRAL_UNIFORM1 ? make_float4(1.0f, 0.0f, 0.0f, 0.0f)
+                                                               : make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    PathStart ps;
+    cameraRay(sc, pixel % sc.obs.width, pixel / sc.obs.width, rng, lam, beta0, ps);
+    SppmRec rec;
+    uint32_t segments = 0;
+    c.L = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (ps.weight > 0.0f) {
+        c.rng = rng;
+        c.flags = kFlagSpecular;
+        c.depth = 0;
+        c.beta = ps.beta;
+        c.o = ps.o;
+        c.prevPdf = 0.0f;
+        c.d = ps.d;
+        c.weight = ps.weight;
+        c.prevP = ps.o;
+        c.pixel = pixel;
+        c.region = ps.region;
+        c.departed = ps.departed;
+        c.face = ~0u;
+        for (bool alive = true; alive; c.depth += 1) {
+            Hit h;
+            const bool found = intersectContact<false, kContacts>(sc, c.o, c.d, __builtin_inff(), c.departed, c.face,
+                                                                  kNoBoundary, c.region, st, kBlock, h);
+            ++segments;
+            const uint4 hraw = found ? make_uint4(__float_as_uint(h.t), h.boundary, h.face, h.slot)
+                                     : make_uint4(__float_as_uint(__builtin_inff()), kNoBoundary, 0u, kEmptyChild);
+            c.shPend = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            alive = shadeOne<true>(c, hraw, counters, &rec);
+            if (c.shPend.x != 0.0f || c.shPend.y != 0.0f || c.shPend.z != 0.0f || c.shPend.w != 0.0f) {
+                float Tr[NW];
+                const float4 l4 = make_float4(c.wl.lambda[0], c.wl.lambda[1], c.wl.lambda[2], c.wl.lambda[3]);
+                if (shadowTransmittance<kContacts>(sc, c.shO, c.shD, c.shDist, c.shStart, c.shLightRegion, c.shExclude,
+                                                   c.shDeparted, c.shFace, l4, simple, st, Tr))
+                    c.L = addShadow(c.L, c.shPend, Tr);
+            }
+        }
+        float X, Y, Z;
+        if (pathXYZ(c.L, c.weight, c.wl, s.invNWY, X, Y, Z)) {
+            // One thread per pixel: the film's words are this thread's alone in this kernel.
+            unsigned long long* f = film + 3 * size_t(pixel);
+            f[0] += (unsigned long long)fixedPoint(X);
+            f[1] += (unsigned long long)fixedPoint(Y);
+            f[2] += (unsigned long long)fixedPoint(Z);
+        }
+        atomicAdd(&s.stats[0], 1ull);
+        atomicAdd(&s.stats[1], (unsigned long long)segments);
+    }
+    float r = 0.0f;
+    if (rec.valid) {
+        r = s.radius[pixel];
+        if (r == 0.0f) {
+            r = fmaxf(s.r0PerMetre * rec.pathLength, s.r0Floor);
+            s.radius[pixel] = r;
+        }
+    }
+    const float w = ps.weight;
+    s.vpP[pixel] = make_float4(rec.p.x, rec.p.y, rec.p.z, r);
+    s.vpN[pixel] = make_float4(rec.n.x, rec.n.y, rec.n.z, __uint_as_float(rec.boundary));
+    s.vpD[pixel] = make_float4(rec.d.x, rec.d.y, rec.d.z, __uint_as_float(rec.valid ? rec.flags | (group << 8) : 0u));
+    s.vpBeta[pixel] = make_float4(rec.beta.x * w, rec.beta.y * w, rec.beta.z * w, rec.beta.w * w);
+}
+
+__global__ void sppmClearGrid(SppmDev s) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= s.gridMask) s.head[i] = kGridNone;
+    if (i == 0) {
+        s.levels[0] = 0x7FFFFFFF;
+        s.levels[1] = int(0x80000001u);
+    }
+}
+
+// The visible points enter the hash (sppm_grid_kernel): each in the one cell of its level holding it.
+__global__ void sppmGridKernel(SppmDev s) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.pixels) return;
+    if (!(__float_as_uint(s.vpD[i].w) & kVpValid)) return;
+    const float4 p = s.vpP[i];
+    const int level = sppmLevel(p.w);
+    const float size = ldexpf(1.0f, level);
+    const uint32_t h = sppmHash(sppmCell(p.x, size), sppmCell(p.y, size), sppmCell(p.z, size), level, s.gridMask);
+    s.next[i] = atomicExch(&s.head[h], i);
+    atomicMin(&s.levels[0], level);
+    atomicMax(&s.levels[1], level);
+}
+
+// The progressive update (sppm_update_kernel): N' = N + αM, r' = r·√(N'/(N+M)), τ' = (τ + Φ)·N'/(N+M);
+// then the iteration's deposits are cleared. A pixel no photon reached keeps its state.
+__global__ void sppmUpdateKernel(SppmDev s) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.pixels) return;
+    const uint32_t M = s.count[i];
+    if (M == 0) return;
+    unsigned long long* bins = s.flux + size_t(i) * (3 * kSplatBins);
+    double phi[3];
+    for (int k = 0; k < 3; ++k) {
+        double v = 0.0;
+        for (int b = kSplatBins - 1; b >= 0; --b) v += ldexp(double(bins[k * kSplatBins + b]), kSplatExp0 + 32 * b);
+        phi[k] = v;
+    }
+    double2 xy = s.tauXY[i], zn = s.tauZN[i];
+    const double N = zn.y, Nn = N + double(s.alpha) * double(M), ratio = Nn / (N + double(M));
+    xy.x = (xy.x + phi[0]) * ratio;
+    xy.y = (xy.y + phi[1]) * ratio;
+    zn.x = (zn.x + phi[2]) * ratio;
+    zn.y = Nn;
+    s.tauXY[i] = xy;
+    s.tauZN[i] = zn;
+    s.radius[i] = float(double(s.radius[i]) * sqrt(ratio));
+    for (int k = 0; k < 3 * kSplatBins; ++k) bins[k] = 0ull;
+    s.count[i] = 0u;
+}
+
 }  // namespace
 }  // namespace owehip
 
@@ -888,6 +1069,10 @@ struct owe_hip_renderer {
     uint64_t* particleIndex = nullptr;
     double* perParticle = nullptr;
     uint32_t particleListCapacity = 0;
+    // SPPM: the device state, photons per iteration and iterations so far.
+    owehip::SppmDev sppm{};
+    uint64_t photons = 0;
+    uint64_t sppmIterations = 0;
     owe_transport_stats stats{};
     hipStream_t stream = nullptr;
     uint32_t stackCap = 16;
@@ -1180,6 +1365,62 @@ void runParticles(owe_hip_renderer* r, uint64_t total, uint32_t pass, const uint
     r->stats.leaks += st[3];
 }
 
+// An iteration's base wavelength rotation: a golden-ratio sequence offset by the seed (the rotation of
+// GpuRenderer::wavelengthRotation, gpu_renderer_passes.cpp:52-56, from this backend's seed hash).
+double sppmRotation(uint64_t seed, uint64_t iteration) {
+    const double offset = double(uint32_t(mix64(seed) >> 32)) / 4294967296.0;
+    const double u = offset + 0.6180339887498949 * double(iteration + 1);
+    return u - std::floor(u);
+}
+
+template <int kCap>
+void launchSppmCamera(owe_hip_renderer* r) {
+    const dim3 grid{(r->sppm.pixels + kBlock - 1) / kBlock}, block{kBlock};
+    if (r->dev.contacts)
+        hipLaunchKernelGGL((sppmCameraKernel<kCap, true>), grid, block, 0, r->stream, r->dev, r->set, r->sppm, r->film,
+                           r->persistCounters, r->simpleShadows ? 1 : 0);
+    else
+        hipLaunchKernelGGL((sppmCameraKernel<kCap, false>), grid, block, 0, r->stream, r->dev, r->set, r->sppm, r->film,
+                           r->persistCounters, r->simpleShadows ? 1 : 0);
+    OWE_HIP_CHECK(hipGetLastError());
+}
+
+// Iterations of SPPM (GpuRenderer::passSppm, gpu_renderer_passes.cpp:259-290): per iteration the hash
+// is cleared, then the camera pass, the hash fill, the photons and the update run in stream order.
+void runSppm(owe_hip_renderer* r, int iterations) {
+    SppmDev& q = r->sppm;
+    OWE_HIP_CHECK(hipMemsetAsync(q.stats, 0, 4 * sizeof(unsigned long long), r->stream));
+    OWE_HIP_CHECK(hipMemsetAsync(r->persistCounters, 0, 4 * sizeof(uint32_t), r->stream));
+    const dim3 block{256}, pixelGrid{(q.pixels + 255) / 256}, gridGrid{(q.gridMask + 256) / 256};
+    for (int k = 0; k < iterations; ++k) {
+        q.iteration = uint32_t(r->sppmIterations);
+        q.rotation = float(sppmRotation(r->set.seed, r->sppmIterations));
+        hipLaunchKernelGGL(sppmClearGrid, gridGrid, block, 0, r->stream, q);
+        if (r->stackCap <= 16) launchSppmCamera<16>(r);
+        else if (r->stackCap <= 32) launchSppmCamera<32>(r);
+        else launchSppmCamera<48>(r);
+        hipLaunchKernelGGL(sppmGridKernel, pixelGrid, block, 0, r->stream, q);
+        if (r->host.emittedPower > 0)
+            for (uint64_t base = 0; base < r->photons; base += r->capacity)
+                launchSppmPhotons(r->dev, r->set, q, base, uint32_t(std::min<uint64_t>(r->capacity, r->photons - base)),
+                                  r->stackCap, r->stream);
+        hipLaunchKernelGGL(sppmUpdateKernel, pixelGrid, block, 0, r->stream, q);
+        OWE_HIP_CHECK(hipGetLastError());
+        r->sppmIterations++;
+    }
+    unsigned long long st[4];
+    uint32_t counters[4];
+    OWE_HIP_CHECK(hipMemcpyAsync(st, q.stats, sizeof st, hipMemcpyDeviceToHost, r->stream));
+    OWE_HIP_CHECK(hipMemcpyAsync(counters, r->persistCounters, sizeof counters, hipMemcpyDeviceToHost, r->stream));
+    OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
+    // Photons of a scene where nothing emits are still emitted (and carry nothing).
+    const uint64_t silent = r->host.emittedPower > 0 ? 0 : r->photons * uint64_t(iterations);
+    r->stats.paths += st[0] + silent;
+    r->stats.segments += st[1];
+    r->stats.inconsistencies += st[2] + counters[2];
+    r->stats.leaks += st[3] + counters[3];
+}
+
 }  // namespace
 }  // namespace owehip
 
@@ -1196,8 +1437,18 @@ int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render
         const owe_render_settings& s = *settings;
         require(s.max_depth >= 0 && s.rr_depth >= 0, "depths must be non-negative");
         require(s.max_depth <= 4096, "max depth above 4096 is not supported (depth is kept in 16 bits)");
-        if (s.integrator != OWE_INTEGRATOR_PATH && s.integrator != OWE_INTEGRATOR_LIGHT)
-            fail(OWE_HIP_ERR_UNSUPPORTED, "unknown integrator (this backend implements path and light)");
+        if (s.integrator != OWE_INTEGRATOR_PATH && s.integrator != OWE_INTEGRATOR_LIGHT && s.integrator != OWE_INTEGRATOR_SPPM)
+            fail(OWE_HIP_ERR_UNSUPPORTED, "unknown integrator (this backend implements path, light and sppm)");
+        const double photonRadius = s.photon_radius > 0 ? s.photon_radius : 2.0;
+        const double photonAlpha = s.photon_alpha > 0 ? s.photon_alpha : 2.0 / 3.0;
+        const double photonsPerPixel = s.photons_per_pixel > 0 ? s.photons_per_pixel : 1.0;
+        const int photonGroups = s.photon_groups > 0 ? s.photon_groups : 4;
+        if (s.integrator == OWE_INTEGRATOR_SPPM) {
+            require(s.photon_radius >= 0 && s.photon_alpha >= 0 && s.photons_per_pixel >= 0 && s.photon_groups >= 0,
+                    "photon settings must not be negative");
+            require(photonAlpha <= 1.0, "photon_alpha must lie in (0, 1]");
+            require(photonGroups <= 255, "photon wavelength groups must be 1 to 255");
+        }
         require(s.spectral == OWE_SPECTRAL_HERO4 || s.spectral == OWE_SPECTRAL_UNIFORM1, "unknown spectral mode");
         if (s.spectral == OWE_SPECTRAL_UNIFORM1)
             require(s.lambda_min >= 360.0 && s.lambda_max <= 830.0 && s.lambda_min < s.lambda_max,
@@ -1272,6 +1523,54 @@ int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render
             r->persistSegments = devAlloc<unsigned long long>(r, 1);
             r->persistCounters = devAlloc<uint32_t>(r, 4);
             r->passSlot = devAlloc<uint32_t>(r, 1);
+        } else if (r->integrator == OWE_INTEGRATOR_SPPM) {
+            require(pixels <= (1u << 30), "SPPM keeps one visible point per pixel: at most 2^30 pixels");
+            SppmDev& q = r->sppm;
+            q.pixels = uint32_t(pixels);
+            q.vpP = devAlloc<float4>(r, pixels);
+            q.vpN = devAlloc<float4>(r, pixels);
+            q.vpD = devAlloc<float4>(r, pixels);
+            q.vpBeta = devAlloc<float4>(r, pixels);
+            q.radius = devAlloc<float>(r, pixels);
+            q.tauXY = devAlloc<double2>(r, pixels);
+            q.tauZN = devAlloc<double2>(r, pixels);
+            q.flux = devAlloc<unsigned long long>(r, pixels * 3 * kSplatBins);
+            q.count = devAlloc<uint32_t>(r, pixels);
+            q.next = devAlloc<uint32_t>(r, pixels);
+            uint64_t gridSize = 1024;
+            while (gridSize < 2 * pixels) gridSize <<= 1;
+            q.head = devAlloc<uint32_t>(r, gridSize);
+            q.gridMask = uint32_t(gridSize - 1);
+            q.levels = devAlloc<int>(r, 2);
+            q.stats = devAlloc<unsigned long long>(r, 4);
+            q.groups = uint32_t(photonGroups);
+            q.alpha = float(photonAlpha);
+            q.invNWY = r->invNWY;
+            // The initial radius (GpuRenderer::resetSppm, gpu_renderer_passes.cpp:238-257): photon_radius
+            // pixel footprints at the distance the path travelled — a pixel's angle for an observer, a
+            // sensor pixel seen from its aim disc's centre; a sensor without one takes a fixed radius.
+            double pixelAngle = 0, floorRadius = 0;
+            const DObserver& ob = h.obs;
+            if (ob.kind == OWE_DETECTOR_OBSERVER) {
+                pixelAngle = 2.0 * double(ob.tanY) / double(ob.height);
+            } else {
+                const double pix = 2.0 * double(ob.halfY) / double(ob.height);
+                const double dx = double(ob.aimCenter[0]) - ob.toWorld[3], dy = double(ob.aimCenter[1]) - ob.toWorld[7],
+                             dz = double(ob.aimCenter[2]) - ob.toWorld[11];
+                const double dist = ob.hasAim ? std::sqrt(dx * dx + dy * dy + dz * dz) : 0.0;
+                if (dist > 0) pixelAngle = pix / dist;
+                else floorRadius = photonRadius * pix;
+            }
+            q.r0PerMetre = float(photonRadius * pixelAngle);
+            q.r0Floor = float(std::max(floorRadius, 1e-9));
+            r->photons = std::max<uint64_t>(1, uint64_t(std::llround(double(pixels) * photonsPerPixel)));
+            r->capacity = uint32_t(std::min<uint64_t>(r->photons, 1u << 22));  // photons per launch
+            r->persistCounters = devAlloc<uint32_t>(r, 4);
+            for (void* p : {static_cast<void*>(q.radius), static_cast<void*>(q.count)})
+                OWE_HIP_CHECK(hipMemset(p, 0, pixels * sizeof(uint32_t)));
+            OWE_HIP_CHECK(hipMemset(q.tauXY, 0, pixels * sizeof(double2)));
+            OWE_HIP_CHECK(hipMemset(q.tauZN, 0, pixels * sizeof(double2)));
+            OWE_HIP_CHECK(hipMemset(q.flux, 0, pixels * 3 * kSplatBins * sizeof(unsigned long long)));
         } else {
             r->capacity = uint32_t(std::min<uint64_t>(pixels * 64, 1u << 22));  // particles per launch
             r->lightFilm = devAlloc<unsigned long long>(r, 3 * kSplatBins * pixels);
@@ -1340,6 +1639,12 @@ int owe_hip_renderer_pass(owe_hip_renderer* r, int spp) {
             r->totalSpp += spp;
             return;
         }
+        if (r->integrator == OWE_INTEGRATOR_SPPM) {  // a pass of spp iterations
+            runSppm(r, spp);
+            r->passes++;
+            r->totalSpp += spp;
+            return;
+        }
         if (r->mode == OWE_HIP_MODE_PERSISTENT) {
             runPersistent(r, uint32_t(spp), uint32_t(r->passes));
         } else {
@@ -1354,6 +1659,39 @@ int owe_hip_renderer_pass(owe_hip_renderer* r, int spp) {
     });
 }
 
+namespace {
+// Per pixel: τ (XYZ), N and r.
+void sppmState(owe_hip_renderer* r, double* out) {
+    const SppmDev& q = r->sppm;
+    const size_t pixels = q.pixels;
+    std::vector<double2> xy(pixels), zn(pixels);
+    std::vector<float> rad(pixels);
+    OWE_HIP_CHECK(hipMemcpyAsync(xy.data(), q.tauXY, pixels * sizeof(double2), hipMemcpyDeviceToHost, r->stream));
+    OWE_HIP_CHECK(hipMemcpyAsync(zn.data(), q.tauZN, pixels * sizeof(double2), hipMemcpyDeviceToHost, r->stream));
+    OWE_HIP_CHECK(hipMemcpyAsync(rad.data(), q.radius, pixels * sizeof(float), hipMemcpyDeviceToHost, r->stream));
+    OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
+    for (size_t i = 0; i < pixels; ++i) {
+        out[5 * i + 0] = xy[i].x;
+        out[5 * i + 1] = xy[i].y;
+        out[5 * i + 2] = zn[i].x;
+        out[5 * i + 3] = zn[i].y;
+        out[5 * i + 4] = double(rad[i]);
+    }
+}
+}  // namespace
+
+int owe_hip_renderer_sppm_state(owe_hip_renderer* r, double* state, uint64_t* photons_per_iteration,
+                                long long* iterations) {
+    return guard([&] {
+        require(r != nullptr, "renderer is null");
+        require(r->integrator == OWE_INTEGRATOR_SPPM, "SPPM state needs an sppm-integrator renderer");
+        OWE_HIP_CHECK(hipSetDevice(r->device));
+        if (state) sppmState(r, state);
+        if (photons_per_iteration) *photons_per_iteration = r->photons;
+        if (iterations) *iterations = (long long)r->sppmIterations;
+    });
+}
+
 int owe_hip_renderer_resolve(owe_hip_renderer* r, double* xyz) {
     return guard([&] {
         require(r != nullptr && xyz != nullptr, "null argument");
@@ -1363,7 +1701,20 @@ int owe_hip_renderer_resolve(owe_hip_renderer* r, double* xyz) {
         OWE_HIP_CHECK(hipMemcpyAsync(film.data(), r->film, n * sizeof(long long), hipMemcpyDeviceToHost, r->stream));
         OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
         const double norm = r->totalSpp > 0 ? 1.0 / (kFilmScale * double(r->totalSpp)) : 0.0;
-        for (size_t i = 0; i < n; ++i) xyz[i] = r->integrator == OWE_INTEGRATOR_PATH ? double(film[i]) * norm : 0.0;
+        for (size_t i = 0; i < n; ++i) xyz[i] = r->integrator != OWE_INTEGRATOR_LIGHT ? double(film[i]) * norm : 0.0;
+        if (r->integrator == OWE_INTEGRATOR_SPPM && r->sppmIterations > 0) {
+            // L = L_direct / iterations + τ / (N_emitted π r²) (sppm.slang:19-20).
+            const size_t pixels = n / 3;
+            std::vector<double> state(5 * pixels);
+            sppmState(r, state.data());
+            const double emitted = double(r->photons) * double(r->sppmIterations);
+            for (size_t i = 0; i < pixels; ++i) {
+                const double rr = state[5 * i + 4];
+                if (!(rr > 0)) continue;
+                const double k = 1.0 / (emitted * 3.141592653589793 * rr * rr);
+                for (int c = 0; c < 3; ++c) xyz[3 * i + size_t(c)] += state[5 * i + size_t(c)] * k;
+            }
+        }
         if (r->lightFilm) {
             // Particle splats ÷ particles emitted (gpu_renderer_film.cpp:17-20): each channel's bins
             // are integers in units of 2^(kSplatExp0 + 32 b).
@@ -1404,6 +1755,16 @@ int owe_hip_renderer_reset(owe_hip_renderer* r) {
         OWE_HIP_CHECK(hipMemsetAsync(r->film, 0, n * sizeof(unsigned long long), r->stream));
         if (r->lightFilm)
             OWE_HIP_CHECK(hipMemsetAsync(r->lightFilm, 0, n * kSplatBins * sizeof(unsigned long long), r->stream));
+        if (r->integrator == OWE_INTEGRATOR_SPPM) {
+            const SppmDev& q = r->sppm;
+            const size_t pixels = q.pixels;
+            OWE_HIP_CHECK(hipMemsetAsync(q.radius, 0, pixels * sizeof(float), r->stream));
+            OWE_HIP_CHECK(hipMemsetAsync(q.count, 0, pixels * sizeof(uint32_t), r->stream));
+            OWE_HIP_CHECK(hipMemsetAsync(q.tauXY, 0, pixels * sizeof(double2), r->stream));
+            OWE_HIP_CHECK(hipMemsetAsync(q.tauZN, 0, pixels * sizeof(double2), r->stream));
+            OWE_HIP_CHECK(hipMemsetAsync(q.flux, 0, pixels * 3 * kSplatBins * sizeof(unsigned long long), r->stream));
+            r->sppmIterations = 0;
+        }
         OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
         r->passes = 0;
         r->totalSpp = 0;

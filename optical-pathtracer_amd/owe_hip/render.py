@@ -46,6 +46,14 @@ class Renderer:
         _lib.check(_lib.load().owe_hip_renderer_resolve(self._h, out.ctypes.data))
         return out
 
+    def sppm_state(self) -> dict:
+        """The SPPM state: per pixel τ (XYZ), N and r, photons per iteration and iterations so far."""
+        st = np.zeros((self.height, self.width, 5), dtype=np.float64)
+        photons, its = ctypes.c_uint64(), ctypes.c_longlong()
+        _lib.check(_lib.load().owe_hip_renderer_sppm_state(self._h, st.ctypes.data, ctypes.byref(photons),
+                                                            ctypes.byref(its)))
+        return dict(tau=st[..., :3], N=st[..., 3], r=st[..., 4], photons=photons.value, iterations=its.value)
+
     def stats(self) -> dict:
         s = CTransportStats()
         _lib.check(_lib.load().owe_hip_renderer_stats(self._h, ctypes.byref(s)))

@@ -27,7 +27,7 @@ SURF_NULL, SURF_DIELECTRIC, SURF_DIFFUSE, SURF_CONDUCTOR, SURF_MIRROR, SURF_ABSO
 SHAPE_MESH, SHAPE_PLANE, SHAPE_SPHERE, SHAPE_CYLINDER, SHAPE_SAG, SHAPE_WAVE = 0, 1, 2, 3, 4, 5
 APERTURE_DISK, APERTURE_RECT, APERTURE_ELLIPSE = 0, 1, 2
 SKY_NONE, SKY_UNIFORM = 0, 1
-INTEGRATOR_PATH, INTEGRATOR_LIGHT = 0, 1
+INTEGRATOR_PATH, INTEGRATOR_LIGHT, INTEGRATOR_SPPM = 0, 1, 2
 SPECTRAL_HERO4, SPECTRAL_UNIFORM1 = 0, 1
 DETECTOR_OBSERVER, DETECTOR_SENSOR = 0, 1
 
@@ -101,7 +101,9 @@ class CScene(ctypes.Structure):
 class CRenderSettings(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint64), ("max_depth", ctypes.c_int32), ("rr_depth", ctypes.c_int32),
                 ("integrator", ctypes.c_int32), ("spectral", ctypes.c_int32), ("lambda_min", ctypes.c_double),
-                ("lambda_max", ctypes.c_double), ("fresnel_floor", ctypes.c_double)]
+                ("lambda_max", ctypes.c_double), ("fresnel_floor", ctypes.c_double),
+                ("photon_radius", ctypes.c_double), ("photon_alpha", ctypes.c_double),
+                ("photons_per_pixel", ctypes.c_double), ("photon_groups", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class CTransportStats(ctypes.Structure):
@@ -440,10 +442,14 @@ class Scene:
 
 
 def settings(seed=1, max_depth=64, rr_depth=6, spectral=SPECTRAL_HERO4, lambda_min=380.0, lambda_max=780.0,
-             fresnel_floor=0.0, integrator=INTEGRATOR_PATH) -> CRenderSettings:
+             fresnel_floor=0.0, integrator=INTEGRATOR_PATH, photon_radius=2.0, photon_alpha=2.0 / 3.0,
+             photons_per_pixel=1.0, photon_groups=4) -> CRenderSettings:
+    """RenderSettings (scene/scene.hpp:15-40); the photon_* fields are the sppm integrator's."""
     s = CRenderSettings()
     s.seed, s.max_depth, s.rr_depth, s.integrator, s.spectral = seed, max_depth, rr_depth, integrator, spectral
     s.lambda_min, s.lambda_max, s.fresnel_floor = lambda_min, lambda_max, fresnel_floor
+    s.photon_radius, s.photon_alpha, s.photons_per_pixel = photon_radius, photon_alpha, photons_per_pixel
+    s.photon_groups = photon_groups
     return s
 
 

@@ -2,6 +2,7 @@
 // C entry points over the world and the camera-path tracer: whole passes into an XYZ film
 // (CpuRenderer::pass, cpu_backend.cpp:52-94, in 16×16 tiles on host threads) and single paths by
 // (pixel, sample) for per-path comparisons with the hip backend.
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstring>
@@ -22,15 +23,178 @@ Settings toSettings(const owe_render_settings& r) {
     s.lambdaMin = r.lambda_min;
     s.lambdaMax = r.lambda_max;
     s.fresnelFloor = r.fresnel_floor;
+    s.photonRadius = r.photon_radius > 0 ? r.photon_radius : 2.0;
+    s.photonAlpha = r.photon_alpha > 0 ? r.photon_alpha : 2.0 / 3.0;
+    s.photonsPerPixel = r.photons_per_pixel > 0 ? r.photons_per_pixel : 1.0;
+    s.photonGroups = r.photon_groups > 0 ? r.photon_groups : 4;
     return s;
 }
 // The stream of sample s of a pixel in the per-sample layout.
 Pcg sampleStream(uint64_t seed, uint64_t pixel, uint64_t s, uint64_t pass) {
     return Pcg(hashPair(hashPair(seed, pixel), s), pass);
 }
+
+// SPPM streams and wavelengths (the hip backend's layout): an iteration's base rotation (a golden-ratio
+// sequence from a seed offset, as gpu_renderer_passes.cpp:52-56), group g's rotation by g/(4 G), the
+// camera path of a pixel in a group chosen by a hash of (pixel, iteration), photon i in group i mod G.
+constexpr uint64_t kSppmCamera = 0x6a09e667f3bcc908ULL, kSppmGroup = 0xbb67ae8584caa73bULL,
+                   kSppmPhoton = 0xa54ff53a5f1d36f1ULL;
+double sppmRotation(uint64_t seed, uint64_t iteration) {
+    const double offset = double(uint32_t(mix64(seed) >> 32)) / 4294967296.0;
+    const double u = offset + 0.6180339887498949 * double(iteration + 1);
+    return u - std::floor(u);
+}
+Wavelengths sppmWavelengths(const Settings& s, double rotation, uint32_t group) {
+    double u = rotation + double(group) / double(4 * s.photonGroups);
+    u -= std::floor(u);
+    return s.spectral == OWE_SPECTRAL_HERO4 ? Wavelengths::hero(u) : Wavelengths::uniform1(u, s.lambdaMin, s.lambdaMax);
+}
 }  // namespace
 
 extern "C" {
+
+// The initial gather radius per metre of unfolded path and its floor (GpuRenderer::resetSppm,
+// gpu_renderer_passes.cpp:238-257): photon_radius pixel footprints, a pixel's angle for an observer
+// or for a sensor seen from its aim disc's centre; a sensor without one takes a fixed radius.
+void oracle_sppm_radius(const void* wh, const owe_render_settings* rs, double out[2]) {
+    const World& w = *static_cast<const World*>(wh);
+    const Settings set = toSettings(*rs);
+    const Observer& ob = w.observer;
+    double pixelAngle = 0, floor = 0;
+    if (!ob.sensor()) {
+        pixelAngle = 2 * ob.tanY / ob.height;
+    } else {
+        const double pixel = 2 * ob.halfY / ob.height;
+        const double d = ob.hasAim ? length(ob.aimCenter - ob.toWorld.point(V3{0, 0, 0})) : 0;
+        if (d > 0) pixelAngle = pixel / d;
+        else floor = set.photonRadius * pixel;
+    }
+    out[0] = set.photonRadius * pixelAngle;
+    out[1] = std::max(floor, 1e-9);
+}
+
+// Iterations [start, start + n) of SPPM (sppm.slang; GpuRenderer::passSppm, gpu_renderer_passes.cpp:259-290)
+// on caller-held state: direct (W·H·3) sums the camera passes' own XYZ; state (W·H·5) is τ (XYZ), N
+// and the gather radius r per pixel. A photon reaches every visible point within r: the reference's
+// multi-level hash finds exactly these (a point lives in one cell of size ≥ 2r, and the 2×2×2 cells
+// about the photon cover its reach), so the oracle searches them directly. stats += paths (camera
+// paths and photons), segments, inconsistencies, leaks.
+void oracle_sppm(const void* wh, const owe_render_settings* rs, int n, int start, int threads, double* direct,
+                 double* state, uint64_t* stats) {
+    const World& w = *static_cast<const World*>(wh);
+    const Settings set = toSettings(*rs);
+    const Tracer tr(w, set);
+    const int W = w.observer.width, H = w.observer.height;
+    const size_t pixels = size_t(W) * size_t(H);
+    const uint64_t photons = std::max<uint64_t>(1, uint64_t(std::llround(double(pixels) * set.photonsPerPixel)));
+    const uint32_t G = uint32_t(set.photonGroups);
+    if (threads <= 0) threads = int(std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
+    double r0[2];
+    oracle_sppm_radius(wh, rs, r0);
+    std::vector<Stats> st(static_cast<size_t>(threads));
+    std::vector<VisiblePoint> vps(pixels);
+    auto parallel = [&](uint64_t count, auto&& body) {
+        std::vector<std::thread> pool;
+        for (int k = 0; k < threads; ++k)
+            pool.emplace_back([&, k] {
+                for (uint64_t i = count * uint64_t(k) / uint64_t(threads); i < count * uint64_t(k + 1) / uint64_t(threads); ++i)
+                    body(k, i);
+            });
+        for (auto& th : pool) th.join();
+    };
+    for (int it = start; it < start + n; ++it) {
+        const double rot = sppmRotation(set.seed, uint64_t(it));
+        // Camera pass: one path per pixel.
+        parallel(pixels, [&](int k, uint64_t pixel) {
+            const uint32_t g = uint32_t(hashPair(hashPair(set.seed ^ kSppmGroup, pixel), uint64_t(it)) % G);
+            Pcg rng(hashPair(set.seed ^ kSppmCamera, pixel), uint64_t(it));
+            double c[3];
+            VisiblePoint& vp = vps[pixel];
+            tr.sppmCamera(int(pixel % uint64_t(W)), int(pixel / uint64_t(W)), sppmWavelengths(set, rot, g), rng,
+                          st[size_t(k)], c, vp);
+            vp.group = g;
+            for (int j = 0; j < 3; ++j) direct[3 * pixel + j] += c[j];
+            if (vp.valid && state[5 * pixel + 4] == 0)
+                state[5 * pixel + 4] = std::max(r0[0] * vp.pathLength, r0[1]);
+        });
+        // The points within reach of a photon: a sweep over x.
+        std::vector<uint32_t> order;
+        double rmax = 0;
+        for (size_t i = 0; i < pixels; ++i)
+            if (vps[i].valid) {
+                order.push_back(uint32_t(i));
+                rmax = std::max(rmax, state[5 * i + 4]);
+            }
+        std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return vps[a].p.x < vps[b].p.x; });
+        std::vector<double> xs(order.size());
+        for (size_t i = 0; i < order.size(); ++i) xs[i] = vps[order[i]].p.x;
+        struct Gather final : Tracer::PhotonSink {
+            const Tracer* tr;
+            const std::vector<VisiblePoint>* vps;
+            const std::vector<uint32_t>* order;
+            const std::vector<double>* xs;
+            const double* state;
+            double rmax;
+            uint32_t group, groups;
+            std::vector<double> flux;  // per pixel: XYZ, M
+            void gather(const Hit& hit, V3 d, const Spec4& beta, const Wavelengths& wl) override {
+                const auto lo = std::lower_bound(xs->begin(), xs->end(), hit.p.x - rmax);
+                const auto hi = std::upper_bound(xs->begin(), xs->end(), hit.p.x + rmax);
+                for (auto q = lo; q != hi; ++q) {
+                    const uint32_t i = (*order)[size_t(q - xs->begin())];
+                    const VisiblePoint& v = (*vps)[i];
+                    const double r = state[5 * size_t(i) + 4];
+                    const V3 dp = hit.p - v.p;
+                    if (dot(dp, dp) > r * r) continue;
+                    // A disc in the point's tangent plane, on a surface facing the same way.
+                    if (std::fabs(dot(dp, v.n)) > 0.25 * r || dot(v.n, hit.n) < 0.5) continue;
+                    if (!v.free && v.group != group) continue;  // wavelengths must agree
+                    double c[3];
+                    if (!tr->sppmValue(v, int(groups), d, beta, wl, c)) continue;
+                    for (int j = 0; j < 3; ++j) flux[4 * size_t(i) + size_t(j)] += c[j];
+                    flux[4 * size_t(i) + 3] += 1;
+                }
+            }
+        };
+        std::vector<Gather> sinks(static_cast<size_t>(threads));
+        for (auto& g : sinks) {
+            g.tr = &tr;
+            g.vps = &vps;
+            g.order = &order;
+            g.xs = &xs;
+            g.state = state;
+            g.rmax = rmax;
+            g.groups = G;
+            g.flux.assign(4 * pixels, 0.0);
+        }
+        parallel(photons, [&](int k, uint64_t i) {
+            Gather& g = sinks[size_t(k)];
+            g.group = uint32_t(i % G);
+            Pcg rng(hashPair(set.seed ^ kSppmPhoton, i), uint64_t(it));
+            tr.sppmPhoton(sppmWavelengths(set, rot, g.group), rng, st[size_t(k)], g);
+        });
+        // The progressive update (sppm_update_kernel): N' = N + αM, r' = r·√(N'/(N+M)), τ' = (τ+Φ)·N'/(N+M).
+        for (size_t i = 0; i < pixels; ++i) {
+            double phi[3] = {0, 0, 0}, M = 0;
+            for (const Gather& g : sinks) {
+                for (int j = 0; j < 3; ++j) phi[j] += g.flux[4 * i + size_t(j)];
+                M += g.flux[4 * i + 3];
+            }
+            if (M == 0) continue;
+            double* s = state + 5 * i;
+            const double N = s[3], Nn = N + set.photonAlpha * M, ratio = Nn / (N + M);
+            for (int j = 0; j < 3; ++j) s[j] = (s[j] + phi[j]) * ratio;
+            s[3] = Nn;
+            s[4] *= std::sqrt(ratio);
+        }
+    }
+    Stats total;
+    for (const Stats& s : st) total.add(s);
+    stats[0] += total.paths;
+    stats[1] += total.segments;
+    stats[2] += total.inconsistencies;
+    stats[3] += total.leaks;
+}
 
 void* oracle_world_create(const owe_scene* s) { return new World(*s); }
 void oracle_world_destroy(void* w) { delete static_cast<World*>(w); }

@@ -256,6 +256,21 @@ struct Settings {
     int spectral = OWE_SPECTRAL_HERO4;
     double lambdaMin = 380, lambdaMax = 780;
     double fresnelFloor = 0;
+    // SPPM (RenderSettings, scene/scene.hpp:27-35).
+    double photonRadius = 2, photonAlpha = 2.0 / 3.0, photonsPerPixel = 1;
+    int photonGroups = 4;
+};
+
+// An SPPM visible point (sppm.slang:33-40): the first non-specular surface a camera path reaches
+// through specular events only.
+struct VisiblePoint {
+    bool valid = false;
+    bool terminated = false;  // the path's secondary wavelengths ended on the way
+    bool free = false;        // seen directly: no scattering and no absorbing medium on the way
+    uint32_t group = 0, boundary = 0;
+    V3 p, n, d;               // position, the boundary's geometric normal, arriving direction
+    Spec4 beta;               // throughput arriving, with the camera weight
+    double pathLength = 0;    // unfolded length from the eye
 };
 
 struct Stats {
@@ -272,7 +287,7 @@ class Tracer {
 public:
     Tracer(const World& w, const Settings& s) : w_(w), s_(s) {}
     // Tracer::radiance (transport.cpp:473-529).
-    Spec4 radiance(Ray ray, uint32_t region, Wavelengths& wl, Pcg& rng, Stats& st) const;
+    Spec4 radiance(Ray ray, uint32_t region, Wavelengths& wl, Pcg& rng, Stats& st, VisiblePoint* vp = nullptr) const;
     Spec4 transmittance(V3 p, const Ray& start, uint32_t region, V3 dir, double dist, const Wavelengths& wl,
                         uint32_t* finalRegion, uint32_t excludeBoundary) const;
     // One camera sample of pixel (x, y): the draws of cpu_backend.cpp:69-84 from `rng`.
@@ -285,7 +300,25 @@ public:
     };
     // One particle of a light-tracing pass (particle_kernel.slang:11-19, particle.slang:356-427):
     // its wavelengths from the first uniform of `rng`, then emission, walk and splats.
-    void particle(Pcg& rng, Stats& st, Splat& out) const;
+    void particle(Pcg& rng, Stats& st, Splat& out) const { particleWalk(nullptr, rng, st, out, nullptr); }
+
+    // SPPM (sppm.slang): a photon arriving at a non-specular surface after at least one scattering event.
+    struct PhotonSink {
+        virtual void gather(const Hit& hit, V3 d, const Spec4& beta, const Wavelengths& wl) = 0;
+        virtual ~PhotonSink() = default;
+    };
+    // The camera pass's path of pixel (x, y) at the iteration's wavelengths `wl` (sppm_camera_kernel):
+    // its own radiance (emission met through specular events, direct light at the visible point) in
+    // xyz, and the visible point, if any, in vp. False when the sample measures nothing.
+    bool sppmCamera(int x, int y, Wavelengths wl, Pcg& rng, Stats& st, double xyz[3], VisiblePoint& vp) const;
+    // One photon at the given wavelengths (traceSppmPhoton, sppm.slang:150-201).
+    void sppmPhoton(Wavelengths wl, Pcg& rng, Stats& st, PhotonSink& sink) const;
+    // A visible point's XYZ for a photon arriving along d with throughput beta at wavelengths wl, once
+    // the point has passed the reach, disc, orientation and group tests (gatherPhoton's per-point
+    // value, sppm.slang:117-136): β_vp · f_vp(wo, wi) · β_photon, a bound point's share scaled by the
+    // groups. False when it is zero.
+    bool sppmValue(const VisiblePoint& vp, int groups, V3 d, const Spec4& beta, const Wavelengths& wl,
+                   double xyz[3]) const;
 
 private:
     struct Link;
@@ -296,6 +329,7 @@ private:
     void connectVertex(const Hit& hit, bool surface, const Optics* o, V3 d, double g, uint32_t region,
                        const Spec4& beta, const Wavelengths& wl, Pcg& rng, Splat& out) const;
     struct Path;
+    void particleWalk(const Wavelengths* fixed, Pcg& rng, Stats& st, Splat& out, PhotonSink* sink) const;
     Spec4 directLighting(const Hit& hit, bool surface, const Optics* o, double nFront, double nBack, V3 d,
                          const Wavelengths& wl, Pcg& rng, double g, uint32_t region) const;
     bool scatterAtSurface(const Hit& hit, const Optics& o, uint32_t other, int depth, Wavelengths& wl, Pcg& rng,

@@ -198,6 +198,17 @@ struct Tracer::Path {
     bool specular = true;
     double prevPdf = 0;
     V3 prevP;
+    // SPPM camera paths (transport.slang:749-762): where the visible point goes, and what decides it.
+    VisiblePoint* vp = nullptr;
+    int scatterEvents = 0, nonDeltaEvents = 0;
+    bool chromatic = false;  // an absorbing, non-scattering medium on the way
+    double pathLength = 0;
+    bool vpDone() const { return vp && vp->valid; }
+    // noteScatter (transport.slang:783-791).
+    void noteScatter(bool delta) {
+        if (!delta) ++nonDeltaEvents;
+        ++scatterEvents;
+    }
     // Russian roulette against the throughput (transport.cpp:369-374).
     bool survives(Pcg& rng) {
         const double q = std::min(1.0, beta.maxValue());
@@ -340,6 +351,19 @@ bool Tracer::scatterAtSurface(const Hit& hit, const Optics& o, uint32_t other, i
                 maskTerminated(wl, path.beta);
                 break;
             }
+    if (path.vp && !o.isDelta() && path.nonDeltaEvents == 0) {
+        // recordVisiblePoint (transport.slang:846-858).
+        VisiblePoint& v = *path.vp;
+        v.valid = true;
+        v.p = hit.p;
+        v.n = hit.n;
+        v.boundary = hit.boundary;
+        v.d = path.ray.d;
+        v.beta = path.beta;
+        v.terminated = wl.secondaryTerminated();
+        v.free = path.scatterEvents == 0 && !path.chromatic;
+        v.pathLength = path.pathLength;
+    }
     if (!o.isDelta()) {
         const Spec4 dl = directLighting(hit, true, &o, it.nFront, it.nBack, path.ray.d, wl, rng, 0, path.region);
         if (!dl.isZero()) path.L += path.beta * dl;
@@ -356,14 +380,16 @@ bool Tracer::scatterAtSurface(const Hit& hit, const Optics& o, uint32_t other, i
     path.specular = s.delta;
     path.prevPdf = s.pdf;
     path.prevP = hit.p;
+    path.noteScatter(s.delta);
     path.ray = w_.spawn(hit, s.wi);
     if (depth >= s_.rrDepth && !path.survives(rng)) return false;
     return true;
 }
 
-Spec4 Tracer::radiance(Ray ray, uint32_t region, Wavelengths& wl, Pcg& rng, Stats& st) const {
+Spec4 Tracer::radiance(Ray ray, uint32_t region, Wavelengths& wl, Pcg& rng, Stats& st, VisiblePoint* vp) const {
     Path path{ray, region, Spec4(), Spec4(1.0), true, 0.0, V3()};
     path.prevP = ray.o;
+    path.vp = vp;
     maskTerminated(wl, path.beta);
     st.paths++;
     for (int depth = 0;; ++depth) {
@@ -375,8 +401,10 @@ Spec4 Tracer::radiance(Ray ray, uint32_t region, Wavelengths& wl, Pcg& rng, Stat
         const MediumEvent ev = sampleMedium(med, wl, found ? hit.t : kInf, rng, path.beta, t);
         if (ev == MediumEvent::End) break;
         if (ev == MediumEvent::Scatter) {
-            // scatterInMedium (transport.cpp:379-397).
+            // scatterInMedium (transport.cpp:379-397). Light an SPPM visible point's scatter segment
+            // meets in a medium is the photons'.
             const V3 p = path.ray.o + path.ray.d * t;
+            if (path.vpDone()) break;
             if (depth >= s_.maxDepth) break;
             Hit mh;
             mh.p = p;
@@ -388,10 +416,12 @@ Spec4 Tracer::radiance(Ray ray, uint32_t region, Wavelengths& wl, Pcg& rng, Stat
             path.prevPdf = sampleHg(-path.ray.d, med.g, u1, u2, wi);
             path.specular = false;
             path.prevP = p;
+            path.noteScatter(false);
             path.ray = Ray{p, wi, kNone, kNone};
             if (!(depth < s_.rrDepth || path.survives(rng))) break;
             continue;
         }
+        if (path.vp && med.absorbs() && !med.scatters()) path.chromatic = true;
         if (!found) {
             if (path.region == 0) {
                 // skyMet (transport.cpp:401-417): the uniform sky, and the sun weighted against NEE.
@@ -418,6 +448,7 @@ Spec4 Tracer::radiance(Ray ray, uint32_t region, Wavelengths& wl, Pcg& rng, Stat
             st.inconsistencies++;
             path.region = inc;
         }
+        if (path.vp && !path.vpDone()) path.pathLength += hit.t;
         const Optics& o = w_.optics[size_t(b.optics)];
         // emissionMet (transport.cpp:420-433).
         if (b.emission >= 0) {
@@ -440,6 +471,7 @@ Spec4 Tracer::radiance(Ray ray, uint32_t region, Wavelengths& wl, Pcg& rng, Stat
             if (depth >= 4 * s_.maxDepth) break;
             continue;
         }
+        if (path.vpDone()) break;  // the visible point's scatter segment has met matter
         if (o.type == OWE_SURF_ABSORBER || o.type == OWE_SURF_DETECTOR) break;
         if (depth >= s_.maxDepth) break;
         if (!scatterAtSurface(hit, o, other, depth, wl, rng, path)) break;
@@ -650,12 +682,21 @@ bool Tracer::emitParticle(Wavelengths& wl, Pcg& rng, bool connect, Emitted& e, S
     return !e.beta.isZero();
 }
 
-void Tracer::particle(Pcg& rng, Stats& st, Splat& out) const {
-    const double u = rng.uniform();
-    Wavelengths wl = s_.spectral == OWE_SPECTRAL_HERO4 ? Wavelengths::hero(u)
-                                                        : Wavelengths::uniform1(u, s_.lambdaMin, s_.lambdaMax);
+// A light-tracing particle (its wavelengths from the first uniform of rng), or with `sink` an SPPM
+// photon at the fixed wavelengths: no observer connections or sensor splats; a photon gathers at
+// every non-specular surface after its first scattering event (sppm.slang:150-201).
+void Tracer::particleWalk(const Wavelengths* fixed, Pcg& rng, Stats& st, Splat& out, PhotonSink* sink) const {
+    Wavelengths wl;
+    if (fixed) {
+        wl = *fixed;
+    } else {
+        const double u = rng.uniform();
+        wl = s_.spectral == OWE_SPECTRAL_HERO4 ? Wavelengths::hero(u) : Wavelengths::uniform1(u, s_.lambdaMin, s_.lambdaMax);
+    }
     const Observer& ob = w_.observer;
-    const bool observer = !ob.sensor();
+    const bool observer = !ob.sensor() && !sink;
+    const bool sensor = ob.sensor() && !sink;
+    int events = 0;
     st.paths++;
     Emitted em;
     if (!emitParticle(wl, rng, observer, em, out)) return;
@@ -680,6 +721,7 @@ void Tracer::particle(Pcg& rng, Stats& st, Splat& out) const {
             V3 wi;
             sampleHg(-ray.d, med.g, u1, u2, wi);
             ray = Ray{mh.p, wi, kNone, kNone};
+            ++events;
             if (depth >= s_.rrDepth) {
                 if (rng.uniform() >= 0.95) break;
                 beta *= 1 / 0.95;
@@ -705,7 +747,7 @@ void Tracer::particle(Pcg& rng, Stats& st, Splat& out) const {
             continue;
         }
         // The sensor records the flux reaching its front (sensorPixel, particle.slang:98-114).
-        if (ob.sensor() && fromFront && hit.boundary == ob.boundary) {
+        if (sensor && fromFront && hit.boundary == ob.boundary) {
             double fx = (hit.pLocal.x / ob.halfX + 1) * 0.5 * ob.width;
             double fy = (1 - hit.pLocal.y / ob.halfY) * 0.5 * ob.height;
             if (ob.flipX) fx = ob.width - fx;
@@ -720,6 +762,7 @@ void Tracer::particle(Pcg& rng, Stats& st, Splat& out) const {
         }
         if (o.type == OWE_SURF_DETECTOR || o.type == OWE_SURF_ABSORBER) break;
         if (!o.isDelta() && observer) connectVertex(hit, true, &o, ray.d, 0, region, beta, wl, rng, out);
+        if (!o.isDelta() && sink && events >= 1) sink->gather(hit, ray.d, beta, wl);
         if (depth >= s_.maxDepth) break;
         // terminateIfDispersive, then the importance-mode scatter (transport.slang:639-684).
         Interface it{&o, hit.n, hit.p, w_.indexOf(b.front, wl.lambda[0]), w_.indexOf(b.back, wl.lambda[0]),
@@ -747,12 +790,67 @@ void Tracer::particle(Pcg& rng, Stats& st, Splat& out) const {
             if (wl.pdf[i] != 0) beta[i] *= secondaryWeight(at(i), ray.d, s, !kRadiance);
         if (s.transmitted) region = other;
         ray = w_.spawn(hit, s.wi);
+        ++events;
         // Particles carry flux: roulette against the event's own throughput.
         if (depth >= s_.rrDepth && s.weight < 1) {
             if (rng.uniform() >= s.weight) break;
             beta *= 1 / s.weight;
         }
     }
+}
+
+// ---------------------------------------------------------------- SPPM (sppm.slang)
+
+bool Tracer::sppmCamera(int x, int y, Wavelengths wl, Pcg& rng, Stats& st, double xyz[3], VisiblePoint& vp) const {
+    xyz[0] = xyz[1] = xyz[2] = 0;
+    vp = VisiblePoint{};
+    const double px = x + rng.uniform(), py = y + rng.uniform();
+    double fx = 0, fy = 0;
+    const Observer& ob = w_.observer;
+    if (ob.sigma > 0) {
+        double u1, u2;
+        rng.draw2(u1, u2);
+        standardNormal2(u1, u2, fx, fy);
+        fx *= ob.sigma;
+        fy *= ob.sigma;
+    }
+    Ray ray;
+    double weight;
+    if (!ob.generate(px + fx, py + fy, rng, ray, weight) || weight <= 0) return false;
+    const Spec4 L = radiance(ray, ob.region, wl, rng, st, &vp);
+    if (vp.valid) vp.beta *= weight;
+    if (L.isZero()) return true;
+    wl.toXYZ(L * weight, xyz);
+    if (!std::isfinite(xyz[0] + xyz[1] + xyz[2])) xyz[0] = xyz[1] = xyz[2] = 0;
+    return true;
+}
+
+void Tracer::sppmPhoton(Wavelengths wl, Pcg& rng, Stats& st, PhotonSink& sink) const {
+    struct None final : Splat {
+        void add(int, int, const double*) override {}
+    } none;
+    particleWalk(&wl, rng, st, none, &sink);
+}
+
+bool Tracer::sppmValue(const VisiblePoint& v, int groups, V3 d, const Spec4& beta, const Wavelengths& wl,
+                       double xyz[3]) const {
+    Wavelengths w = wl;  // a free point is evaluated at the photon's wavelengths
+    if (!v.free && v.terminated) w.terminateSecondary();
+    const Boundary& b = w_.boundaries[v.boundary];
+    const Optics& o = w_.optics[size_t(b.optics)];
+    Spec4 value;
+    for (int i = 0; i < NW; ++i) {
+        if (w.pdf[i] == 0) continue;
+        const Interface it{&o, v.n, v.p, w_.indexOf(b.front, w.lambda[i]), w_.indexOf(b.back, w.lambda[i]), w.lambda[i],
+                           s_.fresnelFloor};
+        double pdf;
+        const double f = evalScatter(it, v.d, -d, kRadiance, pdf);
+        value[i] = (v.free ? v.beta[0] : v.beta[i] * groups) * f * beta[i];
+    }
+    maskTerminated(w, value);
+    if (value.isZero()) return false;
+    w.toXYZ(value, xyz);
+    return std::isfinite(xyz[0] + xyz[1] + xyz[2]);
 }
 
 }  // namespace oracle

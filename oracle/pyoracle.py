@@ -163,6 +163,8 @@ class Oracle(_Lib):
             "render": (None, [_vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp]),
             "camera_paths": (None, [_vp, _vp, _i, _i, _u64, _vp, _vp, _vp, _vp, _i]),
             "render_particles": (None, [_vp, _vp, _i, _i, _i, _vp, _vp]),
+            "sppm": (None, [_vp, _vp, _i, _i, _i, _vp, _vp, _vp]),
+            "sppm_radius": (None, [_vp, _vp, _vp]),
             "particles": (None, [_vp, _vp, _i, _u64, _vp, _vp, _vp, _i]),
             "emission_info": (None, [_vp, _vp]),
             "world_intersect": (None, [_vp, _u64, _vp, _vp, _vp, _vp]),
@@ -267,6 +269,39 @@ class World:
         for p in range(first_pass, first_pass + passes):
             self.o.render_particles(self.h, ctypes.byref(settings), ppp, p, threads, _ptr(xyz), _ptr(stats))
         return xyz / float(self.width * self.height * ppp * passes), stats
+
+    def sppm(self, settings, iterations: int, state=None, threads: int = 0):
+        """SPPM iterations continuing `state` (a dict from an earlier call, or None to start): returns the
+        new state with "direct" (W·H·3 camera-pass XYZ sums), "tau" (W·H·3), "N", "r", "iterations",
+        "photons" (per iteration) and "stats"; sppm_resolve turns it into the image."""
+        W, H = self.width, self.height
+        if state is None:
+            state = dict(direct=np.zeros((H, W, 3)), pix=np.zeros((H, W, 5)), iterations=0,
+                         stats=np.zeros(4, dtype=np.uint64))
+        direct = np.ascontiguousarray(state["direct"]).copy()
+        pix = np.ascontiguousarray(state["pix"]).copy()
+        stats = state["stats"].copy()
+        self.o.sppm(self.h, ctypes.byref(settings), iterations, state["iterations"], threads, _ptr(direct), _ptr(pix),
+                    _ptr(stats))
+        ppp = settings.photons_per_pixel if settings.photons_per_pixel > 0 else 1.0
+        return dict(direct=direct, pix=pix, iterations=state["iterations"] + iterations, stats=stats,
+                    photons=max(1, int(round(W * H * ppp))), tau=pix[..., :3], N=pix[..., 3], r=pix[..., 4])
+
+    def sppm_radius(self, settings):
+        """(initial radius per metre of path, radius floor)."""
+        out = np.zeros(2)
+        self.o.sppm_radius(self.h, ctypes.byref(settings), _ptr(out))
+        return float(out[0]), float(out[1])
+
+    @staticmethod
+    def sppm_resolve(state) -> np.ndarray:
+        """L = L_direct / iterations + τ / (N_emitted π r²) per pixel (sppm.slang:19-20)."""
+        it = state["iterations"]
+        r = state["pix"][..., 4]
+        emitted = float(state["photons"]) * it
+        with np.errstate(divide="ignore", invalid="ignore"):
+            photon = np.where(r[..., None] > 0, state["pix"][..., :3] / (emitted * np.pi * r[..., None] ** 2), 0.0)
+        return state["direct"] / max(it, 1) + photon
 
     def particles(self, settings, index, pass_index: int = 0, threads: int = 0):
         """Per-particle XYZ (summed over its splats, not normalised) and segments."""

@@ -1,0 +1,113 @@
+"""The hip backend's SPPM integrator against the oracle and the reference's own SPPM cases.
+
+* per pixel, one iteration from the same (seed, pixel, iteration) camera streams and (seed, photon,
+  iteration) photon streams: the initial gather radius (from the unfolded path length), the photon
+  count M and the flux τ agree with the fp64 oracle on almost every pixel (the rest are float
+  decisions that fall the other way), and the camera pass's direct part agrees;
+* after many iterations the mean image agrees with the oracle's;
+* the white furnace, L = Le / (1 − ρ) within 2 % (tests/test_backends.cpp:540-574);
+* the caustic seen through a glass plate and in a mirror, ratios within 0.015 of the plate's
+  (1 − R)²/(1 − R²) and the mirror's 0.9 at 48×48 pixels and 2048 iterations
+  (tests/test_references.cpp:212-250);
+* deposits are exact integer sums: two renders are bit-identical.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import sppm_cases as C
+from owe_hip import render
+from owe_hip import scene as S
+from pyoracle import World
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu(sc, st, iterations):
+    r = render.Renderer(sc, st)
+    r.run_pass(iterations)
+    return r
+
+
+@pytest.mark.parametrize("case", ["direct", "plate"])
+def test_sppm_first_iteration_matches_the_oracle(cuda, oracle, case):
+    ls = C.caustic_cases(resolution=32)[case]
+    sc = ls.use()
+    st = ls.settings(seed=7)
+    print(f"sppm {case}: 32x32, 1 iteration", flush=True)
+    r = _gpu(sc, st, 1)
+    g = r.sppm_state()
+    o = World(oracle, sc).sppm(st, 1)
+    alpha = 2.0 / 3.0
+    seen = o["r"] > 0
+    assert seen.mean() > 0.5
+    assert np.array_equal(g["r"] > 0, seen) or (np.logical_xor(g["r"] > 0, seen).mean() < 0.01)
+    both = seen & (g["r"] > 0)
+    rclose = np.abs(g["r"][both] - o["r"][both]) <= 1e-4 * o["r"][both]
+    gm, om = np.rint(g["N"] / alpha), np.rint(o["N"] / alpha)
+    lit = om > 0
+    tclose = np.abs(g["tau"][..., 1] - o["tau"][..., 1]) <= 1e-3 * np.maximum(o["tau"][..., 1], 1e-30) + 1e-12
+    print(f"{case}: radius agrees on {rclose.mean():.4f}, M on {(gm == om).mean():.4f} of pixels "
+          f"({lit.mean():.3f} lit), tau on {tclose.mean():.4f}; mean M gpu {gm.mean():.3f} oracle {om.mean():.3f}",
+          flush=True)
+    assert lit.mean() > 0.05
+    assert rclose.mean() > 0.99
+    assert (gm == om).mean() > 0.97
+    assert tclose.mean() > 0.95
+    s = r.stats()
+    assert s["inconsistencies"] == 0 and s["leaks"] == 0
+    assert s["paths"] == o["stats"][0]
+    # The camera pass's own part: the resolved image less the photons' estimate.
+    img = r.resolve()
+    with np.errstate(divide="ignore", invalid="ignore"):
+        photon = np.where(g["r"][..., None] > 0, g["tau"] / (g["photons"] * np.pi * g["r"][..., None] ** 2), 0.0)
+    gd, od = (img - photon)[..., 1], o["direct"][..., 1]
+    dclose = np.abs(gd - od) <= 1e-3 * np.maximum(np.abs(od), np.abs(gd)) + 1e-9 * max(1.0, float(np.abs(od).max()))
+    print(f"{case}: direct part agrees on {dclose.mean():.4f} of pixels", flush=True)
+    assert dclose.mean() > 0.97
+
+
+def test_sppm_converges_with_the_oracle(cuda, oracle):
+    ls = C.caustic_cases(resolution=32)["plate"]
+    sc = ls.use()
+    st = ls.settings(seed=11)
+    print("sppm plate: 32x32, 256 iterations", flush=True)
+    g = _gpu(sc, st, 256).resolve()[..., 1]
+    o = World.sppm_resolve(World(oracle, sc).sppm(st, 256))[..., 1]
+    print(f"mean Y gpu {g.mean():.6g} oracle {o.mean():.6g}", flush=True)
+    assert g.mean() == pytest.approx(o.mean(), rel=0.01)
+
+
+@pytest.mark.parametrize("rho", [0.0, 0.5])
+def test_sppm_white_furnace(cuda, rho):
+    r = _gpu(S.white_furnace_scene(rho, size=8), S.settings(seed=1, integrator=S.INTEGRATOR_SPPM), 4096)
+    y = r.resolve()[..., 1].mean()
+    print(f"furnace rho={rho}: {y:.5f} (expected {1 / (1 - rho):.5f})", flush=True)
+    assert y == pytest.approx(1 / (1 - rho), abs=0.02 / (1 - rho))
+
+
+def test_sppm_sees_a_caustic_through_a_plate_and_in_a_mirror(cuda):
+    means = {}
+    for name, ls in C.caustic_cases().items():
+        print(f"sppm caustic {name}: 48x48, 2048 iterations", flush=True)
+        r = _gpu(ls.use(), ls.settings(), 2048)
+        means[name] = r.resolve()[..., 1].mean()
+        s = r.stats()
+        assert s["inconsistencies"] == 0 and s["leaks"] == 0
+    plate, mirror = means["plate"] / means["direct"], means["mirror"] / means["virtual"]
+    print(f"through the plate {plate:.4f} (Fresnel {C.plate_transmission():.4f}); in the mirror {mirror:.4f}", flush=True)
+    assert plate == pytest.approx(C.plate_transmission(), abs=0.015)
+    assert mirror == pytest.approx(0.9, abs=0.015)
+
+
+def test_sppm_is_reproducible(cuda):
+    ls = C.caustic_cases(resolution=32)["mirror"]
+    imgs = [_gpu(ls.use(), ls.settings(seed=3), 16).resolve() for _ in range(2)]
+    assert np.array_equal(imgs[0], imgs[1])
+    r = _gpu(ls.use(), ls.settings(seed=3), 8)
+    r.run_pass(8)
+    assert np.array_equal(r.resolve(), imgs[0])  # two passes of 8 iterations are one of 16
+    r.reset()
+    r.run_pass(16)
+    assert np.array_equal(r.resolve(), imgs[0])

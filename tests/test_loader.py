@@ -231,11 +231,25 @@ def test_what_the_backend_cannot_render_is_refused(text, msg):
 
 def test_integrators_of_other_estimators_are_named():
     ls = LD.load_scene_text("observer E { position = (0, -1, 0)  look_at = (0, 0, 0) }\n"
-                            "render { integrator = sppm  seed = 9  max_depth = 12 }")
-    with pytest.raises(F.SceneError, match="integrator 'sppm'"):
+                            "render { integrator = vcm  seed = 9  max_depth = 12 }")
+    with pytest.raises(F.SceneError, match="integrator 'vcm'"):
         ls.settings()
     st = ls.settings(integrator="path")
     assert st.seed == 9 and st.max_depth == 12 and st.integrator == S.INTEGRATOR_PATH
+
+
+def test_sppm_settings_come_from_the_render_block():
+    ls = LD.load_scene_text("observer E { position = (0, -1, 0)  look_at = (0, 0, 0) }\n"
+                            "render { integrator = sppm  photon_radius = 3  photon_alpha = 0.5  photons = 2 }")
+    st = ls.settings()
+    assert st.integrator == S.INTEGRATOR_SPPM
+    assert (st.photon_radius, st.photon_alpha, st.photons_per_pixel, st.photon_groups) == (3.0, 0.5, 2.0, 4)
+    assert LD.load_scene_text("observer E { position = (0, -1, 0)  look_at = (0, 0, 0) }\n"
+                              "render { integrator = sppm }").settings().photon_alpha == pytest.approx(2 / 3)
+    bad = LD.load_scene_text("observer E { position = (0, -1, 0)  look_at = (0, 0, 0) }\n"
+                             "render { integrator = sppm  photon_alpha = 1.5 }")
+    with pytest.raises(F.SceneError, match="photon_alpha in"):
+        bad.settings()
 
 
 BENCHMARKS = ["caustic_glass_sphere", "caustic_seen_in_mirror", "caustic_seen_through_glass", "caustic_water_surface",
