@@ -728,6 +728,66 @@ def bench_scenes(args):
                      for r in res["rows"]]}
 
 
+SPPM_SIZE, SPPM_ITERATIONS = 256, 16
+
+
+def cpu_baseline_sppm(ls, sc, st, seconds_budget: float = 10.0):
+    sys.path.insert(0, str(ROOT / "oracle"))
+    from pyoracle import Oracle, World
+
+    threads = min(16, os.cpu_count() or 1)
+    w = World(Oracle(), sc)
+    state, elapsed = None, 0.0
+    while elapsed < seconds_budget and (state is None or state["iterations"] < 256):
+        t0 = time.perf_counter()
+        state = w.sppm(st, 2, state=state, threads=threads)
+        elapsed += time.perf_counter() - t0
+    n = int(state["stats"][0])
+    return {"value": n / elapsed, "unit": "camera paths + photons/s", "cores": threads, "kind": "port",
+            "sample": f"{state['iterations']} SPPM iterations of the same scene ({n} camera paths and photons, "
+                      f"{elapsed:.1f} s) through the oracle's fp64 restatement, {threads} threads"}
+
+
+def bench_sppm(args):
+    """SPPM (stochastic progressive photon mapping) of a lens caustic seen through a glass plate: per
+    iteration one camera path per pixel to its visible point, the hash fill, one photon per pixel from
+    the sun, the progressive update; a step is a pass of 16 iterations at 256x256."""
+    import torch
+
+    sys.path.insert(0, str(ROOT / "tests"))
+    import sppm_cases as C
+
+    from owe_hip import render
+
+    ls = C.caustic_cases(resolution=SPPM_SIZE)["plate"]
+    sc = ls.use()
+    st = ls.settings(seed=1)
+    r = render.Renderer(sc, st)
+    for _ in range(args.warmup):
+        r.run_pass(SPPM_ITERATIONS)
+    r.reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r.run_pass(SPPM_ITERATIONS)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    stats = r.stats()
+    n = stats["paths"]
+    its = args.steps * SPPM_ITERATIONS
+    log(f"sppm: {its} iterations in {wall:.3f}s; {n / wall / 1e6:.1f} M camera paths + photons/s; "
+        f"{wall * 1e3 / its:.3f} ms/iteration")
+    out = {"value": n / wall, "unit": "camera paths + photons/s", "ms_per_step": wall * 1e3 / args.steps,
+           "config": {"workload": "SPPM, lens caustic seen through a glass plate", "resolution": [SPPM_SIZE, SPPM_SIZE],
+                      "iterations_per_step": SPPM_ITERATIONS, "photons_per_iteration": SPPM_SIZE * SPPM_SIZE,
+                      "ms_per_iteration": wall * 1e3 / its, "segments": stats["segments"],
+                      "mean_Y": float(r.resolve()[..., 1].mean())},
+           "roofline": None}
+    out["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline_sppm(ls, sc, st)
+    r.close()
+    return out
+
+
 def main() -> int:
     import torch  # noqa: F401  (first: its HIP runtime is the process's; see owe_hip/_lib.py)
 
@@ -736,13 +796,14 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="prism", choices=["prism", "rayquery", "light", "lens", "caustic",
-                                                             "telescope", "scenes"])
+                                                             "telescope", "scenes", "sppm"])
     ap.add_argument("--no-rayquery", action="store_true", help="prism: skip the ray-query record beside it")
     ap.add_argument("--no-light", action="store_true", help="prism: skip the light-tracing record beside it")
     ap.add_argument("--no-lens", action="store_true", help="prism: skip the lens-analysis record beside it")
     ap.add_argument("--no-caustic", action="store_true", help="prism: skip the caustic-gradient record beside it")
     ap.add_argument("--no-telescope", action="store_true", help="prism: skip the telescope-gradient record beside it")
     ap.add_argument("--no-scenes", action="store_true", help="prism: skip the scene-file suite record")
+    ap.add_argument("--no-sppm", action="store_true", help="prism: skip the SPPM record")
     ap.add_argument("--no-telescope-render", action="store_true",
                     help="telescope: skip the rendering of the telescopes in the room")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -787,6 +848,11 @@ def main() -> int:
                 result["telescope"]["render"] = bench_telescope_render(ns)
         if not args.no_scenes:
             result["scenes"] = bench_scenes(args)
+        if not args.no_sppm:
+            result["sppm"] = bench_sppm(argparse.Namespace(steps=min(args.steps, 8), warmup=1,
+                                                           no_cpu_baseline=args.no_cpu_baseline))
+    elif args.workload == "sppm":
+        result = bench_sppm(args)
     elif args.workload == "scenes":
         result = bench_scenes(args)
     elif args.workload == "caustic":

@@ -307,7 +307,7 @@ __device__ __forceinline__ bool roulette(PCtx& c, float q) {
 // An SPPM photon arriving at a non-specular surface point along d (gatherPhoton, sppm.slang:88-148):
 // adds its contribution to every visible point within reach, on a disc in the point's tangent plane
 // with a matching surface orientation, whose wavelengths agree with the photon's.
-__device__ void sppmGather(PCtx& c, const SppmDev& s, const Hit& h, f3 d, uint32_t group) {
+__device__ __forceinline__ void sppmGather(PCtx& c, const SppmDev& s, const Hit& h, f3 d, uint32_t group) {
     const DScene& sc = c.sc;
     const int lo = s.levels[0], hi = s.levels[1];
     if (lo > hi) return;  // no visible points this iteration
