@@ -920,75 +920,85 @@ __global__ __launch_bounds__(kBlock) void sppmCameraKernel(DScene sc, Settings s
     __shared__ uint32_t stack[kCap * kBlock];
     uint32_t* st = stack + threadIdx.x;
     const uint32_t pixel = blockIdx.x * kBlock + threadIdx.x;
-    if (pixel >= s.pixels) return;
-    const uint32_t group = uint32_t(hashPair(hashPair(set.seed ^ kSppmGroup, pixel), s.iteration) % uint64_t(s.groups));
-    ShadeCtx c{sc, set};
-    c.wl = sppmWavelengths(set, s.rotation, group, s.groups);
-    Rng rng;
-    rng.seed(hashPair(set.seed ^ kSppmCamera, pixel), s.iteration);
-    const float4 lam = make_float4(c.wl.lambda[0], c.wl.lambda[1], c.wl.lambda[2], c.wl.lambda[3]);
-    const float4 beta0 = set.spectral == OWE_SPECTRAL_UNIFORM1 ? make_float4(1.0f, 0.0f, 0.0f, 0.0f)
-                                                               : make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-    PathStart ps;
-    cameraRay(sc, pixel % sc.obs.width, pixel / sc.obs.width, rng, lam, beta0, ps);
-    SppmRec rec;
-    uint32_t segments = 0;
-    c.L = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (ps.weight > 0.0f) {
-        c.rng = rng;
-        c.flags = kFlagSpecular;
-        c.depth = 0;
-        c.beta = ps.beta;
-        c.o = ps.o;
-        c.prevPdf = 0.0f;
-        c.d = ps.d;
-        c.weight = ps.weight;
-        c.prevP = ps.o;
-        c.pixel = pixel;
-        c.region = ps.region;
-        c.departed = ps.departed;
-        c.face = ~0u;
-        for (bool alive = true; alive; c.depth += 1) {
-            Hit h;
-            const bool found = intersectContact<false, kContacts>(sc, c.o, c.d, __builtin_inff(), c.departed, c.face,
-                                                                  kNoBoundary, c.region, st, kBlock, h);
-            ++segments;
-            const uint4 hraw = found ? make_uint4(__float_as_uint(h.t), h.boundary, h.face, h.slot)
-                                     : make_uint4(__float_as_uint(__builtin_inff()), kNoBoundary, 0u, kEmptyChild);
-            c.shPend = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            alive = shadeOne<true>(c, hraw, counters, &rec);
-            if (c.shPend.x != 0.0f || c.shPend.y != 0.0f || c.shPend.z != 0.0f || c.shPend.w != 0.0f) {
-                float Tr[NW];
-                const float4 l4 = make_float4(c.wl.lambda[0], c.wl.lambda[1], c.wl.lambda[2], c.wl.lambda[3]);
-                if (shadowTransmittance<kContacts>(sc, c.shO, c.shD, c.shDist, c.shStart, c.shLightRegion, c.shExclude,
-                                                   c.shDeparted, c.shFace, l4, simple, st, Tr))
-                    c.L = addShadow(c.L, c.shPend, Tr);
+    uint32_t paths = 0, segments = 0;
+    if (pixel < s.pixels) {
+        const uint32_t group = uint32_t(hashPair(hashPair(set.seed ^ kSppmGroup, pixel), s.iteration) % uint64_t(s.groups));
+        ShadeCtx c{sc, set};
+        c.wl = sppmWavelengths(set, s.rotation, group, s.groups);
+        Rng rng;
+        rng.seed(hashPair(set.seed ^ kSppmCamera, pixel), s.iteration);
+        const float4 lam = make_float4(c.wl.lambda[0], c.wl.lambda[1], c.wl.lambda[2], c.wl.lambda[3]);
+        const float4 beta0 = set.spectral == OWE_SPECTRAL_UNIFORM1 ? make_float4(1.0f, 0.0f, 0.0f, 0.0f)
+                                                                   : make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+        PathStart ps;
+        cameraRay(sc, pixel % sc.obs.width, pixel / sc.obs.width, rng, lam, beta0, ps);
+        SppmRec rec;
+        c.L = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (ps.weight > 0.0f) {
+            c.rng = rng;
+            c.flags = kFlagSpecular;
+            c.depth = 0;
+            c.beta = ps.beta;
+            c.o = ps.o;
+            c.prevPdf = 0.0f;
+            c.d = ps.d;
+            c.weight = ps.weight;
+            c.prevP = ps.o;
+            c.pixel = pixel;
+            c.region = ps.region;
+            c.departed = ps.departed;
+            c.face = ~0u;
+            for (bool alive = true; alive; c.depth += 1) {
+                Hit h;
+                const bool found = intersectContact<false, kContacts>(sc, c.o, c.d, __builtin_inff(), c.departed, c.face,
+                                                                      kNoBoundary, c.region, st, kBlock, h);
+                ++segments;
+                const uint4 hraw = found ? make_uint4(__float_as_uint(h.t), h.boundary, h.face, h.slot)
+                                         : make_uint4(__float_as_uint(__builtin_inff()), kNoBoundary, 0u, kEmptyChild);
+                c.shPend = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                alive = shadeOne<true>(c, hraw, counters, &rec);
+                if (c.shPend.x != 0.0f || c.shPend.y != 0.0f || c.shPend.z != 0.0f || c.shPend.w != 0.0f) {
+                    float Tr[NW];
+                    const float4 l4 = make_float4(c.wl.lambda[0], c.wl.lambda[1], c.wl.lambda[2], c.wl.lambda[3]);
+                    if (shadowTransmittance<kContacts>(sc, c.shO, c.shD, c.shDist, c.shStart, c.shLightRegion, c.shExclude,
+                                                       c.shDeparted, c.shFace, l4, simple, st, Tr))
+                        c.L = addShadow(c.L, c.shPend, Tr);
+                }
+            }
+            float X, Y, Z;
+            if (pathXYZ(c.L, c.weight, c.wl, s.invNWY, X, Y, Z)) {
+                // One thread per pixel: the film's words are this thread's alone in this kernel.
+                unsigned long long* f = film + 3 * size_t(pixel);
+                f[0] += (unsigned long long)fixedPoint(X);
+                f[1] += (unsigned long long)fixedPoint(Y);
+                f[2] += (unsigned long long)fixedPoint(Z);
+            }
+            paths = 1;
+        }
+        float r = 0.0f;
+        if (rec.valid) {
+            r = s.radius[pixel];
+            if (r == 0.0f) {
+                r = fmaxf(s.r0PerMetre * rec.pathLength, s.r0Floor);
+                s.radius[pixel] = r;
             }
         }
-        float X, Y, Z;
-        if (pathXYZ(c.L, c.weight, c.wl, s.invNWY, X, Y, Z)) {
-            // One thread per pixel: the film's words are this thread's alone in this kernel.
-            unsigned long long* f = film + 3 * size_t(pixel);
-            f[0] += (unsigned long long)fixedPoint(X);
-            f[1] += (unsigned long long)fixedPoint(Y);
-            f[2] += (unsigned long long)fixedPoint(Z);
-        }
-        atomicAdd(&s.stats[0], 1ull);
+        const float w = ps.weight;
+        s.vpP[pixel] = make_float4(rec.p.x, rec.p.y, rec.p.z, r);
+        s.vpN[pixel] = make_float4(rec.n.x, rec.n.y, rec.n.z, __uint_as_float(rec.boundary));
+        s.vpD[pixel] = make_float4(rec.d.x, rec.d.y, rec.d.z, __uint_as_float(rec.valid ? rec.flags | (group << 8) : 0u));
+        s.vpBeta[pixel] = make_float4(rec.beta.x * w, rec.beta.y * w, rec.beta.z * w, rec.beta.w * w);
+    }
+    // One pair of counter atomics per wave.
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        paths += __shfl_xor(paths, off);
+        segments += __shfl_xor(segments, off);
+    }
+    if (__lane_id() == 0 && paths) {
+        atomicAdd(&s.stats[0], (unsigned long long)paths);
         atomicAdd(&s.stats[1], (unsigned long long)segments);
     }
-    float r = 0.0f;
-    if (rec.valid) {
-        r = s.radius[pixel];
-        if (r == 0.0f) {
-            r = fmaxf(s.r0PerMetre * rec.pathLength, s.r0Floor);
-            s.radius[pixel] = r;
-        }
-    }
-    const float w = ps.weight;
-    s.vpP[pixel] = make_float4(rec.p.x, rec.p.y, rec.p.z, r);
-    s.vpN[pixel] = make_float4(rec.n.x, rec.n.y, rec.n.z, __uint_as_float(rec.boundary));
-    s.vpD[pixel] = make_float4(rec.d.x, rec.d.y, rec.d.z, __uint_as_float(rec.valid ? rec.flags | (group << 8) : 0u));
-    s.vpBeta[pixel] = make_float4(rec.beta.x * w, rec.beta.y * w, rec.beta.z * w, rec.beta.w * w);
 }
 
 __global__ void sppmClearGrid(SppmDev s) {
@@ -1001,17 +1011,28 @@ __global__ void sppmClearGrid(SppmDev s) {
 }
 
 // The visible points enter the hash (sppm_grid_kernel): each in the one cell of its level holding it.
+// Launched with whole waves (blocks of 256): every lane takes part in the level reduction.
 __global__ void sppmGridKernel(SppmDev s) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= s.pixels) return;
-    if (!(__float_as_uint(s.vpD[i].w) & kVpValid)) return;
-    const float4 p = s.vpP[i];
-    const int level = sppmLevel(p.w);
-    const float size = ldexpf(1.0f, level);
-    const uint32_t h = sppmHash(sppmCell(p.x, size), sppmCell(p.y, size), sppmCell(p.z, size), level, s.gridMask);
-    s.next[i] = atomicExch(&s.head[h], i);
-    atomicMin(&s.levels[0], level);
-    atomicMax(&s.levels[1], level);
+    int lo = 0x7FFFFFFF, hi = int(0x80000001u);
+    if (i < s.pixels && (__float_as_uint(s.vpD[i].w) & kVpValid)) {
+        const float4 p = s.vpP[i];
+        const int level = sppmLevel(p.w);
+        const float size = ldexpf(1.0f, level);
+        const uint32_t h = sppmHash(sppmCell(p.x, size), sppmCell(p.y, size), sppmCell(p.z, size), level, s.gridMask);
+        s.next[i] = atomicExch(&s.head[h], i);
+        lo = hi = level;
+    }
+    // The levels in use: one pair of atomics per wave rather than per point.
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = min(lo, __shfl_xor(lo, off));
+        hi = max(hi, __shfl_xor(hi, off));
+    }
+    if (__lane_id() == 0 && lo <= hi) {
+        atomicMin(&s.levels[0], lo);
+        atomicMax(&s.levels[1], hi);
+    }
 }
 
 // The progressive update (sppm_update_kernel): N' = N + αM, r' = r·√(N'/(N+M)), τ' = (τ + Φ)·N'/(N+M);
