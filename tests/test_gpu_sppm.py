@@ -13,6 +13,8 @@
 """
 from __future__ import annotations
 
+from pathlib import Path
+
 import numpy as np
 import pytest
 
@@ -30,13 +32,26 @@ def _gpu(sc, st, iterations):
     return r
 
 
-@pytest.mark.parametrize("case", ["direct", "plate"])
+def _case(case):
+    """The caustic cases at 32x32, or one of the package's example scenes at 32x24."""
+    if case in ("direct", "plate", "mirror"):
+        ls = C.caustic_cases(resolution=32)[case]
+        return ls, ls.use()
+    from owe_hip import loader as LD
+
+    ls = LD.load_scene(str(Path(LD.__file__).parent / "scenes" / f"{case}.owe"))
+    return ls, ls.use(resolution=(32, 24))
+
+
+# pool: visible points seen through rippled, absorbing water in a basin (chromatic, bound points, contacts);
+# lens_caustic: a caustic seen through a tilted glass plate.
+@pytest.mark.parametrize("case", ["direct", "plate", "pool", "lens_caustic"])
 def test_sppm_first_iteration_matches_the_oracle(cuda, oracle, case):
-    ls = C.caustic_cases(resolution=32)[case]
-    sc = ls.use()
+    ls, sc = _case(case)
     st = ls.settings(seed=7)
+    st.integrator = S.INTEGRATOR_SPPM
     st.photons_per_pixel = 32.0  # one iteration: enough photons to reach many points
-    print(f"sppm {case}: 32x32, 1 iteration of 32 photons per pixel", flush=True)
+    print(f"sppm {case}: {sc.observer['width']}x{sc.observer['height']}, 1 iteration of 32 photons per pixel", flush=True)
     r = _gpu(sc, st, 1)
     g = r.sppm_state()
     o = World(oracle, sc).sppm(st, 1)
@@ -100,6 +115,14 @@ def test_sppm_sees_a_caustic_through_a_plate_and_in_a_mirror(cuda):
     print(f"through the plate {plate:.4f} (Fresnel {C.plate_transmission():.4f}); in the mirror {mirror:.4f}", flush=True)
     assert plate == pytest.approx(C.plate_transmission(), abs=0.015)
     assert mirror == pytest.approx(0.9, abs=0.015)
+
+
+def test_sppm_sensor_measures_disk_irradiance(cuda):
+    """A surface sensor aimed at a Lambertian disc (tests/analytic.hpp:28-60): every camera path meets
+    the emitter, so the camera pass alone carries the irradiance, within 1 %."""
+    a, h, half = 0.3, 0.5, 0.15
+    r = _gpu(S.disk_irradiance_scene(a, h, half, aimed=True), S.settings(seed=1, integrator=S.INTEGRATOR_SPPM), 20000)
+    assert r.resolve()[..., 1].mean() == pytest.approx(S.mean_disk_irradiance(2.0, a, h, half), rel=0.01)
 
 
 def test_sppm_is_reproducible(cuda):

@@ -74,6 +74,16 @@ def test_sppm_initial_radius(oracle):
             assert r0 == 0 and floor == pytest.approx(3.0 * pixel)
 
 
+def test_sppm_sensor_measures_disk_irradiance(oracle):
+    """A surface sensor aimed at a Lambertian disc (tests/analytic.hpp:28-60): every camera path meets
+    the emitter, so the camera pass alone carries the irradiance, within 1 %."""
+    a, h, half = 0.3, 0.5, 0.15
+    w = World(oracle, S.disk_irradiance_scene(a, h, half, aimed=True))
+    st = w.sppm(S.settings(seed=1, integrator=SPPM), 20000)
+    assert World.sppm_resolve(st)[..., 1].mean() == pytest.approx(S.mean_disk_irradiance(2.0, a, h, half), rel=0.01)
+    assert np.all(st["r"] == 0)  # no visible points: the sensor sees only the emitter
+
+
 def test_sppm_iterations_continue(oracle):
     """Two calls of n iterations are one call of 2n: the state carries everything."""
     w = World(oracle, S.white_furnace_scene(0.5, size=8))
