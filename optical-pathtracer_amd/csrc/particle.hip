@@ -44,6 +44,8 @@ struct PCtx {
     float beta[NW];
     float sum[3];
     uint32_t segments, inconsistencies, leaks;
+    uint32_t guideCell;  // SPPM: the guide cell a sun photon of the uniform share teaches
+    bool useful;         // SPPM: the photon reached a visible point
 };
 
 __device__ __forceinline__ void addFixedPoint(unsigned long long* bins, float v) {
@@ -193,8 +195,8 @@ struct PRay {
 };
 
 // emitParticle (particle.slang:306-325): the sky or a light by power, then its emission.
-template <bool kContacts>
-__device__ __forceinline__ bool emitParticle(PCtx& c, bool observer, PRay& ray) {
+template <bool kContacts, bool kGuide = false>
+__device__ __forceinline__ bool emitParticle(PCtx& c, bool observer, PRay& ray, const SppmDev* g = nullptr) {
     const DScene& sc = c.sc;
     const float pSky = sc.env.skyShare;
     const bool fromSky = c.rng.uniform() < pSky;
@@ -226,16 +228,39 @@ __device__ __forceinline__ bool emitParticle(PCtx& c, bool observer, PRay& ray) 
         }
         const f3 dir = -toward;
         const Frame disc = makeFrame(dir);
-        const float v2 = c.rng.uniform(), v1 = c.rng.uniform();
-        float qx, qy;
-        concentricDisk(v1, v2, qx, qy);
+        float qx, qy, pdfUnit = 1.0f / kPiF;  // the point's density per unit-disc area
+        if (kGuide && !fromSky && *g->guideTotal > 0.0f) {
+            // sampleEmissionDisc (particle.slang:205-232): the map's share, or uniform over the disc.
+            const bool guided = c.rng.uniform() < kGuideShare;
+            const float v2 = c.rng.uniform(), v1 = c.rng.uniform();
+            if (guided) {
+                const uint32_t cell = guideSample(g->guideCdf, v1);
+                const float size = 2.0f / float(kGuideRes);
+                qx = (float(cell % kGuideRes) + v2) * size - 1.0f;
+                qy = (float(cell / kGuideRes) + c.rng.uniform()) * size - 1.0f;
+            } else {
+                concentricDisk(v1, v2, qx, qy);
+            }
+            uint32_t cell;
+            pdfUnit = guidePdf(g->guideCdf, *g->guideTotal, qx, qy, cell);
+            if (!(pdfUnit > 0.0f)) return false;
+            if (!guided) c.guideCell = cell;  // the uniform share teaches the map
+        } else {
+            const float v2 = c.rng.uniform(), v1 = c.rng.uniform();
+            concentricDisk(v1, v2, qx, qy);
+            if (kGuide && !fromSky) {
+                uint32_t cell;
+                guidePdf(g->guideCdf, 0.0f, qx, qy, cell);
+                c.guideCell = cell;  // before the map has learnt: every sun photon teaches it
+            }
+        }
         ray.o = ld3(sc.env.center) - dir * (1.01f * R) + disc.s * (qx * R) + disc.t * (qy * R);
         ray.d = dir;
         ray.region = 0;
         ray.departed = kNoBoundary;
         ray.face = ~0u;
         ray.distant = true;
-        const float w = kPiF * R * R / (sel * pdfDir);
+        const float w = kGuide ? R * R / (sel * pdfDir * pdfUnit) : kPiF * R * R / (sel * pdfDir);
 #pragma unroll
         for (int i = 0; i < NW; ++i) c.beta[i] = Le[i] * w;
     } else {
@@ -368,6 +393,7 @@ __device__ __forceinline__ void sppmGather(PCtx& c, const SppmDev& s, const Hit&
                 addFixedPoint(bins + kSplatBins, Y);
                 addFixedPoint(bins + 2 * kSplatBins, Z);
                 atomicAdd(&s.count[i], 1u);
+                c.useful = true;
             }
         }
     }
@@ -384,7 +410,7 @@ __device__ __forceinline__ void traceParticle(PCtx& c, const SppmDev* sp = nullp
     const int maxDepth = c.set.maxDepth, rrDepth = c.set.rrDepth;
     int events = 0;  // scattering events so far (SPPM: direct light is the camera pass's)
     PRay ray;
-    if (!emitParticle<kContacts>(c, observer, ray)) return;
+    if (!emitParticle<kContacts, kSppm>(c, observer, ray, sp)) return;
     for (int depth = 0;; ++depth) {
         Hit h;
         bool found = intersectContact<true, kContacts>(sc, ray.o, ray.d, __builtin_inff(), ray.departed, ray.face, kNoBoundary, ray.region,
@@ -586,7 +612,10 @@ __global__ __launch_bounds__(kPBlock) void sppmPhotonKernel(DScene sc, Settings 
         c.rng.seed(hashPair(set.seed ^ kSppmPhoton, index), s.iteration);
         const uint32_t group = uint32_t(index % uint64_t(s.groups));
         c.wl = sppmWavelengths(set, s.rotation, group, s.groups);
+        c.guideCell = kGridNone;
+        c.useful = false;
         traceParticle<kContacts, true>(c, &s, group);
+        if (c.useful && c.guideCell != kGridNone) atomicAdd(&s.guideCounts[c.guideCell], 1u);
     }
     const uint32_t n[4] = {waveSum(live ? 1u : 0u), waveSum(c.segments), waveSum(c.inconsistencies), waveSum(c.leaks)};
     if (__lane_id() == 0)

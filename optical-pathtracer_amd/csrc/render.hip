@@ -1035,6 +1035,36 @@ __global__ void sppmGridKernel(SppmDev s) {
     }
 }
 
+// The photon guide's CDF from its counts so far (sppm_guide_kernel): one block of 256 threads, 64 cells
+// each; exact integer prefix sums, each normalised in double and stored as a float. The counts are
+// stable here: the previous iteration's photons have finished and this one's have not started.
+__global__ __launch_bounds__(256) void sppmGuideKernel(SppmDev s) {
+    __shared__ unsigned long long part[256];
+    __shared__ unsigned long long total;
+    const uint32_t t = threadIdx.x, per = kGuideCells / 256, base = t * per;
+    unsigned long long sum = 0;
+    for (uint32_t i = 0; i < per; ++i) sum += s.guideCounts[base + i];
+    part[t] = sum;
+    __syncthreads();
+    if (t == 0) {
+        unsigned long long run = 0;
+        for (int k = 0; k < 256; ++k) {
+            const unsigned long long v = part[k];
+            part[k] = run;  // exclusive prefix
+            run += v;
+        }
+        total = run;
+        *s.guideTotal = float(run);
+    }
+    __syncthreads();
+    const double tot = double(total);
+    unsigned long long run = part[t];
+    for (uint32_t i = 0; i < per; ++i) {
+        run += s.guideCounts[base + i];
+        s.guideCdf[base + i] = total > 0 ? float(double(run) / tot) : 0.0f;
+    }
+}
+
 // The progressive update (sppm_update_kernel): N' = N + αM, r' = r·√(N'/(N+M)), τ' = (τ + Φ)·N'/(N+M);
 // then the iteration's deposits are cleared. A pixel no photon reached keeps its state.
 __global__ void sppmUpdateKernel(SppmDev s) {
@@ -1421,6 +1451,7 @@ void runSppm(owe_hip_renderer* r, int iterations) {
         else if (r->stackCap <= 32) launchSppmCamera<32>(r);
         else launchSppmCamera<48>(r);
         hipLaunchKernelGGL(sppmGridKernel, pixelGrid, block, 0, r->stream, q);
+        hipLaunchKernelGGL(sppmGuideKernel, dim3{1}, block, 0, r->stream, q);
         if (r->host.emittedPower > 0)
             for (uint64_t base = 0; base < r->photons; base += r->capacity)
                 launchSppmPhotons(r->dev, r->set, q, base, uint32_t(std::min<uint64_t>(r->capacity, r->photons - base)),
@@ -1564,6 +1595,10 @@ int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render
             q.gridMask = uint32_t(gridSize - 1);
             q.levels = devAlloc<int>(r, 2);
             q.stats = devAlloc<unsigned long long>(r, 4);
+            q.guideCounts = devAlloc<unsigned int>(r, kGuideCells);
+            q.guideCdf = devAlloc<float>(r, kGuideCells);
+            q.guideTotal = devAlloc<float>(r, 1);
+            OWE_HIP_CHECK(hipMemset(q.guideCounts, 0, kGuideCells * sizeof(unsigned int)));
             q.groups = uint32_t(photonGroups);
             q.alpha = float(photonAlpha);
             q.invNWY = r->invNWY;
@@ -1784,6 +1819,7 @@ int owe_hip_renderer_reset(owe_hip_renderer* r) {
             OWE_HIP_CHECK(hipMemsetAsync(q.tauXY, 0, pixels * sizeof(double2), r->stream));
             OWE_HIP_CHECK(hipMemsetAsync(q.tauZN, 0, pixels * sizeof(double2), r->stream));
             OWE_HIP_CHECK(hipMemsetAsync(q.flux, 0, pixels * 3 * kSplatBins * sizeof(unsigned long long), r->stream));
+            OWE_HIP_CHECK(hipMemsetAsync(q.guideCounts, 0, kGuideCells * sizeof(unsigned int), r->stream));
             r->sppmIterations = 0;
         }
         OWE_HIP_CHECK(hipStreamSynchronize(r->stream));

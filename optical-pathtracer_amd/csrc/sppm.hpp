@@ -49,7 +49,45 @@ struct SppmDev {
     float r0Floor;
     float invNWY;        // 1 / (NW ∫ȳ)
     unsigned long long* stats;  // += paths, segments, inconsistencies, leaks
+    // Photon guiding over the sun's emission disc: counts per cell, their CDF (floats), its total.
+    unsigned int* guideCounts;
+    float* guideCdf;
+    float* guideTotal;
 };
+
+// Photon guiding (particle.slang:149-183): kGuideRes² cells over the bounding square of the sun's
+// emission disc, in units of its radius. Once the map has counts, kGuideShare of sun photons are
+// drawn from it and every sun photon is weighted by the mixture density: the estimate is unchanged
+// (the uniform share keeps full support), only its noise falls.
+constexpr int kGuideRes = 128, kGuideCells = kGuideRes * kGuideRes;
+constexpr float kGuideShare = 0.75f;
+
+// The first cell whose CDF exceeds u.
+__device__ __forceinline__ uint32_t guideSample(const float* cdf, float u) {
+    uint32_t lo = 0, len = kGuideCells;
+    while (len > 0) {
+        const uint32_t half = len / 2;
+        if (cdf[lo + half] <= u) {
+            lo += half + 1;
+            len -= half + 1;
+        } else {
+            len = half;
+        }
+    }
+    return lo < uint32_t(kGuideCells) ? lo : uint32_t(kGuideCells - 1);
+}
+
+// The sun's emission density at (qx, qy) per unit-disc area: the map's mixture with the uniform share.
+__device__ __forceinline__ float guidePdf(const float* cdf, float total, float qx, float qy, uint32_t& cell) {
+    const float size = 2.0f / float(kGuideRes);
+    const int cx = int(floorf((qx + 1.0f) / size)), cy = int(floorf((qy + 1.0f) / size));
+    const bool in = cx >= 0 && cy >= 0 && cx < kGuideRes && cy < kGuideRes;
+    cell = in ? uint32_t(cy * kGuideRes + cx) : kGridNone;
+    const float uniform = qx * qx + qy * qy <= 1.0f ? 1.0f / kPiF : 0.0f;
+    if (!(total > 0.0f)) return uniform;
+    const float p = in ? cdf[cell] - (cell > 0 ? cdf[cell - 1] : 0.0f) : 0.0f;
+    return (in ? kGuideShare * p / (size * size) : 0.0f) + (1.0f - kGuideShare) * uniform;
+}
 
 // The level whose cells (size 2^level) are the smallest at least 2r across: exact, from the float's
 // exponent (a log2 rounded the wrong way would give cells smaller than the reach).

@@ -75,12 +75,12 @@ void oracle_sppm_radius(const void* wh, const owe_render_settings* rs, double ou
 
 // Iterations [start, start + n) of SPPM (sppm.slang; GpuRenderer::passSppm, gpu_renderer_passes.cpp:259-290)
 // on caller-held state: direct (W·H·3) sums the camera passes' own XYZ; state (W·H·5) is τ (XYZ), N
-// and the gather radius r per pixel. A photon reaches every visible point within r: the reference's
+// and the gather radius r per pixel; guideCounts (PhotonGuide::kCells) the photon guide's counts. A photon reaches every visible point within r: the reference's
 // multi-level hash finds exactly these (a point lives in one cell of size ≥ 2r, and the 2×2×2 cells
 // about the photon cover its reach), so the oracle searches them directly. stats += paths (camera
 // paths and photons), segments, inconsistencies, leaks.
 void oracle_sppm(const void* wh, const owe_render_settings* rs, int n, int start, int threads, double* direct,
-                 double* state, uint64_t* stats) {
+                 double* state, uint64_t* guideCounts, uint64_t* stats) {
     const World& w = *static_cast<const World*>(wh);
     const Settings set = toSettings(*rs);
     const Tracer tr(w, set);
@@ -93,6 +93,8 @@ void oracle_sppm(const void* wh, const owe_render_settings* rs, int n, int start
     oracle_sppm_radius(wh, rs, r0);
     std::vector<Stats> st(static_cast<size_t>(threads));
     std::vector<VisiblePoint> vps(pixels);
+    PhotonGuide guide;
+    std::copy(guideCounts, guideCounts + PhotonGuide::kCells, guide.counts.begin());
     auto parallel = [&](uint64_t count, auto&& body) {
         std::vector<std::thread> pool;
         for (int k = 0; k < threads; ++k)
@@ -137,6 +139,7 @@ void oracle_sppm(const void* wh, const owe_render_settings* rs, int n, int start
             double rmax;
             uint32_t group, groups;
             std::vector<double> flux;  // per pixel: XYZ, M
+            std::vector<uint32_t> taught;  // guide cells of this thread's useful photons
             void gather(const Hit& hit, V3 d, const Spec4& beta, const Wavelengths& wl) override {
                 const auto lo = std::lower_bound(xs->begin(), xs->end(), hit.p.x - rmax);
                 const auto hi = std::upper_bound(xs->begin(), xs->end(), hit.p.x + rmax);
@@ -153,6 +156,7 @@ void oracle_sppm(const void* wh, const owe_render_settings* rs, int n, int start
                     if (!tr->sppmValue(v, int(groups), d, beta, wl, c)) continue;
                     for (int j = 0; j < 3; ++j) flux[4 * size_t(i) + size_t(j)] += c[j];
                     flux[4 * size_t(i) + 3] += 1;
+                    useful = true;
                 }
             }
         };
@@ -167,12 +171,17 @@ void oracle_sppm(const void* wh, const owe_render_settings* rs, int n, int start
             g.groups = G;
             g.flux.assign(4 * pixels, 0.0);
         }
+        // The guide learnt from the iterations before this one (sppm_guide_kernel).
+        guide.build();
         parallel(photons, [&](int k, uint64_t i) {
             Gather& g = sinks[size_t(k)];
             g.group = uint32_t(i % G);
             Pcg rng(hashPair(set.seed ^ kSppmPhoton, i), uint64_t(it));
-            tr.sppmPhoton(sppmWavelengths(set, rot, g.group), rng, st[size_t(k)], g);
+            const uint32_t cell = tr.sppmPhoton(sppmWavelengths(set, rot, g.group), rng, st[size_t(k)], g, &guide);
+            if (cell != kNone) g.taught.push_back(cell);
         });
+        for (const Gather& g : sinks)
+            for (uint32_t c : g.taught) guide.counts[c]++;
         // The progressive update (sppm_update_kernel): N' = N + αM, r' = r·√(N'/(N+M)), τ' = (τ+Φ)·N'/(N+M).
         for (size_t i = 0; i < pixels; ++i) {
             double phi[3] = {0, 0, 0}, M = 0;
@@ -188,6 +197,7 @@ void oracle_sppm(const void* wh, const owe_render_settings* rs, int n, int start
             s[4] *= std::sqrt(ratio);
         }
     }
+    std::copy(guide.counts.begin(), guide.counts.end(), guideCounts);
     Stats total;
     for (const Stats& s : st) total.add(s);
     stats[0] += total.paths;

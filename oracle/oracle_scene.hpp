@@ -261,6 +261,21 @@ struct Settings {
     int photonGroups = 4;
 };
 
+// SPPM photon guiding (particle.slang:149-183): a map of kRes² cells over the sun's emission disc's
+// bounding square (coordinates in units of the disc's radius) counting the uniform share's photons
+// that reached a visible point; once it has counts, kShare of sun photons are drawn from it and every
+// sun photon is weighted by the mixture density. The CDF is stored as floats, as the GPU keeps it.
+struct PhotonGuide {
+    static constexpr int kRes = 128, kCells = kRes * kRes;
+    static constexpr double kShare = 0.75;
+    std::vector<uint64_t> counts = std::vector<uint64_t>(kCells, 0);
+    std::vector<float> cdf = std::vector<float>(kCells, 0.0f);
+    double total = 0;
+    void build();                              // the CDF of the counts so far
+    uint32_t sample(float u) const;            // the first cell whose CDF exceeds u
+    double pdf(double qx, double qy, uint32_t& cell) const;  // density per unit-disc area; cell of q or kNone
+};
+
 // An SPPM visible point (sppm.slang:33-40): the first non-specular surface a camera path reaches
 // through specular events only.
 struct VisiblePoint {
@@ -300,19 +315,22 @@ public:
     };
     // One particle of a light-tracing pass (particle_kernel.slang:11-19, particle.slang:356-427):
     // its wavelengths from the first uniform of `rng`, then emission, walk and splats.
-    void particle(Pcg& rng, Stats& st, Splat& out) const { particleWalk(nullptr, rng, st, out, nullptr); }
+    void particle(Pcg& rng, Stats& st, Splat& out) const { particleWalk(nullptr, rng, st, out, nullptr, nullptr); }
 
     // SPPM (sppm.slang): a photon arriving at a non-specular surface after at least one scattering event.
     struct PhotonSink {
         virtual void gather(const Hit& hit, V3 d, const Spec4& beta, const Wavelengths& wl) = 0;
+        bool useful = false;  // the current photon reached a visible point
         virtual ~PhotonSink() = default;
     };
     // The camera pass's path of pixel (x, y) at the iteration's wavelengths `wl` (sppm_camera_kernel):
     // its own radiance (emission met through specular events, direct light at the visible point) in
     // xyz, and the visible point, if any, in vp. False when the sample measures nothing.
     bool sppmCamera(int x, int y, Wavelengths wl, Pcg& rng, Stats& st, double xyz[3], VisiblePoint& vp) const;
-    // One photon at the given wavelengths (traceSppmPhoton, sppm.slang:150-201).
-    void sppmPhoton(Wavelengths wl, Pcg& rng, Stats& st, PhotonSink& sink) const;
+    // One photon at the given wavelengths (traceSppmPhoton, sppm.slang:150-201), its sun emission
+    // guided by `guide` when given: the guide cell the photon teaches when it proved useful (the sink
+    // set `useful`), else kNone.
+    uint32_t sppmPhoton(Wavelengths wl, Pcg& rng, Stats& st, PhotonSink& sink, const PhotonGuide* guide) const;
     // A visible point's XYZ for a photon arriving along d with throughput beta at wavelengths wl, once
     // the point has passed the reach, disc, orientation and group tests (gatherPhoton's per-point
     // value, sppm.slang:117-136): β_vp · f_vp(wo, wi) · β_photon, a bound point's share scaled by the
@@ -325,11 +343,13 @@ private:
     struct Emitted;
     bool linkObserver(V3 x, Pcg& rng, Link& k) const;
     void deposit(Splat& out, int px, int py, const Spec4& v, const Wavelengths& wl) const;
-    bool emitParticle(Wavelengths& wl, Pcg& rng, bool connect, Emitted& e, Splat& out) const;
+    bool emitParticle(Wavelengths& wl, Pcg& rng, bool connect, Emitted& e, Splat& out,
+                      const PhotonGuide* guide = nullptr) const;
     void connectVertex(const Hit& hit, bool surface, const Optics* o, V3 d, double g, uint32_t region,
                        const Spec4& beta, const Wavelengths& wl, Pcg& rng, Splat& out) const;
     struct Path;
-    void particleWalk(const Wavelengths* fixed, Pcg& rng, Stats& st, Splat& out, PhotonSink* sink) const;
+    uint32_t particleWalk(const Wavelengths* fixed, Pcg& rng, Stats& st, Splat& out, PhotonSink* sink,
+                          const PhotonGuide* guide) const;
     Spec4 directLighting(const Hit& hit, bool surface, const Optics* o, double nFront, double nBack, V3 d,
                          const Wavelengths& wl, Pcg& rng, double g, uint32_t region) const;
     bool scatterAtSurface(const Hit& hit, const Optics& o, uint32_t other, int depth, Wavelengths& wl, Pcg& rng,

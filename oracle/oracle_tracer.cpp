@@ -521,7 +521,43 @@ struct Tracer::Emitted {
     uint32_t region = 0;
     Spec4 beta;
     bool distant = false;
+    uint32_t guideCell = kNone;  // SPPM: the guide cell a sun photon of the uniform share teaches
 };
+
+void PhotonGuide::build() {
+    total = 0;
+    for (uint64_t c : counts) total += double(c);
+    double run = 0;
+    for (int i = 0; i < kCells; ++i) {
+        run += double(counts[size_t(i)]);
+        cdf[size_t(i)] = total > 0 ? float(run / total) : 0.0f;
+    }
+}
+
+uint32_t PhotonGuide::sample(float u) const {
+    uint32_t lo = 0, len = kCells;
+    while (len > 0) {
+        const uint32_t half = len / 2;
+        if (cdf[lo + half] <= u) {
+            lo += half + 1;
+            len -= half + 1;
+        } else {
+            len = half;
+        }
+    }
+    return std::min<uint32_t>(lo, kCells - 1);
+}
+
+double PhotonGuide::pdf(double qx, double qy, uint32_t& cell) const {
+    const double size = 2.0 / kRes;
+    const int cx = int(std::floor((qx + 1) / size)), cy = int(std::floor((qy + 1) / size));
+    const bool in = cx >= 0 && cy >= 0 && cx < kRes && cy < kRes;
+    cell = in ? uint32_t(cy * kRes + cx) : kNone;
+    const double uniform = qx * qx + qy * qy <= 1 ? 1 / kPi : 0;
+    if (!(total > 0)) return uniform;
+    const double p = in ? double(cdf[cell]) - (cell > 0 ? double(cdf[cell - 1]) : 0.0) : 0.0;
+    return (in ? kShare * p / (size * size) : 0.0) + (1 - kShare) * uniform;
+}
 
 bool Tracer::linkObserver(V3 x, Pcg& rng, Link& k) const {
     const Observer& ob = w_.observer;
@@ -601,7 +637,8 @@ void Tracer::connectVertex(const Hit& hit, bool surface, const Optics* o, V3 d, 
 }
 
 // emitParticle with emitDistant / emitFromArea (particle.slang:234-344), without photon guiding.
-bool Tracer::emitParticle(Wavelengths& wl, Pcg& rng, bool connect, Emitted& e, Splat& out) const {
+bool Tracer::emitParticle(Wavelengths& wl, Pcg& rng, bool connect, Emitted& e, Splat& out,
+                          const PhotonGuide* guide) const {
     e = Emitted{};
     const double pSky = w_.skyShare;
     const bool fromSky = rng.uniform() < pSky;
@@ -630,12 +667,30 @@ bool Tracer::emitParticle(Wavelengths& wl, Pcg& rng, bool connect, Emitted& e, S
         }
         const V3 dir = -toward;
         const Frame disc(dir);
-        double v1, v2, qx, qy;
-        rng.draw2(v1, v2);
-        concentricDisk(v1, v2, qx, qy);
+        double v1, v2, qx, qy, pdfUnit = 1 / kPi;  // the point's density per unit-disc area
+        if (guide && !fromSky && guide->total > 0) {
+            // sampleEmissionDisc (particle.slang:205-232): the map's share, or uniform over the disc.
+            const bool guided = rng.uniform() < PhotonGuide::kShare;
+            rng.draw2(v1, v2);
+            if (guided) {
+                const uint32_t c = guide->sample(float(v1));
+                qx = (double(c % PhotonGuide::kRes) + v2) * (2.0 / PhotonGuide::kRes) - 1;
+                qy = (double(c / PhotonGuide::kRes) + rng.uniform()) * (2.0 / PhotonGuide::kRes) - 1;
+            } else {
+                concentricDisk(v1, v2, qx, qy);
+            }
+            uint32_t cell;
+            pdfUnit = guide->pdf(qx, qy, cell);
+            if (!(pdfUnit > 0)) return false;
+            if (!guided) e.guideCell = cell;  // the uniform share teaches the map
+        } else {
+            rng.draw2(v1, v2);
+            concentricDisk(v1, v2, qx, qy);
+            if (guide && !fromSky) guide->pdf(qx, qy, e.guideCell);  // before the map has learnt: every sun photon
+        }
         e.ray = Ray{w_.sceneCenter() - dir * (1.01 * R) + disc.s * (qx * R) + disc.t * (qy * R), dir, kNone, kNone};
         e.region = 0;
-        e.beta = Le * (1 / (sel * pdfDir * (1 / (kPi * R * R))));
+        e.beta = Le * (1 / (sel * pdfDir * (pdfUnit / (R * R))));
         e.distant = true;
     } else {
         const double sel = (1 - pSky) * w_.lightPdf[li];
@@ -685,7 +740,8 @@ bool Tracer::emitParticle(Wavelengths& wl, Pcg& rng, bool connect, Emitted& e, S
 // A light-tracing particle (its wavelengths from the first uniform of rng), or with `sink` an SPPM
 // photon at the fixed wavelengths: no observer connections or sensor splats; a photon gathers at
 // every non-specular surface after its first scattering event (sppm.slang:150-201).
-void Tracer::particleWalk(const Wavelengths* fixed, Pcg& rng, Stats& st, Splat& out, PhotonSink* sink) const {
+uint32_t Tracer::particleWalk(const Wavelengths* fixed, Pcg& rng, Stats& st, Splat& out, PhotonSink* sink,
+                              const PhotonGuide* guide) const {
     Wavelengths wl;
     if (fixed) {
         wl = *fixed;
@@ -699,7 +755,7 @@ void Tracer::particleWalk(const Wavelengths* fixed, Pcg& rng, Stats& st, Splat& 
     int events = 0;
     st.paths++;
     Emitted em;
-    if (!emitParticle(wl, rng, observer, em, out)) return;
+    if (!emitParticle(wl, rng, observer, em, out, guide)) return kNone;
     Ray ray = em.ray;
     uint32_t region = em.region;
     Spec4 beta = em.beta;
@@ -797,6 +853,7 @@ void Tracer::particleWalk(const Wavelengths* fixed, Pcg& rng, Stats& st, Splat& 
             beta *= 1 / s.weight;
         }
     }
+    return em.guideCell;
 }
 
 // ---------------------------------------------------------------- SPPM (sppm.slang)
@@ -825,11 +882,13 @@ bool Tracer::sppmCamera(int x, int y, Wavelengths wl, Pcg& rng, Stats& st, doubl
     return true;
 }
 
-void Tracer::sppmPhoton(Wavelengths wl, Pcg& rng, Stats& st, PhotonSink& sink) const {
+uint32_t Tracer::sppmPhoton(Wavelengths wl, Pcg& rng, Stats& st, PhotonSink& sink, const PhotonGuide* guide) const {
     struct None final : Splat {
         void add(int, int, const double*) override {}
     } none;
-    particleWalk(&wl, rng, st, none, &sink);
+    sink.useful = false;
+    const uint32_t cell = particleWalk(&wl, rng, st, none, &sink, guide);
+    return sink.useful ? cell : kNone;
 }
 
 bool Tracer::sppmValue(const VisiblePoint& v, int groups, V3 d, const Spec4& beta, const Wavelengths& wl,

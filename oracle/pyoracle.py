@@ -163,7 +163,7 @@ class Oracle(_Lib):
             "render": (None, [_vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp]),
             "camera_paths": (None, [_vp, _vp, _i, _i, _u64, _vp, _vp, _vp, _vp, _i]),
             "render_particles": (None, [_vp, _vp, _i, _i, _i, _vp, _vp]),
-            "sppm": (None, [_vp, _vp, _i, _i, _i, _vp, _vp, _vp]),
+            "sppm": (None, [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp]),
             "sppm_radius": (None, [_vp, _vp, _vp]),
             "particles": (None, [_vp, _vp, _i, _u64, _vp, _vp, _vp, _i]),
             "emission_info": (None, [_vp, _vp]),
@@ -277,14 +277,15 @@ class World:
         W, H = self.width, self.height
         if state is None:
             state = dict(direct=np.zeros((H, W, 3)), pix=np.zeros((H, W, 5)), iterations=0,
-                         stats=np.zeros(4, dtype=np.uint64))
+                         stats=np.zeros(4, dtype=np.uint64), guide=np.zeros(128 * 128, dtype=np.uint64))
         direct = np.ascontiguousarray(state["direct"]).copy()
         pix = np.ascontiguousarray(state["pix"]).copy()
         stats = state["stats"].copy()
+        guide = state["guide"].copy()
         self.o.sppm(self.h, ctypes.byref(settings), iterations, state["iterations"], threads, _ptr(direct), _ptr(pix),
-                    _ptr(stats))
+                    _ptr(guide), _ptr(stats))
         ppp = settings.photons_per_pixel if settings.photons_per_pixel > 0 else 1.0
-        return dict(direct=direct, pix=pix, iterations=state["iterations"] + iterations, stats=stats,
+        return dict(direct=direct, pix=pix, iterations=state["iterations"] + iterations, stats=stats, guide=guide,
                     photons=max(1, int(round(W * H * ppp))), tau=pix[..., :3], N=pix[..., 3], r=pix[..., 4])
 
     def sppm_radius(self, settings):

@@ -84,11 +84,26 @@ def test_sppm_sensor_measures_disk_irradiance(oracle):
     assert np.all(st["r"] == 0)  # no visible points: the sensor sees only the emitter
 
 
+def test_sppm_photon_guide_learns_the_lens(oracle):
+    """The guide counts only sun photons that reached a visible point: in the caustic scene after the
+    floor's direct light (the camera pass's), those are the photons through the 40 mm lens, a small
+    patch of the emission disc — and the guided estimate agrees with the unguided one."""
+    ls = C.caustic_cases(resolution=32)["direct"]
+    w = World(oracle, ls.use())
+    st = w.sppm(ls.settings(seed=2), 64)
+    g = st["guide"]
+    assert g.sum() > 0
+    assert (g > 0).mean() < 0.02  # a lens of radius 20 mm in a disc that covers the 300 mm floor
+    assert World.sppm_resolve(st)[..., 1].mean() > 0
+
+
 def test_sppm_iterations_continue(oracle):
     """Two calls of n iterations are one call of 2n: the state carries everything."""
-    w = World(oracle, S.white_furnace_scene(0.5, size=8))
-    st = S.settings(seed=4, integrator=SPPM)
-    a = w.sppm(st, 6)
-    b = w.sppm(st, 3)
-    b = w.sppm(st, 3, state=b)
-    assert np.array_equal(a["pix"], b["pix"]) and np.array_equal(a["direct"], b["direct"])
+    ls = C.caustic_cases(resolution=16)["direct"]  # sun photons: the guide's counts are state too
+    for w, st in ((World(oracle, S.white_furnace_scene(0.5, size=8)), S.settings(seed=4, integrator=SPPM)),
+                  (World(oracle, ls.use()), ls.settings(seed=4))):
+        a = w.sppm(st, 6)
+        b = w.sppm(st, 3)
+        b = w.sppm(st, 3, state=b)
+        assert np.array_equal(a["pix"], b["pix"]) and np.array_equal(a["direct"], b["direct"])
+        assert np.array_equal(a["guide"], b["guide"])
