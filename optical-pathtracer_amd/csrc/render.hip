@@ -1035,34 +1035,38 @@ __global__ void sppmGridKernel(SppmDev s) {
     }
 }
 
-// The photon guide's CDF from its counts so far (sppm_guide_kernel): one block of 256 threads, 64 cells
-// each; exact integer prefix sums, each normalised in double and stored as a float. The counts are
-// stable here: the previous iteration's photons have finished and this one's have not started.
-__global__ __launch_bounds__(256) void sppmGuideKernel(SppmDev s) {
-    __shared__ unsigned long long part[256];
-    __shared__ unsigned long long total;
-    const uint32_t t = threadIdx.x, per = kGuideCells / 256, base = t * per;
+// The photon guide's CDF from its counts so far (sppm_guide_kernel): one block of 1024 threads, 16
+// cells each. The counts are staged through LDS in coalesced reads, the per-thread sums scanned
+// (Hillis–Steele), each cell's exact integer prefix normalised in double and stored as a float, and
+// the CDF written back coalesced. The counts are stable here: the previous iteration's photons have
+// finished and this one's have not started.
+constexpr uint32_t kGuideThreads = 1024;
+__global__ __launch_bounds__(kGuideThreads) void sppmGuideKernel(SppmDev s) {
+    __shared__ uint32_t cells[kGuideCells];  // counts, then the CDF's float bits (64 KiB)
+    __shared__ unsigned long long part[kGuideThreads];
+    const uint32_t t = threadIdx.x, per = kGuideCells / kGuideThreads, base = t * per;
+    for (uint32_t i = t; i < uint32_t(kGuideCells); i += kGuideThreads) cells[i] = s.guideCounts[i];
+    __syncthreads();
     unsigned long long sum = 0;
-    for (uint32_t i = 0; i < per; ++i) sum += s.guideCounts[base + i];
+    for (uint32_t i = 0; i < per; ++i) sum += cells[base + i];
     part[t] = sum;
     __syncthreads();
-    if (t == 0) {
-        unsigned long long run = 0;
-        for (int k = 0; k < 256; ++k) {
-            const unsigned long long v = part[k];
-            part[k] = run;  // exclusive prefix
-            run += v;
-        }
-        total = run;
-        *s.guideTotal = float(run);
+    for (uint32_t off = 1; off < kGuideThreads; off <<= 1) {
+        const unsigned long long v = t >= off ? part[t - off] : 0ull;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    const unsigned long long total = part[kGuideThreads - 1];
+    const double tot = double(total);
+    unsigned long long run = part[t] - sum;  // exclusive prefix of this thread's cells
+    for (uint32_t i = 0; i < per; ++i) {
+        run += cells[base + i];
+        cells[base + i] = __float_as_uint(total > 0 ? float(double(run) / tot) : 0.0f);
     }
     __syncthreads();
-    const double tot = double(total);
-    unsigned long long run = part[t];
-    for (uint32_t i = 0; i < per; ++i) {
-        run += s.guideCounts[base + i];
-        s.guideCdf[base + i] = total > 0 ? float(double(run) / tot) : 0.0f;
-    }
+    for (uint32_t i = t; i < uint32_t(kGuideCells); i += kGuideThreads) s.guideCdf[i] = __uint_as_float(cells[i]);
+    if (t == 0) *s.guideTotal = float(total);
 }
 
 // The progressive update (sppm_update_kernel): N' = N + αM, r' = r·√(N'/(N+M)), τ' = (τ + Φ)·N'/(N+M);
@@ -1451,7 +1455,7 @@ void runSppm(owe_hip_renderer* r, int iterations) {
         else if (r->stackCap <= 32) launchSppmCamera<32>(r);
         else launchSppmCamera<48>(r);
         hipLaunchKernelGGL(sppmGridKernel, pixelGrid, block, 0, r->stream, q);
-        hipLaunchKernelGGL(sppmGuideKernel, dim3{1}, block, 0, r->stream, q);
+        hipLaunchKernelGGL(sppmGuideKernel, dim3{1}, dim3{kGuideThreads}, 0, r->stream, q);
         if (r->host.emittedPower > 0)
             for (uint64_t base = 0; base < r->photons; base += r->capacity)
                 launchSppmPhotons(r->dev, r->set, q, base, uint32_t(std::min<uint64_t>(r->capacity, r->photons - base)),
