@@ -1,9 +1,10 @@
 """The hip backend's SPPM integrator against the oracle and the reference's own SPPM cases.
 
-* per pixel, one iteration from the same (seed, pixel, iteration) camera streams and (seed, photon,
-  iteration) photon streams: the initial gather radius (from the unfolded path length), the photon
-  count M and the flux τ agree with the fp64 oracle on almost every pixel (the rest are float
-  decisions that fall the other way), and the camera pass's direct part agrees;
+* per pixel, from the same (seed, pixel, iteration) camera streams and (seed, photon, iteration)
+  photon streams: the initial gather radius (from the unfolded path length), the photon count and
+  the flux τ agree with the fp64 oracle on almost every pixel (the rest are float decisions that
+  fall the other way), and the camera pass's direct part agrees — after one iteration, and after
+  three, whose sun photons are drawn from the guide the earlier ones taught;
 * after many iterations the mean image agrees with the oracle's;
 * the white furnace, L = Le / (1 − ρ) within 2 % (tests/test_backends.cpp:540-574);
 * the caustic seen through a glass plate and in a mirror, ratios within 0.015 of the plate's

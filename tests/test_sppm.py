@@ -14,7 +14,9 @@ reach, update) and is pinned by:
   within 0.015, at 48×48 pixels and 2048 iterations. These points are seen through glass or a
   mirror, bound to their wavelength group: a group share not scaled back up would be off by the
   number of groups;
-* the initial radius in closed form (GpuRenderer::resetSppm, gpu_renderer_passes.cpp:238-257).
+* the initial radius in closed form (GpuRenderer::resetSppm, gpu_renderer_passes.cpp:238-257);
+* photon guiding (particle.slang:149-232) learns only the useful patch of the sun's emission disc,
+  and its state carries across calls like the rest.
 """
 from __future__ import annotations
 
