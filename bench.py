@@ -754,9 +754,7 @@ def bench_sppm(args):
     the sun, the progressive update; a step is a pass of 16 iterations at 256x256."""
     import torch
 
-    sys.path.insert(0, str(ROOT / "tests"))
-    import sppm_cases as C
-
+    from owe_hip import caustic_scenes as C
     from owe_hip import render
 
     ls = C.caustic_cases(resolution=SPPM_SIZE)["plate"]

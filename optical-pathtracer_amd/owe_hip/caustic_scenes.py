@@ -1,5 +1,5 @@
-"""Scenes of the SPPM tests (the reference's own SPPM cases, tests/test_references.cpp:183-250 and
-tests/test_backends.cpp:540-574), shared by the oracle and GPU test files.
+"""The lens-caustic scenes of the reference's SPPM case (tests/test_references.cpp:183-250), as scene
+text for the loader: used by the SPPM tests and by bench.py's SPPM workload.
 
 The caustic scene: a sun overhead, an f = 100 mm N-BK7 lens 100 mm above a diffuse floor, an eye
 335 mm from the caustic looking down at it — optionally through a 5 mm N-BK7 plate across the line
@@ -13,7 +13,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from owe_hip import loader as LD
+from . import loader as LD
 
 EYE = np.array([268.3, 0.0, 150.0])
 SPOT = np.zeros(3)

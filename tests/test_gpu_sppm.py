@@ -19,7 +19,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-import sppm_cases as C
+from owe_hip import caustic_scenes as C
 from owe_hip import render
 from owe_hip import scene as S
 from pyoracle import World

@@ -25,7 +25,7 @@ import math
 import numpy as np
 import pytest
 
-import sppm_cases as C
+from owe_hip import caustic_scenes as C
 from owe_hip import scene as S
 from pyoracle import World
 
