@@ -912,8 +912,9 @@ __global__ __launch_bounds__(kBlock, kWaves) void persistentKernel(DScene sc, Se
 
 // The SPPM camera pass (sppm_camera_kernel): one thread per pixel and one camera path per iteration,
 // at the wavelengths of the pixel's group. The path's own radiance goes to the film (one sample per
-// pixel per iteration); its visible point, if any, to the pixel's slot, with the pixel's radius (set
-// at its first visible point from the unfolded path length).
+// pixel per iteration); its visible point, if any, to the pixel's slot with the unfolded path length.
+// It reads nothing the previous iteration's photons or update write, so it runs beside them on a
+// second stream, into the other of two visible-point buffers.
 template <int kCap, bool kContacts>
 __global__ __launch_bounds__(kBlock) void sppmCameraKernel(DScene sc, Settings set, SppmDev s, unsigned long long* film,
                                                            uint32_t* counters, int simple) {
@@ -975,16 +976,9 @@ __global__ __launch_bounds__(kBlock) void sppmCameraKernel(DScene sc, Settings s
             }
             paths = 1;
         }
-        float r = 0.0f;
-        if (rec.valid) {
-            r = s.radius[pixel];
-            if (r == 0.0f) {
-                r = fmaxf(s.r0PerMetre * rec.pathLength, s.r0Floor);
-                s.radius[pixel] = r;
-            }
-        }
+        // w: the unfolded path length; the hash fill turns it into the pixel's radius.
         const float w = ps.weight;
-        s.vpP[pixel] = make_float4(rec.p.x, rec.p.y, rec.p.z, r);
+        s.vpP[pixel] = make_float4(rec.p.x, rec.p.y, rec.p.z, rec.pathLength);
         s.vpN[pixel] = make_float4(rec.n.x, rec.n.y, rec.n.z, __uint_as_float(rec.boundary));
         s.vpD[pixel] = make_float4(rec.d.x, rec.d.y, rec.d.z, __uint_as_float(rec.valid ? rec.flags | (group << 8) : 0u));
         s.vpBeta[pixel] = make_float4(rec.beta.x * w, rec.beta.y * w, rec.beta.z * w, rec.beta.w * w);
@@ -1010,14 +1004,22 @@ __global__ void sppmClearGrid(SppmDev s) {
     }
 }
 
-// The visible points enter the hash (sppm_grid_kernel): each in the one cell of its level holding it.
+// The visible points enter the hash (sppm_grid_kernel): each in the one cell of its level holding it,
+// with the pixel's radius (set at its first visible point from the unfolded path length).
 // Launched with whole waves (blocks of 256): every lane takes part in the level reduction.
 __global__ void sppmGridKernel(SppmDev s) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     int lo = 0x7FFFFFFF, hi = int(0x80000001u);
     if (i < s.pixels && (__float_as_uint(s.vpD[i].w) & kVpValid)) {
-        const float4 p = s.vpP[i];
-        const int level = sppmLevel(p.w);
+        float4 p = s.vpP[i];
+        float r = s.radius[i];
+        if (r == 0.0f) {
+            r = fmaxf(s.r0PerMetre * p.w, s.r0Floor);
+            s.radius[i] = r;
+        }
+        p.w = r;
+        s.vpP[i] = p;
+        const int level = sppmLevel(r);
         const float size = ldexpf(1.0f, level);
         const uint32_t h = sppmHash(sppmCell(p.x, size), sppmCell(p.y, size), sppmCell(p.z, size), level, s.gridMask);
         s.next[i] = atomicExch(&s.head[h], i);
@@ -1126,6 +1128,9 @@ struct owe_hip_renderer {
     uint32_t particleListCapacity = 0;
     // SPPM: the device state, photons per iteration and iterations so far.
     owehip::SppmDev sppm{};
+    float4* vpBuf[2][4] = {};           // two visible-point buffers: P, N, D, beta
+    hipStream_t sppmStream = nullptr;   // the camera pass's stream
+    hipEvent_t camDone = nullptr, photonsDone[2] = {nullptr, nullptr};
     uint64_t photons = 0;
     uint64_t sppmIterations = 0;
     owe_transport_stats stats{};
@@ -1432,11 +1437,11 @@ template <int kCap>
 void launchSppmCamera(owe_hip_renderer* r) {
     const dim3 grid{(r->sppm.pixels + kBlock - 1) / kBlock}, block{kBlock};
     if (r->dev.contacts)
-        hipLaunchKernelGGL((sppmCameraKernel<kCap, true>), grid, block, 0, r->stream, r->dev, r->set, r->sppm, r->film,
-                           r->persistCounters, r->simpleShadows ? 1 : 0);
+        hipLaunchKernelGGL((sppmCameraKernel<kCap, true>), grid, block, 0, r->sppmStream, r->dev, r->set, r->sppm,
+                           r->film, r->persistCounters, r->simpleShadows ? 1 : 0);
     else
-        hipLaunchKernelGGL((sppmCameraKernel<kCap, false>), grid, block, 0, r->stream, r->dev, r->set, r->sppm, r->film,
-                           r->persistCounters, r->simpleShadows ? 1 : 0);
+        hipLaunchKernelGGL((sppmCameraKernel<kCap, false>), grid, block, 0, r->sppmStream, r->dev, r->set, r->sppm,
+                           r->film, r->persistCounters, r->simpleShadows ? 1 : 0);
     OWE_HIP_CHECK(hipGetLastError());
 }
 
@@ -1450,10 +1455,20 @@ void runSppm(owe_hip_renderer* r, int iterations) {
     for (int k = 0; k < iterations; ++k) {
         q.iteration = uint32_t(r->sppmIterations);
         q.rotation = float(sppmRotation(r->set.seed, r->sppmIterations));
-        hipLaunchKernelGGL(sppmClearGrid, gridGrid, block, 0, r->stream, q);
+        const int b = int(r->sppmIterations & 1u);
+        q.vpP = r->vpBuf[b][0];
+        q.vpN = r->vpBuf[b][1];
+        q.vpD = r->vpBuf[b][2];
+        q.vpBeta = r->vpBuf[b][3];
+        // The camera pass on its own stream, once the photons that last read buffer b are done: it
+        // overlaps the previous iteration's photons and update.
+        OWE_HIP_CHECK(hipStreamWaitEvent(r->sppmStream, r->photonsDone[b], 0));
         if (r->stackCap <= 16) launchSppmCamera<16>(r);
         else if (r->stackCap <= 32) launchSppmCamera<32>(r);
         else launchSppmCamera<48>(r);
+        OWE_HIP_CHECK(hipEventRecord(r->camDone, r->sppmStream));
+        hipLaunchKernelGGL(sppmClearGrid, gridGrid, block, 0, r->stream, q);
+        OWE_HIP_CHECK(hipStreamWaitEvent(r->stream, r->camDone, 0));
         hipLaunchKernelGGL(sppmGridKernel, pixelGrid, block, 0, r->stream, q);
         hipLaunchKernelGGL(sppmGuideKernel, dim3{1}, dim3{kGuideThreads}, 0, r->stream, q);
         if (r->host.emittedPower > 0)
@@ -1462,8 +1477,10 @@ void runSppm(owe_hip_renderer* r, int iterations) {
                                   r->stackCap, r->stream);
         hipLaunchKernelGGL(sppmUpdateKernel, pixelGrid, block, 0, r->stream, q);
         OWE_HIP_CHECK(hipGetLastError());
+        OWE_HIP_CHECK(hipEventRecord(r->photonsDone[b], r->stream));
         r->sppmIterations++;
     }
+    OWE_HIP_CHECK(hipStreamSynchronize(r->sppmStream));
     unsigned long long st[4];
     uint32_t counters[4];
     OWE_HIP_CHECK(hipMemcpyAsync(st, q.stats, sizeof st, hipMemcpyDeviceToHost, r->stream));
@@ -1583,10 +1600,15 @@ int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render
             require(pixels <= (1u << 30), "SPPM keeps one visible point per pixel: at most 2^30 pixels");
             SppmDev& q = r->sppm;
             q.pixels = uint32_t(pixels);
-            q.vpP = devAlloc<float4>(r, pixels);
-            q.vpN = devAlloc<float4>(r, pixels);
-            q.vpD = devAlloc<float4>(r, pixels);
-            q.vpBeta = devAlloc<float4>(r, pixels);
+            for (auto& buf : r->vpBuf)
+                for (float4*& a : buf) a = devAlloc<float4>(r, pixels);
+            // Default priority, like the renderer's own stream.
+            OWE_HIP_CHECK(hipStreamCreateWithFlags(&r->sppmStream, hipStreamNonBlocking));
+            OWE_HIP_CHECK(hipEventCreateWithFlags(&r->camDone, hipEventDisableTiming));
+            for (hipEvent_t& e : r->photonsDone) {
+                OWE_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                OWE_HIP_CHECK(hipEventRecord(e, r->stream));
+            }
             q.radius = devAlloc<float>(r, pixels);
             q.tauXY = devAlloc<double2>(r, pixels);
             q.tauZN = devAlloc<double2>(r, pixels);
@@ -1642,6 +1664,10 @@ int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render
     if (st != OWE_HIP_OK && r) {
         for (void* p : r->allocations) (void)hipFree(p);
         if (r->stream) (void)hipStreamDestroy(r->stream);
+        if (r->sppmStream) (void)hipStreamDestroy(r->sppmStream);
+        if (r->camDone) (void)hipEventDestroy(r->camDone);
+        for (hipEvent_t e : r->photonsDone)
+            if (e) (void)hipEventDestroy(e);
         delete r;
     }
     return st;
@@ -1652,11 +1678,16 @@ int owe_hip_renderer_destroy(owe_hip_renderer* r) {
         if (!r) return;
         (void)hipSetDevice(r->device);
         if (r->stream) (void)hipStreamSynchronize(r->stream);
+        if (r->sppmStream) (void)hipStreamSynchronize(r->sppmStream);
         dropGraph(r);
         for (void* p : r->allocations) OWE_HIP_CHECK(hipFree(p));
         for (auto& row : r->ev)
             for (hipEvent_t& e : row)
                 if (e) OWE_HIP_CHECK(hipEventDestroy(e));
+        if (r->sppmStream) OWE_HIP_CHECK(hipStreamDestroy(r->sppmStream));
+        if (r->camDone) OWE_HIP_CHECK(hipEventDestroy(r->camDone));
+        for (hipEvent_t e : r->photonsDone)
+            if (e) OWE_HIP_CHECK(hipEventDestroy(e));
         if (r->stream) OWE_HIP_CHECK(hipStreamDestroy(r->stream));
         delete r;
     });
