@@ -29,9 +29,9 @@ roofline: the closest-hit traversal kernel, priced at the algorithmic bytes per 
 reference algorithm (binary binned-SAH BVH, leaf 4, ordered traversal, src/owe/scene/bvh.hpp:31-78)
 measured on this very batch by the oracle: 40 B of ray and hit + 32 B per node visited (50.685) +
 36 B per triangle tested (7.060) = 1,916.2 B per query; its average duration comes from HIP events
-around each launch on the launch stream. cpu_baseline: the reference's own MeshShape::intersect
-(oracle/_ref, built from the reference sources) on the host threads over a bounded sample, or the
-oracle restatement when that harness is absent.
+around each launch on the launch stream. cpu_baseline: the oracle's fp64 restatement of
+MeshShape::intersect (its BVH equals the reference's node for node) on the host threads over a
+bounded sample of the batch.
 """
 from __future__ import annotations
 
@@ -89,12 +89,11 @@ def measured_counters(kernel: str):
 def cpu_baseline_rayquery(pos, tri, rays, seconds_budget: float = 15.0):
     sys.path.insert(0, str(ROOT / "oracle"))
     import numpy as np
-    from pyoracle import Oracle, Reference, reference_available
+    from pyoracle import Oracle
 
     threads = min(16, os.cpu_count() or 1)
-    kind = "reference" if reference_available() else "port"
-    lib = Reference() if kind == "reference" else Oracle()
-    mesh = lib.mesh(pos.astype(np.float64), tri)
+    kind = "port"
+    mesh = Oracle().mesh(pos.astype(np.float64), tri)
     n = 65536
     done, elapsed = 0, 0.0
     while elapsed < seconds_budget and done < len(rays):
@@ -106,7 +105,7 @@ def cpu_baseline_rayquery(pos, tri, rays, seconds_budget: float = 15.0):
         n = min(4 * n, 1 << 21)
     return {"value": done / elapsed, "unit": "rays/s", "cores": threads, "kind": kind,
             "sample": f"closest hit of the first {done} rays of the batch on the same mesh ({elapsed:.1f} s, "
-                      f"fp64 MeshShape::intersect, {threads} threads)"}
+                      f"the oracle's fp64 MeshShape::intersect, {threads} threads)"}
 
 
 PRISM_SPP = 64

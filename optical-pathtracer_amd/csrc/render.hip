@@ -1131,6 +1131,7 @@ struct owe_hip_renderer {
     float4* vpBuf[2][4] = {};           // two visible-point buffers: P, N, D, beta
     hipStream_t sppmStream = nullptr;   // the camera pass's stream
     hipEvent_t camDone = nullptr, photonsDone[2] = {nullptr, nullptr};
+    hipEvent_t cleared = nullptr;       // the SPPM call's counters are zero (recorded on `stream`)
     uint64_t photons = 0;
     uint64_t sppmIterations = 0;
     owe_transport_stats stats{};
@@ -1447,10 +1448,15 @@ void launchSppmCamera(owe_hip_renderer* r) {
 
 // Iterations of SPPM (GpuRenderer::passSppm, gpu_renderer_passes.cpp:259-290): per iteration the hash
 // is cleared, then the camera pass, the hash fill, the photons and the update run in stream order.
+// The call's statistics and counters are cleared on the render stream, and the camera stream waits
+// for that clear before its first pass (its other waits are on events of earlier passes, recorded
+// before the clear), so no camera-pass count lands in a counter that is then zeroed.
 void runSppm(owe_hip_renderer* r, int iterations) {
     SppmDev& q = r->sppm;
     OWE_HIP_CHECK(hipMemsetAsync(q.stats, 0, 4 * sizeof(unsigned long long), r->stream));
     OWE_HIP_CHECK(hipMemsetAsync(r->persistCounters, 0, 4 * sizeof(uint32_t), r->stream));
+    OWE_HIP_CHECK(hipEventRecord(r->cleared, r->stream));
+    OWE_HIP_CHECK(hipStreamWaitEvent(r->sppmStream, r->cleared, 0));
     const dim3 block{256}, pixelGrid{(q.pixels + 255) / 256}, gridGrid{(q.gridMask + 256) / 256};
     for (int k = 0; k < iterations; ++k) {
         q.iteration = uint32_t(r->sppmIterations);
@@ -1605,6 +1611,7 @@ int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render
             // Default priority, like the renderer's own stream.
             OWE_HIP_CHECK(hipStreamCreateWithFlags(&r->sppmStream, hipStreamNonBlocking));
             OWE_HIP_CHECK(hipEventCreateWithFlags(&r->camDone, hipEventDisableTiming));
+            OWE_HIP_CHECK(hipEventCreateWithFlags(&r->cleared, hipEventDisableTiming));
             for (hipEvent_t& e : r->photonsDone) {
                 OWE_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
                 OWE_HIP_CHECK(hipEventRecord(e, r->stream));
@@ -1666,6 +1673,7 @@ int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render
         if (r->stream) (void)hipStreamDestroy(r->stream);
         if (r->sppmStream) (void)hipStreamDestroy(r->sppmStream);
         if (r->camDone) (void)hipEventDestroy(r->camDone);
+        if (r->cleared) (void)hipEventDestroy(r->cleared);
         for (hipEvent_t e : r->photonsDone)
             if (e) (void)hipEventDestroy(e);
         delete r;
@@ -1686,6 +1694,7 @@ int owe_hip_renderer_destroy(owe_hip_renderer* r) {
                 if (e) OWE_HIP_CHECK(hipEventDestroy(e));
         if (r->sppmStream) OWE_HIP_CHECK(hipStreamDestroy(r->sppmStream));
         if (r->camDone) OWE_HIP_CHECK(hipEventDestroy(r->camDone));
+        if (r->cleared) OWE_HIP_CHECK(hipEventDestroy(r->cleared));
         for (hipEvent_t e : r->photonsDone)
             if (e) OWE_HIP_CHECK(hipEventDestroy(e));
         if (r->stream) OWE_HIP_CHECK(hipStreamDestroy(r->stream));

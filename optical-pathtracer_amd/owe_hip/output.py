@@ -5,9 +5,10 @@ captures.
   — a colour PFM ("PF", width, height, scale −1 for little-endian), rows stored bottom to top, each
   pixel the linear sRGB (D65) of the XYZ estimate in float32; reading inverts that matrix exactly and
   honours a big-endian file (positive scale).
-* ``read_pfm_raw`` / ``benchmark_errors`` / ``glint_mask``: the benchmark runner's reading of a PFM
-  (tools/run_benchmarks.py:39-73) — raw linear values times |scale|, in file row order — and its
-  full-image relMSE, mean ratio and glint-masked relMSE.
+* ``read_pfm_raw``: the benchmark runner's reading of a PFM (tools/run_benchmarks.py:39-48) — raw
+  linear values times |scale|, in file row order;
+* ``benchmark_errors`` / ``glint_mask``: the benchmark metric as docs/BENCHMARKS.md:94-101 defines
+  it — full-image relMSE, mean ratio, and the relMSE outside the sun-glint mask.
 * ``render_record`` / ``write_render_files`` / ``read_display_settings``: the JSON render record
   (src/owe/render/record.cpp:23-141), with the same keys, and the reader of its display block
   (record.cpp:118-139).
@@ -92,35 +93,58 @@ def read_pfm_raw(path: str) -> np.ndarray:
     return rgb * abs(scale)
 
 
+GLINT_WINDOW = 9          # the median window (pixels) and the mask's dilation (its half, 4)
+GLINT_OVER_MEDIAN = 20.0  # a glint exceeds its window median by this factor ...
+GLINT_OVER_MEAN = 10.0    # ... and the image mean by this one
+GLINT_ENERGY_CAP = 0.5    # a mask holding this share of the light or more is not a glint mask
+
+
 def glint_mask(ref: np.ndarray) -> np.ndarray:
-    """Pixels within 4 of a sun glint: a pixel whose channel maximum exceeds 20× its 9×9 median and 10×
-    the image mean seeds the mask, which is dilated by 4 pixels; a mask that would hold half the
-    image's energy or more is dropped (tools/run_benchmarks.py:51-57)."""
-    from numpy.lib.stride_tricks import sliding_window_view
-    m = ref.max(axis=2)
-    median = np.median(sliding_window_view(np.pad(m, 4, mode="edge"), (9, 9)), axis=(2, 3))
-    seeds = (m > 20 * median) & (m > 10 * m.mean())
-    mask = sliding_window_view(np.pad(seeds, 4), (9, 9)).any(axis=(2, 3))
-    return mask if m[mask].sum() < 0.5 * m.sum() else np.zeros_like(mask)
+    """The benchmark metric's diagnostic glint mask (docs/BENCHMARKS.md:95-99), over the reference
+    image's per-pixel channel maximum m: a pixel is a glint when m exceeds 20× the median of m over
+    the 9×9 window centred on it (edges extended by their nearest value) and 10× the mean of m; the
+    mask is every pixel within 4 (Chebyshev distance) of a glint. Dropped — all false — when the
+    masked pixels carry half or more of the image's Σm."""
+    from scipy import ndimage
+
+    peak = np.asarray(ref, np.float64).max(axis=2)
+    local = ndimage.median_filter(peak, size=GLINT_WINDOW, mode="nearest")
+    glints = (peak > GLINT_OVER_MEDIAN * local) & (peak > GLINT_OVER_MEAN * peak.mean())
+    near = ndimage.maximum_filter(glints.astype(np.uint8), size=GLINT_WINDOW, mode="constant", cval=0) > 0
+    if peak[near].sum() >= GLINT_ENERGY_CAP * peak.sum():
+        return np.zeros(peak.shape, dtype=bool)
+    return near
 
 
 def benchmark_errors(img: np.ndarray, ref: np.ndarray, mask: np.ndarray | None = None) -> dict:
-    """Full-image relMSE (mean over pixels and channels of (a − r)² / (r² + 0.01 · mean(r)²)), the same
-    over the pixels outside ``mask``, and the mean ratio (tools/run_benchmarks.py:60-73). A dark
-    reference gives 0 for an equally dark image and None where the image is not."""
-    a, ref = np.asarray(img, np.float64), np.asarray(ref, np.float64)
-    mask = np.zeros(ref.shape[:2], bool) if mask is None else np.asarray(mask, bool)
-    if a.shape != ref.shape or mask.shape != ref.shape[:2]:
-        raise ValueError("render/reference dimensions differ")
-    if not np.isfinite(a).all() or not np.isfinite(ref).all():
-        raise ValueError("nonfinite raw image")
-    mean_ref = ref.mean()
-    denom = ref * ref + 0.01 * mean_ref * mean_ref
-    rel = np.divide((a - ref) ** 2, denom, out=np.zeros_like(a), where=denom > 0)
-    if np.any((denom == 0) & (a != ref)):
+    """The benchmark metric (docs/BENCHMARKS.md:94-101) of a raw image against a reference:
+
+    * ``rel_mse``: the mean over every pixel and channel of (a − r)² / (r² + 0.01 · r̄²), r̄ the
+      reference's mean over pixels and channels;
+    * ``rel_mse_no_glints``: the same mean over the pixels outside ``mask`` (None if none are);
+    * ``mean_ratio``: ā / r̄ (None for a black reference).
+
+    Where a term's denominator is zero (r = 0 and r̄ = 0: a black reference) it counts as 0 when
+    a = r too, and makes the relative error unavailable (every value None) when it does not."""
+    a = np.asarray(img, dtype=np.float64)
+    r = np.asarray(ref, dtype=np.float64)
+    keep = np.ones(r.shape[:2], bool) if mask is None else ~np.asarray(mask, dtype=bool)
+    if a.shape != r.shape or r.ndim != 3 or keep.shape != r.shape[:2]:
+        raise ValueError(f"image {a.shape} and reference {r.shape} (mask {keep.shape}) dimensions differ")
+    if not (np.isfinite(a).all() and np.isfinite(r).all()):
+        raise ValueError("nonfinite value in the image or the reference")
+    r_bar = r.mean()
+    weight = r * r + 0.01 * r_bar * r_bar
+    err2 = (a - r) ** 2
+    dark = weight == 0
+    if (dark & (err2 > 0)).any():
         return {"rel_mse": None, "rel_mse_no_glints": None, "mean_ratio": None}
-    return {"rel_mse": float(rel.mean()), "rel_mse_no_glints": float(rel[~mask].mean()) if (~mask).any() else None,
-            "mean_ratio": float(a.mean() / mean_ref) if mean_ref else None}
+    terms = np.zeros_like(err2)
+    np.divide(err2, weight, out=terms, where=~dark)
+    outside = terms[keep]
+    return {"rel_mse": float(terms.mean()),
+            "rel_mse_no_glints": float(outside.mean()) if outside.size else None,
+            "mean_ratio": float(a.mean() / r_bar) if r_bar != 0 else None}
 
 
 # ---------------------------------------------------------------- hashes

@@ -540,6 +540,82 @@ def white_furnace_scene(rho: float, size: int = 16) -> Scene:
     return sc
 
 
+def _as_spectrum(v) -> Spectrum:
+    return v if isinstance(v, Spectrum) else Spectrum.constant(float(v))
+
+
+def scattering_furnace_scene(sigma_s, sigma_a=0.0, g: float = 0.0, rho: float = 0.0, size: int = 16,
+                             at_centre: bool = False, bubble: float = 0.0) -> Scene:
+    """A unit sphere filled with a participating medium (scattering sigma_s and absorption sigma_a
+    per metre, constants or spectra; Henyey–Greenstein asymmetry g), its inner wall diffuse of albedo
+    rho and emitting unit radiance inward; the eye is inside, at the centre if `at_centre`.
+
+    With sigma_a = 0 the radiance everywhere inside is 1 / (1 − ρ) whatever sigma_s(λ) and g are: a
+    conservative medium leaves a uniform radiance field uniform, and the wall then closes the
+    furnace balance L = 1 + ρL. With absorption the eye at the centre sees, in every direction,
+    E[ϖ^N] (ρ = 0), N the collisions of a walk from the centre to the wall and ϖ = σs/(σs + σa).
+
+    `bubble` > 0 puts the eye at the centre of a clear sphere of that radius behind a null boundary
+    (light tracing then never connects a medium vertex closer than the bubble to the eye: a vertex
+    at distance r contributes ∝ 1/r², whose second moment diverges as r → 0)."""
+    sc = Scene()
+    air = sc.add_medium(Medium.air())
+    fog = Medium.constant(1.0)
+    fog.scattering, fog.absorption, fog.g = _as_spectrum(sigma_s), _as_spectrum(sigma_a), float(g)
+    amb = sc.add_region(air)
+    inside = sc.add_region(sc.add_medium(fog))
+    shell = sc.add_optics(Optics(SURF_DIFFUSE, reflectance=Spectrum.constant(rho)))
+    e = sc.add_emission(Emission(Spectrum.constant(1.0), front=False, back=True))
+    sc.add_sphere(1.0, transform(), amb, inside, shell, e)
+    eye = inside
+    if bubble > 0:
+        eye = sc.add_region(sc.add_medium(Medium.constant(1.0)))
+        sc.add_sphere(bubble, transform(), inside, eye, sc.add_optics(Optics(SURF_NULL)))
+    pos = (0.0, 0.0, 0.0) if at_centre or bubble > 0 else (0.2, 0.1, -0.3)
+    sc.set_observer(size, size, position=pos, look_at=(1, 0.3, 0), fov_y=math.radians(70), region=eye)
+    return sc
+
+
+def rough_glass_scene(alpha: float = 0.2, size: int = 32, sun: bool = True, sky: float = 0.2,
+                      fog: float = 0.0, dome: bool = False) -> Scene:
+    """Frosted glass in sunlight: a rough N-BK7 ball (radius 50 mm, GGX α = `alpha`) and a rough
+    plate (100 × 80 × 10 mm, the same α) standing on a diffuse floor of albedo 0.7 under a 0.53° sun
+    40° high and a uniform sky; the eye looks at both. Every rough-dielectric vertex takes next-event
+    estimation toward the sun (reflection lobe) and samples reflection or transmission through the
+    GGX visible normals; light reaches the floor behind the plate only through it. `fog` > 0 fills
+    the ball with a scattering medium (σs = fog per metre, g = 0.6). `dome` closes the scene in a
+    black sphere of radius 2 m emitting unit radiance inward, in place of sun and sky: every light
+    path then starts on a real surface, so light tracing sees everything camera paths see."""
+    sc = Scene()
+    air = sc.add_medium(Medium.constant(1.000277))
+    glass = Medium.sellmeier(**N_BK7)
+    amb = sc.add_region(air)
+    inside_plate = sc.add_region(sc.add_medium(glass))
+    ball_glass = Medium.sellmeier(**N_BK7)
+    if fog > 0:
+        ball_glass.scattering, ball_glass.g = Spectrum.constant(fog), 0.6
+    inside_ball = sc.add_region(sc.add_medium(ball_glass))
+    frosted = sc.add_optics(Optics(SURF_DIELECTRIC, roughness=alpha))
+    floor = sc.add_optics(Optics(SURF_DIFFUSE, reflectance=Spectrum.constant(0.7)))
+    sc.add_plane(APERTURE_RECT, 0.5, 0.5, transform(), amb, amb, floor)
+    sc.add_sphere(0.05, transform(None, (-0.06, 0.0, 0.0502)), amb, inside_ball, frosted)
+    pos, tri = box_mesh((-0.05, -0.005, 0.0), (0.05, 0.005, 0.08))
+    sc.add_mesh(pos, tri, transform(rotation((0, 0, 1), math.radians(15.0)), (0.07, 0.02, 0.0002)), amb,
+                inside_plate, frosted)
+    if dome:
+        black = sc.add_optics(Optics(SURF_DIFFUSE, reflectance=Spectrum.constant(0.0)))
+        glow = sc.add_emission(Emission(Spectrum.constant(1.0), front=False, back=True))
+        sc.add_sphere(2.0, transform(), sc.add_region(air), amb, black, glow)
+    else:
+        if sun:
+            sc.set_sun(direction_from_azimuth_elevation(math.radians(200.0), math.radians(40.0)),
+                       math.radians(0.53) / 2, Spectrum.blackbody(5778.0, 1e3))
+        if sky > 0:
+            sc.set_uniform_sky(Spectrum.constant(sky))
+    sc.set_observer(size, size, position=(0.0, -0.35, 0.2), look_at=(0.0, 0.0, 0.04), fov_y=math.radians(40))
+    return sc
+
+
 def dielectric_sphere_scene(size: int = 24) -> Scene:
     """A lossless N-SF11-like sphere in a uniform sky is invisible (test_transport.cpp:116-142)."""
     sc = Scene()

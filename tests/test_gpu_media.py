@@ -1,0 +1,153 @@
+"""Participating media on the GPU: scattering (σs, Henyey–Greenstein g) and absorption in the camera,
+particle and SPPM kernels, against closed forms, the brute-force walk and the fp64 oracle.
+
+* the scattering furnace 1 / (1 − ρ) for every g, by camera paths, light tracing and SPPM
+  (tests/test_backends.cpp:540-574 states the furnace for every integrator);
+* absorbing fog seen from the centre of the sphere: the radiance the brute-force walk of
+  tests/media_walk.py finds (E[ϖ^N] with wall bounces), and the stepped fog whose hero-4 secondaries
+  cross a step of σs and σa at 580 nm;
+* per path and per particle, from the same streams, the float kernels and the fp64 oracle agree
+  except where a float decision (a roulette draw, scatter-or-cross near the boundary) falls the
+  other way.
+
+Reference: src/owe/transport/transport.cpp:95-113 (sampleMedium), 379-397 (scatterInMedium);
+core/shared/warps.hpp:145-160 (hgPhase, sampleHg); shaders/particle.slang:356-427.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import media_walk
+from owe_hip import render
+from owe_hip import scene as S
+from pyoracle import World
+from test_oracle_media import DEEP, stepped_fog_expectation, stepped_fog_scene
+
+pytestmark = pytest.mark.gpu
+
+
+def _clean(st) -> bool:
+    """No leaks, and crossings met from the wrong side at most once in 10^7 segments: a medium
+    scatter point drawn within float rounding of a boundary may sit on its far side (the fp64 oracle
+    has the same rule at a rate too small to see)."""
+    return st["leaks"] == 0 and st["inconsistencies"] <= 1e-7 * st["segments"]
+
+
+def _mean_y(sc, st, spp, passes=1):
+    r = render.Renderer(sc, st)
+    for _ in range(passes):
+        r.run_pass(spp)
+    return r.resolve()[..., 1].mean(), r.stats()
+
+
+@pytest.mark.parametrize("g", [-0.5, 0.0, 0.8])
+@pytest.mark.parametrize("rho", [0.0, 0.5])
+def test_scattering_furnace_camera_paths(cuda, g, rho):
+    y, st = _mean_y(S.scattering_furnace_scene(2.0, 0.0, g, rho), S.settings(seed=1, **DEEP), 1024)
+    print(f"fog furnace g={g} rho={rho}: {y:.5f} (expect {1 / (1 - rho):.5f}), {st}", flush=True)
+    assert y == pytest.approx(1 / (1 - rho), rel=0.005)
+    assert st["segments"] > 2 * st["paths"]
+    assert _clean(st)
+
+
+@pytest.mark.parametrize("g", [-0.5, 0.0, 0.8])
+def test_scattering_furnace_light_tracing(cuda, g):
+    sc = S.scattering_furnace_scene(3.0, 0.0, g, 0.5, size=8, bubble=0.3)
+    y, st = _mean_y(sc, S.settings(seed=1, integrator=S.INTEGRATOR_LIGHT, **DEEP), 65536)
+    print(f"fog furnace by particles g={g}: {y:.5f} (expect 2), {st}", flush=True)
+    assert y == pytest.approx(2.0, rel=0.015)
+    assert _clean(st)
+
+
+@pytest.mark.parametrize("rho", [0.0, 0.5])
+def test_scattering_furnace_sppm(cuda, rho):
+    sc = S.scattering_furnace_scene(3.0, 0.0, 0.8, rho, size=8, bubble=0.3)
+    y, st = _mean_y(sc, S.settings(seed=1, integrator=S.INTEGRATOR_SPPM, **DEEP), 2048)
+    print(f"fog furnace by sppm rho={rho}: {y:.5f} (expect {1 / (1 - rho):.5f}), {st}", flush=True)
+    assert y == pytest.approx(1 / (1 - rho), rel=0.02)
+    assert _clean(st)
+
+
+@pytest.mark.parametrize("sigma_s,sigma_a,g,rho", [(3.0, 1.0, 0.0, 0.0), (1.0, 2.0, 0.8, 0.3), (3.0, 1.0, -0.4, 0.5)])
+def test_absorbing_fog_matches_the_brute_force_walk(cuda, sigma_s, sigma_a, g, rho):
+    want, err = media_walk.sphere_radiance(sigma_s, sigma_a, g, rho, walks=400000, seed=7)
+    sc = S.scattering_furnace_scene(sigma_s, sigma_a, g, rho, at_centre=True)
+    y, st = _mean_y(sc, S.settings(seed=2, **DEEP), 4096)
+    print(f"absorbing fog {sigma_s, sigma_a, g, rho}: gpu {y:.5f}, walk {want:.5f} ± {err:.5f}", flush=True)
+    assert y == pytest.approx(want, rel=0.01)
+
+
+def test_hero_wavelengths_share_paths_through_a_stepped_fog(cuda):
+    want = stepped_fog_expectation(360.0, 830.0)
+    y, _ = _mean_y(stepped_fog_scene(), S.settings(seed=3, spectral=S.SPECTRAL_HERO4, **DEEP), 4096)
+    print(f"stepped fog hero-4: gpu {y:.5f}, walks {want:.5f}", flush=True)
+    assert y == pytest.approx(want, rel=0.01)
+
+
+def _path_agreement(oracle, sc, st, spp, side):
+    n = side * side * spp
+    pixel = np.repeat(np.arange(side * side, dtype=np.uint32), spp)
+    sample = np.tile(np.arange(spp, dtype=np.uint32), side * side)
+    gpu = render.Renderer(sc, st).camera_paths(spp, pixel, sample)
+    ora, segs = World(oracle, sc).camera_paths(st, spp, pixel, sample)
+    gy, oy = gpu[:, 1], ora[:, 1]
+    close = np.abs(gy - oy) <= 1e-3 * np.maximum(np.abs(oy), np.abs(gy)) + 1e-6 * np.abs(oy).max()
+    return close.mean(), gy, oy, segs, n
+
+
+@pytest.mark.parametrize("case", ["stepped-fog-hero4", "fog-furnace-g0.8", "fog-bubble"])
+def test_fog_paths_match_the_oracle_path_by_path(cuda, oracle, case):
+    sc, st = {
+        "stepped-fog-hero4": (stepped_fog_scene(size=32), S.settings(seed=5, **DEEP)),
+        "fog-furnace-g0.8": (S.scattering_furnace_scene(2.0, 0.5, 0.8, 0.5, size=32), S.settings(seed=6, **DEEP)),
+        "fog-bubble": (S.scattering_furnace_scene(3.0, 0.2, -0.3, 0.3, size=32, bubble=0.3),
+                       S.settings(seed=7, spectral=S.SPECTRAL_UNIFORM1, **DEEP)),
+    }[case]
+    frac, gy, oy, segs, n = _path_agreement(oracle, sc, st, 16, 32)
+    print(f"{case}: {frac:.5f} of {n} paths within 1e-3; mean Y gpu {gy.mean():.5f} oracle {oy.mean():.5f}; "
+          f"mean segments {segs.mean():.2f}", flush=True)
+    assert segs.mean() > 3
+    assert frac > 0.99
+    assert gy.mean() == pytest.approx(oy.mean(), rel=0.02)
+
+
+@pytest.mark.parametrize("case", ["fog-bubble-g0.8", "stepped-fog-hero4"])
+def test_fog_particles_match_the_oracle_particle_by_particle(cuda, oracle, case):
+    sc, st = {
+        "fog-bubble-g0.8": (S.scattering_furnace_scene(3.0, 0.3, 0.8, 0.5, size=8, bubble=0.3),
+                            S.settings(seed=8, integrator=S.INTEGRATOR_LIGHT, **DEEP)),
+        "stepped-fog-hero4": (stepped_fog_scene(size=8), S.settings(seed=9, integrator=S.INTEGRATOR_LIGHT, **DEEP)),
+    }[case]
+    idx = np.arange(1 << 16, dtype=np.uint64)
+    gpu = render.Renderer(sc, st).particles(idx)
+    ora, segs = World(oracle, sc).particles(st, idx)
+    gy, oy = gpu[:, 1], ora[:, 1]
+    live = (gy != 0) | (oy != 0)
+    close = np.abs(gy - oy) <= 1e-3 * np.maximum(np.abs(oy), np.abs(gy))
+    frac = close[live].mean()
+    tot = abs(gy.sum() - oy.sum()) / abs(oy.sum())
+    print(f"{case}: {live.sum()} contributing particles, {frac:.5f} agree to 1e-3, totals differ by {tot:.2e}, "
+          f"mean segments {segs.mean():.2f}", flush=True)
+    assert live.sum() > 1000 and segs.mean() > 3
+    assert frac >= 0.98
+    assert tot < 2e-2
+
+
+def test_fog_sppm_first_iteration_matches_the_oracle(cuda, oracle):
+    sc = S.scattering_furnace_scene(1.0, 0.2, 0.6, 0.5, size=32, bubble=0.3)
+    st = S.settings(seed=10, integrator=S.INTEGRATOR_SPPM, photons_per_pixel=32.0, **DEEP)
+    r = render.Renderer(sc, st)
+    r.run_pass(1)
+    g = r.sppm_state()
+    o = World(oracle, sc).sppm(st, 1)
+    seen = (o["r"] > 0) & (g["r"] > 0)
+    rclose = (np.abs(g["r"] - o["r"]) <= 1e-4 * o["r"])[seen].mean()
+    nclose = (np.abs(g["N"] - o["N"]) <= 1e-6 * np.maximum(o["N"], 1.0)).mean()
+    tclose = (np.abs(g["tau"][..., 1] - o["tau"][..., 1]) <= 1e-3 * np.maximum(o["tau"][..., 1], 1e-30) + 1e-12).mean()
+    print(f"fog sppm: {seen.mean():.3f} seen; radius agrees on {rclose:.4f}, N on {nclose:.4f}, tau on {tclose:.4f}; "
+          f"mean N gpu {g['N'].mean():.3f} oracle {o['N'].mean():.3f}", flush=True)
+    assert seen.mean() > 0.3 and o["N"].mean() > 0.5  # points behind a medium scatter are the photons' own
+    assert rclose > 0.99 and nclose > 0.9 and tclose > 0.9
+    s = r.stats()
+    assert s["paths"] == o["stats"][0] and _clean(s)

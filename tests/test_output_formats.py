@@ -110,6 +110,31 @@ def test_runner_errors_match_its_tests():
         O.benchmark_errors(np.ones((1, 1, 3)), np.ones((2, 2, 3)))
 
 
+def _glint_mask_by_loops(ref):
+    """The glint rule of docs/BENCHMARKS.md:97-99, pixel by pixel (small images only)."""
+    m = ref.max(axis=2)
+    h, w = m.shape
+    glint = np.zeros((h, w), bool)
+    for y in range(h):
+        for x in range(w):
+            win = [m[min(max(y + dy, 0), h - 1), min(max(x + dx, 0), w - 1)] for dy in range(-4, 5) for dx in range(-4, 5)]
+            glint[y, x] = m[y, x] > 20 * sorted(win)[40] and m[y, x] > 10 * m.mean()
+    mask = np.zeros((h, w), bool)
+    for y, x in zip(*np.nonzero(glint)):
+        mask[max(y - 4, 0):y + 5, max(x - 4, 0):x + 5] = True
+    return mask if m[mask].sum() < 0.5 * m.sum() else np.zeros_like(mask)
+
+
+def test_glint_mask_matches_the_rule_pixel_by_pixel():
+    rng = np.random.default_rng(3)
+    for trial in range(6):
+        img = rng.gamma(2.0, 1.0, size=(23, 31, 3))
+        for _ in range(trial):
+            img[rng.integers(0, 23), rng.integers(0, 31)] = rng.uniform(100, 400, 3)
+        img[0, 0] = 300.0 if trial % 2 else img[0, 0]  # a glint at the corner: edge extension
+        assert np.array_equal(O.glint_mask(img), _glint_mask_by_loops(img)), trial
+
+
 def test_glint_mask():
     img = np.full((40, 40, 3), 1.0)
     img[10, 20] = 500.0
