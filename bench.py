@@ -3,16 +3,21 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload prism|rayquery|light|lens|caustic|telescope]
 
 Workload ``prism`` (the default; plain spectral path tracing of a dispersion prism): an equilateral
-Sellmeier-glass prism (40 mm sides, 20 mm tall) on a 1 m² Lambertian floor of albedo 0.8 in air
-n = 1.000277, a 5778 K blackbody sun of 0.53°; a 256×256 pinhole camera, 64 spp, one wavelength per
-path uniform over 380–780 nm, max depth 16, Russian roulette from depth 4. One step = one pass of
-64 spp = 4,194,304 camera paths. Both forms run — the wavefront kernels and the persistent batched
-kernel replayed from a captured hipGraph, each autotuned first — and ``value`` is the faster form's
-spectral paths per second (the films of the two forms are compared bit for bit). roofline: the
-kernel with the largest share of the pass time, priced at its algorithmic bytes per live path (path
-state 64 B read and written, hit record 16 B, shadow record 32 B). cpu_baseline: the oracle's fp64
-restatement of Tracer::radiance (the reference's Tracer needs stb_image and cannot be built here) on
-the host threads, full frames of the same scene and spp.
+Sellmeier-glass prism (40 mm sides, 20 mm tall) standing on a 1 m² Lambertian floor of albedo 0.8
+(its base face in the floor's plane) in air n = 1.000277, a 5778 K blackbody sun of 0.53°; a 256×256
+pinhole camera, 64 spp, one wavelength per path uniform over 380–780 nm, max depth 16, Russian
+roulette from depth 4. One step = one pass of 64 spp = 4,194,304 camera paths. Both forms run — the
+wavefront kernels and the persistent batched kernel replayed from a captured hipGraph, each
+autotuned first — and ``value`` is the faster form's spectral paths per second (the films of the two
+forms are compared bit for bit). Beside it: the same scene with the prism lifted 0.2 mm (no
+contact), and the quality of the images against the fp64 oracle's 16,384-spp image
+(tests/golden/prism_reference.npz): rel. RMSE as the reference's benchmark runner scores it, for the
+timed image and for fresh images of 64 to 4,096 spp. roofline: the persistent kernel is bound by
+VALU issue and lane divergence, so it is priced in useful lane-instructions per second (profiled
+VALU instructions per launch × 64 × active-lane fraction, over the live launch time) against the
+chip's VALU issue peak; the SoA byte model and the measured HBM bytes sit beside it. cpu_baseline:
+the oracle's fp64 restatement of Tracer::radiance (the reference's Tracer needs stb_image and cannot
+be built here) on the host threads, full frames of the same scene and spp.
 
 Beside it: ``rayquery`` (below), ``light`` (light tracing of the same prism scene, 256×256×64
 particles per pass), ``lens`` (256 seeded achromats × 7 wavelengths in one persistent launch) and
@@ -162,17 +167,57 @@ def _time_passes(r, steps, warmup, events_in_timed: bool):
     return wall, stats, times, img
 
 
-def bench_prism(args):
-    from owe_hip import render
-    from owe_hip import scene as S
+PRISM_REFERENCE = ROOT / "tests" / "golden" / "prism_reference.npz"
+# gfx950 VALU issue peak: 256 CUs x 4 SIMDs x 16 lanes per clock at 2.4 GHz, one lane-instruction
+# per lane per clock (packed FP32 is disabled in every build here).
+VALU_PEAK_TLANE = 256 * 4 * 16 * 2.4e9 / 1e12
 
-    log("prism: building scene")
-    sc = S.dispersion_prism_scene(PRISM_SIZE, PRISM_SIZE)
-    st = S.settings(seed=1, max_depth=16, rr_depth=4, spectral=S.SPECTRAL_UNIFORM1, lambda_min=380.0, lambda_max=780.0)
+
+def prism_quality(sc, st, timed_img, timed_spp):
+    """Relative error of this backend's prism images against the fp64 oracle's 16,384-spp image
+    (tests/golden/prism_reference.npz), as the reference's benchmark runner scores a render:
+    relMSE over pixels and linear-sRGB channels (docs/BENCHMARKS.md:94-101), with and without the
+    sun-glint mask; rel. RMSE = sqrt(relMSE). Images of 64·4^k spp come from fresh renderers on
+    seeds no other measurement uses."""
+    import numpy as np
+
+    from owe_hip import output as O
+    from owe_hip import render
+
+    if not PRISM_REFERENCE.exists():
+        return None
+    ref = np.load(PRISM_REFERENCE)
+    rgb = lambda xyz: np.asarray(xyz, np.float64) @ O.XYZ_TO_LINEAR_SRGB.T  # noqa: E731
+    ref_rgb = rgb(ref["xyz"])
+    mask = O.glint_mask(ref_rgb)
+
+    def score(img):
+        e = O.benchmark_errors(rgb(img), ref_rgb, mask)
+        full, masked = e["rel_mse"], e["rel_mse_no_glints"]
+        return {"rel_rmse": None if full is None else full ** 0.5,
+                "rel_rmse_no_glints": None if masked is None else masked ** 0.5, "mean_ratio": e["mean_ratio"]}
+
+    ladder = {}
+    for k, spp in enumerate((64, 256, 1024, 4096)):
+        st.seed = 500 + k
+        r = render.Renderer(sc, st)
+        for _ in range(spp // PRISM_SPP):
+            r.run_pass(PRISM_SPP)
+        ladder[str(spp)] = score(r.resolve())
+        r.close()
+    reach = [int(k) for k, v in ladder.items() if v["rel_rmse_no_glints"] is not None and v["rel_rmse_no_glints"] <= 0.02]
+    return {"reference": f"fp64 oracle, {int(ref['spp'])} spp, seed {int(ref['seed'])}",
+            "glint_pixels": int(mask.sum()), "timed_image": dict(spp=timed_spp, **score(timed_img)),
+            "by_spp": ladder, "spp_for_2pct_no_glints": min(reach) if reach else None}
+
+
+def _prism_forms(sc, st, args, forms_wanted=("wavefront", "persistent")):
+    from owe_hip import render
+
     forms = {}
     # The plain wavefront form, then the persistent batched form with its autotuned launch shape
-    # and graph replay; the two images are equal bit for bit (checked here on the timed passes).
-    for form in ("wavefront", "persistent"):
+    # and graph replay; the two images are equal bit for bit (checked on the timed passes).
+    for form in forms_wanted:
         r = render.Renderer(sc, st)
         if form == "persistent":
             r.set_mode(render.Renderer.PERSISTENT, use_graph=True)
@@ -181,9 +226,21 @@ def bench_prism(args):
         forms[form] = dict(wall=wall, stats=stats, times=times, mode=r.mode(), tuned_ms=tuned_ms, img=img)
         paths = stats["paths"]
         log(f"prism {form}: {args.steps} passes in {wall:.3f}s; {paths / wall / 1e6:.1f} M paths/s; "
-            f"E_seg {stats['segments'] / max(1, paths):.3f}; shape {r.mode()}; "
+            f"E_seg {stats['segments'] / max(1, paths):.3f}; shape {r.mode()}; inconsistencies "
+            f"{stats['inconsistencies']}, leaks {stats['leaks']}; "
             + ", ".join(f"{k} {ms / args.steps:.3f} ms/pass ({n})" for k, (ms, n) in times.items() if n))
         r.close()
+    return forms
+
+
+def bench_prism(args):
+    from owe_hip import scene as S
+
+    log("prism: building scene")
+    # The prism stands on the floor (its base face lies in the floor's plane: contact resolution).
+    sc = S.dispersion_prism_scene(PRISM_SIZE, PRISM_SIZE, lift=0.0)
+    st = S.settings(seed=1, max_depth=16, rr_depth=4, spectral=S.SPECTRAL_UNIFORM1, lambda_min=380.0, lambda_max=780.0)
+    forms = _prism_forms(sc, st, args)
     same = bool((forms["wavefront"]["img"] == forms["persistent"]["img"]).all())
     best = max(forms, key=lambda f: forms[f]["stats"]["paths"] / forms[f]["wall"])
     F = forms[best]
@@ -201,8 +258,44 @@ def bench_prism(args):
         live = stats["segments"] if dominant in ("intersect", "shade", "shadow") else paths
         dom_bytes = KERNEL_BYTES[dominant] * live
     total_ms = sum(ms for ms, _ in times.values())
-    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    launch_ms = dom_ms / max(1, dom_launches)
+    model_GBs = dom_bytes / (dom_ms * 1e-3) / 1e9
+    kernel = f"{dominant}Kernel"
+    cnt = measured_counters(kernel)
+    traffic = measured_traffic(PROFILE_PRISM_TRAFFIC, kernel)
+    # The kernel's bound is its VALU issue (counters: busy VALU, half the lanes active), priced in
+    # useful lane-instructions per second: the profiled VALU instructions per launch x 64 lanes x
+    # the active-lane fraction, over the live launch time.
+    roofline = {"bound": "valu", "kernel": kernel, "unit": "T lane-instructions/s", "peak": VALU_PEAK_TLANE,
+                "achieved": None, "frac": None, "traffic": traffic, "counters": cnt}
+    if cnt and "valu_insts" in cnt and "lane_utilisation" in cnt:
+        lanes = cnt["valu_insts"] * 64 * cnt["lane_utilisation"]
+        roofline["achieved"] = lanes / (launch_ms * 1e-3) / 1e12
+        roofline["frac"] = roofline["achieved"] / VALU_PEAK_TLANE
+        roofline["issue_frac"] = cnt["valu_insts"] * 64 / (launch_ms * 1e-3) / 1e12 / VALU_PEAK_TLANE
+    roofline["hbm_model"] = {"note": "SoA path-record byte model (224 B per segment + 24 B per path); the "
+                                     "persistent kernel keeps path state in registers, so this prices traffic "
+                                     "it does not make", "achieved_GBs": model_GBs, "peak_GBs": HBM_PEAK_GBS,
+                             "frac": model_GBs / HBM_PEAK_GBS,
+                             "algorithmic_bytes_per_launch": dom_bytes / max(1, dom_launches)}
+    if traffic:
+        roofline["hbm_measured"] = {"bytes_per_launch": traffic,
+                                    "achieved_GBs": traffic / (launch_ms * 1e-3) / 1e9,
+                                    "frac": traffic / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    roofline["pass"] = {"kernel_ms": total_ms / args.steps, "launch_ms": launch_ms}
     img = F["img"]
+    quality = None
+    if not getattr(args, "no_quality", False):
+        log("prism: image quality against the fp64 reference")
+        quality = prism_quality(sc, S.settings(seed=1, max_depth=16, rr_depth=4, spectral=S.SPECTRAL_UNIFORM1),
+                                img, args.steps * PRISM_SPP)
+        log(f"prism quality: {quality}")
+    lifted = None
+    if not getattr(args, "no_lifted", False):
+        # The same scene with the prism lifted 0.2 mm off the floor (no contact), persistent form.
+        log("prism: the lifted variant")
+        lifted_sc = S.dispersion_prism_scene(PRISM_SIZE, PRISM_SIZE, lift=0.0002)
+        lifted = _prism_forms(lifted_sc, st, args, ("persistent",))["persistent"]
     result = {
         "metric": "spectral paths/s at fixed rel. RMSE vs fp64 ref; adjoint grads/s; HBM GB/s",
         "value": paths / wall,
@@ -216,8 +309,8 @@ def bench_prism(args):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (procedural dispersion-prism scene, seeded streams)",
-        "config": {"workload": f"plain spectral path tracing, dispersion prism ({best} form)",
-                   "resolution": [PRISM_SIZE, PRISM_SIZE], "spp": PRISM_SPP,
+        "config": {"workload": f"plain spectral path tracing, dispersion prism standing on the floor ({best} form)",
+                   "resolution": [PRISM_SIZE, PRISM_SIZE], "spp": PRISM_SPP, "lift_m": 0.0,
                    "paths_per_step": PRISM_SIZE * PRISM_SIZE * PRISM_SPP, "max_depth": 16, "rr_depth": 4,
                    "spectral": "1 wavelength per path, uniform 380-780 nm", "segments_per_path": eseg,
                    "kernel_ms_per_step": {k: ms / args.steps for k, (ms, n) in times.items() if n},
@@ -228,14 +321,13 @@ def bench_prism(args):
                                  "autotuned_pass_ms": forms[f]["tuned_ms"]} for f in forms},
                    "persistent_over_wavefront": (forms["persistent"]["stats"]["paths"] / forms["persistent"]["wall"]) /
                                                 (forms["wavefront"]["stats"]["paths"] / forms["wavefront"]["wall"]),
-                   "forms_bit_identical": same},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(PROFILE_PRISM_TRAFFIC, f"{dominant}Kernel"),
-                     "counters": measured_counters(f"{dominant}Kernel"),
-                     "kernel": f"{dominant}Kernel", "algorithmic_bytes_per_launch": dom_bytes / max(1, dom_launches),
-                     "pass": {"algorithmic_bytes": pass_bytes / args.steps, "kernel_ms": total_ms / args.steps,
-                              "achieved_GBs": pass_bytes / (total_ms * 1e-3) / 1e9}},
+                   "forms_bit_identical": same,
+                   "quality": quality,
+                   "lifted_prism": None if lifted is None else {
+                       "lift_m": 0.0002, "paths_per_s": lifted["stats"]["paths"] / lifted["wall"],
+                       "ms_per_step": lifted["wall"] * 1e3 / args.steps, "launch": lifted["mode"],
+                       "inconsistencies": lifted["stats"]["inconsistencies"], "leaks": lifted["stats"]["leaks"]}},
+        "roofline": roofline,
     }
     result["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline_prism(sc, st)
     return result
@@ -807,7 +899,15 @@ def main() -> int:
                     help="only copy a 1 GiB device buffer 3 times (the known-size kernel that calibrates "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-quality", action="store_true", help="prism: skip the image-quality ladder")
+    ap.add_argument("--no-lifted", action="store_true", help="prism: skip the lifted-prism variant")
+    ap.add_argument("--solo", action="store_true",
+                    help="the chosen workload alone: no records beside it, no quality ladder, no lifted variant")
     args = ap.parse_args()
+    if args.solo:
+        for k in ("no_rayquery", "no_light", "no_lens", "no_caustic", "no_telescope", "no_scenes", "no_sppm",
+                  "no_quality", "no_lifted", "no_telescope_render"):
+            setattr(args, k, True)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); every workload here runs on a single "
               "MI355X and no multi-GPU figure is produced.", file=sys.stderr)
