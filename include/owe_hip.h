@@ -40,6 +40,9 @@ const char* owe_hip_last_error(void);
 const char* owe_hip_backend_name(void);
 /* One line: hardware, precision and role. */
 const char* owe_hip_backend_summary(void);
+/* The integrators a renderer of this backend implements, comma-separated: "path,light,sppm"
+ * (owe::Backend::integrators, backend.hpp:41-42). */
+const char* owe_hip_backend_integrators(void);
 /* 1 when a gfx950 device is usable here, else 0 with the reason written to why[0..cap). */
 int owe_hip_available(char* why, size_t cap);
 

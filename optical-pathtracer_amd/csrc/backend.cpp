@@ -50,6 +50,8 @@ const char* owe_hip_backend_summary(void) {
            "IEEE-754 double reference";
 }
 
+const char* owe_hip_backend_integrators(void) { return "path,light,sppm"; }
+
 int owe_hip_available(char* why, size_t cap) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {

@@ -82,6 +82,7 @@ def load() -> ctypes.CDLL:
     _declare(lib, "owe_hip_last_error", ctypes.c_char_p, [])
     _declare(lib, "owe_hip_backend_name", ctypes.c_char_p, [])
     _declare(lib, "owe_hip_backend_summary", ctypes.c_char_p, [])
+    _declare(lib, "owe_hip_backend_integrators", ctypes.c_char_p, [])
     _declare(lib, "owe_hip_available", c_int, [ctypes.c_char_p, ctypes.c_size_t])
     _declare(lib, "owe_hip_device_count", c_int, [ctypes.POINTER(c_int)])
     _declare(lib, "owe_hip_device_info_get", c_int, [c_int, ctypes.POINTER(DeviceInfo)])

@@ -1,0 +1,37 @@
+"""The reference-side adapter (optical-pathtracer_amd/adapter/hip_backend.cpp): owe::Backend and
+owe::Renderer over the C ABI, and the flattening of a built owe::Scene into owe_hip_scene.h.
+
+It compiles against the reference engine's own headers (src/owe/render/backend.hpp:31-52,
+src/owe/render/renderer.hpp:19-79, src/owe/scene/*.hpp) and this repository's include/, warnings
+as errors, wherever the reference tree is mounted; and the integrator list it hands to
+owe::Renderer comes from the library."""
+from __future__ import annotations
+
+import shutil
+import subprocess
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+REFERENCE_SRC = Path("/root/reference/src")
+ADAPTER = ROOT / "optical-pathtracer_amd" / "adapter" / "hip_backend.cpp"
+
+
+@pytest.mark.skipif(not (REFERENCE_SRC / "owe" / "render" / "backend.hpp").exists(),
+                    reason="the reference engine's sources are not mounted here")
+def test_adapter_compiles_against_the_reference_headers():
+    gxx = shutil.which("g++")
+    assert gxx
+    cmd = [gxx, "-std=c++20", "-fsyntax-only", "-Wall", "-Wextra", "-Werror", "-isystem", str(REFERENCE_SRC),
+           "-I", str(ROOT / "include"), "-I", str(ADAPTER.parent), str(ADAPTER)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_backend_integrators_from_the_library():
+    from owe_hip import _lib
+
+    lib = _lib.load()
+    assert lib.owe_hip_backend_integrators().decode() == "path,light,sppm"
+    assert lib.owe_hip_backend_name().decode() == "hip"
