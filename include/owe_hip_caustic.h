@@ -57,6 +57,18 @@ int owe_hip_caustic_forward(owe_hip_caustic* caustic, const float* heights, floa
 int owe_hip_caustic_backward(owe_hip_caustic* caustic, const float* heights, const float* grad_irradiance,
                              float* grad_heights, uint64_t pass, void* stream);
 
+/* The largest irradiance grid (pixels², 512 × 512) owe_hip_caustic_forward_noalloc takes. */
+#define OWE_CAUSTIC_NOALLOC_MAX_PIXELS 262144u
+/* owe_hip_caustic_forward without a handle and without allocating: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under graph capture, serving loops,
+ * harnesses that forbid hipMalloc). The same kernels and result, bit for bit, as create + forward
+ * with these settings; the fixed-point bins are a library-held array of 2 MiB per device, kept
+ * until the process ends. Grids beyond OWE_CAUSTIC_NOALLOC_MAX_PIXELS are refused
+ * (OWE_HIP_ERR_INVALID). Calls that overlap on one device share that array and race: serialise
+ * them (one stream, or events). */
+int owe_hip_caustic_forward_noalloc(int device, const owe_caustic_settings* settings, const float* heights,
+                                    float* irradiance, uint64_t pass, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "common.hpp"
@@ -340,6 +341,56 @@ DMedium toDevMedium(const owe_medium& m) {
     return d;
 }
 
+void checkSettings(const owe_caustic_settings& s) {
+    require(s.nodes >= 4 && s.nodes <= 8192, "nodes per side must lie in [4, 8192]");
+    require(s.pixels >= 2 && s.pixels <= 8192, "pixels per side must lie in [2, 8192]");
+    require(s.particles >= 1 && s.particles <= (uint64_t(1) << 34), "particles must lie in [1, 2^34]");
+    require(s.slab_half > 0 && s.thickness > 0 && s.wall_half > 0, "sizes must be positive");
+    require(s.wall_z > s.thickness, "the wall must lie beyond the slab");
+    require(s.sun_kelvin > 100, "sun temperature must exceed 100 K");
+    require(s.medium.index_kind >= OWE_INDEX_CONSTANT && s.medium.index_kind <= OWE_INDEX_CIDDOR, "unknown index model");
+}
+
+// The kernels' arguments for settings s (everything but the per-call buffers and pass).
+CArgs makeArgs(const owe_caustic_settings& s) {
+    CArgs a{};
+    a.nodes = s.nodes;
+    a.pixels = s.pixels;
+    a.slabHalf = float(s.slab_half);
+    a.cell = float(2 * s.slab_half / (s.nodes - 1));
+    a.z0 = float(s.thickness);
+    a.zWall = float(s.wall_z);
+    a.wallHalf = float(s.wall_half);
+    a.pix = float(2 * s.wall_half / s.pixels);
+    a.particles = s.particles;
+    a.seed = s.seed;
+    a.slab = toDevMedium(s.medium);
+    owe_medium air{};
+    air.index_kind = OWE_INDEX_CIDDOR;
+    a.air = toDevMedium(air);
+    a.kelvin = float(s.sun_kelvin);
+    // ∫ B ȳ dλ over 360–830 nm (composite Simpson, 0.5 nm).
+    const int n = 940;
+    const double hh = 470.0 / n;
+    double sum = planck(360.0, s.sun_kelvin) * cieYHost(360.0) + planck(830.0, s.sun_kelvin) * cieYHost(830.0);
+    for (int k = 1; k < n; ++k) {
+        const double l = 360.0 + k * hh;
+        sum += planck(l, s.sun_kelvin) * cieYHost(l) * ((k & 1) ? 4 : 2);
+    }
+    a.invY = float(1.0 / (sum * hh / 3));
+    const double pix = 2 * s.wall_half / s.pixels;
+    a.area = float((2 * s.slab_half) * (2 * s.slab_half) / (double(s.particles) * pix * pix));
+    return a;
+}
+
+// The forward kernel's grid: 8 blocks of 256 per compute unit.
+int forwardGrid(int device) {
+    int cus = 0;
+    OWE_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    require(cus > 0, "cannot read the device's compute-unit count");
+    return cus * 8;
+}
+
 }  // namespace
 }  // namespace owehip
 
@@ -359,51 +410,17 @@ int owe_hip_caustic_create(int device, const owe_caustic_settings* settings, owe
         require(settings != nullptr && out != nullptr, "null argument");
         *out = nullptr;
         const owe_caustic_settings& s = *settings;
-        require(s.nodes >= 4 && s.nodes <= 8192, "nodes per side must lie in [4, 8192]");
-        require(s.pixels >= 2 && s.pixels <= 8192, "pixels per side must lie in [2, 8192]");
-        require(s.particles >= 1 && s.particles <= (uint64_t(1) << 34), "particles must lie in [1, 2^34]");
-        require(s.slab_half > 0 && s.thickness > 0 && s.wall_half > 0, "sizes must be positive");
-        require(s.wall_z > s.thickness, "the wall must lie beyond the slab");
-        require(s.sun_kelvin > 100, "sun temperature must exceed 100 K");
-        require(s.medium.index_kind >= OWE_INDEX_CONSTANT && s.medium.index_kind <= OWE_INDEX_CIDDOR, "unknown index model");
+        checkSettings(s);
         selectDevice(device);
         c = new owe_hip_caustic;
         c->device = device;
         c->settings = s;
-        hipDeviceProp_t prop;
-        OWE_HIP_CHECK(hipGetDeviceProperties(&prop, device));
-        c->grid = prop.multiProcessorCount * 8;
-        CArgs& a = c->args;
-        a.nodes = s.nodes;
-        a.pixels = s.pixels;
-        a.slabHalf = float(s.slab_half);
-        a.cell = float(2 * s.slab_half / (s.nodes - 1));
-        a.z0 = float(s.thickness);
-        a.zWall = float(s.wall_z);
-        a.wallHalf = float(s.wall_half);
-        a.pix = float(2 * s.wall_half / s.pixels);
-        a.particles = s.particles;
-        a.seed = s.seed;
-        a.slab = toDevMedium(s.medium);
-        owe_medium air{};
-        air.index_kind = OWE_INDEX_CIDDOR;
-        a.air = toDevMedium(air);
-        a.kelvin = float(s.sun_kelvin);
-        // ∫ B ȳ dλ over 360–830 nm (composite Simpson, 0.5 nm).
-        const int n = 940;
-        const double hh = 470.0 / n;
-        double sum = planck(360.0, s.sun_kelvin) * cieYHost(360.0) + planck(830.0, s.sun_kelvin) * cieYHost(830.0);
-        for (int k = 1; k < n; ++k) {
-            const double l = 360.0 + k * hh;
-            sum += planck(l, s.sun_kelvin) * cieYHost(l) * ((k & 1) ? 4 : 2);
-        }
-        a.invY = float(1.0 / (sum * hh / 3));
-        const double pix = 2 * s.wall_half / s.pixels;
-        a.area = float((2 * s.slab_half) * (2 * s.slab_half) / (double(s.particles) * pix * pix));
+        c->grid = forwardGrid(device);
+        c->args = makeArgs(s);
         void* p = nullptr;
         OWE_HIP_CHECK(hipMalloc(&p, size_t(s.pixels) * s.pixels * sizeof(unsigned long long)));
         c->acc = static_cast<unsigned long long*>(p);
-        a.acc = c->acc;
+        c->args.acc = c->acc;
         *out = c;
     });
     if (st != OWE_HIP_OK && c) {
@@ -459,6 +476,69 @@ int owe_hip_caustic_backward(owe_hip_caustic* c, const float* heights, const flo
         a.gradH = grad_heights;
         hipLaunchKernelGGL(causticZero, dim3{1024}, dim3{256}, 0, s, grad_heights, size_t(a.nodes) * a.nodes);
         hipLaunchKernelGGL(causticBackward, dim3{unsigned(c->grid)}, dim3{kCBlock}, 0, s, a);
+        OWE_HIP_CHECK(hipGetLastError());
+    });
+}
+
+// The allocation-free forward. Its fixed-point bins live in `scratch`, one array per device for the
+// life of the process (2 MiB: 512 × 512 pixels at most). Host code never passes the array's name to
+// a kernel (in host code it names a host-side stand-in, not device memory): the first call on a
+// device, with that device current, asks the runtime for the array's device address and keeps it,
+// and every later call on that device uses the kept address. Every call clears the bins it uses
+// before splatting; calls that overlap on one device share the array and race.
+static __device__ __align__(256) unsigned long long scratch[size_t(OWE_CAUSTIC_NOALLOC_MAX_PIXELS)];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+
+struct ScratchSlot {
+    unsigned long long* bins = nullptr;
+    int grid = 0;
+};
+
+ScratchSlot scratchFor(int device) {
+    static std::mutex mu;
+    static ScratchSlot slots[kMaxDevices];
+    require(device < kMaxDevices, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(mu);
+    ScratchSlot& s = slots[device];
+    if (!s.bins) {
+        void* p = nullptr;
+        OWE_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        require(p != nullptr, "the scratch array has no device address");
+        s.grid = forwardGrid(device);
+        s.bins = static_cast<unsigned long long*>(p);
+    }
+    return s;
+}
+
+}  // namespace
+
+int owe_hip_caustic_forward_noalloc(int device, const owe_caustic_settings* settings, const float* heights,
+                                    float* irradiance, uint64_t pass, void* stream) {
+    return guard([&] {
+        require(settings != nullptr && heights != nullptr && irradiance != nullptr, "null argument");
+        const owe_caustic_settings& s = *settings;
+        checkSettings(s);
+        const size_t n = size_t(s.pixels) * s.pixels;
+        if (n > size_t(OWE_CAUSTIC_NOALLOC_MAX_PIXELS))
+            fail(OWE_HIP_ERR_INVALID, "the allocation-free forward holds at most " +
+                                          std::to_string(OWE_CAUSTIC_NOALLOC_MAX_PIXELS) + " pixels (512 x 512); " +
+                                          std::to_string(s.pixels) + " x " + std::to_string(s.pixels) + " asked");
+        selectDevice(device);
+        requireDevicePointer(heights, "heights", device);
+        requireDevicePointer(irradiance, "irradiance", device);
+        const ScratchSlot slot = scratchFor(device);
+        CArgs a = makeArgs(s);
+        a.acc = slot.bins;
+        a.heights = heights;
+        a.pass = pass;
+        a.irradiance = irradiance;
+        const hipStream_t st = static_cast<hipStream_t>(stream);
+        hipLaunchKernelGGL(causticClear, dim3{1024}, dim3{256}, 0, st, slot.bins, n);
+        hipLaunchKernelGGL(causticForward, dim3{unsigned(slot.grid)}, dim3{kCBlock}, 0, st, a);
+        hipLaunchKernelGGL(causticResolve, dim3{1024}, dim3{256}, 0, st, slot.bins, irradiance, n);
         OWE_HIP_CHECK(hipGetLastError());
     });
 }
