@@ -81,14 +81,13 @@ def measured_traffic(path: Path, kernel: str | None = None):
 
 
 def measured_counters(kernel: str):
-    """Occupancy, VALU and L2 figures of `kernel` recorded by the committed rocprofv3 counter summary
-    (a line `counters[kernel]: name=value ...`), if present."""
-    if not PROFILE_COUNTERS.exists():
-        return None
-    m = re.search(r"counters\[" + re.escape(kernel) + r"\]:([^\n]*)", PROFILE_COUNTERS.read_text())
-    if not m:
-        return None
-    return {k: float(v) for k, v in re.findall(r"(\w+)=([0-9.eE+-]+)", m.group(1))}
+    """Occupancy, VALU and L2 figures of `kernel` recorded by a committed rocprofv3 counter summary
+    under profiles/ (a line `counters[kernel]: name=value ...`), if present."""
+    for path in sorted((ROOT / "profiles").glob("*.md")):
+        m = re.search(r"counters\[" + re.escape(kernel) + r"\]:([^\n]*)", path.read_text())
+        if m:
+            return {k: float(v) for k, v in re.findall(r"(\w+)=([0-9.eE+-]+)", m.group(1))}
+    return None
 
 
 def cpu_baseline_rayquery(pos, tri, rays, seconds_budget: float = 15.0):
@@ -518,14 +517,30 @@ def bench_light(args):
     return out
 
 
+def valu_roofline(kernel: str, launch_ms: float):
+    """The VALU roofline of `kernel` from its profiled counters (VALU instructions per launch and
+    active-lane fraction) over the live launch time, or None without a profile line."""
+    cnt = measured_counters(kernel)
+    if not cnt or "valu_insts" not in cnt or "lane_utilisation" not in cnt or not launch_ms > 0:
+        return None
+    achieved = cnt["valu_insts"] * 64 * cnt["lane_utilisation"] / (launch_ms * 1e-3) / 1e12
+    return {"bound": "valu", "kernel": kernel, "unit": "T lane-instructions/s", "peak": VALU_PEAK_TLANE,
+            "achieved": achieved, "frac": achieved / VALU_PEAK_TLANE,
+            "issue_frac": cnt["valu_insts"] * 64 / (launch_ms * 1e-3) / 1e12 / VALU_PEAK_TLANE, "counters": cnt}
+
+
+LENS_PLANES = 5  # through-focus grids per lens, 1e-3 x EFL apart about the paraxial image
+
+
 def bench_lens(args):
     """Batched lens analysis: 256 seeded cemented achromats, each tracing its log-uniform number of
-    rays (4,096 to 1,048,576) at all seven Fraunhofer wavelengths in one persistent launch."""
+    rays (4,096 to 1,048,576) at all seven Fraunhofer wavelengths in one persistent launch, into a
+    through-focus stack of five 128 x 128 grids per lens."""
     from owe_hip import lens as L
 
     lenses = L.seeded_achromats(256)
     traces = 7 * sum(p.rays for p in lenses)
-    b = L.LensBatch(lenses)
+    b = L.LensBatch(lenses, L.c_settings(planes=LENS_PLANES))
     tuned = b.autotune()
     for _ in range(args.warmup):
         b.run()
@@ -539,19 +554,28 @@ def bench_lens(args):
     arrived = sum(r["arrived"] for r in per)
     log(f"lens: {args.steps} runs in {wall:.3f}s; {traces * args.steps / wall / 1e9:.2f} G ray traces/s; "
         f"kernel {kernel_ms / args.steps:.2f} ms/run; arrived {arrived / traces:.3f}")
-    # The grid's histogram atomics are the only algorithmic HBM traffic: 8 B per arrived ray.
-    hist_bytes = 8.0 * arrived
+    # The grids' bin adds are the algorithmic HBM traffic: 8 B per arrived ray and plane (the
+    # per-wave bin tables turn most of them into LDS adds).
+    hist_bytes = 8.0 * arrived * LENS_PLANES
+    k_ms = kernel_ms / args.steps
+    model = {"bound": "hbm", "achieved": hist_bytes / (k_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": hist_bytes / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+             "traffic": measured_traffic(PROFILE_LENS_TRAFFIC, "lensKernel"),
+             "kernel": "lensKernel", "algorithmic_bytes_per_launch": hist_bytes}
+    roofline = valu_roofline("lensKernel", k_ms)
+    if roofline is None:
+        roofline = model
+    else:
+        roofline["traffic"] = model["traffic"]
+        roofline["hbm_model"] = model
     out = {"value": traces * args.steps / wall, "unit": "ray traces/s (lens x wavelength x ray)",
            "ms_per_step": wall * 1e3 / args.steps,
            "config": {"workload": "batched lens analysis, 256 seeded cemented achromats, 7 wavelengths, 0.5 deg field",
                       "lenses": len(lenses), "traces_per_step": traces, "arrived_fraction": arrived / traces,
-                      "kernel_ms_per_step": kernel_ms / args.steps, "autotuned_ms": tuned,
+                      "focus_planes": LENS_PLANES, "grid_pixels": 128,
+                      "kernel_ms_per_step": k_ms, "autotuned_ms": tuned,
                       "rays_min": min(p.rays for p in lenses), "rays_max": max(p.rays for p in lenses)},
-           "roofline": {"bound": "hbm", "achieved": hist_bytes / (kernel_ms / args.steps * 1e-3) / 1e9,
-                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": hist_bytes / (kernel_ms / args.steps * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                        "traffic": measured_traffic(PROFILE_LENS_TRAFFIC, "lensKernel"),
-                        "kernel": "lensKernel", "algorithmic_bytes_per_launch": hist_bytes}}
+           "roofline": roofline}
     b.close()
     if not args.no_cpu_baseline:
         sys.path.insert(0, str(ROOT / "oracle"))
@@ -617,9 +641,15 @@ def bench_caustic(args):
     # Algorithmic HBM traffic of the adjoint: 16 float atomics of 4 B per particle into the heights'
     # gradient, ∂L/∂E read at 9 pixels (4 B each) — the heights and ∂L/∂E are L2-resident.
     adj_bytes = n * (16 * 4 + 9 * 4)
-    out = {"value": n * args.steps / wall, "unit": "particles/s (forward + adjoint)", "ms_per_step": wall * 1e3 / args.steps,
+    # The metric's "adjoint grads/s": whole gradients (all 65,536 height partials, each a sum over
+    # the pass's 1,048,576 particles) per second, forward pass included.
+    out = {"value": args.steps / wall, "unit": "gradients/s (forward + adjoint, 65,536 heights each)",
+           "ms_per_step": wall * 1e3 / args.steps,
            "config": {"workload": "caustic design gradient step, 100 mm PMMA slab, 256x256 heights, 512x512 wall",
                       "particles_per_step": n, "nodes": s.nodes, "pixels": s.pixels,
+                      "height_partials_per_s": s.nodes * s.nodes * args.steps / wall,
+                      "particles_per_s": n * args.steps / wall,
+                      "entry_sampling": "stratified over square tiles of the entry face",
                       "forward_ms": f_ms, "adjoint_ms": b_ms, "adjoint_over_forward": b_ms / f_ms},
            "roofline": {"bound": "hbm", "achieved": adj_bytes / (b_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": adj_bytes / (b_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
@@ -640,11 +670,13 @@ def bench_caustic(args):
             co.forward(Hn, frames, threads=threads)
             el += time.perf_counter() - t0
             frames += 1
-        out["cpu_baseline"] = {"value": frames * n / el, "unit": "particles/s (forward only)", "cores": threads,
-                               "kind": "port",
+        # The CPU has no adjoint: a gradient by central differences costs 2 x 65,536 forward passes.
+        out["cpu_baseline"] = {"value": frames / el / (2 * s.nodes * s.nodes), "unit": "gradients/s (central differences)",
+                               "cores": threads, "kind": "port",
+                               "forward_particles_per_s": frames * n / el,
                                "sample": f"{frames} forward passes of {n} particles ({el:.1f} s) through the oracle's "
-                                         f"fp64 slab, {threads} threads; the CPU has no adjoint — a finite-difference "
-                                         f"gradient costs 2 x {s.nodes * s.nodes} such passes"}
+                                         f"fp64 slab, {threads} threads; a finite-difference gradient costs "
+                                         f"2 x {s.nodes * s.nodes} such passes"}
     return out
 
 

@@ -73,6 +73,11 @@ typedef struct owe_lens_batch_settings {
     int32_t detector_pixels;      /* side of each lens's square irradiance grid; 0: none */
     double detector_half_size;    /* metres about the chief ray's paraxial image point; 0: 5e-3 × EFL */
     double reference_wavelength;  /* nm: the paraxial quantities and the detector plane */
+    /* The through-focus stack: focus_planes grids (1–16; 0 means 1) spaced focus_step metres along
+     * z, centred on the paraxial image plane (0: a step of 1e-3 × |EFL|). */
+    int32_t focus_planes;
+    int32_t reserved;
+    double focus_step;
 } owe_lens_batch_settings;
 
 /* Per lens, over every wavelength together: focusedSpot over all arrived rays, or for an afocal
@@ -125,7 +130,8 @@ int owe_hip_lens_batch_destroy(owe_hip_lens_batch* batch);
 int owe_hip_lens_batch_run(owe_hip_lens_batch* batch);
 /* Results of the last run: per lens (may be null), the moments per lens and wavelength
  * (n_lenses × n_wavelengths × OWE_LENS_MOMENTS doubles; may be null) and the irradiance grids
- * (n_lenses × pixels × pixels floats, unit incident irradiance across the beam; may be null). */
+ * (n_lenses × focus_planes × pixels × pixels floats, unit incident irradiance across the beam,
+ * plane 0 nearest the lens; may be null). */
 int owe_hip_lens_batch_results(owe_hip_lens_batch* batch, owe_lens_result* lenses, double* moments, float* irradiance);
 /* The persistent launch's shape: resident blocks per compute unit and rays per work item (a
  * multiple of 64); 0 keeps the current value. */

@@ -1,8 +1,9 @@
 // Differentiable caustic design: owe_hip_caustic_* of include/owe_hip_caustic.h.
 //
-// Forward: one thread per particle. A particle of pass k is stream (hashCombine(seed, i), k): a
-// wavelength importance-sampled over the visible range and weighted by the sun's luminous spectrum,
-// and a point uniform over the slab's entry face. It enters at normal incidence (weight 1 − R),
+// Forward: one thread per particle, each wave over a contiguous range of particles. A particle of
+// pass k is stream (hashCombine(seed, i), k): a wavelength importance-sampled over the visible range
+// and weighted by the sun's luminous spectrum, and a point on the slab's entry face, stratified over
+// square tiles (a wave's particles share a tile or two). It enters at normal incidence (weight 1 − R),
 // crosses the slab along +z to the exit surface z = t + h(x, y), refracts out of it (weight 1 − R;
 // total internal reflection ends it) and lands on the wall, where its weight spreads over the 3 × 3
 // pixels around it with quadratic B-spline weights (continuously differentiable in the landing
@@ -14,7 +15,9 @@
 // dual numbers (∂/∂h, ∂/∂h_x, ∂/∂h_y at the particle's exit point), which with the spline weights
 // and ∂L/∂E at the nine pixels gives ∂L/∂(h, h_x, h_y) for this particle. h, h_x and h_y are linear in
 // the 4 × 4 heights around the cell (bicubic Hermite patch with central-difference node slopes), so
-// the particle adds its share to those sixteen heights: sixteen float atomics and no stored paths.
+// the particle adds its share to those sixteen heights: no stored paths. Both passes gather their
+// adds (pixels, nodes) in a per-wave table in LDS and send each touched pixel or node to HBM once
+// per wave.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -42,7 +45,8 @@ struct CArgs {
     uint64_t particles, seed, pass;
     DMedium slab, air;
     float kelvin, invY;      // sun temperature; 1 / ∫ B ȳ over 360–830 nm
-    float area;              // slab area / (particles · pixel area): irradiance per unit particle weight
+    float tileArea;          // a tile's area / pixel area: with a tile's particle count, irradiance per unit weight
+    uint32_t tiles;          // tiles per side (a power of two)
     unsigned long long* acc;
     float* irradiance;
     const float* gradE;
@@ -181,6 +185,20 @@ __device__ __forceinline__ uint32_t nodeOf(const CArgs& a, const Patch& p, int r
 struct Particle {
     float x, y, weight, n1, n2;  // weight: luminous weight × entry transmittance × area factor
 };
+// The entry face is cut into tiles × tiles square tiles in Morton order; tile t holds the particles
+// [⌊t·N/T⌋, ⌊(t+1)·N/T⌋) (T tiles, N particles), each uniform over its tile. Stratified sampling
+// (unbiased: each tile's weight is its area over its own count) that also keeps a wave's
+// consecutive particles in one small patch of the slab, so their splats and gradient adds meet the
+// same pixels and nodes.
+__device__ __forceinline__ uint64_t tileStart(uint64_t t, uint64_t n, uint64_t T) { return t * n / T; }
+__device__ __forceinline__ uint32_t mortonHalf(uint32_t v) {  // the even bits of v, packed
+    v &= 0x55555555u;
+    v = (v | (v >> 1)) & 0x33333333u;
+    v = (v | (v >> 2)) & 0x0F0F0F0Fu;
+    v = (v | (v >> 4)) & 0x00FF00FFu;
+    v = (v | (v >> 8)) & 0x0000FFFFu;
+    return v;
+}
 __device__ __forceinline__ Particle drawParticle(const CArgs& a, uint64_t i) {
     Rng rng;
     rng.seed(hashPair(a.seed, i), a.pass);
@@ -188,14 +206,75 @@ __device__ __forceinline__ Particle drawParticle(const CArgs& a, uint64_t i) {
     const float lambda = sampleVisibleF(ul);
     const float pdf = visiblePdfF(lambda);
     const float u2 = rng.uniform(), u1 = rng.uniform();
+    const uint64_t T = uint64_t(a.tiles) * a.tiles;
+    uint64_t t = i * T / a.particles;
+    while (t + 1 < T && tileStart(t + 1, a.particles, T) <= i) ++t;
+    while (t > 0 && tileStart(t, a.particles, T) > i) --t;
+    const uint64_t count = tileStart(t + 1, a.particles, T) - tileStart(t, a.particles, T);
+    const float side = 2.0f * a.slabHalf / float(a.tiles);
     Particle q;
-    q.x = (2.0f * u1 - 1.0f) * a.slabHalf;
-    q.y = (2.0f * u2 - 1.0f) * a.slabHalf;
+    q.x = -a.slabHalf + (float(mortonHalf(uint32_t(t))) + u1) * side;
+    q.y = -a.slabHalf + (float(mortonHalf(uint32_t(t >> 1))) + u2) * side;
     q.n1 = mediumIndex(a.slab, lambda);
     q.n2 = mediumIndex(a.air, lambda);
     const float lum = planckF(lambda, a.kelvin) * cieXYZ(lambda).y / pdf * a.invY;
-    q.weight = lum * (1.0f - fresnelDielectric(1.0f, q.n1 / q.n2)) * a.area;
+    q.weight = lum * (1.0f - fresnelDielectric(1.0f, q.n1 / q.n2)) * (a.tileArea / float(count));
     return q;
+}
+
+// A wave's table in LDS of adds keyed by pixel or node: open addressing over kTableSlots (key,
+// sum) pairs; an add that finds no slot within kTableProbes goes straight to HBM. flush adds every
+// claimed slot to HBM and empties the table (wave-wide, after every lane's adds: the fence keeps
+// the compiler from moving LDS accesses across it; one wave's LDS operations complete in order).
+constexpr int kTableSlots = 512;
+constexpr int kTableProbes = 8;
+constexpr uint32_t kTableEmpty = 0xFFFFFFFFu;
+
+template <class V>
+struct WaveTable {
+    uint32_t* keys;
+    V* sums;
+    __device__ __forceinline__ void clear(uint32_t lane) const {
+        for (uint32_t q = lane; q < uint32_t(kTableSlots); q += 64) {
+            keys[q] = kTableEmpty;
+            sums[q] = V(0);
+        }
+    }
+    __device__ __forceinline__ void add(V* global, uint32_t key, V v) const {
+        uint32_t slot = (key * 2654435761u) >> (32 - 9);
+        static_assert(kTableSlots == 512, "the slot hash takes 9 bits");
+#pragma unroll 1
+        for (int probe = 0; probe < kTableProbes; ++probe) {
+            const uint32_t old = atomicCAS(&keys[slot], kTableEmpty, key);
+            if (old == kTableEmpty || old == key) {
+                atomicAdd(&sums[slot], v);
+                return;
+            }
+            slot = (slot + 1) & uint32_t(kTableSlots - 1);
+        }
+        atomicAdd(global + key, v);
+    }
+    __device__ __forceinline__ void flush(V* global, uint32_t lane) const {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (uint32_t q = lane; q < uint32_t(kTableSlots); q += 64) {
+            const uint32_t key = keys[q];
+            if (key != kTableEmpty) {
+                atomicAdd(global + key, sums[q]);
+                keys[q] = kTableEmpty;
+                sums[q] = V(0);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+};
+
+// The contiguous range of particles wave w of the grid takes: ⌈N / waves⌉ rounded up to whole waves.
+__device__ __forceinline__ void waveRange(uint64_t n, uint64_t& begin, uint64_t& end) {
+    const uint64_t waves = uint64_t(gridDim.x) * (kCBlock / 64);
+    const uint64_t w = (uint64_t(blockIdx.x) * kCBlock + threadIdx.x) >> 6;
+    const uint64_t per = ((n + waves - 1) / waves + 63) / 64 * 64;
+    begin = w * per;
+    end = begin + per < n ? begin + per : n;
 }
 
 __device__ __forceinline__ void heightsAt(const CArgs& a, const Patch& p, float& h, float& fu, float& fv) {
@@ -236,8 +315,14 @@ __device__ __forceinline__ void splatOf(const CArgs& a, float xw, float yw, int&
 }
 
 __global__ __launch_bounds__(kCBlock) void causticForward(CArgs a) {
-    const uint64_t stride = uint64_t(gridDim.x) * kCBlock;
-    for (uint64_t i = uint64_t(blockIdx.x) * kCBlock + threadIdx.x; i < a.particles; i += stride) {
+    __shared__ uint32_t keys[kCBlock / 64][kTableSlots];
+    __shared__ unsigned long long sums[kCBlock / 64][kTableSlots];
+    const uint32_t lane = __lane_id();
+    const WaveTable<unsigned long long> table{keys[threadIdx.x / 64], sums[threadIdx.x / 64]};
+    table.clear(lane);
+    uint64_t begin, end;
+    waveRange(a.particles, begin, end);
+    for (uint64_t i = begin + lane; i < end; i += 64) {
         const Particle q = drawParticle(a, i);
         Patch p;
         patchAt(a, q.x, q.y, p);
@@ -256,9 +341,10 @@ __global__ __launch_bounds__(kCBlock) void causticForward(CArgs a) {
                 const int pi = ci + c - 1, pj = cj + b - 1;
                 if (pi < 0 || pj < 0 || pi >= int(a.pixels) || pj >= int(a.pixels)) continue;
                 const long long units = llrint(double(w * wu[c] * wv[b]) * kSplatUnits);
-                if (units > 0) atomicAdd(&a.acc[size_t(pj) * a.pixels + pi], (unsigned long long)units);
+                if (units > 0) table.add(a.acc, uint32_t(pj) * a.pixels + uint32_t(pi), (unsigned long long)units);
             }
     }
+    table.flush(a.acc, lane);
 }
 
 __global__ void causticClear(unsigned long long* acc, size_t n) {
@@ -273,8 +359,14 @@ __global__ void causticZero(float* g, size_t n) {
 }
 
 __global__ __launch_bounds__(kCBlock) void causticBackward(CArgs a) {
-    const uint64_t stride = uint64_t(gridDim.x) * kCBlock;
-    for (uint64_t i = uint64_t(blockIdx.x) * kCBlock + threadIdx.x; i < a.particles; i += stride) {
+    __shared__ uint32_t keys[kCBlock / 64][kTableSlots];
+    __shared__ float sums[kCBlock / 64][kTableSlots];
+    const uint32_t lane = __lane_id();
+    const WaveTable<float> table{keys[threadIdx.x / 64], sums[threadIdx.x / 64]};
+    table.clear(lane);
+    uint64_t begin, end;
+    waveRange(a.particles, begin, end);
+    for (uint64_t i = begin + lane; i < end; i += 64) {
         const Particle q = drawParticle(a, i);
         Patch p;
         patchAt(a, q.x, q.y, p);
@@ -312,9 +404,10 @@ __global__ __launch_bounds__(kCBlock) void causticBackward(CArgs a) {
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 const float g = gq[0] * p.wf[r][c] + su * p.wu[r][c] + sv * p.wv[r][c];
-                if (g != 0.0f) atomicAdd(&a.gradH[nodeOf(a, p, r, c)], g);
+                if (g != 0.0f) table.add(a.gradH, nodeOf(a, p, r, c), g);
             }
     }
+    table.flush(a.gradH, lane);
 }
 
 double planck(double lnm, double kelvin) {
@@ -339,6 +432,14 @@ DMedium toDevMedium(const owe_medium& m) {
         d.b[k] = float(m.coeff_b[k]);
     }
     return d;
+}
+
+// Tiles per side of the entry face: the largest power of two up to 64 that leaves at least 64
+// particles per tile.
+uint32_t causticTiles(uint64_t particles) {
+    uint32_t g = 1;
+    while (g < 64 && uint64_t(4) * g * g * 64 <= particles) g *= 2;
+    return g;
 }
 
 void checkSettings(const owe_caustic_settings& s) {
@@ -379,7 +480,9 @@ CArgs makeArgs(const owe_caustic_settings& s) {
     }
     a.invY = float(1.0 / (sum * hh / 3));
     const double pix = 2 * s.wall_half / s.pixels;
-    a.area = float((2 * s.slab_half) * (2 * s.slab_half) / (double(s.particles) * pix * pix));
+    a.tiles = causticTiles(s.particles);
+    const double side = 2 * s.slab_half / a.tiles;
+    a.tileArea = float(side * side / (pix * pix));
     return a;
 }
 

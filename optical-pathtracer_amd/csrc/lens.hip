@@ -4,15 +4,22 @@
 // persistent grid: each wave takes the next item with one atomic, its lanes trace the block's rays
 // through the lens surface after surface, reduce the spot's sums across the wave in double and add
 // them to the (lens, wavelength) totals with one atomic per sum; arrived rays also land on the lens's
-// detector grid as 64-bit fixed-point weights, so the irradiance is independent of order. Lenses
-// of very different ray counts (4k to 1M) share the launch without tails of idle blocks.
+// through-focus stack of detector grids as 64-bit fixed-point weights, so the irradiance is
+// independent of order. The grid adds gather first in a per-wave hash table in LDS (bin → sum),
+// which a wave flushes to HBM when its items move to another lens: a spot covers few bins, so most
+// of a lens's rays meet a bin already in the table and HBM sees about one atomic per bin touched
+// instead of one per ray. Lenses of very different ray counts (4k to 1M) share the launch without
+// tails of idle blocks.
 //
 // Ray–surface roots follow SagSurface (src/owe/scene/geometry.cpp:115-166,
-// src/owe/core/shared/surfaces.hpp:13-94) in float32 with the reference's float tolerances, and
-// each root is then polished in float64 — a quadric root re-solved from the approximate hit point
-// (refineRoot), a numeric asphere root by Newton steps — the only fp64 on the GPU path, where
-// strongly curved surfaces would otherwise lose digits. Transmission is the walk's deterministic
-// refraction with weight T = 1 − R (transport.cpp:531-595); TIR or a missed aperture ends a ray.
+// src/owe/core/shared/surfaces.hpp:13-94) in float32 with the reference's float tolerances; each
+// quadric root is then re-solved from the approximate hit point (refineRoot) — in float64 on a
+// steep surface (|c|·r_max > 0.5), where the quadric's terms cancel, and throughout instruments with
+// mirrors or without focal power (metre paths, high angular magnification), in float elsewhere — and a
+// numeric asphere root is polished by Newton steps in float64: the only fp64 of the per-ray path
+// (the spot sums are accumulated in double from float per-ray terms). Transmission is the walk's
+// deterministic refraction with weight T = 1 − R (transport.cpp:531-595); TIR or a missed aperture
+// ends a ray.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -33,6 +40,13 @@ namespace {
 
 constexpr int kLensBlock = 256;
 constexpr double kHistScale = 4294967296.0;  // 2^32 units of weight per unit
+// The per-wave bin table: open addressing over kHashSlots (bin, sum) pairs, at most kHashProbes
+// probes before an add goes straight to HBM.
+constexpr int kHashSlots = 512;
+constexpr int kHashProbes = 8;
+constexpr uint32_t kHashEmpty = 0xFFFFFFFFu;
+// The steepness |c|·r_max above which a quadric root is polished in float64.
+constexpr double kSteepPolish = 0.5;
 
 // A lens row: its sag surface (shapes_dev.hpp; a mirror's annulus outside its hole), the vertex
 // position along the fixed chart, and whether it reflects.
@@ -54,6 +68,7 @@ struct DLensInfo {
     uint32_t rings;           // 0: Fibonacci lattice; r: hexapolar rings
     double rEP, zEP, zStart;  // entrance pupil (sampling radius 0.999 rEP) and launch plane
     float detX, detY, detZ, detHalf;  // detector grid centre, plane, half size (0: no grid)
+    float detStep;            // through-focus plane spacing along z
     int32_t afocal;           // spot sums of angles (afocalSpot) instead of (A, B) at an image
     int32_t mirrors;          // reflecting rows: their vertex planes obstruct
 };
@@ -63,12 +78,12 @@ struct LensArgs {
     const DLensInfo* lens;
     const float* nAfter;      // [surface × nλ]: index after each surface
     const float* nAmbient;    // [lens × nλ]
-    uint32_t nLenses, nLambda, raysPerItem, pixels;
+    uint32_t nLenses, nLambda, raysPerItem, pixels, planes;
     uint64_t totalItems;
     double dirY, dirZ;        // collimated field direction (0, sin θ, cos θ)
     unsigned long long* next;
     double* moments;          // [lens × nλ × OWE_LENS_MOMENTS]
-    unsigned long long* hist; // [lens × pixels²]
+    unsigned long long* hist; // [lens × planes × pixels²]
     // Gradient passes: parameters [paramBase, paramBase + kGroup) of nParams, each (surface within
     // the lens, field); their sums [lens × nλ × OWE_LENS_MAX_PARAMS × kGradSums].
     uint32_t nParams, paramBase;
@@ -96,13 +111,14 @@ __device__ __forceinline__ void pupilPoint(uint64_t i, uint64_t n, uint32_t ring
         y = rr * sin(a);
         return;
     }
-    const double rr = 0.999 * rEP * sqrt((double(i) + 0.5) / double(n));
-    double f = double(i) * 0.6180339887498949;
-    f -= floor(f);
-    double sa, ca;
-    sincospi(2.0 * f, &sa, &ca);
-    x = rr * ca;
-    y = rr * sa;
+    // The lattice in float: the golden-ratio fraction of i from 64-bit fixed point (exact for any
+    // i), the radius and angle to float rounding.
+    const float rr = 0.999f * float(rEP) * sqrtf((float(i) + 0.5f) / float(n));
+    const float f = float(uint32_t((i * 0x9E3779B97F4A7C15ull) >> 40)) * 0x1p-24f;
+    float sa, ca;
+    sincospif(2.0f * f, &sa, &ca);
+    x = double(rr * ca);
+    y = double(rr * sa);
 }
 
 // Whether the ray o + s d, s in (0, sEnd) (sEnd < 0: unbounded), crosses the vertex-plane annulus
@@ -262,13 +278,62 @@ __device__ __forceinline__ double waveSumD(double v) {
     return v;
 }
 
+// A wave's bin table in LDS. Keys are bins of the lens the wave is on (plane · pixels² + row ·
+// pixels + column); a key is claimed by compare-and-swap, its sum grown by a 64-bit LDS add.
+struct BinTable {
+    uint32_t* keys;
+    unsigned long long* sums;
+};
+
+__device__ __forceinline__ void binAdd(BinTable t, unsigned long long* lensHist, uint32_t key, unsigned long long v) {
+    uint32_t slot = (key * 2654435761u) >> (32 - 9);
+    static_assert(kHashSlots == 512, "the slot hash takes 9 bits");
+#pragma unroll 1
+    for (int probe = 0; probe < kHashProbes; ++probe) {
+        const uint32_t old = atomicCAS(&t.keys[slot], kHashEmpty, key);
+        if (old == kHashEmpty || old == key) {
+            atomicAdd(&t.sums[slot], v);
+            return;
+        }
+        slot = (slot + 1) & uint32_t(kHashSlots - 1);
+    }
+    atomicAdd(lensHist + key, v);  // the table is crowded here: straight to HBM
+}
+
+// Adds every claimed bin of the table to the lens's grids and empties the table. Wave-wide: every
+// lane takes part, after every lane's adds (the fence keeps the compiler from moving LDS accesses
+// across it; one wave's LDS operations complete in order).
+__device__ __forceinline__ void binFlush(BinTable t, unsigned long long* lensHist, uint32_t lane) {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    for (uint32_t q = lane; q < uint32_t(kHashSlots); q += 64) {
+        const uint32_t key = t.keys[q];
+        if (key != kHashEmpty) {
+            atomicAdd(lensHist + key, t.sums[q]);
+            t.keys[q] = kHashEmpty;
+            t.sums[q] = 0;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
 // The persistent batch: a wave takes the next (lens, wavelength, block of rays) item, its lanes
 // trace the block, the spot sums (and with kGrad the derivative sums of the pass's parameters) are
-// reduced across the wave in double and added with one atomic each.
+// reduced across the wave in double and added with one atomic each; grid adds go through the
+// wave's bin table, flushed when the wave's next item is another lens's and before it exits.
 template <bool kGrad>
 __global__ __launch_bounds__(kLensBlock) void lensKernel(LensArgs a) {
+    __shared__ uint32_t tableKeys[kLensBlock / 64][kHashSlots];
+    __shared__ unsigned long long tableSums[kLensBlock / 64][kHashSlots];
     const uint32_t lane = __lane_id();
     const bool sums = !kGrad || a.sums;
+    const bool grids = sums && a.pixels > 0;
+    const BinTable table{tableKeys[threadIdx.x / 64], tableSums[threadIdx.x / 64]};
+    for (uint32_t q = lane; q < uint32_t(kHashSlots); q += 64) {
+        table.keys[q] = kHashEmpty;
+        table.sums[q] = 0;
+    }
+    const size_t lensBins = size_t(a.planes) * a.pixels * a.pixels;
+    uint32_t tableLens = ~0u;  // the lens whose bins the table holds
     for (;;) {
         unsigned long long item = 0;
         if (lane == 0) item = atomicAdd(a.next, 1ull);
@@ -287,6 +352,11 @@ __global__ __launch_bounds__(kLensBlock) void lensKernel(LensArgs a) {
         }
         const uint32_t li = lo - 1;
         const DLensInfo& L = a.lens[li];
+        if (grids && li != tableLens) {
+            if (tableLens != ~0u) binFlush(table, a.hist + tableLens * lensBins, lane);
+            tableLens = li;
+        }
+        unsigned long long* lensHist = a.hist + size_t(li) * lensBins;
         const uint64_t local = item - L.itemBase;
         const uint32_t w = uint32_t(local / L.itemsPerLambda);
         const uint64_t first = (local % L.itemsPerLambda) * a.raysPerItem;
@@ -299,50 +369,56 @@ __global__ __launch_bounds__(kLensBlock) void lensKernel(LensArgs a) {
             float T;
             RayGrad G;
             if (!traceLensRay<kGrad>(a, L, li, w, i, p, d, T, G)) continue;
-            const double bx = double(d.x) / double(d.z), by = double(d.y) / double(d.z);
-            const double ax = L.afocal ? double(atanf(float(bx))) : double(p.x) - bx * double(p.z);
-            const double ay = L.afocal ? double(atanf(float(by))) : double(p.y) - by * double(p.z);
+            // The ray's spot terms in float (B = d_xy / d_z, A at z = 0 or the angles), summed in
+            // double.
+            const float iz = 1.0f / d.z;
+            const float bx = d.x * iz, by = d.y * iz;
+            const float ax = L.afocal ? atanf(bx) : p.x - bx * p.z;
+            const float ay = L.afocal ? atanf(by) : p.y - by * p.z;
             if (kGrad) {
-                const float iz = 1.0f / d.z;
 #pragma unroll
                 for (int kp = 0; kp < kGroup; ++kp) {
                     const float dbx = (G.d[0][kp] * d.z - d.x * G.d[2][kp]) * iz * iz;
                     const float dby = (G.d[1][kp] * d.z - d.y * G.d[2][kp]) * iz * iz;
                     float dax, day;
                     if (L.afocal) {
-                        dax = dbx / (1.0f + float(bx * bx));
-                        day = dby / (1.0f + float(by * by));
+                        dax = dbx / (1.0f + bx * bx);
+                        day = dby / (1.0f + by * by);
                     } else {
-                        dax = G.o[0][kp] - dbx * p.z - float(bx) * G.o[2][kp];
-                        day = G.o[1][kp] - dby * p.z - float(by) * G.o[2][kp];
+                        dax = G.o[0][kp] - dbx * p.z - bx * G.o[2][kp];
+                        day = G.o[1][kp] - dby * p.z - by * G.o[2][kp];
                     }
                     gs[kp][0] += dax;
                     gs[kp][1] += day;
                     gs[kp][2] += dbx;
                     gs[kp][3] += dby;
-                    gs[kp][4] += float(ax) * dax + float(ay) * day;
-                    gs[kp][5] += float(bx) * dbx + float(by) * dby;
-                    gs[kp][6] += dax * float(bx) + day * float(by) + float(ax) * dbx + float(ay) * dby;
+                    gs[kp][4] += ax * dax + ay * day;
+                    gs[kp][5] += bx * dbx + by * dby;
+                    gs[kp][6] += dax * bx + day * by + ax * dbx + ay * dby;
                 }
             }
             if (!sums) continue;
             m[0] += 1.0;
-            m[1] += ax;
-            m[2] += ay;
-            m[3] += bx;
-            m[4] += by;
-            m[5] += ax * ax + ay * ay;
-            m[6] += bx * bx + by * by;
-            m[7] += ax * bx + ay * by;
+            m[1] += double(ax);
+            m[2] += double(ay);
+            m[3] += double(bx);
+            m[4] += double(by);
+            m[5] += double(ax * ax + ay * ay);
+            m[6] += double(bx * bx + by * by);
+            m[7] += double(ax * bx + ay * by);
             m[8] += double(T);
-            if (a.pixels > 0 && L.detHalf > 0.0f) {
-                const float s = (L.detZ - p.z) / d.z;
-                const float gx = (p.x + d.x * s - L.detX) / L.detHalf, gy = (p.y + d.y * s - L.detY) / L.detHalf;
-                const float fx = (gx + 1.0f) * 0.5f * float(a.pixels), fy = (1.0f - gy) * 0.5f * float(a.pixels);
-                if (fx >= 0.0f && fy >= 0.0f && fx < float(a.pixels) && fy < float(a.pixels)) {
-                    const uint32_t px = min(uint32_t(fx), a.pixels - 1), py = min(uint32_t(fy), a.pixels - 1);
-                    atomicAdd(&a.hist[(size_t(li) * a.pixels + py) * a.pixels + px],
-                              (unsigned long long)llrint(double(T) * kHistScale));
+            if (grids && L.detHalf > 0.0f) {
+                const unsigned long long v = (unsigned long long)llrint(double(T) * kHistScale);
+                const float half = 0.5f * float(a.planes - 1);
+                for (uint32_t pl = 0; pl < a.planes; ++pl) {
+                    const float z = L.detZ + (float(pl) - half) * L.detStep;
+                    const float s = (z - p.z) * iz;
+                    const float gx = (p.x + d.x * s - L.detX) / L.detHalf, gy = (p.y + d.y * s - L.detY) / L.detHalf;
+                    const float fx = (gx + 1.0f) * 0.5f * float(a.pixels), fy = (1.0f - gy) * 0.5f * float(a.pixels);
+                    if (fx >= 0.0f && fy >= 0.0f && fx < float(a.pixels) && fy < float(a.pixels)) {
+                        const uint32_t px = min(uint32_t(fx), a.pixels - 1), py = min(uint32_t(fy), a.pixels - 1);
+                        binAdd(table, lensHist, (pl * a.pixels + py) * a.pixels + px, v);
+                    }
                 }
             }
         }
@@ -367,6 +443,7 @@ __global__ __launch_bounds__(kLensBlock) void lensKernel(LensArgs a) {
             }
         }
     }
+    if (grids && tableLens != ~0u) binFlush(table, a.hist + tableLens * lensBins, lane);
 }
 
 __global__ void lensBegin(LensArgs a, size_t nMoments, size_t nHist, size_t nGrads) {
@@ -598,6 +675,8 @@ int owe_hip_lens_batch_create(int device, const owe_medium* media, uint32_t n_me
         require(std::fabs(s.field_deg) < 80, "field angle must lie in (-80, 80) degrees");
         require(s.detector_pixels >= 0 && s.detector_pixels <= 4096, "detector pixels must lie in [0, 4096]");
         require(s.detector_half_size >= 0, "detector half size must be non-negative");
+        require(s.focus_planes >= 0 && s.focus_planes <= 16, "focus planes must lie in [0, 16]");
+        require(s.focus_step >= 0 && std::isfinite(s.focus_step), "the focus step must be finite and non-negative");
         for (uint32_t i = 0; i < n_surfaces; ++i) {
             const owe_lens_surface& r = surfaces[i];
             const std::string name = "lens surface " + std::to_string(i);
@@ -624,6 +703,10 @@ int owe_hip_lens_batch_create(int device, const owe_medium* media, uint32_t n_me
         std::vector<DSurf> ds;
         std::vector<float> nAfter(size_t(n_surfaces) * nw, 1.0f), nAmb(size_t(n_lenses) * nw, 1.0f);
         std::vector<uint8_t> used(n_surfaces, 0);
+        // Lenses whose every root is polished in float64: instruments with mirrors or afocal ones,
+        // where metre-long paths and the angular magnification turn a float root's error along
+        // the ray (about 1e-7 of the path) into angular errors comparable to the spot itself.
+        std::vector<uint8_t> precise;
         const double f = s.field_deg * 3.14159265358979323846 / 180;
         for (uint32_t li = 0; li < n_lenses; ++li) {
             const owe_lens& L = lenses[li];
@@ -676,12 +759,14 @@ int owe_hip_lens_batch_create(int device, const owe_medium* media, uint32_t n_me
             I.detX = 0.0f;
             I.detY = float(px.efl * std::tan(f));
             I.detHalf = float(s.detector_half_size > 0 ? s.detector_half_size : 5e-3 * std::fabs(px.efl));
+            I.detStep = float(s.focus_step > 0 ? s.focus_step : 1e-3 * std::fabs(px.efl));
             I.afocal = px.afocal ? 1 : 0;
             I.mirrors = mirrors;
             if (px.afocal) {  // no image plane: no irradiance grid
-                I.detZ = I.detY = I.detHalf = 0.0f;
+                I.detZ = I.detY = I.detHalf = I.detStep = 0.0f;
             }
             b->info.push_back(I);
+            precise.push_back(mirrors > 0 || px.afocal);
         }
         ds.resize(n_surfaces);
         for (uint32_t li = 0; li < n_lenses; ++li) {
@@ -692,6 +777,7 @@ int owe_hip_lens_batch_create(int device, const owe_medium* media, uint32_t n_me
                 DSurf d{};
                 static_cast<DSag&>(d) = makeSag(r.curvature, r.conic, r.asphere, r.semi_diameter,
                                                 r.reflect ? r.hole_radius : 0.0);
+                d.polish64 = precise[li] || std::fabs(r.curvature) * r.semi_diameter > kSteepPolish ? 1 : 0;
                 d.z = float(z);
                 d.reflect = r.reflect ? 1 : 0;
                 ds[L.first_surface + k] = d;
@@ -713,12 +799,13 @@ int owe_hip_lens_batch_create(int device, const owe_medium* media, uint32_t n_me
         a.nLenses = n_lenses;
         a.nLambda = nw;
         a.pixels = uint32_t(s.detector_pixels);
+        a.planes = uint32_t(std::max(1, s.focus_planes));
         a.dirY = std::sin(f);
         a.dirZ = std::cos(f);
         a.next = lensAlloc<unsigned long long>(b, 1);
         b->nMoments = size_t(n_lenses) * nw * OWE_LENS_MOMENTS;
         a.moments = lensAlloc<double>(b, b->nMoments);
-        b->nHist = size_t(n_lenses) * a.pixels * a.pixels;
+        b->nHist = size_t(n_lenses) * a.planes * a.pixels * a.pixels;
         a.hist = lensAlloc<unsigned long long>(b, b->nHist);
         b->nGrads = 0;
         a.grads = lensAlloc<double>(b, size_t(n_lenses) * nw * OWE_LENS_MAX_PARAMS * kGradSums);
@@ -920,18 +1007,19 @@ int owe_hip_lens_batch_results(owe_hip_lens_batch* b, owe_lens_result* lenses, d
             std::vector<unsigned long long> h(b->nHist);
             OWE_HIP_CHECK(hipMemcpy(h.data(), b->args.hist, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
             const double cosT = b->args.dirZ;
+            const size_t lensBins = size_t(b->args.planes) * px * px;
             for (uint32_t li = 0; li < nl; ++li) {
                 // Unit irradiance across the beam: each ray carries π r² cos θ / (rays · nλ) of flux.
                 const DLensInfo& I = b->info[li];
+                float* out = irradiance + size_t(li) * lensBins;
                 if (!(I.detHalf > 0)) {
-                    std::fill(irradiance + size_t(li) * px * px, irradiance + size_t(li + 1) * px * px, 0.0f);
+                    std::fill(out, out + lensBins, 0.0f);
                     continue;
                 }
                 const double pix = 2.0 * I.detHalf / px;
                 const double k = 3.14159265358979323846 * I.rEP * I.rEP * 0.999 * 0.999 * cosT /
                                  (double(I.rays) * nw * kHistScale * pix * pix);
-                for (size_t q = 0; q < size_t(px) * px; ++q)
-                    irradiance[size_t(li) * px * px + q] = float(double(h[size_t(li) * px * px + q]) * k);
+                for (size_t q = 0; q < lensBins; ++q) out[q] = float(double(h[size_t(li) * lensBins + q]) * k);
             }
         }
     });

@@ -231,6 +231,7 @@ DSag makeSag(double c, double k, const double asphere[4], double rMax, double rM
     d.rMax = float(rMax);
     d.rMax2 = float(rMax * rMax);
     d.rMin2 = float(rMin * rMin);
+    d.polish64 = 1;
     d.nA = 4;
     for (int q = 0; q < 4; ++q) d.A[q] = float(asphere[q]);
     while (d.nA > 0 && asphere[d.nA - 1] == 0) --d.nA;

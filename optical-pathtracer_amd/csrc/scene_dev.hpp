@@ -53,7 +53,7 @@ struct DSag {
     float zMin, zMax;             // the numeric path's slab
     float A[4];                   // even asphere A4..A10
     int32_t nA;                   // 0: quadric path
-    int32_t pad;
+    int32_t polish64;             // a quadric root is polished in double (else in float)
 };
 
 struct DWave {

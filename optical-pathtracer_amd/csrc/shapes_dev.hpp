@@ -10,7 +10,9 @@
 //   sum or sampled bicubic Hermite nodes             geometry.cpp:188-344; waveTaper/hermiteBases,
 //                                                    surfaces.hpp:178-200
 //
-// A quadric root is found in float and polished in double from the approximate hit (refineRoot);
+// A quadric root is found in float and polished from the approximate hit (refineRoot): in double on
+// steep surfaces (DSag::polish64; every surface of the world walk), in float on the lens batch's
+// shallow ones;
 // a numeric asphere root is bracketed over 96 samples, bisected in float and polished by Newton
 // steps in double. The wave surface is marched in steps bounded by its Lipschitz constant along
 // the ray (no crossing skipped), then refined by Newton's method. A ray leaving the shape it
@@ -85,6 +87,19 @@ __device__ __forceinline__ double refineSagD(const DSag& s, f3 of, f3 df, double
     return fabs(du) < 1e-3 * (1.0 + fabs(t)) ? t + du : t;
 }
 
+// refineRoot in float, for surfaces flat enough that the re-solved quadric from a point near the
+// surface loses no digits that matter (DSag::polish64 clear).
+__device__ __forceinline__ float refineSagF(const DSag& s, f3 o, f3 d, float t) {
+    const f3 q = o + d * t;
+    const float A = s.cf * (d.x * d.x + d.y * d.y) + s.ckf * d.z * d.z;
+    const float B = 2.0f * (s.cf * (q.x * d.x + q.y * d.y) + s.ckf * q.z * d.z - d.z);
+    const float C = s.cf * (q.x * q.x + q.y * q.y) + s.ckf * q.z * q.z - 2.0f * q.z;
+    float u0, u1;
+    if (solveQuadraticF(A, B, C, u0, u1) == 0) return t;
+    const float du = fabsf(u0) < fabsf(u1) ? u0 : u1;
+    return fabsf(du) < 1e-3f * (1.0f + fabsf(t)) ? t + du : t;
+}
+
 __device__ __forceinline__ float sagF(const DSag& s, float r) {
     const float r2 = r * r;
     float z = s.cf * r2 / (1.0f + sqrtf(fmaxf(0.0f, 1.0f - (1.0f + s.kf) * s.cf * s.cf * r2)));
@@ -116,10 +131,21 @@ __device__ __forceinline__ double dsagD(const DSag& s, double r) {
     }
     return d;
 }
+__device__ __forceinline__ float dsagF(const DSag& s, float r) {
+    const float root = sqrtf(fmaxf(0.0f, 1.0f - (1.0f + s.kf) * s.cf * s.cf * r * r));
+    float d = root > 0.0f ? s.cf * r / root : 0.0f;
+    float m = 4.0f, rp = r * r * r;
+    for (int i = 0; i < s.nA; ++i) {
+        d += s.A[i] * m * rp;
+        rp *= r * r;
+        m += 2.0f;
+    }
+    return d;
+}
 __device__ __forceinline__ f3 sagNormalF(const DSag& s, f3 p) {
     const float r = sqrtf(p.x * p.x + p.y * p.y);
     if (r < 1e-30f) return mk3(0.0f, 0.0f, 1.0f);
-    const float sl = float(dsagD(s, double(r)));
+    const float sl = s.polish64 ? float(dsagD(s, double(r))) : dsagF(s, r);
     return norm3(mk3(-sl * p.x / r, -sl * p.y / r, 1.0f));
 }
 __device__ __forceinline__ bool sagAccept(const DSag& s, f3 q) {
@@ -157,10 +183,19 @@ __device__ __forceinline__ bool sagHit(const DSag& s, f3 o, f3 d, float tmin, fl
             const float tc = k == 0 ? t0 : t1;
             const float m = 1e-3f * (1.0f + fabsf(tc));
             if (tc + m <= tmin || tc - m >= tmax) continue;  // cannot be refined into (tmin, tmax)
-            const double tr = refineSagD(s, o, d, double(tc));
-            if (!(tr > double(tmin) && tr < double(tmax))) continue;
-            const f3 q = mk3(float(double(o.x) + double(d.x) * tr), float(double(o.y) + double(d.y) * tr),
-                             float(double(o.z) + double(d.z) * tr));
+            f3 q;
+            double tr;
+            if (s.polish64) {
+                tr = refineSagD(s, o, d, double(tc));
+                if (!(tr > double(tmin) && tr < double(tmax))) continue;
+                q = mk3(float(double(o.x) + double(d.x) * tr), float(double(o.y) + double(d.y) * tr),
+                        float(double(o.z) + double(d.z) * tr));
+            } else {
+                const float tf = refineSagF(s, o, d, tc);
+                if (!(tf > tmin && tf < tmax)) continue;
+                tr = double(tf);
+                q = o + d * tf;
+            }
             if (!sagAccept(s, q)) continue;
             if (!((1.0f + s.kf) * s.cf * q.z <= 1.0f + 1e-6f)) continue;  // the conic's other sheet
             p = q;

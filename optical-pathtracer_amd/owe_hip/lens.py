@@ -145,7 +145,8 @@ class CLens(ctypes.Structure):
 class CLensSettings(ctypes.Structure):
     _fields_ = [("field_deg", ctypes.c_double), ("wavelengths", ctypes.c_double * 16), ("n_wavelengths", ctypes.c_int32),
                 ("detector_pixels", ctypes.c_int32), ("detector_half_size", ctypes.c_double),
-                ("reference_wavelength", ctypes.c_double)]
+                ("reference_wavelength", ctypes.c_double), ("focus_planes", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("focus_step", ctypes.c_double)]
 
 
 class CLensResult(ctypes.Structure):
@@ -487,13 +488,16 @@ def c_batch(lenses):
     return surf, arr
 
 
-def c_settings(field_deg=0.5, wavelengths=FRAUNHOFER7, pixels=128, half_size=0.0, reference=LAMBDA_D):
+def c_settings(field_deg=0.5, wavelengths=FRAUNHOFER7, pixels=128, half_size=0.0, reference=LAMBDA_D, planes=1,
+               focus_step=0.0):
+    """Batch settings; `planes` grids spaced `focus_step` m (0: 1e-3 × |EFL|) about the paraxial image."""
     s = CLensSettings()
     s.field_deg = field_deg
     for i, l in enumerate(wavelengths):
         s.wavelengths[i] = l
     s.n_wavelengths = len(wavelengths)
     s.detector_pixels, s.detector_half_size, s.reference_wavelength = pixels, half_size, reference
+    s.focus_planes, s.focus_step = planes, focus_step
     return s
 
 
@@ -591,9 +595,10 @@ class LensBatch:
 
     def results(self, irradiance: bool = False):
         n, nw, px = len(self.lenses), self.settings.n_wavelengths, self.settings.detector_pixels
+        planes = max(1, self.settings.focus_planes)
         res = (CLensResult * n)()
         mom = np.zeros((n, nw, MOMENTS))
-        irr = np.zeros((n, px, px), dtype=np.float32) if irradiance and px > 0 else None
+        irr = np.zeros((n, planes, px, px), dtype=np.float32) if irradiance and px > 0 else None
         _lib.check(_lib.load().owe_hip_lens_batch_results(self._h, res, mom.ctypes.data,
                                                           irr.ctypes.data if irr is not None else None))
         per = [{f: getattr(r, f) for f, _ in CLensResult._fields_} for r in res]

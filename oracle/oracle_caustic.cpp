@@ -4,7 +4,8 @@
 // path-replay gradient:
 //   bicubic Hermite patch over the node grid, node slopes by   WaveSurface::height, src/owe/scene/geometry.cpp:251-270,
 //   central differences of the heights (clamped at the edges)  hermiteBases, src/owe/core/shared/surfaces.hpp:190-197
-//   collimated particles, uniform over the entry face, a       emitDistant, particle.slang:234-265
+//   collimated particles, stratified over tiles of the entry   emitDistant, particle.slang:234-265
+//   face (uniform in each), a
 //   luminous spectral weight B(λ) ȳ(λ) / (pdf(λ) ∫ B ȳ)
 //   transmission with weight 1 − R at both faces, Snell out   optics.hpp:14-54; transport.cpp:531-595
 //   of the exit surface, the wall point, a quadratic B-spline splat over 3 × 3 pixels
@@ -30,13 +31,17 @@ struct Part {
 
 struct Slab {
     const owe_caustic_settings& s;
-    double cell, pix, invY, area;
+    double cell, pix, invY, tileArea;
+    uint32_t tiles;  // tiles per side of the entry face
     IndexModel slab, air;
     explicit Slab(const owe_caustic_settings& st) : s(st) {
         cell = 2 * s.slab_half / (s.nodes - 1);
         pix = 2 * s.wall_half / s.pixels;
         invY = 1 / integrateVisible([&](double l) { return planck(l, s.sun_kelvin) * cieY(l); });
-        area = (2 * s.slab_half) * (2 * s.slab_half) / (double(s.particles) * pix * pix);
+        tiles = 1;
+        while (tiles < 64 && uint64_t(4) * tiles * tiles * 64 <= s.particles) tiles *= 2;
+        const double side = 2 * s.slab_half / tiles;
+        tileArea = side * side / (pix * pix);
         slab.kind = s.medium.index_kind;
         slab.terms = s.medium.terms;
         slab.relAir = s.medium.rel_air != 0;
@@ -56,12 +61,26 @@ struct Slab {
         const double ul = u24(r);
         const double lambda = sampleVisible(ul), pdf = visiblePdf(lambda);
         const double u2 = u24(r), u1 = u24(r);
+        // Stratified over tiles × tiles tiles in Morton order: tile t holds particles
+        // [⌊t·N/T⌋, ⌊(t+1)·N/T⌋), uniform over the tile, weighted by the tile's area over its count.
+        const uint64_t T = uint64_t(tiles) * tiles, N = s.particles;
+        auto start = [&](uint64_t t) { return t * N / T; };
+        uint64_t t = i * T / N;
+        while (t + 1 < T && start(t + 1) <= i) ++t;
+        while (t > 0 && start(t) > i) --t;
+        uint32_t tx = 0, ty = 0;
+        for (int b = 0; b < 16; ++b) {
+            tx |= uint32_t((t >> (2 * b)) & 1) << b;
+            ty |= uint32_t((t >> (2 * b + 1)) & 1) << b;
+        }
+        const double side = 2 * s.slab_half / tiles;
         Part p;
-        p.x = (2 * u1 - 1) * s.slab_half;
-        p.y = (2 * u2 - 1) * s.slab_half;
+        p.x = -s.slab_half + (tx + u1) * side;
+        p.y = -s.slab_half + (ty + u2) * side;
         p.n1 = slab.n(lambda);
         p.n2 = air.n(lambda);
-        p.weight = planck(lambda, s.sun_kelvin) * cieY(lambda) / pdf * invY * (1 - fresnelDielectric(1, p.n1 / p.n2)) * area;
+        p.weight = planck(lambda, s.sun_kelvin) * cieY(lambda) / pdf * invY * (1 - fresnelDielectric(1, p.n1 / p.n2)) *
+                   tileArea / double(start(t + 1) - start(t));
         const double fx = (p.x + s.slab_half) / cell, fy = (p.y + s.slab_half) / cell;
         p.ci = std::min(std::max(int(fx), 0), int(s.nodes) - 2);
         p.cj = std::min(std::max(int(fy), 0), int(s.nodes) - 2);

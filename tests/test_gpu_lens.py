@@ -117,7 +117,7 @@ def test_irradiance_holds_the_flux_that_lands_and_is_reproducible(cuda):
     assert np.array_equal(irr, irr2)
     assert [r["arrived"] for r in per] == [r["arrived"] for r in per2]
     for li, p in enumerate(LENSES[:3]):
-        pix = 2 * half / irr.shape[1]
+        pix = 2 * half / irr.shape[-1]
         flux = irr[li].sum() * pix * pix
         beam = math.pi * (0.999 * per[li]["entrance_pupil_radius"]) ** 2 * math.cos(math.radians(0.5))
         print(f"lens {li}: flux on the grid {flux:.6e}, transmitted {per[li]['transmitted'] * beam:.6e}", flush=True)
@@ -140,3 +140,59 @@ def test_autotune_keeps_the_results(cuda):
         assert a["arrived"] == c["arrived"]
         assert c["rms_best"] == pytest.approx(a["rms_best"], rel=1e-9)
     b.close()
+
+
+def _oracle_grid(lo, li, lens, per, settings, planes, step, half, px):
+    """The through-focus grids binned in double from the oracle's rays: every arrived ray of every
+    wavelength carries T·π(0.999 r_EP)² cos θ / (rays · nλ) of flux into the bin its line crosses on
+    each plane (plane k at the paraxial image + (k − (P − 1)/2)·step)."""
+    r = per[li]
+    idx = np.arange(lens.rays, dtype=np.uint64)
+    theta = math.radians(settings.field_deg)
+    nw = settings.n_wavelengths
+    grid = np.zeros((planes, px, px))
+    z0 = r["last_vertex_z"] + r["bfl"]
+    cy = r["efl"] * math.tan(theta)
+    flux = math.pi * (0.999 * r["entrance_pupil_radius"]) ** 2 * math.cos(theta) / (lens.rays * nw)
+    pix = 2 * half / px
+    for w in range(nw):
+        t = lo.trace(li, settings.wavelengths[w], idx)
+        t = t[t[:, 6] > 0]
+        for k in range(planes):
+            z = z0 + (k - 0.5 * (planes - 1)) * step
+            s = (z - t[:, 2]) / t[:, 5]
+            gx = (t[:, 0] + t[:, 3] * s) / half
+            gy = (t[:, 1] + t[:, 4] * s - cy) / half
+            fx, fy = (gx + 1) * 0.5 * px, (1 - gy) * 0.5 * px
+            ok = (fx >= 0) & (fy >= 0) & (fx < px) & (fy < px)
+            np.add.at(grid[k], (fy[ok].astype(int), fx[ok].astype(int)), t[ok, 6] * flux / (pix * pix))
+    return grid
+
+
+def test_through_focus_stack_matches_the_oracle_bin_by_bin(cuda, oracle):
+    """Five planes 40 µm apart about the paraxial image of two lenses: every bin of every plane
+    against the oracle's rays binned in double (a ray within float rounding of a bin edge may land in
+    its neighbour); the middle plane equals a one-plane run bit for bit; the sharpest plane lies at, or next to, the plane nearest the best focus."""
+    lenses = [LENSES[1], LENSES[2]]
+    half, planes, step = 0.25e-3, 5, 40e-6
+    st = L.c_settings(half_size=half, planes=planes, focus_step=step, pixels=64)
+    b = L.LensBatch(lenses, st)
+    b.run()
+    per, _, irr = b.results(irradiance=True)
+    one = L.LensBatch(lenses, L.c_settings(half_size=half, pixels=64))
+    one.run()
+    _, _, irr1 = one.results(irradiance=True)
+    one.close()
+    b.close()
+    assert irr.shape == (2, planes, 64, 64)
+    assert np.array_equal(irr[:, planes // 2], irr1[:, 0])
+    lo = LensOracle(oracle, lenses)
+    for li, p in enumerate(lenses):
+        want = _oracle_grid(lo, li, p, per, st, planes, step, half, 64)
+        got = irr[li].astype(np.float64)
+        close = np.abs(got - want) <= 1e-5 * np.maximum(np.abs(want), np.abs(got)).max()
+        flux = got.sum(axis=(1, 2))
+        print(f"lens {li}: {close.mean():.5f} of bins agree; plane flux {flux / flux.max()}; peak per plane "
+              f"{got.max(axis=(1, 2))}", flush=True)
+        assert close.mean() > 0.995
+        assert np.abs(got.sum() - want.sum()) <= 1e-4 * want.sum()

@@ -113,3 +113,29 @@ def test_target_carries_its_flux():
     assert T.sum() * pix * pix == pytest.approx(0.92 * (2 * s.slab_half) ** 2, rel=1e-5)
     H = C.initial_heights()
     assert np.abs(H).max() == pytest.approx(1e-4, rel=1e-6)
+
+
+@pytest.mark.parametrize("particles", [1 << 17, 100_003])
+def test_stratified_particles_cover_the_entry_face_evenly(oracle, particles):
+    """Particles are stratified over tiles of the entry face (tile t holds ⌊t·N/T⌋ … ⌊(t+1)·N/T⌋ − 1,
+    uniform in the tile, weighted by its area over its own count). Through a flat slab each tile
+    lights exactly its own square of the wall, whether or not T divides N: the flux is the flat slab's,
+    and no tile's square of pixels inside the shadow is doubled or empty."""
+    s = C.caustic_settings(particles=particles)
+    E = CausticOracle(oracle, s).forward(np.zeros((s.nodes, s.nodes)))
+    pix = 2 * s.wall_half / s.pixels
+    expect = luminous_average(lambda l: (1 - fresnel_normal(pmma(l), air(l))) ** 2) * (2 * s.slab_half) ** 2
+    assert E.sum() * pix * pix == pytest.approx(expect, rel=5e-3)
+    tiles = 1
+    while tiles < 64 and 4 * tiles * tiles * 64 <= particles:
+        tiles *= 2
+    edge = int(round(s.slab_half / pix))        # pixels from the centre to the shadow's edge
+    step = int(round(2 * s.slab_half / tiles / pix))  # a tile's side in pixels
+    assert step * tiles == 2 * edge
+    c0 = s.pixels // 2
+    shadow = E[c0 - edge:c0 + edge, c0 - edge:c0 + edge]
+    blocks = shadow.reshape(tiles, step, tiles, step).sum(axis=(1, 3))[1:-1, 1:-1]  # away from the rim
+    # The spectral weights B ȳ / pdf spread widely (a few particles carry much), so a tile's flux varies
+    # by several per cent; a doubled or a missing tile is off by 100 %.
+    med = np.median(blocks)
+    assert blocks.min() > 0.5 * med and blocks.max() < 1.5 * med
