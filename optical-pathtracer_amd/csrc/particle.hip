@@ -219,12 +219,14 @@ __device__ __forceinline__ bool emitParticle(PCtx& c, bool observer, PRay& ray, 
             pdfDir = 1.0f / (4.0f * kPiF);
 #pragma unroll
             for (int i = 0; i < NW; ++i)
-                Le[i] = sc.env.sky == OWE_SKY_UNIFORM ? evalSpec(sc.env.skyRadiance, sc.table, c.wl.lambda[i]) : 0.0f;
+                Le[i] = sc.env.sky == OWE_SKY_UNIFORM && (i == 0 || c.wl.pdf[i] != 0.0f)
+                            ? evalSpec(sc.env.skyRadiance, sc.table, c.wl.lambda[i]) : 0.0f;
         } else {
             toward = toWorld(makeFrame(ld3(sc.env.sunDir)), uniformConeOmc(u1, u2, sc.env.sunOneMinusCos));
             pdfDir = 1.0f / sc.env.sunSolidAngle;
 #pragma unroll
-            for (int i = 0; i < NW; ++i) Le[i] = evalSpec(sc.env.sunRadiance, sc.table, c.wl.lambda[i]);
+            for (int i = 0; i < NW; ++i)
+                Le[i] = i == 0 || c.wl.pdf[i] != 0.0f ? evalSpec(sc.env.sunRadiance, sc.table, c.wl.lambda[i]) : 0.0f;
         }
         const f3 dir = -toward;
         const Frame disc = makeFrame(dir);
@@ -270,7 +272,8 @@ __device__ __forceinline__ bool emitParticle(PCtx& c, bool observer, PRay& ray, 
         const DBoundary& b = sc.boundaries[bi];
         const DEmission& em = sc.emissions[b.emission];
 #pragma unroll
-        for (int i = 0; i < NW; ++i) Le[i] = evalSpec(em.radiance, sc.table, c.wl.lambda[i]);
+        for (int i = 0; i < NW; ++i)
+            Le[i] = i == 0 || c.wl.pdf[i] != 0.0f ? evalSpec(em.radiance, sc.table, c.wl.lambda[i]) : 0.0f;
         const float u3 = c.rng.uniform(), u2 = c.rng.uniform(), u1 = c.rng.uniform();
         (void)u3;  // a mesh's triangle choice; meshes are not in the light table
         f3 pl, nl;

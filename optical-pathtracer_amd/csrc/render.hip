@@ -60,6 +60,11 @@ struct Paths {
     unsigned long long* segments;  // segments traced in the current chunk (sum of live counts)
 };
 
+// Whether wavelength i of the path still carries an estimate: the hero always; a secondary until a
+// dispersive event terminates it (never, with one wavelength per path). Spectra of wavelengths that
+// carry nothing are not evaluated (their throughput is zero).
+__device__ __forceinline__ bool carried(const WL& w, int i) { return i == 0 || w.pdf[i] != 0.0f; }
+
 __device__ __forceinline__ WL loadWL(const float4 l, uint32_t flags, const Settings& s) {
     WL w;
     w.lambda[0] = l.x;
@@ -289,7 +294,7 @@ __device__ __forceinline__ void prepareDirect(ShadeCtx& c, f3 p, bool surface, c
         const Frame f = makeFrame(ld3(sc.env.sunDir));
         wi = toWorld(f, uniformConeOmc(u1, u2, sc.env.sunOneMinusCos));
 #pragma unroll
-        for (int i = 0; i < NW; ++i) Le[i] = evalSpec(sc.env.sunRadiance, sc.table, c.wl.lambda[i]);
+        for (int i = 0; i < NW; ++i) Le[i] = carried(c.wl, i) ? evalSpec(sc.env.sunRadiance, sc.table, c.wl.lambda[i]) : 0.0f;
         lpdf = L.pdf / sc.env.sunSolidAngle;
     } else {
         const DBoundary& b = sc.boundaries[L.boundary];
@@ -305,7 +310,7 @@ __device__ __forceinline__ void prepareDirect(ShadeCtx& c, f3 p, bool surface, c
         const float cosL = dot3(en, -wi);
         const bool lit = (cosL > 0.0f && e.front) || (cosL < 0.0f && e.back);
 #pragma unroll
-        for (int i = 0; i < NW; ++i) Le[i] = lit ? evalSpec(e.radiance, sc.table, c.wl.lambda[i]) : 0.0f;
+        for (int i = 0; i < NW; ++i) Le[i] = lit && carried(c.wl, i) ? evalSpec(e.radiance, sc.table, c.wl.lambda[i]) : 0.0f;
         lpdf = L.pdf * pdfW;
         lightRegion = cosL > 0.0f ? b.front : b.back;
         exclude = L.boundary;
@@ -437,6 +442,7 @@ __device__ __forceinline__ bool shadeOne(ShadeCtx& c, const uint4 hraw, uint32_t
             if (inSun && !(c.flags & kFlagSpecular)) w = powerHeuristic(c.prevPdf, sc.env.sunPdf);
 #pragma unroll
             for (int i = 0; i < NW; ++i) {
+                if (!carried(c.wl, i)) continue;
                 const float sky = sc.env.sky == OWE_SKY_UNIFORM ? evalSpec(sc.env.skyRadiance, sc.table, c.wl.lambda[i]) : 0.0f;
                 const float sun = inSun ? evalSpec(sc.env.sunRadiance, sc.table, c.wl.lambda[i]) : 0.0f;
                 setc(c.L, i, get(c.L, i) + get(c.beta, i) * (sky + sun * w));
@@ -480,7 +486,8 @@ __device__ __forceinline__ bool shadeOne(ShadeCtx& c, const uint4 hraw, uint32_t
             }
 #pragma unroll
             for (int i = 0; i < NW; ++i)
-                setc(c.L, i, get(c.L, i) + get(c.beta, i) * evalSpec(e.radiance, sc.table, c.wl.lambda[i]) * w);
+                if (carried(c.wl, i))
+                    setc(c.L, i, get(c.L, i) + get(c.beta, i) * evalSpec(e.radiance, sc.table, c.wl.lambda[i]) * w);
         }
     }
     const uint32_t hitFace = hraw.w != kEmptyChild ? hraw.z : ~0u;

@@ -414,6 +414,27 @@ __device__ __forceinline__ bool sphereRoots(f3 o, f3 d, float R, float& t0, floa
     return true;
 }
 
+// Out-of-line forms of the large analytic-shape solvers for the world walk: one copy of each in a
+// kernel instead of one per inlined walk (a camera kernel walks for its path and for its shadow
+// ray, with and without contacts), which keeps the walk's hot code — BVH, planes, spheres — small
+// for the instruction cache. The lens batch inlines sagHit itself.
+static __device__ __noinline__ bool sagHitOut(const DSag& s, f3 o, f3 d, float tmin, float tmax, bool departed, float& t,
+                                       f3& p) {
+    return sagHit(s, o, d, tmin, tmax, departed, t, p);
+}
+static __device__ __noinline__ bool waveHitOut(const DWave& w, const float4* data, f3 o, f3 d, float tmin, float tmax,
+                                        bool departed, float depTol, float& t) {
+    return waveHit(w, data, o, d, tmin, tmax, departed, depTol, t);
+}
+static __device__ __noinline__ bool cylinderHitOut(float R, float z0, float z1, f3 o, f3 d, float tmin, float tmax,
+                                            bool departed, float& t) {
+    return cylinderHit(R, z0, z1, o, d, tmin, tmax, departed, t);
+}
+static __device__ __noinline__ f3 sagNormalOut(const DSag& s, f3 p) { return sagNormalF(s, p); }
+static __device__ __noinline__ f3 waveNormalOut(const DWave& w, const float4* data, float x, float y) {
+    return waveNormalF(w, data, x, y);
+}
+
 // The nearest crossing in (0, tmax). departed/departedFace: the spawning boundary and (mesh) face;
 // exclude: an analytic boundary the segment ends on (a sampled light point). kGeometry = false
 // leaves the point and normals unset (a caller that keeps only t, boundary, face and slot).
@@ -429,9 +450,9 @@ __device__ __forceinline__ void analyticGeometry(const DScene& sc, const DBounda
     } else if (b.shape == OWE_SHAPE_CYLINDER) {
         nl = norm3(mk3(pl.x, pl.y, 0.0f));
     } else if (b.shape == OWE_SHAPE_SAG) {
-        nl = sagNormalF(b.sag, pl);
+        nl = sagNormalOut(b.sag, pl);
     } else {
-        nl = waveNormalF(b.wave, sc.waveData, pl.x, pl.y);
+        nl = waveNormalOut(b.wave, sc.waveData, pl.x, pl.y);
     }
 }
 
@@ -488,13 +509,13 @@ __device__ __forceinline__ bool intersectWorld(const DScene& sc, f3 o, f3 d, flo
                 if (!(t > tmin && t < h.t)) continue;
             }
         } else if (b.shape == OWE_SHAPE_CYLINDER) {
-            if (!cylinderHit(b.a, b.z0, b.z1, lo3, ldv, tmin, h.t, own, t)) continue;
+            if (!cylinderHitOut(b.a, b.z0, b.z1, lo3, ldv, tmin, h.t, own, t)) continue;
         } else if (b.shape == OWE_SHAPE_SAG) {
             f3 q;
             const float smin = own && b.sag.nA != 0 ? fmaxf(b.depTol, tmin) : tmin;
-            if (!sagHit(b.sag, lo3, ldv, smin, h.t, own, t, q)) continue;
+            if (!sagHitOut(b.sag, lo3, ldv, smin, h.t, own, t, q)) continue;
         } else {
-            if (!waveHit(b.wave, sc.waveData, lo3, ldv, tmin, h.t, own, b.depTol, t)) continue;
+            if (!waveHitOut(b.wave, sc.waveData, lo3, ldv, tmin, h.t, own, b.depTol, t)) continue;
         }
         h.t = t;
         analytic = i;
