@@ -168,10 +168,22 @@ int owe_hip_renderer_get_mode(const owe_hip_renderer* renderer, int* mode, int* 
                               int* use_graph, int* waves_per_simd, int* refill);
 /* Times passes of spp samples per pixel under candidate launch shapes of the current form
  * (persistent: waves per SIMD × refill threshold × blocks per compute unit × chunk; wavefront: bounce
- * rounds between host reads of the
- * live counters), keeps the fastest, and clears the film and statistics again. Only on a renderer
- * without passes. best_ms (optional) receives the winning pass time. */
+ * rounds between host reads of the live counters × paths in flight, a quarter to all of the path
+ * slots), times the five fastest three more times each and keeps the lowest median, and clears the
+ * film and statistics again. Only on a renderer without passes. best_ms (optional) receives the
+ * winning pass time. With a tuning cache set (below) a recorded shape is applied without timing. */
 int owe_hip_renderer_autotune(owe_hip_renderer* renderer, int spp, double* best_ms);
+
+/* The autotuner's per-device tuning cache: a text file of tuned shapes, one line per (device,
+ * form, pass size, flattened scene, settings) key. owe_hip_renderer_autotune first looks its key
+ * up there (a hit applies the recorded shape without sweeping and reports its recorded time) and
+ * appends what a sweep chooses. path NULL or "" disables it; the default is the environment
+ * variable OWE_HIP_TUNING_CACHE, else none. Process-wide. */
+int owe_hip_set_tuning_cache(const char* path);
+/* The wavefront form's tuned bounce rounds per host sync and paths in flight, and whether the last
+ * autotune came from the cache. */
+int owe_hip_renderer_tuning(const owe_hip_renderer* renderer, int* rounds_per_sync, uint64_t* queue_capacity,
+                            int* from_cache);
 /* Per-kernel timing with HIP events on the renderer's stream (the reference's OWE_GPU_PROFILE,
  * src/owe/backends/gpu/vulkan_context.cpp:350-422). Enabling clears the totals. */
 int owe_hip_renderer_set_timing(owe_hip_renderer* renderer, int enable);

@@ -937,22 +937,36 @@ int owe_hip_lens_batch_autotune(owe_hip_lens_batch* b, double* best_ms) {
     return guard([&] {
         require(b != nullptr, "batch is null");
         OWE_HIP_CHECK(hipSetDevice(b->device));
-        double best = INFINITY;
-        int bestB = b->blocksPerCU;
-        uint32_t bestR = b->args.raysPerItem;
+        // Every shape is timed once; the three fastest three more times each, and the lowest median
+        // wins, so one noisy run cannot pick the shape.
+        struct Cand {
+            int blocks;
+            uint32_t rays;
+            double ms;
+        };
+        std::vector<Cand> cands;
+        auto run = [&](const Cand& c) {
+            b->blocksPerCU = c.blocks;
+            layoutItems(b, c.rays);
+            return runOnce(b);
+        };
         for (int blocks : {1, 2, 4, 8})
             for (uint32_t rays : {256u, 1024u, 4096u, 16384u}) {
-                b->blocksPerCU = blocks;
-                layoutItems(b, rays);
-                const double ms = runOnce(b);
-                if (ms < best) {
-                    best = ms;
-                    bestB = blocks;
-                    bestR = rays;
-                }
+                cands.push_back({blocks, rays, 0.0});
+                cands.back().ms = run(cands.back());
             }
-        b->blocksPerCU = bestB;
-        layoutItems(b, bestR);
+        std::sort(cands.begin(), cands.end(), [](const Cand& x, const Cand& y) { return x.ms < y.ms; });
+        for (size_t i = 0; i < 3 && i < cands.size(); ++i) {
+            double t[3];
+            for (double& x : t) x = run(cands[i]);
+            std::sort(t, t + 3);
+            cands[i].ms = t[1];
+        }
+        std::sort(cands.begin(), cands.begin() + std::min<size_t>(3, cands.size()),
+                  [](const Cand& x, const Cand& y) { return x.ms < y.ms; });
+        const double best = cands.front().ms;
+        b->blocksPerCU = cands.front().blocks;
+        layoutItems(b, cands.front().rays);
         b->ran = false;
         if (best_ms) *best_ms = best;
     });

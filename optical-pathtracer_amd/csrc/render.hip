@@ -19,7 +19,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1118,6 +1121,8 @@ struct owe_hip_renderer {
     std::vector<void*> allocations;
     Paths P{};
     uint32_t capacity = 0;  // path slots
+    uint32_t queueCapacity = 0;  // wavefront paths in flight per chunk (≤ capacity; tuned)
+    bool tunedFromCache = false;
     unsigned long long* film = nullptr;
     double* perPath = nullptr;
     uint32_t* pathPixels = nullptr;
@@ -1755,8 +1760,9 @@ int owe_hip_renderer_pass(owe_hip_renderer* r, int spp) {
         if (r->mode == OWE_HIP_MODE_PERSISTENT) {
             runPersistent(r, uint32_t(spp), uint32_t(r->passes));
         } else {
-            for (uint64_t base = 0; base < total; base += r->capacity) {
-                const uint32_t count = uint32_t(std::min<uint64_t>(r->capacity, total - base));
+            const uint32_t cap = r->queueCapacity ? r->queueCapacity : r->capacity;
+            for (uint64_t base = 0; base < total; base += cap) {
+                const uint32_t count = uint32_t(std::min<uint64_t>(cap, total - base));
                 runPaths(r, base, count, uint32_t(spp), uint32_t(r->passes), false, nullptr);
             }
         }
@@ -1973,6 +1979,138 @@ int owe_hip_renderer_get_mode(const owe_hip_renderer* r, int* mode, int* blocks_
     });
 }
 
+}  // extern "C"
+
+namespace owehip {
+namespace {
+
+// ---------------------------------------------------------------- the autotuner's tuning cache
+
+// A tuned launch shape: the form, the persistent kernel's blocks per CU, path ids per fetch,
+// waves per SIMD and refill threshold, the wavefront's bounce rounds per host sync and paths in
+// flight (0: keep), and the pass time it was chosen at.
+struct TuningRecord {
+    int mode = 0, blocksPerCU = 0, chunk = 0, waves = 0, regen = 0, rounds = 0;
+    uint32_t queue = 0;
+    double ms = 0;
+};
+
+void applyTuning(owe_hip_renderer* r, const TuningRecord& t) {
+    if (t.mode == OWE_HIP_MODE_PERSISTENT) {
+        r->blocksPerCU = t.blocksPerCU;
+        r->chunk = uint32_t(t.chunk);
+        r->persistWaves = t.waves;
+        r->regen = uint32_t(t.regen);
+    } else {
+        if (t.rounds > 0) r->roundsPerSync = t.rounds;
+        if (t.queue > 0) r->queueCapacity = std::min(t.queue, r->capacity);
+    }
+}
+
+uint64_t fnv1a(uint64_t h, const void* data, size_t n) {
+    const unsigned char* p = static_cast<const unsigned char*>(data);
+    for (size_t i = 0; i < n; ++i) h = (h ^ p[i]) * 0x100000001b3ull;
+    return h;
+}
+template <class T>
+uint64_t fnvVec(uint64_t h, const std::vector<T>& v) {
+    const uint64_t n = v.size();
+    h = fnv1a(h, &n, sizeof n);
+    return v.empty() ? h : fnv1a(h, v.data(), v.size() * sizeof(T));
+}
+
+// The cache key: the device (architecture, compute units), the form, the pass size and the scene
+// as flattened for the device (every table the kernels read, the hierarchy's size), the settings
+// that change the work (integrator, spectral mode, depths) — not the seed.
+std::string tuningKey(const owe_hip_renderer* r, int spp) {
+    hipDeviceProp_t p;
+    OWE_HIP_CHECK(hipGetDeviceProperties(&p, r->device));
+    uint64_t h = 0xcbf29ce484222325ull;
+    h = fnv1a(h, p.gcnArchName, std::strlen(p.gcnArchName));
+    const int32_t head[] = {p.multiProcessorCount, r->mode, spp, r->host.obs.width, r->host.obs.height,
+                            r->integrator, r->set.spectral, r->set.maxDepth, r->set.rrDepth,
+                            int32_t(r->host.contacts), int32_t(r->simpleShadows), int32_t(r->stackCap),
+                            int32_t(r->capacity)};
+    h = fnv1a(h, head, sizeof head);
+    h = fnvVec(h, r->host.media);
+    h = fnvVec(h, r->host.regionMedium);
+    h = fnvVec(h, r->host.optics);
+    h = fnvVec(h, r->host.emissions);
+    h = fnvVec(h, r->host.boundaries);
+    h = fnvVec(h, r->host.lights);
+    h = fnvVec(h, r->host.table);
+    h = fnvVec(h, r->host.waveData);
+    h = fnvVec(h, r->host.bvh.nodes);
+    h = fnvVec(h, r->host.bvh.tris);
+    h = fnv1a(h, &r->host.obs, sizeof r->host.obs);
+    char buf[40];
+    std::snprintf(buf, sizeof buf, "owe-hip-tune-1-%016llx", static_cast<unsigned long long>(h));
+    return buf;
+}
+
+std::mutex& tuningMutex() {
+    static std::mutex m;
+    return m;
+}
+// The cache file: set by owe_hip_set_tuning_cache, else $OWE_HIP_TUNING_CACHE, else none.
+std::string& tuningPath() {
+    static std::string path = [] {
+        const char* e = std::getenv("OWE_HIP_TUNING_CACHE");
+        return std::string(e ? e : "");
+    }();
+    return path;
+}
+
+// One line per record: key mode blocks chunk waves regen rounds queue ms; the last line of a key wins.
+bool tuningLookup(const std::string& key, TuningRecord& out) {
+    std::lock_guard<std::mutex> lock(tuningMutex());
+    if (tuningPath().empty()) return false;
+    std::FILE* f = std::fopen(tuningPath().c_str(), "r");
+    if (!f) return false;
+    char k[64];
+    TuningRecord t;
+    bool found = false;
+    while (std::fscanf(f, "%63s %d %d %d %d %d %d %u %lf", k, &t.mode, &t.blocksPerCU, &t.chunk, &t.waves, &t.regen,
+                       &t.rounds, &t.queue, &t.ms) == 9)
+        if (key == k) {
+            out = t;
+            found = true;
+        }
+    std::fclose(f);
+    return found && (out.mode == OWE_HIP_MODE_PERSISTENT || out.mode == OWE_HIP_MODE_WAVEFRONT);
+}
+
+void tuningStore(const std::string& key, const TuningRecord& t) {
+    std::lock_guard<std::mutex> lock(tuningMutex());
+    if (tuningPath().empty()) return;
+    std::FILE* f = std::fopen(tuningPath().c_str(), "a");
+    if (!f) return;  // an unwritable cache only costs the next run its sweep
+    std::fprintf(f, "%s %d %d %d %d %d %d %u %.6f\n", key.c_str(), t.mode, t.blocksPerCU, t.chunk, t.waves, t.regen,
+                 t.rounds, t.queue, t.ms);
+    std::fclose(f);
+}
+
+}  // namespace
+}  // namespace owehip
+
+extern "C" {
+
+int owe_hip_set_tuning_cache(const char* path) {
+    return guard([&] {
+        std::lock_guard<std::mutex> lock(tuningMutex());
+        tuningPath() = path ? path : "";
+    });
+}
+
+int owe_hip_renderer_tuning(const owe_hip_renderer* r, int* rounds_per_sync, uint64_t* queue_capacity, int* from_cache) {
+    return guard([&] {
+        require(r != nullptr, "renderer is null");
+        if (rounds_per_sync) *rounds_per_sync = r->roundsPerSync;
+        if (queue_capacity) *queue_capacity = r->queueCapacity ? r->queueCapacity : r->capacity;
+        if (from_cache) *from_cache = r->tunedFromCache ? 1 : 0;
+    });
+}
+
 int owe_hip_renderer_autotune(owe_hip_renderer* r, int spp, double* best_ms) {
     return guard([&] {
         require(r != nullptr, "renderer is null");
@@ -1980,13 +2118,20 @@ int owe_hip_renderer_autotune(owe_hip_renderer* r, int spp, double* best_ms) {
         require(r->integrator == OWE_INTEGRATOR_PATH, "the autotuner tunes the camera-path forms");
         require(r->passes == 0, "autotune a fresh (or reset) renderer: its passes would enter the film");
         OWE_HIP_CHECK(hipSetDevice(r->device));
+        const std::string key = tuningKey(r, spp);
+        TuningRecord rec;
+        r->tunedFromCache = false;
+        if (tuningLookup(key, rec)) {
+            applyTuning(r, rec);
+            r->tunedFromCache = true;
+            if (best_ms) *best_ms = rec.ms;
+            return;
+        }
         hipEvent_t e0, e1;
         OWE_HIP_CHECK(hipEventCreate(&e0));
         OWE_HIP_CHECK(hipEventCreate(&e1));
         const bool wasTiming = r->timing;
         r->timing = false;
-        double best = INFINITY;
-        int bestA = 0, bestB = 0;
         auto timePass = [&](auto&& run) {
             OWE_HIP_CHECK(hipEventRecord(e0, r->stream));
             run();
@@ -1997,52 +2142,53 @@ int owe_hip_renderer_autotune(owe_hip_renderer* r, int spp, double* best_ms) {
             return double(ms);
         };
         const uint64_t total = uint64_t(r->host.obs.width) * uint64_t(r->host.obs.height) * uint64_t(spp);
+        // Every candidate is timed once; the five fastest are timed three more times each and the
+        // one with the lowest median wins, so a single noisy pass cannot pick the shape.
+        std::vector<TuningRecord> cands;
+        const bool graph = r->useGraph;
+        auto run = [&](const TuningRecord& c) {
+            applyTuning(r, c);
+            if (r->mode == OWE_HIP_MODE_PERSISTENT) {
+                hipLaunchKernelGGL(setPass, dim3{1}, dim3{64}, 0, r->stream, r->passSlot, 0u);
+                return timePass([&] { launchPersistent(r, persistArgs(r, total, uint32_t(spp))); });
+            }
+            const uint32_t cap = r->queueCapacity;
+            return timePass([&] {
+                for (uint64_t base = 0; base < total; base += cap)
+                    runPaths(r, base, uint32_t(std::min<uint64_t>(cap, total - base)), uint32_t(spp), 0u, false, nullptr);
+            });
+        };
         if (r->mode == OWE_HIP_MODE_PERSISTENT) {
-            // Register budget (waves per SIMD) × blocks per compute unit × path ids per lane fetch.
-            const bool graph = r->useGraph;
+            // Register budget (waves per SIMD) × refill threshold × blocks per compute unit × path
+            // ids per lane fetch.
             r->useGraph = false;
-            int bestW = 2, bestR = 1;
             for (int w : {2, 3})
                 for (int g : {1, 16, 32, 48})
                     for (int b : {1, 2, 3, 4, 8})
-                        for (int c : {1, 4, 16}) {
-                            r->persistWaves = w;
-                            r->regen = uint32_t(g);
-                            r->blocksPerCU = b;
-                            r->chunk = uint32_t(c);
-                            hipLaunchKernelGGL(setPass, dim3{1}, dim3{64}, 0, r->stream, r->passSlot, 0u);
-                            const double ms =
-                                timePass([&] { launchPersistent(r, persistArgs(r, total, uint32_t(spp))); });
-                            if (ms < best) {
-                                best = ms;
-                                bestA = b;
-                                bestB = c;
-                                bestW = w;
-                                bestR = g;
-                            }
-                        }
-            r->persistWaves = bestW;
-            r->regen = uint32_t(bestR);
-            r->blocksPerCU = bestA;
-            r->chunk = uint32_t(bestB);
-            r->useGraph = graph;
-            dropGraph(r);
+                        for (int c : {1, 4, 16}) cands.push_back({r->mode, b, c, w, g, 0, 0, 0.0});
         } else {
-            // Bounce rounds enqueued between host reads of the live counters.
-            for (int k : {1, 2, 4, 8}) {
-                r->roundsPerSync = k;
-                const double ms = timePass([&] {
-                    for (uint64_t base = 0; base < total; base += r->capacity)
-                        runPaths(r, base, uint32_t(std::min<uint64_t>(r->capacity, total - base)), uint32_t(spp), 0u,
-                                 false, nullptr);
-                });
-                if (ms < best) {
-                    best = ms;
-                    bestA = k;
-                }
-            }
-            r->roundsPerSync = bestA;
+            // Bounce rounds enqueued between host reads of the live counters × paths in flight.
+            for (int k : {1, 2, 4, 8})
+                for (uint32_t q : {r->capacity / 4, r->capacity / 2, r->capacity})
+                    if (q >= 1024) cands.push_back({r->mode, r->blocksPerCU, int(r->chunk), r->persistWaves, int(r->regen), k, q, 0.0});
         }
+        for (TuningRecord& c : cands) c.ms = run(c);
+        std::sort(cands.begin(), cands.end(), [](const TuningRecord& a, const TuningRecord& b) { return a.ms < b.ms; });
+        const size_t finalists = std::min<size_t>(5, cands.size());
+        for (size_t i = 0; i < finalists; ++i) {
+            double t[3];
+            for (double& x : t) x = run(cands[i]);
+            std::sort(t, t + 3);
+            cands[i].ms = t[1];
+        }
+        std::sort(cands.begin(), cands.begin() + finalists,
+                  [](const TuningRecord& a, const TuningRecord& b) { return a.ms < b.ms; });
+        rec = cands.front();
+        applyTuning(r, rec);
+        r->useGraph = graph;
+        dropGraph(r);
+        tuningStore(key, rec);
+        const double best = rec.ms;
         OWE_HIP_CHECK(hipEventDestroy(e0));
         OWE_HIP_CHECK(hipEventDestroy(e1));
         r->timing = wasTiming;

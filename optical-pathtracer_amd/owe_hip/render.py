@@ -16,6 +16,11 @@ from . import _lib
 from .scene import CRenderSettings, CTransportStats, Scene, settings as make_settings
 
 
+def set_tuning_cache(path) -> None:
+    """The autotuner's per-device tuning cache file (None: none); process-wide."""
+    _lib.check(_lib.load().owe_hip_set_tuning_cache(None if path is None else str(path).encode()))
+
+
 class Renderer:
     def __init__(self, scene: Scene, settings: CRenderSettings | None = None, device: int = 0):
         self.scene = scene
@@ -87,6 +92,13 @@ class Renderer:
         ms = ctypes.c_double()
         _lib.check(_lib.load().owe_hip_renderer_autotune(self._h, int(spp), ctypes.byref(ms)))
         return ms.value
+
+    def tuning(self) -> dict:
+        """The wavefront form's tuned bounce rounds per sync and paths in flight; whether the last
+        autotune was a tuning-cache hit."""
+        k, q, c = ctypes.c_int(), ctypes.c_uint64(), ctypes.c_int()
+        _lib.check(_lib.load().owe_hip_renderer_tuning(self._h, ctypes.byref(k), ctypes.byref(q), ctypes.byref(c)))
+        return {"rounds_per_sync": k.value, "queue_capacity": q.value, "from_cache": bool(c.value)}
 
     def set_timing(self, enable: bool = True) -> None:
         _lib.check(_lib.load().owe_hip_renderer_set_timing(self._h, int(enable)))
