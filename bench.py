@@ -833,9 +833,10 @@ def bench_telescope_render(args, variants_rendered: int = 4):
 
 
 def bench_scenes(args):
-    """The scene-file suite (owe bench): the package's example scenes read from .owe files, one pass of
-    16 camera paths per pixel at 256x192 each in the persistent form, best of 3 fresh renderers; the
-    aggregate paths/s and each scene's digest (FNV-1a of its raw XYZ image)."""
+    """The scene-file suite (owe bench): the reference's five benchmark scenes and the rippled pool read
+    from .owe files, one pass of 16 camera paths per pixel at 256x192 each in the persistent form,
+    best of 3 fresh renderers; the aggregate paths/s and each scene's digest (FNV-1a of its raw XYZ
+    image)."""
     from owe_hip import render, suite
 
     res = suite.run(spp=16, resolution=(256, 192), repeat=3, mode=render.Renderer.PERSISTENT)
@@ -843,8 +844,9 @@ def bench_scenes(args):
         print(f"scenes: {Path(r['scene']).name:<20s} {r['mpaths_per_s']:9.1f} M paths/s  {r['msegs_per_s']:9.1f} "
               f"M segments/s  mean Y {r['mean_Y']:.6g}  digest {r['digest']}", file=sys.stderr, flush=True)
     return {"value": res["mpaths_per_s"] * 1e6, "unit": "paths/s (suite aggregate)",
-            "config": {"workload": "scene files through the loader: prism, lens caustic through glass, caustic in a "
-                                   "mirror, rippled pool, slabs behind a stop, refractor at its exit pupil",
+            "config": {"workload": "scene files through the loader: the reference's benchmarks dispersion_prism, "
+                                   "caustic_seen_through_glass, caustic_seen_in_mirror, double_refraction_slabs, "
+                                   "telescope_exit_pupil_walk, and a rippled pool",
                        "resolution": [256, 192], "spp": 16, "form": "persistent", "timing": "best of 3 passes"},
             "rows": [{k: r[k] for k in ("scene", "detector", "mpaths_per_s", "msegs_per_s", "mean_Y", "digest",
                                         "inconsistencies", "leaks")} | {"scene": Path(r["scene"]).name}

@@ -8,7 +8,7 @@ image, so that a change that moves any result bit shows at once (owe_hip.output.
 
 ``python -m owe_hip.suite [scene.owe[:Detector] …] [--spp 16] [--resolution 256x192] [--repeat 1]
 [--set Block.key=value …] [--json file]`` prints the reference's table and, with --json, its records.
-Without scene arguments it runs the package's example scenes.
+Without scene arguments it runs the reference's benchmark scenes and the rippled pool.
 """
 from __future__ import annotations
 
@@ -22,8 +22,10 @@ from . import loader as LD
 from . import output as O
 
 SCENES = Path(__file__).parent / "scenes"
+# The reference's five benchmark scenes (owe_hip.reference_cases) and the package's rippled pool.
 DEFAULT_CASES = [(str(SCENES / f"{n}.owe"), "") for n in
-                 ("prism_bench", "lens_caustic", "mirror_caustic", "pool", "slabs", "refractor")]
+                 ("dispersion_prism", "caustic_seen_through_glass", "caustic_seen_in_mirror", "double_refraction_slabs",
+                  "telescope_exit_pupil_walk", "pool")]
 
 
 def split_spec(spec: str):

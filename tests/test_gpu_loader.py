@@ -1,10 +1,10 @@
 """Scenes read from `.owe` files render on the GPU as the oracle renders them.
 
-The package's example scenes (owe_hip/scenes: a prism in low sunlight with cards and a screen, a lens
-caustic seen through a tilted glass plate, the caustic seen in a flat mirror beside a concave mirror
-and a lamp, a rippled pool in a stone basin, two tilted slabs behind a stop, and an afocal refractor in
-a tube read from a `.lens` file with the eye at its exit pupil) are loaded through owe_hip.loader and
-rendered at reduced resolution:
+The scene-file benchmark's cases (owe_hip/scenes: the reference's five benchmark scenes — the prism
+in low sunlight between black cards, the lens caustic seen through a tilted plate and in a flat
+mirror, the two slabs behind a stop, the afocal refractor in its tube read from a `.lens` file with
+the eye at its exit pupil — and the package's rippled pool) are loaded through owe_hip.loader and
+rendered with camera paths at reduced resolution:
 
 * per path: the same (seed, pixel, sample, pass) camera streams through the float32 kernels and the
   fp64 oracle agree to 1e-3 on at least 99 % of paths (the rest are float decisions that fall the
@@ -27,7 +27,9 @@ from pyoracle import World
 pytestmark = pytest.mark.gpu
 
 SCENES = Path(LD.__file__).parent / "scenes"
-CASES = {"prism_bench": 4, "lens_caustic": 2, "mirror_caustic": 2, "pool": 3, "slabs": 3, "refractor": 1}
+# Each scene at 1/(2k) of its own resolution.
+CASES = {"dispersion_prism": 4, "caustic_seen_through_glass": 2, "caustic_seen_in_mirror": 2, "pool": 3,
+         "double_refraction_slabs": 3, "telescope_exit_pupil_walk": 1}
 
 
 def _scene(name):
@@ -40,7 +42,7 @@ def _scene(name):
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_loaded_scene_paths_match_the_oracle(cuda, oracle, name):
     ls, sc = _scene(name)
-    st = ls.settings(seed=3)
+    st = ls.settings(integrator="path", seed=3)
     w, h = sc.observer["width"], sc.observer["height"]
     spp = 8
     pixel = np.repeat(np.arange(w * h, dtype=np.uint32), spp)
@@ -63,7 +65,7 @@ def test_loaded_scene_paths_match_the_oracle(cuda, oracle, name):
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_loaded_scene_forms_are_bit_identical(cuda, name):
     ls, sc = _scene(name)
-    st = ls.settings(seed=5)
+    st = ls.settings(integrator="path", seed=5)
     digests = []
     imgs = []
     for mode in (render.Renderer.WAVEFRONT, render.Renderer.PERSISTENT):

@@ -45,9 +45,9 @@ def _case(case):
 
 
 # pool: visible points seen through rippled, absorbing water in a basin (chromatic, bound points, contacts);
-# lens_caustic: a caustic seen through a tilted glass plate.
+# caustic_seen_through_glass: the reference's benchmark scene of a lens caustic seen through a tilted plate.
 # direct over 3 iterations: the second and third draw sun photons from the guide the first taught.
-@pytest.mark.parametrize("case,iterations", [("direct", 1), ("plate", 1), ("pool", 1), ("lens_caustic", 1),
+@pytest.mark.parametrize("case,iterations", [("direct", 1), ("plate", 1), ("pool", 1), ("caustic_seen_through_glass", 1),
                                              ("direct", 3)])
 def test_sppm_first_iteration_matches_the_oracle(cuda, oracle, case, iterations):
     ls, sc = _case(case)
