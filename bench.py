@@ -681,24 +681,9 @@ def bench_caustic(args):
 
 
 def telescope_variants(n: int = 64, rings: int = 200, seed: int = 5):
-    """n perturbed copies of the reference's Gregorian (lenses/gregorian_54x.lens): both mirrors'
-    radii ±0.2 %, the secondary's conic ±0.002, the ocular's radii ±0.5 %, each re-solved afocal."""
-    import numpy as np
-
     from owe_hip import lens as L
 
-    base = L.solve_afocal(L.GREGORIAN, 587.5618).with_rings(rings)
-    rng = np.random.default_rng(seed)
-    out = []
-    for _ in range(n):
-        sf = [L.Surface(**{**q.__dict__}) for q in base.surfaces]
-        sf[0].R *= 1 + rng.uniform(-2e-3, 2e-3)
-        sf[1].R *= 1 + rng.uniform(-2e-3, 2e-3)
-        sf[1].k += rng.uniform(-2e-3, 2e-3)
-        sf[2].R *= 1 + rng.uniform(-5e-3, 5e-3)
-        sf[4].R *= 1 + rng.uniform(-5e-3, 5e-3)
-        out.append(L.solve_afocal(L.Prescription(sf, base.ambient, base.rays, base.name, base.rings), 587.5618))
-    return out
+    return L.gregorian_variants(n, rings, seed)
 
 
 TELESCOPE_PARAMS = [(0, 0), (0, 1), (1, 0), (1, 1), (0, 2), (1, 2), (2, 0), (4, 0)]
@@ -765,52 +750,69 @@ def bench_telescope(args):
     return out
 
 
-def bench_telescope_render(args, variants_rendered: int = 4):
-    """The differentiable telescope's rendering half: perturbed Gregorians as matter, each seen
-    through its ocular in a closed room with a 2,009,780-triangle statue, 512×512 at 256 spp, depth
-    32, by the persistent form (autotuned on the first variant). One step renders
-    `variants_rendered` variants; the full workload is 64."""
-    import numpy as np
+TELESCOPE_VARIANTS, TELESCOPE_SPP, TELESCOPE_PASS_SPP = 64, 256, 16
 
+
+def bench_telescope_render(args):
+    """The differentiable telescope's rendering half: the 64 perturbed Gregorians as matter, each seen
+    through its ocular in a closed room with a 2,009,780-triangle statue, 512×512, depth 32, as ONE
+    batch (owe_hip_batch): every pass renders all 64 variants in one persistent launch. A step is one
+    pass of 16 samples per pixel of every variant (the full 256-spp workload is 16 such passes, the
+    same launch repeated); the launch shape is autotuned on the first variant. Beside it, the first
+    four variants rendered one renderer at a time with the same shape, the loop the batch replaces."""
     from owe_hip import bodies as B
     from owe_hip import render
     from owe_hip import scene as S
 
-    variants = telescope_variants(n=variants_rendered)
+    variants = telescope_variants(n=TELESCOPE_VARIANTS)
     st = S.settings(seed=1, max_depth=32, rr_depth=4)
-    size, spp = 512, 256
-    total_ms, paths, segments, shape = 0.0, 0, 0, None
-    build_s = 0.0
-    for vi, v in enumerate(variants):
-        t0 = time.perf_counter()
-        sc = B.telescope_room_scene(v, size=size)
-        r = render.Renderer(sc, st)
-        build_s += time.perf_counter() - t0
-        r.set_mode(render.Renderer.PERSISTENT, use_graph=True)
-        if shape is None:
-            r.autotune(64)
-            shape = r.mode()
-        else:
-            r.set_mode(render.Renderer.PERSISTENT, shape["blocks_per_cu"], shape["chunk"], True, shape["waves_per_simd"],
-                       shape["refill"])
-        r.set_timing(True)
-        r.run_pass(spp)
-        ms, _ = r.kernel_times()["persistent"]
-        total_ms += ms
-        stt = r.stats()
-        paths += stt["paths"]
-        segments += stt["segments"]
-        r.close()
-    log(f"telescope render: {variants_rendered} variants x {size}x{size}x{spp}: {paths / (total_ms * 1e-3) / 1e6:.1f} M "
-        f"paths/s; E_seg {segments / paths:.2f}; {total_ms / variants_rendered:.1f} ms/variant; scenes built in "
-        f"{build_s:.1f} s; shape {shape}")
-    out = {"value": paths / (total_ms * 1e-3), "unit": "spectral paths/s",
-           "ms_per_step": total_ms,
-           "config": {"workload": "Gregorian telescope as matter in a 2,009,780-triangle room, 512x512x256 spp, "
-                                  "depth 32, persistent form", "variants_rendered": variants_rendered,
-                      "variants_full_workload": 64, "paths_per_variant": size * size * spp,
-                      "segments_per_path": segments / paths, "ms_per_variant": total_ms / variants_rendered,
-                      "launch": shape}}
+    size = 512
+    t0 = time.perf_counter()
+    scenes = [B.telescope_room_scene(v, size=size) for v in variants]
+    t1 = time.perf_counter()
+    batch = render.Batch(scenes, st)
+    t2 = time.perf_counter()
+    lead = batch.members[0]
+    lead.set_mode(render.Renderer.PERSISTENT, use_graph=False)
+    lead.autotune(TELESCOPE_PASS_SPP)
+    shape = lead.mode()
+    batch.run_pass(TELESCOPE_PASS_SPP)  # warm-up
+    batch.set_timing(True)
+    before = batch.stats()
+    steps = max(1, min(args.steps, 2))
+    for _ in range(steps):
+        batch.run_pass(TELESCOPE_PASS_SPP)
+    ms, launches = batch.kernel_time()
+    after = batch.stats()
+    paths = after["paths"] - before["paths"]
+    segments = after["segments"] - before["segments"]
+    # The loop the batch replaces: the first variants one renderer at a time, same shape and pass.
+    loop_ms, loop_paths = 0.0, 0
+    for m in batch.members[:4]:
+        m.set_mode(render.Renderer.PERSISTENT, shape["blocks_per_cu"], shape["chunk"], False, shape["waves_per_simd"],
+                   shape["refill"])
+        m.set_timing(True)
+        m.run_pass(TELESCOPE_PASS_SPP)
+        loop_ms += m.kernel_times()["persistent"][0]
+        loop_paths += size * size * TELESCOPE_PASS_SPP
+    batch_rate = paths / (ms * 1e-3)
+    loop_rate = loop_paths / (loop_ms * 1e-3)
+    log(f"telescope render: {TELESCOPE_VARIANTS} variants x {size}x{size}x{TELESCOPE_PASS_SPP} spp per launch: "
+        f"{batch_rate / 1e6:.1f} M paths/s batched ({ms / launches:.1f} ms/launch), {loop_rate / 1e6:.1f} M paths/s "
+        f"one variant at a time; E_seg {segments / paths:.2f}; scenes built in {t1 - t0:.1f} s, batch created in "
+        f"{t2 - t1:.1f} s; shape {shape}; {after}")
+    out = {"value": batch_rate, "unit": "spectral paths/s",
+           "ms_per_step": ms / launches,
+           "config": {"workload": "64 perturbed Gregorian telescopes as matter in a 2,009,780-triangle room, "
+                                  "512x512, depth 32, all 64 in one persistent launch per pass (owe_hip_batch)",
+                      "variants": TELESCOPE_VARIANTS, "spp_per_launch": TELESCOPE_PASS_SPP,
+                      "spp_full_workload": TELESCOPE_SPP, "paths_per_launch": paths // launches,
+                      "full_workload_ms": ms / launches * TELESCOPE_SPP / TELESCOPE_PASS_SPP,
+                      "segments_per_path": segments / paths, "launch": shape,
+                      "one_at_a_time_paths_per_s": loop_rate, "batched_over_one_at_a_time": batch_rate / loop_rate,
+                      "inconsistencies": after["inconsistencies"], "leaks": after["leaks"],
+                      "host_build_s": {"scenes": t1 - t0, "batch": t2 - t1}}}
+    batch.close()
     if not args.no_cpu_baseline:
         sys.path.insert(0, str(ROOT / "oracle"))
         from pyoracle import Oracle, World

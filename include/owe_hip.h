@@ -192,6 +192,31 @@ int owe_hip_renderer_set_timing(owe_hip_renderer* renderer, int enable);
  * kernel as shade. */
 int owe_hip_renderer_kernel_times(const owe_hip_renderer* renderer, double* ms, uint64_t* launches);
 
+/* ------------------------------------------------------------------ batches of scenes
+ * Many variants of one scene family (perturbed instruments, tolerancing runs) rendered with camera
+ * paths as one persistent launch per pass: the launch's waves drain the first scene's pass, then the
+ * next one's, each scene's paths from its own id counter into its own film. Each member is a
+ * renderer of its scene (created with the settings given, OWE_INTEGRATOR_PATH only), so every
+ * member's film equals what its own persistent pass would give, bit for bit. The reference renders
+ * one scene per Renderer (renderer.hpp:19-79); a caller there loops over renderers — this is the
+ * batching form for that loop, with no counterpart in the reference's interface. */
+typedef struct owe_hip_batch owe_hip_batch;
+/* Flattens, builds and uploads the count scenes (a few host threads at once). */
+int owe_hip_batch_create(int device, const owe_scene* scenes, int32_t count, const owe_render_settings* settings,
+                         owe_hip_batch** out);
+int owe_hip_batch_destroy(owe_hip_batch* batch);
+/* Member index's renderer, owned by the batch (never destroy it): resolve, stats and progress read
+ * what the batch's passes left; its launch shape (owe_hip_renderer_set_mode or _autotune on member
+ * 0) is the batch's. */
+int owe_hip_batch_member(owe_hip_batch* batch, int32_t index, owe_hip_renderer** out);
+/* One pass of spp samples per pixel of every member, in one launch; synchronous. */
+int owe_hip_batch_pass(owe_hip_batch* batch, int spp);
+/* The members' statistics summed. */
+int owe_hip_batch_stats(const owe_hip_batch* batch, owe_transport_stats* out);
+/* The launch's time by HIP events on the batch's stream; enabling clears the total. */
+int owe_hip_batch_set_timing(owe_hip_batch* batch, int enable);
+int owe_hip_batch_kernel_time(const owe_hip_batch* batch, double* ms, uint64_t* launches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,7 +22,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
 #include <mutex>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -820,13 +822,12 @@ __device__ __forceinline__ void flushFilm(unsigned long long* film, uint32_t pix
     acc[0] = acc[1] = acc[2] = 0;
 }
 
-// kWaves: the waves per SIMD the register allocation must allow (2: no spills; 3: more waves to
-// hide latency, at the price of some register spills) — the autotuner measures both.
-template <int kCap, int kWaves, bool kContacts>
-__global__ __launch_bounds__(kBlock, kWaves) void persistentKernel(DScene sc, Settings set, Persist a) {
-    __shared__ uint32_t stack[kCap * kBlock];
-    uint32_t* st = stack + threadIdx.x;
-    const uint32_t pass = *a.pass;
+// The lane loop of the persistent form over one scene's pass (the kernels below run it for one scene
+// or for each scene of a batch in turn); returns the segments the lane traced. The wave leaves it
+// only when every lane has finished and no path ids of the pass are left.
+template <int kCap, bool kContacts>
+__device__ __forceinline__ uint32_t persistDrain(const DScene& sc, const Settings& set, const Persist& a,
+                                                 uint32_t* st, uint32_t pass) {
     const uint64_t passInc = (uint64_t(pass) << 1) | 1u;
     const uint32_t lane = __lane_id();
     ShadeCtx c{sc, set};
@@ -913,9 +914,66 @@ __global__ __launch_bounds__(kBlock, kWaves) void persistentKernel(DScene sc, Se
         }
     }
     flushFilm(a.film, accPix, acc);
+    return segments;
+}
+
+__device__ __forceinline__ void addSegments(unsigned long long* total, uint32_t segments) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) segments += __shfl_xor(segments, off);
-    if (lane == 0 && segments) atomicAdd(a.segments, (unsigned long long)segments);
+    if (__lane_id() == 0 && segments) atomicAdd(total, (unsigned long long)segments);
+}
+
+// kWaves: the waves per SIMD the register allocation must allow (2: no spills; 3: more waves to
+// hide latency, at the price of some register spills) — the autotuner measures both.
+template <int kCap, int kWaves, bool kContacts>
+__global__ __launch_bounds__(kBlock, kWaves) void persistentKernel(DScene sc, Settings set, Persist a) {
+    __shared__ uint32_t stack[kCap * kBlock];
+    addSegments(a.segments, persistDrain<kCap, kContacts>(sc, set, a, stack + threadIdx.x, *a.pass));
+}
+
+// A batch of scenes (variants of one family: perturbed instruments, tolerancing runs) in one launch:
+// every wave drains scene 0's pass, then scene 1's, and so on — scene v's paths come from its own id
+// counter and go to its own film, so each scene's film equals that of its own persistent pass bit for
+// bit. The scene a wave works on is uniform across it, so its tables are read through the scalar unit.
+struct PersistBatch {
+    uint32_t count;                         // scenes
+    uint32_t spp, chunk, regen;
+    const uint32_t* pass;
+    const uint64_t* total;                  // [count] paths of the pass
+    const int32_t* simple;                  // [count] plain shadow walks
+    unsigned long long* const* film;        // [count]
+    unsigned long long* next;               // [count]
+    unsigned long long* segments;           // [count]
+    uint32_t* counters;                     // [4 count]: [2] inconsistencies, [3] leaks
+    float invNWY;
+};
+
+__global__ void persistBatchBegin(PersistBatch b) {
+    for (uint32_t v = threadIdx.x; v < b.count; v += blockDim.x) {
+        b.next[v] = 0;
+        b.segments[v] = 0;
+        for (int k = 0; k < 4; ++k) b.counters[4 * v + k] = 0;
+    }
+}
+
+template <int kCap, int kWaves, bool kContacts>
+__global__ __launch_bounds__(kBlock, kWaves) void persistentBatchKernel(const DScene* __restrict__ scenes, Settings set,
+                                                                       PersistBatch b) {
+    __shared__ uint32_t stack[kCap * kBlock];
+    const uint32_t pass = *b.pass;
+    for (uint32_t v = 0; v < b.count; ++v) {
+        Persist a{};
+        a.total = b.total[v];
+        a.spp = b.spp;
+        a.chunk = b.chunk;
+        a.regen = b.regen;
+        a.next = b.next + v;
+        a.film = b.film[v];
+        a.counters = b.counters + 4 * v;
+        a.invNWY = b.invNWY;
+        a.simple = b.simple[v];
+        addSegments(b.segments + v, persistDrain<kCap, kContacts>(scenes[v], set, a, stack + threadIdx.x, pass));
+    }
 }
 
 // ---------------------------------------------------------------- SPPM
@@ -1175,6 +1233,23 @@ struct owe_hip_renderer {
     uint64_t graphTotal = 0;  // the pass size the graph was captured for
 };
 
+// A batch of renderers whose persistent passes run as one launch (owe_hip_batch_*).
+struct owe_hip_batch {
+    int device = 0;
+    std::vector<owe_hip_renderer*> members;
+    std::vector<void*> allocations;
+    hipStream_t stream = nullptr;
+    owehip::PersistBatch args{};
+    owehip::DScene* scenes = nullptr;
+    uint32_t stackCap = 1;
+    bool contacts = false;
+    int passes = 0;
+    bool timing = false;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    double kernelMs = 0;
+    uint64_t launches = 0;
+};
+
 namespace owehip {
 namespace {
 
@@ -1263,6 +1338,7 @@ void launchBounce(owe_hip_renderer* r, int q, uint32_t upper, uint64_t passInc) 
 // Runs count paths: slots [0, count) with global path ids base + slot, or the (pixel, sample) lists.
 void runPaths(owe_hip_renderer* r, uint64_t base, uint32_t count, uint32_t spp, uint32_t pass, bool lists,
               double* perPath) {
+    if (!r->P.count) allocPaths(r, r->capacity);
     const uint64_t passInc = (uint64_t(pass) << 1) | 1u;
     timed(r, kGenerate, [&] {
         hipLaunchKernelGGL(generateKernel, gridFor(count), dim3{kBlock}, 0, r->stream, r->dev, r->set, r->P, base, count,
@@ -1353,16 +1429,35 @@ void launchPersistent(owe_hip_renderer* r, const Persist& a) {
     hipLaunchKernelGGL(persistBegin, dim3{1}, dim3{64}, 0, r->stream, a);
     const dim3 grid{unsigned(std::max(1, r->computeUnits * r->blocksPerCU))};
     auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, grid, dim3{kBlock}, 0, r->stream, r->dev, r->set, a); };
+    // One LDS stack size (48 entries, every scene's bound) for the persistent kernels: registers hold
+    // them to 2 or 3 waves per SIMD, i.e. 2 or 3 blocks per compute unit, and 3 × 48 KiB of stacks
+    // fit its 160 KiB of LDS, so a smaller stack would not let more waves in.
     const bool c = r->dev.contacts != 0u;
-    if (r->persistWaves == 3) {
-        if (r->stackCap <= 16) c ? go(persistentKernel<16, 3, true>) : go(persistentKernel<16, 3, false>);
-        else if (r->stackCap <= 32) c ? go(persistentKernel<32, 3, true>) : go(persistentKernel<32, 3, false>);
-        else c ? go(persistentKernel<48, 3, true>) : go(persistentKernel<48, 3, false>);
-    } else {
-        if (r->stackCap <= 16) c ? go(persistentKernel<16, 2, true>) : go(persistentKernel<16, 2, false>);
-        else if (r->stackCap <= 32) c ? go(persistentKernel<32, 2, true>) : go(persistentKernel<32, 2, false>);
-        else c ? go(persistentKernel<48, 2, true>) : go(persistentKernel<48, 2, false>);
-    }
+    if (r->persistWaves == 3) c ? go(persistentKernel<48, 3, true>) : go(persistentKernel<48, 3, false>);
+    else c ? go(persistentKernel<48, 2, true>) : go(persistentKernel<48, 2, false>);
+}
+
+void launchPersistentBatch(owe_hip_batch* b, const PersistBatch& a, int waves, int blocksPerCU, int computeUnits) {
+    hipLaunchKernelGGL(persistBatchBegin, dim3{1}, dim3{256}, 0, b->stream, a);
+    const dim3 grid{unsigned(std::max(1, computeUnits * blocksPerCU))};
+    auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, grid, dim3{kBlock}, 0, b->stream, b->scenes, b->members[0]->set, a); };
+    const bool c = b->contacts;  // the persistent form's one stack size, as launchPersistent
+    if (waves == 3) c ? go(persistentBatchKernel<48, 3, true>) : go(persistentBatchKernel<48, 3, false>);
+    else c ? go(persistentBatchKernel<48, 2, true>) : go(persistentBatchKernel<48, 2, false>);
+}
+
+template <class T>
+T* batchAlloc(owe_hip_batch* b, size_t count) {
+    void* p = nullptr;
+    OWE_HIP_CHECK(hipMalloc(&p, std::max<size_t>(1, count) * sizeof(T)));
+    b->allocations.push_back(p);
+    return static_cast<T*>(p);
+}
+template <class T>
+T* batchUpload(owe_hip_batch* b, const std::vector<T>& v) {
+    T* p = batchAlloc<T>(b, v.size());
+    if (!v.empty()) OWE_HIP_CHECK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return p;
 }
 
 void dropGraph(owe_hip_renderer* r) {
@@ -1606,7 +1701,9 @@ int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render
         r->film = devAlloc<unsigned long long>(r, 3 * pixels);
         OWE_HIP_CHECK(hipMemset(r->film, 0, 3 * pixels * sizeof(unsigned long long)));
         if (r->integrator == OWE_INTEGRATOR_PATH) {
-            allocPaths(r, uint32_t(std::min<uint64_t>(pixels * 64, 1u << 22)));
+            // The wavefront form's path slots are allocated at its first pass (runPaths): a renderer
+            // that only runs the persistent form, or only as a batch member, never holds them.
+            r->capacity = uint32_t(std::min<uint64_t>(pixels * 64, 1u << 22));
             hipDeviceProp_t prop;
             OWE_HIP_CHECK(hipGetDeviceProperties(&prop, device));
             r->computeUnits = prop.multiProcessorCount;
@@ -2202,6 +2299,174 @@ int owe_hip_renderer_autotune(owe_hip_renderer* r, int spp, double* best_ms) {
             r->kernelLaunches[k] = 0;
         }
         if (best_ms) *best_ms = best;
+    });
+}
+
+// ---------------------------------------------------------------- batches of scenes
+
+static void destroyBatch(owe_hip_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (void* p : b->allocations) (void)hipFree(p);
+    for (hipEvent_t e : b->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    for (owe_hip_renderer* m : b->members)
+        if (m) (void)owe_hip_renderer_destroy(m);
+    delete b;
+}
+
+int owe_hip_batch_create(int device, const owe_scene* scenes, int32_t count, const owe_render_settings* settings,
+                         owe_hip_batch** out) {
+    owe_hip_batch* b = nullptr;
+    const int st = guard([&] {
+        require(out != nullptr && scenes != nullptr && settings != nullptr, "null argument");
+        *out = nullptr;
+        require(count > 0, "a batch needs at least one scene");
+        require(settings->integrator == OWE_INTEGRATOR_PATH, "a batch renders camera paths (the path integrator)");
+        selectDevice(device);
+        b = new owe_hip_batch;
+        b->device = device;
+        b->members.assign(size_t(count), nullptr);
+        // The scenes are flattened, their hierarchies built and uploaded by a few host threads.
+        std::vector<int> status(size_t(count), OWE_HIP_OK);
+        std::vector<std::string> why(static_cast<size_t>(count));
+        std::atomic<int> next{0};
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        const int threads = int(std::min<unsigned>(unsigned(count), std::min(8u, hw)));
+        auto work = [&] {
+            for (;;) {
+                const int i = next.fetch_add(1);
+                if (i >= count) break;
+                status[size_t(i)] = owe_hip_renderer_create(device, &scenes[i], settings, &b->members[size_t(i)]);
+                if (status[size_t(i)] != OWE_HIP_OK) why[size_t(i)] = owe_hip_last_error();
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int k = 0; k < threads; ++k) pool.emplace_back(work);
+        for (std::thread& t : pool) t.join();
+        for (int i = 0; i < count; ++i)
+            if (status[size_t(i)] != OWE_HIP_OK) fail(status[size_t(i)], "scene " + std::to_string(i) + ": " + why[size_t(i)]);
+        OWE_HIP_CHECK(hipSetDevice(device));
+        OWE_HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        std::vector<DScene> ds;
+        std::vector<int32_t> simple;
+        std::vector<unsigned long long*> films;
+        for (owe_hip_renderer* m : b->members) {
+            ds.push_back(m->dev);
+            simple.push_back(m->simpleShadows ? 1 : 0);
+            films.push_back(m->film);
+            b->stackCap = std::max(b->stackCap, m->stackCap);
+            b->contacts = b->contacts || m->dev.contacts != 0u;
+        }
+        b->scenes = batchUpload(b, ds);
+        PersistBatch& a = b->args;
+        a.count = uint32_t(count);
+        a.pass = batchAlloc<uint32_t>(b, 1);
+        a.total = batchAlloc<uint64_t>(b, size_t(count));
+        a.simple = batchUpload(b, simple);
+        a.film = batchUpload(b, films);
+        a.next = batchAlloc<unsigned long long>(b, size_t(count));
+        a.segments = batchAlloc<unsigned long long>(b, size_t(count));
+        a.counters = batchAlloc<uint32_t>(b, 4 * size_t(count));
+        a.invNWY = b->members[0]->invNWY;
+        *out = b;
+    });
+    if (st != OWE_HIP_OK) destroyBatch(b);
+    return st;
+}
+
+int owe_hip_batch_destroy(owe_hip_batch* b) {
+    return guard([&] { destroyBatch(b); });
+}
+
+int owe_hip_batch_member(owe_hip_batch* b, int32_t index, owe_hip_renderer** out) {
+    return guard([&] {
+        require(b != nullptr && out != nullptr, "null argument");
+        require(index >= 0 && size_t(index) < b->members.size(), "member index out of range");
+        *out = b->members[size_t(index)];
+    });
+}
+
+int owe_hip_batch_pass(owe_hip_batch* b, int spp) {
+    return guard([&] {
+        require(b != nullptr, "batch is null");
+        if (spp <= 0) return;
+        OWE_HIP_CHECK(hipSetDevice(b->device));
+        const size_t n = b->members.size();
+        const owe_hip_renderer* lead = b->members[0];
+        std::vector<uint64_t> total(n);
+        for (size_t v = 0; v < n; ++v)
+            total[v] = uint64_t(b->members[v]->host.obs.width) * uint64_t(b->members[v]->host.obs.height) * uint64_t(spp);
+        PersistBatch a = b->args;
+        a.spp = uint32_t(spp);
+        a.chunk = lead->chunk;
+        a.regen = lead->regen;
+        OWE_HIP_CHECK(hipMemcpyAsync(const_cast<uint64_t*>(a.total), total.data(), n * sizeof(uint64_t),
+                                     hipMemcpyHostToDevice, b->stream));
+        hipLaunchKernelGGL(setPass, dim3{1}, dim3{64}, 0, b->stream, const_cast<uint32_t*>(a.pass), uint32_t(b->passes));
+        if (b->timing) OWE_HIP_CHECK(hipEventRecord(b->ev[0], b->stream));
+        launchPersistentBatch(b, a, lead->persistWaves, lead->blocksPerCU, lead->computeUnits);
+        if (b->timing) OWE_HIP_CHECK(hipEventRecord(b->ev[1], b->stream));
+        OWE_HIP_CHECK(hipGetLastError());
+        std::vector<unsigned long long> segs(n);
+        std::vector<uint32_t> counters(4 * n);
+        OWE_HIP_CHECK(hipMemcpyAsync(segs.data(), a.segments, n * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                     b->stream));
+        OWE_HIP_CHECK(hipMemcpyAsync(counters.data(), a.counters, 4 * n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                     b->stream));
+        OWE_HIP_CHECK(hipStreamSynchronize(b->stream));
+        if (b->timing) {
+            float ms = 0;
+            OWE_HIP_CHECK(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+            b->kernelMs += ms;
+            b->launches++;
+        }
+        for (size_t v = 0; v < n; ++v) {
+            owe_hip_renderer* m = b->members[v];
+            m->stats.paths += total[v];
+            m->stats.segments += segs[v];
+            m->stats.inconsistencies += counters[4 * v + 2];
+            m->stats.leaks += counters[4 * v + 3];
+            m->passes++;
+            m->totalSpp += spp;
+        }
+        b->passes++;
+    });
+}
+
+int owe_hip_batch_stats(const owe_hip_batch* b, owe_transport_stats* out) {
+    return guard([&] {
+        require(b != nullptr && out != nullptr, "null argument");
+        owe_transport_stats s{};
+        for (const owe_hip_renderer* m : b->members) {
+            s.paths += m->stats.paths;
+            s.segments += m->stats.segments;
+            s.inconsistencies += m->stats.inconsistencies;
+            s.leaks += m->stats.leaks;
+        }
+        *out = s;
+    });
+}
+
+int owe_hip_batch_set_timing(owe_hip_batch* b, int enable) {
+    return guard([&] {
+        require(b != nullptr, "batch is null");
+        OWE_HIP_CHECK(hipSetDevice(b->device));
+        if (enable && !b->ev[0])
+            for (hipEvent_t& e : b->ev) OWE_HIP_CHECK(hipEventCreate(&e));
+        b->timing = enable != 0;
+        b->kernelMs = 0;
+        b->launches = 0;
+    });
+}
+
+int owe_hip_batch_kernel_time(const owe_hip_batch* b, double* ms, uint64_t* launches) {
+    return guard([&] {
+        require(b != nullptr && ms != nullptr && launches != nullptr, "null argument");
+        *ms = b->kernelMs;
+        *launches = b->launches;
     });
 }
 

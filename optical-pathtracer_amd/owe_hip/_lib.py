@@ -126,6 +126,13 @@ def load() -> ctypes.CDLL:
     _declare(lib, "owe_hip_caustic_backward", c_int, [c_vp, c_vp, c_vp, c_vp, c_u64, c_vp])
     _declare(lib, "owe_hip_renderer_set_timing", c_int, [c_vp, c_int])
     _declare(lib, "owe_hip_renderer_kernel_times", c_int, [c_vp, c_vp, c_vp])
+    _declare(lib, "owe_hip_batch_create", c_int, [c_int, c_vp, ctypes.c_int32, c_vp, ctypes.POINTER(c_vp)])
+    _declare(lib, "owe_hip_batch_destroy", c_int, [c_vp])
+    _declare(lib, "owe_hip_batch_member", c_int, [c_vp, ctypes.c_int32, ctypes.POINTER(c_vp)])
+    _declare(lib, "owe_hip_batch_pass", c_int, [c_vp, c_int])
+    _declare(lib, "owe_hip_batch_stats", c_int, [c_vp, c_vp])
+    _declare(lib, "owe_hip_batch_set_timing", c_int, [c_vp, c_int])
+    _declare(lib, "owe_hip_batch_kernel_time", c_int, [c_vp, c_vp, c_vp])
     _lib = lib
     return lib
 

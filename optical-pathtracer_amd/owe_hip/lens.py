@@ -460,6 +460,24 @@ def afocal_spot(m) -> dict:
     return dict(arrived=int(n), apparent_angle=float(c[1]), angular_rms=math.sqrt(max(0.0, v)))
 
 
+def gregorian_variants(n: int = 64, rings: int = 200, seed: int = 5) -> list:
+    """n perturbed copies of the reference's Gregorian (lenses/gregorian_54x.lens), the tolerancing
+    family of the differentiable-telescope workload: both mirrors' radii ±0.2 %, the secondary's
+    conic ±0.002, the ocular's radii ±0.5 %, each re-solved afocal; `rings` pupil rings each."""
+    base = solve_afocal(GREGORIAN, 587.5618).with_rings(rings)
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        sf = [Surface(**{**q.__dict__}) for q in base.surfaces]
+        sf[0].R *= 1 + rng.uniform(-2e-3, 2e-3)
+        sf[1].R *= 1 + rng.uniform(-2e-3, 2e-3)
+        sf[1].k += rng.uniform(-2e-3, 2e-3)
+        sf[2].R *= 1 + rng.uniform(-5e-3, 5e-3)
+        sf[4].R *= 1 + rng.uniform(-5e-3, 5e-3)
+        out.append(solve_afocal(Prescription(sf, base.ambient, base.rays, base.name, base.rings), 587.5618))
+    return out
+
+
 def c_surface(s: Surface) -> CLensSurface:
     c = CLensSurface()
     c.curvature = 0.0 if math.isinf(s.R) else 1.0 / (s.R * 1e-3)

@@ -4,7 +4,8 @@ The Python face of the reference's Renderer (src/owe/render/renderer.hpp:19-79) 
 backend, with the path or the light integrator: ``run_pass(spp)`` adds a pass (spp samples, or
 spp particles, per pixel), ``resolve()`` returns the XYZ image [H, W, 3] in double, ``stats()`` the
 TransportStats. ``camera_paths`` and ``particles`` return single paths' or particles' XYZ for
-per-path parity.
+per-path parity. ``Batch`` renders many scenes (variants of one family) as one persistent launch
+per pass (owe_hip_batch_*).
 """
 from __future__ import annotations
 
@@ -13,7 +14,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from .scene import CRenderSettings, CTransportStats, Scene, settings as make_settings
+from .scene import CRenderSettings, CScene, CTransportStats, Scene, settings as make_settings
 
 
 def set_tuning_cache(path) -> None:
@@ -22,20 +23,24 @@ def set_tuning_cache(path) -> None:
 
 
 class Renderer:
-    def __init__(self, scene: Scene, settings: CRenderSettings | None = None, device: int = 0):
+    def __init__(self, scene: Scene, settings: CRenderSettings | None = None, device: int = 0, _handle=None):
         self.scene = scene
         self.settings = settings if settings is not None else make_settings()
+        self.width, self.height = scene.observer["width"], scene.observer["height"]
+        self._owned = _handle is None
+        if _handle is not None:  # a batch member: the batch owns it
+            self._h = _handle
+            return
         c = scene.c()
         h = ctypes.c_void_p()
         _lib.check(_lib.load().owe_hip_renderer_create(device, ctypes.byref(c), ctypes.byref(self.settings),
                                                        ctypes.byref(h)))
         self._h = h
-        self.width, self.height = scene.observer["width"], scene.observer["height"]
 
     def close(self) -> None:
-        if self._h:
+        if self._h and self._owned:
             _lib.check(_lib.load().owe_hip_renderer_destroy(self._h))
-            self._h = ctypes.c_void_p()
+        self._h = ctypes.c_void_p()
 
     def __del__(self):
         try:
@@ -125,3 +130,53 @@ class Renderer:
         _lib.check(_lib.load().owe_hip_renderer_particles(self._h, int(pass_index), index.ctypes.data, len(index),
                                                           out.ctypes.data))
         return out
+
+
+class Batch:
+    """Many scenes rendered with camera paths as one persistent launch per pass (owe_hip_batch_*):
+    ``members[i]`` is scene i's Renderer (owned by the batch: resolve, stats, progress; member 0's
+    launch shape, set or autotuned on it, is the batch's)."""
+
+    def __init__(self, scenes, settings: CRenderSettings | None = None, device: int = 0):
+        self.scenes = list(scenes)
+        self.settings = settings if settings is not None else make_settings()
+        cs = [sc.c() for sc in self.scenes]
+        arr = (CScene * len(cs))(*cs) if cs else None
+        h = ctypes.c_void_p()
+        _lib.check(_lib.load().owe_hip_batch_create(device, arr, len(cs), ctypes.byref(self.settings), ctypes.byref(h)))
+        self._h = h
+        self.members = []
+        for i, sc in enumerate(self.scenes):
+            m = ctypes.c_void_p()
+            _lib.check(_lib.load().owe_hip_batch_member(h, i, ctypes.byref(m)))
+            self.members.append(Renderer(sc, self.settings, _handle=m))
+
+    def close(self) -> None:
+        if self._h:
+            for m in self.members:
+                m._h = ctypes.c_void_p()
+            _lib.check(_lib.load().owe_hip_batch_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run_pass(self, spp: int) -> None:
+        _lib.check(_lib.load().owe_hip_batch_pass(self._h, int(spp)))
+
+    def stats(self) -> dict:
+        s = CTransportStats()
+        _lib.check(_lib.load().owe_hip_batch_stats(self._h, ctypes.byref(s)))
+        return {"paths": s.paths, "segments": s.segments, "inconsistencies": s.inconsistencies, "leaks": s.leaks}
+
+    def set_timing(self, enable: bool = True) -> None:
+        _lib.check(_lib.load().owe_hip_batch_set_timing(self._h, int(enable)))
+
+    def kernel_time(self) -> tuple[float, int]:
+        """(milliseconds, launches) of the batch's persistent launches since timing was enabled."""
+        ms, n = ctypes.c_double(), ctypes.c_uint64()
+        _lib.check(_lib.load().owe_hip_batch_kernel_time(self._h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value

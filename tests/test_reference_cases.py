@@ -163,7 +163,7 @@ def check_walk(rows, centred, lost, back, tight):
     previous = 1.0
     for f, seen, expected in rows:
         print(f"eye {f:.2f}·(rx + re) off the exit pupil: {seen:.4f} of the centred image, disc overlap {expected:.4f}")
-        assert seen <= previous * 1.05
+        assert seen <= previous * 1.05 + 2e-3  # never brighter beyond noise (stray light at the 1e-3 level once lost)
         assert abs(seen - expected) <= 0.12 + 0.1 * expected  # the reference test's own bound
         assert abs(seen - expected) <= tight
         previous = seen
