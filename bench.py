@@ -474,7 +474,10 @@ def cpu_baseline_light(sc, st, seconds_budget: float = 10.0):
 
 def bench_light(args):
     """Light tracing of the prism scene: 256×256×64 particles per pass from the sun, connected to
-    the eye from every diffuse vertex, splatted exactly."""
+    the eye from every diffuse vertex, splatted exactly — in both forms of the particle pass: one
+    thread per particle (the reference's form) and the lane-refill form (autotuned), which must give
+    the same image bit for bit; the faster is the leg's value."""
+    import numpy as np
     import torch
 
     from owe_hip import render
@@ -483,37 +486,62 @@ def bench_light(args):
     sc = S.dispersion_prism_scene(PRISM_SIZE, PRISM_SIZE)
     st = S.settings(seed=1, max_depth=16, rr_depth=4, spectral=S.SPECTRAL_UNIFORM1, lambda_min=380.0, lambda_max=780.0,
                     integrator=S.INTEGRATOR_LIGHT)
-    r = render.Renderer(sc, st)
-    for _ in range(args.warmup):
-        r.run_pass(PRISM_SPP)
-    r.reset()
-    r.set_timing(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        r.run_pass(PRISM_SPP)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    stats = r.stats()
-    ms, launches = r.kernel_times()["shade"]
+    forms = {}
+    images = {}
+    for name, mode in (("per_particle", render.Renderer.WAVEFRONT), ("refill", render.Renderer.PERSISTENT)):
+        r = render.Renderer(sc, st)
+        r.set_mode(mode)
+        if mode == render.Renderer.PERSISTENT:
+            r.autotune(PRISM_SPP)
+        for _ in range(args.warmup):
+            r.run_pass(PRISM_SPP)
+        r.reset()
+        r.set_timing(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            r.run_pass(PRISM_SPP)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        stats = r.stats()
+        ms, _ = r.kernel_times()["shade"]
+        images[name] = r.resolve()
+        forms[name] = {"particles_per_s": stats["paths"] / wall, "ms_per_step": wall * 1e3 / args.steps,
+                       "kernel_ms_per_step": ms / args.steps, "stats": stats, "launch": r.mode()}
+        log(f"light ({name}): {args.steps} passes in {wall:.3f}s; {stats['paths'] / wall / 1e6:.1f} M particles/s; "
+            f"segments/particle {stats['segments'] / max(1, stats['paths']):.3f}; kernel {ms / args.steps:.3f} ms/pass; "
+            f"shape {r.mode()}")
+        r.close()
+    best = max(forms, key=lambda k: forms[k]["particles_per_s"])
+    f = forms[best]
+    stats = f["stats"]
     n = stats["paths"]
-    log(f"light: {args.steps} passes in {wall:.3f}s; {n / wall / 1e6:.1f} M particles/s; "
-        f"segments/particle {stats['segments'] / max(1, n):.3f}; particle kernel {ms / args.steps:.3f} ms/pass")
+    kernel = "particleRefillKernel" if best == "refill" else "particleKernel"
     # Algorithmic HBM bytes: each particle vertex connects to the eye at most once and deposits three
     # 64-bit bin adds (24 B); the particle's own state never leaves registers.
     splat_bytes = 24.0 * stats["segments"] / args.steps
-    k_ms = ms / args.steps
-    out = {"value": n / wall, "unit": "particles/s", "ms_per_step": wall * 1e3 / args.steps,
+    k_ms = f["kernel_ms_per_step"]
+    hbm = {"bound": "hbm", "achieved": splat_bytes / (k_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": splat_bytes / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "traffic": measured_traffic(PROFILE_LIGHT_TRAFFIC, kernel), "kernel": kernel,
+           "algorithmic_bytes_per_launch": splat_bytes}
+    roofline = valu_roofline(kernel, k_ms)
+    if roofline:
+        roofline["traffic"] = hbm["traffic"]
+        roofline["hbm_model"] = hbm
+    else:
+        roofline = hbm
+    out = {"value": f["particles_per_s"], "unit": "particles/s", "ms_per_step": f["ms_per_step"],
            "config": {"workload": "light tracing (particles), dispersion prism", "resolution": [PRISM_SIZE, PRISM_SIZE],
                       "particles_per_step": PRISM_SIZE * PRISM_SIZE * PRISM_SPP, "max_depth": 16, "rr_depth": 4,
-                      "segments_per_particle": stats["segments"] / max(1, n),
-                      "kernel_ms_per_step": k_ms, "mean_Y": float(r.resolve()[..., 1].mean())},
-           "roofline": {"bound": "hbm", "achieved": splat_bytes / (k_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": splat_bytes / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                        "traffic": measured_traffic(PROFILE_LIGHT_TRAFFIC, "particleKernel"),
-                        "kernel": "particleKernel", "algorithmic_bytes_per_launch": splat_bytes}}
+                      "segments_per_particle": stats["segments"] / max(1, n), "form": best,
+                      "kernel_ms_per_step": k_ms, "mean_Y": float(images[best][..., 1].mean()),
+                      "forms": {k: {kk: v[kk] for kk in ("particles_per_s", "ms_per_step", "kernel_ms_per_step", "launch")}
+                                for k, v in forms.items()},
+                      "refill_over_per_particle": forms["refill"]["particles_per_s"] / forms["per_particle"]["particles_per_s"],
+                      "forms_bit_identical": bool(np.array_equal(images["per_particle"], images["refill"]))},
+           "roofline": roofline}
     out["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline_light(sc, st)
-    r.close()
     return out
 
 

@@ -159,7 +159,12 @@ int owe_hip_renderer_particles(owe_hip_renderer* renderer, int pass, const uint6
  * keeps it) selects the persistent kernel compiled for that many resident waves per SIMD (3 trades
  * register spills for latency hiding); refill (1..64; 0 keeps it) is how many of a wave's lanes must
  * have finished their paths before the wave starts new ones together; use_graph replays each pass
- * from a captured HIP graph. No counterpart in the reference (whose GPU backend has one form). */
+ * from a captured HIP graph. No counterpart in the reference (whose GPU backend has one form).
+ * A light-integrator renderer has the same two modes for its particle pass: one thread per particle
+ * (OWE_HIP_MODE_WAVEFRONT, the reference's particle kernel's form, the default) or the lane-refill
+ * form (OWE_HIP_MODE_PERSISTENT: a grid of blocks_per_cu blocks per compute unit, each lane advancing
+ * its particle a segment at a time and taking chunk particle ids at a time, refill as above;
+ * waves_per_simd and use_graph do not apply). Both give the same image bit for bit. */
 #define OWE_HIP_MODE_WAVEFRONT 0
 #define OWE_HIP_MODE_PERSISTENT 1
 int owe_hip_renderer_set_mode(owe_hip_renderer* renderer, int mode, int blocks_per_cu, int chunk, int use_graph,
@@ -169,7 +174,8 @@ int owe_hip_renderer_get_mode(const owe_hip_renderer* renderer, int* mode, int* 
 /* Times passes of spp samples per pixel under candidate launch shapes of the current form
  * (persistent: waves per SIMD × refill threshold × blocks per compute unit × chunk; wavefront: bounce
  * rounds between host reads of the live counters × paths in flight, a quarter to all of the path
- * slots), times the five fastest three more times each and keeps the lowest median, and clears the
+ * slots; a light-integrator renderer in the lane-refill form: refill threshold × blocks per compute
+ * unit × chunk), times the five fastest three more times each and keeps the lowest median, and clears the
  * film and statistics again. Only on a renderer without passes. best_ms (optional) receives the
  * winning pass time. With a tuning cache set (below) a recorded shape is applied without timing. */
 int owe_hip_renderer_autotune(owe_hip_renderer* renderer, int spp, double* best_ms);

@@ -18,6 +18,8 @@
 // (refineDistantHit, particle.slang:143-147).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "common.hpp"
 #include "particle.hpp"
 #include "shading_dev.hpp"
@@ -27,6 +29,7 @@ namespace owehip {
 namespace {
 
 constexpr int kPBlock = 128;
+constexpr int kRefillStack = 48;  // the refill kernel's LDS stack: every scene's bound
 
 struct Link {
     f3 q, dir;
@@ -402,19 +405,25 @@ __device__ __forceinline__ void sppmGather(PCtx& c, const SppmDev& s, const Hit&
     }
 }
 
-// traceParticle (particle.slang:356-427), without the hybrid partition; with kSppm an SPPM photon
-// (traceSppmPhoton, sppm.slang:150-201): no observer connections or sensor splats, a gather at every
-// non-specular surface after the first scattering event.
-template <bool kContacts, bool kSppm = false>
-__device__ __forceinline__ void traceParticle(PCtx& c, const SppmDev* sp = nullptr, uint32_t group = 0) {
+// A particle between segments: its ray, the segments and scattering events so far (SPPM: direct
+// light is the camera pass's, so photons gather only after their first event).
+struct PState {
+    PRay ray;
+    int depth, events;
+};
+
+// One segment of the walk: returns whether the particle goes on. traceParticle runs it to the end in
+// one thread; the lane-refill kernel advances every lane of a wave one segment at a time.
+template <bool kContacts, bool kSppm>
+__device__ __forceinline__ bool particleSegment(PCtx& c, PState& st, const SppmDev* sp, uint32_t group) {
     const DScene& sc = c.sc;
     const DObserver& ob = sc.obs;
     const bool observer = !kSppm && ob.kind == OWE_DETECTOR_OBSERVER;
     const int maxDepth = c.set.maxDepth, rrDepth = c.set.rrDepth;
-    int events = 0;  // scattering events so far (SPPM: direct light is the camera pass's)
-    PRay ray;
-    if (!emitParticle<kContacts, kSppm>(c, observer, ray, sp)) return;
-    for (int depth = 0;; ++depth) {
+    PRay& ray = st.ray;
+    int& events = st.events;
+    const int depth = st.depth;
+    {
         Hit h;
         bool found = intersectContact<true, kContacts>(sc, ray.o, ray.d, __builtin_inff(), ray.departed, ray.face, kNoBoundary, ray.region,
                                       c.stack, kPBlock, h);
@@ -442,7 +451,7 @@ __device__ __forceinline__ void traceParticle(PCtx& c, const SppmDev* sp = nullp
                     c.beta[i] *= evalSpec(med.scattering, sc.table, c.wl.lambda[i]) * expf(-sT[i] * tm) / pdf0;
                 const f3 p = ray.o + ray.d * tm;
                 if (observer) connectVertex<kContacts>(c, p, false, nullptr, kNoBoundary, ~0u, mk3(0, 0, 0), ray.d, med.g, ray.region);
-                if (depth >= maxDepth) return;
+                if (depth >= maxDepth) return false;
                 const float u2 = c.rng.uniform(), u1 = c.rng.uniform();
                 f3 wi;
                 sampleHg(-ray.d, med.g, u1, u2, wi);
@@ -451,21 +460,21 @@ __device__ __forceinline__ void traceParticle(PCtx& c, const SppmDev* sp = nullp
                 ray.departed = kNoBoundary;
                 ray.face = ~0u;
                 ++events;
-                if (depth >= rrDepth && !roulette(c, 0.95f)) return;
-                continue;
+                if (depth >= rrDepth && !roulette(c, 0.95f)) return false;
+                return ++st.depth, true;
             }
-            if (!found) return;
+            if (!found) return false;
             const float pass0 = expf(-sT[0] * seg);
 #pragma unroll
             for (int i = 0; i < NW; ++i) c.beta[i] *= expf(-sT[i] * seg) / pass0;
         } else if (med.flags & 1) {
-            if (!found) return;
+            if (!found) return false;
 #pragma unroll
             for (int i = 0; i < NW; ++i) c.beta[i] *= expf(-evalSpec(med.absorption, sc.table, c.wl.lambda[i]) * h.t);
         }
         if (!found) {
             if (ray.region != 0) c.leaks++;
-            return;
+            return false;
         }
         // meetBoundary (transport.slang:600-615).
         const uint32_t bi = h.boundary;
@@ -483,8 +492,8 @@ __device__ __forceinline__ void traceParticle(PCtx& c, const SppmDev* sp = nullp
             ray.o = h.p;
             ray.departed = bi;
             ray.face = hitFace;
-            if (depth >= 4 * maxDepth) return;
-            continue;
+            if (depth >= 4 * maxDepth) return false;
+            return ++st.depth, true;
         }
         if (!kSppm && !observer && fromFront && bi == ob.sensor) {
             // sensorPixel (particle.slang:98-114): the flux per pixel area.
@@ -505,12 +514,12 @@ __device__ __forceinline__ void traceParticle(PCtx& c, const SppmDev* sp = nullp
                 deposit(c, min(int(fx), ob.width - 1), min(int(fy), ob.height - 1), v);
             }
         }
-        if (o.type == OWE_SURF_DETECTOR || o.type == OWE_SURF_ABSORBER) return;
+        if (o.type == OWE_SURF_DETECTOR || o.type == OWE_SURF_ABSORBER) return false;
         const float hero = c.wl.lambda[0];
         const Iface it{&o, h.n, regionIndex(sc, b.front, hero), regionIndex(sc, b.back, hero), hero, c.set.fresnelFloor};
         if (!o.delta && observer) connectVertex<kContacts>(c, h.p, true, &it, bi, hitFace, h.n, ray.d, 0.0f, ray.region);
         if (kSppm && !o.delta && events >= 1) sppmGather(c, *sp, h, ray.d, group);
-        if (depth >= maxDepth) return;
+        if (depth >= maxDepth) return false;
         auto at = [&](int i) {
             Iface r = it;
             r.nFront = regionIndex(sc, b.front, c.wl.lambda[i]);
@@ -536,7 +545,7 @@ __device__ __forceinline__ void traceParticle(PCtx& c, const SppmDev* sp = nullp
         // scatterAtSurface in importance mode (transport.slang:671-684).
         const float u3 = c.rng.uniform(), u2 = c.rng.uniform(), u1 = c.rng.uniform();
         SSample s;
-        if (!sampleScatter(sc, it, ray.d, u1, u2, u3, false, s) || s.weight <= 0.0f) return;
+        if (!sampleScatter(sc, it, ray.d, u1, u2, u3, false, s) || s.weight <= 0.0f) return false;
         c.beta[0] *= s.weight;
 #pragma unroll
         for (int i = 1; i < NW; ++i)
@@ -548,14 +557,52 @@ __device__ __forceinline__ void traceParticle(PCtx& c, const SppmDev* sp = nullp
         ray.face = hitFace;
         ++events;
         // Particles carry flux: roulette against the event's own throughput.
-        if (depth >= rrDepth && s.weight < 1.0f && !roulette(c, s.weight)) return;
+        if (depth >= rrDepth && s.weight < 1.0f && !roulette(c, s.weight)) return false;
+    }
+    ++st.depth;
+    return true;
+}
+
+// traceParticle (particle.slang:356-427), without the hybrid partition; with kSppm an SPPM photon
+// (traceSppmPhoton, sppm.slang:150-201): no observer connections or sensor splats, a gather at every
+// non-specular surface after the first scattering event.
+template <bool kContacts, bool kSppm = false>
+__device__ __forceinline__ void traceParticle(PCtx& c, const SppmDev* sp = nullptr, uint32_t group = 0) {
+    const bool observer = !kSppm && c.sc.obs.kind == OWE_DETECTOR_OBSERVER;
+    PState st;
+    st.depth = 0;
+    st.events = 0;
+    if (!emitParticle<kContacts, kSppm>(c, observer, st.ray, sp)) return;
+    while (particleSegment<kContacts, kSppm>(c, st, sp, group)) {
     }
 }
+
 
 __device__ __forceinline__ uint32_t waveSum(uint32_t v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
     return v;
+}
+
+// Particle `index` of pass `pass`: its own stream (seed, index, pass) and its wavelengths.
+__device__ __forceinline__ void startParticle(PCtx& c, const Settings& set, uint64_t index, uint32_t pass) {
+    c.rng.seed(hashPair(set.seed ^ 0x5bd1e995ull, index), pass);
+    const float u = c.rng.uniform();
+    if (set.spectral == OWE_SPECTRAL_UNIFORM1) {
+        const float l = set.lambdaMin + (set.lambdaMax - set.lambdaMin) * u;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) c.wl.lambda[k] = l;
+        c.wl.pdf[0] = 1.0f / (set.lambdaMax - set.lambdaMin) / NW;
+        c.wl.pdf[1] = c.wl.pdf[2] = c.wl.pdf[3] = 0.0f;
+    } else {
+#pragma unroll
+        for (int k = 0; k < NW; ++k) {
+            float up = u + float(k) / NW;
+            if (up >= 1.0f) up -= 1.0f;
+            c.wl.lambda[k] = sampleVisibleF(up);
+            c.wl.pdf[k] = visiblePdfF(c.wl.lambda[k]);
+        }
+    }
 }
 
 template <int kCap, bool kContacts>
@@ -567,24 +614,7 @@ __global__ __launch_bounds__(kPBlock) void particleKernel(DScene sc, Settings se
     c.segments = c.inconsistencies = c.leaks = 0;
     const bool live = i < pb.count;
     if (live) {
-        const uint64_t index = pb.index ? pb.index[i] : pb.base + i;
-        c.rng.seed(hashPair(set.seed ^ 0x5bd1e995ull, index), pb.pass);
-        const float u = c.rng.uniform();
-        if (set.spectral == OWE_SPECTRAL_UNIFORM1) {
-            const float l = set.lambdaMin + (set.lambdaMax - set.lambdaMin) * u;
-#pragma unroll
-            for (int k = 0; k < NW; ++k) c.wl.lambda[k] = l;
-            c.wl.pdf[0] = 1.0f / (set.lambdaMax - set.lambdaMin) / NW;
-            c.wl.pdf[1] = c.wl.pdf[2] = c.wl.pdf[3] = 0.0f;
-        } else {
-#pragma unroll
-            for (int k = 0; k < NW; ++k) {
-                float up = u + float(k) / NW;
-                if (up >= 1.0f) up -= 1.0f;
-                c.wl.lambda[k] = sampleVisibleF(up);
-                c.wl.pdf[k] = visiblePdfF(c.wl.lambda[k]);
-            }
-        }
+        startParticle(c, set, pb.index ? pb.index[i] : pb.base + i, pb.pass);
         traceParticle<kContacts>(c);
         if (pb.perParticle) {
 #pragma unroll
@@ -593,6 +623,61 @@ __global__ __launch_bounds__(kPBlock) void particleKernel(DScene sc, Settings se
     }
     const uint32_t n[4] = {waveSum(live ? 1u : 0u), waveSum(c.segments), waveSum(c.inconsistencies), waveSum(c.leaks)};
     if (__lane_id() == 0)
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (n[k]) atomicAdd(&pb.stats[k], (unsigned long long)n[k]);
+}
+
+// The lane-refill form of a particle pass: a grid sized to the chip, each lane advancing its particle
+// one segment per iteration with particleSegment — the device code particleKernel runs, so every
+// particle deposits the same splats. A lane whose particle has ended waits until `regen` lanes of
+// its wave are idle; then they start the next particles of their chunks together, so the wave runs
+// the emission code once for many lanes and stays full instead of idling until its longest walk
+// ends. Particle ids come in chunks of `chunk` from one device counter (one atomic per wave per
+// refill). The bins are integers, so the image equals particleKernel's bit for bit.
+template <bool kContacts>
+__global__ __launch_bounds__(kPBlock) void particleRefillKernel(DScene sc, Settings set, ParticleBatch pb) {
+    __shared__ uint32_t stack[kRefillStack * kPBlock];
+    PCtx c{sc, set, pb, stack + threadIdx.x};
+    c.sum[0] = c.sum[1] = c.sum[2] = 0.0f;
+    c.segments = c.inconsistencies = c.leaks = 0;
+    const bool observer = sc.obs.kind == OWE_DETECTOR_OBSERVER;
+    const uint32_t lane = __lane_id();
+    PState st;
+    st.depth = st.events = 0;
+    bool live = false, done = false;
+    uint64_t id = 0, end = 0;
+    uint32_t started = 0;
+    for (;;) {
+        const uint64_t idle = __ballot(!live && !done);
+        const bool refill = uint32_t(__popcll(idle)) >= pb.regen || __ballot(live) == 0;
+        const bool needChunk = refill && !live && !done && id >= end;
+        const uint64_t ask = __ballot(needChunk);
+        if (ask) {
+            const int leader = __ffsll((long long)ask) - 1;
+            unsigned long long base = 0;
+            if (int(lane) == leader) base = atomicAdd(pb.next, (unsigned long long)__popcll(ask) * pb.chunk);
+            base = __shfl(base, leader);
+            if (needChunk) {
+                id = base + uint64_t(__popcll(ask & ((1ull << lane) - 1ull))) * pb.chunk;
+                end = id + pb.chunk < pb.total ? id + pb.chunk : pb.total;
+                done = id >= pb.total;
+            }
+        }
+        if (refill && !live && !done) {
+            startParticle(c, set, pb.base + id, pb.pass);
+            st.depth = 0;
+            st.events = 0;
+            live = emitParticle<kContacts>(c, observer, st.ray);
+            ++started;
+            ++id;
+        }
+        if (__ballot(live || !done) == 0) break;  // every lane has finished and no ids are left
+        if (!live) continue;
+        live = particleSegment<kContacts, false>(c, st, nullptr, 0u);
+    }
+    const uint32_t n[4] = {waveSum(started), waveSum(c.segments), waveSum(c.inconsistencies), waveSum(c.leaks)};
+    if (lane == 0)
 #pragma unroll
         for (int k = 0; k < 4; ++k)
             if (n[k]) atomicAdd(&pb.stats[k], (unsigned long long)n[k]);
@@ -658,6 +743,15 @@ void launchParticles(const DScene& sc, const Settings& set, const ParticleBatch&
         else if (stackCap <= 32) hipLaunchKernelGGL((particleKernel<32, false>), grid, block, 0, stream, sc, set, b);
         else hipLaunchKernelGGL((particleKernel<48, false>), grid, block, 0, stream, sc, set, b);
     }
+    OWE_HIP_CHECK(hipGetLastError());
+}
+
+void launchParticlesRefill(const DScene& sc, const Settings& set, const ParticleBatch& b, uint32_t blocks,
+                           hipStream_t stream) {
+    if (b.total == 0) return;
+    const dim3 grid{std::max(1u, blocks)}, block{kPBlock};
+    if (sc.contacts) hipLaunchKernelGGL((particleRefillKernel<true>), grid, block, 0, stream, sc, set, b);
+    else hipLaunchKernelGGL((particleRefillKernel<false>), grid, block, 0, stream, sc, set, b);
     OWE_HIP_CHECK(hipGetLastError());
 }
 

@@ -24,10 +24,21 @@ struct ParticleBatch {
     double* perParticle;        // when not null: count × 3 XYZ sums per particle instead of the film
     unsigned long long* stats;  // += paths, segments, inconsistencies, leaks
     float invNWY;               // 1 / (NW ∫ȳ)
+    // The lane-refill form: particles base + [0, total) of the pass, handed out `chunk` ids at a time
+    // from the device counter `next` (zero at launch); a wave starts new particles once `regen` of
+    // its lanes are idle.
+    uint64_t total;
+    unsigned long long* next;
+    uint32_t chunk, regen;
 };
 
-// Enqueues the batch on `stream`; stackCap (16, 32 or 48) bounds the BVH traversal stack.
+// Enqueues the batch on `stream`, one thread per particle; stackCap (16, 32 or 48) bounds the BVH
+// traversal stack.
 void launchParticles(const DScene& sc, const Settings& set, const ParticleBatch& b, uint32_t stackCap,
                      hipStream_t stream);
+// Enqueues the pass in the lane-refill form on a grid of `blocks` blocks (the scene's stack bound
+// must not exceed 48).
+void launchParticlesRefill(const DScene& sc, const Settings& set, const ParticleBatch& b, uint32_t blocks,
+                           hipStream_t stream);
 
 }  // namespace owehip

@@ -1192,6 +1192,7 @@ struct owe_hip_renderer {
     // Light integrator: the exact splat bins, the pass's counters, and the particles emitted so far.
     unsigned long long* lightFilm = nullptr;
     unsigned long long* particleStats = nullptr;
+    unsigned long long* particleNext = nullptr;  // the lane-refill particle form's id counter
     double particleCount = 0;
     uint64_t* particleIndex = nullptr;
     double* perParticle = nullptr;
@@ -1510,17 +1511,35 @@ void runParticles(owe_hip_renderer* r, uint64_t total, uint32_t pass, const uint
     if (!list) r->particleCount += double(total);
     if (!(r->host.emittedPower > 0)) return;  // nothing emits: the particle estimate is zero
     OWE_HIP_CHECK(hipMemsetAsync(r->particleStats, 0, 4 * sizeof(unsigned long long), r->stream));
-    for (uint64_t base = 0; base < total; base += r->capacity) {
+    if (r->mode == OWE_HIP_MODE_PERSISTENT && !list && !perParticle) {
+        // The lane-refill form: the whole pass in one launch on a grid sized to the chip.
         ParticleBatch b{};
-        b.base = list ? 0 : base;
-        b.index = list ? list + base : nullptr;
-        b.count = uint32_t(std::min<uint64_t>(r->capacity, total - base));
+        b.total = total;
         b.pass = pass;
         b.film = r->lightFilm;
-        b.perParticle = perParticle ? perParticle + 3 * base : nullptr;
         b.stats = r->particleStats;
         b.invNWY = r->invNWY;
-        timed(r, kShade, [&] { launchParticles(r->dev, r->set, b, r->stackCap, r->stream); });
+        b.next = r->particleNext;
+        b.chunk = r->chunk;
+        b.regen = r->regen;
+        OWE_HIP_CHECK(hipMemsetAsync(r->particleNext, 0, sizeof(unsigned long long), r->stream));
+        timed(r, kShade, [&] {
+            launchParticlesRefill(r->dev, r->set, b, uint32_t(std::max(1, r->computeUnits * r->blocksPerCU)), r->stream);
+        });
+    } else {
+        // One thread per particle, in launches of at most `capacity` particles.
+        for (uint64_t base = 0; base < total; base += r->capacity) {
+            ParticleBatch b{};
+            b.base = list ? 0 : base;
+            b.index = list ? list + base : nullptr;
+            b.count = uint32_t(std::min<uint64_t>(r->capacity, total - base));
+            b.pass = pass;
+            b.film = r->lightFilm;
+            b.perParticle = perParticle ? perParticle + 3 * base : nullptr;
+            b.stats = r->particleStats;
+            b.invNWY = r->invNWY;
+            timed(r, kShade, [&] { launchParticles(r->dev, r->set, b, r->stackCap, r->stream); });
+        }
     }
     unsigned long long st[4];
     OWE_HIP_CHECK(hipMemcpyAsync(st, r->particleStats, sizeof st, hipMemcpyDeviceToHost, r->stream));
@@ -1774,6 +1793,13 @@ int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render
             r->lightFilm = devAlloc<unsigned long long>(r, 3 * kSplatBins * pixels);
             OWE_HIP_CHECK(hipMemset(r->lightFilm, 0, 3 * kSplatBins * pixels * sizeof(unsigned long long)));
             r->particleStats = devAlloc<unsigned long long>(r, 4);
+            r->particleNext = devAlloc<unsigned long long>(r, 1);
+            hipDeviceProp_t prop;
+            OWE_HIP_CHECK(hipGetDeviceProperties(&prop, device));
+            r->computeUnits = prop.multiProcessorCount;
+            r->blocksPerCU = 8;
+            r->chunk = 4;
+            r->regen = 32;
         }
         *out = r;
     });
@@ -2046,8 +2072,8 @@ int owe_hip_renderer_set_mode(owe_hip_renderer* r, int mode, int blocks_per_cu, 
     return guard([&] {
         require(r != nullptr, "renderer is null");
         require(mode == OWE_HIP_MODE_WAVEFRONT || mode == OWE_HIP_MODE_PERSISTENT, "unknown mode");
-        require(mode == OWE_HIP_MODE_WAVEFRONT || r->integrator == OWE_INTEGRATOR_PATH,
-                "the persistent form traces camera paths");
+        require(mode == OWE_HIP_MODE_WAVEFRONT || r->integrator != OWE_INTEGRATOR_SPPM,
+                "the persistent forms trace camera paths or particles (not SPPM iterations)");
         require(blocks_per_cu >= 0 && blocks_per_cu <= 64, "blocks per compute unit must lie in [0, 64]");
         require(chunk >= 0 && chunk <= 1024, "chunk must lie in [0, 1024]");
         require(waves_per_simd == 0 || waves_per_simd == 2 || waves_per_simd == 3, "waves per SIMD must be 0, 2 or 3");
@@ -2212,7 +2238,9 @@ int owe_hip_renderer_autotune(owe_hip_renderer* r, int spp, double* best_ms) {
     return guard([&] {
         require(r != nullptr, "renderer is null");
         require(spp > 0, "spp must be positive");
-        require(r->integrator == OWE_INTEGRATOR_PATH, "the autotuner tunes the camera-path forms");
+        require(r->integrator == OWE_INTEGRATOR_PATH ||
+                    (r->integrator == OWE_INTEGRATOR_LIGHT && r->mode == OWE_HIP_MODE_PERSISTENT),
+                "the autotuner tunes the camera-path forms and the particle pass's lane-refill form");
         require(r->passes == 0, "autotune a fresh (or reset) renderer: its passes would enter the film");
         OWE_HIP_CHECK(hipSetDevice(r->device));
         const std::string key = tuningKey(r, spp);
@@ -2245,6 +2273,7 @@ int owe_hip_renderer_autotune(owe_hip_renderer* r, int spp, double* best_ms) {
         const bool graph = r->useGraph;
         auto run = [&](const TuningRecord& c) {
             applyTuning(r, c);
+            if (r->integrator == OWE_INTEGRATOR_LIGHT) return timePass([&] { runParticles(r, total, 0u); });
             if (r->mode == OWE_HIP_MODE_PERSISTENT) {
                 hipLaunchKernelGGL(setPass, dim3{1}, dim3{64}, 0, r->stream, r->passSlot, 0u);
                 return timePass([&] { launchPersistent(r, persistArgs(r, total, uint32_t(spp))); });
@@ -2255,7 +2284,12 @@ int owe_hip_renderer_autotune(owe_hip_renderer* r, int spp, double* best_ms) {
                     runPaths(r, base, uint32_t(std::min<uint64_t>(cap, total - base)), uint32_t(spp), 0u, false, nullptr);
             });
         };
-        if (r->mode == OWE_HIP_MODE_PERSISTENT) {
+        if (r->integrator == OWE_INTEGRATOR_LIGHT) {
+            // Refill threshold × blocks per compute unit × particle ids per lane fetch.
+            for (int g : {1, 16, 32, 48})
+                for (int b : {2, 4, 8, 16})
+                    for (int c : {1, 4, 16}) cands.push_back({r->mode, b, c, r->persistWaves, g, 0, 0, 0.0});
+        } else if (r->mode == OWE_HIP_MODE_PERSISTENT) {
             // Register budget (waves per SIMD) × refill threshold × blocks per compute unit × path
             // ids per lane fetch.
             r->useGraph = false;
@@ -2292,6 +2326,9 @@ int owe_hip_renderer_autotune(owe_hip_renderer* r, int spp, double* best_ms) {
         // The tuning passes leave nothing behind.
         const size_t n = 3 * size_t(r->host.obs.width) * size_t(r->host.obs.height);
         OWE_HIP_CHECK(hipMemsetAsync(r->film, 0, n * sizeof(unsigned long long), r->stream));
+        if (r->lightFilm)
+            OWE_HIP_CHECK(hipMemsetAsync(r->lightFilm, 0, n * kSplatBins * sizeof(unsigned long long), r->stream));
+        r->particleCount = 0;
         OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
         r->stats = {};
         for (int k = 0; k < kTimers; ++k) {
