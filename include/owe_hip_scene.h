@@ -189,6 +189,8 @@ typedef struct owe_scene {
 #define OWE_INTEGRATOR_PATH 0
 #define OWE_INTEGRATOR_LIGHT 1
 #define OWE_INTEGRATOR_SPPM 2
+#define OWE_INTEGRATOR_VCM 3
+#define OWE_INTEGRATOR_BDPT 4
 
 /* Wavelength sampling: the reference's four stratified hero wavelengths importance-sampled over
  * 360–830 nm (core/wavelengths.hpp:36-45), or one wavelength per path uniform over
@@ -207,7 +209,10 @@ typedef struct owe_render_settings {
     /* Stochastic progressive photon mapping (RenderSettings, scene/scene.hpp:27-35): the initial
      * gather radius in pixel footprints, the radius-reduction alpha in (0, 1], photons per pixel and
      * iteration, and wavelength groups per iteration (1–255). A zero selects the reference's
-     * default (2, 2/3, 1, 4); the other integrators ignore them. */
+     * default (2, 2/3, 1, 4). Vertex connection and merging reads the same fields as its merging
+     * radius, its shrink, its light subpaths per pixel and iteration and its wavelength groups
+     * (1–8, default 1: vcmWavelengthGroups, scene.hpp:38); bdpt ignores the radius. The path and
+     * light integrators ignore them. */
     double photon_radius, photon_alpha, photons_per_pixel;
     int32_t photon_groups;
     int32_t reserved;

@@ -24,6 +24,7 @@
 #include "particle.hpp"
 #include "shading_dev.hpp"
 #include "sppm.hpp"
+#include "vcm.hpp"
 
 namespace owehip {
 namespace {
@@ -197,9 +198,21 @@ struct PRay {
     bool distant;
 };
 
+// What vertex connection and merging needs of an emission (Emitted, particle.slang:192-202): the
+// pdf of the emitted position and direction with the choice of the light, next-event estimation's
+// pdf of a distant direction, the |cos| at an area emitter (1 for distant light), and the area
+// emitter's light index, point and normal in its own frame.
+struct EmitInfo {
+    bool distant;
+    float emissionPdfW, directPdfW, usedCos;
+    uint32_t light;
+    f3 pL, nL;
+};
+
 // emitParticle (particle.slang:306-325): the sky or a light by power, then its emission.
 template <bool kContacts, bool kGuide = false>
-__device__ __forceinline__ bool emitParticle(PCtx& c, bool observer, PRay& ray, const SppmDev* g = nullptr) {
+__device__ __forceinline__ bool emitParticle(PCtx& c, bool observer, PRay& ray, const SppmDev* g = nullptr,
+                                             EmitInfo* info = nullptr) {
     const DScene& sc = c.sc;
     const float pSky = sc.env.skyShare;
     const bool fromSky = c.rng.uniform() < pSky;
@@ -268,6 +281,13 @@ __device__ __forceinline__ bool emitParticle(PCtx& c, bool observer, PRay& ray, 
         const float w = kGuide ? R * R / (sel * pdfDir * pdfUnit) : kPiF * R * R / (sel * pdfDir);
 #pragma unroll
         for (int i = 0; i < NW; ++i) c.beta[i] = Le[i] * w;
+        if (info) {
+            info->distant = true;
+            info->emissionPdfW = 1.0f / w;
+            info->directPdfW = fromSky ? 0.0f : sc.lights[li].pdf / sc.env.sunSolidAngle;
+            info->usedCos = 1.0f;
+            info->light = kNoBoundary;
+        }
     } else {
         // emitFromArea (particle.slang:281-304): a point, a side, a cosine-distributed direction.
         const float sel = (1.0f - pSky) * sc.lights[li].pdf;
@@ -317,6 +337,15 @@ __device__ __forceinline__ bool emitParticle(PCtx& c, bool observer, PRay& ray, 
         const float w = kPiF / (sel * pdfA * sidePdf);
 #pragma unroll
         for (int i = 0; i < NW; ++i) c.beta[i] = Le[i] * w;
+        if (info) {
+            info->distant = false;
+            info->usedCos = fabsf(dot3(n, ray.d));
+            info->emissionPdfW = sel * pdfA * sidePdf * info->usedCos * kInvPiF;
+            info->directPdfW = 0.0f;
+            info->light = li;
+            info->pL = pl;
+            info->nL = nl;
+        }
     }
     bool any = c.beta[0] != 0.0f;
 #pragma unroll
@@ -683,6 +712,194 @@ __global__ __launch_bounds__(kPBlock) void particleRefillKernel(DScene sc, Setti
             if (n[k]) atomicAdd(&pb.stats[k], (unsigned long long)n[k]);
 }
 
+// ---------------------------------------------------------------- vertex connection and merging
+
+// Into group `group`'s cache (cacheLightVertex, vcm.slang:199-227).
+__device__ __forceinline__ void vcmCache(const VcmDev& v, const Hit& h, f3 d, const float beta[NW], const VcmMis& mis,
+                                         bool terminated, uint32_t group) {
+    atomicAdd(&v.counts[0], 1u);
+    const uint32_t slot = atomicAdd(&v.counts[4 + group], 1u);
+    if (slot >= v.capacity) {
+        atomicAdd(&v.counts[1], 1u);
+        return;
+    }
+    LightVertex& y = v.cache[size_t(group) * v.capacity + slot];
+    const uint32_t face = h.slot != kEmptyChild ? h.face : ~0u;
+    y.p = make_float4(h.p.x, h.p.y, h.p.z, __uint_as_float(h.boundary));
+    y.n = make_float4(h.n.x, h.n.y, h.n.z, mis.dVCM);
+    y.d = make_float4(d.x, d.y, d.z, __uint_as_float((terminated ? kLvTerminated : 0u) | ((face + 1u) << 1)));
+    y.beta = make_float4(beta[0], beta[1], beta[2], beta[3]);
+    y.mis = make_float4(mis.dVC, mis.dVM, __uint_as_float(group), 0.0f);
+}
+
+// A light vertex to the eye (light tracing, t = 1; lightVertexToEye, vcm.slang:229-244): only to the
+// pixels of this path's group this iteration, scaled by G (each pixel takes 1/G of the light paths).
+template <bool kContacts>
+__device__ __forceinline__ void vcmToEye(PCtx& c, const VcmDev& v, const Hit& h, const Iface& it, f3 d,
+                                         const VcmMis& mis, uint32_t group) {
+    Link k;
+    if (!linkObserver(c, h.p, k)) return;
+    if (vcmPixelGroup(uint32_t(k.py) * uint32_t(c.sc.obs.width) + uint32_t(k.px), v.iteration, v.groups) != group) return;
+    const DScene& sc = c.sc;
+    float f[NW], pdfFwd;
+    if (!vcmEval(sc, it, h.boundary, c.wl, d, k.dir, false, f, pdfFwd)) return;
+    float pdfRev;
+    evalScatter(sc, it, -k.dir, -d, true, pdfRev);
+    const float cosX = fabsf(dot3(h.n, k.dir));
+    const float cameraPdfA = k.factor * cosX;
+    const float wLight = cameraPdfA / v.lightPaths * (vcmEta(v, h.p) + mis.dVCM + mis.dVC * pdfRev);
+    const float w = 1.0f / (wLight + 1.0f);
+    const DBoundary& b = sc.boundaries[h.boundary];
+    const uint32_t start = dot3(k.dir, h.n) > 0.0f ? b.front : b.back;
+    float Tr[NW];
+    if (!walkTransmittance<kContacts>(sc, h.p, k.dir, len3(k.q - h.p), start, h.boundary,
+                                      h.slot != kEmptyChild ? h.face : ~0u, kNoBoundary, sc.obs.region, c.wl.lambda,
+                                      c.stack, kPBlock, Tr))
+        return;
+    float val[NW];
+    const float s = cosX * k.factor * w * float(v.groups);
+#pragma unroll
+    for (int i = 0; i < NW; ++i) val[i] = c.beta[i] * f[i] * Tr[i] * s;
+    deposit(c, k.px, k.py, val);
+}
+
+// One light subpath (traceLightVcm, vcm.slang:246-292): its non-specular vertices cached and
+// connected to the eye.
+template <bool kContacts>
+__device__ __forceinline__ void vcmLightPath(PCtx& c, const VcmDev& v, uint32_t group) {
+    const DScene& sc = c.sc;
+    const int maxDepth = c.set.maxDepth, rrDepth = c.set.rrDepth;
+    PRay ray;
+    EmitInfo em;
+    if (!emitParticle<kContacts>(c, false, ray, nullptr, &em) || !(em.emissionPdfW > 0.0f)) return;
+    VcmMis mis{0.0f, 0.0f, 0.0f};
+    float segment = 0.0f;  // distance since the last scattering vertex (null boundaries do not scatter)
+    bool first = true;
+    for (int depth = 0;; ++depth) {
+        Hit h;
+        bool found = intersectContact<true, kContacts>(sc, ray.o, ray.d, __builtin_inff(), ray.departed, ray.face,
+                                                       kNoBoundary, ray.region, c.stack, kPBlock, h);
+        if (found && depth == 0 && ray.distant) {
+            const float back = 1e-5f * (1.0f + fmaxf(fabsf(h.p.x), fmaxf(fabsf(h.p.y), fabsf(h.p.z))));
+            Hit again;
+            if (intersectWorld(sc, h.p - ray.d * back, ray.d, 4.0f * back, kNoBoundary, ~0u, kNoBoundary, c.stack,
+                               kPBlock, again))
+                h = again;
+        }
+        c.segments++;
+        const DMedium& med = sc.media[sc.regionMedium[ray.region]];
+        if (med.flags & 1) {  // attenuation only (scattering media are refused)
+            if (!found) return;
+#pragma unroll
+            for (int i = 0; i < NW; ++i) c.beta[i] *= expf(-evalSpec(med.absorption, sc.table, c.wl.lambda[i]) * h.t);
+        }
+        if (!found) {
+            if (ray.region != 0) c.leaks++;
+            return;
+        }
+        const uint32_t bi = h.boundary;
+        const DBoundary& b = sc.boundaries[bi];
+        const bool fromFront = dot3(ray.d, h.n) < 0.0f;
+        const uint32_t inc = fromFront ? b.front : b.back, other = fromFront ? b.back : b.front;
+        if (inc != ray.region) {
+            c.inconsistencies++;
+            ray.region = inc;
+        }
+        segment += h.t;
+        const DOptics& o = sc.optics[b.optics];
+        const uint32_t hitFace = h.slot != kEmptyChild ? h.face : ~0u;
+        if (o.type == OWE_SURF_NULL) {
+            ray.region = other;
+            ray.o = h.p;
+            ray.departed = bi;
+            ray.face = hitFace;
+            if (depth >= 4 * maxDepth) return;
+            continue;
+        }
+        if (o.type == OWE_SURF_DETECTOR || o.type == OWE_SURF_ABSORBER) return;
+        const f3 d = ray.d;
+        if (first) {
+            // firstLightVertexMis (vcm.slang:185-197): next-event estimation would have chosen the
+            // emitter point from here.
+            mis.dVC = em.usedCos / em.emissionPdfW;
+            if (em.distant) {
+                mis.dVCM = em.directPdfW / em.emissionPdfW;
+            } else {
+                const DBoundary& lb = sc.boundaries[sc.lights[em.light].boundary];
+                const float directPdfW = sc.lights[em.light].pdf * pdfFrom(lb, xfPoint(lb.toLocal, h.p), em.pL, em.nL);
+                mis.dVCM = directPdfW * em.usedCos / em.emissionPdfW;
+            }
+            mis.dVM = mis.dVC * vcmInvEta(v, h.p);
+        }
+        mis.arrive(first ? 1.0f : segment, fabsf(dot3(h.n, d)));
+        first = false;
+        segment = 0.0f;
+        const float hero = c.wl.lambda[0];
+        const Iface it{&o, h.n, regionIndex(sc, b.front, hero), regionIndex(sc, b.back, hero), hero, c.set.fresnelFloor};
+        if (!o.delta) {
+            vcmCache(v, h, d, c.beta, mis, secondaryTerminated(c.wl), group);
+            vcmToEye<kContacts>(c, v, h, it, d, mis, group);
+        }
+        if (depth >= maxDepth) return;
+        // terminateIfDispersive (transport.slang:639-648).
+        if (o.type == OWE_SURF_DIELECTRIC && !secondaryTerminated(c.wl)) {
+            const float a0 = it.nBack / it.nFront;
+#pragma unroll
+            for (int i = 1; i < NW; ++i) {
+                if (c.wl.pdf[i] == 0.0f) continue;
+                const Iface ai = vcmIface(sc, bi, h.n, c.wl.lambda[i], c.set.fresnelFloor);
+                if (fabsf(a0 - ai.nBack / ai.nFront) > 1e-12f * fabsf(a0)) {
+                    c.wl.pdf[0] /= NW;
+                    c.wl.pdf[1] = c.wl.pdf[2] = c.wl.pdf[3] = 0.0f;
+                    c.beta[1] = c.beta[2] = c.beta[3] = 0.0f;
+                    break;
+                }
+            }
+        }
+        const float u3 = c.rng.uniform(), u2 = c.rng.uniform(), u1 = c.rng.uniform();
+        SSample s;
+        if (!sampleScatter(sc, it, d, u1, u2, u3, false, s) || s.weight <= 0.0f) return;
+        c.beta[0] *= s.weight;
+#pragma unroll
+        for (int i = 1; i < NW; ++i)
+            if (c.wl.pdf[i] != 0.0f)
+                c.beta[i] *= secondaryWeight(sc, vcmIface(sc, bi, h.n, c.wl.lambda[i], c.set.fresnelFloor), d, s, false);
+        float pdfRev = 0.0f;
+        if (!s.delta) evalScatter(sc, it, -s.wi, -d, true, pdfRev);
+        mis.scatter(h.n, s, pdfRev, vcmEta(v, h.p));
+        if (s.transmitted) ray.region = other;
+        ray.o = h.p;
+        ray.d = s.wi;
+        ray.departed = bi;
+        ray.face = hitFace;
+        ray.distant = false;
+        if (depth >= rrDepth && s.weight < 1.0f && !roulette(c, s.weight)) return;
+    }
+}
+
+// One thread per light subpath j of the iteration: its own stream (seed, j, iteration) and the
+// wavelengths of its group, j mod G (vcm_light_kernel.slang).
+template <bool kContacts>
+__global__ __launch_bounds__(kPBlock) void vcmLightKernel(DScene sc, Settings set, VcmDev v, ParticleBatch pb) {
+    __shared__ uint32_t stack[kRefillStack * kPBlock];
+    const uint32_t j = blockIdx.x * kPBlock + threadIdx.x;
+    PCtx c{sc, set, pb, stack + threadIdx.x};
+    c.sum[0] = c.sum[1] = c.sum[2] = 0.0f;
+    c.segments = c.inconsistencies = c.leaks = 0;
+    const bool live = j < pb.count;
+    if (live) {
+        c.rng.seed(hashPair(set.seed ^ kVcmLight, j), v.iteration);
+        const uint32_t group = j % v.groups;
+        c.wl = sppmWavelengths(set, v.rotation, group, v.groups);
+        vcmLightPath<kContacts>(c, v, group);
+    }
+    const uint32_t n[4] = {waveSum(live ? 1u : 0u), waveSum(c.segments), waveSum(c.inconsistencies), waveSum(c.leaks)};
+    if (__lane_id() == 0)
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (n[k]) atomicAdd(&pb.stats[k], (unsigned long long)n[k]);
+}
+
 // One thread per SPPM photon (sppm_photon_kernel): photon i of the iteration has its own stream
 // (seed, i, iteration) and the wavelengths of its group, i mod G.
 template <int kCap, bool kContacts>
@@ -743,6 +960,20 @@ void launchParticles(const DScene& sc, const Settings& set, const ParticleBatch&
         else if (stackCap <= 32) hipLaunchKernelGGL((particleKernel<32, false>), grid, block, 0, stream, sc, set, b);
         else hipLaunchKernelGGL((particleKernel<48, false>), grid, block, 0, stream, sc, set, b);
     }
+    OWE_HIP_CHECK(hipGetLastError());
+}
+
+void launchVcmLight(const DScene& sc, const Settings& set, const VcmDev& v, unsigned long long* lightFilm,
+                    unsigned long long* stats, float invNWY, uint32_t count, hipStream_t stream) {
+    if (count == 0) return;
+    ParticleBatch pb{};
+    pb.count = count;
+    pb.film = lightFilm;
+    pb.stats = stats;
+    pb.invNWY = invNWY;
+    const dim3 grid{(count + kPBlock - 1) / kPBlock}, block{kPBlock};
+    if (sc.contacts) hipLaunchKernelGGL((vcmLightKernel<true>), grid, block, 0, stream, sc, set, v, pb);
+    else hipLaunchKernelGGL((vcmLightKernel<false>), grid, block, 0, stream, sc, set, v, pb);
     OWE_HIP_CHECK(hipGetLastError());
 }
 

@@ -32,6 +32,7 @@
 #include "particle.hpp"
 #include "shading_dev.hpp"
 #include "sppm.hpp"
+#include "vcm.hpp"
 
 using namespace owehip;
 
@@ -1169,6 +1170,447 @@ __global__ void sppmUpdateKernel(SppmDev s) {
 }  // namespace
 }  // namespace owehip
 
+// ---------------------------------------------------------------- vertex connection and merging
+
+// The iteration's cached light vertices into the merging hash, each at the level of the merging
+// radius where it lies (vcm_grid_kernel.slang).
+__global__ void vcmGridKernel(VcmDev v) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t g = i / max(v.capacity, 1u);
+    if (g >= v.groups || i - g * v.capacity >= min(v.counts[4 + g], v.capacity)) return;
+    const float4 y = v.cache[i].p;
+    const f3 p = mk3(y.x, y.y, y.z);
+    const int level = sppmLevel(vcmRadius(v, p));
+    const float size = ldexpf(1.0f, level);
+    const uint32_t h = sppmHash(sppmCell(p.x, size), sppmCell(p.y, size), sppmCell(p.z, size), level, v.gridMask);
+    v.next[i] = atomicExch(&v.head[h], i);
+}
+
+// XYZ of a contribution per wavelength (Wavelengths::toXYZ), with the wavelengths the joined subpaths
+// share: when the light side left only its hero, only the hero counts (joinedXYZ, vcm.slang:319-324).
+__device__ __forceinline__ f3 vcmXYZ(const WL& camera, bool lightTerminated, const float v[NW], float invNWY) {
+    WL w = camera;
+    if (lightTerminated && !secondaryTerminated(w)) {
+        w.pdf[0] /= NW;
+        w.pdf[1] = w.pdf[2] = w.pdf[3] = 0.0f;
+    }
+    f3 c = mk3(0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+        if (w.pdf[k] == 0.0f || v[k] == 0.0f) continue;
+        c = c + cieXYZ(w.lambda[k]) * (v[k] / w.pdf[k] * invNWY);
+    }
+    return c;
+}
+
+// The camera side of an iteration's light emission pdfs (vcm.slang:105-143).
+__device__ __forceinline__ float vcmAreaEmissionPdf(const DScene& sc, const DBoundary& b, uint32_t li, float cosL) {
+    const DEmission& e = sc.emissions[b.emission];
+    const float side = e.front && e.back ? 0.5f : 1.0f;
+    return (1.0f - sc.env.skyShare) * sc.lights[li].pdf / b.area * side * fabsf(cosL) * kInvPiF;
+}
+__device__ __forceinline__ float vcmDistantEmissionPdf(const DScene& sc, float sel, float pdfDir) {
+    const float R = sc.env.emitRadius;
+    return sel * pdfDir / (kPiF * R * R);
+}
+
+struct VcmCamera {
+    const DScene& sc;
+    const Settings& set;
+    const VcmDev& v;
+    Rng rng;
+    WL wl;
+    float beta[NW];
+    VcmMis mis;
+    uint32_t* stack;
+    uint32_t group, M;  // the pixel's group and its cache's vertex count
+    f3 L;               // XYZ so far
+};
+
+// A non-specular camera vertex: what every strategy at it shares.
+struct VcmVertex {
+    Hit h;
+    Iface it;
+    f3 d;
+    float eta;
+    uint32_t face;
+};
+
+template <bool kContacts>
+__device__ __forceinline__ bool vcmWalk(const VcmCamera& c, f3 o, f3 dir, float dist, uint32_t start, uint32_t departed,
+                                        uint32_t face, uint32_t exclude, uint32_t arrival, float Tr[NW]) {
+    return walkTransmittance<kContacts>(c.sc, o, dir, dist, start, departed, face, exclude, arrival, c.wl.lambda, c.stack,
+                                        kBlock, Tr);
+}
+
+// Next-event estimation (s = 1; cameraNextEvent, vcm.slang:402-425): the light sampled as the path
+// tracer samples it, weighted against the BSDF sample and the light subpaths that leave it.
+template <bool kContacts>
+__device__ __forceinline__ void vcmNextEvent(VcmCamera& c, const VcmVertex& x) {
+    const DScene& sc = c.sc;
+    if (sc.nLights == 0) return;
+    const float ul = c.rng.uniform();
+    uint32_t li = 0;
+    while (li + 1 < sc.nLights && sc.lights[li].cdf <= ul) ++li;
+    const DLight Lt = sc.lights[li];
+    const float u1 = c.rng.uniform(), u2 = c.rng.uniform(), u3 = c.rng.uniform();
+    f3 wi;
+    float dist = __builtin_inff(), lpdf, cosAtLight = 1.0f, emissionPdf;
+    float Le[NW];
+    uint32_t arrival = kRegionNone, exclude = kNoBoundary;
+    if (Lt.kind == 1) {
+        const Frame f = makeFrame(ld3(sc.env.sunDir));
+        wi = toWorld(f, uniformConeOmc(u1, u2, sc.env.sunOneMinusCos));
+#pragma unroll
+        for (int i = 0; i < NW; ++i) Le[i] = c.wl.pdf[i] != 0.0f ? evalSpec(sc.env.sunRadiance, sc.table, c.wl.lambda[i]) : 0.0f;
+        lpdf = Lt.pdf / sc.env.sunSolidAngle;
+        emissionPdf = vcmDistantEmissionPdf(sc, (1.0f - sc.env.skyShare) * Lt.pdf, 1.0f / sc.env.sunSolidAngle);
+    } else {
+        const DBoundary& b = sc.boundaries[Lt.boundary];
+        f3 pl, nl;
+        float pdfW;
+        if (!sampleFrom(b, xfPoint(b.toLocal, x.h.p), u1, u2, u3, pl, nl, pdfW)) return;
+        const f3 ep = xfPoint(b.toWorld, pl), en = norm3(xfVector(b.toWorld, nl));
+        const f3 dv = ep - x.h.p;
+        dist = len3(dv);
+        if (!(dist > 0.0f)) return;
+        wi = dv / dist;
+        const DEmission& e = sc.emissions[b.emission];
+        const float cosL = dot3(en, -wi);
+        const bool lit = (cosL > 0.0f && e.front) || (cosL < 0.0f && e.back);
+#pragma unroll
+        for (int i = 0; i < NW; ++i) Le[i] = lit && c.wl.pdf[i] != 0.0f ? evalSpec(e.radiance, sc.table, c.wl.lambda[i]) : 0.0f;
+        lpdf = Lt.pdf * pdfW;
+        cosAtLight = fabsf(cosL);
+        emissionPdf = vcmAreaEmissionPdf(sc, b, li, cosL);
+        arrival = cosL > 0.0f ? b.front : b.back;
+        exclude = Lt.boundary;
+    }
+    if (!(lpdf > 0.0f) || !(cosAtLight > 0.0f)) return;
+    float f[NW], pdfFwd;
+    if (!vcmEval(sc, x.it, x.h.boundary, c.wl, x.d, wi, true, f, pdfFwd)) return;
+    float pdfRev;
+    evalScatter(sc, x.it, -wi, -x.d, true, pdfRev);
+    const float cosToLight = fabsf(dot3(x.h.n, wi));
+    const float wLight = pdfFwd / lpdf;
+    const float wCamera = emissionPdf * cosToLight / (lpdf * cosAtLight) * (x.eta + c.mis.dVCM + c.mis.dVC * pdfRev);
+    const float w = 1.0f / (wLight + 1.0f + wCamera);
+    const DBoundary& hb = sc.boundaries[x.h.boundary];
+    const uint32_t start = dot3(wi, x.h.n) > 0.0f ? hb.front : hb.back;
+    float Tr[NW];
+    if (!vcmWalk<kContacts>(c, x.h.p, wi, dist, start, x.h.boundary, x.face, exclude, arrival, Tr)) return;
+    float val[NW];
+    const float s = cosToLight * w / lpdf;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) val[i] = c.beta[i] * f[i] * Le[i] * Tr[i] * s;
+    c.L = c.L + vcmXYZ(c.wl, false, val, c.v.invNWY);
+}
+
+// A connection to one light vertex y drawn uniformly from the group's cache of M (s, t ≥ 2;
+// cameraConnect, vcm.slang:428-462).
+template <bool kContacts>
+__device__ __forceinline__ void vcmConnect(VcmCamera& c, const VcmVertex& x) {
+    const DScene& sc = c.sc;
+    const VcmDev& v = c.v;
+    if (c.M == 0) return;
+    const uint32_t pick = min(uint32_t(c.rng.uniform() * float(c.M)), c.M - 1);
+    const LightVertex y = v.cache[size_t(c.group) * v.capacity + pick];
+    const f3 yp = mk3(y.p.x, y.p.y, y.p.z), yn = mk3(y.n.x, y.n.y, y.n.z), yd = mk3(y.d.x, y.d.y, y.d.z);
+    const f3 dv = yp - x.h.p;
+    const float dist2 = dot3(dv, dv);
+    if (!(dist2 > 0.0f)) return;
+    const float dist = sqrtf(dist2);
+    const f3 dir = dv / dist;
+    const uint32_t yflags = __float_as_uint(y.d.w), yb = __float_as_uint(y.p.w);
+    const bool lightTerminated = (yflags & kLvTerminated) != 0u;
+    WL wj = c.wl;
+    if (lightTerminated && !secondaryTerminated(wj)) {
+        wj.pdf[0] /= NW;
+        wj.pdf[1] = wj.pdf[2] = wj.pdf[3] = 0.0f;
+    }
+    float fc[NW], pdfCamFwd;
+    if (!vcmEval(sc, x.it, x.h.boundary, wj, x.d, dir, true, fc, pdfCamFwd)) return;
+    const Iface hl = vcmIface(sc, yb, yn, wj.lambda[0], c.set.fresnelFloor);
+    float fl[NW], pdfLightFwd;
+    if (!vcmEval(sc, hl, yb, wj, yd, -dir, false, fl, pdfLightFwd)) return;
+    float pdfCamRev, pdfLightRev;
+    evalScatter(sc, x.it, -dir, -x.d, true, pdfCamRev);
+    evalScatter(sc, hl, dir, -yd, true, pdfLightRev);
+    const float cosX = fabsf(dot3(x.h.n, dir)), cosY = fabsf(dot3(yn, dir));
+    const float camPdfA = pdfCamFwd * cosY / dist2, lightPdfA = pdfLightFwd * cosX / dist2;
+    const float wLight = camPdfA * (vcmEta(v, yp) + y.n.w + y.mis.x * pdfLightRev);
+    const float wCamera = lightPdfA * (x.eta + c.mis.dVCM + c.mis.dVC * pdfCamRev);
+    const float w = 1.0f / (wLight + 1.0f + wCamera);
+    const DBoundary& xb = sc.boundaries[x.h.boundary];
+    const DBoundary& lb = sc.boundaries[yb];
+    const uint32_t start = dot3(dir, x.h.n) > 0.0f ? xb.front : xb.back;
+    const uint32_t arrival = dot3(-dir, yn) > 0.0f ? lb.front : lb.back;
+    // The walk stops 1e-5 of the segment short of y (its own surface is the far end).
+    float Tr[NW];
+    if (!walkTransmittance<kContacts>(sc, x.h.p, dir, dist * (1.0f - 1e-5f), start, x.h.boundary, x.face, kNoBoundary,
+                                      arrival, wj.lambda, c.stack, kBlock, Tr))
+        return;
+    const float g = cosX * cosY / dist2 * w * float(c.M) / v.lightPaths;
+    const float yb4[NW] = {y.beta.x, y.beta.y, y.beta.z, y.beta.w};
+    float val[NW];
+#pragma unroll
+    for (int i = 0; i < NW; ++i) val[i] = c.beta[i] * fc[i] * yb4[i] * fl[i] * Tr[i] * g;
+    c.L = c.L + vcmXYZ(c.wl, lightTerminated, val, c.v.invNWY);
+}
+
+// Merging with the group's light vertices within the merging radius of the vertex (cameraMerge,
+// vcm.slang:465-520).
+__device__ __forceinline__ void vcmMerge(VcmCamera& c, const VcmVertex& x) {
+    const DScene& sc = c.sc;
+    const VcmDev& v = c.v;
+    if (!(v.mergePerMetre > 0.0f)) return;
+    const f3 p = x.h.p;
+    const float r = vcmRadius(v, p), r2 = r * r;
+    const float invEta = 1.0f / x.eta;
+    const float dx = len3(p);
+    const float near = dx > 0.0f ? fmaxf(dx - r, 0.0f) / dx : 0.0f, far = dx > 0.0f ? (dx + r) / dx : 1.0f;
+    // The levels light vertices within reach live on: their radii span those at the nearest and
+    // farthest distance from the eye in reach (almost always one level; at most the top four are
+    // searched, which leaves out only vertices within a merging radius of the eye itself).
+    const int levelHi = sppmLevel(vcmRadius(v, p * far));
+    const int levelLo = max(sppmLevel(vcmRadius(v, p * near)), levelHi - 3);
+    for (int level = levelLo; level <= levelHi; ++level) {
+        const float size = ldexpf(1.0f, level);
+        const f3 q = p - mk3(0.5f, 0.5f, 0.5f) * size;
+        const int bx = sppmCell(q.x, size), by = sppmCell(q.y, size), bz = sppmCell(q.z, size);
+        for (int c8 = 0; c8 < 8; ++c8) {
+            const int cx = bx + (c8 & 1), cy = by + ((c8 >> 1) & 1), cz = bz + ((c8 >> 2) & 1);
+            for (uint32_t i = v.head[sppmHash(cx, cy, cz, level, v.gridMask)]; i != kGridNone; i = v.next[i]) {
+                const float4 vp = v.cache[i].p;
+                const f3 yp = mk3(vp.x, vp.y, vp.z);
+                const f3 dp = yp - p;
+                if (dot3(dp, dp) > r2) continue;
+                if (sppmLevel(vcmRadius(v, yp)) != level || sppmCell(yp.x, size) != cx || sppmCell(yp.y, size) != cy ||
+                    sppmCell(yp.z, size) != cz)
+                    continue;
+                const LightVertex y = v.cache[i];
+                const f3 yn = mk3(y.n.x, y.n.y, y.n.z);
+                if (fabsf(dot3(dp, x.h.n)) > 0.25f * r || dot3(yn, x.h.n) < 0.5f) continue;
+                if (__float_as_uint(y.mis.z) != c.group) continue;
+                const f3 wi = -mk3(y.d.x, y.d.y, y.d.z);
+                const bool lightTerminated = (__float_as_uint(y.d.w) & kLvTerminated) != 0u;
+                WL wj = c.wl;
+                if (lightTerminated && !secondaryTerminated(wj)) {
+                    wj.pdf[0] /= NW;
+                    wj.pdf[1] = wj.pdf[2] = wj.pdf[3] = 0.0f;
+                }
+                float fc[NW], pdfFwd;
+                if (!vcmEval(sc, x.it, x.h.boundary, wj, x.d, wi, true, fc, pdfFwd)) continue;
+                float pdfRev;
+                evalScatter(sc, x.it, -wi, -x.d, true, pdfRev);
+                const float wLight = y.n.w * invEta + y.mis.y * pdfFwd;
+                const float wCamera = c.mis.dVCM * invEta + c.mis.dVM * pdfRev;
+                const float w = 1.0f / (wLight + 1.0f + wCamera);
+                const float yb4[NW] = {y.beta.x, y.beta.y, y.beta.z, y.beta.w};
+                float val[NW];
+#pragma unroll
+                for (int k = 0; k < NW; ++k) val[k] = c.beta[k] * fc[k] * yb4[k] * (w * invEta);
+                c.L = c.L + vcmXYZ(c.wl, lightTerminated, val, c.v.invNWY);
+            }
+        }
+    }
+}
+
+// One camera path of pixel `pixel` (traceCameraVcm, vcm.slang:523-593); its contributions, every
+// strategy but light tracing, go to the camera film.
+template <bool kContacts>
+__global__ __launch_bounds__(kBlock) void vcmCameraKernel(DScene sc, Settings set, VcmDev v, unsigned long long* stats) {
+    __shared__ uint32_t stack[48 * kBlock];
+    const uint32_t pixel = blockIdx.x * kBlock + threadIdx.x;
+    const DObserver& ob = sc.obs;
+    const uint32_t pixels = uint32_t(ob.width) * uint32_t(ob.height);
+    uint32_t nPaths = 0, nSegments = 0, nInconsistent = 0, nLeaks = 0;
+    if (pixel < pixels) {
+        VcmCamera c{sc, set, v};
+        c.stack = stack + threadIdx.x;
+        c.rng.seed(hashPair(set.seed ^ kVcmCamera, pixel), v.iteration);
+        c.group = vcmPixelGroup(pixel, v.iteration, v.groups);
+        c.wl = sppmWavelengths(set, v.rotation, c.group, v.groups);
+        c.M = min(v.counts[4 + c.group], v.capacity);
+        c.L = mk3(0.0f, 0.0f, 0.0f);
+        PathStart ps;
+        cameraRay(sc, pixel % uint32_t(ob.width), pixel / uint32_t(ob.width), c.rng, make_float4(0, 0, 0, 0),
+                  make_float4(1, 1, 1, 1), ps);
+        const f3 fwd = ld3(ob.fwd);
+        const float cosCam = dot3(ps.d, fwd);
+        const float W = float(ob.width), H = float(ob.height);
+        const float pixelArea = (2.0f * ob.tanX / W) * (2.0f * ob.tanY / H);
+        if (ps.weight > 0.0f && cosCam > 0.0f) {
+            ++nPaths;
+            const float cameraPdfW = 1.0f / (pixelArea * cosCam * cosCam * cosCam);
+            c.mis = VcmMis{v.lightPaths / cameraPdfW, 0.0f, 0.0f};
+#pragma unroll
+            for (int i = 0; i < NW; ++i) c.beta[i] = c.wl.pdf[i] != 0.0f ? ps.weight : 0.0f;
+            f3 o = ps.o, d = ps.d, prevP = ps.o;
+            uint32_t region = ps.region, departed = ps.departed, face = ~0u;
+            float segment = 0.0f;
+            int pathLength = 0;
+            const int maxDepth = set.maxDepth, rrDepth = set.rrDepth;
+            for (int depth = 0;; ++depth) {
+                Hit h;  // with its geometry: every strategy below reads the point and the normal
+                const bool found = intersectContact<true, kContacts>(sc, o, d, __builtin_inff(), departed, face, kNoBoundary,
+                                                                     region, c.stack, kBlock, h);
+                ++nSegments;
+                if (found && !(isfinite(h.p.x + h.p.y + h.p.z) && isfinite(h.n.x + h.n.y + h.n.z))) break;
+                const DMedium& med = sc.media[sc.regionMedium[region]];
+                if (med.flags & 1) {  // attenuation only (scattering media are refused)
+                    if (!found) break;
+#pragma unroll
+                    for (int i = 0; i < NW; ++i) c.beta[i] *= expf(-evalSpec(med.absorption, sc.table, c.wl.lambda[i]) * h.t);
+                }
+                if (!found) {
+                    if (region == 0) {
+                        // cameraSkyMet (vcm.slang:360-376): the sky and the sun, weighted after a vertex.
+                        const f3 sd = ld3(sc.env.sunDir), dd = d - sd;
+                        const bool inSun = sc.env.hasSun && 0.5f * dot3(dd, dd) <= sc.env.sunOneMinusCos;
+                        float wSky = 1.0f, wSun = 1.0f;
+                        if (pathLength > 0) {
+                            wSky = 1.0f / (1.0f + vcmDistantEmissionPdf(sc, sc.env.skyShare, 1.0f / (4.0f * kPiF)) * c.mis.dVC);
+                            float sunLightPdf = 0.0f;
+                            for (uint32_t li = 0; li < sc.nLights; ++li)
+                                if (sc.lights[li].kind == 1) sunLightPdf = sc.lights[li].pdf;
+                            if (inSun && sc.env.sunSolidAngle > 0.0f)
+                                wSun = 1.0f / (1.0f + sunLightPdf / sc.env.sunSolidAngle * c.mis.dVCM +
+                                               vcmDistantEmissionPdf(sc, (1.0f - sc.env.skyShare) * sunLightPdf,
+                                                                     1.0f / sc.env.sunSolidAngle) * c.mis.dVC);
+                        }
+                        float val[NW];
+#pragma unroll
+                        for (int i = 0; i < NW; ++i) {
+                            const float sky = sc.env.sky == OWE_SKY_UNIFORM && c.wl.pdf[i] != 0.0f
+                                                  ? evalSpec(sc.env.skyRadiance, sc.table, c.wl.lambda[i]) : 0.0f;
+                            const float sun = inSun && c.wl.pdf[i] != 0.0f ? evalSpec(sc.env.sunRadiance, sc.table, c.wl.lambda[i]) : 0.0f;
+                            val[i] = c.beta[i] * (sky * wSky + sun * wSun);
+                        }
+                        c.L = c.L + vcmXYZ(c.wl, false, val, v.invNWY);
+                    } else {
+                        ++nLeaks;
+                    }
+                    break;
+                }
+                const uint32_t bi = h.boundary;
+                const DBoundary& b = sc.boundaries[bi];
+                const bool fromFront = dot3(d, h.n) < 0.0f;
+                const uint32_t inc = fromFront ? b.front : b.back, other = fromFront ? b.back : b.front;
+                if (inc != region) {
+                    ++nInconsistent;
+                    region = inc;
+                }
+                segment += h.t;
+                const DOptics& o2 = sc.optics[b.optics];
+                const uint32_t hitFace = h.slot != kEmptyChild ? h.face : ~0u;
+                if (o2.type == OWE_SURF_NULL) {
+                    region = other;
+                    o = h.p;
+                    departed = bi;
+                    face = hitFace;
+                    if (depth >= 4 * maxDepth) break;
+                    continue;
+                }
+                const float cosIn = fabsf(dot3(h.n, d));
+                const float dist = segment;
+                c.mis.arrive(dist, cosIn);
+                segment = 0.0f;
+                ++pathLength;
+                // cameraEmissionMet (vcm.slang:380-399).
+                if (b.emission >= 0) {
+                    const DEmission& e = sc.emissions[b.emission];
+                    if (fromFront ? e.front : e.back) {
+                        float w = 1.0f;
+                        if (pathLength > 1 && b.light >= 0) {
+                            f3 pl, nl;
+                            const f3 lo = xfPoint(b.toLocal, o), ld = xfVector(b.toLocal, d);
+                            analyticGeometry(sc, b, lo, ld, h.t, pl, nl);
+                            const float directPdfA = sc.lights[b.light].pdf * pdfFrom(b, xfPoint(b.toLocal, prevP), pl, nl) *
+                                                     cosIn / (dist * dist);
+                            const float emissionPdfW = vcmAreaEmissionPdf(sc, b, uint32_t(b.light), cosIn);
+                            w = 1.0f / (1.0f + directPdfA * c.mis.dVCM + emissionPdfW * c.mis.dVC);
+                        }
+                        float val[NW];
+#pragma unroll
+                        for (int i = 0; i < NW; ++i)
+                            val[i] = c.wl.pdf[i] != 0.0f ? c.beta[i] * evalSpec(e.radiance, sc.table, c.wl.lambda[i]) * w : 0.0f;
+                        c.L = c.L + vcmXYZ(c.wl, false, val, v.invNWY);
+                    }
+                }
+                if (o2.type == OWE_SURF_ABSORBER || o2.type == OWE_SURF_DETECTOR) break;
+                if (depth >= maxDepth) break;
+                const float hero = c.wl.lambda[0];
+                const Iface it{&o2, h.n, regionIndex(sc, b.front, hero), regionIndex(sc, b.back, hero), hero,
+                               set.fresnelFloor};
+                // terminateIfDispersive (transport.slang:639-648).
+                if (o2.type == OWE_SURF_DIELECTRIC && !secondaryTerminated(c.wl)) {
+                    const float a0 = it.nBack / it.nFront;
+#pragma unroll
+                    for (int i = 1; i < NW; ++i) {
+                        if (c.wl.pdf[i] == 0.0f) continue;
+                        const Iface ai = vcmIface(sc, bi, h.n, c.wl.lambda[i], set.fresnelFloor);
+                        if (fabsf(a0 - ai.nBack / ai.nFront) > 1e-12f * fabsf(a0)) {
+                            c.wl.pdf[0] /= NW;
+                            c.wl.pdf[1] = c.wl.pdf[2] = c.wl.pdf[3] = 0.0f;
+                            c.beta[1] = c.beta[2] = c.beta[3] = 0.0f;
+                            break;
+                        }
+                    }
+                }
+                const float eta = vcmEta(v, h.p);
+                if (!o2.delta) {
+                    const VcmVertex x{h, it, d, eta, hitFace};
+                    vcmNextEvent<kContacts>(c, x);
+                    vcmConnect<kContacts>(c, x);
+                    vcmMerge(c, x);
+                }
+                const float u3 = c.rng.uniform(), u2 = c.rng.uniform(), u1 = c.rng.uniform();
+                SSample s;
+                if (!sampleScatter(sc, it, d, u1, u2, u3, true, s) || s.weight <= 0.0f) break;
+                c.beta[0] *= s.weight;
+#pragma unroll
+                for (int i = 1; i < NW; ++i)
+                    if (c.wl.pdf[i] != 0.0f)
+                        c.beta[i] *= secondaryWeight(sc, vcmIface(sc, bi, h.n, c.wl.lambda[i], set.fresnelFloor), d, s, true);
+                float pdfRev = 0.0f;
+                if (!s.delta) evalScatter(sc, it, -s.wi, -d, true, pdfRev);
+                c.mis.scatter(h.n, s, pdfRev, eta);
+                if (s.transmitted) region = other;
+                prevP = h.p;
+                o = h.p;
+                d = s.wi;
+                departed = bi;
+                face = hitFace;
+                if (depth >= rrDepth) {
+                    float q = 0.0f;
+#pragma unroll
+                    for (int i = 0; i < NW; ++i) q = fmaxf(q, c.beta[i]);
+                    q = fminf(1.0f, q);
+                    if (c.rng.uniform() >= q) break;
+#pragma unroll
+                    for (int i = 0; i < NW; ++i) c.beta[i] /= q;
+                }
+            }
+            if (isfinite(c.L.x + c.L.y + c.L.z)) {
+                unsigned long long* fp = v.film + 3 * size_t(pixel);
+                const long long a = fixedPoint(c.L.x), bb = fixedPoint(c.L.y), cc = fixedPoint(c.L.z);
+                if (a) atomicAdd(&fp[0], (unsigned long long)a);
+                if (bb) atomicAdd(&fp[1], (unsigned long long)bb);
+                if (cc) atomicAdd(&fp[2], (unsigned long long)cc);
+            }
+        }
+    }
+    const uint32_t n[4] = {nPaths, nSegments, nInconsistent, nLeaks};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        uint32_t t = n[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
+        if (__lane_id() == 0 && t) atomicAdd(&stats[k], (unsigned long long)t);
+    }
+}
+
 // ---------------------------------------------------------------- the renderer object
 
 struct owe_hip_renderer {
@@ -1206,6 +1648,15 @@ struct owe_hip_renderer {
     uint64_t photons = 0;
     uint64_t sppmIterations = 0;
     owe_transport_stats stats{};
+    // Vertex connection and merging: the light vertex cache, its counters and merging hash, light
+    // subpaths per iteration (all groups), the merging radius per metre at the first iteration, and
+    // the iterations so far.
+    owehip::VcmDev vcm{};
+    uint64_t vcmLightPaths = 0;
+    uint32_t vcmGridSize = 0;
+    float vcmRadiusPerMetre = 0.0f;
+    float vcmAlpha = 2.0f / 3.0f;
+    uint64_t vcmIterations = 0;
     hipStream_t stream = nullptr;
     uint32_t stackCap = 16;
     bool simpleShadows = true;
@@ -1577,6 +2028,52 @@ void launchSppmCamera(owe_hip_renderer* r) {
 // The call's statistics and counters are cleared on the render stream, and the camera stream waits
 // for that clear before its first pass (its other waits are on events of earlier passes, recorded
 // before the clear), so no camera-pass count lands in a counter that is then zeroed.
+// Vertex connection and merging: per iteration the light subpaths fill the cache (and splat to the
+// eye), the cached vertices enter the merging hash (vcm), then one camera path per pixel
+// (GpuRenderer::runVcmIterations, gpu_renderer_passes.cpp:136-199). The cache holds every vertex a
+// subpath can leave, so it never overflows; the kernels check that it did not.
+void runVcm(owe_hip_renderer* r, int iterations) {
+    VcmDev v = r->vcm;
+    const bool merging = r->vcmRadiusPerMetre > 0.0f;
+    OWE_HIP_CHECK(hipMemsetAsync(r->particleStats, 0, 4 * sizeof(unsigned long long), r->stream));
+    for (int it = 0; it < iterations; ++it) {
+        const uint64_t iteration = r->vcmIterations + uint64_t(it);
+        v.iteration = uint32_t(iteration);
+        v.rotation = float(sppmRotation(r->set.seed, iteration));
+        v.mergePerMetre =
+            merging ? float(double(r->vcmRadiusPerMetre) * std::pow(double(iteration + 1), (double(r->vcmAlpha) - 1.0) / 2.0))
+                    : 0.0f;
+        OWE_HIP_CHECK(hipMemsetAsync(v.counts, 0, 16 * sizeof(uint32_t), r->stream));
+        if (merging) OWE_HIP_CHECK(hipMemsetAsync(v.head, 0xFF, size_t(r->vcmGridSize) * sizeof(uint32_t), r->stream));
+        if (r->host.emittedPower > 0)
+            launchVcmLight(r->dev, r->set, v, r->lightFilm, r->particleStats, r->invNWY, uint32_t(r->vcmLightPaths), r->stream);
+        if (merging) {
+            const uint64_t n = uint64_t(v.capacity) * v.groups;
+            hipLaunchKernelGGL(vcmGridKernel, dim3{unsigned((n + 255) / 256)}, dim3{256}, 0, r->stream, v);
+        }
+        const uint32_t pixels = uint32_t(r->host.obs.width) * uint32_t(r->host.obs.height);
+        if (r->dev.contacts)
+            hipLaunchKernelGGL((vcmCameraKernel<true>), gridFor(pixels), dim3{kBlock}, 0, r->stream, r->dev, r->set, v,
+                               r->particleStats);
+        else
+            hipLaunchKernelGGL((vcmCameraKernel<false>), gridFor(pixels), dim3{kBlock}, 0, r->stream, r->dev, r->set, v,
+                               r->particleStats);
+        OWE_HIP_CHECK(hipGetLastError());
+    }
+    uint32_t counts[2] = {0, 0};
+    unsigned long long st[4];
+    OWE_HIP_CHECK(hipMemcpyAsync(counts, v.counts, sizeof counts, hipMemcpyDeviceToHost, r->stream));
+    OWE_HIP_CHECK(hipMemcpyAsync(st, r->particleStats, sizeof st, hipMemcpyDeviceToHost, r->stream));
+    OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
+    if (counts[1] != 0) fail(OWE_HIP_ERR_DEVICE, "vcm: the light vertex cache overflowed");
+    r->vcmIterations += uint64_t(iterations);
+    r->particleCount += double(r->vcmLightPaths) * double(iterations);
+    r->stats.paths += st[0];
+    r->stats.segments += st[1];
+    r->stats.inconsistencies += st[2];
+    r->stats.leaks += st[3];
+}
+
 void runSppm(owe_hip_renderer* r, int iterations) {
     SppmDev& q = r->sppm;
     OWE_HIP_CHECK(hipMemsetAsync(q.stats, 0, 4 * sizeof(unsigned long long), r->stream));
@@ -1642,8 +2139,16 @@ int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render
         const owe_render_settings& s = *settings;
         require(s.max_depth >= 0 && s.rr_depth >= 0, "depths must be non-negative");
         require(s.max_depth <= 4096, "max depth above 4096 is not supported (depth is kept in 16 bits)");
-        if (s.integrator != OWE_INTEGRATOR_PATH && s.integrator != OWE_INTEGRATOR_LIGHT && s.integrator != OWE_INTEGRATOR_SPPM)
-            fail(OWE_HIP_ERR_UNSUPPORTED, "unknown integrator (this backend implements path, light and sppm)");
+        if (s.integrator != OWE_INTEGRATOR_PATH && s.integrator != OWE_INTEGRATOR_LIGHT && s.integrator != OWE_INTEGRATOR_SPPM &&
+            s.integrator != OWE_INTEGRATOR_VCM && s.integrator != OWE_INTEGRATOR_BDPT)
+            fail(OWE_HIP_ERR_UNSUPPORTED, "unknown integrator (this backend implements path, light, sppm, vcm and bdpt)");
+        const bool vcmLike = s.integrator == OWE_INTEGRATOR_VCM || s.integrator == OWE_INTEGRATOR_BDPT;
+        if (vcmLike) {
+            require(s.photon_radius >= 0 && s.photon_alpha >= 0 && s.photons_per_pixel >= 0 && s.photon_groups >= 0,
+                    "photon settings must not be negative");
+            require((s.photon_alpha > 0 ? s.photon_alpha : 2.0 / 3.0) <= 1.0, "photon_alpha must lie in (0, 1]");
+            require(s.photon_groups <= 8, "vcm wavelength groups must be 1 to 8");
+        }
         const double photonRadius = s.photon_radius > 0 ? s.photon_radius : 2.0;
         const double photonAlpha = s.photon_alpha > 0 ? s.photon_alpha : 2.0 / 3.0;
         const double photonsPerPixel = s.photons_per_pixel > 0 ? s.photons_per_pixel : 1.0;
@@ -1668,6 +2173,13 @@ int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render
         if (h.bvh.stackBound > 48)
             fail(OWE_HIP_ERR_UNSUPPORTED, "scene hierarchy needs a traversal stack of " + std::to_string(h.bvh.stackBound) +
                                               " entries; the kernels hold at most 48");
+        if (vcmLike) {
+            if (h.obs.kind != OWE_DETECTOR_OBSERVER)
+                fail(OWE_HIP_ERR_UNSUPPORTED, "vcm and bdpt render for an ideal observer (the reference's GPU backend too)");
+            for (const DMedium& m : h.media)
+                if (m.flags & 2)
+                    fail(OWE_HIP_ERR_UNSUPPORTED, "vcm and bdpt here do not sample scattering media (render with path)");
+        }
         r->stackCap = std::max<uint32_t>(1, h.bvh.stackBound);
         OWE_HIP_CHECK(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
         DScene& d = r->dev;
@@ -1788,6 +2300,41 @@ int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render
             OWE_HIP_CHECK(hipMemset(q.tauXY, 0, pixels * sizeof(double2)));
             OWE_HIP_CHECK(hipMemset(q.tauZN, 0, pixels * sizeof(double2)));
             OWE_HIP_CHECK(hipMemset(q.flux, 0, pixels * 3 * kSplatBins * sizeof(unsigned long long)));
+        } else if (vcmLike) {
+            // Light subpaths per iteration (photons_per_pixel per pixel, a multiple of the groups),
+            // a cache every subpath's vertices fit in (one per scattering event up to max_depth + 1),
+            // the merging hash, and the merging radius: photon_radius pixel footprints per metre from
+            // the eye, shrinking as i^((α − 1)/2) (gpu_renderer_passes.cpp:136-147).
+            const uint32_t G = uint32_t(s.photon_groups > 0 ? s.photon_groups : 1);
+            const uint64_t want = std::max<uint64_t>(1, uint64_t(std::llround(double(pixels) * photonsPerPixel)));
+            r->vcmLightPaths = (want + G - 1) / G * G;
+            require(r->vcmLightPaths <= (1u << 30), "vcm: at most 2^30 light subpaths per iteration");
+            const uint64_t perGroup = r->vcmLightPaths / G * (uint64_t(s.max_depth) + 1);
+            const uint64_t vertices = perGroup * G;
+            if (vertices * sizeof(LightVertex) > (uint64_t(16) << 30) || vertices >= 0xFFFFFFFFull)
+                fail(OWE_HIP_ERR_UNSUPPORTED, "vcm: the light vertex cache for this many subpaths and this depth would "
+                                              "exceed 16 GiB (lower photons_per_pixel or max_depth)");
+            VcmDev& v = r->vcm;
+            v.cache = devAlloc<LightVertex>(r, vertices);
+            v.counts = devAlloc<uint32_t>(r, 16);
+            v.capacity = uint32_t(perGroup);
+            v.groups = G;
+            v.lightPaths = float(r->vcmLightPaths / G);
+            uint64_t gridSize = 1024;
+            while (gridSize < std::min<uint64_t>(2 * vertices, uint64_t(1) << 28)) gridSize <<= 1;
+            r->vcmGridSize = uint32_t(gridSize);
+            v.head = devAlloc<uint32_t>(r, gridSize);
+            v.next = devAlloc<uint32_t>(r, vertices);
+            v.gridMask = uint32_t(gridSize - 1);
+            v.film = r->film;
+            v.invNWY = r->invNWY;
+            v.mergeFloor = 1e-9f;
+            r->vcmRadiusPerMetre = s.integrator == OWE_INTEGRATOR_VCM
+                                       ? float(photonRadius * 2.0 * double(h.obs.tanY) / double(h.obs.height)) : 0.0f;
+            r->vcmAlpha = float(photonAlpha);
+            r->lightFilm = devAlloc<unsigned long long>(r, 3 * kSplatBins * pixels);
+            OWE_HIP_CHECK(hipMemset(r->lightFilm, 0, 3 * kSplatBins * pixels * sizeof(unsigned long long)));
+            r->particleStats = devAlloc<unsigned long long>(r, 4);
         } else {
             r->capacity = uint32_t(std::min<uint64_t>(pixels * 64, 1u << 22));  // particles per launch
             r->lightFilm = devAlloc<unsigned long long>(r, 3 * kSplatBins * pixels);
@@ -1870,6 +2417,12 @@ int owe_hip_renderer_pass(owe_hip_renderer* r, int spp) {
         const uint64_t total = uint64_t(r->host.obs.width) * uint64_t(r->host.obs.height) * uint64_t(spp);
         if (r->integrator == OWE_INTEGRATOR_LIGHT) {
             runParticles(r, total, uint32_t(r->passes));
+            r->passes++;
+            r->totalSpp += spp;
+            return;
+        }
+        if (r->integrator == OWE_INTEGRATOR_VCM || r->integrator == OWE_INTEGRATOR_BDPT) {  // spp iterations
+            runVcm(r, spp);
             r->passes++;
             r->totalSpp += spp;
             return;
@@ -2002,6 +2555,7 @@ int owe_hip_renderer_reset(owe_hip_renderer* r) {
             OWE_HIP_CHECK(hipMemsetAsync(q.guideCounts, 0, kGuideCells * sizeof(unsigned int), r->stream));
             r->sppmIterations = 0;
         }
+        r->vcmIterations = 0;
         OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
         r->passes = 0;
         r->totalSpp = 0;
@@ -2072,8 +2626,8 @@ int owe_hip_renderer_set_mode(owe_hip_renderer* r, int mode, int blocks_per_cu, 
     return guard([&] {
         require(r != nullptr, "renderer is null");
         require(mode == OWE_HIP_MODE_WAVEFRONT || mode == OWE_HIP_MODE_PERSISTENT, "unknown mode");
-        require(mode == OWE_HIP_MODE_WAVEFRONT || r->integrator != OWE_INTEGRATOR_SPPM,
-                "the persistent forms trace camera paths or particles (not SPPM iterations)");
+        require(mode == OWE_HIP_MODE_WAVEFRONT || r->integrator == OWE_INTEGRATOR_PATH || r->integrator == OWE_INTEGRATOR_LIGHT,
+                "the persistent forms trace camera paths or particles (not SPPM or VCM iterations)");
         require(blocks_per_cu >= 0 && blocks_per_cu <= 64, "blocks per compute unit must lie in [0, 64]");
         require(chunk >= 0 && chunk <= 1024, "chunk must lie in [0, 1024]");
         require(waves_per_simd == 0 || waves_per_simd == 2 || waves_per_simd == 3, "waves per SIMD must be 0, 2 or 3");

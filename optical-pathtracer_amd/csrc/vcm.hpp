@@ -1,0 +1,132 @@
+// Vertex connection and merging (the reference's vcm and bdpt integrators, src/owe/backends/gpu/
+// shaders/vcm.slang, after Georgiev, Křivánek, Davidovič & Slusallek 2012, with a light vertex cache
+// in place of per-pixel light subpaths): the state the light pass (particle.hip) and the camera pass
+// (render.hip) share, and the recursive MIS quantities both ends carry.
+//
+// One iteration: N_L light subpaths leave the lights as particles do; each non-specular vertex goes
+// into its wavelength group's cache with its throughput and Georgiev's dVCM, dVC, dVM, and is
+// connected to the eye (light tracing, splatted exactly); with merging the cached vertices enter the
+// multi-level hash of the SPPM pass at the level of the merging radius where they lie; then one
+// camera path per pixel takes, at every non-specular vertex, next-event estimation, a connection to
+// one vertex drawn uniformly from its group's cache (weighted by the cache's size over N_L) and, with
+// merging, every cached vertex of its group within the merging radius; emission it meets is weighted
+// too. Every strategy is weighted by the balance heuristic over all the others. Participating media
+// only attenuate here (a scene whose media scatter is refused).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "sppm.hpp"
+
+namespace owehip {
+
+// Stream keys of this backend: light subpath j of an iteration, the camera path of (pixel, iteration),
+// the pixel's group.
+constexpr uint64_t kVcmLight = 0x510e527fade682d1ull, kVcmCamera = 0x9b05688c2b3e6c1full,
+                   kVcmGroup = 0x1f83d9abfb41bd6bull;
+
+// A cached light vertex (80 B).
+struct LightVertex {
+    float4 p;     // camera-relative position; w: boundary (bits)
+    float4 n;     // geometric normal (toward the boundary's front); w: dVCM
+    float4 d;     // direction the light arrived along; w: flags (bits): kLvTerminated | (face + 1) << 1
+    float4 beta;  // the subpath's throughput arriving (per wavelength)
+    float4 mis;   // x: dVC, y: dVM, z: group (bits), w: unused
+};
+constexpr uint32_t kLvTerminated = 1u;
+
+struct VcmDev {
+    LightVertex* cache;    // groups × capacity vertices; group g's are [g·capacity, g·capacity + count)
+    uint32_t* counts;      // [0] vertices offered, [1] vertices that did not fit, [4 + g] group g's count
+    uint32_t capacity;     // vertices per group
+    uint32_t groups;       // wavelength groups G
+    float lightPaths;      // N_L: light subpaths per group this iteration
+    uint32_t iteration;
+    float rotation;        // the iteration's base wavelength rotation
+    float mergePerMetre;   // merging radius per metre of distance from the eye; 0: no merging (bdpt)
+    float mergeFloor;
+    uint32_t* head;        // the merging hash: bucket → first vertex (kGridNone: none) ...
+    uint32_t* next;        // ... vertex → next vertex of its bucket
+    uint32_t gridMask;
+    unsigned long long* film;  // the camera film (32.32 fixed point, three channels per pixel)
+    float invNWY;              // 1 / (NW ∫ȳ)
+};
+
+__device__ __forceinline__ uint32_t vcmPixelGroup(uint32_t pixel, uint32_t iteration, uint32_t groups) {
+    return uint32_t(mix64((uint64_t(iteration) << 32 | pixel) ^ kVcmGroup) % uint64_t(groups));
+}
+
+__device__ __forceinline__ float vcmRadius(const VcmDev& v, f3 p) {
+    return fmaxf(v.mergePerMetre * len3(p), v.mergeFloor);
+}
+// η = π r² N_L: merging's weight relative to connections at p (0 without merging), and its inverse.
+__device__ __forceinline__ float vcmEta(const VcmDev& v, f3 p) {
+    if (!(v.mergePerMetre > 0.0f)) return 0.0f;
+    const float r = vcmRadius(v, p);
+    return kPiF * r * r * v.lightPaths;
+}
+__device__ __forceinline__ float vcmInvEta(const VcmDev& v, f3 p) {
+    const float e = vcmEta(v, p);
+    return e > 0.0f ? 1.0f / e : 0.0f;
+}
+
+// A subpath's MIS quantities (Georgiev's recursive dVCM, dVC, dVM; vcm.slang:150-178), carried from
+// vertex to vertex, in the balance heuristic.
+struct VcmMis {
+    float dVCM, dVC, dVM;
+
+    // Arriving at a vertex `dist` beyond the last one at |cos| cosIn.
+    __device__ __forceinline__ void arrive(float dist, float cosIn) {
+        dVCM *= dist * dist;
+        dVCM /= cosIn;
+        dVC /= cosIn;
+        dVM /= cosIn;
+    }
+    // After the scattering sample s at a vertex of normal n, arrived along d; pdfRev is the
+    // reverse sample's pdf (arriving along −s.wi, leaving along −d), eta the vertex's η.
+    __device__ __forceinline__ void scatter(f3 n, const SSample& s, float pdfRev, float eta) {
+        const float cosOut = fabsf(dot3(n, s.wi));
+        if (s.delta) {
+            dVCM = 0.0f;
+            dVC *= cosOut;
+            dVM *= cosOut;
+            return;
+        }
+        dVC = cosOut / s.pdf * (dVC * pdfRev + dVCM + eta);
+        dVM = cosOut / s.pdf * (dVM * pdfRev + dVCM * (eta > 0.0f ? 1.0f / eta : 0.0f) + 1.0f);
+        dVCM = 1.0f / s.pdf;
+    }
+};
+
+// The interface a surface at boundary bi presents at wavelength lambda, with normal n.
+__device__ __forceinline__ Iface vcmIface(const DScene& sc, uint32_t bi, f3 n, float lambda, float floor) {
+    const DBoundary& b = sc.boundaries[bi];
+    return Iface{&sc.optics[b.optics], n, regionIndex(sc, b.front, lambda), regionIndex(sc, b.back, lambda), lambda,
+                 floor};
+}
+
+// f per wavelength of the interface at boundary bi (hero interface `it`) for arrival d and departure
+// wi, with the hero's pdf; wavelengths with pdf 0 give 0.
+__device__ __forceinline__ bool vcmEval(const DScene& sc, const Iface& it, uint32_t bi, const WL& wl, f3 d, f3 wi,
+                                         bool radiance, float f[NW], float& pdf) {
+    bool any = false;
+    pdf = 0.0f;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+        f[i] = 0.0f;
+        if (wl.pdf[i] == 0.0f) continue;
+        float p;
+        f[i] = i == 0 ? evalScatter(sc, it, d, wi, radiance, p)
+                      : evalScatter(sc, vcmIface(sc, bi, it.n, wl.lambda[i], it.floor), d, wi, radiance, p);
+        if (i == 0) pdf = p;
+        any = any || f[i] != 0.0f;
+    }
+    return any;
+}
+
+// Enqueues an iteration's light subpaths [0, count) (light tracing into the particle film's bins, the
+// cache filled) on `stream`.
+void launchVcmLight(const DScene& sc, const Settings& set, const VcmDev& v, unsigned long long* lightFilm,
+                    unsigned long long* stats, float invNWY, uint32_t count, hipStream_t stream);
+
+}  // namespace owehip

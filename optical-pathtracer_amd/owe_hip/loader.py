@@ -196,14 +196,16 @@ class LoadedScene:
     def settings(self, integrator: str | None = None, seed: int | None = None, max_depth: int | None = None,
                  rr_depth: int = 6, spectral: int = S.SPECTRAL_HERO4) -> S.CRenderSettings:
         """The render block's settings for this backend. The integrator is the block's (or the one
-        given): path, light or sppm (with the block's photon_radius, photon_alpha and photons,
-        scene_loader.cpp:110-121); VCM, BDPT and hybrid are estimators this backend does not have —
-        pass integrator="path" (their unbiased common limit) to render such a scene."""
+        given): path, light, sppm, vcm or bdpt (the photon estimators with the block's photon_radius,
+        photon_alpha and photons, scene_loader.cpp:110-121, and vcm's wavelength groups); hybrid is
+        an estimator this backend does not have — pass integrator="path" (the unbiased common limit)
+        to render such a scene."""
         name = integrator or self.render.get("integrator", "path")
-        kinds = {"path": S.INTEGRATOR_PATH, "light": S.INTEGRATOR_LIGHT, "sppm": S.INTEGRATOR_SPPM}
+        kinds = {"path": S.INTEGRATOR_PATH, "light": S.INTEGRATOR_LIGHT, "sppm": S.INTEGRATOR_SPPM,
+                 "vcm": S.INTEGRATOR_VCM, "bdpt": S.INTEGRATOR_BDPT}
         if name not in kinds:
             raise SceneError(f"{self.source_path}: integrator '{name}' is not available in the hip backend "
-                             "(path, light or sppm)")
+                             "(path, light, sppm, vcm or bdpt)")
         radius = float(self.render.get("photon_radius", 2.0))
         alpha = float(self.render.get("photon_alpha", 2.0 / 3.0))
         photons = float(self.render.get("photons", 1.0))
@@ -213,7 +215,8 @@ class LoadedScene:
                           max_depth=int(self.render.get("max_depth", 64) if max_depth is None else max_depth),
                           rr_depth=rr_depth, spectral=spectral, integrator=kinds[name],
                           fresnel_floor=float(self.render.get("fresnel_floor", 0.0)),
-                          photon_radius=radius, photon_alpha=alpha, photons_per_pixel=photons)
+                          photon_radius=radius, photon_alpha=alpha, photons_per_pixel=photons,
+                          photon_groups=int(self.render.get("vcm_groups", 1)) if name in ("vcm", "bdpt") else None)
 
 
 # ---------------------------------------------------------------- the loader

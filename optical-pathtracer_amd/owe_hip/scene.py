@@ -27,7 +27,7 @@ SURF_NULL, SURF_DIELECTRIC, SURF_DIFFUSE, SURF_CONDUCTOR, SURF_MIRROR, SURF_ABSO
 SHAPE_MESH, SHAPE_PLANE, SHAPE_SPHERE, SHAPE_CYLINDER, SHAPE_SAG, SHAPE_WAVE = 0, 1, 2, 3, 4, 5
 APERTURE_DISK, APERTURE_RECT, APERTURE_ELLIPSE = 0, 1, 2
 SKY_NONE, SKY_UNIFORM = 0, 1
-INTEGRATOR_PATH, INTEGRATOR_LIGHT, INTEGRATOR_SPPM = 0, 1, 2
+INTEGRATOR_PATH, INTEGRATOR_LIGHT, INTEGRATOR_SPPM, INTEGRATOR_VCM, INTEGRATOR_BDPT = 0, 1, 2, 3, 4
 SPECTRAL_HERO4, SPECTRAL_UNIFORM1 = 0, 1
 DETECTOR_OBSERVER, DETECTOR_SENSOR = 0, 1
 
@@ -443,8 +443,12 @@ class Scene:
 
 def settings(seed=1, max_depth=64, rr_depth=6, spectral=SPECTRAL_HERO4, lambda_min=380.0, lambda_max=780.0,
              fresnel_floor=0.0, integrator=INTEGRATOR_PATH, photon_radius=2.0, photon_alpha=2.0 / 3.0,
-             photons_per_pixel=1.0, photon_groups=4) -> CRenderSettings:
-    """RenderSettings (scene/scene.hpp:15-40); the photon_* fields are the sppm integrator's."""
+             photons_per_pixel=1.0, photon_groups=None) -> CRenderSettings:
+    """RenderSettings (scene/scene.hpp:15-40); the photon_* fields are the sppm integrator's, and
+    vcm's (merging radius, its shrink, light subpaths per pixel, wavelength groups) and bdpt's.
+    photon_groups defaults to the reference's: 4 for sppm, 1 for vcm and bdpt."""
+    if photon_groups is None:
+        photon_groups = 1 if integrator in (INTEGRATOR_VCM, INTEGRATOR_BDPT) else 4
     s = CRenderSettings()
     s.seed, s.max_depth, s.rr_depth, s.integrator, s.spectral = seed, max_depth, rr_depth, integrator, spectral
     s.lambda_min, s.lambda_max, s.fresnel_floor = lambda_min, lambda_max, fresnel_floor

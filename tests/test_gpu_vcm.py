@@ -1,0 +1,90 @@
+"""Vertex connection and merging (vcm) and bidirectional path tracing (bdpt) against the closed forms
+and cases the reference checks its own implementation with.
+
+* the white furnace for every integrator, L = Le / (1 − ρ) within 2 % (tests/test_backends.cpp:540-574);
+* the lens caustic seen through a 5 mm N-BK7 plate: through / direct within 0.015 of the plate's
+  ȳ-weighted (1 − R)²/(1 − R²) at 48×48 and 2048 iterations (tests/test_vcm.cpp:311-351 — the eye
+  looks through glass at a point lit through glass: merging's case);
+* both draw the path tracer's image of a scene every strategy reaches (tests/test_vcm.cpp:297-309:
+  mean within 0.6 % there; here within 1.5 % of the path tracer at the sample counts below);
+* light tracing, connections and merging all contribute: vcm's image equals bdpt's in the mean
+  where merging is redundant.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from owe_hip import caustic_scenes as C
+from owe_hip import render
+from owe_hip import scene as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _render(sc, st, passes, spp):
+    r = render.Renderer(sc, st)
+    for _ in range(passes):
+        r.run_pass(spp)
+    return r
+
+
+@pytest.mark.parametrize("integrator", [S.INTEGRATOR_VCM, S.INTEGRATOR_BDPT])
+@pytest.mark.parametrize("rho", [0.0, 0.5])
+def test_white_furnace(cuda, integrator, rho):
+    name = "vcm" if integrator == S.INTEGRATOR_VCM else "bdpt"
+    print(f"{name} furnace rho={rho}: 8x8, 2048 iterations", flush=True)
+    r = _render(S.white_furnace_scene(rho, size=8), S.settings(seed=1, integrator=integrator), 1, 2048)
+    y = r.resolve()[..., 1].mean()
+    s = r.stats()
+    print(f"{name} furnace rho={rho}: {y:.5f} (expected {1 / (1 - rho):.5f}); {s}", flush=True)
+    assert y == pytest.approx(1 / (1 - rho), rel=0.02)
+    assert s["leaks"] == 0 and s["inconsistencies"] == 0
+
+
+def test_vcm_sees_a_caustic_through_a_glass_plate(cuda):
+    cases = C.caustic_cases(resolution=48)
+    means = {}
+    for name in ("direct", "plate"):
+        ls = cases[name]
+        print(f"vcm caustic {name}: 48x48, 2048 iterations", flush=True)
+        r = _render(ls.use(), ls.settings(integrator="vcm"), 1, 2048)
+        means[name] = r.resolve()[..., 1].mean()
+        s = r.stats()
+        print(f"  mean Y {means[name]:.6g}; {s}", flush=True)
+        assert s["leaks"] == 0
+    ratio = means["plate"] / means["direct"]
+    print(f"vcm through the plate {ratio:.4f} (Fresnel {C.plate_transmission():.4f})", flush=True)
+    assert ratio == pytest.approx(C.plate_transmission(), abs=0.015)
+
+
+def _sky_prism():
+    sc = S.dispersion_prism_scene(32, 32, sun=False)
+    sc.set_uniform_sky(S.Spectrum.constant(0.3))
+    return sc
+
+
+@pytest.mark.parametrize("integrator", [S.INTEGRATOR_VCM, S.INTEGRATOR_BDPT])
+def test_draws_the_path_tracers_image(cuda, integrator):
+    name = "vcm" if integrator == S.INTEGRATOR_VCM else "bdpt"
+    sc = _sky_prism()
+    path = _render(sc, S.settings(seed=2, max_depth=16, rr_depth=6), 4, 1024).resolve()[..., 1]
+    print(f"{name} against path on the sky-lit prism, 32x32: path 4096 spp, {name} 4096 iterations", flush=True)
+    img = _render(sc, S.settings(seed=3, max_depth=16, rr_depth=6, integrator=integrator), 4, 1024).resolve()[..., 1]
+    # 4×4-pixel blocks: the relative error of each block's mean.
+    blocks = lambda a: a.reshape(8, 4, 8, 4).mean(axis=(1, 3))  # noqa: E731
+    rel = np.abs(blocks(img) - blocks(path)) / np.maximum(blocks(path), 1e-3 * path.mean())
+    print(f"{name}: mean {img.mean():.6g} path {path.mean():.6g} ratio {img.mean() / path.mean():.4f}; "
+          f"worst block {rel.max():.4f}", flush=True)
+    assert img.mean() / path.mean() == pytest.approx(1.0, abs=0.015)
+    assert np.median(rel) < 0.05
+
+
+def test_unsupported_cases_are_refused(cuda):
+    from owe_hip import _lib
+
+    fog = S.scattering_furnace_scene(1.0, 0.0, 0.0, 0.5, size=8)
+    with pytest.raises(_lib.OweHipError):
+        render.Renderer(fog, S.settings(integrator=S.INTEGRATOR_VCM))
+    with pytest.raises(_lib.OweHipError):
+        render.Renderer(S.submerged_sensor_scene(8), S.settings(integrator=S.INTEGRATOR_BDPT))

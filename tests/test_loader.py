@@ -231,11 +231,22 @@ def test_what_the_backend_cannot_render_is_refused(text, msg):
 
 def test_integrators_of_other_estimators_are_named():
     ls = LD.load_scene_text("observer E { position = (0, -1, 0)  look_at = (0, 0, 0) }\n"
-                            "render { integrator = vcm  seed = 9  max_depth = 12 }")
-    with pytest.raises(F.SceneError, match="integrator 'vcm'"):
+                            "render { integrator = hybrid  seed = 9  max_depth = 12 }")
+    with pytest.raises(F.SceneError, match="integrator 'hybrid'"):
         ls.settings()
     st = ls.settings(integrator="path")
     assert st.seed == 9 and st.max_depth == 12 and st.integrator == S.INTEGRATOR_PATH
+
+
+def test_vcm_and_bdpt_settings_come_from_the_render_block():
+    ls = LD.load_scene_text("observer E { position = (0, -1, 0)  look_at = (0, 0, 0) }\n"
+                            "render { integrator = vcm  photon_radius = 3  photons = 2  vcm_groups = 2 }")
+    st = ls.settings()
+    assert st.integrator == S.INTEGRATOR_VCM
+    assert (st.photon_radius, st.photons_per_pixel, st.photon_groups) == (3.0, 2.0, 2)
+    st = LD.load_scene_text("observer E { position = (0, -1, 0)  look_at = (0, 0, 0) }\n"
+                            "render { integrator = bdpt }").settings()
+    assert st.integrator == S.INTEGRATOR_BDPT and st.photon_groups == 1
 
 
 def test_sppm_settings_come_from_the_render_block():
