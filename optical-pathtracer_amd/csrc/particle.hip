@@ -714,16 +714,11 @@ __global__ __launch_bounds__(kPBlock) void particleRefillKernel(DScene sc, Setti
 
 // ---------------------------------------------------------------- vertex connection and merging
 
-// Into group `group`'s cache (cacheLightVertex, vcm.slang:199-227).
+// Into the light subpath's staging slots (cacheLightVertex, vcm.slang:199-227): vertex n of path p.
 __device__ __forceinline__ void vcmCache(const VcmDev& v, const Hit& h, f3 d, const float beta[NW], const VcmMis& mis,
-                                         bool terminated, uint32_t group) {
-    atomicAdd(&v.counts[0], 1u);
-    const uint32_t slot = atomicAdd(&v.counts[4 + group], 1u);
-    if (slot >= v.capacity) {
-        atomicAdd(&v.counts[1], 1u);
-        return;
-    }
-    LightVertex& y = v.cache[size_t(group) * v.capacity + slot];
+                                         bool terminated, uint32_t group, uint32_t p, uint32_t& n) {
+    if (n >= v.pathSlots) return;  // cannot happen: one cached vertex per scattering event up to max_depth
+    LightVertex& y = v.staging[size_t(p) * v.pathSlots + n++];
     const uint32_t face = h.slot != kEmptyChild ? h.face : ~0u;
     y.p = make_float4(h.p.x, h.p.y, h.p.z, __uint_as_float(h.boundary));
     y.n = make_float4(h.n.x, h.n.y, h.n.z, mis.dVCM);
@@ -766,7 +761,7 @@ __device__ __forceinline__ void vcmToEye(PCtx& c, const VcmDev& v, const Hit& h,
 // One light subpath (traceLightVcm, vcm.slang:246-292): its non-specular vertices cached and
 // connected to the eye.
 template <bool kContacts>
-__device__ __forceinline__ void vcmLightPath(PCtx& c, const VcmDev& v, uint32_t group) {
+__device__ __forceinline__ void vcmLightPath(PCtx& c, const VcmDev& v, uint32_t group, uint32_t p, uint32_t& n) {
     const DScene& sc = c.sc;
     const int maxDepth = c.set.maxDepth, rrDepth = c.set.rrDepth;
     PRay ray;
@@ -837,7 +832,7 @@ __device__ __forceinline__ void vcmLightPath(PCtx& c, const VcmDev& v, uint32_t 
         const float hero = c.wl.lambda[0];
         const Iface it{&o, h.n, regionIndex(sc, b.front, hero), regionIndex(sc, b.back, hero), hero, c.set.fresnelFloor};
         if (!o.delta) {
-            vcmCache(v, h, d, c.beta, mis, secondaryTerminated(c.wl), group);
+            vcmCache(v, h, d, c.beta, mis, secondaryTerminated(c.wl), group, p, n);
             vcmToEye<kContacts>(c, v, h, it, d, mis, group);
         }
         if (depth >= maxDepth) return;
@@ -890,8 +885,11 @@ __global__ __launch_bounds__(kPBlock) void vcmLightKernel(DScene sc, Settings se
     if (live) {
         c.rng.seed(hashPair(set.seed ^ kVcmLight, j), v.iteration);
         const uint32_t group = j % v.groups;
+        const uint32_t p = group * v.pathsPerGroup + j / v.groups;
         c.wl = sppmWavelengths(set, v.rotation, group, v.groups);
-        vcmLightPath<kContacts>(c, v, group);
+        uint32_t n = 0;
+        vcmLightPath<kContacts>(c, v, group, p, n);
+        v.pathCount[p] = n;
     }
     const uint32_t n[4] = {waveSum(live ? 1u : 0u), waveSum(c.segments), waveSum(c.inconsistencies), waveSum(c.leaks)};
     if (__lane_id() == 0)

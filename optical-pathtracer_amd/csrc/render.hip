@@ -1172,12 +1172,56 @@ __global__ void sppmUpdateKernel(SppmDev s) {
 
 // ---------------------------------------------------------------- vertex connection and merging
 
+// The light subpaths' offsets in the cache: an exclusive scan of their vertex counts by one block of
+// kScanThreads threads, each over a contiguous run of the paths (an iteration has at most a few
+// million subpaths); offset[n] receives the total.
+constexpr int kScanThreads = 1024;
+__global__ __launch_bounds__(kScanThreads) void vcmScanKernel(const uint32_t* count, uint32_t* offset, uint32_t n) {
+    __shared__ uint32_t part[kScanThreads];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (n + kScanThreads - 1) / kScanThreads;
+    const uint32_t lo = min(n, t * per), hi = min(n, lo + per);
+    uint32_t sum = 0;
+    for (uint32_t k = lo; k < hi; ++k) sum += count[k];
+    part[t] = sum;
+    __syncthreads();
+    for (uint32_t step = 1; step < kScanThreads; step <<= 1) {  // inclusive Hillis–Steele scan of the runs
+        const uint32_t add = t >= step ? part[t - step] : 0u;
+        __syncthreads();
+        part[t] += add;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;
+    for (uint32_t k = lo; k < hi; ++k) {
+        offset[k] = run;
+        run += count[k];
+    }
+    if (t == kScanThreads - 1) offset[n] = part[t];
+}
+
+// Each light subpath's staged vertices to their place in the cache; thread p < paths.
+__global__ void vcmCompactKernel(VcmDev v, uint32_t paths) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= paths) return;
+    const uint32_t n = v.pathCount[p], o = v.offset[p];
+    for (uint32_t k = 0; k < n; ++k) v.cache[o + k] = v.staging[size_t(p) * v.pathSlots + k];
+}
+
+// Group g's range of the cache: thread g < G.
+__global__ void vcmRangesKernel(VcmDev v) {
+    const uint32_t g = threadIdx.x;
+    if (g == 0) v.counts[0] = v.offset[v.groups * v.pathsPerGroup];
+    if (g >= v.groups) return;
+    const uint32_t first = v.offset[g * v.pathsPerGroup], last = v.offset[(g + 1) * v.pathsPerGroup];
+    v.counts[4 + g] = last - first;
+    v.counts[16 + g] = first;
+}
+
 // The iteration's cached light vertices into the merging hash, each at the level of the merging
 // radius where it lies (vcm_grid_kernel.slang).
 __global__ void vcmGridKernel(VcmDev v) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t g = i / max(v.capacity, 1u);
-    if (g >= v.groups || i - g * v.capacity >= min(v.counts[4 + g], v.capacity)) return;
+    if (i >= min(v.counts[0], v.capacity)) return;
     const float4 y = v.cache[i].p;
     const f3 p = mk3(y.x, y.y, y.z);
     const int level = sppmLevel(vcmRadius(v, p));
@@ -1223,7 +1267,7 @@ struct VcmCamera {
     float beta[NW];
     VcmMis mis;
     uint32_t* stack;
-    uint32_t group, M;  // the pixel's group and its cache's vertex count
+    uint32_t group, M, base;  // the pixel's group, its cache's vertex count and first vertex
     f3 L;               // XYZ so far
 };
 
@@ -1314,7 +1358,7 @@ __device__ __forceinline__ void vcmConnect(VcmCamera& c, const VcmVertex& x) {
     const VcmDev& v = c.v;
     if (c.M == 0) return;
     const uint32_t pick = min(uint32_t(c.rng.uniform() * float(c.M)), c.M - 1);
-    const LightVertex y = v.cache[size_t(c.group) * v.capacity + pick];
+    const LightVertex y = v.cache[c.base + pick];
     const f3 yp = mk3(y.p.x, y.p.y, y.p.z), yn = mk3(y.n.x, y.n.y, y.n.z), yd = mk3(y.d.x, y.d.y, y.d.z);
     const f3 dv = yp - x.h.p;
     const float dist2 = dot3(dv, dv);
@@ -1431,7 +1475,9 @@ __global__ __launch_bounds__(kBlock) void vcmCameraKernel(DScene sc, Settings se
         c.rng.seed(hashPair(set.seed ^ kVcmCamera, pixel), v.iteration);
         c.group = vcmPixelGroup(pixel, v.iteration, v.groups);
         c.wl = sppmWavelengths(set, v.rotation, c.group, v.groups);
-        c.M = min(v.counts[4 + c.group], v.capacity);
+        c.M = v.counts[4 + c.group];
+        c.base = v.counts[16 + c.group];
+        if (c.base > v.capacity || c.M > v.capacity - c.base) c.M = 0;  // cannot happen: the cache holds every vertex
         c.L = mk3(0.0f, 0.0f, 0.0f);
         PathStart ps;
         cameraRay(sc, pixel % uint32_t(ob.width), pixel / uint32_t(ob.width), c.rng, make_float4(0, 0, 0, 0),
@@ -2028,10 +2074,11 @@ void launchSppmCamera(owe_hip_renderer* r) {
 // The call's statistics and counters are cleared on the render stream, and the camera stream waits
 // for that clear before its first pass (its other waits are on events of earlier passes, recorded
 // before the clear), so no camera-pass count lands in a counter that is then zeroed.
-// Vertex connection and merging: per iteration the light subpaths fill the cache (and splat to the
-// eye), the cached vertices enter the merging hash (vcm), then one camera path per pixel
-// (GpuRenderer::runVcmIterations, gpu_renderer_passes.cpp:136-199). The cache holds every vertex a
-// subpath can leave, so it never overflows; the kernels check that it did not.
+// Vertex connection and merging: per iteration the light subpaths stage their vertices (and splat to
+// the eye), the scan and compaction order them into the cache, the cached vertices enter the merging
+// hash (vcm), then one camera path per pixel (GpuRenderer::runVcmIterations,
+// gpu_renderer_passes.cpp:136-199). The cache holds every vertex a subpath can leave, so it never
+// overflows.
 void runVcm(owe_hip_renderer* r, int iterations) {
     VcmDev v = r->vcm;
     const bool merging = r->vcmRadiusPerMetre > 0.0f;
@@ -2043,13 +2090,18 @@ void runVcm(owe_hip_renderer* r, int iterations) {
         v.mergePerMetre =
             merging ? float(double(r->vcmRadiusPerMetre) * std::pow(double(iteration + 1), (double(r->vcmAlpha) - 1.0) / 2.0))
                     : 0.0f;
-        OWE_HIP_CHECK(hipMemsetAsync(v.counts, 0, 16 * sizeof(uint32_t), r->stream));
+        OWE_HIP_CHECK(hipMemsetAsync(v.counts, 0, 32 * sizeof(uint32_t), r->stream));
         if (merging) OWE_HIP_CHECK(hipMemsetAsync(v.head, 0xFF, size_t(r->vcmGridSize) * sizeof(uint32_t), r->stream));
-        if (r->host.emittedPower > 0)
-            launchVcmLight(r->dev, r->set, v, r->lightFilm, r->particleStats, r->invNWY, uint32_t(r->vcmLightPaths), r->stream);
-        if (merging) {
-            const uint64_t n = uint64_t(v.capacity) * v.groups;
-            hipLaunchKernelGGL(vcmGridKernel, dim3{unsigned((n + 255) / 256)}, dim3{256}, 0, r->stream, v);
+        const uint32_t paths = uint32_t(r->vcmLightPaths);
+        if (r->host.emittedPower > 0) {
+            launchVcmLight(r->dev, r->set, v, r->lightFilm, r->particleStats, r->invNWY, paths, r->stream);
+            hipLaunchKernelGGL(vcmScanKernel, dim3{1}, dim3{kScanThreads}, 0, r->stream, v.pathCount, v.offset, paths);
+            hipLaunchKernelGGL(vcmCompactKernel, dim3{(paths + 255) / 256}, dim3{256}, 0, r->stream, v, paths);
+            hipLaunchKernelGGL(vcmRangesKernel, dim3{1}, dim3{64}, 0, r->stream, v);
+            if (merging) {
+                const uint64_t n = v.capacity;
+                hipLaunchKernelGGL(vcmGridKernel, dim3{unsigned((n + 255) / 256)}, dim3{256}, 0, r->stream, v);
+            }
         }
         const uint32_t pixels = uint32_t(r->host.obs.width) * uint32_t(r->host.obs.height);
         if (r->dev.contacts)
@@ -2060,12 +2112,9 @@ void runVcm(owe_hip_renderer* r, int iterations) {
                                r->particleStats);
         OWE_HIP_CHECK(hipGetLastError());
     }
-    uint32_t counts[2] = {0, 0};
     unsigned long long st[4];
-    OWE_HIP_CHECK(hipMemcpyAsync(counts, v.counts, sizeof counts, hipMemcpyDeviceToHost, r->stream));
     OWE_HIP_CHECK(hipMemcpyAsync(st, r->particleStats, sizeof st, hipMemcpyDeviceToHost, r->stream));
     OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
-    if (counts[1] != 0) fail(OWE_HIP_ERR_DEVICE, "vcm: the light vertex cache overflowed");
     r->vcmIterations += uint64_t(iterations);
     r->particleCount += double(r->vcmLightPaths) * double(iterations);
     r->stats.paths += st[0];
@@ -2309,15 +2358,20 @@ int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render
             const uint64_t want = std::max<uint64_t>(1, uint64_t(std::llround(double(pixels) * photonsPerPixel)));
             r->vcmLightPaths = (want + G - 1) / G * G;
             require(r->vcmLightPaths <= (1u << 30), "vcm: at most 2^30 light subpaths per iteration");
-            const uint64_t perGroup = r->vcmLightPaths / G * (uint64_t(s.max_depth) + 1);
-            const uint64_t vertices = perGroup * G;
-            if (vertices * sizeof(LightVertex) > (uint64_t(16) << 30) || vertices >= 0xFFFFFFFFull)
+            const uint64_t slots = uint64_t(s.max_depth) + 1;
+            const uint64_t vertices = r->vcmLightPaths * slots;
+            if (2 * vertices * sizeof(LightVertex) > (uint64_t(16) << 30) || vertices >= 0xFFFFFFFFull)
                 fail(OWE_HIP_ERR_UNSUPPORTED, "vcm: the light vertex cache for this many subpaths and this depth would "
                                               "exceed 16 GiB (lower photons_per_pixel or max_depth)");
             VcmDev& v = r->vcm;
             v.cache = devAlloc<LightVertex>(r, vertices);
-            v.counts = devAlloc<uint32_t>(r, 16);
-            v.capacity = uint32_t(perGroup);
+            v.staging = devAlloc<LightVertex>(r, vertices);
+            v.pathCount = devAlloc<uint32_t>(r, r->vcmLightPaths);
+            v.offset = devAlloc<uint32_t>(r, r->vcmLightPaths + 1);
+            v.counts = devAlloc<uint32_t>(r, 32);
+            v.capacity = uint32_t(vertices);
+            v.pathSlots = uint32_t(slots);
+            v.pathsPerGroup = uint32_t(r->vcmLightPaths / G);
             v.groups = G;
             v.lightPaths = float(r->vcmLightPaths / G);
             uint64_t gridSize = 1024;

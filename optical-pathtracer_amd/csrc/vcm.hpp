@@ -35,10 +35,22 @@ struct LightVertex {
 };
 constexpr uint32_t kLvTerminated = 1u;
 
+// The cache is filled in a deterministic order: light subpath j of the iteration (group g = j mod G,
+// i = j / G) is path p = g·N_L + i; it writes its vertices to staging slots [p·K, p·K + n_p) (K =
+// max_depth + 1 bounds n_p) and n_p to pathCount[p]; an exclusive scan of the counts gives each
+// path's offset, and its vertices are copied to cache[offset_p, offset_p + n_p). Group g's vertices
+// are then the contiguous range [counts[16 + g], counts[16 + g] + counts[4 + g]) in path order — the
+// same cache whatever order the subpaths finish in, so a camera path's connection draw picks the
+// same vertex every time.
 struct VcmDev {
-    LightVertex* cache;    // groups × capacity vertices; group g's are [g·capacity, g·capacity + count)
-    uint32_t* counts;      // [0] vertices offered, [1] vertices that did not fit, [4 + g] group g's count
-    uint32_t capacity;     // vertices per group
+    LightVertex* cache;    // the iteration's vertices, group by group, in path order
+    LightVertex* staging;  // K slots per light subpath
+    uint32_t* pathCount;   // vertices per light subpath (path p = g·N_L + i)
+    uint32_t* offset;      // their exclusive prefix sums; offset[paths] = the total
+    uint32_t* counts;      // [0] the total; [4 + g] group g's vertex count; [16 + g] its first vertex
+    uint32_t capacity;     // vertices the cache holds (every subpath's K)
+    uint32_t pathSlots;    // K
+    uint32_t pathsPerGroup;
     uint32_t groups;       // wavelength groups G
     float lightPaths;      // N_L: light subpaths per group this iteration
     uint32_t iteration;

@@ -300,6 +300,98 @@ void oracle_camera_paths(const void* wh, const owe_render_settings* rs, int spp,
     for (auto& th : pool) th.join();
 }
 
+// Iterations [start, start + n) of vertex connection and merging (vcm.slang; merging off: bdpt) in the
+// hip backend's layout: N = ⌈round(W·H·photons_per_pixel)/G⌉·G light subpaths per iteration, subpath
+// j of group g = j mod G on the stream (hashCombine(seed ^ kVcmLight, j), iteration) at the group's
+// wavelengths; the cache is the subpaths' non-specular vertices in the order (g, j / G, vertex); the
+// camera path of a pixel runs on (hashCombine(seed ^ kVcmCamera, pixel), iteration) in the group a
+// hash of (iteration, pixel) picks, and merges with every cached vertex of its group within the
+// radius (the multi-level hash finds the same set). camera (W·H·3) += the camera paths' XYZ (the
+// caller divides by the iterations); light (W·H·3) += the light subpaths' splats (the caller divides
+// by N times the iterations); stats += paths (light and camera), segments, inconsistencies, leaks;
+// cached[it - start] = the iteration's cache size when cached is given.
+void oracle_vcm(const void* wh, const owe_render_settings* rs, int merging, int n, int start, int threads,
+                double* camera, double* light, uint64_t* stats, uint64_t* cached) {
+    constexpr uint64_t kVcmLight = 0x510e527fade682d1ULL, kVcmCamera = 0x9b05688c2b3e6c1fULL,
+                       kVcmGroup = 0x1f83d9abfb41bd6bULL;
+    const World& w = *static_cast<const World*>(wh);
+    Settings set = toSettings(*rs);
+    const uint32_t G = uint32_t(rs->photon_groups > 0 ? rs->photon_groups : 1);
+    set.photonGroups = int(G);
+    const Tracer tr(w, set);
+    const int W = w.observer.width, H = w.observer.height;
+    const size_t pixels = size_t(W) * size_t(H);
+    const uint64_t want = std::max<uint64_t>(1, uint64_t(std::llround(double(pixels) * set.photonsPerPixel)));
+    const uint64_t N = (want + G - 1) / G * G, perGroup = N / G;
+    if (threads <= 0) threads = int(std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
+    struct Film final : Tracer::Splat {
+        int width = 0;
+        std::vector<double> v;
+        void add(int px, int py, const double c[3]) override {
+            if (!std::isfinite(c[0] + c[1] + c[2])) return;
+            for (int k = 0; k < 3; ++k) v[3 * (size_t(py) * size_t(width) + size_t(px)) + size_t(k)] += c[k];
+        }
+    };
+    auto parallel = [&](uint64_t count, auto&& body) {
+        std::vector<std::thread> pool;
+        for (int k = 0; k < threads; ++k)
+            pool.emplace_back([&, k] {
+                for (uint64_t i = count * uint64_t(k) / uint64_t(threads); i < count * uint64_t(k + 1) / uint64_t(threads); ++i)
+                    body(k, i);
+            });
+        for (auto& th : pool) th.join();
+    };
+    std::vector<Stats> st(static_cast<size_t>(threads));
+    std::vector<Film> films(static_cast<size_t>(threads));
+    for (Film& f : films) {
+        f.width = W;
+        f.v.assign(3 * pixels, 0.0);
+    }
+    const double perMetre = merging ? double(float(set.photonRadius * 2.0 * double(float(w.observer.tanY)) / double(H))) : 0.0;
+    for (int it = start; it < start + n; ++it) {
+        Tracer::VcmParams p;
+        p.lightPaths = double(perGroup);
+        p.groups = G;
+        p.iteration = uint32_t(it);
+        p.mergePerMetre = merging ? perMetre * std::pow(double(it + 1), (set.photonAlpha - 1.0) / 2.0) : 0.0;
+        const double rot = sppmRotation(set.seed, uint64_t(it));
+        std::vector<std::vector<Tracer::VcmVertex>> paths(N);
+        parallel(N, [&](int k, uint64_t j) {
+            const uint32_t g = uint32_t(j % G);
+            Pcg rng(hashPair(set.seed ^ kVcmLight, j), uint64_t(it));
+            tr.vcmLight(sppmWavelengths(set, rot, g), rng, st[size_t(k)], films[size_t(k)], p, g,
+                        paths[size_t(g) * perGroup + j / G]);
+        });
+        std::vector<Tracer::VcmVertex> cache;
+        std::vector<uint64_t> first(G + 1, 0);
+        for (uint32_t g = 0; g < G; ++g) {
+            first[g] = cache.size();
+            for (uint64_t i = 0; i < perGroup; ++i)
+                for (const auto& y : paths[size_t(g) * perGroup + i]) cache.push_back(y);
+        }
+        first[G] = cache.size();
+        if (cached) cached[it - start] = cache.size();
+        paths.clear();
+        parallel(pixels, [&](int k, uint64_t pixel) {
+            const uint32_t g = uint32_t(mix64((uint64_t(uint32_t(it)) << 32 | pixel) ^ kVcmGroup) % uint64_t(G));
+            Pcg rng(hashPair(set.seed ^ kVcmCamera, pixel), uint64_t(it));
+            double c[3];
+            if (tr.vcmCamera(int(pixel % uint64_t(W)), int(pixel / uint64_t(W)), sppmWavelengths(set, rot, g), rng,
+                             st[size_t(k)], p, cache.data() + first[g], uint32_t(first[g + 1] - first[g]), c))
+                for (int q = 0; q < 3; ++q) camera[3 * pixel + size_t(q)] += c[q];
+        });
+    }
+    Stats total;
+    for (int k = 0; k < threads; ++k) {
+        for (size_t j = 0; j < 3 * pixels; ++j) light[j] += films[size_t(k)].v[j];
+        total.add(st[size_t(k)]);
+    }
+    stats[0] += total.paths;
+    stats[1] += total.segments;
+    stats[2] += total.inconsistencies;
+    stats[3] += total.leaks;
+}
+
 // Adds one light-tracing pass into xyz (W·H·3): particles [0, W·H·ppp), particle i on the stream
 // (hashCombine(seed ^ 0x5bd1e995, i), pass) as particle_kernel.slang:15-16, each thread into its own
 // film, summed in thread order. The caller divides by the particle count (gpu_renderer_film.cpp:17-20).

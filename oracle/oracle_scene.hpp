@@ -338,9 +338,33 @@ public:
     bool sppmValue(const VisiblePoint& vp, int groups, V3 d, const Spec4& beta, const Wavelengths& wl,
                    double xyz[3]) const;
 
+    // Vertex connection and merging (the vcm and bdpt integrators, vcm.slang): a light vertex of the
+    // iteration's cache, and an iteration's parameters.
+    struct VcmVertex {
+        V3 p, n, d;  // position, the boundary's geometric normal, the direction the light arrived along
+        Spec4 beta;
+        double dVCM = 0, dVC = 0, dVM = 0;
+        uint32_t boundary = kNone;
+        bool terminated = false;  // the subpath left only its hero wavelength
+    };
+    struct VcmParams {
+        double lightPaths = 1;     // N_L, light subpaths per group
+        double mergePerMetre = 0;  // merging radius per metre from the eye (0: bdpt)
+        uint32_t groups = 1, iteration = 0;
+    };
+    // Light subpath of group `group` (traceLightVcm, vcm.slang:246-292): its non-specular vertices
+    // appended to `cache` in order, its connections to the eye (light tracing) to `out`.
+    void vcmLight(Wavelengths wl, Pcg& rng, Stats& st, Splat& out, const VcmParams& p, uint32_t group,
+                  std::vector<VcmVertex>& cache) const;
+    // The camera path of pixel (x, y) (traceCameraVcm, vcm.slang:523-593) against its group's M cached
+    // vertices: its XYZ from every strategy but light tracing. False when the sample measures nothing.
+    bool vcmCamera(int x, int y, Wavelengths wl, Pcg& rng, Stats& st, const VcmParams& p, const VcmVertex* cache,
+                   uint32_t M, double xyz[3]) const;
+
 private:
     struct Link;
     struct Emitted;
+    double vcmEta(const VcmParams& p, V3 x) const;
     bool linkObserver(V3 x, Pcg& rng, Link& k) const;
     void deposit(Splat& out, int px, int py, const Spec4& v, const Wavelengths& wl) const;
     bool emitParticle(Wavelengths& wl, Pcg& rng, bool connect, Emitted& e, Splat& out,

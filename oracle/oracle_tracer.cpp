@@ -522,6 +522,12 @@ struct Tracer::Emitted {
     Spec4 beta;
     bool distant = false;
     uint32_t guideCell = kNone;  // SPPM: the guide cell a sun photon of the uniform share teaches
+    // VCM (Emitted, particle.slang:192-202): the pdf of the emitted position and direction with the
+    // light's choice, next-event estimation's pdf of a distant direction, |cos| at an area emitter
+    // (1 for distant light), the area emitter's light index, point and normal in its own frame.
+    double emissionPdfW = 0, directPdfW = 0, usedCos = 1;
+    size_t light = size_t(-1);
+    V3 pL, nL;
 };
 
 void PhotonGuide::build() {
@@ -690,8 +696,10 @@ bool Tracer::emitParticle(Wavelengths& wl, Pcg& rng, bool connect, Emitted& e, S
         }
         e.ray = Ray{w_.sceneCenter() - dir * (1.01 * R) + disc.s * (qx * R) + disc.t * (qy * R), dir, kNone, kNone};
         e.region = 0;
-        e.beta = Le * (1 / (sel * pdfDir * (pdfUnit / (R * R))));
+        e.emissionPdfW = sel * pdfDir * (pdfUnit / (R * R));
+        e.beta = Le * (1 / e.emissionPdfW);
         e.distant = true;
+        e.directPdfW = fromSky ? 0.0 : w_.lightPdf[li] / w_.env.sunSolidAngle();
     } else {
         const double sel = (1 - pSky) * w_.lightPdf[li];
         const uint32_t bi = w_.lights[li].boundary;
@@ -732,6 +740,11 @@ bool Tracer::emitParticle(Wavelengths& wl, Pcg& rng, bool connect, Emitted& e, S
         e.ray = Ray{p, dir, bi, prim};
         e.region = front ? b.front : b.back;
         e.beta = Le * (kPi / (sel * pdfA * sidePdf));
+        e.usedCos = std::fabs(dot(n, dir));
+        e.emissionPdfW = sel * pdfA * sidePdf * e.usedCos / kPi;
+        e.light = li;
+        e.pL = pL;
+        e.nL = nL;
     }
     maskTerminated(wl, e.beta);
     return !e.beta.isZero();
@@ -911,5 +924,483 @@ bool Tracer::sppmValue(const VisiblePoint& v, int groups, V3 d, const Spec4& bet
     w.toXYZ(value, xyz);
     return std::isfinite(xyz[0] + xyz[1] + xyz[2]);
 }
+
+// ---------------------------------------------------------------- vertex connection and merging (vcm.slang)
+
+namespace {
+
+// Georgiev's recursive MIS quantities in the balance heuristic (vcm.slang:150-178).
+struct VcmMis {
+    double dVCM = 0, dVC = 0, dVM = 0;
+    void arrive(double dist, double cosIn) {
+        dVCM *= dist * dist;
+        dVCM /= cosIn;
+        dVC /= cosIn;
+        dVM /= cosIn;
+    }
+    void scatter(V3 n, const Sample& s, double pdfRev, double eta) {
+        const double cosOut = std::fabs(dot(n, s.wi));
+        if (s.delta) {
+            dVCM = 0;
+            dVC *= cosOut;
+            dVM *= cosOut;
+            return;
+        }
+        dVC = cosOut / s.pdf * (dVC * pdfRev + dVCM + eta);
+        dVM = cosOut / s.pdf * (dVM * pdfRev + dVCM * (eta > 0 ? 1 / eta : 0.0) + 1);
+        dVCM = 1 / s.pdf;
+    }
+};
+
+// f per wavelength at the interface (hero `it`, boundary b) for arrival d and departure wi, with the
+// hero's pdf.
+Spec4 vcmEval(const World& w, const Interface& it, const Boundary& b, const Wavelengths& wl, V3 d, V3 wi, bool radiance,
+              double& pdf) {
+    Spec4 f(0.0);
+    pdf = 0;
+    for (int i = 0; i < NW; ++i) {
+        if (wl.pdf[i] == 0) continue;
+        Interface a = it;
+        if (i > 0) {
+            a.nFront = w.indexOf(b.front, wl.lambda[i]);
+            a.nBack = w.indexOf(b.back, wl.lambda[i]);
+            a.lambda = wl.lambda[i];
+        }
+        double q;
+        f[i] = evalScatter(a, d, wi, radiance, q);
+        if (i == 0) pdf = q;
+    }
+    return f;
+}
+
+double reversePdf(const Interface& it, V3 arriving, V3 leaving) {
+    double pdf;
+    evalScatter(it, arriving, leaving, kRadiance, pdf);
+    return pdf;
+}
+
+// XYZ of a contribution at the camera path's wavelengths, with only the hero when the light side left
+// only its hero (joinedXYZ, vcm.slang:319-324).
+void joinedXYZ(Wavelengths wl, bool lightTerminated, const Spec4& v, double xyz[3]) {
+    if (lightTerminated) wl.terminateSecondary();
+    Spec4 m = v;
+    maskTerminated(wl, m);
+    wl.toXYZ(m, xyz);
+}
+
+uint32_t vcmPixelGroup(uint32_t pixel, uint32_t iteration, uint32_t groups);
+
+}  // namespace
+
+double Tracer::vcmEta(const VcmParams& p, V3 x) const {
+    if (!(p.mergePerMetre > 0)) return 0;
+    const double r = std::max(p.mergePerMetre * length(x - w_.observer.position), 1e-9);
+    return kPi * r * r * p.lightPaths;
+}
+
+void Tracer::vcmLight(Wavelengths wl, Pcg& rng, Stats& st, Splat& out, const VcmParams& p, uint32_t group,
+                      std::vector<VcmVertex>& cache) const {
+    st.paths++;
+    Emitted em;
+    if (!emitParticle(wl, rng, false, em, out, nullptr) || !(em.emissionPdfW > 0)) return;
+    Ray ray = em.ray;
+    uint32_t region = em.region;
+    Spec4 beta = em.beta;
+    VcmMis mis;
+    double segment = 0;
+    bool first = true;
+    const int W = w_.observer.width;
+    for (int depth = 0;; ++depth) {
+        Hit hit;
+        const bool found = w_.intersect(ray, kInf, hit, region);
+        st.segments++;
+        const Medium& med = w_.mediumOf(region);
+        if (med.absorbs()) {  // attenuation only: the backend refuses media that scatter
+            if (!found) return;
+            for (int i = 0; i < NW; ++i) beta[i] *= std::exp(-med.absorption.eval(wl.lambda[i]) * hit.t);
+        }
+        if (!found) {
+            if (region != 0) st.leaks++;
+            return;
+        }
+        const Boundary& b = w_.boundaries[hit.boundary];
+        const bool fromFront = dot(ray.d, hit.n) < 0;
+        const uint32_t inc = fromFront ? b.front : b.back, other = fromFront ? b.back : b.front;
+        if (inc != region) {
+            st.inconsistencies++;
+            region = inc;
+        }
+        segment += hit.t;
+        const Optics& o = w_.optics[size_t(b.optics)];
+        if (o.type == OWE_SURF_NULL) {
+            region = other;
+            ray = w_.spawn(hit, ray.d);
+            if (depth >= 4 * s_.maxDepth) return;
+            continue;
+        }
+        if (o.type == OWE_SURF_DETECTOR || o.type == OWE_SURF_ABSORBER) return;
+        const V3 d = ray.d;
+        if (first) {
+            // firstLightVertexMis (vcm.slang:185-197).
+            mis.dVC = em.usedCos / em.emissionPdfW;
+            if (em.distant) {
+                mis.dVCM = em.directPdfW / em.emissionPdfW;
+            } else {
+                const Boundary& lb = w_.boundaries[w_.lights[em.light].boundary];
+                const double directPdfW = w_.lightPdf[em.light] * w_.pdfFrom(lb, lb.toLocal.point(hit.p), em.pL, em.nL);
+                mis.dVCM = directPdfW * em.usedCos / em.emissionPdfW;
+            }
+            const double eta = vcmEta(p, hit.p);
+            mis.dVM = mis.dVC * (eta > 0 ? 1 / eta : 0.0);
+        }
+        mis.arrive(first ? 1 : segment, std::fabs(dot(hit.n, d)));
+        first = false;
+        segment = 0;
+        const Interface it{&o, hit.n, hit.p, w_.indexOf(b.front, wl.lambda[0]), w_.indexOf(b.back, wl.lambda[0]),
+                           wl.lambda[0], s_.fresnelFloor};
+        if (!o.isDelta()) {
+            VcmVertex y;
+            y.p = hit.p;
+            y.n = hit.n;
+            y.d = d;
+            y.beta = beta;
+            y.dVCM = mis.dVCM;
+            y.dVC = mis.dVC;
+            y.dVM = mis.dVM;
+            y.boundary = hit.boundary;
+            y.terminated = wl.secondaryTerminated();
+            cache.push_back(y);
+            // lightVertexToEye (vcm.slang:229-244): only to the pixels of this group, scaled by G.
+            Link k;
+            if (linkObserver(hit.p, rng, k) &&
+                vcmPixelGroup(uint32_t(k.py) * uint32_t(W) + uint32_t(k.px), p.iteration, p.groups) == group) {
+                double pdfFwd;
+                const Spec4 f = vcmEval(w_, it, b, wl, d, k.dir, !kRadiance, pdfFwd);
+                if (!f.isZero()) {
+                    const double pdfRev = reversePdf(it, -k.dir, -d);
+                    const double cosX = std::fabs(dot(hit.n, k.dir));
+                    const double wLight = k.factor * cosX / p.lightPaths * (vcmEta(p, hit.p) + mis.dVCM + mis.dVC * pdfRev);
+                    const double wgt = 1 / (wLight + 1);
+                    uint32_t arrival = kNone;
+                    const Ray from{hit.p, k.dir, hit.boundary, hit.prim};
+                    const Spec4 Tr = transmittance(hit.p, from, dot(k.dir, hit.n) > 0 ? b.front : b.back, k.dir,
+                                                   length(k.q - hit.p), wl, &arrival, kNone);
+                    if (arrival == w_.observer.region && !Tr.isZero())
+                        deposit(out, k.px, k.py, beta * f * Tr * (cosX * k.factor * wgt * double(p.groups)), wl);
+                }
+            }
+        }
+        if (depth >= s_.maxDepth) return;
+        auto at = [&](int i) {
+            Interface r = it;
+            r.nFront = w_.indexOf(b.front, wl.lambda[i]);
+            r.nBack = w_.indexOf(b.back, wl.lambda[i]);
+            r.lambda = wl.lambda[i];
+            return r;
+        };
+        if (o.type == OWE_SURF_DIELECTRIC && !wl.secondaryTerminated())
+            for (int i = 1; i < NW; ++i)
+                if (wl.pdf[i] != 0 && isDispersive(it, at(i))) {
+                    wl.terminateSecondary();
+                    maskTerminated(wl, beta);
+                    break;
+                }
+        double u1, u2, u3;
+        rng.draw3(u1, u2, u3);
+        Sample s;
+        if (!sampleScatter(it, d, u1, u2, u3, !kRadiance, s) || s.weight <= 0) return;
+        beta[0] *= s.weight;
+        for (int i = 1; i < NW; ++i)
+            if (wl.pdf[i] != 0) beta[i] *= secondaryWeight(at(i), d, s, !kRadiance);
+        const double pdfRev = s.delta ? 0.0 : reversePdf(it, -s.wi, -d);
+        mis.scatter(hit.n, s, pdfRev, vcmEta(p, hit.p));
+        if (s.transmitted) region = other;
+        ray = w_.spawn(hit, s.wi);
+        if (depth >= s_.rrDepth && s.weight < 1) {
+            if (rng.uniform() >= s.weight) return;
+            beta *= 1 / s.weight;
+        }
+    }
+}
+
+bool Tracer::vcmCamera(int x, int y, Wavelengths wl, Pcg& rng, Stats& st, const VcmParams& p, const VcmVertex* cache,
+                       uint32_t M, double xyz[3]) const {
+    xyz[0] = xyz[1] = xyz[2] = 0;
+    const Observer& ob = w_.observer;
+    const double px = x + rng.uniform(), py = y + rng.uniform();
+    double fx = 0, fy = 0;
+    if (ob.sigma > 0) {
+        double u1, u2;
+        rng.draw2(u1, u2);
+        standardNormal2(u1, u2, fx, fy);
+        fx *= ob.sigma;
+        fy *= ob.sigma;
+    }
+    Ray ray;
+    double weight;
+    if (!ob.generate(px + fx, py + fy, rng, ray, weight) || weight <= 0) return false;
+    const double cosCam = dot(ray.d, ob.fwd);
+    if (!(cosCam > 0)) return false;
+    st.paths++;
+    const double pixelArea = (2 * ob.tanX / ob.width) * (2 * ob.tanY / ob.height);
+    const double cameraPdfW = 1 / (pixelArea * cosCam * cosCam * cosCam);
+    VcmMis mis;
+    mis.dVCM = p.lightPaths / cameraPdfW;
+    Spec4 beta(weight);
+    maskTerminated(wl, beta);
+    uint32_t region = ob.region;
+    double segment = 0;
+    V3 prevP = ray.o;
+    int pathLength = 0;
+    double L[3] = {0, 0, 0};
+    auto add = [&](bool lightTerminated, const Spec4& v) {
+        double c[3];
+        joinedXYZ(wl, lightTerminated, v, c);
+        for (int k = 0; k < 3; ++k) L[k] += c[k];
+    };
+    const double R = w_.sceneRadius() * 1.001;
+    double sunLightPdf = 0;
+    for (size_t li = 0; li < w_.lights.size(); ++li)
+        if (w_.lights[li].kind == 1) sunLightPdf = w_.lightPdf[li];
+    for (int depth = 0;; ++depth) {
+        Hit hit;
+        const bool found = w_.intersect(ray, kInf, hit, region);
+        st.segments++;
+        const Medium& med = w_.mediumOf(region);
+        if (med.absorbs()) {
+            if (!found) break;
+            for (int i = 0; i < NW; ++i) beta[i] *= std::exp(-med.absorption.eval(wl.lambda[i]) * hit.t);
+        }
+        if (!found) {
+            if (region == 0) {
+                // cameraSkyMet (vcm.slang:360-376).
+                double wSky = 1, wSun = 1;
+                const bool inSun = w_.env.hasSun && dot(ray.d, w_.env.sunDir) >= std::cos(w_.env.sunAngularRadius);
+                if (pathLength > 0) {
+                    wSky = 1 / (1 + w_.skyShare / (4 * kPi) / (kPi * R * R) * mis.dVC);
+                    if (inSun && w_.env.sunSolidAngle() > 0)
+                        wSun = 1 / (1 + sunLightPdf / w_.env.sunSolidAngle() * mis.dVCM +
+                                    (1 - w_.skyShare) * sunLightPdf / w_.env.sunSolidAngle() / (kPi * R * R) * mis.dVC);
+                }
+                Spec4 v;
+                for (int i = 0; i < NW; ++i) {
+                    if (wl.pdf[i] == 0) continue;
+                    const double sky = w_.env.skyAt(wl.lambda[i]);
+                    const double sun = inSun ? w_.env.sunRadiance.eval(wl.lambda[i]) : 0.0;
+                    v[i] = beta[i] * (sky * wSky + sun * wSun);
+                }
+                add(false, v);
+            } else {
+                st.leaks++;
+            }
+            break;
+        }
+        const Boundary& b = w_.boundaries[hit.boundary];
+        const bool fromFront = dot(ray.d, hit.n) < 0;
+        const uint32_t inc = fromFront ? b.front : b.back, other = fromFront ? b.back : b.front;
+        if (inc != region) {
+            st.inconsistencies++;
+            region = inc;
+        }
+        segment += hit.t;
+        const Optics& o = w_.optics[size_t(b.optics)];
+        if (o.type == OWE_SURF_NULL) {
+            region = other;
+            ray = w_.spawn(hit, ray.d);
+            if (depth >= 4 * s_.maxDepth) break;
+            continue;
+        }
+        const V3 d = ray.d;
+        const double cosIn = std::fabs(dot(hit.n, d));
+        const double dist = segment;
+        mis.arrive(dist, cosIn);
+        segment = 0;
+        ++pathLength;
+        // cameraEmissionMet (vcm.slang:380-399).
+        if (b.emission >= 0) {
+            const Emission& e = w_.emissions[size_t(b.emission)];
+            if (fromFront ? e.front : e.back) {
+                double wgt = 1;
+                const int li = w_.lightOfBoundary[hit.boundary];
+                if (pathLength > 1 && li >= 0) {
+                    const double directPdfA =
+                        w_.lightPdf[size_t(li)] * w_.pdfFrom(b, b.toLocal.point(prevP), hit.pLocal, hit.nLocal) * cosIn / (dist * dist);
+                    const double side = e.front && e.back ? 0.5 : 1.0;
+                    const double emissionPdfW = (1 - w_.skyShare) * w_.lightPdf[size_t(li)] / b.area() * side * cosIn / kPi;
+                    wgt = 1 / (1 + directPdfA * mis.dVCM + emissionPdfW * mis.dVC);
+                }
+                add(false, beta * wl.eval(e.radiance) * wgt);
+            }
+        }
+        if (o.type == OWE_SURF_ABSORBER || o.type == OWE_SURF_DETECTOR) break;
+        if (depth >= s_.maxDepth) break;
+        const Interface it{&o, hit.n, hit.p, w_.indexOf(b.front, wl.lambda[0]), w_.indexOf(b.back, wl.lambda[0]),
+                           wl.lambda[0], s_.fresnelFloor};
+        auto at = [&](int i) {
+            Interface r = it;
+            r.nFront = w_.indexOf(b.front, wl.lambda[i]);
+            r.nBack = w_.indexOf(b.back, wl.lambda[i]);
+            r.lambda = wl.lambda[i];
+            return r;
+        };
+        if (o.type == OWE_SURF_DIELECTRIC && !wl.secondaryTerminated())
+            for (int i = 1; i < NW; ++i)
+                if (wl.pdf[i] != 0 && isDispersive(it, at(i))) {
+                    wl.terminateSecondary();
+                    maskTerminated(wl, beta);
+                    break;
+                }
+        const double eta = vcmEta(p, hit.p);
+        if (!o.isDelta()) {
+            // Next-event estimation (cameraNextEvent, vcm.slang:402-425).
+            if (!w_.lights.empty()) {
+                const size_t li = w_.pickLight(rng.uniform());
+                const double u1 = rng.uniform(), u2 = rng.uniform(), u3 = rng.uniform();
+                const Light& Lt = w_.lights[li];
+                V3 wi;
+                double ldist = kInf, lpdf = 0, cosAtLight = 1, emissionPdf = 0;
+                Spec4 Le;
+                uint32_t arrival = kNone, exclude = kNone;
+                bool ok = true;
+                if (Lt.kind == 1) {
+                    wi = Frame(w_.env.sunDir).toWorld(uniformCone(u1, u2, std::cos(w_.env.sunAngularRadius)));
+                    Le = wl.eval(w_.env.sunRadiance);
+                    lpdf = w_.lightPdf[li] / w_.env.sunSolidAngle();
+                    emissionPdf = (1 - w_.skyShare) * w_.lightPdf[li] / w_.env.sunSolidAngle() / (kPi * R * R);
+                } else {
+                    const Boundary& lb = w_.boundaries[Lt.boundary];
+                    V3 pL, nL;
+                    double pdfW;
+                    ok = w_.sampleFrom(lb, lb.toLocal.point(hit.p), u1, u2, u3, pL, nL, pdfW);
+                    if (ok) {
+                        const V3 ep = lb.toWorld.point(pL), en = normalize(lb.toWorld.vector(nL));
+                        const V3 dv = ep - hit.p;
+                        ldist = length(dv);
+                        ok = ldist > 0;
+                        if (ok) {
+                            wi = dv / ldist;
+                            const Emission& e = w_.emissions[size_t(lb.emission)];
+                            const double cosL = dot(en, -wi);
+                            Le = ((cosL > 0 && e.front) || (cosL < 0 && e.back)) ? wl.eval(e.radiance) : Spec4(0.0);
+                            lpdf = w_.lightPdf[li] * pdfW;
+                            cosAtLight = std::fabs(cosL);
+                            const double side = e.front && e.back ? 0.5 : 1.0;
+                            emissionPdf = (1 - w_.skyShare) * w_.lightPdf[li] / lb.area() * side * std::fabs(cosL) / kPi;
+                            arrival = cosL > 0 ? lb.front : lb.back;
+                            exclude = Lt.boundary;
+                        }
+                    }
+                }
+                maskTerminated(wl, Le);
+                if (ok && lpdf > 0 && cosAtLight > 0 && !Le.isZero()) {
+                    double pdfFwd;
+                    const Spec4 f = vcmEval(w_, it, b, wl, d, wi, kRadiance, pdfFwd);
+                    if (!f.isZero()) {
+                        const double pdfRev = reversePdf(it, -wi, -d);
+                        const double cosToLight = std::fabs(dot(hit.n, wi));
+                        const double wLight = pdfFwd / lpdf;
+                        const double wCamera = emissionPdf * cosToLight / (lpdf * cosAtLight) * (eta + mis.dVCM + mis.dVC * pdfRev);
+                        const double wgt = 1 / (wLight + 1 + wCamera);
+                        const Ray from{hit.p, wi, hit.boundary, hit.prim};
+                        const uint32_t start = dot(wi, hit.n) > 0 ? b.front : b.back;
+                        uint32_t reached = kNone;
+                        Spec4 Tr = transmittance(hit.p, from, start, wi, ldist, wl, std::isfinite(ldist) ? &reached : nullptr,
+                                                 exclude);
+                        if (std::isfinite(ldist) && reached != arrival) Tr = Spec4(0.0);
+                        if (!Tr.isZero()) add(false, beta * f * Le * Tr * (cosToLight * wgt / lpdf));
+                    }
+                }
+            }
+            // A connection to one cached vertex drawn uniformly (cameraConnect, vcm.slang:428-462).
+            if (M > 0) {
+                // The pick as the float path forms it: the uniform's leading 24 bits times M, rounded in
+                // single precision, so that both draw the same vertex.
+                const float u24 = float(std::floor(rng.uniform() * 16777216.0) / 16777216.0);
+                const uint32_t pick = std::min(uint32_t(u24 * float(M)), M - 1);
+                const VcmVertex& yv = cache[pick];
+                const V3 dv = yv.p - hit.p;
+                const double dist2 = dot(dv, dv);
+                if (dist2 > 0) {
+                    const double dl = std::sqrt(dist2);
+                    const V3 dir = dv / dl;
+                    Wavelengths wj = wl;
+                    if (yv.terminated) wj.terminateSecondary();
+                    double pdfCamFwd, pdfLightFwd;
+                    const Spec4 fc = vcmEval(w_, it, b, wj, d, dir, kRadiance, pdfCamFwd);
+                    const Boundary& lb = w_.boundaries[yv.boundary];
+                    const Optics& lo = w_.optics[size_t(lb.optics)];
+                    const Interface hl{&lo, yv.n, yv.p, w_.indexOf(lb.front, wj.lambda[0]), w_.indexOf(lb.back, wj.lambda[0]),
+                                       wj.lambda[0], s_.fresnelFloor};
+                    const Spec4 fl = vcmEval(w_, hl, lb, wj, yv.d, -dir, !kRadiance, pdfLightFwd);
+                    if (!(fc * fl).isZero()) {
+                        const double pdfCamRev = reversePdf(it, -dir, -d), pdfLightRev = reversePdf(hl, dir, -yv.d);
+                        const double cosX = std::fabs(dot(hit.n, dir)), cosY = std::fabs(dot(yv.n, dir));
+                        const double camPdfA = pdfCamFwd * cosY / dist2, lightPdfA = pdfLightFwd * cosX / dist2;
+                        const double wLight = camPdfA * (vcmEta(p, yv.p) + yv.dVCM + yv.dVC * pdfLightRev);
+                        const double wCamera = lightPdfA * (eta + mis.dVCM + mis.dVC * pdfCamRev);
+                        const double wgt = 1 / (wLight + 1 + wCamera);
+                        const Ray from{hit.p, dir, hit.boundary, hit.prim};
+                        uint32_t reached = kNone;
+                        Spec4 Tr = transmittance(hit.p, from, dot(dir, hit.n) > 0 ? b.front : b.back, dir, dl * (1 - 1e-5), wj,
+                                                 &reached, kNone);
+                        if (reached != (dot(-dir, yv.n) > 0 ? lb.front : lb.back)) Tr = Spec4(0.0);
+                        if (!Tr.isZero())
+                            add(yv.terminated, beta * fc * yv.beta * fl * Tr * (cosX * cosY / dist2 * wgt * double(M) / p.lightPaths));
+                    }
+                }
+            }
+            // Merging with the group's vertices within the merging radius (cameraMerge, vcm.slang:465-520).
+            if (p.mergePerMetre > 0) {
+                const double r = std::max(p.mergePerMetre * length(hit.p - ob.position), 1e-9), r2 = r * r;
+                const double invEta = 1 / eta;
+                for (uint32_t m = 0; m < M; ++m) {
+                    const VcmVertex& yv = cache[m];
+                    const V3 dp = yv.p - hit.p;
+                    if (dot(dp, dp) > r2) continue;
+                    if (std::fabs(dot(dp, hit.n)) > 0.25 * r || dot(yv.n, hit.n) < 0.5) continue;
+                    const V3 wi = -yv.d;
+                    Wavelengths wj = wl;
+                    if (yv.terminated) wj.terminateSecondary();
+                    double pdfFwd;
+                    const Spec4 fc = vcmEval(w_, it, b, wj, d, wi, kRadiance, pdfFwd);
+                    if (fc.isZero()) continue;
+                    const double pdfRev = reversePdf(it, -wi, -d);
+                    const double wLight = yv.dVCM * invEta + yv.dVM * pdfFwd;
+                    const double wCamera = mis.dVCM * invEta + mis.dVM * pdfRev;
+                    const double wgt = 1 / (wLight + 1 + wCamera);
+                    add(yv.terminated, beta * fc * yv.beta * (wgt * invEta));
+                }
+            }
+        }
+        double u1, u2, u3;
+        rng.draw3(u1, u2, u3);
+        Sample s;
+        if (!sampleScatter(it, d, u1, u2, u3, kRadiance, s) || s.weight <= 0) break;
+        beta[0] *= s.weight;
+        for (int i = 1; i < NW; ++i)
+            if (wl.pdf[i] != 0) beta[i] *= secondaryWeight(at(i), d, s, kRadiance);
+        const double pdfRev = s.delta ? 0.0 : reversePdf(it, -s.wi, -d);
+        mis.scatter(hit.n, s, pdfRev, eta);
+        if (s.transmitted) region = other;
+        prevP = hit.p;
+        ray = w_.spawn(hit, s.wi);
+        if (depth >= s_.rrDepth) {
+            double q = 0;
+            for (int i = 0; i < NW; ++i) q = std::max(q, beta[i]);
+            q = std::min(1.0, q);
+            if (rng.uniform() >= q) break;
+            beta *= 1 / q;
+        }
+    }
+    if (std::isfinite(L[0] + L[1] + L[2]))
+        for (int k = 0; k < 3; ++k) xyz[k] = L[k];
+    return true;
+}
+
+namespace {
+uint32_t vcmPixelGroup(uint32_t pixel, uint32_t iteration, uint32_t groups) {
+    return uint32_t(mix64((uint64_t(iteration) << 32 | pixel) ^ 0x1f83d9abfb41bd6bULL) % uint64_t(groups));
+}
+}  // namespace
 
 }  // namespace oracle

@@ -165,6 +165,7 @@ class Oracle(_Lib):
             "render_particles": (None, [_vp, _vp, _i, _i, _i, _vp, _vp]),
             "sppm": (None, [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp]),
             "sppm_radius": (None, [_vp, _vp, _vp]),
+            "vcm": (None, [_vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
             "particles": (None, [_vp, _vp, _i, _u64, _vp, _vp, _vp, _i]),
             "emission_info": (None, [_vp, _vp]),
             "world_intersect": (None, [_vp, _u64, _vp, _vp, _vp, _vp]),
@@ -287,6 +288,24 @@ class World:
         ppp = settings.photons_per_pixel if settings.photons_per_pixel > 0 else 1.0
         return dict(direct=direct, pix=pix, iterations=state["iterations"] + iterations, stats=stats, guide=guide,
                     photons=max(1, int(round(W * H * ppp))), tau=pix[..., :3], N=pix[..., 3], r=pix[..., 4])
+
+    def vcm(self, settings, iterations: int, merging: bool = True, start: int = 0, threads: int = 0):
+        """Vertex connection and merging (bdpt when merging is off), iterations [start, start + n):
+        the camera paths' XYZ and the light subpaths' splats summed over the iterations, each [H, W, 3],
+        the light subpaths per iteration N, the stats and each iteration's cache size. The image is
+        camera / iterations + light / (N · iterations)."""
+        W, H = self.width, self.height
+        camera, light = np.zeros((H, W, 3)), np.zeros((H, W, 3))
+        stats = np.zeros(4, dtype=np.uint64)
+        cached = np.zeros(max(iterations, 1), dtype=np.uint64)
+        self.o.vcm(self.h, ctypes.byref(settings), int(bool(merging)), iterations, start, threads, _ptr(camera),
+                   _ptr(light), _ptr(stats), _ptr(cached))
+        G = settings.photon_groups if settings.photon_groups > 0 else 1
+        ppp = settings.photons_per_pixel if settings.photons_per_pixel > 0 else 1.0
+        want = max(1, int(round(W * H * ppp)))
+        N = (want + G - 1) // G * G
+        return dict(camera=camera, light=light, paths=N, stats=stats, cached=cached[:iterations],
+                    image=camera / max(iterations, 1) + light / (N * max(iterations, 1)))
 
     def sppm_radius(self, settings):
         """(initial radius per metre of path, radius floor)."""

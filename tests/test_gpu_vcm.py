@@ -88,3 +88,43 @@ def test_unsupported_cases_are_refused(cuda):
         render.Renderer(fog, S.settings(integrator=S.INTEGRATOR_VCM))
     with pytest.raises(_lib.OweHipError):
         render.Renderer(S.submerged_sensor_scene(8), S.settings(integrator=S.INTEGRATOR_BDPT))
+
+
+# The caustic through the plate (every strategy: light tracing, next-event estimation, connections
+# through glass refused, merging), the reference's benchmark scene of a lens caustic seen through a
+# tilted plate, and the pool (absorbing rippled water, contacts); bdpt on the plate; three
+# iterations of the plate (the shrinking radius, the rotation of the wavelengths).
+@pytest.mark.parametrize("case,integrator,iterations", [("plate", "vcm", 1), ("plate", "bdpt", 1),
+                                                        ("caustic_seen_through_glass", "vcm", 1), ("pool", "vcm", 1),
+                                                        ("plate", "vcm", 3)])
+def test_matches_the_oracle_pixel_by_pixel(cuda, oracle, case, integrator, iterations):
+    """The cache is filled in the same order on both (light subpath by light subpath), so the camera
+    path of a pixel draws the same light vertex to connect to and merges with the same vertices:
+    after a few iterations the images agree pixel by pixel but for float decisions (a roulette or
+    Fresnel draw at the threshold, a vertex at the merging radius)."""
+    from pathlib import Path
+
+    from owe_hip import loader as LD
+    from pyoracle import World
+
+    if case in ("plate",):
+        ls = C.caustic_cases(resolution=32)[case]
+        sc = ls.use()
+    else:
+        ls = LD.load_scene(str(Path(LD.__file__).parent / "scenes" / f"{case}.owe"))
+        sc = ls.use(resolution=(32, 24))
+    st = ls.settings(seed=5, integrator=integrator)
+    print(f"{integrator} {case}: {sc.observer['width']}x{sc.observer['height']}, {iterations} iterations", flush=True)
+    r = _render(sc, st, 1, iterations)
+    g = r.resolve()[..., 1]
+    o = World(oracle, sc).vcm(st, iterations, merging=integrator == "vcm")
+    y = o["image"][..., 1]
+    close = np.abs(g - y) <= 1e-3 * np.maximum(np.abs(g), np.abs(y)) + 1e-7 * max(float(np.abs(y).max()), 1e-30)
+    lit = y > 0
+    print(f"  agrees on {close.mean():.4f} of pixels ({lit.mean():.3f} lit); mean gpu {g.mean():.6g} oracle "
+          f"{y.mean():.6g}; cache {list(o['cached'])}", flush=True)
+    assert lit.mean() > 0.2
+    assert close.mean() > 0.95
+    s = r.stats()
+    assert s["paths"] == o["stats"][0]
+    assert s["leaks"] == o["stats"][3] == 0
