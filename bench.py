@@ -941,6 +941,67 @@ def bench_sppm(args):
     return out
 
 
+VCM_SIZE, VCM_ITERATIONS = 256, 16
+
+
+def cpu_baseline_vcm(ls, sc, st, seconds_budget: float = 10.0):
+    sys.path.insert(0, str(ROOT / "oracle"))
+    from pyoracle import Oracle, World
+
+    threads = min(16, os.cpu_count() or 1)
+    w = World(Oracle(), sc)
+    n, its, elapsed = 0, 0, 0.0
+    while elapsed < seconds_budget and its < 256:
+        t0 = time.perf_counter()
+        r = w.vcm(st, 2, merging=True, start=its, threads=threads)
+        elapsed += time.perf_counter() - t0
+        n += int(r["stats"][0])
+        its += 2
+    return {"value": n / elapsed, "unit": "camera paths + light subpaths/s", "cores": threads, "kind": "port",
+            "sample": f"{its} vcm iterations of the same scene ({n} camera paths and light subpaths, {elapsed:.1f} s) "
+                      f"through the oracle's fp64 restatement (merging by a sweep over the group's vertices sorted by x), "
+                      f"{threads} threads"}
+
+
+def bench_vcm(args):
+    """Vertex connection and merging of a lens caustic seen through a glass plate: per iteration one
+    light subpath per pixel (cached, splatted to the eye), the cache's scan and compaction, the merging
+    hash, one camera path per pixel with next-event estimation, a connection and merging; a step is a
+    pass of 16 iterations at 256x256."""
+    import torch
+
+    from owe_hip import caustic_scenes as C
+    from owe_hip import render
+
+    ls = C.caustic_cases(resolution=VCM_SIZE)["plate"]
+    sc = ls.use()
+    st = ls.settings(seed=1, integrator="vcm")
+    r = render.Renderer(sc, st)
+    for _ in range(args.warmup):
+        r.run_pass(VCM_ITERATIONS)
+    r.reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r.run_pass(VCM_ITERATIONS)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    stats = r.stats()
+    n = stats["paths"]
+    its = args.steps * VCM_ITERATIONS
+    log(f"vcm: {its} iterations in {wall:.3f}s; {n / wall / 1e6:.1f} M camera paths + light subpaths/s; "
+        f"{wall * 1e3 / its:.3f} ms/iteration")
+    out = {"value": n / wall, "unit": "camera paths + light subpaths/s", "ms_per_step": wall * 1e3 / args.steps,
+           "config": {"workload": "vcm, lens caustic seen through a glass plate", "resolution": [VCM_SIZE, VCM_SIZE],
+                      "iterations_per_step": VCM_ITERATIONS, "light_subpaths_per_iteration": VCM_SIZE * VCM_SIZE,
+                      "ms_per_iteration": wall * 1e3 / its, "segments": stats["segments"],
+                      "mean_Y": float(r.resolve()[..., 1].mean())},
+           "roofline": None}
+    out["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline_vcm(ls, sc, st)
+    r.close()
+    return out
+
+
 def main() -> int:
     import torch  # noqa: F401  (first: its HIP runtime is the process's; see owe_hip/_lib.py)
 
@@ -949,7 +1010,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="prism", choices=["prism", "rayquery", "light", "lens", "caustic",
-                                                             "telescope", "scenes", "sppm"])
+                                                             "telescope", "scenes", "sppm", "vcm"])
     ap.add_argument("--no-rayquery", action="store_true", help="prism: skip the ray-query record beside it")
     ap.add_argument("--no-light", action="store_true", help="prism: skip the light-tracing record beside it")
     ap.add_argument("--no-lens", action="store_true", help="prism: skip the lens-analysis record beside it")
@@ -957,6 +1018,7 @@ def main() -> int:
     ap.add_argument("--no-telescope", action="store_true", help="prism: skip the telescope-gradient record beside it")
     ap.add_argument("--no-scenes", action="store_true", help="prism: skip the scene-file suite record")
     ap.add_argument("--no-sppm", action="store_true", help="prism: skip the SPPM record")
+    ap.add_argument("--no-vcm", action="store_true", help="prism: skip the vertex connection and merging record")
     ap.add_argument("--no-telescope-render", action="store_true",
                     help="telescope: skip the rendering of the telescopes in the room")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -970,7 +1032,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.solo:
         for k in ("no_rayquery", "no_light", "no_lens", "no_caustic", "no_telescope", "no_scenes", "no_sppm",
-                  "no_quality", "no_lifted", "no_telescope_render"):
+                  "no_vcm", "no_quality", "no_lifted", "no_telescope_render"):
             setattr(args, k, True)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); every workload here runs on a single "
@@ -1012,8 +1074,13 @@ def main() -> int:
         if not args.no_sppm:
             result["sppm"] = bench_sppm(argparse.Namespace(steps=min(args.steps, 8), warmup=1,
                                                            no_cpu_baseline=args.no_cpu_baseline))
+        if not args.no_vcm:
+            result["vcm"] = bench_vcm(argparse.Namespace(steps=min(args.steps, 8), warmup=1,
+                                                         no_cpu_baseline=args.no_cpu_baseline))
     elif args.workload == "sppm":
         result = bench_sppm(args)
+    elif args.workload == "vcm":
+        result = bench_vcm(args)
     elif args.workload == "scenes":
         result = bench_scenes(args)
     elif args.workload == "caustic":

@@ -171,6 +171,15 @@ int owe_hip_renderer_set_mode(owe_hip_renderer* renderer, int mode, int blocks_p
                               int waves_per_simd, int refill);
 int owe_hip_renderer_get_mode(const owe_hip_renderer* renderer, int* mode, int* blocks_per_cu, int* chunk,
                               int* use_graph, int* waves_per_simd, int* refill);
+/* Vertex connection and merging (vcm, bdpt): how many iterations of a pass run together, each in
+ * its own slot of the light vertex cache, offsets and merging hash (1 to the number the renderer's
+ * caches hold; 0: that number, chosen at creation so that a launch carries enough light subpaths to
+ * fill the device, within a 16 GiB cache budget). The image does not depend on it: every iteration
+ * fills its cache in subpath order and sums its merges in integers, so any number in flight gives
+ * the same image bit for bit. No counterpart in the reference (whose GPU backend runs one iteration
+ * after another, gpu_renderer_passes.cpp:136-199). */
+int owe_hip_renderer_set_vcm_in_flight(owe_hip_renderer* renderer, int iterations);
+int owe_hip_renderer_get_vcm_in_flight(const owe_hip_renderer* renderer, int* iterations, int* most);
 /* Times passes of spp samples per pixel under candidate launch shapes of the current form
  * (persistent: waves per SIMD × refill threshold × blocks per compute unit × chunk; wavefront: bounce
  * rounds between host reads of the live counters × paths in flight, a quarter to all of the path

@@ -716,7 +716,7 @@ __global__ __launch_bounds__(kPBlock) void particleRefillKernel(DScene sc, Setti
 
 // Into the light subpath's staging slots (cacheLightVertex, vcm.slang:199-227): vertex n of path p.
 __device__ __forceinline__ void vcmCache(const VcmDev& v, const Hit& h, f3 d, const float beta[NW], const VcmMis& mis,
-                                         bool terminated, uint32_t group, uint32_t p, uint32_t& n) {
+                                         bool terminated, uint32_t group, uint32_t slot, uint32_t p, uint32_t& n) {
     if (n >= v.pathSlots) return;  // cannot happen: one cached vertex per scattering event up to max_depth
     LightVertex& y = v.staging[size_t(p) * v.pathSlots + n++];
     const uint32_t face = h.slot != kEmptyChild ? h.face : ~0u;
@@ -724,7 +724,7 @@ __device__ __forceinline__ void vcmCache(const VcmDev& v, const Hit& h, f3 d, co
     y.n = make_float4(h.n.x, h.n.y, h.n.z, mis.dVCM);
     y.d = make_float4(d.x, d.y, d.z, __uint_as_float((terminated ? kLvTerminated : 0u) | ((face + 1u) << 1)));
     y.beta = make_float4(beta[0], beta[1], beta[2], beta[3]);
-    y.mis = make_float4(mis.dVC, mis.dVM, __uint_as_float(group), 0.0f);
+    y.mis = make_float4(mis.dVC, mis.dVM, __uint_as_float(group), __uint_as_float(slot));
 }
 
 // A light vertex to the eye (light tracing, t = 1; lightVertexToEye, vcm.slang:229-244): only to the
@@ -761,7 +761,8 @@ __device__ __forceinline__ void vcmToEye(PCtx& c, const VcmDev& v, const Hit& h,
 // One light subpath (traceLightVcm, vcm.slang:246-292): its non-specular vertices cached and
 // connected to the eye.
 template <bool kContacts>
-__device__ __forceinline__ void vcmLightPath(PCtx& c, const VcmDev& v, uint32_t group, uint32_t p, uint32_t& n) {
+__device__ __forceinline__ void vcmLightPath(PCtx& c, const VcmDev& v, uint32_t group, uint32_t slot, uint32_t p,
+                                             uint32_t& n) {
     const DScene& sc = c.sc;
     const int maxDepth = c.set.maxDepth, rrDepth = c.set.rrDepth;
     PRay ray;
@@ -832,7 +833,7 @@ __device__ __forceinline__ void vcmLightPath(PCtx& c, const VcmDev& v, uint32_t 
         const float hero = c.wl.lambda[0];
         const Iface it{&o, h.n, regionIndex(sc, b.front, hero), regionIndex(sc, b.back, hero), hero, c.set.fresnelFloor};
         if (!o.delta) {
-            vcmCache(v, h, d, c.beta, mis, secondaryTerminated(c.wl), group, p, n);
+            vcmCache(v, h, d, c.beta, mis, secondaryTerminated(c.wl), group, slot, p, n);
             vcmToEye<kContacts>(c, v, h, it, d, mis, group);
         }
         if (depth >= maxDepth) return;
@@ -872,23 +873,25 @@ __device__ __forceinline__ void vcmLightPath(PCtx& c, const VcmDev& v, uint32_t 
     }
 }
 
-// One thread per light subpath j of the iteration: its own stream (seed, j, iteration) and the
-// wavelengths of its group, j mod G (vcm_light_kernel.slang).
+// One thread per light subpath j of slot k's iteration (thread k·N + j): its own stream (seed, j,
+// iteration) and the wavelengths of its group, j mod G (vcm_light_kernel.slang).
 template <bool kContacts>
-__global__ __launch_bounds__(kPBlock) void vcmLightKernel(DScene sc, Settings set, VcmDev v, ParticleBatch pb) {
+__global__ __launch_bounds__(kPBlock, OWE_HIP_VCM_WAVES) void vcmLightKernel(DScene sc, Settings set, VcmDev v, ParticleBatch pb) {
     __shared__ uint32_t stack[kRefillStack * kPBlock];
-    const uint32_t j = blockIdx.x * kPBlock + threadIdx.x;
+    const uint32_t t = blockIdx.x * kPBlock + threadIdx.x;
     PCtx c{sc, set, pb, stack + threadIdx.x};
     c.sum[0] = c.sum[1] = c.sum[2] = 0.0f;
     c.segments = c.inconsistencies = c.leaks = 0;
-    const bool live = j < pb.count;
+    const bool live = t < pb.count;
     if (live) {
-        c.rng.seed(hashPair(set.seed ^ kVcmLight, j), v.iteration);
+        const uint32_t k = t / v.slotPaths, j = t - k * v.slotPaths;
+        const VcmDev s = v.at(k);
+        c.rng.seed(hashPair(set.seed ^ kVcmLight, j), s.iteration);
         const uint32_t group = j % v.groups;
-        const uint32_t p = group * v.pathsPerGroup + j / v.groups;
-        c.wl = sppmWavelengths(set, v.rotation, group, v.groups);
+        const uint32_t p = k * v.slotPaths + group * v.pathsPerGroup + j / v.groups;
+        c.wl = sppmWavelengths(set, s.rotation, group, v.groups);
         uint32_t n = 0;
-        vcmLightPath<kContacts>(c, v, group, p, n);
+        vcmLightPath<kContacts>(c, s, group, k, p, n);
         v.pathCount[p] = n;
     }
     const uint32_t n[4] = {waveSum(live ? 1u : 0u), waveSum(c.segments), waveSum(c.inconsistencies), waveSum(c.leaks)};

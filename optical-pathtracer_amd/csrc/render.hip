@@ -1172,9 +1172,48 @@ __global__ void sppmUpdateKernel(SppmDev s) {
 
 // ---------------------------------------------------------------- vertex connection and merging
 
-// The light subpaths' offsets in the cache: an exclusive scan of their vertex counts by one block of
-// kScanThreads threads, each over a contiguous run of the paths (an iteration has at most a few
-// million subpaths); offset[n] receives the total.
+// The light subpaths' offsets in the cache, in three launches: per-tile sums of the vertex counts
+// (kScanTile consecutive subpaths per block, read coalesced), an exclusive scan of the tile sums by
+// one block, then each tile's own scan plus its tile's offset, with the subpath's staged vertices
+// copied to their place in the cache.
+constexpr uint32_t kScanBlock = 256, kScanPer = 4, kScanTile = kScanBlock * kScanPer;
+
+// Exclusive prefix of x over the block's threads (thread order), and the block's total.
+__device__ __forceinline__ uint32_t blockExclusiveSum(uint32_t x, uint32_t& total) {
+    __shared__ uint32_t waveTotal[kScanBlock / 64];
+    const uint32_t lane = __lane_id(), wave = threadIdx.x / 64;
+    uint32_t inc = x;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(inc, off);
+        if (lane >= uint32_t(off)) inc += y;
+    }
+    if (lane == 63) waveTotal[wave] = inc;
+    __syncthreads();
+    uint32_t before = 0;
+    total = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kScanBlock / 64; ++w) {
+        const uint32_t t = waveTotal[w];
+        if (w < wave) before += t;
+        total += t;
+    }
+    return before + inc - x;
+}
+
+__global__ __launch_bounds__(kScanBlock) void vcmTileSumKernel(const uint32_t* count, uint32_t n, uint32_t* tileSum) {
+    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kScanPer; ++k)
+        if (base + k < n) sum += count[base + k];
+    uint32_t total;
+    blockExclusiveSum(sum, total);
+    if (threadIdx.x == 0) tileSum[blockIdx.x] = total;
+}
+
+// Exclusive scan of n values by one block of kScanThreads threads, each over a contiguous run;
+// offset[n] receives the total.
 constexpr int kScanThreads = 1024;
 __global__ __launch_bounds__(kScanThreads) void vcmScanKernel(const uint32_t* count, uint32_t* offset, uint32_t n) {
     __shared__ uint32_t part[kScanThreads];
@@ -1199,35 +1238,58 @@ __global__ __launch_bounds__(kScanThreads) void vcmScanKernel(const uint32_t* co
     if (t == kScanThreads - 1) offset[n] = part[t];
 }
 
-// Each light subpath's staged vertices to their place in the cache; thread p < paths.
-__global__ void vcmCompactKernel(VcmDev v, uint32_t paths) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= paths) return;
-    const uint32_t n = v.pathCount[p], o = v.offset[p];
-    for (uint32_t k = 0; k < n; ++k) v.cache[o + k] = v.staging[size_t(p) * v.pathSlots + k];
+// Each subpath's offset (its tile's offset plus its place in the tile) and its staged vertices to
+// their place in the cache; n = slots · N subpaths; offset[n] receives the total.
+__global__ __launch_bounds__(kScanBlock) void vcmScanCompactKernel(VcmDev v, uint32_t n) {
+    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
+    uint32_t c[kScanPer], sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kScanPer; ++k) {
+        c[k] = base + k < n ? v.pathCount[base + k] : 0u;
+        sum += c[k];
+    }
+    uint32_t total;
+    uint32_t o = v.tileOffset[blockIdx.x] + blockExclusiveSum(sum, total);
+#pragma unroll
+    for (uint32_t k = 0; k < kScanPer; ++k) {
+        if (base + k >= n) break;
+        v.offset[base + k] = o;
+        const LightVertex* from = v.staging + size_t(base + k) * v.pathSlots;
+        for (uint32_t i = 0; i < c[k]; ++i) v.cache[o + i] = from[i];
+        o += c[k];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanBlock - 1) v.offset[n] = o;
 }
 
-// Group g's range of the cache: thread g < G.
+// Group g's range of slot k's cache: thread 8k + g.
 __global__ void vcmRangesKernel(VcmDev v) {
-    const uint32_t g = threadIdx.x;
-    if (g == 0) v.counts[0] = v.offset[v.groups * v.pathsPerGroup];
-    if (g >= v.groups) return;
-    const uint32_t first = v.offset[g * v.pathsPerGroup], last = v.offset[(g + 1) * v.pathsPerGroup];
-    v.counts[4 + g] = last - first;
-    v.counts[16 + g] = first;
+    const uint32_t k = threadIdx.x / 8, g = threadIdx.x % 8;
+    if (k >= v.slots || g >= v.groups) return;
+    const uint32_t p = k * v.slotPaths + g * v.pathsPerGroup;
+    const uint32_t first = v.offset[p], last = v.offset[p + v.pathsPerGroup];
+    v.counts[32 * k + 4 + g] = last - first;
+    v.counts[32 * k + 16 + g] = first;
 }
 
-// The iteration's cached light vertices into the merging hash, each at the level of the merging
-// radius where it lies (vcm_grid_kernel.slang).
+// The cached light vertices into their slot's merging hash, each at the level of the merging radius
+// where it lies (vcm_grid_kernel.slang).
+// A grid of at most kGridBlocks blocks strides over the vertices the slots hold (a few percent of the
+// cache's capacity, which bounds them).
+constexpr uint32_t kGridBlocks = 2048;
 __global__ void vcmGridKernel(VcmDev v) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= min(v.counts[0], v.capacity)) return;
-    const float4 y = v.cache[i].p;
-    const f3 p = mk3(y.x, y.y, y.z);
-    const int level = sppmLevel(vcmRadius(v, p));
-    const float size = ldexpf(1.0f, level);
-    const uint32_t h = sppmHash(sppmCell(p.x, size), sppmCell(p.y, size), sppmCell(p.z, size), level, v.gridMask);
-    v.next[i] = atomicExch(&v.head[h], i);
+    const uint32_t total = min(v.offset[v.slots * v.slotPaths], v.capacity);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const LightVertex& y = v.cache[i];
+        const float4 q = y.p;
+        const uint32_t slot = __float_as_uint(y.mis.w);
+        if (slot >= v.slots) continue;  // cannot happen: the light kernel writes its own slot
+        const VcmDev s = v.at(slot);
+        const f3 p = mk3(q.x, q.y, q.z);
+        const int level = sppmLevel(vcmRadius(s, p));
+        const float size = ldexpf(1.0f, level);
+        const uint32_t h = sppmHash(sppmCell(p.x, size), sppmCell(p.y, size), sppmCell(p.z, size), level, v.gridMask);
+        v.next[i] = atomicExch(&s.head[h], i);
+    }
 }
 
 // XYZ of a contribution per wavelength (Wavelengths::toXYZ), with the wavelengths the joined subpaths
@@ -1268,7 +1330,9 @@ struct VcmCamera {
     VcmMis mis;
     uint32_t* stack;
     uint32_t group, M, base;  // the pixel's group, its cache's vertex count and first vertex
-    f3 L;               // XYZ so far
+    f3 L;               // XYZ so far but merging's
+    long long Lm[3];    // merging's XYZ in film units: integer sums, whatever order the hash visits in
+    bool bad;           // a merge gave a non-finite value (the sample is dropped, as for L)
 };
 
 // A non-specular camera vertex: what every strategy at it shares.
@@ -1454,23 +1518,34 @@ __device__ __forceinline__ void vcmMerge(VcmCamera& c, const VcmVertex& x) {
                 float val[NW];
 #pragma unroll
                 for (int k = 0; k < NW; ++k) val[k] = c.beta[k] * fc[k] * yb4[k] * (w * invEta);
-                c.L = c.L + vcmXYZ(c.wl, lightTerminated, val, c.v.invNWY);
+                const f3 m = vcmXYZ(c.wl, lightTerminated, val, c.v.invNWY);
+                if (!isfinite(m.x + m.y + m.z)) {
+                    c.bad = true;
+                    continue;
+                }
+                c.Lm[0] += fixedPoint(m.x);
+                c.Lm[1] += fixedPoint(m.y);
+                c.Lm[2] += fixedPoint(m.z);
             }
         }
     }
 }
 
-// One camera path of pixel `pixel` (traceCameraVcm, vcm.slang:523-593); its contributions, every
-// strategy but light tracing, go to the camera film.
+// One camera path of pixel `pixel` in slot k's iteration (thread k·pixels + pixel; traceCameraVcm,
+// vcm.slang:523-593); its contributions, every strategy but light tracing, go to the camera film.
 template <bool kContacts>
-__global__ __launch_bounds__(kBlock) void vcmCameraKernel(DScene sc, Settings set, VcmDev v, unsigned long long* stats) {
+__global__ __launch_bounds__(kBlock, OWE_HIP_VCM_WAVES) void vcmCameraKernel(DScene sc, Settings set, VcmDev vAll, unsigned long long* stats) {
     __shared__ uint32_t stack[48 * kBlock];
-    const uint32_t pixel = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     const DObserver& ob = sc.obs;
     const uint32_t pixels = uint32_t(ob.width) * uint32_t(ob.height);
     uint32_t nPaths = 0, nSegments = 0, nInconsistent = 0, nLeaks = 0;
-    if (pixel < pixels) {
+    if (t < pixels * vAll.slots) {
+        const uint32_t slot = t / pixels, pixel = t - slot * pixels;
+        const VcmDev v = vAll.at(slot);
         VcmCamera c{sc, set, v};
+        c.Lm[0] = c.Lm[1] = c.Lm[2] = 0;
+        c.bad = false;
         c.stack = stack + threadIdx.x;
         c.rng.seed(hashPair(set.seed ^ kVcmCamera, pixel), v.iteration);
         c.group = vcmPixelGroup(pixel, v.iteration, v.groups);
@@ -1638,9 +1713,10 @@ __global__ __launch_bounds__(kBlock) void vcmCameraKernel(DScene sc, Settings se
                     for (int i = 0; i < NW; ++i) c.beta[i] /= q;
                 }
             }
-            if (isfinite(c.L.x + c.L.y + c.L.z)) {
+            if (isfinite(c.L.x + c.L.y + c.L.z) && !c.bad) {
                 unsigned long long* fp = v.film + 3 * size_t(pixel);
-                const long long a = fixedPoint(c.L.x), bb = fixedPoint(c.L.y), cc = fixedPoint(c.L.z);
+                const long long a = fixedPoint(c.L.x) + c.Lm[0], bb = fixedPoint(c.L.y) + c.Lm[1],
+                                cc = fixedPoint(c.L.z) + c.Lm[2];
                 if (a) atomicAdd(&fp[0], (unsigned long long)a);
                 if (bb) atomicAdd(&fp[1], (unsigned long long)bb);
                 if (cc) atomicAdd(&fp[2], (unsigned long long)cc);
@@ -1703,6 +1779,7 @@ struct owe_hip_renderer {
     float vcmRadiusPerMetre = 0.0f;
     float vcmAlpha = 2.0f / 3.0f;
     uint64_t vcmIterations = 0;
+    uint32_t vcmSlotsMax = 1, vcmInFlight = 1;  // iterations the caches hold, and run per launch
     hipStream_t stream = nullptr;
     uint32_t stackCap = 16;
     bool simpleShadows = true;
@@ -2082,35 +2159,42 @@ void launchSppmCamera(owe_hip_renderer* r) {
 void runVcm(owe_hip_renderer* r, int iterations) {
     VcmDev v = r->vcm;
     const bool merging = r->vcmRadiusPerMetre > 0.0f;
+    const uint32_t N = v.slotPaths;
+    const uint32_t pixels = uint32_t(r->host.obs.width) * uint32_t(r->host.obs.height);
     OWE_HIP_CHECK(hipMemsetAsync(r->particleStats, 0, 4 * sizeof(unsigned long long), r->stream));
-    for (int it = 0; it < iterations; ++it) {
-        const uint64_t iteration = r->vcmIterations + uint64_t(it);
-        v.iteration = uint32_t(iteration);
-        v.rotation = float(sppmRotation(r->set.seed, iteration));
-        v.mergePerMetre =
-            merging ? float(double(r->vcmRadiusPerMetre) * std::pow(double(iteration + 1), (double(r->vcmAlpha) - 1.0) / 2.0))
-                    : 0.0f;
-        OWE_HIP_CHECK(hipMemsetAsync(v.counts, 0, 32 * sizeof(uint32_t), r->stream));
-        if (merging) OWE_HIP_CHECK(hipMemsetAsync(v.head, 0xFF, size_t(r->vcmGridSize) * sizeof(uint32_t), r->stream));
-        const uint32_t paths = uint32_t(r->vcmLightPaths);
+    for (int it = 0; it < iterations;) {
+        const uint32_t S = std::min<uint32_t>(r->vcmInFlight, uint32_t(iterations - it));
+        const uint64_t first = r->vcmIterations + uint64_t(it);
+        v.iteration = uint32_t(first);
+        v.slots = S;
+        v.rotationOffset = double(uint32_t(mix64(r->set.seed) >> 32)) / 4294967296.0;  // as sppmRotation
+        v.mergeBase = merging ? r->vcmRadiusPerMetre : 0.0f;
+        v.mergeExponent = (double(r->vcmAlpha) - 1.0) / 2.0;
+        OWE_HIP_CHECK(hipMemsetAsync(v.counts, 0, 32 * S * sizeof(uint32_t), r->stream));
+        if (merging)
+            OWE_HIP_CHECK(hipMemsetAsync(v.head, 0xFF, size_t(S) * r->vcmGridSize * sizeof(uint32_t), r->stream));
+        const uint32_t paths = S * N;
         if (r->host.emittedPower > 0) {
+            const uint32_t tiles = (paths + kScanTile - 1) / kScanTile;
             launchVcmLight(r->dev, r->set, v, r->lightFilm, r->particleStats, r->invNWY, paths, r->stream);
-            hipLaunchKernelGGL(vcmScanKernel, dim3{1}, dim3{kScanThreads}, 0, r->stream, v.pathCount, v.offset, paths);
-            hipLaunchKernelGGL(vcmCompactKernel, dim3{(paths + 255) / 256}, dim3{256}, 0, r->stream, v, paths);
-            hipLaunchKernelGGL(vcmRangesKernel, dim3{1}, dim3{64}, 0, r->stream, v);
+            hipLaunchKernelGGL(vcmTileSumKernel, dim3{tiles}, dim3{kScanBlock}, 0, r->stream, v.pathCount, paths, v.tileSum);
+            hipLaunchKernelGGL(vcmScanKernel, dim3{1}, dim3{kScanThreads}, 0, r->stream, v.tileSum, v.tileOffset, tiles);
+            hipLaunchKernelGGL(vcmScanCompactKernel, dim3{tiles}, dim3{kScanBlock}, 0, r->stream, v, paths);
+            hipLaunchKernelGGL(vcmRangesKernel, dim3{1}, dim3{8 * kVcmSlots}, 0, r->stream, v);
             if (merging) {
-                const uint64_t n = v.capacity;
-                hipLaunchKernelGGL(vcmGridKernel, dim3{unsigned((n + 255) / 256)}, dim3{256}, 0, r->stream, v);
+                const uint64_t n = uint64_t(paths) * v.pathSlots;
+                const unsigned blocks = unsigned(std::min<uint64_t>(kGridBlocks, (n + 255) / 256));
+                hipLaunchKernelGGL(vcmGridKernel, dim3{blocks}, dim3{256}, 0, r->stream, v);
             }
         }
-        const uint32_t pixels = uint32_t(r->host.obs.width) * uint32_t(r->host.obs.height);
         if (r->dev.contacts)
-            hipLaunchKernelGGL((vcmCameraKernel<true>), gridFor(pixels), dim3{kBlock}, 0, r->stream, r->dev, r->set, v,
+            hipLaunchKernelGGL((vcmCameraKernel<true>), gridFor(S * pixels), dim3{kBlock}, 0, r->stream, r->dev, r->set, v,
                                r->particleStats);
         else
-            hipLaunchKernelGGL((vcmCameraKernel<false>), gridFor(pixels), dim3{kBlock}, 0, r->stream, r->dev, r->set, v,
+            hipLaunchKernelGGL((vcmCameraKernel<false>), gridFor(S * pixels), dim3{kBlock}, 0, r->stream, r->dev, r->set, v,
                                r->particleStats);
         OWE_HIP_CHECK(hipGetLastError());
+        it += int(S);
     }
     unsigned long long st[4];
     OWE_HIP_CHECK(hipMemcpyAsync(st, r->particleStats, sizeof st, hipMemcpyDeviceToHost, r->stream));
@@ -2359,26 +2443,45 @@ int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render
             r->vcmLightPaths = (want + G - 1) / G * G;
             require(r->vcmLightPaths <= (1u << 30), "vcm: at most 2^30 light subpaths per iteration");
             const uint64_t slots = uint64_t(s.max_depth) + 1;
-            const uint64_t vertices = r->vcmLightPaths * slots;
-            if (2 * vertices * sizeof(LightVertex) > (uint64_t(16) << 30) || vertices >= 0xFFFFFFFFull)
+            const uint64_t vertices = r->vcmLightPaths * slots;  // per iteration
+            const uint64_t budget = uint64_t(16) << 30;          // cache + staging, every slot
+            if (2 * vertices * sizeof(LightVertex) > budget || vertices >= 0xFFFFFFFFull)
                 fail(OWE_HIP_ERR_UNSUPPORTED, "vcm: the light vertex cache for this many subpaths and this depth would "
                                               "exceed 16 GiB (lower photons_per_pixel or max_depth)");
+            // Iterations in flight: enough light subpaths per launch for sixteen waves per SIMD of
+            // the chip (2^20 on 256 compute units: eight rounds of its resident waves), within the
+            // budget and the 32-bit vertex indices.
+            hipDeviceProp_t prop;
+            OWE_HIP_CHECK(hipGetDeviceProperties(&prop, device));
+            const uint64_t fill = uint64_t(prop.multiProcessorCount) * 4 * 16 * 64;
+            uint64_t inFlight = std::min<uint64_t>(kVcmSlots, (fill + r->vcmLightPaths - 1) / r->vcmLightPaths);
+            inFlight = std::min<uint64_t>(inFlight, budget / (2 * vertices * sizeof(LightVertex)));
+            inFlight = std::min<uint64_t>(inFlight, 0xFFFFFFFEull / vertices);
+            inFlight = std::max<uint64_t>(inFlight, 1);
+            r->vcmSlotsMax = uint32_t(inFlight);
+            r->vcmInFlight = uint32_t(inFlight);
+            const uint64_t allPaths = inFlight * r->vcmLightPaths, allVertices = inFlight * vertices;
+            const uint64_t tiles = (allPaths + kScanTile - 1) / kScanTile;
             VcmDev& v = r->vcm;
-            v.cache = devAlloc<LightVertex>(r, vertices);
-            v.staging = devAlloc<LightVertex>(r, vertices);
-            v.pathCount = devAlloc<uint32_t>(r, r->vcmLightPaths);
-            v.offset = devAlloc<uint32_t>(r, r->vcmLightPaths + 1);
-            v.counts = devAlloc<uint32_t>(r, 32);
-            v.capacity = uint32_t(vertices);
+            v.cache = devAlloc<LightVertex>(r, allVertices);
+            v.staging = devAlloc<LightVertex>(r, allVertices);
+            v.pathCount = devAlloc<uint32_t>(r, allPaths);
+            v.offset = devAlloc<uint32_t>(r, allPaths + 1);
+            v.counts = devAlloc<uint32_t>(r, 32 * kVcmSlots);
+            v.tileSum = devAlloc<uint32_t>(r, tiles);
+            v.tileOffset = devAlloc<uint32_t>(r, tiles + 1);
+            v.capacity = uint32_t(allVertices);
             v.pathSlots = uint32_t(slots);
             v.pathsPerGroup = uint32_t(r->vcmLightPaths / G);
             v.groups = G;
+            v.slotPaths = uint32_t(r->vcmLightPaths);
+            v.slots = 1;
             v.lightPaths = float(r->vcmLightPaths / G);
             uint64_t gridSize = 1024;
             while (gridSize < std::min<uint64_t>(2 * vertices, uint64_t(1) << 28)) gridSize <<= 1;
             r->vcmGridSize = uint32_t(gridSize);
-            v.head = devAlloc<uint32_t>(r, gridSize);
-            v.next = devAlloc<uint32_t>(r, vertices);
+            v.head = devAlloc<uint32_t>(r, gridSize * inFlight);
+            v.next = devAlloc<uint32_t>(r, allVertices);
             v.gridMask = uint32_t(gridSize - 1);
             v.film = r->film;
             v.invNWY = r->invNWY;
@@ -2694,6 +2797,28 @@ int owe_hip_renderer_set_mode(owe_hip_renderer* r, int mode, int blocks_per_cu, 
         if (refill > 0) r->regen = uint32_t(refill);
         r->useGraph = use_graph != 0;
         dropGraph(r);
+    });
+}
+
+int owe_hip_renderer_set_vcm_in_flight(owe_hip_renderer* r, int iterations) {
+    return guard([&] {
+        require(r != nullptr, "renderer is null");
+        require(r->integrator == OWE_INTEGRATOR_VCM || r->integrator == OWE_INTEGRATOR_BDPT,
+                "iterations in flight apply to the vcm and bdpt integrators");
+        require(iterations >= 0 && uint32_t(iterations) <= r->vcmSlotsMax,
+                "vcm: iterations in flight must lie in [0, " + std::to_string(r->vcmSlotsMax) +
+                    "] (the caches this renderer holds)");
+        r->vcmInFlight = iterations == 0 ? r->vcmSlotsMax : uint32_t(iterations);
+    });
+}
+
+int owe_hip_renderer_get_vcm_in_flight(const owe_hip_renderer* r, int* iterations, int* most) {
+    return guard([&] {
+        require(r != nullptr, "renderer is null");
+        require(r->integrator == OWE_INTEGRATOR_VCM || r->integrator == OWE_INTEGRATOR_BDPT,
+                "iterations in flight apply to the vcm and bdpt integrators");
+        if (iterations) *iterations = int(r->vcmInFlight);
+        if (most) *most = int(r->vcmSlotsMax);
     });
 }
 

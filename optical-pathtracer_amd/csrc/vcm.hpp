@@ -31,37 +31,70 @@ struct LightVertex {
     float4 n;     // geometric normal (toward the boundary's front); w: dVCM
     float4 d;     // direction the light arrived along; w: flags (bits): kLvTerminated | (face + 1) << 1
     float4 beta;  // the subpath's throughput arriving (per wavelength)
-    float4 mis;   // x: dVC, y: dVM, z: group (bits), w: unused
+    float4 mis;   // x: dVC, y: dVM, z: group (bits), w: the iteration's slot (bits)
 };
 constexpr uint32_t kLvTerminated = 1u;
 
-// The cache is filled in a deterministic order: light subpath j of the iteration (group g = j mod G,
-// i = j / G) is path p = g·N_L + i; it writes its vertices to staging slots [p·K, p·K + n_p) (K =
-// max_depth + 1 bounds n_p) and n_p to pathCount[p]; an exclusive scan of the counts gives each
-// path's offset, and its vertices are copied to cache[offset_p, offset_p + n_p). Group g's vertices
-// are then the contiguous range [counts[16 + g], counts[16 + g] + counts[4 + g]) in path order — the
-// same cache whatever order the subpaths finish in, so a camera path's connection draw picks the
-// same vertex every time.
+// Iterations in flight: the iterations of a pass are independent but for their merging radius and
+// wavelength rotation, so up to kVcmSlots of them run in each launch, each in its own slot of the
+// cache, the offsets and the merging hash (a 256² image has 65,536 light subpaths per iteration: a
+// quarter of a wave per SIMD of the chip; eight in flight fill it).
+constexpr uint32_t kVcmSlots = 16;
+
+// Resident waves per SIMD the light and camera kernels are compiled for: their registers are capped
+// to fit two (a few spills; measured on a 256² caustic with eight iterations in flight, 0.31 ms per
+// iteration at one wave, 0.22 at two, 0.26 at three).
+#ifndef OWE_HIP_VCM_WAVES
+#define OWE_HIP_VCM_WAVES 2
+#endif
+
+// The cache is filled in a deterministic order: light subpath j of slot k's iteration (group g =
+// j mod G, i = j / G) is path p = k·N + g·N_L + i; it writes its vertices to staging slots [p·K,
+// p·K + n_p) (K = max_depth + 1 bounds n_p) and n_p to pathCount[p]; an exclusive scan of the counts
+// gives each path's offset, and its vertices are copied to cache[offset_p, offset_p + n_p). Group g
+// of slot k is then the contiguous range [counts[32k + 16 + g], … + counts[32k + 4 + g]) in path
+// order — the same cache whatever order the subpaths finish in, so a camera path's connection draw
+// picks the same vertex every time, with any number of iterations in flight.
 struct VcmDev {
-    LightVertex* cache;    // the iteration's vertices, group by group, in path order
+    LightVertex* cache;    // the slots' vertices, slot by slot, group by group, in path order
     LightVertex* staging;  // K slots per light subpath
-    uint32_t* pathCount;   // vertices per light subpath (path p = g·N_L + i)
-    uint32_t* offset;      // their exclusive prefix sums; offset[paths] = the total
-    uint32_t* counts;      // [0] the total; [4 + g] group g's vertex count; [16 + g] its first vertex
-    uint32_t capacity;     // vertices the cache holds (every subpath's K)
+    uint32_t* pathCount;   // vertices per light subpath (path p = k·N + g·N_L + i)
+    uint32_t* offset;      // their exclusive prefix sums; offset[slots · N] = the total
+    uint32_t* counts;      // per slot 32: [4 + g] group g's vertex count; [16 + g] its first vertex
+    uint32_t* tileSum;     // the scan's per-tile sums ...
+    uint32_t* tileOffset;  // ... and their exclusive prefix sums
+    uint32_t capacity;     // vertices the cache holds (every subpath's K, in every slot)
     uint32_t pathSlots;    // K
     uint32_t pathsPerGroup;
     uint32_t groups;       // wavelength groups G
-    float lightPaths;      // N_L: light subpaths per group this iteration
-    uint32_t iteration;
-    float rotation;        // the iteration's base wavelength rotation
-    float mergePerMetre;   // merging radius per metre of distance from the eye; 0: no merging (bdpt)
+    uint32_t slotPaths;    // N = G·N_L: light subpaths per iteration
+    uint32_t slots;        // iterations in flight in this launch
+    float lightPaths;      // N_L: light subpaths per group and iteration
+    uint32_t iteration;    // slot 0's iteration (slot k runs iteration + k); in a slot's view, its own
+    float rotation;        // a slot's view: its iteration's base wavelength rotation
+    float mergePerMetre;   // a slot's view: merging radius per metre from the eye; 0: no merging (bdpt)
     float mergeFloor;
-    uint32_t* head;        // the merging hash: bucket → first vertex (kGridNone: none) ...
-    uint32_t* next;        // ... vertex → next vertex of its bucket
+    uint32_t* head;        // the merging hash, gridMask + 1 buckets per slot: bucket → first vertex ...
+    uint32_t* next;        // ... vertex → next vertex of its bucket (kGridNone ends a bucket)
     uint32_t gridMask;
     unsigned long long* film;  // the camera film (32.32 fixed point, three channels per pixel)
     float invNWY;              // 1 / (NW ∫ȳ)
+    float mergeBase;           // the merging radius per metre at the first iteration (0: bdpt) ...
+    double mergeExponent;      // ... times (i + 1)^mergeExponent at iteration i: (α − 1)/2
+    double rotationOffset;     // the wavelength rotation of iteration i: frac(offset + φ⁻¹(i + 1))
+
+    // Slot k's view: its iteration, rotation, radius, counts and hash.
+    __device__ __forceinline__ VcmDev at(uint32_t k) const {
+        VcmDev s = *this;
+        s.iteration = iteration + k;
+        const double u = rotationOffset + 0.6180339887498949 * double(s.iteration + 1u);
+        s.rotation = float(u - floor(u));
+        s.mergePerMetre = mergeBase > 0.0f ? float(double(mergeBase) * pow(double(s.iteration + 1u), mergeExponent))
+                                           : 0.0f;
+        s.counts = counts + 32u * k;
+        s.head = head + size_t(k) * (size_t(gridMask) + 1);
+        return s;
+    }
 };
 
 __device__ __forceinline__ uint32_t vcmPixelGroup(uint32_t pixel, uint32_t iteration, uint32_t groups) {
@@ -136,8 +169,8 @@ __device__ __forceinline__ bool vcmEval(const DScene& sc, const Iface& it, uint3
     return any;
 }
 
-// Enqueues an iteration's light subpaths [0, count) (light tracing into the particle film's bins, the
-// cache filled) on `stream`.
+// Enqueues the light subpaths [0, count) of the slots' iterations (light tracing into the particle
+// film's bins, the staging slots filled) on `stream`; count = slots · N.
 void launchVcmLight(const DScene& sc, const Settings& set, const VcmDev& v, unsigned long long* lightFilm,
                     unsigned long long* stats, float invNWY, uint32_t count, hipStream_t stream);
 

@@ -106,6 +106,8 @@ def load() -> ctypes.CDLL:
     _declare(lib, "owe_hip_renderer_particles", c_int, [c_vp, c_int, c_vp, c_u64, c_vp])
     _declare(lib, "owe_hip_renderer_set_mode", c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_int])
     _declare(lib, "owe_hip_renderer_get_mode", c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp])
+    _declare(lib, "owe_hip_renderer_set_vcm_in_flight", c_int, [c_vp, c_int])
+    _declare(lib, "owe_hip_renderer_get_vcm_in_flight", c_int, [c_vp, c_vp, c_vp])
     _declare(lib, "owe_hip_set_tuning_cache", c_int, [ctypes.c_char_p])
     _declare(lib, "owe_hip_renderer_tuning", c_int, [c_vp, ctypes.POINTER(c_int), ctypes.POINTER(c_u64), ctypes.POINTER(c_int)])
     _declare(lib, "owe_hip_renderer_autotune", c_int, [c_vp, c_int, c_vp])

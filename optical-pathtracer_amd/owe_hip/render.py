@@ -92,6 +92,17 @@ class Renderer:
         return dict(mode=v[0].value, blocks_per_cu=v[1].value, chunk=v[2].value, use_graph=bool(v[3].value),
                     waves_per_simd=v[4].value, refill=v[5].value)
 
+    def set_vcm_in_flight(self, iterations: int) -> None:
+        """vcm/bdpt: iterations run together (1 up to what the caches hold; 0: all of those). The
+        image is the same bit for bit whatever the number."""
+        _lib.check(_lib.load().owe_hip_renderer_set_vcm_in_flight(self._h, int(iterations)))
+
+    def vcm_in_flight(self) -> tuple[int, int]:
+        """(iterations run together, the most the renderer's caches hold)."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        _lib.check(_lib.load().owe_hip_renderer_get_vcm_in_flight(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
     def autotune(self, spp: int) -> float:
         """Picks the fastest launch shape of the current form over passes of spp; returns its ms."""
         ms = ctypes.c_double()

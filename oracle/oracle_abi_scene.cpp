@@ -306,7 +306,7 @@ void oracle_camera_paths(const void* wh, const owe_render_settings* rs, int spp,
 // wavelengths; the cache is the subpaths' non-specular vertices in the order (g, j / G, vertex); the
 // camera path of a pixel runs on (hashCombine(seed ^ kVcmCamera, pixel), iteration) in the group a
 // hash of (iteration, pixel) picks, and merges with every cached vertex of its group within the
-// radius (the multi-level hash finds the same set). camera (W·H·3) += the camera paths' XYZ (the
+// radius (the multi-level hash finds the same set; here a sweep over the group's vertices by x). camera (W·H·3) += the camera paths' XYZ (the
 // caller divides by the iterations); light (W·H·3) += the light subpaths' splats (the caller divides
 // by N times the iterations); stats += paths (light and camera), segments, inconsistencies, leaks;
 // cached[it - start] = the iteration's cache size when cached is given.
@@ -372,12 +372,24 @@ void oracle_vcm(const void* wh, const owe_render_settings* rs, int merging, int 
         first[G] = cache.size();
         if (cached) cached[it - start] = cache.size();
         paths.clear();
+        // Each group's vertices by x for merging's sweep.
+        std::vector<uint32_t> index(cache.size());
+        std::vector<double> xs(cache.size());
+        for (uint32_t g = 0; g < G; ++g) {
+            uint32_t* ix = index.data() + first[g];
+            const uint32_t m = uint32_t(first[g + 1] - first[g]);
+            for (uint32_t i = 0; i < m; ++i) ix[i] = i;
+            const Tracer::VcmVertex* c = cache.data() + first[g];
+            std::sort(ix, ix + m, [&](uint32_t a, uint32_t b) { return c[a].p.x < c[b].p.x; });
+            for (uint32_t i = 0; i < m; ++i) xs[first[g] + i] = c[ix[i]].p.x;
+        }
         parallel(pixels, [&](int k, uint64_t pixel) {
             const uint32_t g = uint32_t(mix64((uint64_t(uint32_t(it)) << 32 | pixel) ^ kVcmGroup) % uint64_t(G));
             Pcg rng(hashPair(set.seed ^ kVcmCamera, pixel), uint64_t(it));
             double c[3];
             if (tr.vcmCamera(int(pixel % uint64_t(W)), int(pixel / uint64_t(W)), sppmWavelengths(set, rot, g), rng,
-                             st[size_t(k)], p, cache.data() + first[g], uint32_t(first[g + 1] - first[g]), c))
+                             st[size_t(k)], p, cache.data() + first[g], uint32_t(first[g + 1] - first[g]), c,
+                             Tracer::VcmSweep{xs.data() + first[g], index.data() + first[g]}))
                 for (int q = 0; q < 3; ++q) camera[3 * pixel + size_t(q)] += c[q];
         });
     }

@@ -352,6 +352,12 @@ public:
         double mergePerMetre = 0;  // merging radius per metre from the eye (0: bdpt)
         uint32_t groups = 1, iteration = 0;
     };
+    // The group's cached vertices ordered by x (their x and index in the group), for merging: the
+    // vertices within r of p have |x − p.x| ≤ r. Without it merging visits every vertex.
+    struct VcmSweep {
+        const double* x;
+        const uint32_t* index;
+    };
     // Light subpath of group `group` (traceLightVcm, vcm.slang:246-292): its non-specular vertices
     // appended to `cache` in order, its connections to the eye (light tracing) to `out`.
     void vcmLight(Wavelengths wl, Pcg& rng, Stats& st, Splat& out, const VcmParams& p, uint32_t group,
@@ -359,7 +365,7 @@ public:
     // The camera path of pixel (x, y) (traceCameraVcm, vcm.slang:523-593) against its group's M cached
     // vertices: its XYZ from every strategy but light tracing. False when the sample measures nothing.
     bool vcmCamera(int x, int y, Wavelengths wl, Pcg& rng, Stats& st, const VcmParams& p, const VcmVertex* cache,
-                   uint32_t M, double xyz[3]) const;
+                   uint32_t M, double xyz[3], const VcmSweep& sweep = VcmSweep{nullptr, nullptr}) const;
 
 private:
     struct Link;

@@ -1124,7 +1124,7 @@ void Tracer::vcmLight(Wavelengths wl, Pcg& rng, Stats& st, Splat& out, const Vcm
 }
 
 bool Tracer::vcmCamera(int x, int y, Wavelengths wl, Pcg& rng, Stats& st, const VcmParams& p, const VcmVertex* cache,
-                       uint32_t M, double xyz[3]) const {
+                       uint32_t M, double xyz[3], const VcmSweep& sweep) const {
     xyz[0] = xyz[1] = xyz[2] = 0;
     const Observer& ob = w_.observer;
     const double px = x + rng.uniform(), py = y + rng.uniform();
@@ -1353,8 +1353,13 @@ bool Tracer::vcmCamera(int x, int y, Wavelengths wl, Pcg& rng, Stats& st, const 
             if (p.mergePerMetre > 0) {
                 const double r = std::max(p.mergePerMetre * length(hit.p - ob.position), 1e-9), r2 = r * r;
                 const double invEta = 1 / eta;
-                for (uint32_t m = 0; m < M; ++m) {
-                    const VcmVertex& yv = cache[m];
+                uint32_t lo = 0, hi = M;
+                if (sweep.x) {
+                    lo = uint32_t(std::lower_bound(sweep.x, sweep.x + M, hit.p.x - r) - sweep.x);
+                    hi = uint32_t(std::upper_bound(sweep.x, sweep.x + M, hit.p.x + r) - sweep.x);
+                }
+                for (uint32_t m = lo; m < hi; ++m) {
+                    const VcmVertex& yv = cache[sweep.x ? sweep.index[m] : m];
                     const V3 dp = yv.p - hit.p;
                     if (dot(dp, dp) > r2) continue;
                     if (std::fabs(dot(dp, hit.n)) > 0.25 * r || dot(yv.n, hit.n) < 0.5) continue;

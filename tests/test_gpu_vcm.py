@@ -128,3 +128,28 @@ def test_matches_the_oracle_pixel_by_pixel(cuda, oracle, case, integrator, itera
     s = r.stats()
     assert s["paths"] == o["stats"][0]
     assert s["leaks"] == o["stats"][3] == 0
+
+
+@pytest.mark.parametrize("integrator", ["vcm", "bdpt"])
+def test_iterations_in_flight_give_the_same_image(cuda, integrator):
+    """Iterations run together, each in its own slot of the cache and the merging hash, give the
+    image of iterations run one at a time bit for bit (each iteration's cache is filled in subpath
+    order and its merges are summed in integers), and a second renderer repeats it exactly."""
+    ls = C.caustic_cases(resolution=48)["plate"]
+    sc, st = ls.use(), ls.settings(seed=9, integrator=integrator)
+    images = []
+    for inflight in (1, 0, 3):
+        r = render.Renderer(sc, st)
+        r.set_vcm_in_flight(inflight)
+        n, most = r.vcm_in_flight()
+        print(f"{integrator}: {n} iterations in flight (at most {most}), 2 passes of 7", flush=True)
+        r.run_pass(7)
+        r.run_pass(7)
+        images.append(r.resolve())
+        r.close()
+    assert most > 1
+    assert np.array_equal(images[0], images[1]) and np.array_equal(images[0], images[2])
+    r = render.Renderer(sc, st)
+    with pytest.raises(Exception):
+        r.set_vcm_in_flight(most + 1)
+    r.close()
