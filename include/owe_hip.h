@@ -40,7 +40,7 @@ const char* owe_hip_last_error(void);
 const char* owe_hip_backend_name(void);
 /* One line: hardware, precision and role. */
 const char* owe_hip_backend_summary(void);
-/* The integrators a renderer of this backend implements, comma-separated: "path,light,sppm"
+/* The integrators a renderer of this backend implements, comma-separated: "path,light,sppm,vcm,bdpt"
  * (owe::Backend::integrators, backend.hpp:41-42). */
 const char* owe_hip_backend_integrators(void);
 /* 1 when a gfx950 device is usable here, else 0 with the reason written to why[0..cap). */
@@ -122,8 +122,13 @@ typedef struct owe_hip_renderer owe_hip_renderer;
  * the particles emitted, gpu_renderer_film.cpp:17-20) and OWE_INTEGRATOR_SPPM (stochastic
  * progressive photon mapping, sppm.slang; a pass of spp runs spp iterations, each one camera path
  * per pixel and photons_per_pixel·W·H photons, GpuRenderer::passSppm, gpu_renderer_passes.cpp:259-290;
- * resolve gives L_direct / iterations + τ / (N_emitted π r²)). Fails with OWE_HIP_ERR_UNSUPPORTED
- * for any other integrator (renderer.hpp:22-36). */
+ * resolve gives L_direct / iterations + τ / (N_emitted π r²)), OWE_INTEGRATOR_VCM and
+ * OWE_INTEGRATOR_BDPT (vertex connection and merging, vcm.slang, and the same without merging,
+ * GpuRenderer::runVcmIterations, gpu_renderer_passes.cpp:136-199: a pass of spp runs spp iterations,
+ * each photons_per_pixel·W·H light subpaths cached and splatted to the eye and one camera path per
+ * pixel; resolve gives the camera paths' sum / iterations + the splats / (light subpaths ·
+ * iterations); scenes with scattering media or a surface sensor are refused). Fails with
+ * OWE_HIP_ERR_UNSUPPORTED for any other integrator (renderer.hpp:22-36). */
 int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render_settings* settings,
                             owe_hip_renderer** out);
 int owe_hip_renderer_destroy(owe_hip_renderer* renderer);
