@@ -983,9 +983,10 @@ __global__ __launch_bounds__(kBlock, kWaves) void persistentBatchKernel(const DS
 // at the wavelengths of the pixel's group. The path's own radiance goes to the film (one sample per
 // pixel per iteration); its visible point, if any, to the pixel's slot with the unfolded path length.
 // It reads nothing the previous iteration's photons or update write, so it runs beside them on a
-// second stream, into the other of two visible-point buffers.
+// second stream, into the other of two visible-point buffers. Compiled for two waves per SIMD (with
+// contacts its registers would allow one).
 template <int kCap, bool kContacts>
-__global__ __launch_bounds__(kBlock) void sppmCameraKernel(DScene sc, Settings set, SppmDev s, unsigned long long* film,
+__global__ __launch_bounds__(kBlock, 2) void sppmCameraKernel(DScene sc, Settings set, SppmDev s, unsigned long long* film,
                                                            uint32_t* counters, int simple) {
     __shared__ uint32_t stack[kCap * kBlock];
     uint32_t* st = stack + threadIdx.x;
