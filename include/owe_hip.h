@@ -40,7 +40,7 @@ const char* owe_hip_last_error(void);
 const char* owe_hip_backend_name(void);
 /* One line: hardware, precision and role. */
 const char* owe_hip_backend_summary(void);
-/* The integrators a renderer of this backend implements, comma-separated: "path,light,sppm,vcm,bdpt"
+/* The integrators a renderer of this backend implements, comma-separated: "path,light,sppm,vcm,bdpt,hybrid"
  * (owe::Backend::integrators, backend.hpp:41-42). */
 const char* owe_hip_backend_integrators(void);
 /* 1 when a gfx950 device is usable here, else 0 with the reason written to why[0..cap). */
@@ -127,8 +127,11 @@ typedef struct owe_hip_renderer owe_hip_renderer;
  * GpuRenderer::runVcmIterations, gpu_renderer_passes.cpp:136-199: a pass of spp runs spp iterations,
  * each photons_per_pixel·W·H light subpaths cached and splatted to the eye and one camera path per
  * pixel; resolve gives the camera paths' sum / iterations + the splats / (light subpaths ·
- * iterations); scenes with scattering media or a surface sensor are refused). Fails with
- * OWE_HIP_ERR_UNSUPPORTED for any other integrator (renderer.hpp:22-36). */
+ * iterations); scenes with scattering media or a surface sensor are refused) and
+ * OWE_INTEGRATOR_HYBRID (a pass of spp runs the camera paths, then W·H·spp particles, with the
+ * partition of particle.slang:358-417 and transport.slang:745-906: light tracing owns the paths
+ * eye → D → S⁺ → light, camera paths every other; resolve adds both films; observers only). Fails
+ * with OWE_HIP_ERR_UNSUPPORTED for any other integrator (renderer.hpp:22-36). */
 int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render_settings* settings,
                             owe_hip_renderer** out);
 int owe_hip_renderer_destroy(owe_hip_renderer* renderer);

@@ -191,6 +191,7 @@ typedef struct owe_scene {
 #define OWE_INTEGRATOR_SPPM 2
 #define OWE_INTEGRATOR_VCM 3
 #define OWE_INTEGRATOR_BDPT 4
+#define OWE_INTEGRATOR_HYBRID 5
 
 /* Wavelength sampling: the reference's four stratified hero wavelengths importance-sampled over
  * 360–830 nm (core/wavelengths.hpp:36-45), or one wavelength per path uniform over

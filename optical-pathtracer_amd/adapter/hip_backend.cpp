@@ -306,9 +306,12 @@ public:
         s.seed = rs.seed;
         s.max_depth = rs.maxDepth;
         s.rr_depth = rs.rrDepth;
-        s.integrator = rs.integrator == "light"  ? OWE_INTEGRATOR_LIGHT
-                       : rs.integrator == "sppm" ? OWE_INTEGRATOR_SPPM
-                                                 : OWE_INTEGRATOR_PATH;
+        s.integrator = rs.integrator == "light"    ? OWE_INTEGRATOR_LIGHT
+                       : rs.integrator == "sppm"   ? OWE_INTEGRATOR_SPPM
+                       : rs.integrator == "vcm"    ? OWE_INTEGRATOR_VCM
+                       : rs.integrator == "bdpt"   ? OWE_INTEGRATOR_BDPT
+                       : rs.integrator == "hybrid" ? OWE_INTEGRATOR_HYBRID
+                                                   : OWE_INTEGRATOR_PATH;
         s.spectral = OWE_SPECTRAL_HERO4;
         s.lambda_min = 380.0;
         s.lambda_max = 780.0;
@@ -316,7 +319,8 @@ public:
         s.photon_radius = rs.photonRadius;
         s.photon_alpha = rs.photonAlpha;
         s.photons_per_pixel = rs.photonsPerPixel;
-        s.photon_groups = rs.photonWavelengthGroups;
+        s.photon_groups = rs.integrator == "vcm" || rs.integrator == "bdpt" ? rs.vcmWavelengthGroups
+                                                                            : rs.photonWavelengthGroups;
         check(owe_hip_renderer_create(rs.device < 0 ? 0 : rs.device, &flat_.scene, &s, &r_));
     }
     ~HipRenderer() override {

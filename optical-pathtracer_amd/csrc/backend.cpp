@@ -50,7 +50,7 @@ const char* owe_hip_backend_summary(void) {
            "IEEE-754 double reference";
 }
 
-const char* owe_hip_backend_integrators(void) { return "path,light,sppm,vcm,bdpt"; }
+const char* owe_hip_backend_integrators(void) { return "path,light,sppm,vcm,bdpt,hybrid"; }
 
 int owe_hip_available(char* why, size_t cap) {
     int n = 0;

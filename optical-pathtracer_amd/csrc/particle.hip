@@ -439,6 +439,7 @@ __device__ __forceinline__ void sppmGather(PCtx& c, const SppmDev& s, const Hit&
 struct PState {
     PRay ray;
     int depth, events;
+    int nonDelta, deltas;  // non-specular and specular events (the hybrid partition)
 };
 
 // One segment of the walk: returns whether the particle goes on. traceParticle runs it to the end in
@@ -452,6 +453,10 @@ __device__ __forceinline__ bool particleSegment(PCtx& c, PState& st, const SppmD
     PRay& ray = st.ray;
     int& events = st.events;
     const int depth = st.depth;
+    // The hybrid partition (particle.slang:358-417): connect only at the first non-specular vertex,
+    // after at least one specular event, and end there.
+    const bool partition = !kSppm && (c.set.mode & kModePartition) != 0;
+    const bool mayConnect = !partition || (st.nonDelta == 0 && st.deltas >= 1);
     {
         Hit h;
         bool found = intersectContact<true, kContacts>(sc, ray.o, ray.d, __builtin_inff(), ray.departed, ray.face, kNoBoundary, ray.region,
@@ -479,7 +484,10 @@ __device__ __forceinline__ bool particleSegment(PCtx& c, PState& st, const SppmD
                 for (int i = 0; i < NW; ++i)
                     c.beta[i] *= evalSpec(med.scattering, sc.table, c.wl.lambda[i]) * expf(-sT[i] * tm) / pdf0;
                 const f3 p = ray.o + ray.d * tm;
-                if (observer) connectVertex<kContacts>(c, p, false, nullptr, kNoBoundary, ~0u, mk3(0, 0, 0), ray.d, med.g, ray.region);
+                if (observer && mayConnect)
+                    connectVertex<kContacts>(c, p, false, nullptr, kNoBoundary, ~0u, mk3(0, 0, 0), ray.d, med.g, ray.region);
+                ++st.nonDelta;
+                if (partition) return false;
                 if (depth >= maxDepth) return false;
                 const float u2 = c.rng.uniform(), u1 = c.rng.uniform();
                 f3 wi;
@@ -546,8 +554,13 @@ __device__ __forceinline__ bool particleSegment(PCtx& c, PState& st, const SppmD
         if (o.type == OWE_SURF_DETECTOR || o.type == OWE_SURF_ABSORBER) return false;
         const float hero = c.wl.lambda[0];
         const Iface it{&o, h.n, regionIndex(sc, b.front, hero), regionIndex(sc, b.back, hero), hero, c.set.fresnelFloor};
-        if (!o.delta && observer) connectVertex<kContacts>(c, h.p, true, &it, bi, hitFace, h.n, ray.d, 0.0f, ray.region);
+        if (!o.delta && observer && mayConnect)
+            connectVertex<kContacts>(c, h.p, true, &it, bi, hitFace, h.n, ray.d, 0.0f, ray.region);
         if (kSppm && !o.delta && events >= 1) sppmGather(c, *sp, h, ray.d, group);
+        if (!o.delta) {
+            ++st.nonDelta;
+            if (partition) return false;
+        }
         if (depth >= maxDepth) return false;
         auto at = [&](int i) {
             Iface r = it;
@@ -585,6 +598,7 @@ __device__ __forceinline__ bool particleSegment(PCtx& c, PState& st, const SppmD
         ray.departed = bi;
         ray.face = hitFace;
         ++events;
+        if (s.delta) ++st.deltas;
         // Particles carry flux: roulette against the event's own throughput.
         if (depth >= rrDepth && s.weight < 1.0f && !roulette(c, s.weight)) return false;
     }
@@ -592,15 +606,16 @@ __device__ __forceinline__ bool particleSegment(PCtx& c, PState& st, const SppmD
     return true;
 }
 
-// traceParticle (particle.slang:356-427), without the hybrid partition; with kSppm an SPPM photon
-// (traceSppmPhoton, sppm.slang:150-201): no observer connections or sensor splats, a gather at every
-// non-specular surface after the first scattering event.
+// traceParticle (particle.slang:356-427), with the hybrid partition when the settings ask for it
+// (no direct view of an emitter then); with kSppm an SPPM photon (traceSppmPhoton,
+// sppm.slang:150-201): no observer connections or sensor splats, a gather at every non-specular
+// surface after the first scattering event.
 template <bool kContacts, bool kSppm = false>
 __device__ __forceinline__ void traceParticle(PCtx& c, const SppmDev* sp = nullptr, uint32_t group = 0) {
-    const bool observer = !kSppm && c.sc.obs.kind == OWE_DETECTOR_OBSERVER;
+    const bool observer = !kSppm && c.sc.obs.kind == OWE_DETECTOR_OBSERVER && !(c.set.mode & kModePartition);
     PState st;
     st.depth = 0;
-    st.events = 0;
+    st.events = st.nonDelta = st.deltas = 0;
     if (!emitParticle<kContacts, kSppm>(c, observer, st.ray, sp)) return;
     while (particleSegment<kContacts, kSppm>(c, st, sp, group)) {
     }
@@ -670,10 +685,10 @@ __global__ __launch_bounds__(kPBlock) void particleRefillKernel(DScene sc, Setti
     PCtx c{sc, set, pb, stack + threadIdx.x};
     c.sum[0] = c.sum[1] = c.sum[2] = 0.0f;
     c.segments = c.inconsistencies = c.leaks = 0;
-    const bool observer = sc.obs.kind == OWE_DETECTOR_OBSERVER;
+    const bool observer = sc.obs.kind == OWE_DETECTOR_OBSERVER && !(set.mode & kModePartition);  // emitter seen directly
     const uint32_t lane = __lane_id();
     PState st;
-    st.depth = st.events = 0;
+    st.depth = st.events = st.nonDelta = st.deltas = 0;
     bool live = false, done = false;
     uint64_t id = 0, end = 0;
     uint32_t started = 0;
@@ -696,7 +711,7 @@ __global__ __launch_bounds__(kPBlock) void particleRefillKernel(DScene sc, Setti
         if (refill && !live && !done) {
             startParticle(c, set, pb.base + id, pb.pass);
             st.depth = 0;
-            st.events = 0;
+            st.events = st.nonDelta = st.deltas = 0;
             live = emitParticle<kContacts>(c, observer, st.ray);
             ++started;
             ++id;

@@ -42,7 +42,37 @@ namespace {
 constexpr int kBlock = 256;
 constexpr double kFilmScale = 4294967296.0;  // 2^32 units per unit of XYZ
 
-enum : uint32_t { kFlagSpecular = 1u, kFlagTerminated = 2u };
+enum : uint32_t {
+    kFlagSpecular = 1u,
+    kFlagTerminated = 2u,
+    // The hybrid partition's record of a camera path's scattering events (noteScatter,
+    // transport.slang:783-791): any event yet, the first one non-delta, a delta event since the last
+    // non-delta one, the non-delta events (two bits, saturating at 2); and a medium scattering vertex
+    // met (the medium-only pass).
+    kFlagEvents = 4u,
+    kFlagFirstNonDelta = 8u,
+    kFlagDeltaSince = 16u,
+    kFlagMediumScattered = 32u,
+    kNonDeltaShift = 6,
+    kNonDeltaMask = 3u << kNonDeltaShift,
+};
+
+__device__ __forceinline__ void noteScatter(uint32_t& flags, bool delta) {
+    if (delta) {
+        flags |= kFlagDeltaSince;
+    } else {
+        if (!(flags & kFlagEvents)) flags |= kFlagFirstNonDelta;
+        const uint32_t n = min(((flags & kNonDeltaMask) >> kNonDeltaShift) + 1u, 2u);
+        flags = (flags & ~(kNonDeltaMask | kFlagDeltaSince)) | (n << kNonDeltaShift);
+    }
+    flags |= kFlagEvents;
+}
+
+// stepPath's ownedByLightTracing (transport.slang:890-891): one non-delta event, the first, then at
+// least one delta event — light tracing's path eye → D → S⁺ → light.
+__device__ __forceinline__ bool ownedByLightTracing(uint32_t flags) {
+    return ((flags & kNonDeltaMask) >> kNonDeltaShift) == 1u && (flags & kFlagFirstNonDelta) && (flags & kFlagDeltaSince);
+}
 
 // Structure-of-arrays path state, one slot per path of the current chunk.
 struct Paths {
@@ -412,6 +442,7 @@ __device__ __forceinline__ bool shadeOne(ShadeCtx& c, const uint4 hraw, uint32_t
             const f3 p = c.o + c.d * tm;
             if (kSppm && rec->valid) return false;
             if (int(c.depth) >= c.set.maxDepth) return false;
+            c.flags |= kFlagMediumScattered;
             prepareDirect(c, p, false, nullptr, kNoBoundary, 0, kEmptyChild, mk3(0, 0, 0), med.g);
             const float u2 = c.rng.uniform(), u1 = c.rng.uniform();
             f3 wi;
@@ -421,6 +452,7 @@ __device__ __forceinline__ bool shadeOne(ShadeCtx& c, const uint4 hraw, uint32_t
                 rec->nonDelta++;
             }
             c.flags &= ~kFlagSpecular;
+            noteScatter(c.flags, false);
             c.prevP = p;
             c.o = p;
             c.d = wi;
@@ -438,8 +470,13 @@ __device__ __forceinline__ bool shadeOne(ShadeCtx& c, const uint4 hraw, uint32_t
         for (int i = 0; i < NW; ++i) setc(c.beta, i, get(c.beta, i) * expf(-evalSpec(med.absorption, sc.table, c.wl.lambda[i]) * t));
     }
     if (kSppm && (med.flags & 1) && !(med.flags & 2)) rec->chromatic = true;
+    // Light this path may count: under the hybrid partition not what light tracing owns; in the
+    // medium-only pass only after a medium scattering vertex (stepPath, transport.slang:868-906).
+    const bool counts = !(c.set.mode & kModeMediumOnly) || (c.flags & kFlagMediumScattered);
+    const bool owned = (c.set.mode & kModePartition) && ownedByLightTracing(c.flags);
     if (!found) {
         if (c.region == 0) {
+            if (owned || !counts) return false;
             // skyMet (transport.cpp:401-417); the sun's disc test in the cancellation-free form
             // 1 − cos θ = |d − s|² / 2 for unit vectors.
             const f3 s = ld3(sc.env.sunDir), dd = c.d - s;
@@ -482,7 +519,7 @@ __device__ __forceinline__ bool shadeOne(ShadeCtx& c, const uint4 hraw, uint32_t
     if (kSppm && !rec->valid) rec->pathLength += t;
     const DOptics& o = sc.optics[b.optics];
     // emissionMet (transport.cpp:420-433).
-    if (b.emission >= 0) {
+    if (b.emission >= 0 && !owned && counts) {
         const DEmission& e = sc.emissions[b.emission];
         if (fromFront ? e.front : e.back) {
             float w = 1.0f;
@@ -542,7 +579,7 @@ __device__ __forceinline__ bool shadeOne(ShadeCtx& c, const uint4 hraw, uint32_t
         rec->flags = kVpValid | (secondaryTerminated(c.wl) ? kVpTerminated : 0u) |
                      (rec->events == 0 && !rec->chromatic ? kVpFree : 0u);
     }
-    if (!o.delta) prepareDirect(c, p, true, &it, bi, hraw.z, hraw.w, n, 0.0f);
+    if (!o.delta && counts) prepareDirect(c, p, true, &it, bi, hraw.z, hraw.w, n, 0.0f);
     const float u3 = c.rng.uniform(), u2 = c.rng.uniform(), u1 = c.rng.uniform();
     SSample s;
     if (!sampleScatter(sc, it, c.d, u1, u2, u3, true, s) || s.weight <= 0.0f) return false;
@@ -557,6 +594,7 @@ __device__ __forceinline__ bool shadeOne(ShadeCtx& c, const uint4 hraw, uint32_t
     if (s.transmitted) c.region = other;
     if (s.delta) c.flags |= kFlagSpecular;
     else c.flags &= ~kFlagSpecular;
+    noteScatter(c.flags, s.delta);
     c.prevPdf = s.pdf;
     c.prevP = p;
     c.o = p;
@@ -2274,8 +2312,9 @@ int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render
         require(s.max_depth >= 0 && s.rr_depth >= 0, "depths must be non-negative");
         require(s.max_depth <= 4096, "max depth above 4096 is not supported (depth is kept in 16 bits)");
         if (s.integrator != OWE_INTEGRATOR_PATH && s.integrator != OWE_INTEGRATOR_LIGHT && s.integrator != OWE_INTEGRATOR_SPPM &&
-            s.integrator != OWE_INTEGRATOR_VCM && s.integrator != OWE_INTEGRATOR_BDPT)
-            fail(OWE_HIP_ERR_UNSUPPORTED, "unknown integrator (this backend implements path, light, sppm, vcm and bdpt)");
+            s.integrator != OWE_INTEGRATOR_VCM && s.integrator != OWE_INTEGRATOR_BDPT && s.integrator != OWE_INTEGRATOR_HYBRID)
+            fail(OWE_HIP_ERR_UNSUPPORTED,
+                 "unknown integrator (this backend implements path, light, sppm, vcm, bdpt and hybrid)");
         const bool vcmLike = s.integrator == OWE_INTEGRATOR_VCM || s.integrator == OWE_INTEGRATOR_BDPT;
         if (vcmLike) {
             require(s.photon_radius >= 0 && s.photon_alpha >= 0 && s.photons_per_pixel >= 0 && s.photon_groups >= 0,
@@ -2314,6 +2353,9 @@ int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render
                 if (m.flags & 2)
                     fail(OWE_HIP_ERR_UNSUPPORTED, "vcm and bdpt here do not sample scattering media (render with path)");
         }
+        if (s.integrator == OWE_INTEGRATOR_HYBRID && h.obs.kind != OWE_DETECTOR_OBSERVER)
+            fail(OWE_HIP_ERR_UNSUPPORTED,
+                 "the hybrid integrator needs a virtual observer; use path or light for surface sensors (renderer.hpp:34-35)");
         r->stackCap = std::max<uint32_t>(1, h.bvh.stackBound);
         OWE_HIP_CHECK(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
         DScene& d = r->dev;
@@ -2347,6 +2389,7 @@ int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render
         r->set.lambdaMin = float(s.lambda_min);
         r->set.lambdaMax = float(s.lambda_max);
         r->set.fresnelFloor = float(s.fresnel_floor);
+        r->set.mode = s.integrator == OWE_INTEGRATOR_HYBRID ? kModePartition : 0;
         // 1 / (NW · ∫ȳ) with ∫ȳ by composite Simpson over [360, 830] nm at 0.5 nm (core/spectrum.cpp:11-23).
         {
             auto cy = [](double l) {
@@ -2365,10 +2408,17 @@ int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render
         const uint64_t pixels = uint64_t(h.obs.width) * uint64_t(h.obs.height);
         r->film = devAlloc<unsigned long long>(r, 3 * pixels);
         OWE_HIP_CHECK(hipMemset(r->film, 0, 3 * pixels * sizeof(unsigned long long)));
-        if (r->integrator == OWE_INTEGRATOR_PATH) {
+        if (r->integrator == OWE_INTEGRATOR_PATH || r->integrator == OWE_INTEGRATOR_HYBRID) {
             // The wavefront form's path slots are allocated at its first pass (runPaths): a renderer
-            // that only runs the persistent form, or only as a batch member, never holds them.
+            // that only runs the persistent form, or only as a batch member, never holds them. The
+            // hybrid integrator's particles (same capacity per launch) splat to the particle film.
             r->capacity = uint32_t(std::min<uint64_t>(pixels * 64, 1u << 22));
+            if (r->integrator == OWE_INTEGRATOR_HYBRID) {
+                r->lightFilm = devAlloc<unsigned long long>(r, 3 * kSplatBins * pixels);
+                OWE_HIP_CHECK(hipMemset(r->lightFilm, 0, 3 * kSplatBins * pixels * sizeof(unsigned long long)));
+                r->particleStats = devAlloc<unsigned long long>(r, 4);
+                r->particleNext = devAlloc<unsigned long long>(r, 1);
+            }
             hipDeviceProp_t prop;
             OWE_HIP_CHECK(hipGetDeviceProperties(&prop, device));
             r->computeUnits = prop.multiProcessorCount;
@@ -2591,6 +2641,8 @@ int owe_hip_renderer_pass(owe_hip_renderer* r, int spp) {
             r->totalSpp += spp;
             return;
         }
+        // path, and hybrid's camera paths (its particles follow, GpuRenderer::pass,
+        // gpu_renderer.cpp:166-172).
         if (r->mode == OWE_HIP_MODE_PERSISTENT) {
             runPersistent(r, uint32_t(spp), uint32_t(r->passes));
         } else {
@@ -2601,6 +2653,7 @@ int owe_hip_renderer_pass(owe_hip_renderer* r, int spp) {
             }
         }
         OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
+        if (r->integrator == OWE_INTEGRATOR_HYBRID) runParticles(r, total, uint32_t(r->passes));
         r->passes++;
         r->totalSpp += spp;
     });
@@ -2726,7 +2779,8 @@ int owe_hip_renderer_camera_paths(owe_hip_renderer* r, int spp, int pass, const 
                                   uint64_t n, double* xyz) {
     return guard([&] {
         require(r != nullptr, "renderer is null");
-        require(r->integrator == OWE_INTEGRATOR_PATH, "camera paths need a path-integrator renderer");
+        require(r->integrator == OWE_INTEGRATOR_PATH || r->integrator == OWE_INTEGRATOR_HYBRID,
+                "camera paths need a path- or hybrid-integrator renderer");
         require(spp > 0 && pass >= 0, "spp must be positive and pass non-negative");
         if (n == 0) return;
         require(pixel != nullptr && sample != nullptr && xyz != nullptr, "null buffer");
@@ -2755,7 +2809,8 @@ int owe_hip_renderer_camera_paths(owe_hip_renderer* r, int spp, int pass, const 
 int owe_hip_renderer_particles(owe_hip_renderer* r, int pass, const uint64_t* index, uint64_t n, double* xyz) {
     return guard([&] {
         require(r != nullptr, "renderer is null");
-        require(r->integrator == OWE_INTEGRATOR_LIGHT, "particles need a light-integrator renderer");
+        require(r->integrator == OWE_INTEGRATOR_LIGHT || r->integrator == OWE_INTEGRATOR_HYBRID,
+                "particles need a light- or hybrid-integrator renderer");
         require(pass >= 0, "pass must be non-negative");
         if (n == 0) return;
         require(index != nullptr && xyz != nullptr, "null buffer");
@@ -2784,7 +2839,8 @@ int owe_hip_renderer_set_mode(owe_hip_renderer* r, int mode, int blocks_per_cu, 
     return guard([&] {
         require(r != nullptr, "renderer is null");
         require(mode == OWE_HIP_MODE_WAVEFRONT || mode == OWE_HIP_MODE_PERSISTENT, "unknown mode");
-        require(mode == OWE_HIP_MODE_WAVEFRONT || r->integrator == OWE_INTEGRATOR_PATH || r->integrator == OWE_INTEGRATOR_LIGHT,
+        require(mode == OWE_HIP_MODE_WAVEFRONT || r->integrator == OWE_INTEGRATOR_PATH || r->integrator == OWE_INTEGRATOR_LIGHT ||
+                    r->integrator == OWE_INTEGRATOR_HYBRID,
                 "the persistent forms trace camera paths or particles (not SPPM or VCM iterations)");
         require(blocks_per_cu >= 0 && blocks_per_cu <= 64, "blocks per compute unit must lie in [0, 64]");
         require(chunk >= 0 && chunk <= 1024, "chunk must lie in [0, 1024]");

@@ -13,9 +13,15 @@
 namespace owehip {
 
 // What every transport kernel is told of a render besides the scene.
+// Settings::mode bits: the hybrid integrator's partition (light tracing owns the paths eye → D →
+// S⁺ → light, camera paths every other; kFlagPartition, scene.slang:124), and camera paths that
+// count only light reaching them through a medium scattering vertex (kFlagMediumOnly, the path
+// pass beside vcm and bdpt in scenes whose media scatter, gpu_renderer.cpp:43-47, 171).
+constexpr int kModePartition = 1, kModeMediumOnly = 2;
+
 struct Settings {
     uint64_t seed;
-    int maxDepth, rrDepth, spectral, pad;
+    int maxDepth, rrDepth, spectral, mode;
     float lambdaMin, lambdaMax, fresnelFloor, pad2;
 };
 

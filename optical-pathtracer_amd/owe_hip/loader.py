@@ -196,16 +196,15 @@ class LoadedScene:
     def settings(self, integrator: str | None = None, seed: int | None = None, max_depth: int | None = None,
                  rr_depth: int = 6, spectral: int = S.SPECTRAL_HERO4) -> S.CRenderSettings:
         """The render block's settings for this backend. The integrator is the block's (or the one
-        given): path, light, sppm, vcm or bdpt (the photon estimators with the block's photon_radius,
-        photon_alpha and photons, scene_loader.cpp:110-121, and vcm's wavelength groups); hybrid is
-        an estimator this backend does not have — pass integrator="path" (the unbiased common limit)
-        to render such a scene."""
+        given): path, light, hybrid, sppm, vcm or bdpt (the reference's six, scene.hpp:19; the photon
+        estimators with the block's photon_radius, photon_alpha and photons, scene_loader.cpp:110-121,
+        and vcm's wavelength groups)."""
         name = integrator or self.render.get("integrator", "path")
         kinds = {"path": S.INTEGRATOR_PATH, "light": S.INTEGRATOR_LIGHT, "sppm": S.INTEGRATOR_SPPM,
-                 "vcm": S.INTEGRATOR_VCM, "bdpt": S.INTEGRATOR_BDPT}
+                 "vcm": S.INTEGRATOR_VCM, "bdpt": S.INTEGRATOR_BDPT, "hybrid": S.INTEGRATOR_HYBRID}
         if name not in kinds:
             raise SceneError(f"{self.source_path}: integrator '{name}' is not available in the hip backend "
-                             "(path, light, sppm, vcm or bdpt)")
+                             "(path, light, hybrid, sppm, vcm or bdpt)")
         radius = float(self.render.get("photon_radius", 2.0))
         alpha = float(self.render.get("photon_alpha", 2.0 / 3.0))
         photons = float(self.render.get("photons", 1.0))

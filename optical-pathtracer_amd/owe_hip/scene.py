@@ -27,7 +27,7 @@ SURF_NULL, SURF_DIELECTRIC, SURF_DIFFUSE, SURF_CONDUCTOR, SURF_MIRROR, SURF_ABSO
 SHAPE_MESH, SHAPE_PLANE, SHAPE_SPHERE, SHAPE_CYLINDER, SHAPE_SAG, SHAPE_WAVE = 0, 1, 2, 3, 4, 5
 APERTURE_DISK, APERTURE_RECT, APERTURE_ELLIPSE = 0, 1, 2
 SKY_NONE, SKY_UNIFORM = 0, 1
-INTEGRATOR_PATH, INTEGRATOR_LIGHT, INTEGRATOR_SPPM, INTEGRATOR_VCM, INTEGRATOR_BDPT = 0, 1, 2, 3, 4
+INTEGRATOR_PATH, INTEGRATOR_LIGHT, INTEGRATOR_SPPM, INTEGRATOR_VCM, INTEGRATOR_BDPT, INTEGRATOR_HYBRID = 0, 1, 2, 3, 4, 5
 SPECTRAL_HERO4, SPECTRAL_UNIFORM1 = 0, 1
 DETECTOR_OBSERVER, DETECTOR_SENSOR = 0, 1
 
@@ -530,9 +530,12 @@ def dispersion_prism_scene(width=256, height=256, sun_elevation=math.radians(30.
     return sc
 
 
-def white_furnace_scene(rho: float, size: int = 16) -> Scene:
+def white_furnace_scene(rho: float, size: int = 16, glass: float = 0.0) -> Scene:
     """A closed diffuse sphere of albedo rho emitting unit radiance inward: L = 1 / (1 − ρ)
-    (tests/test_transport.cpp:83-114)."""
+    (tests/test_transport.cpp:83-114). With glass > 0 a lossless dielectric ball of that radius
+    (n = 1.5) sits inside, behind the eye: a uniform radiance field stays uniform through it, so the
+    eye still sees 1 / (1 − ρ) everywhere, part of it light that crossed the glass on its way to the
+    wall the eye sees."""
     sc = Scene()
     air = sc.add_medium(Medium.air())
     amb = sc.add_region(air)
@@ -540,6 +543,9 @@ def white_furnace_scene(rho: float, size: int = 16) -> Scene:
     shell = sc.add_optics(Optics(SURF_DIFFUSE, reflectance=Spectrum.constant(rho)))
     e = sc.add_emission(Emission(Spectrum.constant(1.0), front=False, back=True))
     sc.add_sphere(1.0, transform(), amb, inside, shell, e)
+    if glass > 0:
+        ball = sc.add_region(sc.add_medium(Medium.constant(1.5)))
+        sc.add_sphere(glass, transform(None, (-0.45, 0.0, 0.0)), inside, ball, sc.add_optics(Optics(SURF_DIELECTRIC)))
     sc.set_observer(size, size, position=(0.2, 0.1, -0.3), look_at=(1, 0.3, 0), fov_y=math.radians(70), region=inside)
     return sc
 

@@ -27,6 +27,7 @@ Settings toSettings(const owe_render_settings& r) {
     s.photonAlpha = r.photon_alpha > 0 ? r.photon_alpha : 2.0 / 3.0;
     s.photonsPerPixel = r.photons_per_pixel > 0 ? r.photons_per_pixel : 1.0;
     s.photonGroups = r.photon_groups > 0 ? r.photon_groups : 4;
+    s.partition = r.integrator == OWE_INTEGRATOR_HYBRID;
     return s;
 }
 // The stream of sample s of a pixel in the per-sample layout.

@@ -259,6 +259,11 @@ struct Settings {
     // SPPM (RenderSettings, scene/scene.hpp:27-35).
     double photonRadius = 2, photonAlpha = 2.0 / 3.0, photonsPerPixel = 1;
     int photonGroups = 4;
+    // The hybrid integrator's partition (kFlagPartition, scene.slang:124): light tracing owns the
+    // paths eye → D → S⁺ → light, camera paths every other; and camera paths that count only light
+    // reaching the eye through a medium scattering vertex (kFlagMediumOnly: the path-traced part of
+    // vcm and bdpt in scenes whose media scatter, gpu_renderer.cpp:43-47).
+    bool partition = false, mediumOnly = false;
 };
 
 // SPPM photon guiding (particle.slang:149-183): a map of kRes² cells over the sun's emission disc's

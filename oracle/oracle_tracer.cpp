@@ -201,12 +201,23 @@ struct Tracer::Path {
     // SPPM camera paths (transport.slang:749-762): where the visible point goes, and what decides it.
     VisiblePoint* vp = nullptr;
     int scatterEvents = 0, nonDeltaEvents = 0;
+    // The hybrid partition (transport.slang:745-747): delta events since the last non-delta one,
+    // whether the first event was non-delta; and whether a medium scattering vertex was met.
+    int deltaSinceNonDelta = 0;
+    bool firstIsNonDelta = false, mediumScattered = false;
+    bool counts = true;  // this step's light counts (medium-only paths: after a medium scattering vertex)
     bool chromatic = false;  // an absorbing, non-scattering medium on the way
     double pathLength = 0;
     bool vpDone() const { return vp && vp->valid; }
     // noteScatter (transport.slang:783-791).
     void noteScatter(bool delta) {
-        if (!delta) ++nonDeltaEvents;
+        if (delta) {
+            ++deltaSinceNonDelta;
+        } else {
+            if (scatterEvents == 0) firstIsNonDelta = true;
+            ++nonDeltaEvents;
+            deltaSinceNonDelta = 0;
+        }
         ++scatterEvents;
     }
     // Russian roulette against the throughput (transport.cpp:369-374).
@@ -364,7 +375,7 @@ bool Tracer::scatterAtSurface(const Hit& hit, const Optics& o, uint32_t other, i
         v.free = path.scatterEvents == 0 && !path.chromatic;
         v.pathLength = path.pathLength;
     }
-    if (!o.isDelta()) {
+    if (!o.isDelta() && path.counts) {
         const Spec4 dl = directLighting(hit, true, &o, it.nFront, it.nBack, path.ray.d, wl, rng, 0, path.region);
         if (!dl.isZero()) path.L += path.beta * dl;
     }
@@ -406,6 +417,7 @@ Spec4 Tracer::radiance(Ray ray, uint32_t region, Wavelengths& wl, Pcg& rng, Stat
             const V3 p = path.ray.o + path.ray.d * t;
             if (path.vpDone()) break;
             if (depth >= s_.maxDepth) break;
+            path.mediumScattered = true;
             Hit mh;
             mh.p = p;
             const Spec4 dl = directLighting(mh, false, nullptr, 1, 1, path.ray.d, wl, rng, med.g, path.region);
@@ -422,8 +434,14 @@ Spec4 Tracer::radiance(Ray ray, uint32_t region, Wavelengths& wl, Pcg& rng, Stat
             continue;
         }
         if (path.vp && med.absorbs() && !med.scatters()) path.chromatic = true;
+        // Light this path may count (stepPath, transport.slang:868-891): under the hybrid partition
+        // not the paths light tracing owns; medium-only, only after a medium scattering vertex.
+        const bool owned = s_.partition && path.nonDeltaEvents == 1 && path.firstIsNonDelta && path.deltaSinceNonDelta >= 1;
+        const bool counts = !s_.mediumOnly || path.mediumScattered;
+        path.counts = counts;
         if (!found) {
             if (path.region == 0) {
+                if (owned || !counts) break;
                 // skyMet (transport.cpp:401-417): the uniform sky, and the sun weighted against NEE.
                 Spec4 sky, sun;
                 for (int i = 0; i < NW; ++i) {
@@ -451,7 +469,7 @@ Spec4 Tracer::radiance(Ray ray, uint32_t region, Wavelengths& wl, Pcg& rng, Stat
         if (path.vp && !path.vpDone()) path.pathLength += hit.t;
         const Optics& o = w_.optics[size_t(b.optics)];
         // emissionMet (transport.cpp:420-433).
-        if (b.emission >= 0) {
+        if (b.emission >= 0 && !owned && counts) {
             const Emission& e = w_.emissions[size_t(b.emission)];
             if (fromFront ? e.front : e.back) {
                 double w = 1;
@@ -765,10 +783,13 @@ uint32_t Tracer::particleWalk(const Wavelengths* fixed, Pcg& rng, Stats& st, Spl
     const Observer& ob = w_.observer;
     const bool observer = !ob.sensor() && !sink;
     const bool sensor = ob.sensor() && !sink;
-    int events = 0;
+    // The hybrid partition (particle.slang:358-417): a particle connects to the eye only at its
+    // first non-specular vertex and only after at least one specular event, and ends there.
+    const bool partition = s_.partition && !sink;
+    int events = 0, nonDelta = 0, deltas = 0;
     st.paths++;
     Emitted em;
-    if (!emitParticle(wl, rng, observer, em, out, guide)) return kNone;
+    if (!emitParticle(wl, rng, observer && !partition, em, out, guide)) return kNone;
     Ray ray = em.ray;
     uint32_t region = em.region;
     Spec4 beta = em.beta;
@@ -780,10 +801,13 @@ uint32_t Tracer::particleWalk(const Wavelengths* fixed, Pcg& rng, Stats& st, Spl
         double t;
         const MediumEvent ev = sampleMedium(med, wl, found ? hit.t : kInf, rng, beta, t);
         if (ev == MediumEvent::End) break;
+        const bool mayConnect = !partition || (nonDelta == 0 && deltas >= 1);
         if (ev == MediumEvent::Scatter) {
             Hit mh;
             mh.p = ray.o + ray.d * t;
-            if (observer) connectVertex(mh, false, nullptr, ray.d, med.g, region, beta, wl, rng, out);
+            if (observer && mayConnect) connectVertex(mh, false, nullptr, ray.d, med.g, region, beta, wl, rng, out);
+            ++nonDelta;
+            if (partition) break;
             if (depth >= s_.maxDepth) break;
             double u1, u2;
             rng.draw2(u1, u2);
@@ -830,8 +854,12 @@ uint32_t Tracer::particleWalk(const Wavelengths* fixed, Pcg& rng, Stats& st, Spl
                 deposit(out, std::min(int(fx), ob.width - 1), std::min(int(fy), ob.height - 1), beta * (1 / area), wl);
         }
         if (o.type == OWE_SURF_DETECTOR || o.type == OWE_SURF_ABSORBER) break;
-        if (!o.isDelta() && observer) connectVertex(hit, true, &o, ray.d, 0, region, beta, wl, rng, out);
+        if (!o.isDelta() && observer && mayConnect) connectVertex(hit, true, &o, ray.d, 0, region, beta, wl, rng, out);
         if (!o.isDelta() && sink && events >= 1) sink->gather(hit, ray.d, beta, wl);
+        if (!o.isDelta()) {
+            ++nonDelta;
+            if (partition) break;
+        }
         if (depth >= s_.maxDepth) break;
         // terminateIfDispersive, then the importance-mode scatter (transport.slang:639-684).
         Interface it{&o, hit.n, hit.p, w_.indexOf(b.front, wl.lambda[0]), w_.indexOf(b.back, wl.lambda[0]),
@@ -860,6 +888,7 @@ uint32_t Tracer::particleWalk(const Wavelengths* fixed, Pcg& rng, Stats& st, Spl
         if (s.transmitted) region = other;
         ray = w_.spawn(hit, s.wi);
         ++events;
+        if (s.delta) ++deltas;
         // Particles carry flux: roulette against the event's own throughput.
         if (depth >= s_.rrDepth && s.weight < 1) {
             if (rng.uniform() >= s.weight) break;

@@ -33,5 +33,5 @@ def test_backend_integrators_from_the_library():
     from owe_hip import _lib
 
     lib = _lib.load()
-    assert lib.owe_hip_backend_integrators().decode() == "path,light,sppm,vcm,bdpt"
+    assert lib.owe_hip_backend_integrators().decode() == "path,light,sppm,vcm,bdpt,hybrid"
     assert lib.owe_hip_backend_name().decode() == "hip"

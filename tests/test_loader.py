@@ -231,11 +231,14 @@ def test_what_the_backend_cannot_render_is_refused(text, msg):
 
 def test_integrators_of_other_estimators_are_named():
     ls = LD.load_scene_text("observer E { position = (0, -1, 0)  look_at = (0, 0, 0) }\n"
-                            "render { integrator = hybrid  seed = 9  max_depth = 12 }")
-    with pytest.raises(F.SceneError, match="integrator 'hybrid'"):
+                            "render { integrator = mlt  seed = 9  max_depth = 12 }")
+    with pytest.raises(F.SceneError, match="integrator 'mlt'"):
         ls.settings()
     st = ls.settings(integrator="path")
     assert st.seed == 9 and st.max_depth == 12 and st.integrator == S.INTEGRATOR_PATH
+    st = LD.load_scene_text("observer E { position = (0, -1, 0)  look_at = (0, 0, 0) }\n"
+                            "render { integrator = hybrid }").settings()
+    assert st.integrator == S.INTEGRATOR_HYBRID
 
 
 def test_vcm_and_bdpt_settings_come_from_the_render_block():
