@@ -127,7 +127,9 @@ typedef struct owe_hip_renderer owe_hip_renderer;
  * GpuRenderer::runVcmIterations, gpu_renderer_passes.cpp:136-199: a pass of spp runs spp iterations,
  * each photons_per_pixel·W·H light subpaths cached and splatted to the eye and one camera path per
  * pixel; resolve gives the camera paths' sum / iterations + the splats / (light subpaths ·
- * iterations); scenes with scattering media or a surface sensor are refused) and
+ * iterations; with scattering media the bidirectional paths take media as extinction and a camera
+ * pass of spp paths per pixel adds the light that reached the eye through a medium scattering
+ * vertex, gpu_renderer.cpp:43-47; a surface sensor is refused) and
  * OWE_INTEGRATOR_HYBRID (a pass of spp runs the camera paths, then W·H·spp particles, with the
  * partition of particle.slang:358-417 and transport.slang:745-906: light tracing owns the paths
  * eye → D → S⁺ → light, camera paths every other; resolve adds both films; observers only). Fails

@@ -799,10 +799,13 @@ __device__ __forceinline__ void vcmLightPath(PCtx& c, const VcmDev& v, uint32_t 
         }
         c.segments++;
         const DMedium& med = sc.media[sc.regionMedium[ray.region]];
-        if (med.flags & 1) {  // attenuation only (scattering media are refused)
+        if (med.flags) {  // extinction only: light scattered in a medium is the medium-only camera pass's
             if (!found) return;
 #pragma unroll
-            for (int i = 0; i < NW; ++i) c.beta[i] *= expf(-evalSpec(med.absorption, sc.table, c.wl.lambda[i]) * h.t);
+            for (int i = 0; i < NW; ++i) {
+                const float l = c.wl.lambda[i];
+                c.beta[i] *= expf(-(evalSpec(med.absorption, sc.table, l) + evalSpec(med.scattering, sc.table, l)) * h.t);
+            }
         }
         if (!found) {
             if (ray.region != 0) c.leaks++;

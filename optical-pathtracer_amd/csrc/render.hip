@@ -1618,10 +1618,13 @@ __global__ __launch_bounds__(kBlock, OWE_HIP_VCM_WAVES) void vcmCameraKernel(DSc
                 ++nSegments;
                 if (found && !(isfinite(h.p.x + h.p.y + h.p.z) && isfinite(h.n.x + h.n.y + h.n.z))) break;
                 const DMedium& med = sc.media[sc.regionMedium[region]];
-                if (med.flags & 1) {  // attenuation only (scattering media are refused)
+                if (med.flags) {  // extinction only: light scattered in a medium is the medium-only pass's
                     if (!found) break;
 #pragma unroll
-                    for (int i = 0; i < NW; ++i) c.beta[i] *= expf(-evalSpec(med.absorption, sc.table, c.wl.lambda[i]) * h.t);
+                    for (int i = 0; i < NW; ++i) {
+                        const float l = c.wl.lambda[i];
+                        c.beta[i] *= expf(-(evalSpec(med.absorption, sc.table, l) + evalSpec(med.scattering, sc.table, l)) * h.t);
+                    }
                 }
                 if (!found) {
                     if (region == 0) {
@@ -1819,6 +1822,7 @@ struct owe_hip_renderer {
     float vcmAlpha = 2.0f / 3.0f;
     uint64_t vcmIterations = 0;
     uint32_t vcmSlotsMax = 1, vcmInFlight = 1;  // iterations the caches hold, and run per launch
+    bool vcmMediumPass = false;  // scattering media: a medium-only camera pass beside each vcm pass
     hipStream_t stream = nullptr;
     uint32_t stackCap = 16;
     bool simpleShadows = true;
@@ -2349,9 +2353,10 @@ int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render
         if (vcmLike) {
             if (h.obs.kind != OWE_DETECTOR_OBSERVER)
                 fail(OWE_HIP_ERR_UNSUPPORTED, "vcm and bdpt render for an ideal observer (the reference's GPU backend too)");
-            for (const DMedium& m : h.media)
-                if (m.flags & 2)
-                    fail(OWE_HIP_ERR_UNSUPPORTED, "vcm and bdpt here do not sample scattering media (render with path)");
+            // With scattering media, vcm and bdpt take the paths without a medium scattering vertex
+            // (media as extinction) and camera paths, in a pass of their own, all the others
+            // (gpu_renderer.cpp:43-47, 171).
+            for (const DMedium& m : h.media) r->vcmMediumPass = r->vcmMediumPass || (m.flags & 2) != 0;
         }
         if (s.integrator == OWE_INTEGRATOR_HYBRID && h.obs.kind != OWE_DETECTOR_OBSERVER)
             fail(OWE_HIP_ERR_UNSUPPORTED,
@@ -2389,7 +2394,7 @@ int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render
         r->set.lambdaMin = float(s.lambda_min);
         r->set.lambdaMax = float(s.lambda_max);
         r->set.fresnelFloor = float(s.fresnel_floor);
-        r->set.mode = s.integrator == OWE_INTEGRATOR_HYBRID ? kModePartition : 0;
+        r->set.mode = s.integrator == OWE_INTEGRATOR_HYBRID ? kModePartition : r->vcmMediumPass ? kModeMediumOnly : 0;
         // 1 / (NW · ∫ȳ) with ∫ȳ by composite Simpson over [360, 830] nm at 0.5 nm (core/spectrum.cpp:11-23).
         {
             auto cy = [](double l) {
@@ -2543,6 +2548,7 @@ int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render
             r->lightFilm = devAlloc<unsigned long long>(r, 3 * kSplatBins * pixels);
             OWE_HIP_CHECK(hipMemset(r->lightFilm, 0, 3 * kSplatBins * pixels * sizeof(unsigned long long)));
             r->particleStats = devAlloc<unsigned long long>(r, 4);
+            if (r->vcmMediumPass) r->capacity = uint32_t(std::min<uint64_t>(pixels * 64, 1u << 22));  // paths per launch
         } else {
             r->capacity = uint32_t(std::min<uint64_t>(pixels * 64, 1u << 22));  // particles per launch
             r->lightFilm = devAlloc<unsigned long long>(r, 3 * kSplatBins * pixels);
@@ -2631,6 +2637,15 @@ int owe_hip_renderer_pass(owe_hip_renderer* r, int spp) {
         }
         if (r->integrator == OWE_INTEGRATOR_VCM || r->integrator == OWE_INTEGRATOR_BDPT) {  // spp iterations
             runVcm(r, spp);
+            if (r->vcmMediumPass) {
+                // The medium-only camera pass, spp paths per pixel (GpuRenderer::pass,
+                // gpu_renderer.cpp:171), into the same camera film.
+                for (uint64_t base = 0; base < total; base += r->capacity) {
+                    const uint32_t count = uint32_t(std::min<uint64_t>(r->capacity, total - base));
+                    runPaths(r, base, count, uint32_t(spp), uint32_t(r->passes), false, nullptr);
+                }
+                OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
+            }
             r->passes++;
             r->totalSpp += spp;
             return;

@@ -11,7 +11,8 @@
 // one vertex drawn uniformly from its group's cache (weighted by the cache's size over N_L) and, with
 // merging, every cached vertex of its group within the merging radius; emission it meets is weighted
 // too. Every strategy is weighted by the balance heuristic over all the others. Participating media
-// only attenuate here (a scene whose media scatter is refused).
+// only attenuate the subpaths here (by their extinction σa + σs): in a scene whose media scatter,
+// a camera pass of its own adds the light that reached the eye through a medium scattering vertex.
 #pragma once
 
 #include <hip/hip_runtime.h>

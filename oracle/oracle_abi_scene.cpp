@@ -28,6 +28,9 @@ Settings toSettings(const owe_render_settings& r) {
     s.photonsPerPixel = r.photons_per_pixel > 0 ? r.photons_per_pixel : 1.0;
     s.photonGroups = r.photon_groups > 0 ? r.photon_groups : 4;
     s.partition = r.integrator == OWE_INTEGRATOR_HYBRID;
+    // Camera paths rendered under vcm or bdpt settings are the medium-only pass that accompanies
+    // those integrators (light that reached the eye through a medium scattering vertex).
+    s.mediumOnly = r.integrator == OWE_INTEGRATOR_VCM || r.integrator == OWE_INTEGRATOR_BDPT;
     return s;
 }
 // The stream of sample s of a pixel in the per-sample layout.

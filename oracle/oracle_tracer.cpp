@@ -1044,9 +1044,10 @@ void Tracer::vcmLight(Wavelengths wl, Pcg& rng, Stats& st, Splat& out, const Vcm
         const bool found = w_.intersect(ray, kInf, hit, region);
         st.segments++;
         const Medium& med = w_.mediumOf(region);
-        if (med.absorbs()) {  // attenuation only: the backend refuses media that scatter
+        if (med.absorbs() || med.scatters()) {  // extinction only: scattered light is the medium-only pass's
             if (!found) return;
-            for (int i = 0; i < NW; ++i) beta[i] *= std::exp(-med.absorption.eval(wl.lambda[i]) * hit.t);
+            const Spec4 sT = sigmaT(med, wl);
+            for (int i = 0; i < NW; ++i) beta[i] *= std::exp(-sT[i] * hit.t);
         }
         if (!found) {
             if (region != 0) st.leaks++;
@@ -1196,9 +1197,10 @@ bool Tracer::vcmCamera(int x, int y, Wavelengths wl, Pcg& rng, Stats& st, const 
         const bool found = w_.intersect(ray, kInf, hit, region);
         st.segments++;
         const Medium& med = w_.mediumOf(region);
-        if (med.absorbs()) {
+        if (med.absorbs() || med.scatters()) {
             if (!found) break;
-            for (int i = 0; i < NW; ++i) beta[i] *= std::exp(-med.absorption.eval(wl.lambda[i]) * hit.t);
+            const Spec4 sT = sigmaT(med, wl);
+            for (int i = 0; i < NW; ++i) beta[i] *= std::exp(-sT[i] * hit.t);
         }
         if (!found) {
             if (region == 0) {

@@ -76,3 +76,20 @@ def test_cache_holds_one_vertex_per_non_specular_event(oracle):
     w = World(oracle, S.white_furnace_scene(0.0, size=8))
     r = w.vcm(S.settings(seed=5, integrator=S.INTEGRATOR_VCM), 3, merging=True)
     assert list(r["cached"]) == [r["paths"]] * 3
+
+
+@pytest.mark.parametrize("integrator", [S.INTEGRATOR_VCM, S.INTEGRATOR_BDPT])
+def test_scattering_furnace(oracle, integrator):
+    """With a medium that scatters, vcm and bdpt take media as extinction and a camera pass counts
+    only light that reached the eye through a medium scattering vertex (gpu_renderer.cpp:43-47):
+    together they keep the scattering furnace at 1/(1 − ρ); each alone does not."""
+    rho = 0.5
+    w = World(oracle, S.scattering_furnace_scene(1.5, 0.0, 0.5, rho, size=8))
+    st = S.settings(seed=8, integrator=integrator, max_depth=200)
+    its = 512
+    bidir = w.vcm(st, its, merging=integrator == S.INTEGRATOR_VCM)["image"][..., 1].mean()
+    medium, mst = w.render(st, its, layout=1)  # vcm settings: the medium-only camera pass
+    total = bidir + medium[..., 1].mean()
+    print(f"bidirectional {bidir:.4f} + medium pass {medium[..., 1].mean():.4f} = {total:.4f}")
+    assert total == pytest.approx(1 / (1 - rho), rel=0.015)
+    assert 0.1 < medium[..., 1].mean() < 0.9 * total
