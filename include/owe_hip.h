@@ -148,6 +148,18 @@ int owe_hip_renderer_sppm_state(owe_hip_renderer* renderer, double* state, uint6
                                 long long* iterations);
 /* Renderer::stats: paths, segments, inconsistencies, leaks (transport_stats.hpp:12-31). */
 int owe_hip_renderer_stats(const owe_hip_renderer* renderer, owe_transport_stats* out);
+/* TransportStats::first (transport_stats.hpp:15-23): the first region inconsistency since creation or
+ * reset — a ray reached `boundary` from `ray_region` where the boundary's side holds
+ * `boundary_region`; `previous` is the boundary it left; p, origin and direction are in world
+ * coordinates; `primitive` is the mesh triangle met (its input index); previous_primitive is not
+ * tracked. With several kernels in flight "first" is whichever claimed it first (as in the reference's
+ * GPU backend). All fields are 0xFFFFFFFF / 0 and *found is 0 when there was none. */
+typedef struct owe_inconsistency {
+    uint32_t boundary, ray_region, boundary_region, previous;
+    double p[3], origin[3], direction[3];
+    uint32_t previous_primitive, primitive;
+} owe_inconsistency;
+int owe_hip_renderer_first_inconsistency(const owe_hip_renderer* renderer, owe_inconsistency* out, int* found);
 /* Renderer::passes() and samplesPerPixel() (renderer.hpp:48-49). */
 int owe_hip_renderer_progress(const owe_hip_renderer* renderer, int* passes, long long* samples_per_pixel);
 /* Renderer::resetProgress (renderer.hpp:63-69): clears the film, the pass count and the statistics. */

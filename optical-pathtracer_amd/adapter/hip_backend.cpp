@@ -354,6 +354,16 @@ private:
         stats_.segments = t.segments;
         stats_.inconsistencies = t.inconsistencies;
         stats_.leaks = t.leaks;
+        owe_inconsistency f{};
+        int found = 0;
+        check(owe_hip_renderer_first_inconsistency(r_, &f, &found));
+        if (found) {
+            auto none = [](uint32_t v) { return v == 0xFFFFFFFFu ? kNone : v; };
+            stats_.first = {f.boundary, none(f.ray_region), none(f.boundary_region), none(f.previous),
+                            Vec3(f.p[0], f.p[1], f.p[2]), Vec3(f.origin[0], f.origin[1], f.origin[2]),
+                            Vec3(f.direction[0], f.direction[1], f.direction[2]), none(f.previous_primitive),
+                            none(f.primitive)};
+        }
     }
 
     HipFlatScene flat_;

@@ -520,6 +520,7 @@ __device__ __forceinline__ bool particleSegment(PCtx& c, PState& st, const SppmD
         const uint32_t inc = fromFront ? b.front : b.back, other = fromFront ? b.back : b.front;
         if (inc != ray.region) {
             c.inconsistencies++;
+            noteInconsistency(sc, bi, ray.region, inc, ray.departed, h.p, ray.o, ray.d, h.slot);
             ray.region = inc;
         }
         const DOptics& o = sc.optics[b.optics];
@@ -817,6 +818,7 @@ __device__ __forceinline__ void vcmLightPath(PCtx& c, const VcmDev& v, uint32_t 
         const uint32_t inc = fromFront ? b.front : b.back, other = fromFront ? b.back : b.front;
         if (inc != ray.region) {
             c.inconsistencies++;
+            noteInconsistency(sc, bi, ray.region, inc, ray.departed, h.p, ray.o, ray.d, h.slot);
             ray.region = inc;
         }
         segment += h.t;

@@ -514,6 +514,7 @@ __device__ __forceinline__ bool shadeOne(ShadeCtx& c, const uint4 hraw, uint32_t
     const uint32_t inc = fromFront ? b.front : b.back, other = fromFront ? b.back : b.front;
     if (inc != c.region) {
         atomicAdd(&counters[2], 1u);
+        noteInconsistency(sc, bi, c.region, inc, c.departed, p, c.o, c.d, hraw.w);
         c.region = inc;
     }
     if (kSppm && !rec->valid) rec->pathLength += t;
@@ -1662,6 +1663,7 @@ __global__ __launch_bounds__(kBlock, OWE_HIP_VCM_WAVES) void vcmCameraKernel(DSc
                 const uint32_t inc = fromFront ? b.front : b.back, other = fromFront ? b.back : b.front;
                 if (inc != region) {
                     ++nInconsistent;
+                    noteInconsistency(sc, bi, region, inc, departed, h.p, o, d, h.slot);
                     region = inc;
                 }
                 segment += h.t;
@@ -1823,6 +1825,7 @@ struct owe_hip_renderer {
     uint64_t vcmIterations = 0;
     uint32_t vcmSlotsMax = 1, vcmInFlight = 1;  // iterations the caches hold, and run per launch
     bool vcmMediumPass = false;  // scattering media: a medium-only camera pass beside each vcm pass
+    uint32_t* firstInconsistency = nullptr;  // the first region inconsistency's record (kFirstWords words)
     hipStream_t stream = nullptr;
     uint32_t stackCap = 16;
     bool simpleShadows = true;
@@ -2381,6 +2384,9 @@ int owe_hip_renderer_create(int device, const owe_scene* scene, const owe_render
         d.nRegions = uint32_t(h.regionMedium.size());
         d.stackCap = r->stackCap;
         d.contacts = h.contacts ? 1u : 0u;
+        r->firstInconsistency = devAlloc<uint32_t>(r, kFirstWords);
+        OWE_HIP_CHECK(hipMemset(r->firstInconsistency, 0, kFirstWords * sizeof(uint32_t)));
+        d.firstInconsistency = r->firstInconsistency;
         d.env = h.env;
         d.obs = h.obs;
         bool anyNull = false, anyMedium = false;
@@ -2754,6 +2760,33 @@ int owe_hip_renderer_stats(const owe_hip_renderer* r, owe_transport_stats* out) 
     });
 }
 
+int owe_hip_renderer_first_inconsistency(const owe_hip_renderer* r, owe_inconsistency* out, int* found) {
+    return guard([&] {
+        require(r != nullptr && out != nullptr && found != nullptr, "null argument");
+        OWE_HIP_CHECK(hipSetDevice(r->device));
+        uint32_t w[kFirstWords];
+        OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
+        OWE_HIP_CHECK(hipMemcpy(w, r->firstInconsistency, sizeof w, hipMemcpyDeviceToHost));
+        *out = owe_inconsistency{};
+        out->boundary = out->ray_region = out->boundary_region = out->previous = 0xFFFFFFFFu;
+        out->previous_primitive = out->primitive = 0xFFFFFFFFu;
+        *found = w[0] != 0u ? 1 : 0;
+        if (!*found) return;
+        auto none = [](uint32_t v) { return v == kNoBoundary || v == kRegionNone ? 0xFFFFFFFFu : v; };
+        out->boundary = w[1];
+        out->ray_region = none(w[2]);
+        out->boundary_region = none(w[3]);
+        out->previous = none(w[4]);
+        for (int k = 0; k < 3; ++k) {  // camera-relative floats → world coordinates
+            out->p[k] = double(__builtin_bit_cast(float, w[5 + k])) + r->host.origin[k];
+            out->origin[k] = double(__builtin_bit_cast(float, w[8 + k])) + r->host.origin[k];
+            out->direction[k] = double(__builtin_bit_cast(float, w[11 + k]));
+        }
+        out->previous_primitive = w[14];
+        out->primitive = w[15];
+    });
+}
+
 int owe_hip_renderer_progress(const owe_hip_renderer* r, int* passes, long long* samples_per_pixel) {
     return guard([&] {
         require(r != nullptr, "renderer is null");
@@ -2768,6 +2801,7 @@ int owe_hip_renderer_reset(owe_hip_renderer* r) {
         OWE_HIP_CHECK(hipSetDevice(r->device));
         const size_t n = 3 * size_t(r->host.obs.width) * size_t(r->host.obs.height);
         OWE_HIP_CHECK(hipMemsetAsync(r->film, 0, n * sizeof(unsigned long long), r->stream));
+        OWE_HIP_CHECK(hipMemsetAsync(r->firstInconsistency, 0, kFirstWords * sizeof(uint32_t), r->stream));
         if (r->lightFilm)
             OWE_HIP_CHECK(hipMemsetAsync(r->lightFilm, 0, n * kSplatBins * sizeof(unsigned long long), r->stream));
         if (r->integrator == OWE_INTEGRATOR_SPPM) {

@@ -131,6 +131,7 @@ struct DScene {
     const TriRecord* tris;
     uint32_t nBoundaries, nLights, nRegions, stackCap;
     uint32_t contacts;  // two distinct boundaries touch or overlap: rays resolve contacts (intersectContact)
+    uint32_t* firstInconsistency;  // the first region inconsistency's record (kFirstWords words), or null
     DEnv env;
     DObserver obs;
 };

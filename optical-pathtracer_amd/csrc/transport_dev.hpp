@@ -540,6 +540,29 @@ __device__ __forceinline__ bool intersectWorld(const DScene& sc, f3 o, f3 d, flo
     return true;
 }
 
+// The first region inconsistency of a renderer (TransportStats::first, scene/transport_stats.hpp:12-31;
+// noteInconsistency, transport.slang:579-597): word 0 claims the record; 1 the boundary met, 2 the
+// ray's region, 3 the boundary's region on the side met, 4 the boundary the ray left, 5–7 the
+// camera-relative point (float bits), 8–10 the segment's origin, 11–13 its direction, 14 the
+// primitive it left (not tracked: ~0), 15 the one it met (the mesh triangle's input index, ~0 for an
+// analytic boundary).
+constexpr int kFirstWords = 16;
+__device__ __forceinline__ void noteInconsistency(const DScene& sc, uint32_t boundary, uint32_t rayRegion,
+                                                  uint32_t boundaryRegion, uint32_t departed, f3 p, f3 o, f3 d,
+                                                  uint32_t slot) {
+    uint32_t* w = sc.firstInconsistency;
+    if (!w || atomicCAS(w, 0u, 1u) != 0u) return;
+    w[1] = boundary;
+    w[2] = rayRegion;
+    w[3] = boundaryRegion;
+    w[4] = departed;
+    const float v[9] = {p.x, p.y, p.z, o.x, o.y, o.z, d.x, d.y, d.z};
+#pragma unroll
+    for (int k = 0; k < 9; ++k) w[5 + k] = __float_as_uint(v[k]);
+    w[14] = ~0u;
+    w[15] = slot != kEmptyChild ? sc.tris[slot].prim : ~0u;
+}
+
 // Contact tolerance at a point (camera-relative coordinates): 2^-21 of its largest coordinate, two to
 // four float ulps.
 __device__ __forceinline__ float contactTol(f3 p) {

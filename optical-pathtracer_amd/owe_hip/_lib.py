@@ -99,6 +99,7 @@ def load() -> ctypes.CDLL:
     _declare(lib, "owe_hip_renderer_pass", c_int, [c_vp, c_int])
     _declare(lib, "owe_hip_renderer_resolve", c_int, [c_vp, c_vp])
     _declare(lib, "owe_hip_renderer_stats", c_int, [c_vp, c_vp])
+    _declare(lib, "owe_hip_renderer_first_inconsistency", c_int, [c_vp, c_vp, c_vp])
     _declare(lib, "owe_hip_renderer_sppm_state", c_int, [c_vp, c_vp, c_vp, c_vp])
     _declare(lib, "owe_hip_renderer_progress", c_int, [c_vp, ctypes.POINTER(c_int), ctypes.POINTER(ctypes.c_longlong)])
     _declare(lib, "owe_hip_renderer_reset", c_int, [c_vp])

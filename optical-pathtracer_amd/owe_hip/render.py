@@ -69,6 +69,22 @@ class Renderer:
         _lib.check(_lib.load().owe_hip_renderer_stats(self._h, ctypes.byref(s)))
         return {"paths": s.paths, "segments": s.segments, "inconsistencies": s.inconsistencies, "leaks": s.leaks}
 
+    def first_inconsistency(self) -> dict | None:
+        """TransportStats::first: the first region inconsistency (boundary, ray_region, boundary_region,
+        previous, p, origin, direction, primitive; world coordinates), or None."""
+        class _Rec(ctypes.Structure):
+            _fields_ = [("boundary", ctypes.c_uint32), ("ray_region", ctypes.c_uint32),
+                        ("boundary_region", ctypes.c_uint32), ("previous", ctypes.c_uint32),
+                        ("p", ctypes.c_double * 3), ("origin", ctypes.c_double * 3), ("direction", ctypes.c_double * 3),
+                        ("previous_primitive", ctypes.c_uint32), ("primitive", ctypes.c_uint32)]
+        rec, found = _Rec(), ctypes.c_int()
+        _lib.check(_lib.load().owe_hip_renderer_first_inconsistency(self._h, ctypes.byref(rec), ctypes.byref(found)))
+        if not found.value:
+            return None
+        return dict(boundary=rec.boundary, ray_region=rec.ray_region, boundary_region=rec.boundary_region,
+                    previous=rec.previous, p=tuple(rec.p), origin=tuple(rec.origin), direction=tuple(rec.direction),
+                    primitive=rec.primitive)
+
     def progress(self) -> tuple[int, int]:
         p, n = ctypes.c_int(), ctypes.c_longlong()
         _lib.check(_lib.load().owe_hip_renderer_progress(self._h, ctypes.byref(p), ctypes.byref(n)))

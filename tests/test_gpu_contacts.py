@@ -69,3 +69,25 @@ def test_particles_never_reach_a_covered_sensor(cuda):
     print(f"particles: contact {img.max()}, separated {lit[..., 1].mean():.6g}", flush=True)
     assert (img == 0).all()
     assert lit[..., 1].mean() > 1e-3
+
+
+def test_first_inconsistency_is_recorded(cuda):
+    """TransportStats::first (transport_stats.hpp:15-23): a slab sunk 100 µm into its support meets
+    the overlap (the support's top at z = 0 inside the slab, the slab's bottom at z = −100 µm inside
+    the support); the renderer records the first such crossing with its regions and world point, and
+    a scene in contact records none."""
+    for edits, expect in (((), False), (["Slab.position=(0,0,0.0499)"], True)):
+        ls = LD.load_scene_text(SLAB, edits=edits)
+        r = render.Renderer(ls.scene, ls.settings())
+        r.run_pass(16)
+        first, st = r.first_inconsistency(), r.stats()
+        print(f"edits {list(edits)}: {st['inconsistencies']} inconsistencies, first {first}", flush=True)
+        assert (first is not None) == expect == (st["inconsistencies"] > 0)
+        if first:
+            assert first["ray_region"] != first["boundary_region"]
+            assert first["boundary"] < len(ls.scene.boundaries)
+            assert -2e-4 <= first["p"][2] <= 2e-4  # on the overlap's faces
+            assert abs(np.linalg.norm(first["direction"]) - 1.0) < 1e-5
+            r.reset()
+            assert r.first_inconsistency() is None
+        r.close()
