@@ -321,6 +321,7 @@ public:
         s.photons_per_pixel = rs.photonsPerPixel;
         s.photon_groups = rs.integrator == "vcm" || rs.integrator == "bdpt" ? rs.vcmWavelengthGroups
                                                                             : rs.photonWavelengthGroups;
+        settings_c_ = s;
         check(owe_hip_renderer_create(rs.device < 0 ? 0 : rs.device, &flat_.scene, &s, &r_));
     }
     ~HipRenderer() override {
@@ -339,9 +340,18 @@ public:
     }
     std::string backend() const override { return std::string("hip: ") + owe_hip_backend_summary(); }
     bool resetObserver() override {
-        // The flattened observer is the renderer's own copy: a changed pose or resolution needs a
-        // new renderer (false), as the interface allows.
-        return false;
+        // The device scene is rebased on the eye (camera-relative floats), so a new pose or
+        // resolution is a new flattened scene and device renderer; the samples are discarded, as
+        // the interface says.
+        HipFlatScene next;
+        flattenForHip(scene_, *det_, next);
+        owe_hip_renderer* fresh = nullptr;
+        check(owe_hip_renderer_create(settings_.device < 0 ? 0 : settings_.device, &next.scene, &settings_c_, &fresh));
+        owe_hip_renderer_destroy(r_);
+        r_ = fresh;
+        flat_ = std::move(next);
+        resetProgress();
+        return true;
     }
 
 private:
@@ -367,6 +377,7 @@ private:
     }
 
     HipFlatScene flat_;
+    owe_render_settings settings_c_{};
     owe_hip_renderer* r_ = nullptr;
 };
 
