@@ -59,6 +59,7 @@ PROFILE_TRAFFIC = ROOT / "profiles" / "rayquery_closest_traffic.md"
 PROFILE_PRISM_TRAFFIC = ROOT / "profiles" / "prism_camera_paths_traffic.md"
 PROFILE_LIGHT_TRAFFIC = ROOT / "profiles" / "light_particles_traffic.md"
 PROFILE_LENS_TRAFFIC = ROOT / "profiles" / "lens_batch_traffic.md"
+PROFILE_VCM = ROOT / "profiles" / "vcm_kernels.md"
 PROFILE_CAUSTIC_TRAFFIC = ROOT / "profiles" / "caustic_adjoint_traffic.md"
 PROFILE_COUNTERS = ROOT / "profiles" / "occupancy_valu_l2_counters.md"
 
@@ -989,14 +990,32 @@ def bench_vcm(args):
     stats = r.stats()
     n = stats["paths"]
     its = args.steps * VCM_ITERATIONS
+    img = r.resolve()
+    # The kernels' own times, with HIP events on the renderer's stream, over as many passes again:
+    # the camera kernel (counted as shade) and the light-subpath kernel (counted as generate).
+    r.set_timing(True)
+    for _ in range(args.steps):
+        r.run_pass(VCM_ITERATIONS)
+    times = r.kernel_times()
+    r.set_timing(False)
+    cam_ms, cam_n = times["shade"]
+    light_ms, light_n = times["generate"]
+    cam_launch = cam_ms / max(1, cam_n)
     log(f"vcm: {its} iterations in {wall:.3f}s; {n / wall / 1e6:.1f} M camera paths + light subpaths/s; "
-        f"{wall * 1e3 / its:.3f} ms/iteration")
+        f"{wall * 1e3 / its:.3f} ms/iteration; camera kernel {cam_launch:.3f} ms, light kernel "
+        f"{light_ms / max(1, light_n):.3f} ms per launch of {VCM_ITERATIONS} iterations")
+    roofline = valu_roofline("vcmCameraKernel", cam_launch)
+    if roofline:
+        roofline["traffic"] = measured_traffic(PROFILE_VCM, "vcmCameraKernel")
     out = {"value": n / wall, "unit": "camera paths + light subpaths/s", "ms_per_step": wall * 1e3 / args.steps,
            "config": {"workload": "vcm, lens caustic seen through a glass plate", "resolution": [VCM_SIZE, VCM_SIZE],
                       "iterations_per_step": VCM_ITERATIONS, "light_subpaths_per_iteration": VCM_SIZE * VCM_SIZE,
+                      "iterations_in_flight": r.vcm_in_flight()[0],
                       "ms_per_iteration": wall * 1e3 / its, "segments": stats["segments"],
-                      "mean_Y": float(r.resolve()[..., 1].mean())},
-           "roofline": None}
+                      "camera_kernel_ms_per_launch": cam_launch,
+                      "light_kernel_ms_per_launch": light_ms / max(1, light_n),
+                      "mean_Y": float(img[..., 1].mean())},
+           "roofline": roofline}
     out["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline_vcm(ls, sc, st)
     r.close()
     return out

@@ -226,7 +226,8 @@ int owe_hip_renderer_tuning(const owe_hip_renderer* renderer, int* rounds_per_sy
 int owe_hip_renderer_set_timing(owe_hip_renderer* renderer, int enable);
 /* Milliseconds and launches per kernel since timing was enabled, in the order generate, intersect,
  * shade, shadow, splat, persistent (6 entries each); a light-integrator renderer counts its particle
- * kernel as shade. */
+ * kernel as shade; a vcm or bdpt renderer its light-subpath kernel as generate and its camera kernel
+ * as shade (the medium-only pass's wavefront kernels under their own names). */
 int owe_hip_renderer_kernel_times(const owe_hip_renderer* renderer, double* ms, uint64_t* launches);
 
 /* ------------------------------------------------------------------ batches of scenes

@@ -2222,7 +2222,8 @@ void runVcm(owe_hip_renderer* r, int iterations) {
         const uint32_t paths = S * N;
         if (r->host.emittedPower > 0) {
             const uint32_t tiles = (paths + kScanTile - 1) / kScanTile;
-            launchVcmLight(r->dev, r->set, v, r->lightFilm, r->particleStats, r->invNWY, paths, r->stream);
+            timed(r, kGenerate,
+                  [&] { launchVcmLight(r->dev, r->set, v, r->lightFilm, r->particleStats, r->invNWY, paths, r->stream); });
             hipLaunchKernelGGL(vcmTileSumKernel, dim3{tiles}, dim3{kScanBlock}, 0, r->stream, v.pathCount, paths, v.tileSum);
             hipLaunchKernelGGL(vcmScanKernel, dim3{1}, dim3{kScanThreads}, 0, r->stream, v.tileSum, v.tileOffset, tiles);
             hipLaunchKernelGGL(vcmScanCompactKernel, dim3{tiles}, dim3{kScanBlock}, 0, r->stream, v, paths);
@@ -2233,13 +2234,20 @@ void runVcm(owe_hip_renderer* r, int iterations) {
                 hipLaunchKernelGGL(vcmGridKernel, dim3{blocks}, dim3{256}, 0, r->stream, v);
             }
         }
-        if (r->dev.contacts)
-            hipLaunchKernelGGL((vcmCameraKernel<true>), gridFor(S * pixels), dim3{kBlock}, 0, r->stream, r->dev, r->set, v,
-                               r->particleStats);
-        else
-            hipLaunchKernelGGL((vcmCameraKernel<false>), gridFor(S * pixels), dim3{kBlock}, 0, r->stream, r->dev, r->set, v,
-                               r->particleStats);
+        timed(r, kShade, [&] {
+            if (r->dev.contacts)
+                hipLaunchKernelGGL((vcmCameraKernel<true>), gridFor(S * pixels), dim3{kBlock}, 0, r->stream, r->dev, r->set,
+                                   v, r->particleStats);
+            else
+                hipLaunchKernelGGL((vcmCameraKernel<false>), gridFor(S * pixels), dim3{kBlock}, 0, r->stream, r->dev, r->set,
+                                   v, r->particleStats);
+        });
         OWE_HIP_CHECK(hipGetLastError());
+        if (r->timing) {
+            OWE_HIP_CHECK(hipStreamSynchronize(r->stream));
+            if (r->host.emittedPower > 0) collectTimes(r, {kGenerate});
+            collectTimes(r, {kShade});
+        }
         it += int(S);
     }
     unsigned long long st[4];
