@@ -1363,7 +1363,9 @@ __device__ __forceinline__ float vcmDistantEmissionPdf(const DScene& sc, float s
 struct VcmCamera {
     const DScene& sc;
     const Settings& set;
-    const VcmDev& v;
+    const VcmDev& v;          // the launch's (its slot's own values below)
+    float perMetre;           // the slot's merging radius per metre from the eye (0: bdpt)
+    const uint32_t* head;     // the slot's merging hash
     Rng rng;
     WL wl;
     float beta[NW];
@@ -1486,7 +1488,7 @@ __device__ __forceinline__ void vcmConnect(VcmCamera& c, const VcmVertex& x) {
     evalScatter(sc, hl, dir, -yd, true, pdfLightRev);
     const float cosX = fabsf(dot3(x.h.n, dir)), cosY = fabsf(dot3(yn, dir));
     const float camPdfA = pdfCamFwd * cosY / dist2, lightPdfA = pdfLightFwd * cosX / dist2;
-    const float wLight = camPdfA * (vcmEta(v, yp) + y.n.w + y.mis.x * pdfLightRev);
+    const float wLight = camPdfA * (vcmEtaAt(v, c.perMetre, yp) + y.n.w + y.mis.x * pdfLightRev);
     const float wCamera = lightPdfA * (x.eta + c.mis.dVCM + c.mis.dVC * pdfCamRev);
     const float w = 1.0f / (wLight + 1.0f + wCamera);
     const DBoundary& xb = sc.boundaries[x.h.boundary];
@@ -1511,29 +1513,29 @@ __device__ __forceinline__ void vcmConnect(VcmCamera& c, const VcmVertex& x) {
 __device__ __forceinline__ void vcmMerge(VcmCamera& c, const VcmVertex& x) {
     const DScene& sc = c.sc;
     const VcmDev& v = c.v;
-    if (!(v.mergePerMetre > 0.0f)) return;
+    if (!(c.perMetre > 0.0f)) return;
     const f3 p = x.h.p;
-    const float r = vcmRadius(v, p), r2 = r * r;
+    const float r = vcmRadiusAt(v, c.perMetre, p), r2 = r * r;
     const float invEta = 1.0f / x.eta;
     const float dx = len3(p);
     const float near = dx > 0.0f ? fmaxf(dx - r, 0.0f) / dx : 0.0f, far = dx > 0.0f ? (dx + r) / dx : 1.0f;
     // The levels light vertices within reach live on: their radii span those at the nearest and
     // farthest distance from the eye in reach (almost always one level; at most the top four are
     // searched, which leaves out only vertices within a merging radius of the eye itself).
-    const int levelHi = sppmLevel(vcmRadius(v, p * far));
-    const int levelLo = max(sppmLevel(vcmRadius(v, p * near)), levelHi - 3);
+    const int levelHi = sppmLevel(vcmRadiusAt(v, c.perMetre, p * far));
+    const int levelLo = max(sppmLevel(vcmRadiusAt(v, c.perMetre, p * near)), levelHi - 3);
     for (int level = levelLo; level <= levelHi; ++level) {
         const float size = ldexpf(1.0f, level);
         const f3 q = p - mk3(0.5f, 0.5f, 0.5f) * size;
         const int bx = sppmCell(q.x, size), by = sppmCell(q.y, size), bz = sppmCell(q.z, size);
         for (int c8 = 0; c8 < 8; ++c8) {
             const int cx = bx + (c8 & 1), cy = by + ((c8 >> 1) & 1), cz = bz + ((c8 >> 2) & 1);
-            for (uint32_t i = v.head[sppmHash(cx, cy, cz, level, v.gridMask)]; i != kGridNone; i = v.next[i]) {
+            for (uint32_t i = c.head[sppmHash(cx, cy, cz, level, v.gridMask)]; i != kGridNone; i = v.next[i]) {
                 const float4 vp = v.cache[i].p;
                 const f3 yp = mk3(vp.x, vp.y, vp.z);
                 const f3 dp = yp - p;
                 if (dot3(dp, dp) > r2) continue;
-                if (sppmLevel(vcmRadius(v, yp)) != level || sppmCell(yp.x, size) != cx || sppmCell(yp.y, size) != cy ||
+                if (sppmLevel(vcmRadiusAt(v, c.perMetre, yp)) != level || sppmCell(yp.x, size) != cx || sppmCell(yp.y, size) != cy ||
                     sppmCell(yp.z, size) != cz)
                     continue;
                 const LightVertex y = v.cache[i];
@@ -1581,17 +1583,21 @@ __global__ __launch_bounds__(kBlock, OWE_HIP_VCM_WAVES) void vcmCameraKernel(DSc
     const uint32_t pixels = uint32_t(ob.width) * uint32_t(ob.height);
     uint32_t nPaths = 0, nSegments = 0, nInconsistent = 0, nLeaks = 0;
     if (t < pixels * vAll.slots) {
+        // The slot's own values are computed here, not copied with the rest of the launch's record,
+        // which stays a kernel argument.
         const uint32_t slot = t / pixels, pixel = t - slot * pixels;
-        const VcmDev v = vAll.at(slot);
-        VcmCamera c{sc, set, v};
+        const VcmDev& v = vAll;
+        const uint32_t iteration = v.iteration + slot;
+        const uint32_t* counts = v.counts + 32u * slot;
+        VcmCamera c{sc, set, v, v.slotPerMetre(slot), v.slotHead(slot)};
         c.Lm[0] = c.Lm[1] = c.Lm[2] = 0;
         c.bad = false;
         c.stack = stack + threadIdx.x;
-        c.rng.seed(hashPair(set.seed ^ kVcmCamera, pixel), v.iteration);
-        c.group = vcmPixelGroup(pixel, v.iteration, v.groups);
-        c.wl = sppmWavelengths(set, v.rotation, c.group, v.groups);
-        c.M = v.counts[4 + c.group];
-        c.base = v.counts[16 + c.group];
+        c.rng.seed(hashPair(set.seed ^ kVcmCamera, pixel), iteration);
+        c.group = vcmPixelGroup(pixel, iteration, v.groups);
+        c.wl = sppmWavelengths(set, v.slotRotation(slot), c.group, v.groups);
+        c.M = counts[4 + c.group];
+        c.base = counts[16 + c.group];
         if (c.base > v.capacity || c.M > v.capacity - c.base) c.M = 0;  // cannot happen: the cache holds every vertex
         c.L = mk3(0.0f, 0.0f, 0.0f);
         PathStart ps;
@@ -1723,7 +1729,7 @@ __global__ __launch_bounds__(kBlock, OWE_HIP_VCM_WAVES) void vcmCameraKernel(DSc
                         }
                     }
                 }
-                const float eta = vcmEta(v, h.p);
+                const float eta = vcmEtaAt(v, c.perMetre, h.p);
                 if (!o2.delta) {
                     const VcmVertex x{h, it, d, eta, hitFace};
                     vcmNextEvent<kContacts>(c, x);

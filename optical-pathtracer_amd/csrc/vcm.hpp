@@ -84,16 +84,23 @@ struct VcmDev {
     double mergeExponent;      // ... times (i + 1)^mergeExponent at iteration i: (α − 1)/2
     double rotationOffset;     // the wavelength rotation of iteration i: frac(offset + φ⁻¹(i + 1))
 
-    // Slot k's view: its iteration, rotation, radius, counts and hash.
+    // Slot k's iteration, wavelength rotation, merging radius per metre, counts and hash.
+    __device__ __forceinline__ float slotRotation(uint32_t k) const {
+        const double u = rotationOffset + 0.6180339887498949 * double(iteration + k + 1u);
+        return float(u - floor(u));
+    }
+    __device__ __forceinline__ float slotPerMetre(uint32_t k) const {
+        return mergeBase > 0.0f ? float(double(mergeBase) * pow(double(iteration + k + 1u), mergeExponent)) : 0.0f;
+    }
+    __device__ __forceinline__ uint32_t* slotHead(uint32_t k) const { return head + size_t(k) * (size_t(gridMask) + 1); }
+    // Slot k's view: a copy with those values in place of the launch's.
     __device__ __forceinline__ VcmDev at(uint32_t k) const {
         VcmDev s = *this;
         s.iteration = iteration + k;
-        const double u = rotationOffset + 0.6180339887498949 * double(s.iteration + 1u);
-        s.rotation = float(u - floor(u));
-        s.mergePerMetre = mergeBase > 0.0f ? float(double(mergeBase) * pow(double(s.iteration + 1u), mergeExponent))
-                                           : 0.0f;
+        s.rotation = slotRotation(k);
+        s.mergePerMetre = slotPerMetre(k);
         s.counts = counts + 32u * k;
-        s.head = head + size_t(k) * (size_t(gridMask) + 1);
+        s.head = slotHead(k);
         return s;
     }
 };
@@ -102,15 +109,18 @@ __device__ __forceinline__ uint32_t vcmPixelGroup(uint32_t pixel, uint32_t itera
     return uint32_t(mix64((uint64_t(iteration) << 32 | pixel) ^ kVcmGroup) % uint64_t(groups));
 }
 
-__device__ __forceinline__ float vcmRadius(const VcmDev& v, f3 p) {
-    return fmaxf(v.mergePerMetre * len3(p), v.mergeFloor);
+// The merging radius at p for a radius per metre perMetre (a slot's).
+__device__ __forceinline__ float vcmRadiusAt(const VcmDev& v, float perMetre, f3 p) {
+    return fmaxf(perMetre * len3(p), v.mergeFloor);
 }
+__device__ __forceinline__ float vcmRadius(const VcmDev& v, f3 p) { return vcmRadiusAt(v, v.mergePerMetre, p); }
 // η = π r² N_L: merging's weight relative to connections at p (0 without merging), and its inverse.
-__device__ __forceinline__ float vcmEta(const VcmDev& v, f3 p) {
-    if (!(v.mergePerMetre > 0.0f)) return 0.0f;
-    const float r = vcmRadius(v, p);
+__device__ __forceinline__ float vcmEtaAt(const VcmDev& v, float perMetre, f3 p) {
+    if (!(perMetre > 0.0f)) return 0.0f;
+    const float r = vcmRadiusAt(v, perMetre, p);
     return kPiF * r * r * v.lightPaths;
 }
+__device__ __forceinline__ float vcmEta(const VcmDev& v, f3 p) { return vcmEtaAt(v, v.mergePerMetre, p); }
 __device__ __forceinline__ float vcmInvEta(const VcmDev& v, f3 p) {
     const float e = vcmEta(v, p);
     return e > 0.0f ? 1.0f / e : 0.0f;

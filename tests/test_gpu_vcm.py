@@ -175,3 +175,33 @@ def test_iterations_in_flight_give_the_same_image(cuda, integrator):
     with pytest.raises(Exception):
         r.set_vcm_in_flight(most + 1)
     r.close()
+
+
+def _sky_plate(glass: bool):
+    from owe_hip import loader as LD
+
+    return LD.load_scene_text("""
+units = m
+world { sky = uniform(rgb(0.7, 0.3, 1, luminance = 1)) }
+""" + ("""
+body Plate { type = box  size = (20, 20, 0.01)  position = (0, 0, 0)  medium = N-BK7 }
+""" if glass else "") + """
+observer Eye { position = (0, 0, 1)  look_at = (0, 0, 0)  up = (0, 1, 0)  fov = 20deg  pupil = 1mm  focus = 1
+               resolution = (8, 8) }
+render { detector = Eye  max_depth = 16 }
+""")
+
+
+def test_dispersive_plate_keeps_the_sky_colour(cuda):
+    """A lossless dispersive plate in an isotropic coloured sky cannot change its radiance; the glass
+    terminates the secondary wavelengths, so each channel of vcm's and bdpt's image through it must
+    equal the path tracer's without it (tests/test_vcm.cpp:260-295: within 1.5 %)."""
+    ref_ls = _sky_plate(False)
+    ref = _render(ref_ls.use(), ref_ls.settings(integrator="path", seed=11), 1, 4096).resolve()
+    ls = _sky_plate(True)
+    for integrator in ("bdpt", "vcm"):
+        img = _render(ls.use(), ls.settings(integrator=integrator, seed=11), 1, 4096).resolve()
+        ratio = img.sum(axis=(0, 1)) / ref.sum(axis=(0, 1))
+        print(f"{integrator} through the dispersive plate, XYZ ratios {ratio}", flush=True)
+        assert (ref.sum(axis=(0, 1)) > 0).all()
+        assert np.abs(ratio - 1).max() < 0.015
