@@ -58,9 +58,20 @@ __global__ __launch_bounds__(kQueryBlock) void traceBatchKernel(const WideNode* 
     }
 }
 
+// Result staging of the refill form. A wave takes its rays in increasing order, so they end in nearly
+// that order, but a lane's store of its own result touches a cache line that no neighbour has filled
+// yet. Each wave therefore holds kStageSlots LDS slots of 64 results: group g of its range (rays
+// start + 64g … start + 64g + 63) is staged in slot g % kStageSlots when that slot is free as the
+// wave first takes a ray of g, and leaves as one coalesced 64-lane store per output array once all
+// of its rays have ended. A group whose slot is still held writes its results directly, so a slow
+// ray never holds back the refill.
+constexpr uint32_t kStageSlots = 4;
+constexpr uint32_t kNoGroup = 0xFFFFFFFFu;
+
 // Persistent lanes with refill: wave w owns rays [w·per, min(n, (w + 1)·per)). Every branch that
-// decides a refill is wave-uniform (ballots and the range); no barrier is used, so waves run
-// independently and each exits once its range is drained and its lanes are idle.
+// decides a refill or a flush is wave-uniform (ballots, the range and the slots' owners); no barrier
+// is used, so waves run independently and each exits once its range is drained and its lanes are
+// idle, by which time every staged group has been flushed.
 template <int kCap, bool kAny>
 __global__ __launch_bounds__(kQueryBlock) void traceRefillKernel(const WideNode* __restrict__ nodes,
                                                                  const TriRecord* __restrict__ tris,
@@ -69,10 +80,17 @@ __global__ __launch_bounds__(kQueryBlock) void traceRefillKernel(const WideNode*
                                                                  uint32_t* __restrict__ primOut,
                                                                  float* __restrict__ tOut) {
     __shared__ uint32_t stack[kCap * kQueryBlock];
+    __shared__ uint2 staged[kQueryBlock / 64][kStageSlots][64];
     const uint32_t lane = threadIdx.x & 63u;
+    uint2(*stage)[64] = staged[threadIdx.x >> 6];
     const uint64_t wave = (uint64_t(blockIdx.x) * kQueryBlock + threadIdx.x) >> 6;
-    uint64_t next = wave * per;
+    const uint64_t start = wave * per;
+    uint64_t next = start;
     const uint64_t end = next < n ? (n - next < per ? n : next + per) : next;
+    uint32_t owner[kStageSlots], held[kStageSlots];
+#pragma unroll
+    for (uint32_t j = 0; j < kStageSlots; ++j) owner[j] = kNoGroup, held[j] = 0;
+    uint32_t opened = 0;  // groups of the range whose slot has been decided
     TravState s;
     uint64_t ray = 0;
     bool active = false;
@@ -88,12 +106,54 @@ __global__ __launch_bounds__(kQueryBlock) void traceRefillKernel(const WideNode*
                 active = true;
             }
             next = end - next > nIdle ? next + nIdle : end;
+            // The taken rays (at least one: refill ≥ 1) span at most two new groups; each takes its
+            // slot if the slot is free.
+            const uint32_t last = uint32_t((next - 1 - start) >> 6);
+            for (; opened <= last; ++opened) {
+#pragma unroll
+                for (uint32_t j = 0; j < kStageSlots; ++j)
+                    if (opened % kStageSlots == j && owner[j] == kNoGroup) owner[j] = opened, held[j] = 0;
+            }
         }
         if (__ballot(active) == 0) break;
+        bool done = false;
+        uint32_t prim = 0;
+        float t = 0.0f;
         if (active && stepBvh4<kAny>(s, nodes, tris, stack + threadIdx.x, kQueryBlock)) {
-            primOut[ray] = s.hit.slot == kEmptyChild ? 0xFFFFFFFFu : tris[s.hit.slot].prim;
-            tOut[ray] = s.hit.slot == kEmptyChild ? __builtin_inff() : s.hit.t;
+            prim = s.hit.slot == kEmptyChild ? 0xFFFFFFFFu : tris[s.hit.slot].prim;
+            t = s.hit.slot == kEmptyChild ? __builtin_inff() : s.hit.t;
             active = false;
+            done = true;
+        }
+        if (__ballot(done) == 0) continue;
+        const uint32_t group = uint32_t((ray - start) >> 6), slot = group % kStageSlots;
+        bool staging = false;
+#pragma unroll
+        for (uint32_t j = 0; j < kStageSlots; ++j) staging |= slot == j && owner[j] == group;
+        if (done) {
+            if (staging) {
+                stage[slot][(ray - start) & 63u] = make_uint2(prim, __float_as_uint(t));
+            } else {
+                primOut[ray] = prim;
+                tOut[ray] = t;
+            }
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kStageSlots; ++j) {
+            held[j] += uint32_t(__popcll(__ballot(done && staging && slot == j)));
+            if (owner[j] == kNoGroup) continue;
+            const uint64_t base = start + 64ull * owner[j];
+            const uint32_t size = end - base < 64u ? uint32_t(end - base) : 64u;
+            if (held[j] != size) continue;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            if (lane < size) {
+                const uint2 e = stage[j][lane];
+                primOut[base + lane] = e.x;
+                tOut[base + lane] = __uint_as_float(e.y);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            owner[j] = kNoGroup;
+            held[j] = 0;
         }
     }
 }
