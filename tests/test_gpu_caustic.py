@@ -139,7 +139,7 @@ def test_autograd_function_and_descent(setup):
         Hv = Hs.clone().requires_grad_(True)
         L = ((c.irradiance(Hv, it).double() - T.double()) ** 2).mean()
         L.backward()
-        losses.append(float(L))
+        losses.append(float(L.detach()))
         with torch.no_grad():
             Hs = Hs - 2e-6 * Hv.grad / Hv.grad.abs().max()
     print(f"losses: {losses}", flush=True)
