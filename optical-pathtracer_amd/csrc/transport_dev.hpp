@@ -576,18 +576,16 @@ __device__ __forceinline__ float contactTol(f3 p) {
 // from the ray's region and is not a detector — two rounds at most (one inlined query in a loop of
 // at most three). Without contacts it is the plain intersectWorld.
 // kContacts: compiled for scenes with contacts (the launch picks it from sc.contacts); without it
-// this is the plain intersectWorld, with no contact code in the kernel. A kernel compiled with it
-// may still meet a scene without contacts (a batch whose other members have them): its query is the
-// loop's first round from t > 0 with nothing skipped, which is the plain intersectWorld, so the
-// kernel holds one inlined world query, not two.
+// this is the plain intersectWorld, with no contact code in the kernel.
 template <bool kGeometry = true, bool kContacts = true>
 __device__ __forceinline__ bool intersectContact(const DScene& sc, f3 o, f3 d, float tmax, uint32_t departed,
                                                  uint32_t departedFace, uint32_t exclude, uint32_t region,
                                                  uint32_t* stack, uint32_t stride, Hit& h) {
-    if (!kContacts) return intersectWorld<kGeometry>(sc, o, d, tmax, departed, departedFace, exclude, stack, stride, h);
-    const bool contacts = sc.contacts != 0u;
-    const float dep = contacts && departed != kNoBoundary ? contactTol(o) : 0.0f;
-    float lo = contacts ? -dep : 0.0f, hi = tmax;
+    if (!kContacts || !sc.contacts)
+        return intersectWorld<kGeometry>(sc, o, d, tmax, departed, departedFace, exclude, stack, stride, h);
+    const bool contacts = true;
+    const float dep = departed != kNoBoundary ? contactTol(o) : 0.0f;
+    float lo = -dep, hi = tmax;
     uint32_t skipB = kNoBoundary, skipF = ~0u;
     bool found = false, fits = false;
     for (int round = 0; round < 3; ++round) {
